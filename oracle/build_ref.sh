@@ -1,0 +1,25 @@
+#!/bin/sh
+# Builds oracle/_ref/libffref_ref.so: FreeFEM's own femlib sources
+# (HashMatrix.cpp, CG.cpp, QuadratureFormular.cpp) and the no-graphics host
+# globals of src/Graphics/gggg.cpp, compiled where they lie
+# under the reference tree, plus the test driver ref_driver.cpp.
+# -DCMAKE is the reference's own switch that skips the autoconf config.h
+# (src/Graphics/rgraph.hpp:55-57).  Flags follow the reference build:
+# g++ -O3 -mavx2 -std=gnu++14 (no FMA contraction).
+# Test infrastructure only; needs /root/reference (absent on GPU hosts).
+set -e
+REF=${FF_REFERENCE:-/root/reference}
+HERE=$(cd "$(dirname "$0")" && pwd)
+OUT="$HERE/_ref"
+[ -d "$REF/src/femlib" ] || { echo "reference tree not found at $REF" >&2; exit 3; }
+mkdir -p "$OUT"
+CXXFLAGS="-O3 -mavx2 -std=gnu++14 -fPIC -DCMAKE -DNCHECKPTR -w"
+INC="-I$REF/src/femlib -I$REF/src/Graphics -I$REF/src/fflib -I$REF/src/libMesh"
+for f in HashMatrix CG QuadratureFormular; do
+  g++ $CXXFLAGS $INC -c "$REF/src/femlib/$f.cpp" -o "$OUT/$f.o"
+done
+g++ $CXXFLAGS $INC -c "$REF/src/Graphics/gggg.cpp" -o "$OUT/gggg.o"
+g++ $CXXFLAGS $INC -c "$HERE/ref_driver.cpp" -o "$OUT/ref_driver.o"
+g++ -shared -o "$OUT/libffref_ref.so" "$OUT/ref_driver.o" "$OUT/HashMatrix.o" \
+  "$OUT/CG.o" "$OUT/QuadratureFormular.o" "$OUT/gggg.o"
+echo "built $OUT/libffref_ref.so"

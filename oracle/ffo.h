@@ -1,0 +1,122 @@
+/*
+ * ffo.h — CPU restatement ("oracle") of FreeFEM's assembly + Jacobi-CG path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and only
+ * to compare against or to time as the CPU baseline.  The product (libffhip)
+ * never links, loads or calls it.
+ *
+ * Every function restates the reference algorithm it cites (paths relative to
+ * the FreeFEM source tree).  Arithmetic order follows the reference wherever the
+ * reference fixes it (element loop order, quadrature loop, term loop, HashMatrix
+ * accumulation order), so values agree with the compiled reference subset in
+ * oracle/_ref bit for bit where both are evaluated the same way.
+ */
+#ifndef FFO_H
+#define FFO_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- meshes ------------------------------------------------------------ */
+/* cube(nx,ny,nz): msh3.cpp:7879-8132 (BuildCube, kind=6), labels as Cube_Op
+ * 8134-8164.  out = {nv, nt, nbe}. */
+void ffo_cube_counts(int nx, int ny, int nz, int64_t out[3]);
+int ffo_cube(int nx, int ny, int nz, const int32_t labels[6], double *xyz,
+             int32_t *tets, int32_t *btri, int32_t *blab);
+/* square(nx,ny): lgmesh.cpp:1229-1384 (Carre_, flags=0).  out = {nv, nt, nbe}. */
+void ffo_square_counts(int nx, int ny, int64_t out[3]);
+int ffo_square(int nx, int ny, const int32_t labels[4], double *xy,
+               int32_t *tris, int32_t *bedges, int32_t *blab);
+
+/* ---- quadrature --------------------------------------------------------- */
+/* QuadratureFormular_Tet_5 (QuadratureFormular.cpp:725-743): 14 points,
+ * barycentric (x,y,z) and weight (sum 1). */
+int ffo_qf_tet5(double *pts, double *w);
+/* QuadratureFormular_T_5 (QuadratureFormular.cpp:172-186): 7 points. */
+int ffo_qf_tri5(double *pts, double *w);
+
+/* ---- geometry ----------------------------------------------------------- */
+/* Tet mesure: Mesh3dn.hpp:65-71 with R3.hpp:90-106 det(). */
+double ffo_tet_mesure(const double *v0, const double *v1, const double *v2,
+                      const double *v3);
+/* Tet::Gradlambda: Mesh3dn.hpp:126-137.  v = 4 vertices xyz, G = 4 gradients */
+void ffo_tet_gradlambda(const double *v, double mes, double *G);
+
+/* ---- sparsity pattern (HashMatrix CSR view) ----------------------------- */
+/* The pattern of HashMatrix after `A += mate` over every element
+ * (HashMatrix.cpp:1294-1332 inserts every (ni[i], nj[j]) pair) sorted by
+ * Sortij/Buildp (HashMatrix.cpp:670-682, 992-1030).  elems = nel x nloc global
+ * dof ids.  Returns nnz (count pass fills rowptr). */
+int64_t ffo_pattern_rowptr(int n, int64_t nel, int nloc, const int32_t *elems,
+                           int64_t *rowptr);
+int ffo_pattern_colidx(int n, int64_t nel, int nloc, const int32_t *elems,
+                       const int64_t *rowptr, int32_t *colidx);
+/* All (i,j) pairs sorted + deduplicated — the brute-force cross-check. */
+int64_t ffo_pattern_bruteforce(int n, int64_t nel, int nloc,
+                               const int32_t *elems, int64_t *rowptr,
+                               int32_t *colidx);
+
+/* ---- P1 assembly -------------------------------------------------------- */
+enum { FFO_COEF_CONST = 0, FFO_COEF_SIN3 = 1, FFO_COEF_P0 = 2 };
+/* Coefficient kappa(x,y,z): CONST -> c0; SIN3 -> 1+0.5*sin(2 pi x)sin(2 pi y)
+ * sin(2 pi z) evaluated as the script expression; P0 -> kappa_e[element]. */
+/* Element_Op full 3D (problem.cpp:6337-6437) for
+ * int3d(Th)(kappa*(dx(u)*dx(v)+dy(u)*dy(v)+dz(u)*dz(v))), P1 (P012_3d.cpp:123-175)
+ * with QF Tet_5. Ke row-major 4x4, row = test function. */
+int ffo_p1_element_3d(const double *xyz, const int32_t *tet, int kind,
+                      double c0, double kappa_e, double *Ke);
+/* AssembleBilinearForm 3D element loop (problem.cpp:1398-1405) + HashMatrix +=
+ * (element order, HashMatrix.cpp:1294-1332) into a CSR pattern. */
+int ffo_assemble_p1_3d(int nv, int64_t nt, const double *xyz,
+                       const int32_t *tets, int kind, double c0,
+                       const double *kappa_e, const int64_t *rowptr,
+                       const int32_t *colidx, double *vals);
+/* 2D: Element_Op 2D (problem.cpp:6063) with P1 triangles, QF T_5. */
+int ffo_p1_element_2d(const double *xy, const int32_t *tri, int kind,
+                      double c0, double kappa_e, double *Ke);
+int ffo_assemble_p1_2d(int nv, int64_t nt, const double *xy,
+                       const int32_t *tris, int kind, double c0,
+                       const double *kappa_e, const int64_t *rowptr,
+                       const int32_t *colidx, double *vals);
+/* Element_rhs 3D (problem.cpp:7917-7985) for int3d(Th)(f*v), f constant,
+ * accumulated into b in element order. */
+int ffo_rhs_p1_3d(int nv, int64_t nt, const double *xyz, const int32_t *tets,
+                  double f, double *b);
+int ffo_rhs_p1_2d(int nv, int64_t nt, const double *xy, const int32_t *tris,
+                  double f, double *b);
+
+/* ---- Dirichlet (tgv) ---------------------------------------------------- */
+/* AssembleBC (problem.cpp:10039-10194): dofs of boundary faces whose label is
+ * in labs[] are marked (P1: the face vertices). */
+int ffo_mark_bc(int n, int64_t nbf, int nvf, const int32_t *bfaces,
+                const int32_t *blab, int nlab, const int32_t *labs,
+                uint8_t *onbc);
+/* HashMatrix::SetBC with tgv>0 (HashMatrix.cpp:1194-1238): a_ii = tgv; and
+ * the right-hand side B[ddf] = tgv*g (problem.cpp:10176). */
+int ffo_set_bc(int n, const int64_t *rowptr, const int32_t *colidx,
+               double *vals, const uint8_t *onbc, double tgv);
+int ffo_rhs_bc(int n, const uint8_t *onbc, double tgv, double g, double *b);
+
+/* ---- SpMV and CG -------------------------------------------------------- */
+void ffo_spmv(int n, const int64_t *rowptr, const int32_t *colidx,
+              const double *vals, const double *x, double *y);
+void ffo_spmv_t(int n, const int64_t *rowptr, const int32_t *colidx,
+                const double *vals, const double *x, double *y);
+/* HashMatrix::gettgv (HashMatrix.cpp:1340-1371), ratio 1e6. */
+double ffo_gettgv(int n, const int64_t *rowptr, const int32_t *colidx,
+                  const double *vals, int *ntgv);
+/* SolverCG::dosolver + HMatVirtPrecon (VirtualSolverCG.hpp:13-192) +
+ * ConjugueGradient (CG.cpp:195-265).  x holds the initial guess on entry.
+ * stats = {iterations, gCg0, gCg_final, converged(0/1/2)}.  trans != 0 solves
+ * with A^T (HMatVirt(A,trans)).  Returns the reference's nret. */
+int ffo_pcg(int n, const int64_t *rowptr, const int32_t *colidx,
+            const double *vals, const double *b, double *x, double eps,
+            int itmax, double tgv, int trans, double *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

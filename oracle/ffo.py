@@ -1,0 +1,225 @@
+"""ctypes bindings for the oracle (libffo.so) and the compiled reference subset
+(oracle/_ref/libffref_ref.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker.  The product never imports it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_REF = None
+
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C")
+_i64p = np.ctypeslib.ndpointer(np.int64, flags="C")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C")
+_u8p = np.ctypeslib.ndpointer(np.uint8, flags="C")
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_l = ctypes.c_int64
+_d = ctypes.c_double
+
+COEF_CONST, COEF_SIN3, COEF_P0 = 0, 1, 2
+
+
+def lib():
+    """Load libffo.so (build it with `make -C oracle`)."""
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(HERE, "libffo.so")
+        L = ctypes.CDLL(path)
+        sig = {
+            "ffo_cube_counts": (None, [_i, _i, _i, _i64p]),
+            "ffo_cube": (_i, [_i, _i, _i, _i32p, _f64p, _i32p, _i32p, _i32p]),
+            "ffo_square_counts": (None, [_i, _i, _i64p]),
+            "ffo_square": (_i, [_i, _i, _i32p, _f64p, _i32p, _i32p, _i32p]),
+            "ffo_qf_tet5": (_i, [_f64p, _f64p]),
+            "ffo_qf_tri5": (_i, [_f64p, _f64p]),
+            "ffo_tet_mesure": (_d, [_f64p, _f64p, _f64p, _f64p]),
+            "ffo_tet_gradlambda": (None, [_f64p, _d, _f64p]),
+            "ffo_pattern_rowptr": (_l, [_i, _l, _i, _i32p, _i64p]),
+            "ffo_pattern_colidx": (_i, [_i, _l, _i, _i32p, _i64p, _i32p]),
+            "ffo_pattern_bruteforce": (_l, [_i, _l, _i, _i32p, _i64p, _i32p]),
+            "ffo_p1_element_3d": (_i, [_f64p, _i32p, _i, _d, _d, _f64p]),
+            "ffo_p1_element_2d": (_i, [_f64p, _i32p, _i, _d, _d, _f64p]),
+            "ffo_assemble_p1_3d": (_i, [_i, _l, _f64p, _i32p, _i, _d, _vp, _i64p, _i32p, _f64p]),
+            "ffo_assemble_p1_2d": (_i, [_i, _l, _f64p, _i32p, _i, _d, _vp, _i64p, _i32p, _f64p]),
+            "ffo_rhs_p1_3d": (_i, [_i, _l, _f64p, _i32p, _d, _f64p]),
+            "ffo_rhs_p1_2d": (_i, [_i, _l, _f64p, _i32p, _d, _f64p]),
+            "ffo_mark_bc": (_i, [_i, _l, _i, _i32p, _i32p, _i, _i32p, _u8p]),
+            "ffo_set_bc": (_i, [_i, _i64p, _i32p, _f64p, _u8p, _d]),
+            "ffo_rhs_bc": (_i, [_i, _u8p, _d, _d, _f64p]),
+            "ffo_spmv": (None, [_i, _i64p, _i32p, _f64p, _f64p, _f64p]),
+            "ffo_spmv_t": (None, [_i, _i64p, _i32p, _f64p, _f64p, _f64p]),
+            "ffo_gettgv": (_d, [_i, _i64p, _i32p, _f64p, ctypes.POINTER(_i)]),
+            "ffo_pcg": (_i, [_i, _i64p, _i32p, _f64p, _f64p, _f64p, _d, _i, _d, _i, _f64p]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def ref_available():
+    return os.path.exists(os.path.join(HERE, "_ref", "libffref_ref.so"))
+
+
+def ref():
+    """Load the compiled reference subset (built by oracle/build_ref.sh)."""
+    global _REF
+    if _REF is None:
+        L = ctypes.CDLL(os.path.join(HERE, "_ref", "libffref_ref.so"))
+        sig = {
+            "ref_qf_tet5": (_i, [_f64p, _f64p]),
+            "ref_qf_tri5": (_i, [_f64p, _f64p]),
+            "ref_tet_geom": (_i, [_f64p, _f64p, _f64p]),
+            "ref_hm_new": (_vp, [_i]),
+            "ref_hm_free": (None, [_vp]),
+            "ref_hm_add_p1_3d": (_i, [_vp, _i, _l, _f64p, _i32p, _i, _d, _vp]),
+            "ref_hm_setbc": (_i, [_vp, _u8p, _d]),
+            "ref_hm_nnz": (_l, [_vp]),
+            "ref_hm_gettgv": (_d, [_vp, ctypes.POINTER(_i)]),
+            "ref_hm_matmul": (_i, [_vp, _f64p, _f64p, _i]),
+            "ref_hm_csr": (_i, [_vp, _i64p, _i32p, _f64p]),
+            "ref_hm_cg": (_i, [_vp, _f64p, _f64p, _d, _i, _d, _i, _f64p]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _REF = L
+    return _REF
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+# ---------------------------------------------------------------- meshes
+def cube(n, ny=None, nz=None, labels=(1, 2, 3, 4, 5, 6)):
+    ny = n if ny is None else ny
+    nz = n if nz is None else nz
+    c = np.zeros(3, np.int64)
+    lib().ffo_cube_counts(n, ny, nz, c)
+    nv, nt, nbe = (int(v) for v in c)
+    xyz = np.zeros(3 * nv)
+    tets = np.zeros(4 * nt, np.int32)
+    btri = np.zeros(3 * nbe, np.int32)
+    blab = np.zeros(nbe, np.int32)
+    rc = lib().ffo_cube(n, ny, nz, np.asarray(labels, np.int32), xyz, tets, btri, blab)
+    assert rc == 0, rc
+    return dict(nv=nv, nt=nt, nbe=nbe, xyz=xyz, tets=tets, btri=btri, blab=blab)
+
+
+def square(n, ny=None, labels=(1, 2, 3, 4)):
+    ny = n if ny is None else ny
+    c = np.zeros(3, np.int64)
+    lib().ffo_square_counts(n, ny, c)
+    nv, nt, nbe = (int(v) for v in c)
+    xy = np.zeros(2 * nv)
+    tris = np.zeros(3 * nt, np.int32)
+    bed = np.zeros(2 * nbe, np.int32)
+    blab = np.zeros(nbe, np.int32)
+    rc = lib().ffo_square(n, ny, np.asarray(labels, np.int32), xy, tris, bed, blab)
+    assert rc == 0, rc
+    return dict(nv=nv, nt=nt, nbe=nbe, xy=xy, tris=tris, bedges=bed, blab=blab)
+
+
+# ---------------------------------------------------------------- pattern
+def pattern(n, elems, nloc):
+    nel = elems.size // nloc
+    rowptr = np.zeros(n + 1, np.int64)
+    nnz = lib().ffo_pattern_rowptr(n, nel, nloc, elems, rowptr)
+    colidx = np.zeros(nnz, np.int32)
+    rc = lib().ffo_pattern_colidx(n, nel, nloc, elems, rowptr, colidx)
+    assert rc == 0, rc
+    return rowptr, colidx
+
+
+def pattern_bruteforce(n, elems, nloc):
+    nel = elems.size // nloc
+    rowptr = np.zeros(n + 1, np.int64)
+    colidx = np.zeros(nel * nloc * nloc, np.int32)
+    nnz = lib().ffo_pattern_bruteforce(n, nel, nloc, elems, rowptr, colidx)
+    return rowptr, colidx[:nnz].copy()
+
+
+def assemble_p1_3d(m, rowptr, colidx, kind=COEF_SIN3, c0=1.0, kappa_e=None):
+    vals = np.zeros(colidx.size)
+    ke = None if kappa_e is None else np.ascontiguousarray(kappa_e, np.float64)
+    rc = lib().ffo_assemble_p1_3d(m["nv"], m["nt"], m["xyz"], m["tets"], kind, c0,
+                                  _ptr(ke), rowptr, colidx, vals)
+    assert rc == 0, rc
+    return vals
+
+
+def assemble_p1_2d(m, rowptr, colidx, kind=COEF_CONST, c0=1.0, kappa_e=None):
+    vals = np.zeros(colidx.size)
+    ke = None if kappa_e is None else np.ascontiguousarray(kappa_e, np.float64)
+    rc = lib().ffo_assemble_p1_2d(m["nv"], m["nt"], m["xy"], m["tris"], kind, c0,
+                                  _ptr(ke), rowptr, colidx, vals)
+    assert rc == 0, rc
+    return vals
+
+
+def rhs_p1_3d(m, f=1.0):
+    b = np.zeros(m["nv"])
+    lib().ffo_rhs_p1_3d(m["nv"], m["nt"], m["xyz"], m["tets"], f, b)
+    return b
+
+
+def rhs_p1_2d(m, f=1.0):
+    b = np.zeros(m["nv"])
+    lib().ffo_rhs_p1_2d(m["nv"], m["nt"], m["xy"], m["tris"], f, b)
+    return b
+
+
+def mark_bc(n, bfaces, blab, nvf, labels):
+    onbc = np.zeros(n, np.uint8)
+    labs = np.asarray(labels, np.int32)
+    lib().ffo_mark_bc(n, blab.size, nvf, bfaces, blab, labs.size, labs, onbc)
+    return onbc
+
+
+def set_bc(rowptr, colidx, vals, onbc, tgv=1e30):
+    rc = lib().ffo_set_bc(onbc.size, rowptr, colidx, vals, onbc, tgv)
+    assert rc == 0, rc
+
+
+def rhs_bc(b, onbc, tgv=1e30, g=0.0):
+    lib().ffo_rhs_bc(onbc.size, onbc, tgv, g, b)
+
+
+def spmv(rowptr, colidx, vals, x, trans=False):
+    y = np.zeros(rowptr.size - 1)
+    f = lib().ffo_spmv_t if trans else lib().ffo_spmv
+    f(rowptr.size - 1, rowptr, colidx, vals, np.ascontiguousarray(x, np.float64), y)
+    return y
+
+
+def pcg(rowptr, colidx, vals, b, x0=None, eps=1e-10, itmax=0, tgv=1e30, trans=False):
+    n = rowptr.size - 1
+    x = np.zeros(n) if x0 is None else np.array(x0, np.float64)
+    st = np.zeros(4)
+    lib().ffo_pcg(n, rowptr, colidx, vals, np.ascontiguousarray(b, np.float64), x, eps,
+                  itmax, tgv, int(trans), st)
+    return x, dict(iters=int(st[0]), gcg0=st[1], gcg=st[2], nret=int(st[3]))
+
+
+def poisson_p1_3d(n, kind=COEF_SIN3, c0=1.0, kappa_e=None, labels=(1, 2, 3, 4, 5, 6),
+                  tgv=1e30, f=1.0):
+    """cube(n) P1 Poisson system exactly as `matrix A = va(Vh,Vh)` and
+    `b = vb(0,Vh)` build it: pattern, values, tgv rows, right-hand side."""
+    m = cube(n)
+    rowptr, colidx = pattern(m["nv"], m["tets"], 4)
+    vals = assemble_p1_3d(m, rowptr, colidx, kind, c0, kappa_e)
+    onbc = mark_bc(m["nv"], m["btri"], m["blab"], 3, labels)
+    set_bc(rowptr, colidx, vals, onbc, tgv)
+    b = rhs_p1_3d(m, f)
+    rhs_bc(b, onbc, tgv, 0.0)
+    return m, rowptr, colidx, vals, onbc, b

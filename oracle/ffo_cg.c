@@ -1,0 +1,129 @@
+/*
+ * ffo_cg.c — oracle restatement of FreeFEM's sparse mat-vec and Jacobi-
+ * preconditioned conjugate gradient (SolverCG + ConjugueGradient).
+ * TEST INFRASTRUCTURE ONLY (see ffo.h): the checker, never the product.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "ffo.h"
+
+/* y = A x in CSR row order (the HashMatrix non-half product
+ * HashMatrix.cpp:1138-1139, summed per row in column order). */
+void ffo_spmv(int n, const int64_t *rowptr, const int32_t *colidx,
+              const double *vals, const double *x, double *y) {
+  for (int i = 0; i < n; ++i) {
+    double s = 0.;
+    for (int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k) s += vals[k] * x[colidx[k]];
+    y[i] = s;
+  }
+}
+
+/* y = A^T x (addMatMul with Transpose: i/j swapped, HashMatrix.cpp:1094). */
+void ffo_spmv_t(int n, const int64_t *rowptr, const int32_t *colidx,
+                const double *vals, const double *x, double *y) {
+  memset(y, 0, sizeof(double) * (size_t)n);
+  for (int i = 0; i < n; ++i)
+    for (int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k) y[colidx[k]] += vals[k] * x[i];
+}
+
+static double diag_of(const int64_t *rowptr, const int32_t *colidx,
+                      const double *vals, int i, int *found) {
+  for (int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k)
+    if (colidx[k] == i) { *found = 1; return vals[k]; }
+  *found = 0;
+  return 0.;
+}
+
+double ffo_gettgv(int n, const int64_t *rowptr, const int32_t *colidx,
+                  const double *vals, int *pntgv) {
+  double ttgv = 0, max1 = 0;
+  int ntgv = 0;
+  for (int ii = 0; ii < n; ++ii) {
+    int f;
+    double a = diag_of(rowptr, colidx, vals, ii, &f);
+    if (!f) continue;
+    if (a > ttgv) { max1 = ttgv; ttgv = a; ntgv = 1; }
+    else if (a == ttgv) ++ntgv;
+    else if (a > max1) max1 = a;
+  }
+  if (max1 * 1e6 > ttgv) { ttgv = 0; ntgv = 0; }
+  if (pntgv) *pntgv = ntgv;
+  return ttgv;
+}
+
+static double dot(int n, const double *x, const double *y) {
+  double s = 0;
+  for (int i = 0; i < n; ++i) s += x[i] * y[i];
+  return s;
+}
+
+int ffo_pcg(int n, const int64_t *rowptr, const int32_t *colidx,
+            const double *vals, const double *b, double *x, double eps,
+            int itmax, double tgv, int trans, double *stats) {
+  /* HMatVirtPrecon (VirtualSolverCG.hpp:27-69): tgv rows, Jacobi diag1 */
+  int ntgv1 = 0;
+  const double tgvm = ffo_gettgv(n, rowptr, colidx, vals, &ntgv1);
+  uint8_t *wcl = NULL;
+  double *d1 = (double *)malloc(sizeof(double) * (size_t)n);
+  if (ntgv1) {
+    double tgve = tgvm;
+    if (tgve <= 0) tgve = 1e200;
+    wcl = (uint8_t *)malloc((size_t)n);
+    for (int i = 0; i < n; ++i) {
+      int f;
+      wcl[i] = diag_of(rowptr, colidx, vals, i, &f) == tgve;
+    }
+  }
+  for (int i = 0; i < n; ++i) {
+    int f;
+    d1[i] = diag_of(rowptr, colidx, vals, i, &f);
+  }
+  for (int i = 0; i < n; ++i) d1[i] = (d1[i] * d1[i] < 1e-60) ? 1 : 1. / d1[i];
+  /* SetInitWithBC (VirtualSolverCG.hpp:93-99) */
+  if (wcl)
+    for (int i = 0; i < n; ++i)
+      if (wcl[i]) x[i] = b[i] / tgv;
+  if (itmax <= 0) itmax = n; /* SolverCG(ds): itmax>0 ? itmax : n */
+
+  /* ConjugueGradient (CG.cpp:195-265) */
+  double *G = (double *)malloc(sizeof(double) * (size_t)n);
+  double *CG = (double *)malloc(sizeof(double) * (size_t)n);
+  double *H = (double *)malloc(sizeof(double) * (size_t)n);
+  double *AH = CG; /* same array as CG in the reference */
+  void (*mv)(int, const int64_t *, const int32_t *, const double *, const double *, double *) =
+      trans ? ffo_spmv_t : ffo_spmv;
+  mv(n, rowptr, colidx, vals, x, G);
+  for (int i = 0; i < n; ++i) G[i] += -1. * b[i];          /* G = Ax - b */
+  for (int i = 0; i < n; ++i) CG[i] = d1[i] * G[i];         /* C.matmul */
+  double gCg = dot(n, G, CG), gCgp;
+  const double gCg0 = gCg;
+  for (int i = 0; i < n; ++i) H[i] = CG[i];
+  for (int i = 0; i < n; ++i) H[i] *= -1.;                  /* H = -CG */
+  double eps2 = eps * eps;
+  if (eps > 0) { eps2 *= gCg; }
+  int nret = 0, iters = itmax;
+  if (gCg != gCg) nret = -1;
+  else if (gCg < 1e-30) { nret = 2; iters = 0; }
+  else
+    for (int iter = 1; iter <= itmax; ++iter) {
+      gCgp = gCg;
+      const double gh = dot(n, G, H);
+      mv(n, rowptr, colidx, vals, H, AH);
+      const double rho = -gh / dot(n, H, AH);
+      for (int i = 0; i < n; ++i) x[i] += rho * H[i];
+      for (int i = 0; i < n; ++i) G[i] += rho * AH[i];
+      for (int i = 0; i < n; ++i) CG[i] = d1[i] * G[i];
+      gCg = dot(n, G, CG);
+      const double gamma = gCg / gCgp;
+      for (int i = 0; i < n; ++i) H[i] *= gamma;
+      for (int i = 0; i < n; ++i) H[i] += -1. * CG[i];
+      if (gCg < eps2) { nret = 1; iters = iter; break; }
+    }
+  if (stats) {
+    stats[0] = iters; stats[1] = gCg0; stats[2] = gCg; stats[3] = nret;
+  }
+  free(G); free(CG); free(H); free(d1); free(wcl);
+  return nret;
+}

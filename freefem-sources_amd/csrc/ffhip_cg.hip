@@ -1,0 +1,840 @@
+// ffhip_cg.hip — Jacobi-preconditioned conjugate gradient on one MI355X.
+//
+// Replaces SolverCG / HMatVirtPrecon / ConjugueGradient
+// (src/femlib/VirtualSolverCG.hpp:13-192, src/femlib/CG.cpp:195-265).
+//
+// Layout: the solver streams the matrix in sliced ELL with 64-row slices
+// (one wavefront per slice, one lane per row, column-major inside a slice),
+// so every matrix load of a wave is one contiguous 256 B (int32) / 512 B
+// (FP64) segment and no cross-lane reduction is needed per row.
+//
+// Iteration (same recurrences as ConjugueGradient):
+//   K1: H = gamma*H_prev - C g  (formed on the fly for every gathered column)
+//       AH = A H, partial sums of (H,AH) and (g,H)
+//   K2: rho = -(g,H)/(H,AH); x += rho H; g += rho AH; Cg = D^-1 g; (g,Cg)
+// Each kernel starts by re-reducing the previous kernel's per-block partial
+// sums in a fixed order, so all blocks derive identical scalars with no
+// cross-block atomics or fences, and the result is bitwise reproducible.
+// Convergence is decided on the device (sticky done flag); the host only
+// polls every few iterations.
+//
+// MIXED mode: the FP32 copy of D^-1/2 A D^-1/2 is streamed by an FP32 CG
+// whose iterate is folded into the FP64 solution and whose residual is
+// replaced by the FP64 residual b - A x at reliable-update points
+// (iterative refinement that keeps the Krylov direction); dot products are
+// accumulated in FP64 and convergence is only accepted on the FP64 residual.
+#include <hipcub/hipcub.hpp>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "ffhip_internal.hpp"
+
+namespace ffhip {
+void build_v2e(ffhip_mesh m);
+int read_flag(ffhip_ctx ctx, int *d);
+bool transpose_map(ffhip_ctx ctx, ffhip_mat A, int64_t *tmap);
+
+constexpr int kSlice = 64;
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / kSlice;
+
+struct CgState {
+  double gcg_prev[2];  // (g, Cg) at the start of the iteration, by parity
+  double gcg0, eps2;
+  double sigma;        // MIXED: scale of the FP32 residual
+  double rho32;        // MIXED: (r32, r32) at the start of the last executed step
+  int done;            // 0 run, 1 converged, 2 gcg0 < 1e-30, 3 itmax, 4 NaN
+  int iters;
+  int need_update;     // MIXED: reliable update requested
+  int refinements;
+};
+
+// ---------------------------------------------------------------- reductions
+// Sum of P partials (stride `stride`, component `c`), identical in every block.
+__device__ inline double reduce_partials(const double *__restrict__ p, int P, int stride, int c,
+                                         double *lds) {
+  double s = 0.;
+  for (int i = threadIdx.x; i < P; i += kBlock) s += p[(int64_t)i * stride + c];
+  return block_sum<kBlock>(s, lds);
+}
+
+template <class T>
+__device__ inline T ntload(const T *p) { return __builtin_nontemporal_load(p); }
+
+// logical slice range of this block
+__device__ inline void block_slices(int64_t S, int64_t &s0, int64_t &s1) {
+  const int64_t G = gridDim.x, lb = xcd_block_id();
+  s0 = (lb * S) / G;
+  s1 = ((lb + 1) * S) / G;
+}
+
+// ---------------------------------------------------------------- SELL build
+__global__ void k_sell_len(int64_t n, int64_t S, const int64_t *__restrict__ rowptr,
+                           int32_t *__restrict__ len, int64_t *__restrict__ cnt) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S) return;
+  int64_t m = 0;
+  for (int l = 0; l < kSlice; ++l) {
+    const int64_t r = s * kSlice + l;
+    if (r < n) {
+      const int64_t c = rowptr[r + 1] - rowptr[r];
+      m = c > m ? c : m;
+    }
+  }
+  len[s] = (int32_t)m;
+  cnt[s] = m * kSlice;
+}
+
+// vals_src may be permuted through tmap (transposed values)
+__global__ void k_sell_fill(int64_t n, int64_t S, const int64_t *__restrict__ rowptr,
+                            const int32_t *__restrict__ colidx, const double *__restrict__ vals,
+                            const int64_t *__restrict__ tmap, const int64_t *__restrict__ sptr,
+                            const int32_t *__restrict__ len, const double *__restrict__ dsc,
+                            int32_t *__restrict__ scol, double *__restrict__ sval,
+                            float *__restrict__ sval32) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= S * kSlice) return;
+  const int64_t s = r / kSlice;
+  const int lane = (int)(r % kSlice);
+  const int L = len[s];
+  const int64_t base = sptr[s] + lane;
+  const int64_t rp = r < n ? rowptr[r] : 0, rl = r < n ? rowptr[r + 1] - rp : 0;
+  for (int k = 0; k < L; ++k) {
+    const int64_t idx = base + (int64_t)k * kSlice;
+    if (k < rl) {
+      const int32_t c = colidx[rp + k];
+      const double a = tmap ? vals[tmap[rp + k]] : vals[rp + k];
+      if (scol) scol[idx] = c;
+      if (sval) sval[idx] = a;
+      if (sval32) sval32[idx] = (float)(a * dsc[r] * dsc[c]);
+    } else {
+      const int32_t c = r < n ? (int32_t)r : 0;
+      if (scol) scol[idx] = c;
+      if (sval) sval[idx] = 0.;
+      if (sval32) sval32[idx] = 0.f;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- Jacobi / tgv
+__global__ void k_diag_vals(int64_t n, const int64_t *__restrict__ diag,
+                            const double *__restrict__ vals, double *__restrict__ d,
+                            uint8_t *__restrict__ has) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int64_t s = diag[i];
+  d[i] = s >= 0 ? vals[s] : 0.;
+  has[i] = s >= 0;
+}
+
+// pass 1: max diagonal; pass 2: count of the max, max of the rest
+// (HashMatrix::gettgv, HashMatrix.cpp:1340-1371)
+__global__ void k_tgv_pass(int64_t n, const double *__restrict__ d, const uint8_t *__restrict__ has,
+                           double top, int pass, double *__restrict__ out) {
+  __shared__ double lds[kBlock / kWave];
+  double m = 0., c = 0.;  // the reference starts from ttgv = max1 = 0
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+    if (!has[i]) continue;
+    const double a = d[i];
+    if (pass == 1) m = fmax(m, a);
+    else if (a == top) c += 1.;
+    else m = fmax(m, a);
+  }
+  // block max via shuffles
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, kWave));
+  const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  if (lane == 0) lds[w] = m;
+  __syncthreads();
+  double bm = 0.;
+  for (int i = 0; i < kBlock / kWave; ++i) bm = fmax(bm, lds[i]);
+  const double bc = block_sum<kBlock>(c, lds);
+  if (threadIdx.x == 0) { out[2 * blockIdx.x] = bm; out[2 * blockIdx.x + 1] = bc; }
+}
+
+__global__ void k_jacobi(int64_t n, const double *__restrict__ d, const uint8_t *__restrict__ has,
+                         double tgve, int with_wcl, double *__restrict__ dinv,
+                         double *__restrict__ dsc, uint8_t *__restrict__ wcl) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double a = has[i] ? d[i] : 0.;  // a missing diagonal reads as 0 (then 1)
+  dinv[i] = (a * a < 1e-60) ? 1. : 1. / a;
+  dsc[i] = a > 0. ? 1. / sqrt(a) : 1.;
+  wcl[i] = with_wcl ? (has[i] && d[i] == tgve) : 0;
+}
+
+// SetInitWithBC (VirtualSolverCG.hpp:93-99)
+__global__ void k_init_bc(int64_t n, const uint8_t *__restrict__ wcl, const double *__restrict__ b,
+                          double tgv, double *__restrict__ x) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && wcl[i]) x[i] = b[i] / tgv;
+}
+
+// ---------------------------------------------------------------- FP64 PCG
+// G = A x - b, CG = D^-1 G, H0 = 0, partial (G, CG)
+__global__ __launch_bounds__(kBlock) void k_pcg_init(
+    int64_t n, int64_t S, const int64_t *__restrict__ sptr, const int32_t *__restrict__ slen,
+    const int32_t *__restrict__ scol, const double *__restrict__ sval,
+    const double *__restrict__ x, const double *__restrict__ b, const double *__restrict__ dinv,
+    double *__restrict__ G, double *__restrict__ CG, double *__restrict__ H0,
+    double *__restrict__ pout, CgState *st) {
+  __shared__ double lds[kBlock / kWave];
+  int64_t s0, s1;
+  block_slices(S, s0, s1);
+  const int w = threadIdx.x / kSlice, lane = threadIdx.x % kSlice;
+  double part = 0.;
+  for (int64_t s = s0 + w; s < s1; s += kWavesPerBlock) {
+    const int64_t base = sptr[s] + lane;
+    const int L = slen[s];
+    double acc = 0.;
+    for (int k = 0; k < L; ++k) {
+      const int64_t idx = base + (int64_t)k * kSlice;
+      acc += ntload(sval + idx) * x[ntload(scol + idx)];
+    }
+    const int64_t r = s * kSlice + lane;
+    if (r < n) {
+      const double g = acc + -1. * b[r];
+      const double c = dinv[r] * g;
+      G[r] = g; CG[r] = c; H0[r] = 0.;
+      part += g * c;
+    }
+  }
+  const double tot = block_sum<kBlock>(part, lds);
+  if (threadIdx.x == 0) pout[xcd_block_id()] = tot;
+  if (blockIdx.x == 0 && threadIdx.x == 0) { st->done = 0; st->iters = 0; }
+}
+
+__global__ __launch_bounds__(kBlock) void k_pcg_dir(
+    int it, int itmax, double eps, int64_t n, int64_t S, const int64_t *__restrict__ sptr,
+    const int32_t *__restrict__ slen, const int32_t *__restrict__ scol,
+    const double *__restrict__ sval, const double *__restrict__ Hold,
+    const double *__restrict__ CG, const double *__restrict__ G, double *__restrict__ Hnew,
+    double *__restrict__ AH, const double *__restrict__ pin, double *__restrict__ pout,
+    CgState *st) {
+  __shared__ double lds[kBlock / kWave];
+  if (st->done) return;
+  const int P = gridDim.x;
+  const double gcg = reduce_partials(pin, P, 1, 0, lds);
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  double gamma = 0.;
+  if (it == 1) {
+    const double eps2 = eps > 0 ? eps * eps * gcg : eps * eps;
+    int done = 0;
+    if (gcg != gcg) done = 4;
+    else if (gcg < 1e-30) done = 2;
+    if (lead) {
+      st->gcg0 = gcg; st->eps2 = eps2; st->gcg_prev[1] = gcg;
+      if (done) { st->done = done; st->iters = 0; }
+    }
+    if (done) return;
+  } else {
+    const double gcgp = st->gcg_prev[(it - 1) & 1];
+    const double eps2 = st->eps2;
+    gamma = gcg / gcgp;
+    int done = 0;
+    if (gcg != gcg) done = 4;
+    else if (gcg < eps2) done = 1;
+    else if (it - 1 >= itmax) done = 3;
+    if (lead) {
+      st->gcg_prev[it & 1] = gcg;
+      if (done) { st->done = done; st->iters = it - 1; }
+    }
+    if (done) return;
+  }
+  int64_t s0, s1;
+  block_slices(S, s0, s1);
+  const int w = threadIdx.x / kSlice, lane = threadIdx.x % kSlice;
+  double p_hah = 0., p_gh = 0.;
+  for (int64_t s = s0 + w; s < s1; s += kWavesPerBlock) {
+    const int64_t base = sptr[s] + lane;
+    const int L = slen[s];
+    double acc = 0.;
+    for (int k = 0; k < L; ++k) {
+      const int64_t idx = base + (int64_t)k * kSlice;
+      const int32_t j = ntload(scol + idx);
+      acc += ntload(sval + idx) * (gamma * Hold[j] - CG[j]);
+    }
+    const int64_t r = s * kSlice + lane;
+    if (r < n) {
+      const double h = gamma * Hold[r] - CG[r];
+      Hnew[r] = h;
+      AH[r] = acc;
+      p_hah += h * acc;
+      p_gh += G[r] * h;
+    }
+  }
+  const double a = block_sum<kBlock>(p_hah, lds);
+  const double c = block_sum<kBlock>(p_gh, lds);
+  if (threadIdx.x == 0) {
+    const int lb = xcd_block_id();
+    pout[2 * lb] = a;
+    pout[2 * lb + 1] = c;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_pcg_update(
+    int64_t n, int64_t S, const double *__restrict__ H, const double *__restrict__ AH,
+    const double *__restrict__ dinv, double *__restrict__ x, double *__restrict__ G,
+    double *__restrict__ CG, const double *__restrict__ pin, double *__restrict__ pout,
+    CgState *st) {
+  __shared__ double lds[kBlock / kWave];
+  if (st->done) return;
+  const int P = gridDim.x;
+  const double hah = reduce_partials(pin, P, 2, 0, lds);
+  const double gh = reduce_partials(pin, P, 2, 1, lds);
+  const double rho = -gh / hah;
+  int64_t s0, s1;
+  block_slices(S, s0, s1);
+  double part = 0.;
+  const int64_t r0 = s0 * kSlice, r1 = (s1 * kSlice < n) ? s1 * kSlice : n;
+  for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) {
+    x[r] += rho * H[r];
+    const double g = G[r] + rho * AH[r];
+    G[r] = g;
+    const double c = dinv[r] * g;
+    CG[r] = c;
+    part += g * c;
+  }
+  const double tot = block_sum<kBlock>(part, lds);
+  if (threadIdx.x == 0) pout[xcd_block_id()] = tot;
+}
+
+// ---------------------------------------------------------------- residual check
+// r = b - A x (FP64); partials (r,r), (b,b), (r, D^-1 r)
+__global__ __launch_bounds__(kBlock) void k_residual(
+    int64_t n, int64_t S, const int64_t *__restrict__ sptr, const int32_t *__restrict__ slen,
+    const int32_t *__restrict__ scol, const double *__restrict__ sval,
+    const double *__restrict__ x, const double *__restrict__ b, const double *__restrict__ dinv,
+    double *__restrict__ rout, double *__restrict__ pout) {
+  __shared__ double lds[kBlock / kWave];
+  int64_t s0, s1;
+  block_slices(S, s0, s1);
+  const int w = threadIdx.x / kSlice, lane = threadIdx.x % kSlice;
+  double prr = 0., pbb = 0., prc = 0.;
+  for (int64_t s = s0 + w; s < s1; s += kWavesPerBlock) {
+    const int64_t base = sptr[s] + lane;
+    const int L = slen[s];
+    double acc = 0.;
+    for (int k = 0; k < L; ++k) {
+      const int64_t idx = base + (int64_t)k * kSlice;
+      acc += ntload(sval + idx) * x[ntload(scol + idx)];
+    }
+    const int64_t r = s * kSlice + lane;
+    if (r < n) {
+      const double rr = b[r] - acc;
+      if (rout) rout[r] = rr;
+      prr += rr * rr;
+      pbb += b[r] * b[r];
+      prc += rr * (dinv[r] * rr);
+    }
+  }
+  const double a = block_sum<kBlock>(prr, lds);
+  const double c = block_sum<kBlock>(pbb, lds);
+  const double e = block_sum<kBlock>(prc, lds);
+  if (threadIdx.x == 0) {
+    const int lb = xcd_block_id();
+    pout[3 * lb] = a; pout[3 * lb + 1] = c; pout[3 * lb + 2] = e;
+  }
+}
+
+// plain SELL mat-vec for measurement: y = A x
+template <class V, class X>
+__global__ __launch_bounds__(kBlock) void k_sell_spmv(
+    int64_t n, int64_t S, const int64_t *__restrict__ sptr, const int32_t *__restrict__ slen,
+    const int32_t *__restrict__ scol, const V *__restrict__ sval, const X *__restrict__ x,
+    X *__restrict__ y) {
+  int64_t s0, s1;
+  block_slices(S, s0, s1);
+  const int w = threadIdx.x / kSlice, lane = threadIdx.x % kSlice;
+  for (int64_t s = s0 + w; s < s1; s += kWavesPerBlock) {
+    const int64_t base = sptr[s] + lane;
+    const int L = slen[s];
+    X acc = 0;
+    for (int k = 0; k < L; ++k) {
+      const int64_t idx = base + (int64_t)k * kSlice;
+      acc += (X)ntload(sval + idx) * x[ntload(scol + idx)];
+    }
+    const int64_t r = s * kSlice + lane;
+    if (r < n) y[r] = acc;
+  }
+}
+
+
+// ---------------------------------------------------------------- MIXED (FP32 inner)
+// Mixed-precision state lives in CgState: sigma (scale of r32), rho32 (last
+// (r32,r32)), and the per-parity (r32,r32) history below.
+struct MixState {
+  double rho_prev[2];   // (r32, r32) at the start of inner iteration, by parity
+  double cbeta;         // coefficient of p_old in the first iteration after an update
+  int first_it;         // inner iteration that uses cbeta
+  int since_update;     // host-side bookkeeping copy
+};
+
+// K_m1: p = r32 + cb*p_old (on the fly for every gathered column); q = Ahat p;
+// partial (p, q).  Requests a reliable update when (r32, r32) fell by 1e-4
+// since the last update, or the recurrence says the FP64 criterion is met.
+__global__ __launch_bounds__(kBlock) void k_mix_dir(
+    int it, int itmax, int64_t n, int64_t S, const int64_t *__restrict__ sptr,
+    const int32_t *__restrict__ slen, const int32_t *__restrict__ scol,
+    const float *__restrict__ sval, const float *__restrict__ pold,
+    const float *__restrict__ r32, float *__restrict__ pnew, float *__restrict__ q,
+    const double *__restrict__ pin, double *__restrict__ pout, CgState *st, MixState *ms) {
+  __shared__ double lds[kBlock / kWave];
+  if (st->done || st->need_update) return;
+  const int P = gridDim.x;
+  const double rho = reduce_partials(pin, P, 1, 0, lds);
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  const double sig = st->sigma;
+  int req = 0;
+  if (rho != rho) req = 4;
+  else if (rho * sig * sig < st->eps2 || rho < 1e-4) req = 1;  // check on the FP64 residual
+  else if (it - 1 >= itmax) req = 3;
+  if (req) {
+    if (lead) {
+      if (req == 4) st->done = 4;
+      else if (req == 3) st->done = 3;
+      else st->need_update = 1;
+      // (r32, r32) of the last executed iteration: the denominator of beta once
+      // the newest recurrence residual is replaced by the FP64 one
+      st->rho32 = ms->rho_prev[(it - 1) & 1];
+      st->iters = it - 1;
+    }
+    return;
+  }
+  const double cb = (it == ms->first_it) ? ms->cbeta : rho / ms->rho_prev[(it - 1) & 1];
+  if (lead) { ms->rho_prev[it & 1] = rho; st->iters = it - 1; }
+  const float cbf = (float)cb;
+  int64_t s0, s1;
+  block_slices(S, s0, s1);
+  const int w = threadIdx.x / kSlice, lane = threadIdx.x % kSlice;
+  double part = 0.;
+  for (int64_t s = s0 + w; s < s1; s += kWavesPerBlock) {
+    const int64_t base = sptr[s] + lane;
+    const int L = slen[s];
+    float acc = 0.f;
+    for (int k = 0; k < L; ++k) {
+      const int64_t idx = base + (int64_t)k * kSlice;
+      const int32_t j = ntload(scol + idx);
+      acc += ntload(sval + idx) * (r32[j] + cbf * pold[j]);
+    }
+    const int64_t r = s * kSlice + lane;
+    if (r < n) {
+      const float p = r32[r] + cbf * pold[r];
+      pnew[r] = p;
+      q[r] = acc;
+      part += (double)p * (double)acc;
+    }
+  }
+  const double tot = block_sum<kBlock>(part, lds);
+  if (threadIdx.x == 0) pout[xcd_block_id()] = tot;
+}
+
+// K_m2: alpha = rho/(p,q); y32 += alpha p; r32 -= alpha q; partial (r32, r32)
+__global__ __launch_bounds__(kBlock) void k_mix_update(
+    int it, int64_t n, int64_t S, const float *__restrict__ p, const float *__restrict__ q,
+    float *__restrict__ y32, float *__restrict__ r32, const double *__restrict__ pin,
+    double *__restrict__ pout, CgState *st, MixState *ms) {
+  __shared__ double lds[kBlock / kWave];
+  if (st->done || st->need_update) return;
+  const int P = gridDim.x;
+  const double pq = reduce_partials(pin, P, 1, 0, lds);
+  const double alpha = ms->rho_prev[it & 1] / pq;
+  const float af = (float)alpha;
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->iters = it;
+  int64_t s0, s1;
+  block_slices(S, s0, s1);
+  double part = 0.;
+  const int64_t r0 = s0 * kSlice, r1 = (s1 * kSlice < n) ? s1 * kSlice : n;
+  for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) {
+    y32[r] += af * p[r];
+    const float rr = r32[r] - af * q[r];
+    r32[r] = rr;
+    part += (double)rr * (double)rr;
+  }
+  const double tot = block_sum<kBlock>(part, lds);
+  if (threadIdx.x == 0) pout[xcd_block_id()] = tot;
+}
+
+// fold the FP32 correction into the FP64 solution: x += D^-1/2 (sigma y32); y32 = 0
+__global__ void k_mix_fold(int64_t n, double sigma, const double *__restrict__ dsc,
+                           float *__restrict__ y32, double *__restrict__ x) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  x[i] += dsc[i] * (sigma * (double)y32[i]);
+  y32[i] = 0.f;
+}
+
+// new scaled residual r32 = D^-1/2 r / sigma (r = b - A x in FP64); partial (r32,r32)
+__global__ __launch_bounds__(kBlock) void k_mix_rescale(
+    int64_t n, int64_t S, double inv_sigma, const double *__restrict__ dsc,
+    const double *__restrict__ r, float *__restrict__ r32, double *__restrict__ pout) {
+  __shared__ double lds[kBlock / kWave];
+  int64_t s0, s1;
+  block_slices(S, s0, s1);
+  double part = 0.;
+  const int64_t r0 = s0 * kSlice, r1 = (s1 * kSlice < n) ? s1 * kSlice : n;
+  for (int64_t i = r0 + threadIdx.x; i < r1; i += kBlock) {
+    const float v = (float)(dsc[i] * r[i] * inv_sigma);
+    r32[i] = v;
+    part += (double)v * (double)v;
+  }
+  const double tot = block_sum<kBlock>(part, lds);
+  if (threadIdx.x == 0) pout[xcd_block_id()] = tot;
+}
+
+static unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace ffhip
+
+using namespace ffhip;
+
+struct ffhip_cg_s {
+  ffhip_ctx ctx = nullptr;
+  ffhip_mat A = nullptr;
+  int mode = 0;
+  int64_t n = 0, S = 0, ents = 0;
+  int grid = 0;
+  // SELL
+  int64_t *sptr = nullptr;
+  int32_t *slen = nullptr, *scol = nullptr;
+  double *sval = nullptr;    // A
+  double *svalT = nullptr;   // A^T (built on first transposed solve)
+  float *sval32 = nullptr;   // D^-1/2 A D^-1/2 (MIXED)
+  float *sval32T = nullptr;
+  // Jacobi / tgv
+  double *dinv = nullptr, *dsc = nullptr;
+  uint8_t *wcl = nullptr;
+  double tgvm = 0.;
+  int ntgv = 0;
+  // work
+  double *xw = nullptr, *bw = nullptr, *G = nullptr, *CG = nullptr, *H[2] = {nullptr, nullptr},
+         *AH = nullptr;
+  float *r32 = nullptr, *p32[2] = {nullptr, nullptr}, *q32 = nullptr, *y32 = nullptr;
+  double *pA = nullptr, *pB = nullptr, *pC = nullptr;
+  CgState *st = nullptr;
+  MixState *ms = nullptr;
+};
+
+static void cg_build_sell(ffhip_cg cg, bool transposed) {
+  ffhip_ctx ctx = cg->ctx;
+  ffhip_mat A = cg->A;
+  hipStream_t s = ctx->stream;
+  int64_t *tmap = nullptr;
+  if (transposed) {
+    tmap = dalloc<int64_t>(A->nnz);
+    // transposed values through the symmetric-pattern slot map
+    const bool ok = transpose_map(ctx, A, tmap);
+    if (!ok) dfree(tmap);
+    FFHIP_REQUIRE(ok, FFHIP_ERR_UNSUPPORTED, "transposed solve needs a symmetric pattern");
+  }
+  if (!cg->sptr) {
+    cg->sptr = dalloc<int64_t>(cg->S + 1);
+    cg->slen = dalloc<int32_t>(cg->S);
+    int64_t *cnt = dalloc<int64_t>(cg->S + 1);
+    FFHIP_CHECK(hipMemsetAsync(cnt + cg->S, 0, 8, s));
+    k_sell_len<<<nblk(cg->S, 256), 256, 0, s>>>(A->n, cg->S, A->rowptr, cg->slen, cnt);
+    exclusive_scan_i64(ctx, cnt, cg->sptr, cg->S + 1);
+    cg->ents = read_i64(ctx, cg->sptr + cg->S);
+    dfree(cnt);
+    cg->scol = dalloc<int32_t>(cg->ents);
+  }
+  double *&dv = transposed ? cg->svalT : cg->sval;
+  float *&fv = transposed ? cg->sval32T : cg->sval32;
+  if (!dv) dv = dalloc<double>(cg->ents);
+  if (cg->mode == FFHIP_CG_MIXED && !fv) fv = dalloc<float>(cg->ents);
+  k_sell_fill<<<nblk(cg->S * kSlice, 256), 256, 0, s>>>(
+      A->n, cg->S, A->rowptr, A->colidx, A->vals, tmap, cg->sptr, cg->slen, cg->dsc,
+      transposed ? nullptr : cg->scol, dv, cg->mode == FFHIP_CG_MIXED ? fv : nullptr);
+  FFHIP_CHECK(hipGetLastError());
+  FFHIP_CHECK(hipStreamSynchronize(s));
+  dfree(tmap);
+}
+
+static void cg_setup_jacobi(ffhip_cg cg) {
+  ffhip_ctx ctx = cg->ctx;
+  ffhip_mat A = cg->A;
+  hipStream_t s = ctx->stream;
+  const int64_t n = A->n;
+  double *d = dalloc<double>(n);
+  uint8_t *has = dalloc<uint8_t>(n);
+  k_diag_vals<<<nblk(n, 256), 256, 0, s>>>(n, A->diag, A->vals, d, has);
+  const int GB = 512;
+  double *part = dalloc<double>(2 * GB);
+  std::vector<double> hp(2 * GB);
+  k_tgv_pass<<<GB, kBlock, 0, s>>>(n, d, has, 0., 1, part);
+  FFHIP_CHECK(hipMemcpyAsync(hp.data(), part, 16 * GB, hipMemcpyDeviceToHost, s));
+  FFHIP_CHECK(hipStreamSynchronize(s));
+  double top = 0.;
+  for (int i = 0; i < GB; ++i) top = std::max(top, hp[2 * i]);
+  k_tgv_pass<<<GB, kBlock, 0, s>>>(n, d, has, top, 2, part);
+  FFHIP_CHECK(hipMemcpyAsync(hp.data(), part, 16 * GB, hipMemcpyDeviceToHost, s));
+  FFHIP_CHECK(hipStreamSynchronize(s));
+  double second = 0., count = 0.;
+  for (int i = 0; i < GB; ++i) { second = std::max(second, hp[2 * i]); count += hp[2 * i + 1]; }
+  // gettgv: a top value not 1e6 x above the next distinct value is no tgv
+  double ttgv = top;
+  int ntgv = (int)count;
+  if (second * 1e6 > ttgv) { ttgv = 0; ntgv = 0; }
+  cg->tgvm = ttgv;
+  cg->ntgv = ntgv;
+  const double tgve = ttgv <= 0 ? 1e200 : ttgv;
+  k_jacobi<<<nblk(n, 256), 256, 0, s>>>(n, d, has, tgve, ntgv > 0, cg->dinv, cg->dsc, cg->wcl);
+  FFHIP_CHECK(hipGetLastError());
+  FFHIP_CHECK(hipStreamSynchronize(s));
+  dfree(d); dfree(has); dfree(part);
+}
+
+
+// residual r = b - A x into cg->G; returns (r, D^-1 r) summed on the host in
+// block order
+static double mix_residual(ffhip_cg cg, const double *sval) {
+  hipStream_t s = cg->ctx->stream;
+  k_residual<<<cg->grid, kBlock, 0, s>>>(cg->n, cg->S, cg->sptr, cg->slen, cg->scol, sval, cg->xw,
+                                         cg->bw, cg->dinv, cg->G, cg->pC);
+  FFHIP_CHECK(hipGetLastError());
+  std::vector<double> h(3 * (size_t)cg->grid);
+  FFHIP_CHECK(hipMemcpyAsync(h.data(), cg->pC, 24 * cg->grid, hipMemcpyDeviceToHost, s));
+  FFHIP_CHECK(hipStreamSynchronize(s));
+  double g = 0.;
+  for (int i = 0; i < cg->grid; ++i) g += h[3 * i + 2];
+  return g;
+}
+
+// FP32 Jacobi-scaled CG with FP64 reliable updates (see the file comment).
+int ffhip_cg_solve_mixed(ffhip_cg cg, const double *sval, const float *sval32, double eps,
+                         int itmax, CgState &hs) {
+  ffhip_ctx ctx = cg->ctx;
+  hipStream_t s = ctx->stream;
+  const int64_t n = cg->n;
+  const int G = cg->grid;
+  FFHIP_CHECK(hipMemsetAsync(cg->p32[0], 0, 4 * n, s));
+  FFHIP_CHECK(hipMemsetAsync(cg->p32[1], 0, 4 * n, s));
+  FFHIP_CHECK(hipMemsetAsync(cg->y32, 0, 4 * n, s));
+  memset(&hs, 0, sizeof(hs));
+  MixState hm;
+  memset(&hm, 0, sizeof(hm));
+  double gcg = mix_residual(cg, sval);
+  hs.refinements = 1;
+  hs.gcg0 = gcg;
+  hs.eps2 = eps > 0 ? eps * eps * gcg : eps * eps;
+  if (gcg != gcg) { hs.done = 4; return 0; }
+  if (gcg < 1e-30) { hs.done = 2; return 0; }
+  double sigma = std::sqrt(gcg);
+  double rho_last = 0.;
+  int it = 1;
+  hm.cbeta = 0.;
+  hm.first_it = 1;
+  for (;;) {
+    // (re)start the FP32 recurrence from the FP64 residual
+    k_mix_rescale<<<G, kBlock, 0, s>>>(n, cg->S, 1. / sigma, cg->dsc, cg->G, cg->r32, cg->pA);
+    hs.sigma = sigma;
+    hs.need_update = 0;
+    hs.iters = it - 1;
+    FFHIP_CHECK(hipMemcpyAsync(cg->st, &hs, sizeof(hs), hipMemcpyHostToDevice, s));
+    FFHIP_CHECK(hipMemcpyAsync(cg->ms, &hm, sizeof(hm), hipMemcpyHostToDevice, s));
+    const int chunk = 16;
+    for (;;) {
+      for (int c = 0; c < chunk; ++c, ++it) {
+        const float *pold = cg->p32[(it - 1) & 1];
+        float *pnew = cg->p32[it & 1];
+        k_mix_dir<<<G, kBlock, 0, s>>>(it, itmax, n, cg->S, cg->sptr, cg->slen, cg->scol, sval32,
+                                       pold, cg->r32, pnew, cg->q32, cg->pA, cg->pB, cg->st, cg->ms);
+        k_mix_update<<<G, kBlock, 0, s>>>(it, n, cg->S, pnew, cg->q32, cg->y32, cg->r32, cg->pB,
+                                          cg->pA, cg->st, cg->ms);
+      }
+      FFHIP_CHECK(hipGetLastError());
+      FFHIP_CHECK(hipMemcpyAsync(&hs, cg->st, sizeof(hs), hipMemcpyDeviceToHost, s));
+      FFHIP_CHECK(hipStreamSynchronize(s));
+      if (hs.done || hs.need_update) break;
+    }
+    if (hs.done) {  // NaN or itmax: fold what we have
+      k_mix_fold<<<nblk(n, 256), 256, 0, s>>>(n, sigma, cg->dsc, cg->y32, cg->xw);
+      return 0;
+    }
+    // reliable update: fold, recompute the FP64 residual, accept or rescale
+    if (hs.iters + 1 == hm.first_it && hs.refinements > 1) {  // no inner progress since the last update
+      hs.done = 3;
+      k_mix_fold<<<nblk(n, 256), 256, 0, s>>>(n, sigma, cg->dsc, cg->y32, cg->xw);
+      return 0;
+    }
+    it = hs.iters + 1;
+    rho_last = hs.rho32;
+    k_mix_fold<<<nblk(n, 256), 256, 0, s>>>(n, sigma, cg->dsc, cg->y32, cg->xw);
+    gcg = mix_residual(cg, sval);
+    hs.refinements += 1;
+    if (gcg != gcg) { hs.done = 4; return 0; }
+    if (gcg < hs.eps2) { hs.done = 1; return 0; }
+    if (it - 1 >= itmax) { hs.done = 3; return 0; }
+    const double sig_new = std::sqrt(gcg);
+    // p_true = r_true + beta p_true_old, beta = gcg / (rho_last sigma^2), in units of sig_new
+    hm.cbeta = (gcg / (rho_last * sigma * sigma)) * (sigma / sig_new);
+    hm.first_it = it;
+    sigma = sig_new;
+  }
+}
+
+extern "C" {
+
+int ffhip_cg_create(ffhip_ctx ctx, ffhip_mat A, int mode, ffhip_cg *out) {
+  return guard([&] {
+    FFHIP_REQUIRE(ctx && A && out, FFHIP_ERR_ARG, "ffhip_cg_create: null argument");
+    FFHIP_REQUIRE(mode == FFHIP_CG_FP64 || mode == FFHIP_CG_MIXED, FFHIP_ERR_ARG,
+                  "ffhip_cg_create: mode %d", mode);
+    ffhip_cg cg = new ffhip_cg_s();
+    cg->ctx = ctx; cg->A = A; cg->mode = mode; cg->n = A->n;
+    cg->S = (A->n + kSlice - 1) / kSlice;
+    int64_t g = (cg->S + kWavesPerBlock - 1) / kWavesPerBlock;
+    const int64_t gmax = 8 * (int64_t)ctx->num_cu;
+    if (g > gmax) g = gmax;
+    if (g >= 64) g -= g % 8;  // whole XCD groups
+    cg->grid = (int)g;
+    const int64_t n = A->n;
+    cg->dinv = dalloc<double>(n);
+    cg->dsc = dalloc<double>(n);
+    cg->wcl = dalloc<uint8_t>(n);
+    cg_setup_jacobi(cg);
+    cg_build_sell(cg, false);
+    cg->xw = dalloc<double>(n); cg->bw = dalloc<double>(n);
+    cg->G = dalloc<double>(n); cg->CG = dalloc<double>(n);
+    cg->H[0] = dalloc<double>(n); cg->H[1] = dalloc<double>(n);
+    cg->AH = dalloc<double>(n);
+    if (mode == FFHIP_CG_MIXED) {
+      cg->r32 = dalloc<float>(n); cg->q32 = dalloc<float>(n); cg->y32 = dalloc<float>(n);
+      cg->p32[0] = dalloc<float>(n); cg->p32[1] = dalloc<float>(n);
+    }
+    cg->pA = dalloc<double>(3 * (int64_t)cg->grid);
+    cg->pB = dalloc<double>(3 * (int64_t)cg->grid);
+    cg->pC = dalloc<double>(3 * (int64_t)cg->grid);
+    cg->st = dalloc<CgState>(1);
+    cg->ms = dalloc<MixState>(1);
+    *out = cg;
+  });
+}
+
+int ffhip_cg_update_values(ffhip_cg cg) {
+  return guard([&] {
+    FFHIP_REQUIRE(cg, FFHIP_ERR_ARG, "ffhip_cg_update_values: null solver");
+    cg_setup_jacobi(cg);
+    cg_build_sell(cg, false);
+    if (cg->svalT) cg_build_sell(cg, true);
+  });
+}
+
+static void true_residual(ffhip_cg cg, const double *sval, const double *x, const double *b,
+                          double out[3]) {
+  hipStream_t s = cg->ctx->stream;
+  k_residual<<<cg->grid, kBlock, 0, s>>>(cg->n, cg->S, cg->sptr, cg->slen, cg->scol, sval, x, b,
+                                         cg->dinv, nullptr, cg->pC);
+  FFHIP_CHECK(hipGetLastError());
+  std::vector<double> h(3 * (size_t)cg->grid);
+  FFHIP_CHECK(hipMemcpyAsync(h.data(), cg->pC, 24 * cg->grid, hipMemcpyDeviceToHost, s));
+  FFHIP_CHECK(hipStreamSynchronize(s));
+  out[0] = out[1] = out[2] = 0.;
+  for (int i = 0; i < cg->grid; ++i)
+    for (int c = 0; c < 3; ++c) out[c] += h[3 * i + c];
+}
+
+int ffhip_cg_solve(ffhip_cg cg, const double *b, double *x, int on_device, double eps, int itmax,
+                   double tgv, int trans, ffhip_cg_stats *stats) {
+  return guard([&] {
+    FFHIP_REQUIRE(cg && b && x, FFHIP_ERR_ARG, "ffhip_cg_solve: null argument");
+    ffhip_ctx ctx = cg->ctx;
+    hipStream_t s = ctx->stream;
+    const int64_t n = cg->n;
+    if (trans && !cg->svalT) cg_build_sell(cg, true);
+    const double *sval = trans ? cg->svalT : cg->sval;
+    const float *sval32 = trans ? cg->sval32T : cg->sval32;
+    if (itmax <= 0) itmax = (int)n;  // SolverCG(ds): itmax > 0 ? itmax : n
+    FFHIP_CHECK(hipMemcpyAsync(cg->bw, b, 8 * n, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+    FFHIP_CHECK(hipMemcpyAsync(cg->xw, x, 8 * n, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+    ctx->tic();
+    if (cg->ntgv > 0) k_init_bc<<<nblk(n, 256), 256, 0, s>>>(n, cg->wcl, cg->bw, tgv, cg->xw);
+    CgState hs;
+    memset(&hs, 0, sizeof(hs));
+    if (cg->mode == FFHIP_CG_FP64) {
+      const int G = cg->grid;
+      k_pcg_init<<<G, kBlock, 0, s>>>(n, cg->S, cg->sptr, cg->slen, cg->scol, sval, cg->xw, cg->bw,
+                                     cg->dinv, cg->G, cg->CG, cg->H[0], cg->pA, cg->st);
+      FFHIP_CHECK(hipGetLastError());
+      const int chunk = 16;
+      for (int it = 1;; ++it) {
+        const double *Hold = cg->H[(it - 1) & 1];
+        double *Hnew = cg->H[it & 1];
+        k_pcg_dir<<<G, kBlock, 0, s>>>(it, itmax, eps, n, cg->S, cg->sptr, cg->slen, cg->scol, sval,
+                                       Hold, cg->CG, cg->G, Hnew, cg->AH, cg->pA, cg->pB, cg->st);
+        k_pcg_update<<<G, kBlock, 0, s>>>(n, cg->S, Hnew, cg->AH, cg->dinv, cg->xw, cg->G, cg->CG,
+                                          cg->pB, cg->pA, cg->st);
+        if (it % chunk == 0) {
+          FFHIP_CHECK(hipGetLastError());
+          FFHIP_CHECK(hipMemcpyAsync(&hs, cg->st, sizeof(hs), hipMemcpyDeviceToHost, s));
+          FFHIP_CHECK(hipStreamSynchronize(s));
+          if (hs.done) break;
+        }
+      }
+    } else {
+      ffhip_cg_solve_mixed(cg, sval, sval32, eps, itmax, hs);
+    }
+    ctx->toc();
+    const double ms = ctx->last_ms;
+    double res[3];
+    true_residual(cg, sval, cg->xw, cg->bw, res);
+    FFHIP_CHECK(hipMemcpyAsync(x, cg->xw, 8 * n, on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+    FFHIP_CHECK(hipStreamSynchronize(s));
+    if (stats) {
+      stats->iters = hs.iters;
+      stats->converged = hs.done == 1 ? 1 : (hs.done == 2 ? 2 : 0);
+      stats->refinements = hs.refinements;
+      stats->gcg0 = hs.gcg0;
+      stats->gcg = cg->mode == FFHIP_CG_FP64 ? hs.gcg_prev[(hs.iters + 1) & 1] : res[2];
+      stats->true_gcg = res[2];
+      stats->true_relres = res[1] > 0 ? std::sqrt(res[0] / res[1]) : std::sqrt(res[0]);
+      stats->ms_solve = ms;
+    }
+    FFHIP_REQUIRE(hs.done != 4, FFHIP_ERR_NAN, "ffhip_cg_solve: NaN in CG (bad matrix)");
+    FFHIP_REQUIRE(hs.done != 3, FFHIP_ERR_NOCONV,
+                  "ffhip_cg_solve: ConjugueGradient did not converge in %d iterations", itmax);
+  });
+}
+
+int ffhip_cg_free(ffhip_cg cg) {
+  return guard([&] {
+    if (!cg) return;
+    (void)hipStreamSynchronize(cg->ctx->stream);
+    dfree(cg->sptr); dfree(cg->slen); dfree(cg->scol); dfree(cg->sval); dfree(cg->svalT);
+    dfree(cg->sval32); dfree(cg->sval32T); dfree(cg->dinv); dfree(cg->dsc); dfree(cg->wcl);
+    dfree(cg->xw); dfree(cg->bw); dfree(cg->G); dfree(cg->CG); dfree(cg->H[0]); dfree(cg->H[1]);
+    dfree(cg->AH); dfree(cg->r32); dfree(cg->q32); dfree(cg->y32); dfree(cg->p32[0]);
+    dfree(cg->p32[1]); dfree(cg->pA); dfree(cg->pB); dfree(cg->pC); dfree(cg->st); dfree(cg->ms);
+    delete cg;
+  });
+}
+
+int ffhip_bench_spmv(ffhip_cg cg, int mode, int reps, const void *x, void *y, double *avg_ms) {
+  return guard([&] {
+    FFHIP_REQUIRE(cg && x && y && avg_ms && reps > 0, FFHIP_ERR_ARG, "ffhip_bench_spmv: bad arguments");
+    FFHIP_REQUIRE(mode == FFHIP_CG_FP64 || cg->sval32, FFHIP_ERR_ARG,
+                  "ffhip_bench_spmv: FP32 matrix only exists for a MIXED solver");
+    ffhip_ctx ctx = cg->ctx;
+    hipStream_t s = ctx->stream;
+    auto launch = [&] {
+      if (mode == FFHIP_CG_FP64)
+        k_sell_spmv<double, double><<<cg->grid, kBlock, 0, s>>>(cg->n, cg->S, cg->sptr, cg->slen,
+                                                               cg->scol, cg->sval, (const double *)x,
+                                                               (double *)y);
+      else
+        k_sell_spmv<float, float><<<cg->grid, kBlock, 0, s>>>(cg->n, cg->S, cg->sptr, cg->slen,
+                                                             cg->scol, cg->sval32, (const float *)x,
+                                                             (float *)y);
+    };
+    launch();  // warm
+    FFHIP_CHECK(hipGetLastError());
+    ctx->tic();
+    for (int i = 0; i < reps; ++i) launch();
+    FFHIP_CHECK(hipGetLastError());
+    ctx->toc();
+    *avg_ms = ctx->last_ms / reps;
+  });
+}
+
+}  // extern "C"

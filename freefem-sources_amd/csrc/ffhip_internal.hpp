@@ -1,0 +1,156 @@
+// ffhip_internal.hpp — shared internals of libffhip (not part of the C-ABI).
+//
+// Device-memory objects behind the opaque handles of include/ffhip.h, error
+// plumbing, and small device helpers.  Everything here targets gfx950
+// (CDNA4, wave64).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/ffhip.h"
+
+namespace ffhip {
+
+// ---------------------------------------------------------------- errors
+void set_error(const char *fmt, ...);
+
+struct Error {
+  int code;
+};
+
+#define FFHIP_CHECK(expr)                                                      \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess) {                                                    \
+      ::ffhip::set_error("%s:%d: %s -> %s", __FILE__, __LINE__, #expr,         \
+                         hipGetErrorString(e_));                               \
+      throw ::ffhip::Error{FFHIP_ERR_HIP};                                     \
+    }                                                                          \
+  } while (0)
+
+#define FFHIP_REQUIRE(cond, code, ...)                                         \
+  do {                                                                         \
+    if (!(cond)) {                                                             \
+      ::ffhip::set_error(__VA_ARGS__);                                         \
+      throw ::ffhip::Error{code};                                              \
+    }                                                                          \
+  } while (0)
+
+// Runs f() and maps exceptions to C-ABI return codes.
+template <class F>
+int guard(F &&f) {
+  try {
+    f();
+    return FFHIP_OK;
+  } catch (const Error &e) {
+    return e.code;
+  } catch (...) {
+    set_error("unexpected C++ exception");
+    return FFHIP_ERR_INTERNAL;
+  }
+}
+
+// ---------------------------------------------------------------- memory
+template <class T>
+T *dalloc(size_t n) {
+  void *p = nullptr;
+  if (n == 0) n = 1;
+  FFHIP_CHECK(hipMalloc(&p, n * sizeof(T)));
+  return (T *)p;
+}
+template <class T>
+void dfree(T *&p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+}  // namespace ffhip
+
+// ---------------------------------------------------------------- handles
+struct ffhip_ctx_s {
+  int device = 0;
+  hipStream_t stream = nullptr;  // default priority (never a priority stream)
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int num_cu = 256;
+  // reusable scratch for library-internal temporaries (scans, partials)
+  void *scratch = nullptr;
+  size_t scratch_bytes = 0;
+  void *ensure_scratch(size_t bytes);
+  double last_ms = 0.;  // device time of the last timed call
+  void tic();           // record ev0 on the stream
+  void toc();           // record ev1, synchronise, set last_ms
+};
+
+struct ffhip_mesh_s {
+  ffhip_ctx ctx = nullptr;
+  int dim = 3;      // 3: tetrahedra, 2: triangles
+  int nloc = 4;     // vertices per element
+  int nbloc = 3;    // vertices per boundary element
+  int64_t nv = 0, nt = 0, nbe = 0;
+  double *xyz = nullptr;    // nv * dim, vertex-major (x,y,z per vertex)
+  int32_t *elems = nullptr; // nt * nloc
+  int32_t *belems = nullptr;// nbe * nbloc
+  int32_t *blab = nullptr;  // nbe
+  // dof -> incident (element, local vertex) list, element order (built lazily)
+  int64_t *v2e_ptr = nullptr;  // nv + 1
+  int32_t *v2e = nullptr;      // nt * nloc entries: e * nloc + l
+};
+
+struct ffhip_mat_s {
+  ffhip_ctx ctx = nullptr;
+  int64_t n = 0, nnz = 0;
+  int64_t *rowptr = nullptr;   // n + 1
+  int32_t *colidx = nullptr;   // nnz (sorted within each row)
+  double *vals = nullptr;      // nnz
+  int64_t *diag = nullptr;     // n: slot of (i,i), -1 if absent
+  // element-to-row map: for incidence entry k of row i (mesh v2e order), the
+  // offsets within row i of the columns of that element's vertices
+  uint32_t *slotmap = nullptr; // (nt*nloc) packed 4 x uint8 (P1, nloc <= 4)
+  int max_row = 0;
+  ffhip_mesh mesh = nullptr;   // mesh the pattern was built from (if any)
+};
+
+namespace ffhip {
+
+// ---------------------------------------------------------------- device helpers
+constexpr int kWave = 64;
+
+// wave64 sum of a double (DPP/permute based shuffles)
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+// Deterministic block reduction of one double; result valid in every thread.
+template <int BLOCK>
+__device__ inline double block_sum(double v, double *lds /* BLOCK/64 */) {
+  v = wave_sum(v);
+  const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  __syncthreads();
+  if (lane == 0) lds[w] = v;
+  __syncthreads();
+  double s = 0.;
+#pragma unroll
+  for (int i = 0; i < BLOCK / kWave; ++i) s += lds[i];
+  return s;
+}
+
+// XCD-aware logical block id: the hardware dispatches block b to XCD b % 8;
+// logical ids are renumbered so each XCD owns a contiguous block range (and
+// so a contiguous row range whose x-gathers stay in that XCD's L2).
+__device__ inline int xcd_block_id() {
+  const int G = gridDim.x, b = blockIdx.x;
+  if (G % 8) return b;
+  return (b % 8) * (G / 8) + (b / 8);
+}
+
+void exclusive_scan_i64(ffhip_ctx ctx, const int64_t *in, int64_t *out,
+                        int64_t n);
+int64_t read_i64(ffhip_ctx ctx, const int64_t *p);
+
+}  // namespace ffhip

@@ -1,0 +1,357 @@
+"""Python host mirror of the ffhip FreeFEM plugin, over libffhip's C-ABI.
+
+The production entry point is the FreeFEM plugin (plugin/ffhip.cpp,
+`load "ffhip"`); this module mirrors the same operations for tests and
+benchmarks:
+
+    cube(n)                     -> mesh3 cube(n,n,n)        (msh3.cpp:7879)
+    Matrix.p1(mesh)             -> HashMatrix pattern         (HashMatrix.cpp:858)
+    Matrix.assemble_p1(...)     -> matrix A = va(Vh,Vh)       (problem.cpp:1116)
+    Matrix.set_bc(labels)       -> + on(labels, u=0)          (problem.cpp:10039)
+    rhs_p1(mesh, f, labels)     -> b = vb(0,Vh)               (problem.cpp:10878)
+    CG(A).solve(b, x, eps)      -> u[] = A^-1*b, solver=CG    (VirtualSolverCG.hpp:169)
+
+Errors raise FFHipError with the library's message (the plugin maps the same
+codes to ExecError).  There is no CPU fallback: if libffhip.so is missing or
+no GPU is present, every call raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(os.path.dirname(HERE), "libffhip.so")
+
+COEF_CONST, COEF_SIN3, COEF_P0 = 0, 1, 2
+CG_FP64, CG_MIXED = 0, 1
+ERRORS = {-1: "bad argument", -2: "HIP error", -3: "CG did not converge", -4: "NaN in CG",
+          -5: "unsupported", -6: "internal error"}
+
+
+class FFHipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"ffhip error {code} ({ERRORS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class CGStats(ctypes.Structure):
+    _fields_ = [("iters", ctypes.c_int), ("converged", ctypes.c_int),
+                ("refinements", ctypes.c_int), ("reserved", ctypes.c_int),
+                ("gcg0", ctypes.c_double), ("gcg", ctypes.c_double),
+                ("true_gcg", ctypes.c_double), ("true_relres", ctypes.c_double),
+                ("ms_solve", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+_vp = ctypes.c_void_p
+_i = ctypes.c_int
+_l = ctypes.c_int64
+_d = ctypes.c_double
+_pvp = ctypes.POINTER(ctypes.c_void_p)
+_pl = ctypes.POINTER(ctypes.c_int64)
+
+_SIGS = {
+    "ffhip_last_error": (ctypes.c_char_p, []),
+    "ffhip_version": (ctypes.c_char_p, []),
+    "ffhip_init": (_i, [_i, _pvp]),
+    "ffhip_ctx_free": (_i, [_vp]),
+    "ffhip_sync": (_i, [_vp]),
+    "ffhip_mesh3_cube": (_i, [_vp, _i, _i, _i, _vp, _pvp]),
+    "ffhip_mesh2_square": (_i, [_vp, _i, _i, _vp, _pvp]),
+    "ffhip_mesh3_upload": (_i, [_vp, _l, _l, _l, _vp, _vp, _vp, _vp, _pvp]),
+    "ffhip_mesh_info": (_i, [_vp, _pl, _pl, _pl]),
+    "ffhip_mesh_download": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "ffhip_mesh_free": (_i, [_vp]),
+    "ffhip_pattern_p1": (_i, [_vp, _vp, _pvp]),
+    "ffhip_mat_info": (_i, [_vp, _pl, _pl]),
+    "ffhip_assemble_p1": (_i, [_vp, _vp, _vp, _i, _d, _vp, _i]),
+    "ffhip_set_bc": (_i, [_vp, _vp, _vp, _vp, _i, _d, _vp]),
+    "ffhip_rhs_p1": (_i, [_vp, _vp, _d, _vp, _i, _d, _d, _vp, _i]),
+    "ffhip_mat_download": (_i, [_vp, _vp, _vp, _vp]),
+    "ffhip_mat_upload": (_i, [_vp, _l, _l, _vp, _vp, _vp, _pvp]),
+    "ffhip_mat_device_ptrs": (_i, [_vp, _pvp, _pvp, _pvp]),
+    "ffhip_mat_free": (_i, [_vp]),
+    "ffhip_spmv": (_i, [_vp, _vp, _vp, _vp, _i]),
+    "ffhip_cg_create": (_i, [_vp, _vp, _i, _pvp]),
+    "ffhip_cg_update_values": (_i, [_vp]),
+    "ffhip_cg_solve": (_i, [_vp, _vp, _vp, _i, _d, _i, _d, _i, ctypes.POINTER(CGStats)]),
+    "ffhip_cg_free": (_i, [_vp]),
+    "ffhip_bench_spmv": (_i, [_vp, _i, _i, _vp, _vp, ctypes.POINTER(_d)]),
+    "ffhip_last_ms": (_d, [_vp]),
+}
+
+_LIB = None
+
+
+def lib():
+    """Load libffhip.so (fails loudly when it was not built).
+
+    torch, when importable, is imported first: its wheel ships its own HIP
+    runtime (SONAME libamdhip64.so.7), and libffhip must bind to that same
+    runtime rather than load a second one from /opt/rocm, so that device
+    pointers and the device itself are shared within the process."""
+    global _LIB
+    if _LIB is None:
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
+        if not os.path.exists(LIBPATH):
+            raise FFHipError(-6, f"{LIBPATH} not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIBPATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _LIB = L
+    return _LIB
+
+
+def check(rc):
+    if rc != 0:
+        raise FFHipError(rc, lib().ffhip_last_error().decode())
+
+
+def _hp(a):
+    """host pointer of a C-contiguous numpy array (or None)"""
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _dp(t):
+    """device pointer of a torch tensor (or an int address)"""
+    if t is None:
+        return None
+    if isinstance(t, int):
+        return ctypes.c_void_p(t)
+    assert t.is_contiguous() and t.is_cuda
+    # the library stream is ordered with the legacy default stream; a caller
+    # on another torch stream is drained first
+    import torch
+    torch.cuda.current_stream().synchronize()
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Context:
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        check(lib().ffhip_init(device, ctypes.byref(h)))
+        self.h = h
+
+    def sync(self):
+        check(lib().ffhip_sync(self.h))
+
+    def last_ms(self):
+        return lib().ffhip_last_ms(self.h)
+
+    def close(self):
+        if self.h:
+            lib().ffhip_ctx_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_CTX = None
+
+
+def context():
+    global _CTX
+    if _CTX is None:
+        _CTX = Context(0)
+    return _CTX
+
+
+class Mesh:
+    def __init__(self, h, ctx, dim):
+        self.h, self.ctx, self.dim = h, ctx, dim
+        nv, nt, nbe = _l(), _l(), _l()
+        check(lib().ffhip_mesh_info(h, ctypes.byref(nv), ctypes.byref(nt), ctypes.byref(nbe)))
+        self.nv, self.nt, self.nbe = nv.value, nt.value, nbe.value
+
+    def download(self):
+        d = self.dim
+        nloc = d + 1
+        xyz = np.zeros(d * self.nv)
+        el = np.zeros(nloc * self.nt, np.int32)
+        be = np.zeros(d * self.nbe, np.int32)
+        bl = np.zeros(self.nbe, np.int32)
+        check(lib().ffhip_mesh_download(self.h, _hp(xyz), _hp(el), _hp(be), _hp(bl)))
+        return dict(nv=self.nv, nt=self.nt, nbe=self.nbe, xyz=xyz, elems=el, belems=be, blab=bl)
+
+    def free(self):
+        if self.h:
+            lib().ffhip_mesh_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def cube(nx, ny=None, nz=None, labels=(1, 2, 3, 4, 5, 6), ctx=None):
+    ctx = ctx or context()
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    lab = np.asarray(labels, np.int32)
+    h = ctypes.c_void_p()
+    check(lib().ffhip_mesh3_cube(ctx.h, nx, ny, nz, _hp(lab), ctypes.byref(h)))
+    return Mesh(h, ctx, 3)
+
+
+def square(nx, ny=None, labels=(1, 2, 3, 4), ctx=None):
+    ctx = ctx or context()
+    ny = nx if ny is None else ny
+    lab = np.asarray(labels, np.int32)
+    h = ctypes.c_void_p()
+    check(lib().ffhip_mesh2_square(ctx.h, nx, ny, _hp(lab), ctypes.byref(h)))
+    return Mesh(h, ctx, 2)
+
+
+def mesh3_upload(xyz, tets, btri, blab, ctx=None):
+    ctx = ctx or context()
+    xyz = np.ascontiguousarray(xyz, np.float64)
+    tets = np.ascontiguousarray(tets, np.int32)
+    btri = np.ascontiguousarray(btri, np.int32)
+    blab = np.ascontiguousarray(blab, np.int32)
+    h = ctypes.c_void_p()
+    check(lib().ffhip_mesh3_upload(ctx.h, xyz.size // 3, tets.size // 4, blab.size, _hp(xyz),
+                                   _hp(tets), _hp(btri), _hp(blab), ctypes.byref(h)))
+    return Mesh(h, ctx, 3)
+
+
+class Matrix:
+    def __init__(self, h, ctx, mesh=None):
+        self.h, self.ctx, self.mesh = h, ctx, mesh
+        n, nnz = _l(), _l()
+        check(lib().ffhip_mat_info(h, ctypes.byref(n), ctypes.byref(nnz)))
+        self.n, self.nnz = n.value, nnz.value
+
+    @staticmethod
+    def p1(mesh):
+        h = ctypes.c_void_p()
+        check(lib().ffhip_pattern_p1(mesh.ctx.h, mesh.h, ctypes.byref(h)))
+        return Matrix(h, mesh.ctx, mesh)
+
+    @staticmethod
+    def upload(rowptr, colidx, vals, ctx=None):
+        ctx = ctx or context()
+        rowptr = np.ascontiguousarray(rowptr, np.int64)
+        colidx = np.ascontiguousarray(colidx, np.int32)
+        vals = np.ascontiguousarray(vals, np.float64)
+        h = ctypes.c_void_p()
+        check(lib().ffhip_mat_upload(ctx.h, rowptr.size - 1, colidx.size, _hp(rowptr), _hp(colidx),
+                                     _hp(vals), ctypes.byref(h)))
+        return Matrix(h, ctx)
+
+    def assemble_p1(self, kind=COEF_SIN3, c0=1.0, kappa_e=None):
+        """kappa_e: numpy (host) or torch cuda tensor (device) of nt doubles"""
+        on_dev = 0
+        ptr = None
+        if kappa_e is not None:
+            if isinstance(kappa_e, np.ndarray):
+                kappa_e = np.ascontiguousarray(kappa_e, np.float64)
+                ptr = _hp(kappa_e)
+            else:
+                ptr, on_dev = _dp(kappa_e), 1
+        check(lib().ffhip_assemble_p1(self.ctx.h, self.mesh.h, self.h, kind, c0, ptr, on_dev))
+        return self.ctx.last_ms()
+
+    def set_bc(self, labels, tgv=1e30, want_onbc=False):
+        lab = np.asarray(labels, np.int32)
+        onbc = np.zeros(self.n, np.uint8) if want_onbc else None
+        check(lib().ffhip_set_bc(self.ctx.h, self.mesh.h, self.h, _hp(lab), lab.size, tgv, _hp(onbc)))
+        return onbc if want_onbc else self.ctx.last_ms()
+
+    def download(self):
+        rp = np.zeros(self.n + 1, np.int64)
+        ci = np.zeros(self.nnz, np.int32)
+        va = np.zeros(self.nnz)
+        check(lib().ffhip_mat_download(self.h, _hp(rp), _hp(ci), _hp(va)))
+        return rp, ci, va
+
+    def device_ptrs(self):
+        a, b, c = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        check(lib().ffhip_mat_device_ptrs(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return a.value, b.value, c.value
+
+    def spmv(self, x, y, trans=False):
+        """device vectors (torch cuda tensors)"""
+        check(lib().ffhip_spmv(self.ctx.h, self.h, _dp(x), _dp(y), int(trans)))
+
+    def free(self):
+        if self.h:
+            lib().ffhip_mat_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def rhs_p1(mesh, f=1.0, labels=(), tgv=1e30, g=0.0, out=None):
+    """b = vb(0,Vh) for int(f*v) + on(labels, u=g).  out: torch cuda tensor
+    (device) or None (returns a numpy array)."""
+    lab = np.asarray(labels, np.int32)
+    if out is None:
+        b = np.zeros(mesh.nv)
+        check(lib().ffhip_rhs_p1(mesh.ctx.h, mesh.h, f, _hp(lab), lab.size, tgv, g, _hp(b), 0))
+        return b
+    check(lib().ffhip_rhs_p1(mesh.ctx.h, mesh.h, f, _hp(lab), lab.size, tgv, g, _dp(out), 1))
+    return out
+
+
+class CG:
+    def __init__(self, A, mode=CG_FP64):
+        h = ctypes.c_void_p()
+        check(lib().ffhip_cg_create(A.ctx.h, A.h, mode, ctypes.byref(h)))
+        self.h, self.A, self.mode = h, A, mode
+
+    def update_values(self):
+        check(lib().ffhip_cg_update_values(self.h))
+
+    def solve(self, b, x=None, eps=1e-10, itmax=0, tgv=1e30, trans=False, raise_on_error=True):
+        """numpy b/x (host) or torch cuda tensors (device).  Returns (x, stats)."""
+        st = CGStats()
+        if isinstance(b, np.ndarray):
+            b = np.ascontiguousarray(b, np.float64)
+            x = np.zeros(b.size) if x is None else np.array(x, np.float64)
+            rc = lib().ffhip_cg_solve(self.h, _hp(b), _hp(x), 0, eps, itmax, tgv, int(trans),
+                                      ctypes.byref(st))
+        else:
+            import torch
+            x = torch.zeros_like(b) if x is None else x
+            rc = lib().ffhip_cg_solve(self.h, _dp(b), _dp(x), 1, eps, itmax, tgv, int(trans),
+                                      ctypes.byref(st))
+        if raise_on_error:
+            check(rc)
+        return x, st.as_dict()
+
+    def bench_spmv(self, mode, reps, x, y):
+        ms = _d()
+        check(lib().ffhip_bench_spmv(self.h, mode, reps, _dp(x), _dp(y), ctypes.byref(ms)))
+        return ms.value
+
+    def free(self):
+        if self.h:
+            lib().ffhip_cg_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

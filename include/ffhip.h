@@ -1,0 +1,165 @@
+/*
+ * ffhip.h — C-ABI of libffhip: MI355X-native replacement for FreeFEM's
+ * bilinear-form assembly (AssembleBilinearForm / Element_Op -> HashMatrix),
+ * the CSR build (HashMatrix::CSR) and the Jacobi-preconditioned CG solve
+ * (SolverCG / ConjugueGradient).
+ *
+ * Plain C types only (no torch, no C++ classes).  Every entry point returns 0
+ * on success or a negative FFHIP_ERR_* code; ffhip_last_error() gives the
+ * message.  All calls are synchronous from the caller's point of view (the
+ * library's stream is synchronised before returning), matching the
+ * single-threaded, synchronous solver contract of FreeFEM's plugins
+ * (src/femlib/VirtualSolver.hpp:190-243).
+ *
+ * Citations are to the FreeFEM source tree (src/...).
+ */
+#ifndef FFHIP_H
+#define FFHIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  FFHIP_OK = 0,
+  FFHIP_ERR_ARG = -1,        /* bad argument / size mismatch (ExecError)   */
+  FFHIP_ERR_HIP = -2,        /* HIP runtime failure                        */
+  FFHIP_ERR_NOCONV = -3,     /* CG did not converge (VirtualSolverCG.hpp:188) */
+  FFHIP_ERR_NAN = -4,        /* NaN in CG (CG.cpp:228 assert)             */
+  FFHIP_ERR_UNSUPPORTED = -5,/* mesh / form outside the supported cases    */
+  FFHIP_ERR_INTERNAL = -6
+};
+
+typedef struct ffhip_ctx_s *ffhip_ctx;   /* device, stream, scratch        */
+typedef struct ffhip_mesh_s *ffhip_mesh; /* device-resident mesh            */
+typedef struct ffhip_mat_s *ffhip_mat;   /* device-resident CSR matrix      */
+typedef struct ffhip_cg_s *ffhip_cg;     /* device-resident CG solver       */
+
+const char *ffhip_last_error(void);
+const char *ffhip_version(void);
+
+/* Library init on one GPU (plugin Load_Init, load.cpp:93-105 / InitFunct.hpp:76-98). */
+int ffhip_init(int device, ffhip_ctx *ctx);
+int ffhip_ctx_free(ffhip_ctx ctx);
+int ffhip_sync(ffhip_ctx ctx);
+
+/* ---- meshes ------------------------------------------------------------ */
+/* cube(nx,ny,nz) built on the device, identical to BuildCube kind=6
+ * (src/fflib/msh3.cpp:7879-8132; labels as Cube_Op 8134-8164: labels[6] in
+ * the order of the script's label=[...] argument, default {1,...,6}). */
+int ffhip_mesh3_cube(ffhip_ctx ctx, int nx, int ny, int nz,
+                     const int32_t labels[6], ffhip_mesh *mesh);
+/* square(nx,ny) built on the device, identical to Carre_ flags=0
+ * (src/fflib/lgmesh.cpp:1229-1384; labels bottom,right,top,left). */
+int ffhip_mesh2_square(ffhip_ctx ctx, int nx, int ny, const int32_t labels[4],
+                       ffhip_mesh *mesh);
+/* Upload a Mesh3 (src/femlib/Mesh3dn.hpp:143): vertices xyz[3*nv], tets[4*nt]
+ * (0-based vertex ids, Tet vertex order), boundary triangles btri[3*nbe] with
+ * labels blab[nbe].  Host pointers. */
+int ffhip_mesh3_upload(ffhip_ctx ctx, int64_t nv, int64_t nt, int64_t nbe,
+                       const double *xyz, const int32_t *tets,
+                       const int32_t *btri, const int32_t *blab,
+                       ffhip_mesh *mesh);
+int ffhip_mesh_info(ffhip_mesh mesh, int64_t *nv, int64_t *nt, int64_t *nbe);
+/* Copy the mesh back to host arrays (any pointer may be NULL). */
+int ffhip_mesh_download(ffhip_mesh mesh, double *xyz, int32_t *elems,
+                        int32_t *belems, int32_t *blab);
+int ffhip_mesh_free(ffhip_mesh mesh);
+
+/* ---- P1 matrix: pattern, values, Dirichlet ----------------------------- */
+/* Integer CSR pattern of the P1 stiffness matrix, bit-exact with the
+ * HashMatrix that `A += mate` builds over the element loop
+ * (src/femlib/HashMatrix.cpp:1294-1332) once sorted by HashMatrix::CSR
+ * (HashMatrix.cpp:858-867, Sortij 670-682, Buildp 992-1030).  Also builds the
+ * element-to-row map used by the atomic-free assembly. */
+int ffhip_pattern_p1(ffhip_ctx ctx, ffhip_mesh mesh, ffhip_mat *mat);
+int ffhip_mat_info(ffhip_mat mat, int64_t *n, int64_t *nnz);
+
+enum { FFHIP_COEF_CONST = 0, FFHIP_COEF_SIN3 = 1, FFHIP_COEF_P0 = 2 };
+/* Values of int3d(Th)(kappa*(dx(u)*dx(v)+dy(u)*dy(v)+dz(u)*dz(v))) with the
+ * default Tet_5 rule (src/fflib/problem.cpp:14102-14124), replacing
+ * AssembleBilinearForm 3D (problem.cpp:1116-1417) + Element_Op
+ * (problem.cpp:6337-6437).  kappa: CONST -> c0; SIN3 ->
+ * 1+0.5*sin(2 pi x)sin(2 pi y)sin(2 pi z); P0 -> kappa_e[element] (nt values,
+ * host pointer, or device pointer when kappa_on_device != 0).
+ * 2D meshes: int2d with T_5, terms dx,dy. */
+int ffhip_assemble_p1(ffhip_ctx ctx, ffhip_mesh mesh, ffhip_mat mat, int kind,
+                      double c0, const double *kappa_e, int kappa_on_device);
+/* on(labels..., u=g) with tgv > 0: AssembleBC (problem.cpp:10039-10194) then
+ * HashMatrix::SetBC (HashMatrix.cpp:1194-1238): a_ii = tgv on the dofs of
+ * boundary elements whose label is listed.  onbc (optional, host, n bytes)
+ * receives the marked dofs. */
+int ffhip_set_bc(ffhip_ctx ctx, ffhip_mesh mesh, ffhip_mat mat,
+                 const int32_t *labels, int nlab, double tgv, uint8_t *onbc);
+/* Right-hand side of int3d(Th)(f*v) (+ on(...)=g rows set to tgv*g):
+ * AssembleLinearForm/Element_rhs 3D (problem.cpp:10878, 7917-7985) and the B
+ * part of AssembleBC (problem.cpp:10176).  b is a device pointer (n doubles)
+ * when on_device != 0, else a host pointer. */
+int ffhip_rhs_p1(ffhip_ctx ctx, ffhip_mesh mesh, double f,
+                 const int32_t *labels, int nlab, double tgv, double g,
+                 double *b, int on_device);
+/* CSR out (host): rowptr[n+1] (int64), colidx[nnz], vals[nnz]; any NULL skipped. */
+int ffhip_mat_download(ffhip_mat mat, int64_t *rowptr, int32_t *colidx,
+                       double *vals);
+/* CSR in (host) — the solver path of a FreeFEM HashMatrix: A->CSR(ia,ja,a)
+ * in SolverHIPCG::fac_symbolic/fac_numeric (HashMatrix.cpp:858-867). */
+int ffhip_mat_upload(ffhip_ctx ctx, int64_t n, int64_t nnz,
+                     const int64_t *rowptr, const int32_t *colidx,
+                     const double *vals, ffhip_mat *mat);
+/* Device pointers of the CSR arrays (owned by mat). */
+int ffhip_mat_device_ptrs(ffhip_mat mat, int64_t **rowptr, int32_t **colidx,
+                          double **vals);
+int ffhip_mat_free(ffhip_mat mat);
+
+/* y = A x (trans = 0) or A^T x (trans = 1) on device vectors:
+ * HashMatrix::addMatMul (HashMatrix.cpp:1086-1154) without the accumulate. */
+int ffhip_spmv(ffhip_ctx ctx, ffhip_mat mat, const double *x, double *y,
+               int trans);
+
+/* ---- Jacobi-preconditioned CG ------------------------------------------ */
+enum { FFHIP_CG_FP64 = 0, FFHIP_CG_MIXED = 1 };
+typedef struct {
+  int iters;          /* CG iterations (inner FP32 iterations for MIXED)   */
+  int converged;      /* ConjugueGradient's nret: 1 converged, 2 gCg0<1e-30, 0 no */
+  int refinements;    /* MIXED: FP64 residual recomputations              */
+  int reserved;
+  double gcg0;        /* (g0, D^-1 g0), g0 = A x0 - b                       */
+  double gcg;         /* final (g, D^-1 g) (recurrence for FP64, true for MIXED) */
+  double true_gcg;    /* final (g, D^-1 g) recomputed in FP64 from x         */
+  double true_relres; /* ||b - A x||_2 / ||b||_2 in FP64                     */
+  double ms_solve;    /* device time of the solve (HIP events)              */
+} ffhip_cg_stats;
+/* Solver setup for a matrix (SolverCG ctor + HMatVirtPrecon,
+ * src/femlib/VirtualSolverCG.hpp:27-69, 157-167): tgv detection
+ * (HashMatrix::gettgv HashMatrix.cpp:1340-1371), Jacobi diagonal, and the
+ * device layouts the kernels stream (sliced-ELL FP64; for MIXED also the
+ * FP32 Jacobi-scaled copy). */
+int ffhip_cg_create(ffhip_ctx ctx, ffhip_mat mat, int mode, ffhip_cg *cg);
+/* Re-read the values of mat (same pattern) — SolverHIPCG::fac_numeric. */
+int ffhip_cg_update_values(ffhip_cg cg);
+/* Solve A x = b (SolverCG::dosolver, VirtualSolverCG.hpp:169-190, one rhs):
+ * x holds the initial guess on entry (warm start, RNM.hpp:568-569); tgv rows
+ * are initialised to b_i/tgv (SetInitWithBC); stop when
+ * (g, D^-1 g) < eps^2 (g0, D^-1 g0) for eps > 0, or < eps^2 for eps < 0
+ * (CG.cpp:226,247).  itmax <= 0 means n.  b, x: device pointers when
+ * on_device != 0, else host.  trans != 0 solves A^T x = b (SolveT). */
+int ffhip_cg_solve(ffhip_cg cg, const double *b, double *x, int on_device,
+                   double eps, int itmax, double tgv, int trans,
+                   ffhip_cg_stats *stats);
+int ffhip_cg_free(ffhip_cg cg);
+
+/* ---- measurement helpers ---------------------------------------------- */
+/* Average device time (ms) of `reps` back-to-back launches of the solver's
+ * SpMV kernel (mode FP64 or MIXED's FP32), timed with HIP events on the
+ * library stream; x, y device vectors of n doubles (FP64) / floats (FP32). */
+int ffhip_bench_spmv(ffhip_cg cg, int mode, int reps, const void *x, void *y,
+                     double *avg_ms);
+/* Device time (ms) of the last pattern / assembly / bc / rhs call. */
+double ffhip_last_ms(ffhip_ctx ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

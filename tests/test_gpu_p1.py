@@ -1,0 +1,245 @@
+"""GPU parity of the P1 path through libffhip's C-ABI against the oracle.
+
+Bars: integer mesh / pattern data bit-exact; matrix values within
+max|A_gpu - A_ref| <= 1e-12 * max|A_ref| (off the tgv diagonal, which must be
+exactly tgv); right-hand side within 1e-13 relative; CG solutions within 1e-8
+relative of the oracle's FP64 solution at eps = 1e-10, iteration counts within
+2 of the oracle's (summation order differs), and the FP64 criterion
+(g, D^-1 g) < eps^2 (g0, D^-1 g0) met by both GPU modes.
+"""
+import numpy as np
+import pytest
+
+import ffo
+
+pytestmark = pytest.mark.gpu
+
+ffhip = pytest.importorskip("ffhip")
+LABELS = (1, 2, 3, 4, 5, 6)
+
+
+def _vals_close(va, vo, tol=1e-12):
+    big = vo >= 1e29
+    assert np.array_equal(va[big], vo[big])
+    scale = np.max(np.abs(vo[~big]))
+    return np.max(np.abs(va[~big] - vo[~big])) <= tol * scale
+
+
+@pytest.mark.parametrize("dims", [(1, 1, 1), (2, 2, 2), (5, 5, 5), (3, 4, 5), (8, 8, 8)])
+def test_cube_mesh_bitexact(dims):
+    nx, ny, nz = dims
+    m = ffhip.cube(nx, ny, nz, labels=(11, 12, 13, 14, 15, 16))
+    d = m.download()
+    o = ffo.cube(nx, ny, nz, labels=(11, 12, 13, 14, 15, 16))
+    assert (d["nv"], d["nt"], d["nbe"]) == (o["nv"], o["nt"], o["nbe"])
+    assert np.array_equal(d["xyz"], o["xyz"])
+    assert np.array_equal(d["elems"], o["tets"])
+    assert np.array_equal(d["belems"], o["btri"]) and np.array_equal(d["blab"], o["blab"])
+
+
+@pytest.mark.parametrize("n", [7, 40])
+def test_square_mesh_bitexact(n):
+    m = ffhip.square(n, n + 3)
+    d = m.download()
+    o = ffo.square(n, n + 3)
+    assert np.array_equal(d["xyz"], o["xy"]) and np.array_equal(d["elems"], o["tris"])
+    assert np.array_equal(d["belems"], o["bedges"]) and np.array_equal(d["blab"], o["blab"])
+
+
+@pytest.mark.parametrize("n", [1, 2, 6, 13])
+def test_pattern_bitexact(n):
+    m = ffhip.cube(n)
+    A = ffhip.Matrix.p1(m)
+    rp, ci, _ = A.download()
+    o = ffo.cube(n)
+    rpo, cio = ffo.pattern(o["nv"], o["tets"], 4)
+    assert np.array_equal(rp, rpo) and np.array_equal(ci, cio)
+
+
+@pytest.mark.parametrize("name", ["p1_cube4_sin3.npz", "p1_cube6_const.npz", "p1_cube5_p0.npz"])
+def test_matrix_and_rhs_vs_golden(name):
+    g = np.load(__file__.rsplit("/", 1)[0] + "/golden/" + name)
+    n, kind = int(g["n"]), int(g["kind"])
+    m = ffhip.cube(n)
+    A = ffhip.Matrix.p1(m)
+    A.assemble_p1(kind, 1.0, g["kappa_e"] if "kappa_e" in g.files else None)
+    onbc = A.set_bc(LABELS, want_onbc=True)
+    rp, ci, va = A.download()
+    assert np.array_equal(rp, g["rowptr"]) and np.array_equal(ci, g["colidx"])
+    assert np.array_equal(onbc, g["onbc"])
+    assert _vals_close(va, g["vals"])
+    b = ffhip.rhs_p1(m, 1.0, LABELS)
+    assert np.max(np.abs(b - g["b"])) <= 1e-13 * np.max(np.abs(g["b"]))
+    x, st = ffhip.CG(A).solve(b, eps=1e-10)
+    assert abs(st["iters"] - int(g["iters"])) <= 2
+    assert np.linalg.norm(x - g["x"]) <= 1e-8 * np.linalg.norm(g["x"])
+
+
+@pytest.mark.parametrize("n,kind", [(10, ffo.COEF_SIN3), (9, ffo.COEF_CONST), (7, ffo.COEF_P0)])
+def test_assembly_vs_oracle(n, kind):
+    mo = ffo.cube(n)
+    ke = np.exp(0.5 * np.random.default_rng(n).standard_normal(mo["nt"])) if kind == ffo.COEF_P0 else None
+    m, rpo, cio, vo, onbco, bo = ffo.poisson_p1_3d(n, kind, 1.0, ke)
+    M = ffhip.cube(n)
+    A = ffhip.Matrix.p1(M)
+    A.assemble_p1(kind, 1.0, ke)
+    onbc = A.set_bc(LABELS, want_onbc=True)
+    rp, ci, va = A.download()
+    assert np.array_equal(rp, rpo) and np.array_equal(ci, cio) and np.array_equal(onbc, onbco)
+    assert _vals_close(va, vo)
+    b = ffhip.rhs_p1(M, 1.0, LABELS)
+    assert np.max(np.abs(b - bo)) <= 1e-13 * np.max(np.abs(bo))
+
+
+def test_kappa_on_device_matches_host():
+    import torch
+    n = 6
+    M = ffhip.cube(n)
+    ke = np.exp(0.5 * np.random.default_rng(1).standard_normal(M.nt))
+    A1 = ffhip.Matrix.p1(M)
+    A1.assemble_p1(ffhip.COEF_P0, 1.0, ke)
+    A2 = ffhip.Matrix.p1(M)
+    A2.assemble_p1(ffhip.COEF_P0, 1.0, torch.tensor(ke, device="cuda"))
+    assert np.array_equal(A1.download()[2], A2.download()[2])
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("n", [6, 20])
+def test_cg_vs_oracle(mode, n):
+    m, rpo, cio, vo, onbco, bo = ffo.poisson_p1_3d(n)
+    xo, so = ffo.pcg(rpo, cio, vo, bo, eps=1e-10)
+    M = ffhip.cube(n)
+    A = ffhip.Matrix.p1(M)
+    A.assemble_p1(ffhip.COEF_SIN3)
+    A.set_bc(LABELS)
+    b = ffhip.rhs_p1(M, 1.0, LABELS)
+    cg = ffhip.CG(A, mode)
+    x, st = cg.solve(b, eps=1e-10)
+    assert st["converged"] == 1
+    assert st["true_gcg"] < 1e-20 * st["gcg0"] * (1.0 + 1e-6)
+    assert abs(st["gcg0"] - so["gcg0"]) <= 1e-12 * so["gcg0"]
+    if mode == 0:
+        assert abs(st["iters"] - so["iters"]) <= 2
+    else:
+        assert st["iters"] <= 1.25 * so["iters"] + 5
+    assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
+
+
+def test_cg_deterministic_and_warm_start():
+    n = 12
+    M = ffhip.cube(n)
+    A = ffhip.Matrix.p1(M)
+    A.assemble_p1(ffhip.COEF_SIN3)
+    A.set_bc(LABELS)
+    b = ffhip.rhs_p1(M, 1.0, LABELS)
+    cg = ffhip.CG(A)
+    x1, s1 = cg.solve(b, eps=1e-10)
+    x2, s2 = cg.solve(b, eps=1e-10)
+    assert np.array_equal(x1, x2) and s1["iters"] == s2["iters"]
+    # warm start (x is not reset, RNM.hpp:568-569) with an absolute criterion
+    # (eps < 0, CG.cpp:226): from the solution the first iteration already
+    # satisfies it, as in the oracle
+    x3, s3 = cg.solve(b, x=x1, eps=-1e-9)
+    rp, ci, va = A.download()
+    xo, so = ffo.pcg(rp, ci, va, b, x0=x1, eps=-1e-9)
+    assert s3["iters"] == so["iters"] == 1
+
+
+def test_cg_nonconvergence_raises():
+    n = 8
+    M = ffhip.cube(n)
+    A = ffhip.Matrix.p1(M)
+    A.assemble_p1(ffhip.COEF_SIN3)
+    A.set_bc(LABELS)
+    b = ffhip.rhs_p1(M, 1.0, LABELS)
+    with pytest.raises(ffhip.FFHipError) as e:
+        ffhip.CG(A).solve(b, eps=1e-10, itmax=3)
+    assert e.value.code == -3
+
+
+def test_upload_path_and_spmv():
+    import torch
+    m, rpo, cio, vo, onbco, bo = ffo.poisson_p1_3d(7)
+    A = ffhip.Matrix.upload(rpo, cio, vo)
+    x = np.random.default_rng(0).standard_normal(rpo.size - 1)
+    xd = torch.tensor(x, device="cuda")
+    yd = torch.empty_like(xd)
+    A.spmv(xd, yd)
+    y = ffo.spmv(rpo, cio, vo, x)
+    assert np.allclose(yd.cpu().numpy(), y, rtol=1e-13, atol=1e-13 * np.abs(y).max())
+    A.spmv(xd, yd, trans=True)
+    yt = ffo.spmv(rpo, cio, vo, x, trans=True)
+    assert np.allclose(yd.cpu().numpy(), yt, rtol=1e-13, atol=1e-13 * np.abs(yt).max())
+    x2, st = ffhip.CG(A).solve(bo, eps=1e-10)
+    xo, so = ffo.pcg(rpo, cio, vo, bo, eps=1e-10)
+    assert np.linalg.norm(x2 - xo) <= 1e-8 * np.linalg.norm(xo)
+
+
+def test_transposed_solve():
+    mo = ffo.cube(6)
+    ke = np.exp(0.5 * np.random.default_rng(3).standard_normal(mo["nt"]))
+    m, rpo, cio, vo, onbco, bo = ffo.poisson_p1_3d(6, ffo.COEF_P0, 1.0, ke)
+    A = ffhip.Matrix.upload(rpo, cio, vo)
+    x, st = ffhip.CG(A).solve(bo, eps=1e-10, trans=True)
+    xo, so = ffo.pcg(rpo, cio, vo, bo, eps=1e-10, trans=True)
+    assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
+
+
+def test_2d_square_poisson_vs_oracle():
+    n = 30
+    s = ffo.square(n)
+    rpo, cio = ffo.pattern(s["nv"], s["tris"], 3)
+    vo = ffo.assemble_p1_2d(s, rpo, cio)
+    onbc = ffo.mark_bc(s["nv"], s["bedges"], s["blab"], 2, (1, 2, 3, 4))
+    ffo.set_bc(rpo, cio, vo, onbc)
+    bo = ffo.rhs_p1_2d(s, 1.0)
+    ffo.rhs_bc(bo, onbc)
+    M = ffhip.square(n)
+    A = ffhip.Matrix.p1(M)
+    A.assemble_p1(ffhip.COEF_CONST, 1.0)
+    on2 = A.set_bc((1, 2, 3, 4), want_onbc=True)
+    rp, ci, va = A.download()
+    assert np.array_equal(rp, rpo) and np.array_equal(ci, cio) and np.array_equal(on2, onbc)
+    assert _vals_close(va, vo)
+    b = ffhip.rhs_p1(M, 1.0, (1, 2, 3, 4))
+    assert np.max(np.abs(b - bo)) <= 1e-13 * np.max(np.abs(bo))
+    xo, so = ffo.pcg(rpo, cio, vo, bo, eps=1e-10)
+    x, st = ffhip.CG(A).solve(b, eps=1e-10)
+    assert abs(st["iters"] - so["iters"]) <= 2
+    assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
+
+
+def _cube_nnz(n):
+    E = 3 * n * (n + 1) ** 2 + 3 * n * n * (n + 1) + n ** 3
+    return (n + 1) ** 3 + 2 * E
+
+
+@pytest.mark.parametrize("n", [64])
+def test_full_size_properties(n):
+    """cube(64), the 3D Poisson configuration: nnz closed form, zero row sums
+    before Dirichlet (constants are in the kernel), symmetry, both CG modes
+    reach the FP64 criterion."""
+    import torch
+    M = ffhip.cube(n)
+    A = ffhip.Matrix.p1(M)
+    assert A.nnz == _cube_nnz(n)
+    A.assemble_p1(ffhip.COEF_SIN3)
+    one = torch.ones(A.n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(one)
+    A.spmv(one, y)
+    rp, ci, va = A.download()
+    assert y.abs().max().item() <= 1e-12 * np.abs(va).max()
+    yt = torch.empty_like(one)
+    xr = torch.rand(A.n, dtype=torch.float64, device="cuda")
+    A.spmv(xr, y)
+    A.spmv(xr, yt, trans=True)
+    assert (y - yt).abs().max().item() <= 1e-13 * y.abs().max().item()
+    A.set_bc(LABELS)
+    b = torch.empty(A.n, dtype=torch.float64, device="cuda")
+    ffhip.rhs_p1(M, 1.0, LABELS, out=b)
+    sols = []
+    for mode in (0, 1):
+        x, st = ffhip.CG(A, mode).solve(b, eps=1e-10)
+        assert st["converged"] == 1 and st["true_gcg"] < 1.0001e-20 * st["gcg0"]
+        sols.append(x)
+    assert (sols[0] - sols[1]).norm().item() <= 1e-8 * sols[0].norm().item()

@@ -514,6 +514,33 @@ struct ffhip_cg_s {
   double *pA = nullptr, *pB = nullptr, *pC = nullptr;
   CgState *st = nullptr;
   MixState *ms = nullptr;
+  // per-kernel timing (measurement only): 3 events per iteration of a chunk
+  int profile = 0;
+  std::vector<hipEvent_t> ev;
+  double t_mv = 0., t_up = 0.;
+  int n_timed = 0;
+  void ensure_events(int k) {
+    while ((int)ev.size() < k) {
+      hipEvent_t e;
+      FFHIP_CHECK(hipEventCreate(&e));
+      ev.push_back(e);
+    }
+  }
+  void mark(int slot) {
+    if (profile) FFHIP_CHECK(hipEventRecord(ev[slot], ctx->stream));
+  }
+  // after a synchronised chunk: accumulate the iterations [it0, it1] that ran
+  void collect(int chunk_first_it, int nits, int last_exec_it) {
+    if (!profile) return;
+    for (int c = 0; c < nits; ++c) {
+      const int it = chunk_first_it + c;
+      if (it > last_exec_it) break;
+      float a = 0.f, b = 0.f;
+      FFHIP_CHECK(hipEventElapsedTime(&a, ev[3 * c], ev[3 * c + 1]));
+      FFHIP_CHECK(hipEventElapsedTime(&b, ev[3 * c + 1], ev[3 * c + 2]));
+      t_mv += a; t_up += b; ++n_timed;
+    }
+  }
 };
 
 static void cg_build_sell(ffhip_cg cg, bool transposed) {
@@ -634,18 +661,24 @@ int ffhip_cg_solve_mixed(ffhip_cg cg, const double *sval, const float *sval32, d
     FFHIP_CHECK(hipMemcpyAsync(cg->st, &hs, sizeof(hs), hipMemcpyHostToDevice, s));
     FFHIP_CHECK(hipMemcpyAsync(cg->ms, &hm, sizeof(hm), hipMemcpyHostToDevice, s));
     const int chunk = 16;
+    cg->ensure_events(3 * chunk);
     for (;;) {
+      const int it0 = it;
       for (int c = 0; c < chunk; ++c, ++it) {
         const float *pold = cg->p32[(it - 1) & 1];
         float *pnew = cg->p32[it & 1];
+        cg->mark(3 * c);
         k_mix_dir<<<G, kBlock, 0, s>>>(it, itmax, n, cg->S, cg->sptr, cg->slen, cg->scol, sval32,
                                        pold, cg->r32, pnew, cg->q32, cg->pA, cg->pB, cg->st, cg->ms);
+        cg->mark(3 * c + 1);
         k_mix_update<<<G, kBlock, 0, s>>>(it, n, cg->S, pnew, cg->q32, cg->y32, cg->r32, cg->pB,
                                           cg->pA, cg->st, cg->ms);
+        cg->mark(3 * c + 2);
       }
       FFHIP_CHECK(hipGetLastError());
       FFHIP_CHECK(hipMemcpyAsync(&hs, cg->st, sizeof(hs), hipMemcpyDeviceToHost, s));
       FFHIP_CHECK(hipStreamSynchronize(s));
+      cg->collect(it0, chunk, hs.iters);
       if (hs.done || hs.need_update) break;
     }
     if (hs.done) {  // NaN or itmax: fold what we have
@@ -752,23 +785,31 @@ int ffhip_cg_solve(ffhip_cg cg, const double *b, double *x, int on_device, doubl
     if (cg->ntgv > 0) k_init_bc<<<nblk(n, 256), 256, 0, s>>>(n, cg->wcl, cg->bw, tgv, cg->xw);
     CgState hs;
     memset(&hs, 0, sizeof(hs));
+    cg->t_mv = cg->t_up = 0.;
+    cg->n_timed = 0;
     if (cg->mode == FFHIP_CG_FP64) {
       const int G = cg->grid;
       k_pcg_init<<<G, kBlock, 0, s>>>(n, cg->S, cg->sptr, cg->slen, cg->scol, sval, cg->xw, cg->bw,
                                      cg->dinv, cg->G, cg->CG, cg->H[0], cg->pA, cg->st);
       FFHIP_CHECK(hipGetLastError());
       const int chunk = 16;
+      cg->ensure_events(3 * chunk);
       for (int it = 1;; ++it) {
         const double *Hold = cg->H[(it - 1) & 1];
         double *Hnew = cg->H[it & 1];
+        const int c = (it - 1) % chunk;
+        cg->mark(3 * c);
         k_pcg_dir<<<G, kBlock, 0, s>>>(it, itmax, eps, n, cg->S, cg->sptr, cg->slen, cg->scol, sval,
                                        Hold, cg->CG, cg->G, Hnew, cg->AH, cg->pA, cg->pB, cg->st);
+        cg->mark(3 * c + 1);
         k_pcg_update<<<G, kBlock, 0, s>>>(n, cg->S, Hnew, cg->AH, cg->dinv, cg->xw, cg->G, cg->CG,
                                           cg->pB, cg->pA, cg->st);
+        cg->mark(3 * c + 2);
         if (it % chunk == 0) {
           FFHIP_CHECK(hipGetLastError());
           FFHIP_CHECK(hipMemcpyAsync(&hs, cg->st, sizeof(hs), hipMemcpyDeviceToHost, s));
           FFHIP_CHECK(hipStreamSynchronize(s));
+          cg->collect(it - chunk + 1, chunk, hs.done ? hs.iters : it);
           if (hs.done) break;
         }
       }
@@ -790,6 +831,9 @@ int ffhip_cg_solve(ffhip_cg cg, const double *b, double *x, int on_device, doubl
       stats->true_gcg = res[2];
       stats->true_relres = res[1] > 0 ? std::sqrt(res[0] / res[1]) : std::sqrt(res[0]);
       stats->ms_solve = ms;
+      stats->timed = cg->n_timed;
+      stats->ms_matvec_avg = cg->n_timed ? cg->t_mv / cg->n_timed : 0.;
+      stats->ms_update_avg = cg->n_timed ? cg->t_up / cg->n_timed : 0.;
     }
     FFHIP_REQUIRE(hs.done != 4, FFHIP_ERR_NAN, "ffhip_cg_solve: NaN in CG (bad matrix)");
     FFHIP_REQUIRE(hs.done != 3, FFHIP_ERR_NOCONV,
@@ -797,10 +841,18 @@ int ffhip_cg_solve(ffhip_cg cg, const double *b, double *x, int on_device, doubl
   });
 }
 
+int ffhip_cg_set_profiling(ffhip_cg cg, int on) {
+  return guard([&] {
+    FFHIP_REQUIRE(cg, FFHIP_ERR_ARG, "ffhip_cg_set_profiling: null solver");
+    cg->profile = on ? 1 : 0;
+  });
+}
+
 int ffhip_cg_free(ffhip_cg cg) {
   return guard([&] {
     if (!cg) return;
     (void)hipStreamSynchronize(cg->ctx->stream);
+    for (hipEvent_t e : cg->ev) (void)hipEventDestroy(e);
     dfree(cg->sptr); dfree(cg->slen); dfree(cg->scol); dfree(cg->sval); dfree(cg->svalT);
     dfree(cg->sval32); dfree(cg->sval32T); dfree(cg->dinv); dfree(cg->dsc); dfree(cg->wcl);
     dfree(cg->xw); dfree(cg->bw); dfree(cg->G); dfree(cg->CG); dfree(cg->H[0]); dfree(cg->H[1]);

@@ -40,10 +40,12 @@ class CGStats(ctypes.Structure):
                 ("refinements", ctypes.c_int), ("reserved", ctypes.c_int),
                 ("gcg0", ctypes.c_double), ("gcg", ctypes.c_double),
                 ("true_gcg", ctypes.c_double), ("true_relres", ctypes.c_double),
-                ("ms_solve", ctypes.c_double)]
+                ("ms_solve", ctypes.c_double), ("ms_matvec_avg", ctypes.c_double),
+                ("ms_update_avg", ctypes.c_double), ("timed", ctypes.c_int),
+                ("reserved2", ctypes.c_int)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("reserved")}
 
 
 _vp = ctypes.c_void_p
@@ -79,6 +81,7 @@ _SIGS = {
     "ffhip_cg_update_values": (_i, [_vp]),
     "ffhip_cg_solve": (_i, [_vp, _vp, _vp, _i, _d, _i, _d, _i, ctypes.POINTER(CGStats)]),
     "ffhip_cg_free": (_i, [_vp]),
+    "ffhip_cg_set_profiling": (_i, [_vp, _i]),
     "ffhip_bench_spmv": (_i, [_vp, _i, _i, _vp, _vp, ctypes.POINTER(_d)]),
     "ffhip_last_ms": (_d, [_vp]),
 }
@@ -319,6 +322,9 @@ class CG:
         h = ctypes.c_void_p()
         check(lib().ffhip_cg_create(A.ctx.h, A.h, mode, ctypes.byref(h)))
         self.h, self.A, self.mode = h, A, mode
+
+    def set_profiling(self, on=True):
+        check(lib().ffhip_cg_set_profiling(self.h, int(on)))
 
     def update_values(self):
         check(lib().ffhip_cg_update_values(self.h))

@@ -130,6 +130,13 @@ typedef struct {
   double true_gcg;    /* final (g, D^-1 g) recomputed in FP64 from x         */
   double true_relres; /* ||b - A x||_2 / ||b||_2 in FP64                     */
   double ms_solve;    /* device time of the solve (HIP events)              */
+  /* with ffhip_cg_set_profiling(cg, 1): average device time of one launch of
+   * the iteration's mat-vec kernel and of its vector-update kernel, over the
+   * `timed` iterations of this solve (HIP events on the library stream) */
+  double ms_matvec_avg;
+  double ms_update_avg;
+  int timed;
+  int reserved2;
 } ffhip_cg_stats;
 /* Solver setup for a matrix (SolverCG ctor + HMatVirtPrecon,
  * src/femlib/VirtualSolverCG.hpp:27-69, 157-167): tgv detection
@@ -149,6 +156,8 @@ int ffhip_cg_solve(ffhip_cg cg, const double *b, double *x, int on_device,
                    double eps, int itmax, double tgv, int trans,
                    ffhip_cg_stats *stats);
 int ffhip_cg_free(ffhip_cg cg);
+/* Per-kernel HIP-event timing of the solve loop (measurement only). */
+int ffhip_cg_set_profiling(ffhip_cg cg, int on);
 
 /* ---- measurement helpers ---------------------------------------------- */
 /* Average device time (ms) of `reps` back-to-back launches of the solver's

@@ -40,6 +40,20 @@ constexpr int kSlice = 64;
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kSlice;
 
+// Tuning of the streaming mat-vec kernels: unroll depth of a row, the waves
+// per SIMD the register allocation targets, and resident blocks per CU.
+#ifndef FFHIP_SELL_U
+#define FFHIP_SELL_U 8
+#endif
+#ifndef FFHIP_WPE
+#define FFHIP_WPE 4
+#endif
+#ifndef FFHIP_GRID_PER_CU
+#define FFHIP_GRID_PER_CU (FFHIP_WPE)
+#endif
+#define FFHIP_MATVEC_ATTR __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FFHIP_WPE, FFHIP_WPE)))
+constexpr int kU = FFHIP_SELL_U;
+
 struct CgState {
   double gcg_prev[2];  // (g, Cg) at the start of the iteration, by parity
   double gcg0, eps2;
@@ -62,6 +76,43 @@ __device__ inline double reduce_partials(const double *__restrict__ p, int P, in
 
 template <class T>
 __device__ inline T ntload(const T *p) { return __builtin_nontemporal_load(p); }
+
+// One row of a slice: sum_k a_k * X(j_k) over the L (wave-uniform) entries of
+// this lane's row.  Unrolled by U: the U column and value loads of a round
+// are all issued before any gather, so every wave keeps 2U streaming loads
+// and U (or 2U) gathers in flight instead of one dependent chain per entry.
+template <int U, class V, class ACC, class F>
+__device__ inline ACC sell_row(const int32_t *__restrict__ colp, const V *__restrict__ valp,
+                               int L, F xval) {
+  ACC acc = 0;
+  for (int k = 0; k < L; k += U) {
+    int32_t j[U];
+    V a[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      j[u] = 0;  // past the row end: column 0 with a zero value
+      a[u] = 0;
+      if (k + u < L) {
+        j[u] = ntload(colp + (int64_t)(k + u) * kSlice);
+        a[u] = ntload(valp + (int64_t)(k + u) * kSlice);
+      }
+    }
+    ACC xv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) xv[u] = xval(j[u]);  // all gathers issued together
+#pragma unroll
+    for (int u = 0; u < U; ++u) acc += (ACC)a[u] * xv[u];
+  }
+  return acc;
+}
+
+// wave-uniform slice index (kept in scalar registers)
+__device__ inline int64_t uniform(int64_t v) {
+  const int lo = __builtin_amdgcn_readfirstlane((int)(v & 0xffffffff));
+  const int hi = __builtin_amdgcn_readfirstlane((int)(v >> 32));
+  return ((int64_t)hi << 32) | (uint32_t)lo;
+}
+__device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x / kSlice); }
 
 // logical slice range of this block
 __device__ inline void block_slices(int64_t S, int64_t &s0, int64_t &s1) {
@@ -173,7 +224,7 @@ __global__ void k_init_bc(int64_t n, const uint8_t *__restrict__ wcl, const doub
 
 // ---------------------------------------------------------------- FP64 PCG
 // G = A x - b, CG = D^-1 G, H0 = 0, partial (G, CG)
-__global__ __launch_bounds__(kBlock) void k_pcg_init(
+__global__ FFHIP_MATVEC_ATTR void k_pcg_init(
     int64_t n, int64_t S, const int64_t *__restrict__ sptr, const int32_t *__restrict__ slen,
     const int32_t *__restrict__ scol, const double *__restrict__ sval,
     const double *__restrict__ x, const double *__restrict__ b, const double *__restrict__ dinv,
@@ -182,16 +233,13 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(
   __shared__ double lds[kBlock / kWave];
   int64_t s0, s1;
   block_slices(S, s0, s1);
-  const int w = threadIdx.x / kSlice, lane = threadIdx.x % kSlice;
+  const int w = wave_id(), lane = threadIdx.x % kSlice;
   double part = 0.;
   for (int64_t s = s0 + w; s < s1; s += kWavesPerBlock) {
-    const int64_t base = sptr[s] + lane;
-    const int L = slen[s];
-    double acc = 0.;
-    for (int k = 0; k < L; ++k) {
-      const int64_t idx = base + (int64_t)k * kSlice;
-      acc += ntload(sval + idx) * x[ntload(scol + idx)];
-    }
+    const int64_t base = uniform(sptr[s]) + lane;
+    const int L = __builtin_amdgcn_readfirstlane(slen[s]);
+    const double acc = sell_row<kU, double, double>(scol + base, sval + base, L,
+                                                   [&](int32_t j) { return x[j]; });
     const int64_t r = s * kSlice + lane;
     if (r < n) {
       const double g = acc + -1. * b[r];
@@ -205,7 +253,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(
   if (blockIdx.x == 0 && threadIdx.x == 0) { st->done = 0; st->iters = 0; }
 }
 
-__global__ __launch_bounds__(kBlock) void k_pcg_dir(
+__global__ FFHIP_MATVEC_ATTR void k_pcg_dir(
     int it, int itmax, double eps, int64_t n, int64_t S, const int64_t *__restrict__ sptr,
     const int32_t *__restrict__ slen, const int32_t *__restrict__ scol,
     const double *__restrict__ sval, const double *__restrict__ Hold,
@@ -244,17 +292,13 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dir(
   }
   int64_t s0, s1;
   block_slices(S, s0, s1);
-  const int w = threadIdx.x / kSlice, lane = threadIdx.x % kSlice;
+  const int w = wave_id(), lane = threadIdx.x % kSlice;
   double p_hah = 0., p_gh = 0.;
   for (int64_t s = s0 + w; s < s1; s += kWavesPerBlock) {
-    const int64_t base = sptr[s] + lane;
-    const int L = slen[s];
-    double acc = 0.;
-    for (int k = 0; k < L; ++k) {
-      const int64_t idx = base + (int64_t)k * kSlice;
-      const int32_t j = ntload(scol + idx);
-      acc += ntload(sval + idx) * (gamma * Hold[j] - CG[j]);
-    }
+    const int64_t base = uniform(sptr[s]) + lane;
+    const int L = __builtin_amdgcn_readfirstlane(slen[s]);
+    const double acc = sell_row<kU, double, double>(
+        scol + base, sval + base, L, [&](int32_t j) { return gamma * Hold[j] - CG[j]; });
     const int64_t r = s * kSlice + lane;
     if (r < n) {
       const double h = gamma * Hold[r] - CG[r];
@@ -302,7 +346,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(
 
 // ---------------------------------------------------------------- residual check
 // r = b - A x (FP64); partials (r,r), (b,b), (r, D^-1 r)
-__global__ __launch_bounds__(kBlock) void k_residual(
+__global__ FFHIP_MATVEC_ATTR void k_residual(
     int64_t n, int64_t S, const int64_t *__restrict__ sptr, const int32_t *__restrict__ slen,
     const int32_t *__restrict__ scol, const double *__restrict__ sval,
     const double *__restrict__ x, const double *__restrict__ b, const double *__restrict__ dinv,
@@ -310,16 +354,13 @@ __global__ __launch_bounds__(kBlock) void k_residual(
   __shared__ double lds[kBlock / kWave];
   int64_t s0, s1;
   block_slices(S, s0, s1);
-  const int w = threadIdx.x / kSlice, lane = threadIdx.x % kSlice;
+  const int w = wave_id(), lane = threadIdx.x % kSlice;
   double prr = 0., pbb = 0., prc = 0.;
   for (int64_t s = s0 + w; s < s1; s += kWavesPerBlock) {
-    const int64_t base = sptr[s] + lane;
-    const int L = slen[s];
-    double acc = 0.;
-    for (int k = 0; k < L; ++k) {
-      const int64_t idx = base + (int64_t)k * kSlice;
-      acc += ntload(sval + idx) * x[ntload(scol + idx)];
-    }
+    const int64_t base = uniform(sptr[s]) + lane;
+    const int L = __builtin_amdgcn_readfirstlane(slen[s]);
+    const double acc = sell_row<kU, double, double>(scol + base, sval + base, L,
+                                                   [&](int32_t j) { return x[j]; });
     const int64_t r = s * kSlice + lane;
     if (r < n) {
       const double rr = b[r] - acc;
@@ -340,21 +381,17 @@ __global__ __launch_bounds__(kBlock) void k_residual(
 
 // plain SELL mat-vec for measurement: y = A x
 template <class V, class X>
-__global__ __launch_bounds__(kBlock) void k_sell_spmv(
+__global__ FFHIP_MATVEC_ATTR void k_sell_spmv(
     int64_t n, int64_t S, const int64_t *__restrict__ sptr, const int32_t *__restrict__ slen,
     const int32_t *__restrict__ scol, const V *__restrict__ sval, const X *__restrict__ x,
     X *__restrict__ y) {
   int64_t s0, s1;
   block_slices(S, s0, s1);
-  const int w = threadIdx.x / kSlice, lane = threadIdx.x % kSlice;
+  const int w = wave_id(), lane = threadIdx.x % kSlice;
   for (int64_t s = s0 + w; s < s1; s += kWavesPerBlock) {
-    const int64_t base = sptr[s] + lane;
-    const int L = slen[s];
-    X acc = 0;
-    for (int k = 0; k < L; ++k) {
-      const int64_t idx = base + (int64_t)k * kSlice;
-      acc += (X)ntload(sval + idx) * x[ntload(scol + idx)];
-    }
+    const int64_t base = uniform(sptr[s]) + lane;
+    const int L = __builtin_amdgcn_readfirstlane(slen[s]);
+    const X acc = sell_row<kU, V, X>(scol + base, sval + base, L, [&](int32_t j) { return x[j]; });
     const int64_t r = s * kSlice + lane;
     if (r < n) y[r] = acc;
   }
@@ -374,7 +411,7 @@ struct MixState {
 // K_m1: p = r32 + cb*p_old (on the fly for every gathered column); q = Ahat p;
 // partial (p, q).  Requests a reliable update when (r32, r32) fell by 1e-4
 // since the last update, or the recurrence says the FP64 criterion is met.
-__global__ __launch_bounds__(kBlock) void k_mix_dir(
+__global__ FFHIP_MATVEC_ATTR void k_mix_dir(
     int it, int itmax, int64_t n, int64_t S, const int64_t *__restrict__ sptr,
     const int32_t *__restrict__ slen, const int32_t *__restrict__ scol,
     const float *__restrict__ sval, const float *__restrict__ pold,
@@ -407,17 +444,13 @@ __global__ __launch_bounds__(kBlock) void k_mix_dir(
   const float cbf = (float)cb;
   int64_t s0, s1;
   block_slices(S, s0, s1);
-  const int w = threadIdx.x / kSlice, lane = threadIdx.x % kSlice;
+  const int w = wave_id(), lane = threadIdx.x % kSlice;
   double part = 0.;
   for (int64_t s = s0 + w; s < s1; s += kWavesPerBlock) {
-    const int64_t base = sptr[s] + lane;
-    const int L = slen[s];
-    float acc = 0.f;
-    for (int k = 0; k < L; ++k) {
-      const int64_t idx = base + (int64_t)k * kSlice;
-      const int32_t j = ntload(scol + idx);
-      acc += ntload(sval + idx) * (r32[j] + cbf * pold[j]);
-    }
+    const int64_t base = uniform(sptr[s]) + lane;
+    const int L = __builtin_amdgcn_readfirstlane(slen[s]);
+    const float acc = sell_row<kU, float, float>(scol + base, sval + base, L,
+                                                [&](int32_t j) { return r32[j] + cbf * pold[j]; });
     const int64_t r = s * kSlice + lane;
     if (r < n) {
       const float p = r32[r] + cbf * pold[r];
@@ -718,7 +751,7 @@ int ffhip_cg_create(ffhip_ctx ctx, ffhip_mat A, int mode, ffhip_cg *out) {
     cg->ctx = ctx; cg->A = A; cg->mode = mode; cg->n = A->n;
     cg->S = (A->n + kSlice - 1) / kSlice;
     int64_t g = (cg->S + kWavesPerBlock - 1) / kWavesPerBlock;
-    const int64_t gmax = 8 * (int64_t)ctx->num_cu;
+    const int64_t gmax = FFHIP_GRID_PER_CU * (int64_t)ctx->num_cu;
     if (g > gmax) g = gmax;
     if (g >= 64) g -= g % 8;  // whole XCD groups
     cg->grid = (int)g;

@@ -21,7 +21,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBPATH = os.path.join(os.path.dirname(HERE), "libffhip.so")
+LIBPATH = os.environ.get("FFHIP_LIB") or os.path.join(os.path.dirname(HERE), "libffhip.so")
 
 COEF_CONST, COEF_SIN3, COEF_P0 = 0, 1, 2
 CG_FP64, CG_MIXED = 0, 1

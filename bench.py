@@ -156,6 +156,9 @@ def main():
     bytes_mv32 = 8 * nnz + 16 * nv
     achieved = bytes_mv / (mv_ms * 1e-3) / 1e9
     s64, sm = res[-1]["s64"], res[-1]["sm"]
+    copy_bytes = 1 << 30
+    copy_ms = mesh.ctx.bench_copy(copy_bytes, 20)
+    copy_gbs = 2 * copy_bytes / (copy_ms * 1e-3) / 1e9  # read + write
     line = {
         "metric": METRIC,
         "value": round(nnz / t_asm / 1e6, 2),
@@ -198,6 +201,7 @@ def main():
             "kernel": "k_pcg_dir (FP64 SELL mat-vec + direction update + 2 dots)",
             "bytes_per_launch": bytes_mv,
             "avg_launch_ms": round(mv_ms, 5),
+            "copy_reference_GBs": round(copy_gbs, 1),
             "fp32_matvec": {"kernel": "k_mix_dir", "bytes_per_launch": bytes_mv32,
                             "avg_launch_ms": round(mv_ms32, 5),
                             "achieved_GBs": round(bytes_mv32 / (mv_ms32 * 1e-3) / 1e9, 1)},

@@ -238,6 +238,10 @@ __global__ __launch_bounds__(256) void k_p1_element_3d(
       for (int k = 0; k < 3; ++k) P[k] = l0 * v[0][k] + px * v[1][k] + py * v[2][k] + pz * v[3][k];
       S += (mes * qf5_weight(q)) * kappa_sin3(P[0], P[1], P[2]);
     }
+  } else if (kind == FFHIP_COEF_QP) {
+    S = 0.;
+#pragma unroll
+    for (int q = 0; q < 14; ++q) S += (mes * qf5_weight(q)) * kappa_e[14 * e + q];
   } else {
     const double kc = (kind == FFHIP_COEF_P0) ? kappa_e[e] : c0;
     S = 0.;
@@ -288,6 +292,8 @@ __global__ __launch_bounds__(256) void k_p1_element_2d(
       const double Y = l0 * v[0][1] + px * v[1][1] + py * v[2][1];
       const double tp = 2 * M_PI;
       kc = 1 + (0.5 * sin(tp * X)) * sin(tp * Y);
+    } else if (kind == FFHIP_COEF_QP) {
+      kc = kappa_e[7 * e + q];
     } else {
       kc = (kind == FFHIP_COEF_P0) ? kappa_e[e] : c0;
     }
@@ -580,17 +586,20 @@ int ffhip_assemble_p1(ffhip_ctx ctx, ffhip_mesh m, ffhip_mat A, int kind, double
   return guard([&] {
     FFHIP_REQUIRE(ctx && m && A && A->mesh == m && A->slotmap, FFHIP_ERR_ARG,
                   "ffhip_assemble_p1: matrix pattern was not built from this mesh");
-    FFHIP_REQUIRE(kind == FFHIP_COEF_CONST || kind == FFHIP_COEF_SIN3 || kind == FFHIP_COEF_P0,
+    FFHIP_REQUIRE(kind == FFHIP_COEF_CONST || kind == FFHIP_COEF_SIN3 || kind == FFHIP_COEF_P0 ||
+                      kind == FFHIP_COEF_QP,
                   FFHIP_ERR_ARG, "ffhip_assemble_p1: coefficient kind %d", kind);
-    FFHIP_REQUIRE(kind != FFHIP_COEF_P0 || kappa_e, FFHIP_ERR_ARG, "ffhip_assemble_p1: kappa_e");
+    const bool arr = kind == FFHIP_COEF_P0 || kind == FFHIP_COEF_QP;
+    FFHIP_REQUIRE(!arr || kappa_e, FFHIP_ERR_ARG, "ffhip_assemble_p1: kappa_e");
     hipStream_t s = ctx->stream;
     const int nu = m->nloc == 4 ? 10 : 6;
     double *Ke = dalloc<double>(nu * m->nt);
     const double *kd = kappa_e;
     double *kbuf = nullptr;
-    if (kind == FFHIP_COEF_P0 && !kappa_on_device) {
-      kbuf = dalloc<double>(m->nt);
-      FFHIP_CHECK(hipMemcpyAsync(kbuf, kappa_e, 8 * m->nt, hipMemcpyHostToDevice, s));
+    if (arr && !kappa_on_device) {
+      const int64_t cnt = kind == FFHIP_COEF_P0 ? m->nt : m->nt * (m->dim == 3 ? 14 : 7);
+      kbuf = dalloc<double>(cnt);
+      FFHIP_CHECK(hipMemcpyAsync(kbuf, kappa_e, 8 * cnt, hipMemcpyHostToDevice, s));
       kd = kbuf;
     }
     if (m->dim == 2) upload_t5_once();
@@ -721,6 +730,14 @@ int ffhip_mat_upload(ffhip_ctx ctx, int64_t n, int64_t nnz, const int64_t *rowpt
     FFHIP_CHECK(hipMemcpyAsync(A->diag, d.data(), 8 * n, hipMemcpyHostToDevice, s));
     FFHIP_CHECK(hipStreamSynchronize(s));
     *out = A;
+  });
+}
+
+int ffhip_mat_set_values(ffhip_mat A, const double *vals) {
+  return guard([&] {
+    FFHIP_REQUIRE(A && vals, FFHIP_ERR_ARG, "ffhip_mat_set_values: null argument");
+    FFHIP_CHECK(hipMemcpyAsync(A->vals, vals, 8 * A->nnz, hipMemcpyHostToDevice, A->ctx->stream));
+    FFHIP_CHECK(hipStreamSynchronize(A->ctx->stream));
   });
 }
 

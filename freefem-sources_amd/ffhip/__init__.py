@@ -23,7 +23,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.environ.get("FFHIP_LIB") or os.path.join(os.path.dirname(HERE), "libffhip.so")
 
-COEF_CONST, COEF_SIN3, COEF_P0 = 0, 1, 2
+COEF_CONST, COEF_SIN3, COEF_P0, COEF_QP = 0, 1, 2, 3
 CG_FP64, CG_MIXED = 0, 1
 ERRORS = {-1: "bad argument", -2: "HIP error", -3: "CG did not converge", -4: "NaN in CG",
           -5: "unsupported", -6: "internal error"}
@@ -75,6 +75,7 @@ _SIGS = {
     "ffhip_mat_download": (_i, [_vp, _vp, _vp, _vp]),
     "ffhip_mat_upload": (_i, [_vp, _l, _l, _vp, _vp, _vp, _pvp]),
     "ffhip_mat_device_ptrs": (_i, [_vp, _pvp, _pvp, _pvp]),
+    "ffhip_mat_set_values": (_i, [_vp, _vp]),
     "ffhip_mat_free": (_i, [_vp]),
     "ffhip_spmv": (_i, [_vp, _vp, _vp, _vp, _i]),
     "ffhip_cg_create": (_i, [_vp, _vp, _i, _pvp]),
@@ -84,6 +85,7 @@ _SIGS = {
     "ffhip_cg_set_profiling": (_i, [_vp, _i]),
     "ffhip_bench_spmv": (_i, [_vp, _i, _i, _vp, _vp, ctypes.POINTER(_d)]),
     "ffhip_last_ms": (_d, [_vp]),
+    "ffhip_bench_copy": (_i, [_vp, _l, _i, ctypes.POINTER(_d)]),
 }
 
 _LIB = None
@@ -151,6 +153,12 @@ class Context:
 
     def last_ms(self):
         return lib().ffhip_last_ms(self.h)
+
+    def bench_copy(self, nbytes, reps=20):
+        """average ms of a plain device copy of nbytes (read + write)"""
+        ms = _d()
+        check(lib().ffhip_bench_copy(self.h, int(nbytes), reps, ctypes.byref(ms)))
+        return ms.value
 
     def close(self):
         if self.h:
@@ -283,6 +291,11 @@ class Matrix:
         va = np.zeros(self.nnz)
         check(lib().ffhip_mat_download(self.h, _hp(rp), _hp(ci), _hp(va)))
         return rp, ci, va
+
+    def set_values(self, vals):
+        vals = np.ascontiguousarray(vals, np.float64)
+        assert vals.size == self.nnz
+        check(lib().ffhip_mat_set_values(self.h, _hp(vals)))
 
     def device_ptrs(self):
         a, b, c = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()

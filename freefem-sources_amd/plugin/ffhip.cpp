@@ -1,0 +1,343 @@
+// ffhip.cpp — FreeFEM plugin: `load "ffhip"`.
+//
+// Drop-in MI355X path for
+//   matrix A = va(Vh, Vh, solver=HIPCG);   u[] = A^-1 * b;
+// on 3D P1 Poisson-type forms, through libffhip's C-ABI (include/ffhip.h):
+//
+//  * solvers "HIPCG" (FP64 Jacobi-PCG) and "HIPCGMIXED" (FP32 matrix and
+//    inner CG, FP64 residual refinement), registered with addsolver
+//    (src/femlib/SparseLinearSolver.hpp:96-102) with the orTypeSol of SolverCG
+//    (src/femlib/VirtualSolverCG.hpp:116) so the matrix gets the same full
+//    (non-half) storage as with solver=CG.  `ffhipreplacecg(1)` re-points
+//    solver=CG itself (TheFFSolver::ChangeSolver, SparseLinearSolver.hpp:59-71).
+//  * a higher-preference "=" / "<-" operator for
+//    (Matrice_Creuse<double>*, Call_FormBilinear<v_fes3,v_fes3>*)
+//    (the overload of OpMatrixtoBilinearForm, src/fflib/problem.hpp:897-927,
+//    registered at src/fflib/lgfem.cpp:6669-6683; the highest pref wins,
+//    src/fflib/AFunction2.cpp:87-107).  It recognises
+//        int3d(Th)(c*(dx(u)*dx(v)+dy(u)*dy(v)+dz(u)*dz(v))) [+ on(labels,u=g)]
+//    on a scalar P1 space with the default Tet_5 rule; the coefficient c is
+//    evaluated by FreeFEM at the 14 quadrature points of every element (as
+//    Element_Op does, problem.cpp:6392-6395) unless it is a constant, and the
+//    pattern, values and tgv rows are built on the GPU.  Any other form falls
+//    back to FreeFEM's own creationBlockOfMatrixToBilinearForm
+//    (problem.hpp:1663-1714), as the PETSc plugin does for its Mat type.
+//
+// Build (FreeFEM's plugin script):  ff-c++ -auto ffhip.cpp -I<repo>/include
+//   -L<repo>/freefem-sources_amd -lffhip
+#include <ff++.hpp>
+
+#include <cstring>
+#include <set>
+#include <vector>
+
+#include "AFunction_ext.hpp"
+#include "problem.hpp"
+
+#include "ffhip.h"
+
+using namespace std;
+using namespace Fem2D;
+
+// defined in src/fflib/lgfem.cpp:7792 (namespace Fem2D), exported by the
+// FreeFEM executable; declared where it is used, as src/fflib/problem.cpp:109 does
+namespace Fem2D {
+void Expandsetoflab(Stack stack, const BC_set &bc, set<long> &setoflab);
+}
+
+namespace {
+
+ffhip_ctx hip_ctx() {
+  static ffhip_ctx ctx = nullptr;
+  if (!ctx) {
+    if (ffhip_init(0, &ctx) != FFHIP_OK) {
+      cerr << "ffhip: " << ffhip_last_error() << endl;
+      ExecError("ffhip: no usable MI355X device");
+    }
+  }
+  return ctx;
+}
+
+void hip_check(int rc, const char *what) {
+  if (rc == FFHIP_OK) return;
+  cerr << "ffhip: " << what << ": " << ffhip_last_error() << endl;
+  ExecError(what);
+}
+
+// ---------------------------------------------------------------- solver
+template <int MODE>
+class SolverHIPCG : public VirtualSolver<int, double> {
+ public:
+  // 1 unsym, 2 herm, 4 sym, 8 pos, 32 seq: as SolverCG
+  static const int orTypeSol = 1 | 2 | 4 | 8 | 32;
+  typedef HashMatrix<int, double> HMat;
+  HMat *ptA;
+  ffhip_mat M;
+  ffhip_cg cg;
+  int verb, itermax;
+  double eps, tgv;
+  double *veps;
+  long *getnbiter;
+  long cn, cs;
+
+  SolverHIPCG(HMat &A, const Data_Sparse_Solver &ds, Stack)
+      : ptA(&A), M(nullptr), cg(nullptr), verb(ds.verb), itermax(ds.itmax > 0 ? ds.itmax : A.n),
+        eps(ds.epsilon), tgv(ds.tgv), veps(ds.veps), getnbiter(ds.getnbiter), cn(0), cs(0) {
+    ffassert(A.n == A.m);
+    if (verb > 4)
+      cout << " ## SolverHIPCG" << (MODE ? "MIXED " : " ") << A.n << " eps " << eps << " itmax "
+           << itermax << endl;
+  }
+
+  void UpdateState() {
+    if (ptA->GetReDoNumerics()) cn++;
+    if (ptA->GetReDoSymbolic()) cs++;
+    this->ChangeCodeState(ptA->n, cs, cn);
+  }
+
+  // the pattern: HashMatrix::CSR (sorted, HashMatrix.cpp:858-867) to the GPU
+  void fac_symbolic() {
+    int *ia = 0, *ja = 0;
+    double *a = 0;
+    ptA->CSR(ia, ja, a);
+    const int64_t n = ptA->n, nnz = ia[n];
+    vector<int64_t> rp(ia, ia + n + 1);
+    if (cg) { ffhip_cg_free(cg); cg = nullptr; }
+    if (M) { ffhip_mat_free(M); M = nullptr; }
+    hip_check(ffhip_mat_upload(hip_ctx(), n, nnz, rp.data(), ja, a, &M), "HIPCG: matrix upload");
+    hip_check(ffhip_cg_create(hip_ctx(), M, MODE ? FFHIP_CG_MIXED : FFHIP_CG_FP64, &cg),
+              "HIPCG: solver setup");
+  }
+
+  void fac_numeric() {
+    int *ia = 0, *ja = 0;
+    double *a = 0;
+    ptA->CSR(ia, ja, a);
+    hip_check(ffhip_mat_set_values(M, a), "HIPCG: matrix values");
+    hip_check(ffhip_cg_update_values(cg), "HIPCG: solver values");
+  }
+
+  // SolverCG::dosolver semantics (VirtualSolverCG.hpp:169-190)
+  void dosolver(double *x, double *b, int N, int trans) {
+    int err = 0;
+    if (getnbiter) *getnbiter = 0;
+    for (int k = 0; k < N; ++k) {
+      ffhip_cg_stats st;
+      const int rc = ffhip_cg_solve(cg, b + (size_t)k * ptA->n, x + (size_t)k * ptA->n, 0, eps,
+                                    itermax, tgv, trans, &st);
+      if (rc == FFHIP_ERR_NOCONV) err++;
+      else hip_check(rc, "HIPCG: solve");
+      if (rc == FFHIP_OK && getnbiter) *getnbiter += st.iters;
+      if (veps) *veps = st.gcg;
+      if (verb > 1)
+        cout << " HIPCG: " << (st.converged ? "converge after " : "no convergence after ") << st.iters
+             << " g=" << st.gcg << " (" << st.ms_solve << " ms on the GPU)" << endl;
+    }
+    if (err) {
+      cerr << "Error: ConjugueGradient do not converge nb end =" << err << endl;
+      ffassert(0);
+    }
+  }
+
+  ~SolverHIPCG() {
+    // device buffers only: the HashMatrix may already be deleted
+    // (type_isdeleted, HashMatrix.hpp:62) and is not touched here
+    if (cg) ffhip_cg_free(cg);
+    if (M) ffhip_mat_free(M);
+  }
+};
+
+long ffhip_replace_cg(long on) {
+  if (on) TheFFSolver<int, double>::ChangeSolver("CG", "HIPCG");
+  return on;
+}
+
+// ---------------------------------------------------------------- assembly
+typedef Call_FormBilinear<v_fes3, v_fes3> CFB3;
+
+struct Recognised {
+  const FormBilinear *bf = nullptr;
+  C_F0 coef;                  // the common coefficient of the three terms
+  bool coef_const = false;
+  double c0 = 0.;
+  set<long> labels;           // Dirichlet labels (all BC_set of the form)
+};
+
+// int3d(Th)(c*(dx(u)*dx(v)+dy(u)*dy(v)+dz(u)*dz(v))) + on(...) on P1, or false
+bool recognise(Stack stack, const FESpace3 &Uh, const FESpace3 &Vh, const list<C_F0> &largs,
+               Recognised &r) {
+  if (&Uh != &Vh || Uh.N != 1 || Uh.TFE[0] != &DataFE<Mesh3>::P1) return false;
+  if (Uh.NbOfDF != Uh.Th.nv) return false;
+  FreeFempp::TypeVarForm<double> *tvf = FreeFempp::TypeVarForm<double>::Global;
+  for (list<C_F0>::const_iterator ii = largs.begin(); ii != largs.end(); ++ii) {
+    const aType t = ii->left();
+    const Expression e = ii->LeftValue();
+    if (t == tvf->tFL) continue;  // linear terms do not enter the matrix
+    if (t == tvf->tBC) {
+      const BC_set *bc = dynamic_cast<const BC_set *>(e);
+      if (!bc) return false;
+      for (size_t k = 0; k < bc->bc.size(); ++k)
+        if (bc->bc[k].first != 0) return false;
+      set<long> labs;
+      Fem2D::Expandsetoflab(stack, *bc, labs);
+      r.labels.insert(labs.begin(), labs.end());
+      continue;
+    }
+    if (t != tvf->tFB || r.bf) return false;  // exactly one bilinear term
+    const FormBilinear *bf = dynamic_cast<const FormBilinear *>(e);
+    if (!bf || bf->di->kind != CDomainOfIntegration::int3d || !bf->di->what.empty()) return false;
+    if (GetAny<const Mesh3 *>((*bf->di->Th)(stack)) != &Uh.Th) return false;
+    if (bf->di->FIV(stack).n != QuadratureFormular_Tet_5.n) return false;  // default Tet_5 only
+    const BilinearOperator &Op = *bf->b;
+    if (Op.v.size() != 3) return false;
+    bool seen[4] = {false, false, false, false};
+    int k = 0;
+    for (BilinearOperator::const_iterator l = Op.v.begin(); l != Op.v.end(); ++l, ++k) {
+      const BilinearOperator::K &ll = *l;
+      const int jcomp = ll.first.first.first, jop = ll.first.first.second;
+      const int icomp = ll.first.second.first, iop = ll.first.second.second;
+      if (jcomp || icomp || jop != iop || jop < op_dx || jop > op_dz || seen[jop]) return false;
+      seen[jop] = true;
+      if (k == 0) r.coef = ll.second;
+      else if (ll.second.LeftValue() != r.coef.LeftValue()) return false;
+    }
+    r.bf = bf;
+  }
+  if (!r.bf) return false;
+  if (r.coef.EvaluableWithOutStack()) {
+    r.coef_const = true;
+    r.c0 = GetAny<double>(r.coef.eval(stack));
+  }
+  return true;
+}
+
+// the coefficient at the 14 Tet_5 points of every element, evaluated by the
+// interpreter as Element_Op does (problem.cpp:6385-6395)
+void eval_coef_qp(Stack stack, const FESpace3 &Uh, const C_F0 &coef, vector<double> &kq) {
+  const Mesh3 &Th = Uh.Th;
+  const GQuadratureFormular<R3> &FI = QuadratureFormular_Tet_5;
+  kq.resize((size_t)Th.nt * FI.n);
+  MeshPoint mp = *MeshPointStack(stack);
+  for (int it = 0; it < Th.nt; ++it) {
+    const FElement3 K(Uh[it]);
+    for (int q = 0; q < FI.n; ++q) {
+      const R3 Pt(FI[q]);
+      MeshPointStack(stack)->set(K.T(Pt), Pt, K);
+      kq[(size_t)it * FI.n + q] = GetAny<double>(coef.eval(stack));
+    }
+  }
+  *MeshPointStack(stack) = mp;
+}
+
+void assemble_on_gpu(Stack stack, const FESpace3 &Uh, const Recognised &r, double tgv,
+                     Matrice_Creuse<double> &A) {
+  const Mesh3 &Th = Uh.Th;
+  vector<double> xyz(3 * (size_t)Th.nv);
+  vector<int32_t> tets(4 * (size_t)Th.nt), btri(3 * (size_t)Th.nbe), blab(Th.nbe);
+  for (int i = 0; i < Th.nv; ++i) {
+    xyz[3 * i] = Th(i).x; xyz[3 * i + 1] = Th(i).y; xyz[3 * i + 2] = Th(i).z;
+  }
+  for (int k = 0; k < Th.nt; ++k)
+    for (int j = 0; j < 4; ++j) tets[4 * k + j] = Th(Th[k][j]);  // P1 dof = vertex id
+  for (int k = 0; k < Th.nbe; ++k) {
+    for (int j = 0; j < 3; ++j) btri[3 * k + j] = Th(Th.be(k)[j]);
+    blab[k] = Th.be(k).lab;
+  }
+  ffhip_ctx ctx = hip_ctx();
+  ffhip_mesh m = nullptr;
+  ffhip_mat M = nullptr;
+  hip_check(ffhip_mesh3_upload(ctx, Th.nv, Th.nt, Th.nbe, xyz.data(), tets.data(), btri.data(),
+                               blab.data(), &m), "ffhip: mesh upload");
+  hip_check(ffhip_pattern_p1(ctx, m, &M), "ffhip: pattern");
+  if (r.coef_const) {
+    hip_check(ffhip_assemble_p1(ctx, m, M, FFHIP_COEF_CONST, r.c0, nullptr, 0), "ffhip: assembly");
+  } else {
+    vector<double> kq;
+    eval_coef_qp(stack, Uh, r.coef, kq);
+    hip_check(ffhip_assemble_p1(ctx, m, M, FFHIP_COEF_QP, 0., kq.data(), 0), "ffhip: assembly");
+  }
+  if (!r.labels.empty()) {
+    vector<int32_t> labs(r.labels.begin(), r.labels.end());
+    hip_check(ffhip_set_bc(ctx, m, M, labs.data(), (int)labs.size(), tgv, nullptr), "ffhip: on()");
+  }
+  int64_t n = 0, nnz = 0;
+  ffhip_mat_info(M, &n, &nnz);
+  vector<int64_t> rp(n + 1);
+  vector<int> ia(n + 1), ja(nnz);
+  vector<double> aa(nnz);
+  hip_check(ffhip_mat_download(M, rp.data(), ja.data(), aa.data()), "ffhip: download");
+  for (int64_t i = 0; i <= n; ++i) ia[i] = (int)rp[i];
+  ffhip_mat_free(M);
+  ffhip_mesh_free(m);
+  // the result is an ordinary HashMatrix (HashMatrix::set with CSR input,
+  // HashMatrix.cpp:697-728) so every later FreeFEM operation works on it
+  MatriceMorse<double> *pH = new MatriceMorse<double>((int)n, (int)n, (int)nnz, 0);
+  pH->set((int)n, (int)n, 0, (size_t)nnz, ia.data(), ja.data(), aa.data(), 0, 1);
+  A.A.master(pH);
+}
+
+class OpHipMatrixP1 : public OneOperator {
+ public:
+  int init;
+  class Op : public E_F0mps {
+   public:
+    CFB3 *b;
+    Expression a;
+    int init;
+    Op(Expression aa, Expression bb, int ii)
+        : b(new CFB3(*dynamic_cast<const CFB3 *>(bb))), a(aa), init(ii) {
+      ffassert(b && b->nargs);
+    }
+    operator aType() const { return atype<Matrice_Creuse<double> *>(); }
+    AnyType operator()(Stack stack) const {
+      pfes3 *pUh = GetAny<pfes3 *>((*b->euh)(stack));
+      pfes3 *pVh = GetAny<pfes3 *>((*b->evh)(stack));
+      const FESpace3 *PUh = (FESpace3 *)**pUh;
+      const FESpace3 *PVh = (FESpace3 *)**pVh;
+      Data_Sparse_Solver ds;
+      ds.factorize = 0;
+      ds.initmat = true;
+      const int np = OpCall_FormBilinear_np::n_name_param - NB_NAME_PARM_HMAT;
+      SetEnd_Data_Sparse_Solver<double>(stack, ds, b->nargs, np);
+      WhereStackOfPtr2Free(stack) = new StackOfPtr2Free(stack);
+      Matrice_Creuse<double> &A(*GetAny<Matrice_Creuse<double> *>((*a)(stack)));
+      if (init) A.init();
+      if (!PUh || !PVh) return SetAny<Matrice_Creuse<double> *>(&A);
+      Recognised r;
+      if (ds.sym == 0 && ds.tgv > 0 && recognise(stack, *PUh, *PVh, b->largs, r)) {
+        A.A = 0;
+        A.Uh = *PUh;
+        A.Vh = *PVh;
+        assemble_on_gpu(stack, *PUh, r, ds.tgv, A);
+      } else {
+        creationBlockOfMatrixToBilinearForm<double, Mesh3, FESpace3, FESpace3>(PUh, PVh, ds.sym, ds.tgv,
+                                                                               b->largs, stack, A);
+      }
+      A.pHM()->half = ds.sym;
+      SetSolver(stack, false, *A.A, ds);
+      return SetAny<Matrice_Creuse<double> *>(&A);
+    }
+  };
+  E_F0 *code(const basicAC_F0 &args) const {
+    return new Op(to<Matrice_Creuse<double> *>(args[0]), args[1], init);
+  }
+  OpHipMatrixP1(int ii)
+      : OneOperator(atype<Matrice_Creuse<double> *>(), atype<Matrice_Creuse<double> *>(),
+                    atype<const CFB3 *>()),
+        init(ii) {
+    pref = 10;  // preferred over OpMatrixtoBilinearForm (pref 0)
+  }
+};
+
+}  // namespace
+
+static void Load_Init() {
+  addsolver<SolverHIPCG<0> >("HIPCG", 50);
+  addsolver<SolverHIPCG<1> >("HIPCGMIXED", 50);
+  Global.Add("ffhipreplacecg", "(", new OneOperator1<long, long>(ffhip_replace_cg));
+  TheOperators->Add("=", new OpHipMatrixP1(0));
+  TheOperators->Add("<-", new OpHipMatrixP1(1));
+  if (verbosity > 1) cout << " load: ffhip (" << ffhip_version() << ")" << endl;
+}
+
+LOADFUNC(Load_Init)

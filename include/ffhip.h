@@ -77,14 +77,17 @@ int ffhip_mesh_free(ffhip_mesh mesh);
 int ffhip_pattern_p1(ffhip_ctx ctx, ffhip_mesh mesh, ffhip_mat *mat);
 int ffhip_mat_info(ffhip_mat mat, int64_t *n, int64_t *nnz);
 
-enum { FFHIP_COEF_CONST = 0, FFHIP_COEF_SIN3 = 1, FFHIP_COEF_P0 = 2 };
+enum { FFHIP_COEF_CONST = 0, FFHIP_COEF_SIN3 = 1, FFHIP_COEF_P0 = 2, FFHIP_COEF_QP = 3 };
 /* Values of int3d(Th)(kappa*(dx(u)*dx(v)+dy(u)*dy(v)+dz(u)*dz(v))) with the
  * default Tet_5 rule (src/fflib/problem.cpp:14102-14124), replacing
  * AssembleBilinearForm 3D (problem.cpp:1116-1417) + Element_Op
  * (problem.cpp:6337-6437).  kappa: CONST -> c0; SIN3 ->
- * 1+0.5*sin(2 pi x)sin(2 pi y)sin(2 pi z); P0 -> kappa_e[element] (nt values,
- * host pointer, or device pointer when kappa_on_device != 0).
- * 2D meshes: int2d with T_5, terms dx,dy. */
+ * 1+0.5*sin(2 pi x)sin(2 pi y)sin(2 pi z); P0 -> kappa_e[element] (nt values);
+ * QP -> kappa_e[element * nq + q], the coefficient already evaluated at the
+ * nq quadrature points of each element (nq = 14 for Tet_5, 7 for T_5) — the
+ * values FreeFEM's interpreter produces for any scalar coefficient expression
+ * (problem.cpp:6392-6395).  kappa_e is a host pointer, or a device pointer
+ * when kappa_on_device != 0.  2D meshes: int2d with T_5, terms dx,dy. */
 int ffhip_assemble_p1(ffhip_ctx ctx, ffhip_mesh mesh, ffhip_mat mat, int kind,
                       double c0, const double *kappa_e, int kappa_on_device);
 /* on(labels..., u=g) with tgv > 0: AssembleBC (problem.cpp:10039-10194) then
@@ -108,6 +111,10 @@ int ffhip_mat_download(ffhip_mat mat, int64_t *rowptr, int32_t *colidx,
 int ffhip_mat_upload(ffhip_ctx ctx, int64_t n, int64_t nnz,
                      const int64_t *rowptr, const int32_t *colidx,
                      const double *vals, ffhip_mat *mat);
+/* New values for an existing pattern (host pointer, nnz doubles, CSR order):
+ * the numeric phase of a matrix whose HashMatrix values changed
+ * (HashMatrix::GetReDoNumerics, HashMatrix.hpp:278). */
+int ffhip_mat_set_values(ffhip_mat mat, const double *vals);
 /* Device pointers of the CSR arrays (owned by mat). */
 int ffhip_mat_device_ptrs(ffhip_mat mat, int64_t **rowptr, int32_t **colidx,
                           double **vals);
@@ -165,6 +172,10 @@ int ffhip_cg_set_profiling(ffhip_cg cg, int on);
  * library stream; x, y device vectors of n doubles (FP64) / floats (FP32). */
 int ffhip_bench_spmv(ffhip_cg cg, int mode, int reps, const void *x, void *y,
                      double *avg_ms);
+/* Average device time (ms) of `reps` launches of a plain streaming copy of
+ * nbytes (library-owned buffers): the achievable-bandwidth reference and the
+ * scale for hardware byte counters. */
+int ffhip_bench_copy(ffhip_ctx ctx, int64_t nbytes, int reps, double *avg_ms);
 /* Device time (ms) of the last pattern / assembly / bc / rhs call. */
 double ffhip_last_ms(ffhip_ctx ctx);
 

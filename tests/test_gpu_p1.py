@@ -243,3 +243,35 @@ def test_full_size_properties(n):
         assert st["converged"] == 1 and st["true_gcg"] < 1.0001e-20 * st["gcg0"]
         sols.append(x)
     assert (sols[0] - sols[1]).norm().item() <= 1e-8 * sols[0].norm().item()
+
+
+def test_qp_coefficient_matches_analytic():
+    """kappa handed over at the 14 Tet_5 points (the plugin's path for any
+    interpreted coefficient) gives the SIN3 matrix to rounding."""
+    n = 7
+    o = ffo.cube(n)
+    p = np.zeros(42); w = np.zeros(14)
+    ffo.lib().ffo_qf_tet5(p, w)
+    P = p.reshape(14, 3)
+    X = o["xyz"].reshape(-1, 3)[o["tets"].reshape(-1, 4)]            # nt x 4 x 3
+    lam = np.concatenate([1 - P.sum(1, keepdims=True), P], axis=1)    # 14 x 4
+    Q = np.einsum("qi,eik->eqk", lam, X)                              # nt x 14 x 3
+    kq = 1 + 0.5 * np.sin(2 * np.pi * Q[..., 0]) * np.sin(2 * np.pi * Q[..., 1]) * np.sin(2 * np.pi * Q[..., 2])
+    M = ffhip.cube(n)
+    A1 = ffhip.Matrix.p1(M)
+    A1.assemble_p1(ffhip.COEF_SIN3)
+    A2 = ffhip.Matrix.p1(M)
+    A2.assemble_p1(ffhip.COEF_QP, 0.0, np.ascontiguousarray(kq.ravel()))
+    v1, v2 = A1.download()[2], A2.download()[2]
+    assert np.max(np.abs(v1 - v2)) <= 1e-13 * np.max(np.abs(v1))
+
+
+def test_set_values_and_update():
+    m, rpo, cio, vo, onbco, bo = ffo.poisson_p1_3d(6)
+    A = ffhip.Matrix.upload(rpo, cio, vo)
+    cg = ffhip.CG(A)
+    x1, _ = cg.solve(bo, eps=1e-10)
+    A.set_values(2.0 * vo)   # tgv rows become 2e30, still the tgv of the matrix
+    cg.update_values()
+    x2, _ = cg.solve(bo, eps=1e-10)
+    assert np.linalg.norm(2.0 * x2 - x1) <= 1e-8 * np.linalg.norm(x1)

@@ -157,8 +157,9 @@ __global__ void k_square_bedges(int nx, int ny, const int32_t *lab4,
 }
 
 // plain streaming copy, 16 B per lane per step (measurement reference)
-__global__ __launch_bounds__(256) void k_copy(int64_t n16, const int4 *__restrict__ src,
-                                              int4 *__restrict__ dst) {
+typedef int v4i __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(256) void k_copy(int64_t n16, const v4i *__restrict__ src,
+                                              v4i *__restrict__ dst) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16;
        i += (int64_t)gridDim.x * blockDim.x)
     dst[i] = __builtin_nontemporal_load(src + i);
@@ -237,7 +238,7 @@ int ffhip_bench_copy(ffhip_ctx ctx, int64_t nbytes, int reps, double *avg_ms) {
   return guard([&] {
     FFHIP_REQUIRE(ctx && avg_ms && reps > 0 && nbytes >= 16, FFHIP_ERR_ARG, "ffhip_bench_copy: bad arguments");
     const int64_t n16 = nbytes / 16;
-    int4 *a = dalloc<int4>(n16), *b = dalloc<int4>(n16);
+    v4i *a = dalloc<v4i>(n16), *b = dalloc<v4i>(n16);
     FFHIP_CHECK(hipMemsetAsync(a, 0, 16 * n16, ctx->stream));
     const unsigned grid = 32 * (unsigned)ctx->num_cu;  // 8 blocks of 256 per SIMD group
     k_copy<<<grid, 256, 0, ctx->stream>>>(n16, a, b);

@@ -41,6 +41,19 @@ LABELS = (1, 2, 3, 4, 5, 6)
 HBM_PEAK_GBS = 8000.0
 
 
+def measured_traffic(kernel="k_pcg_dir"):
+    """HBM bytes per launch from the committed PMC summary (FETCH_SIZE and
+    WRITE_SIZE scaled by a copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "k_pcg_dir_cube160.md")
+    try:
+        for line in open(path):
+            if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+                return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(n=64, iters=100):
     """FreeFEM's compiled HashMatrix/CG (or the oracle port) on one core."""
     import numpy as np
@@ -197,7 +210,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic() if n == 160 else None,
             "kernel": "k_pcg_dir (FP64 SELL mat-vec + direction update + 2 dots)",
             "bytes_per_launch": bytes_mv,
             "avg_launch_ms": round(mv_ms, 5),

@@ -6,11 +6,13 @@ sin(2 pi z) at the 14 Tet_5 points, f = 1, u = 0 on labels 1..6 (tgv = 1e30),
 CG to eps = 1e-10 in FreeFEM's criterion (g, D^-1 g) < eps^2 (g0, D^-1 g0).
 
 One step = what `matrix A = va(Vh,Vh,solver=CG); b = vb(0,Vh); u[] = A^-1*b`
-does, twice solved:
+does, twice solved, plus one adjoint gradient evaluation:
   assembly   : CSR pattern + element values + Dirichlet rows + solver setup
   rhs        : b
   solve FP64 : plain FP64 Jacobi-PCG
   solve MIXED: FP32 matrix/inner CG with FP64 residual refinement
+  adjoint    : dJ/dkappa_e for J = 1/2 int (u-ud)^2 with lognormal kappa_e
+               (target ud solved with kappa = 1; forward, transposed, gradient)
 The mesh is built on the device before the timed region (inputs resident in
 HBM).  value = assembled Mnnz/s (nnz / assembly time); the CG times, the
 iteration counts and the accuracies are reported beside it.
@@ -152,11 +154,39 @@ def main():
         cg64.free(); cgm.free(); A.free()
         return out
 
+    # one adjoint dJ/dkappa_e evaluation with lognormal kappa_e (same mesh):
+    # ud solve (kappa = 1), forward solve, misfit, transposed solve, gradient
+    import ffhip.adjoint as adj
+    kappa_e = torch.exp(0.5 * torch.randn(mesh.nt, dtype=torch.float64, device="cuda",
+                                          generator=torch.Generator(device="cuda").manual_seed(20261202)))
+
+    def adjoint_eval():
+        t0 = time.perf_counter()
+        P = adj.P1Diffusion(n, eps=1e-10)
+        t_ud = time.perf_counter() - t0
+        kt = kappa_e.clone().requires_grad_(True)
+        t0 = time.perf_counter()
+        J = adj.objective(kt, P)
+        t_fwd = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        J.backward()
+        torch.cuda.synchronize()
+        t_bwd = time.perf_counter() - t0
+        out = dict(J=J.item(), t_setup_ud=t_ud, t_forward=t_fwd, t_adjoint_grad=t_bwd,
+                   iters_fwd=P.last["forward"]["iters"], iters_adj=P.last["adjoint"]["iters"],
+                   gnorm=kt.grad.norm().item())
+        P.cg.free(); P.A.free(); P.mesh.free()
+        return out
+
     for _ in range(args.warmup):
         step(False)
     torch.cuda.synchronize()
     T0 = time.perf_counter()
-    res = [step(True) for _ in range(args.steps)]
+    res = []
+    for _ in range(args.steps):
+        r = step(True)
+        r["adj"] = adjoint_eval()
+        res.append(r)
     torch.cuda.synchronize()
     T = time.perf_counter() - T0
 
@@ -204,6 +234,11 @@ def main():
         "final_gcg_over_gcg0": {"fp64": s64["true_gcg"] / s64["gcg0"], "mixed": sm["true_gcg"] / sm["gcg0"]},
         "true_relres": {"fp64": s64["true_relres"], "mixed": sm["true_relres"]},
         "mixed_relerr_vs_fp64": res[-1]["rel"],
+        "adjoint": {"J": res[-1]["adj"]["J"], "grad_norm": res[-1]["adj"]["gnorm"],
+                    "ud_solve_s": round(float(np.mean([r["adj"]["t_setup_ud"] for r in res])), 5),
+                    "forward_s": round(float(np.mean([r["adj"]["t_forward"] for r in res])), 5),
+                    "adjoint_and_gradient_s": round(float(np.mean([r["adj"]["t_adjoint_grad"] for r in res])), 5),
+                    "iters_forward": res[-1]["adj"]["iters_fwd"], "iters_adjoint": res[-1]["adj"]["iters_adj"]},
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 1),

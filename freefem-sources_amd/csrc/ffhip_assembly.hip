@@ -437,6 +437,75 @@ __global__ void k_rhs_p1_2d(int64_t n, const double *__restrict__ xy,
   b[v] = s;
 }
 
+// ---------------------------------------------------------------- adjoint blocks
+// per element: misfit 1/2 int_K e^2 = |K|/40 (sum e_i^2 + (sum e_i)^2), e = u - ud
+__global__ __launch_bounds__(256) void k_p1_misfit_elem(
+    int64_t nt, const double *__restrict__ xyz, const int32_t *__restrict__ tets,
+    const double *__restrict__ u, const double *__restrict__ ud, double *__restrict__ part) {
+  __shared__ double lds[256 / kWave];
+  double acc = 0.;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < nt; e += (int64_t)gridDim.x * 256) {
+    const int4 tv = reinterpret_cast<const int4 *>(tets)[e];
+    const int32_t id[4] = {tv.x, tv.y, tv.z, tv.w};
+    double v[4][3], ev[4];
+    for (int i = 0; i < 4; ++i) {
+      for (int k = 0; k < 3; ++k) v[i][k] = xyz[3 * (int64_t)id[i] + k];
+      ev[i] = u[id[i]] - ud[id[i]];
+    }
+    const double mes = tet_mesure(v);
+    const double s1 = ev[0] + ev[1] + ev[2] + ev[3];
+    const double s2 = ev[0] * ev[0] + ev[1] * ev[1] + ev[2] * ev[2] + ev[3] * ev[3];
+    acc += mes / 40. * (s2 + s1 * s1);
+  }
+  const double tot = block_sum<256>(acc, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = tot;
+}
+
+// (M e)_v = sum over incident elements of |K|/20 (e_v + sum_K e_j); 0 on Dirichlet rows
+__global__ void k_p1_mass_apply(int64_t n, const double *__restrict__ xyz,
+                                const int32_t *__restrict__ tets, const int64_t *__restrict__ v2e_ptr,
+                                const int32_t *__restrict__ v2e, const double *__restrict__ u,
+                                const double *__restrict__ ud, const uint8_t *__restrict__ onbc,
+                                double *__restrict__ r) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  if (onbc && onbc[v]) { r[v] = 0.; return; }
+  const double ev = u[v] - ud[v];
+  double s = 0.;
+  for (int64_t k = v2e_ptr[v]; k < v2e_ptr[v + 1]; ++k) {
+    const int64_t e = v2e[k] >> 2;
+    const int4 tv = reinterpret_cast<const int4 *>(tets)[e];
+    const int32_t id[4] = {tv.x, tv.y, tv.z, tv.w};
+    double vv[4][3], se = 0.;
+    for (int i = 0; i < 4; ++i) {
+      for (int kk = 0; kk < 3; ++kk) vv[i][kk] = xyz[3 * (int64_t)id[i] + kk];
+      se += u[id[i]] - ud[id[i]];
+    }
+    s += tet_mesure(vv) / 20. * (ev + se);
+  }
+  r[v] = s;
+}
+
+// g_e = -|K| grad u_K . grad lambda_K
+__global__ __launch_bounds__(256) void k_p1_grad_kappa(
+    int64_t nt, const double *__restrict__ xyz, const int32_t *__restrict__ tets,
+    const double *__restrict__ u, const double *__restrict__ lam, double *__restrict__ g) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nt) return;
+  const int4 tv = reinterpret_cast<const int4 *>(tets)[e];
+  const int32_t id[4] = {tv.x, tv.y, tv.z, tv.w};
+  double v[4][3], G[4][3];
+  for (int i = 0; i < 4; ++i)
+    for (int k = 0; k < 3; ++k) v[i][k] = xyz[3 * (int64_t)id[i] + k];
+  const double mes = tet_geom(v, G);
+  double gu[3] = {0., 0., 0.}, gl[3] = {0., 0., 0.};
+  for (int i = 0; i < 4; ++i) {
+    const double ui = u[id[i]], li = lam[id[i]];
+    for (int k = 0; k < 3; ++k) { gu[k] += ui * G[i][k]; gl[k] += li * G[i][k]; }
+  }
+  g[e] = -mes * (gu[0] * gl[0] + gu[1] * gl[1] + gu[2] * gl[2]);
+}
+
 // ---------------------------------------------------------------- CSR mat-vec
 __global__ void k_spmv_csr(int64_t n, const int64_t *__restrict__ rowptr,
                            const int32_t *__restrict__ colidx, const double *__restrict__ vals,
@@ -684,6 +753,61 @@ int ffhip_rhs_p1(ffhip_ctx ctx, ffhip_mesh m, double f, const int32_t *labels, i
     }
     dfree(onbc);
     dfree(labs);
+  });
+}
+
+int ffhip_p1_misfit(ffhip_ctx ctx, ffhip_mesh m, const double *u, const double *ud,
+                    const int32_t *labels, int nlab, double *J, double *r) {
+  return guard([&] {
+    FFHIP_REQUIRE(ctx && m && u && ud && m->dim == 3, FFHIP_ERR_ARG, "ffhip_p1_misfit: bad arguments");
+    hipStream_t s = ctx->stream;
+    build_v2e(m);
+    ctx->tic();
+    if (J) {
+      const int GB = 1024;
+      double *part = dalloc<double>(GB);
+      k_p1_misfit_elem<<<GB, 256, 0, s>>>(m->nt, m->xyz, m->elems, u, ud, part);
+      FFHIP_CHECK(hipGetLastError());
+      std::vector<double> h(GB);
+      FFHIP_CHECK(hipMemcpyAsync(h.data(), part, 8 * GB, hipMemcpyDeviceToHost, s));
+      FFHIP_CHECK(hipStreamSynchronize(s));
+      double tot = 0.;
+      for (int i = 0; i < GB; ++i) tot += h[i];
+      *J = tot;
+      dfree(part);
+    }
+    if (r) {
+      uint8_t *onbc = nullptr;
+      int32_t *labs = nullptr;
+      if (nlab > 0) {
+        onbc = dalloc<uint8_t>(m->nv);
+        labs = dalloc<int32_t>(nlab);
+        FFHIP_CHECK(hipMemcpyAsync(labs, labels, 4 * nlab, hipMemcpyHostToDevice, s));
+        FFHIP_CHECK(hipMemsetAsync(onbc, 0, m->nv, s));
+        if (m->nbe)
+          k_mark_bc<<<nblocks(m->nbe, 256), 256, 0, s>>>(m->nbe, m->nbloc, m->belems, m->blab, labs,
+                                                         nlab, onbc);
+      }
+      k_p1_mass_apply<<<nblocks(m->nv, 256), 256, 0, s>>>(m->nv, m->xyz, m->elems, m->v2e_ptr, m->v2e,
+                                                          u, ud, onbc, r);
+      FFHIP_CHECK(hipGetLastError());
+      FFHIP_CHECK(hipStreamSynchronize(s));
+      dfree(onbc);
+      dfree(labs);
+    }
+    ctx->toc();
+  });
+}
+
+int ffhip_p1_grad_kappa(ffhip_ctx ctx, ffhip_mesh m, const double *u, const double *lambda,
+                        double *g) {
+  return guard([&] {
+    FFHIP_REQUIRE(ctx && m && u && lambda && g && m->dim == 3, FFHIP_ERR_ARG,
+                  "ffhip_p1_grad_kappa: bad arguments");
+    ctx->tic();
+    k_p1_grad_kappa<<<nblocks(m->nt, 256), 256, 0, ctx->stream>>>(m->nt, m->xyz, m->elems, u, lambda, g);
+    FFHIP_CHECK(hipGetLastError());
+    ctx->toc();
   });
 }
 

@@ -86,6 +86,8 @@ _SIGS = {
     "ffhip_bench_spmv": (_i, [_vp, _i, _i, _vp, _vp, ctypes.POINTER(_d)]),
     "ffhip_last_ms": (_d, [_vp]),
     "ffhip_bench_copy": (_i, [_vp, _l, _i, ctypes.POINTER(_d)]),
+    "ffhip_p1_misfit": (_i, [_vp, _vp, _vp, _vp, _vp, _i, ctypes.POINTER(_d), _vp]),
+    "ffhip_p1_grad_kappa": (_i, [_vp, _vp, _vp, _vp, _vp]),
 }
 
 _LIB = None
@@ -374,3 +376,23 @@ class CG:
             self.free()
         except Exception:
             pass
+
+
+def p1_misfit(mesh, u, ud, labels=(), want_rhs=True):
+    """J = 1/2 int (u-ud)^2 and r = M (u-ud) with Dirichlet rows zeroed
+    (u, ud torch cuda tensors).  Returns (J, r)."""
+    import torch
+    lab = np.asarray(labels, np.int32)
+    J = _d()
+    r = torch.empty_like(u) if want_rhs else None
+    check(lib().ffhip_p1_misfit(mesh.ctx.h, mesh.h, _dp(u), _dp(ud), _hp(lab), lab.size,
+                                ctypes.byref(J), _dp(r)))
+    return J.value, r
+
+
+def p1_grad_kappa(mesh, u, lam, out=None):
+    """g_e = -|K| grad u_K . grad lambda_K (torch cuda tensors)"""
+    import torch
+    g = torch.empty(mesh.nt, dtype=torch.float64, device=u.device) if out is None else out
+    check(lib().ffhip_p1_grad_kappa(mesh.ctx.h, mesh.h, _dp(u), _dp(lam), _dp(g)))
+    return g

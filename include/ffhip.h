@@ -103,6 +103,23 @@ int ffhip_set_bc(ffhip_ctx ctx, ffhip_mesh mesh, ffhip_mat mat,
 int ffhip_rhs_p1(ffhip_ctx ctx, ffhip_mesh mesh, double f,
                  const int32_t *labels, int nlab, double tgv, double g,
                  double *b, int on_device);
+/* ---- adjoint building blocks (3D P1) ----------------------------------- */
+/* J = 1/2 int_Th (u - ud)^2 and the adjoint right-hand side
+ * r = M (u - ud) with the rows of the Dirichlet dofs (boundary labels listed)
+ * set to 0 — the mass varf int3d(Th)(u*v) of FreeFEM (Element_Op with op_id
+ * terms, problem.cpp:6384-6437; Tet_5 is exact for it) applied to u - ud, and
+ * the zeroed Dirichlet rows of the adjoint problem.  u, ud, r: device
+ * pointers (n doubles); J: host pointer. FreeFEM has no adjoint routine; this
+ * is the continuous adjoint of its composed varfs. */
+int ffhip_p1_misfit(ffhip_ctx ctx, ffhip_mesh mesh, const double *u,
+                    const double *ud, const int32_t *labels, int nlab,
+                    double *J, double *r);
+/* g[e] = dJ/dkappa_e = - int_K grad u . grad lambda  (= -|K| grad u_K . grad
+ * lambda_K for P1, exact), the P0 linear form int3d(Th)(-(grad u.grad l)*w)
+ * of the reference's composed varfs.  Gradients are recomputed from the
+ * coordinates, not stored.  u, lambda, g: device pointers. */
+int ffhip_p1_grad_kappa(ffhip_ctx ctx, ffhip_mesh mesh, const double *u,
+                        const double *lambda, double *g);
 /* CSR out (host): rowptr[n+1] (int64), colidx[nnz], vals[nnz]; any NULL skipped. */
 int ffhip_mat_download(ffhip_mat mat, int64_t *rowptr, int32_t *colidx,
                        double *vals);

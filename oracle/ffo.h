@@ -88,6 +88,22 @@ int ffo_rhs_p1_3d(int nv, int64_t nt, const double *xyz, const int32_t *tets,
 int ffo_rhs_p1_2d(int nv, int64_t nt, const double *xy, const int32_t *tris,
                   double f, double *b);
 
+/* ---- adjoint building blocks (3D P1) ---------------------------------- */
+/* mass varf int3d(Th)(u*v): Element_Op (problem.cpp:6384-6437) with op_id
+ * terms, P1 values l_i = barycentrics, Tet_5; accumulated like
+ * ffo_assemble_p1_3d. */
+int ffo_assemble_p1_mass_3d(int nv, int64_t nt, const double *xyz,
+                            const int32_t *tets, const int64_t *rowptr,
+                            const int32_t *colidx, double *vals);
+/* 1/2 int3d(Th)((u-ud)^2) with Tet_5 (the script expression's integral). */
+double ffo_p1_misfit_3d(int64_t nt, const double *xyz, const int32_t *tets,
+                        const double *u, const double *ud);
+/* P0 linear form int3d(Th)(-(dx(u)*dx(l)+dy(u)*dy(l)+dz(u)*dz(l))*w):
+ * Element_rhs (problem.cpp:7940-7978) with the P0 basis (w = 1 on K,
+ * P012_3d.cpp:46-80), one value per element. */
+int ffo_p0_grad_3d(int64_t nt, const double *xyz, const int32_t *tets,
+                   const double *u, const double *lam, double *g);
+
 /* ---- Dirichlet (tgv) ---------------------------------------------------- */
 /* AssembleBC (problem.cpp:10039-10194): dofs of boundary faces whose label is
  * in labs[] are marked (P1: the face vertices). */

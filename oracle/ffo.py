@@ -50,6 +50,9 @@ def lib():
             "ffo_rhs_p1_3d": (_i, [_i, _l, _f64p, _i32p, _d, _f64p]),
             "ffo_rhs_p1_2d": (_i, [_i, _l, _f64p, _i32p, _d, _f64p]),
             "ffo_mark_bc": (_i, [_i, _l, _i, _i32p, _i32p, _i, _i32p, _u8p]),
+            "ffo_assemble_p1_mass_3d": (_i, [_i, _l, _f64p, _i32p, _i64p, _i32p, _f64p]),
+            "ffo_p1_misfit_3d": (_d, [_l, _f64p, _i32p, _f64p, _f64p]),
+            "ffo_p0_grad_3d": (_i, [_l, _f64p, _i32p, _f64p, _f64p, _f64p]),
             "ffo_set_bc": (_i, [_i, _i64p, _i32p, _f64p, _u8p, _d]),
             "ffo_rhs_bc": (_i, [_i, _u8p, _d, _d, _f64p]),
             "ffo_spmv": (None, [_i, _i64p, _i32p, _f64p, _f64p, _f64p]),
@@ -223,3 +226,48 @@ def poisson_p1_3d(n, kind=COEF_SIN3, c0=1.0, kappa_e=None, labels=(1, 2, 3, 4, 5
     b = rhs_p1_3d(m, f)
     rhs_bc(b, onbc, tgv, 0.0)
     return m, rowptr, colidx, vals, onbc, b
+
+
+# ---------------------------------------------------------------- adjoint
+def mass_p1_3d(m, rowptr, colidx):
+    vals = np.zeros(colidx.size)
+    rc = lib().ffo_assemble_p1_mass_3d(m["nv"], m["nt"], m["xyz"], m["tets"], rowptr, colidx, vals)
+    assert rc == 0, rc
+    return vals
+
+
+def misfit_p1_3d(m, u, ud):
+    return lib().ffo_p1_misfit_3d(m["nt"], m["xyz"], m["tets"], np.ascontiguousarray(u, np.float64),
+                                  np.ascontiguousarray(ud, np.float64))
+
+
+def p0_grad_3d(m, u, lam):
+    g = np.zeros(m["nt"])
+    lib().ffo_p0_grad_3d(m["nt"], m["xyz"], m["tets"], np.ascontiguousarray(u, np.float64),
+                         np.ascontiguousarray(lam, np.float64), g)
+    return g
+
+
+def adjoint_p1_3d(n, kappa_e, eps=1e-12, labels=(1, 2, 3, 4, 5, 6)):
+    """J(kappa) = 1/2 int (u - ud)^2 and dJ/dkappa_e composed from FreeFEM's
+    varfs as a script would: forward solve with P0 kappa, ud from kappa = 1,
+    mass varf applied to u - ud with Dirichlet rows zeroed, transposed solve
+    (A'^-1, lgmat.cpp:3215-3220), and the P0 varf -int grad u . grad l."""
+    m = cube(n)
+    rp, ci = pattern(m["nv"], m["tets"], 4)
+    onbc = mark_bc(m["nv"], m["btri"], m["blab"], 3, labels)
+    b = rhs_p1_3d(m, 1.0)
+    rhs_bc(b, onbc)
+    v1 = assemble_p1_3d(m, rp, ci, COEF_CONST, 1.0)
+    set_bc(rp, ci, v1, onbc)
+    ud, _ = pcg(rp, ci, v1, b, eps=eps)
+    vk = assemble_p1_3d(m, rp, ci, COEF_P0, 1.0, kappa_e)
+    set_bc(rp, ci, vk, onbc)
+    u, _ = pcg(rp, ci, vk, b, eps=eps)
+    J = misfit_p1_3d(m, u, ud)
+    mv = mass_p1_3d(m, rp, ci)
+    r = spmv(rp, ci, mv, u - ud)
+    r[onbc == 1] = 0.0
+    lam, _ = pcg(rp, ci, vk, r, eps=eps, trans=True)
+    g = p0_grad_3d(m, u, lam)
+    return dict(mesh=m, u=u, ud=ud, J=J, lam=lam, g=g, rowptr=rp, colidx=ci, vals=vk, b=b, onbc=onbc)

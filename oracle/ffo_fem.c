@@ -390,3 +390,77 @@ int ffo_rhs_bc(int n, const uint8_t *onbc, double tgv, double g, double *b) {
     if (onbc[i]) b[i] = tgv * g;
   return 0;
 }
+
+/* ---- adjoint building blocks ------------------------------------------- */
+int ffo_assemble_p1_mass_3d(int nv, int64_t nt, const double *xyz,
+                            const int32_t *tets, const int64_t *rowptr,
+                            const int32_t *colidx, double *vals) {
+  double qp[42], qw[14];
+  ffo_qf_tet5(qp, qw);
+  memset(vals, 0, sizeof(double) * (size_t)rowptr[nv]);
+  for (int64_t e = 0; e < nt; ++e) {
+    const int32_t *t = tets + 4 * e;
+    double v[12], Ke[16];
+    for (int i = 0; i < 4; ++i)
+      for (int k = 0; k < 3; ++k) v[3 * i + k] = xyz[3 * (int64_t)t[i] + k];
+    const double mes = ffo_tet_mesure(v, v + 3, v + 6, v + 9);
+    for (int i = 0; i < 16; ++i) Ke[i] = 0.;
+    for (int q = 0; q < 14; ++q) {
+      const double coef = mes * qw[q];
+      const double *Ph = qp + 3 * q;
+      const double l[4] = {1. - (Ph[0] + Ph[1] + Ph[2]), Ph[0], Ph[1], Ph[2]};
+      double ccc = 1.;
+      ccc *= coef;
+      for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) Ke[4 * i + j] += ccc * l[i] * l[j];
+    }
+    for (int il = 0; il < 4; ++il)
+      for (int jl = 0; jl < 4; ++jl) {
+        int64_t s = find_slot(rowptr, colidx, t[il], t[jl]);
+        if (s < 0) return -3;
+        vals[s] += Ke[4 * il + jl];
+      }
+  }
+  return 0;
+}
+
+double ffo_p1_misfit_3d(int64_t nt, const double *xyz, const int32_t *tets,
+                        const double *u, const double *ud) {
+  double qp[42], qw[14], J = 0.;
+  ffo_qf_tet5(qp, qw);
+  for (int64_t e = 0; e < nt; ++e) {
+    const int32_t *t = tets + 4 * e;
+    const double mes = ffo_tet_mesure(xyz + 3 * (int64_t)t[0], xyz + 3 * (int64_t)t[1],
+                                      xyz + 3 * (int64_t)t[2], xyz + 3 * (int64_t)t[3]);
+    for (int q = 0; q < 14; ++q) {
+      const double *Ph = qp + 3 * q;
+      const double l[4] = {1. - (Ph[0] + Ph[1] + Ph[2]), Ph[0], Ph[1], Ph[2]};
+      double eq = 0.;
+      for (int i = 0; i < 4; ++i) eq += (u[t[i]] - ud[t[i]]) * l[i];
+      J += mes * qw[q] * (eq * eq);
+    }
+  }
+  return 0.5 * J;
+}
+
+int ffo_p0_grad_3d(int64_t nt, const double *xyz, const int32_t *tets,
+                   const double *u, const double *lam, double *g) {
+  double qp[42], qw[14];
+  ffo_qf_tet5(qp, qw);
+  for (int64_t e = 0; e < nt; ++e) {
+    const int32_t *t = tets + 4 * e;
+    double v[12], G[12];
+    for (int i = 0; i < 4; ++i)
+      for (int k = 0; k < 3; ++k) v[3 * i + k] = xyz[3 * (int64_t)t[i] + k];
+    const double mes = ffo_tet_mesure(v, v + 3, v + 6, v + 9);
+    ffo_tet_gradlambda(v, mes, G);
+    double gu[3] = {0, 0, 0}, gl[3] = {0, 0, 0};
+    for (int i = 0; i < 4; ++i)
+      for (int k = 0; k < 3; ++k) { gu[k] += u[t[i]] * G[3 * i + k]; gl[k] += lam[t[i]] * G[3 * i + k]; }
+    const double c = -(gu[0] * gl[0] + gu[1] * gl[1] + gu[2] * gl[2]);
+    double b = 0.;
+    for (int q = 0; q < 14; ++q) b += mes * qw[q] * c * 1.;
+    g[e] = b;
+  }
+  return 0;
+}

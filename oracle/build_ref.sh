@@ -1,7 +1,8 @@
 #!/bin/sh
 # Builds oracle/_ref/libffref_ref.so: FreeFEM's own femlib sources
-# (HashMatrix.cpp, CG.cpp, QuadratureFormular.cpp) and the no-graphics host
-# globals of src/Graphics/gggg.cpp, compiled where they lie
+# (HashMatrix, CG, QuadratureFormular, and the Mesh3 / FESpace3 / P1-P2
+# element sources), libMesh's libmeshb7.c (used by Mesh3dn.cpp) and the
+# no-graphics host globals of src/Graphics/gggg.cpp, compiled where they lie
 # under the reference tree, plus the test driver ref_driver.cpp.
 # -DCMAKE is the reference's own switch that skips the autoconf config.h
 # (src/Graphics/rgraph.hpp:55-57).  Flags follow the reference build:
@@ -15,11 +16,16 @@ OUT="$HERE/_ref"
 mkdir -p "$OUT"
 CXXFLAGS="-O3 -mavx2 -std=gnu++14 -fPIC -DCMAKE -DNCHECKPTR -w"
 INC="-I$REF/src/femlib -I$REF/src/Graphics -I$REF/src/fflib -I$REF/src/libMesh"
-for f in HashMatrix CG QuadratureFormular; do
-  g++ $CXXFLAGS $INC -c "$REF/src/femlib/$f.cpp" -o "$OUT/$f.o"
+FEMLIB="HashMatrix CG QuadratureFormular Mesh1dn Mesh2dn Mesh3dn MeshSn MeshLn GQuadTree FQuadTree \
+  FESpacen fem P012_1d P012_2d P012_3d P012_3dSurf P012_3dCurve mshptg splitsimplex"
+for f in $FEMLIB; do
+  g++ $CXXFLAGS $INC -c "$REF/src/femlib/$f.cpp" -o "$OUT/$f.o" &
 done
+gcc -O2 -fPIC -w -I"$REF/src/libMesh" -c "$REF/src/libMesh/libmeshb7.c" -o "$OUT/libmeshb7.o" &
+wait
 g++ $CXXFLAGS $INC -c "$REF/src/Graphics/gggg.cpp" -o "$OUT/gggg.o"
 g++ $CXXFLAGS $INC -c "$HERE/ref_driver.cpp" -o "$OUT/ref_driver.o"
-g++ -shared -o "$OUT/libffref_ref.so" "$OUT/ref_driver.o" "$OUT/HashMatrix.o" \
-  "$OUT/CG.o" "$OUT/QuadratureFormular.o" "$OUT/gggg.o"
+OBJS=""
+for f in $FEMLIB; do OBJS="$OBJS $OUT/$f.o"; done
+g++ -shared -o "$OUT/libffref_ref.so" "$OUT/ref_driver.o" $OBJS "$OUT/gggg.o" "$OUT/libmeshb7.o"
 echo "built $OUT/libffref_ref.so"

@@ -104,6 +104,38 @@ double ffo_p1_misfit_3d(int64_t nt, const double *xyz, const int32_t *tets,
 int ffo_p0_grad_3d(int64_t nt, const double *xyz, const int32_t *tets,
                    const double *u, const double *lam, double *g);
 
+/* ---- P2 and vector P2 (3D) --------------------------------------------- */
+/* P2 nodes by first appearance over the elements: 4 vertex keys then the 6
+ * edge keys (a,b) of Tet::nvedge {01,02,03,12,13,23} per element
+ * (GenericMesh::BuildDFNumbering, GenericMesh.hpp:1880-1920).  elem_nodes =
+ * nt x 10; returns the number of nodes. */
+int64_t ffo_p2_nodes(int nv, int64_t nt, const int32_t *tets, int32_t *elem_nodes);
+/* P2 basis at a reference point (P012_3d.cpp:199-270): f[10], df[10*3]
+ * given the 4 P1 gradients G[12]. */
+void ffo_p2_basis(const double *phat, const double *G, double *f, double *df);
+/* Lame element matrix of int3d(lambda*div(u)div(v) + 2mu eps(u):eps(v)) with
+ * Tet_5, vector [P2,P2,P2] with local dofs comp-blocked (c*10+i,
+ * FESpacen.cpp:403-428): Ke 30x30 row-major (row = test function). */
+int ffo_p2_elast_element(const double *xyz, const int32_t *tet, double lambda,
+                         double mu, double *Ke);
+/* element dofs of [P2,P2,P2]: dof(c*10+i) = 3*node_i + c (FESpacen.hpp:582) */
+int ffo_p2v_elem_dofs(int64_t nt, const int32_t *elem_nodes, int32_t *elem_dofs);
+/* assembly of the Lame matrix into a CSR pattern of the 30-dof elements */
+int ffo_assemble_p2_elast(int64_t nt, const double *xyz, const int32_t *tets,
+                          const int32_t *elem_dofs, double lambda, double mu,
+                          int64_t ndof, const int64_t *rowptr,
+                          const int32_t *colidx, double *vals);
+/* int3d(f . v), constant body force f[3] (Element_rhs, problem.cpp:7940-7978) */
+int ffo_rhs_p2_elast(int64_t nt, const double *xyz, const int32_t *tets,
+                     const int32_t *elem_dofs, const double *f, int64_t ndof,
+                     double *b);
+/* Dirichlet dofs of boundary triangles with listed labels: their 3 vertex
+ * nodes and 3 edge nodes, all components (AssembleBC, problem.cpp:10120-10164). */
+int ffo_p2v_mark_bc(int nv, int64_t nt, const int32_t *tets,
+                    const int32_t *elem_nodes, int64_t nbe, const int32_t *btri,
+                    const int32_t *blab, int nlab, const int32_t *labs,
+                    int64_t ndof, uint8_t *onbc);
+
 /* ---- Dirichlet (tgv) ---------------------------------------------------- */
 /* AssembleBC (problem.cpp:10039-10194): dofs of boundary faces whose label is
  * in labs[] are marked (P1: the face vertices). */

@@ -53,6 +53,13 @@ def lib():
             "ffo_assemble_p1_mass_3d": (_i, [_i, _l, _f64p, _i32p, _i64p, _i32p, _f64p]),
             "ffo_p1_misfit_3d": (_d, [_l, _f64p, _i32p, _f64p, _f64p]),
             "ffo_p0_grad_3d": (_i, [_l, _f64p, _i32p, _f64p, _f64p, _f64p]),
+            "ffo_p2_nodes": (_l, [_i, _l, _i32p, _i32p]),
+            "ffo_p2_basis": (None, [_f64p, _f64p, _f64p, _f64p]),
+            "ffo_p2_elast_element": (_i, [_f64p, _i32p, _d, _d, _f64p]),
+            "ffo_p2v_elem_dofs": (_i, [_l, _i32p, _i32p]),
+            "ffo_assemble_p2_elast": (_i, [_l, _f64p, _i32p, _i32p, _d, _d, _l, _i64p, _i32p, _f64p]),
+            "ffo_rhs_p2_elast": (_i, [_l, _f64p, _i32p, _i32p, _f64p, _l, _f64p]),
+            "ffo_p2v_mark_bc": (_i, [_i, _l, _i32p, _i32p, _l, _i32p, _i32p, _i, _i32p, _l, _u8p]),
             "ffo_set_bc": (_i, [_i, _i64p, _i32p, _f64p, _u8p, _d]),
             "ffo_rhs_bc": (_i, [_i, _u8p, _d, _d, _f64p]),
             "ffo_spmv": (None, [_i, _i64p, _i32p, _f64p, _f64p, _f64p]),
@@ -90,6 +97,8 @@ def ref():
             "ref_hm_matmul": (_i, [_vp, _f64p, _f64p, _i]),
             "ref_hm_csr": (_i, [_vp, _i64p, _i32p, _f64p]),
             "ref_hm_cg": (_i, [_vp, _f64p, _f64p, _d, _i, _d, _i, _f64p]),
+            "ref_fespace_dofs": (_l, [_i, _f64p, _l, _i32p, _i, _vp, ctypes.POINTER(_i)]),
+            "ref_p2_fb": (_i, [_f64p, _f64p, _f64p, _f64p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -271,3 +280,51 @@ def adjoint_p1_3d(n, kappa_e, eps=1e-12, labels=(1, 2, 3, 4, 5, 6)):
     lam, _ = pcg(rp, ci, vk, r, eps=eps, trans=True)
     g = p0_grad_3d(m, u, lam)
     return dict(mesh=m, u=u, ud=ud, J=J, lam=lam, g=g, rowptr=rp, colidx=ci, vals=vk, b=b, onbc=onbc)
+
+
+# ---------------------------------------------------------------- P2 elasticity
+def p2_nodes(m):
+    en = np.zeros(10 * m["nt"], np.int32)
+    nn = lib().ffo_p2_nodes(m["nv"], m["nt"], m["tets"], en)
+    return int(nn), en
+
+
+def p2v_dofs(m):
+    nn, en = p2_nodes(m)
+    ed = np.zeros(30 * m["nt"], np.int32)
+    lib().ffo_p2v_elem_dofs(m["nt"], en, ed)
+    return nn, en, ed
+
+
+def elasticity_p2(n, lam=1.0, mu=0.5, f=(0.0, 0.0, -1.0), labels=(4,), tgv=1e30):
+    """cube(n) [P2,P2,P2] Lame system as `matrix A = va(Vh,Vh)` builds it:
+    pattern, values, tgv rows (clamped labels), body-force right-hand side."""
+    m = cube(n)
+    nn, en, ed = p2v_dofs(m)
+    ndof = 3 * nn
+    rp, ci = pattern(ndof, ed, 30)
+    vals = np.zeros(ci.size)
+    rc = lib().ffo_assemble_p2_elast(m["nt"], m["xyz"], m["tets"], ed, lam, mu, ndof, rp, ci, vals)
+    assert rc == 0, rc
+    onbc = np.zeros(ndof, np.uint8)
+    labs = np.asarray(labels, np.int32)
+    rc = lib().ffo_p2v_mark_bc(m["nv"], m["nt"], m["tets"], en, m["nbe"], m["btri"], m["blab"],
+                              labs.size, labs, ndof, onbc)
+    assert rc == 0, rc
+    set_bc(rp, ci, vals, onbc, tgv)
+    b = np.zeros(ndof)
+    lib().ffo_rhs_p2_elast(m["nt"], m["xyz"], m["tets"], ed, np.asarray(f, np.float64), ndof, b)
+    rhs_bc(b, onbc, tgv, 0.0)
+    return dict(mesh=m, nnodes=nn, elem_nodes=en, elem_dofs=ed, ndof=ndof, rowptr=rp, colidx=ci,
+                vals=vals, onbc=onbc, b=b)
+
+
+def ref_fespace_dofs(m, kind):
+    """element dofs of FreeFEM's own FESpace3 (kind 1 P1, 2 P2, 3 [P2,P2,P2])"""
+    R = ref()
+    nd = ctypes.c_int()
+    ndf = R.ref_fespace_dofs(m["nv"], m["xyz"], m["nt"], m["tets"], kind, None, ctypes.byref(nd))
+    ed = np.zeros(m["nt"] * nd.value, np.int32)
+    R.ref_fespace_dofs(m["nv"], m["xyz"], m["nt"], m["tets"], kind, ed.ctypes.data_as(ctypes.c_void_p),
+                       ctypes.byref(nd))
+    return int(ndf), ed

@@ -8,8 +8,10 @@
 // dependency of libffhip.
 //
 // The reference pieces linked in: src/femlib/HashMatrix.cpp, CG.cpp,
-// QuadratureFormular.cpp (compiled from /root/reference with -DCMAKE, the
-// reference's own switch that skips the autoconf config.h).  What the driver
+// QuadratureFormular.cpp, and the mesh / finite-element sources that build a
+// Mesh3 and its FESpace3 (Mesh*dn.cpp, FESpacen.cpp, P012_*.cpp, ...; the
+// list is in oracle/build_ref.sh), compiled from /root/reference with
+// -DCMAKE, the reference's own switch that skips the autoconf config.h.  What the driver
 // restates itself (the interpreter-bound parts that cannot be compiled alone):
 // the Element_Op quadrature loop (problem.cpp:6384-6437) and the two tiny
 // CGMatVirt adaptors of VirtualSolverCG.hpp (HMatVirt 148-153, the diag1 branch
@@ -24,6 +26,7 @@
 #include "HashMatrix.hpp"
 #include "Mesh3dn.hpp"
 #include "QuadratureFormular.hpp"
+#include "FESpacen.hpp"
 
 // The host-program globals (verbosity, mpirank, ShowDebugStack) come from the
 // reference's src/Graphics/gggg.cpp, linked in.
@@ -209,6 +212,70 @@ int ref_hm_cg(void *h, const double *b, double *x, double eps, int itmax,
   int res = ConjugueGradient<int, double>(AA, C, bb.data(), x, itermx, epss, 0);
   if (stats) { stats[0] = itermx; stats[1] = epss; stats[3] = res; }
   return res;
+}
+
+// Mesh3 (Mesh3dn.cpp:1416) from arrays, owned by the returned object
+static Mesh3 *make_mesh3(int nv, const double *xyz, int64_t nt, const int32_t *tets) {
+  verbosity = 0;
+  Vertex3 *v = new Vertex3[nv];
+  for (int i = 0; i < nv; ++i) {
+    v[i].x = xyz[3 * i]; v[i].y = xyz[3 * i + 1]; v[i].z = xyz[3 * i + 2]; v[i].lab = 0;
+  }
+  Tet *t = new Tet[nt];
+  for (int64_t k = 0; k < nt; ++k) {
+    int iv[4] = {tets[4 * k], tets[4 * k + 1], tets[4 * k + 2], tets[4 * k + 3]};
+    t[k].set(v, iv, 0);
+  }
+  return new Mesh3(nv, (int)nt, 0, v, t, new Triangle3[0]);
+}
+
+// Global dofs of every element of an FESpace3 built by FreeFEM's own
+// BuildDFNumbering (GenericMesh.hpp:1711-1954): kind 1 = P1, 2 = P2,
+// 3 = [P2,P2,P2] (GTypeOfFESum, FESpacen.cpp:403-428).  Returns NbOfDF.
+int64_t ref_fespace_dofs(int nv, const double *xyz, int64_t nt, const int32_t *tets, int kind,
+                         int32_t *elem_dofs, int *ndofloc) {
+  Mesh3 *Th = make_mesh3(nv, xyz, nt, tets);
+  FESpace3 *Vh = nullptr;
+  GTypeOfFESum<Mesh3> *sum = nullptr;
+  if (kind == 1) Vh = new FESpace3(*Th, DataFE<Mesh3>::P1);
+  else if (kind == 2) Vh = new FESpace3(*Th, DataFE<Mesh3>::P2);
+  else {
+    KN<GTypeOfFE<Mesh3> const *> t(3);
+    t[0] = t[1] = t[2] = &DataFE<Mesh3>::P2;
+    sum = new GTypeOfFESum<Mesh3>(t);
+    Vh = new FESpace3(*Th, *sum);
+  }
+  const int nd = (*Vh)[0].NbDoF();
+  *ndofloc = nd;
+  if (elem_dofs)
+    for (int64_t k = 0; k < nt; ++k) {
+      FElement3 K((*Vh)[(int)k]);
+      for (int i = 0; i < nd; ++i) elem_dofs[k * nd + i] = K(i);
+    }
+  const int64_t ndf = Vh->NbOfDF;
+  delete Vh;  // releases its counted reference to the mesh
+  delete sum;
+  Th->destroy();  // RefCounter (RefCounter.hpp:55-66)
+  return ndf;
+}
+
+// P2 basis values and gradients on one tet at a reference point
+// (TypeOfFE_P2Lagrange3d::FB, P012_3d.cpp:199-270): f[10], df[10*3]
+int ref_p2_fb(const double *v12, const double *phat, double *f, double *df) {
+  const int32_t tet[4] = {0, 1, 2, 3};
+  Mesh3 *Th = make_mesh3(4, v12, 1, tet);
+  {
+    FESpace3 Vh(*Th, DataFE<Mesh3>::P2);
+    FElement3 K(Vh[0]);
+    KNMK<double> val(10, 1, last_operatortype);  // FB zeroes every operator slot
+    K.BF(Fop_D0 | Fop_D1, R3(phat[0], phat[1], phat[2]), val);
+    for (int i = 0; i < 10; ++i) {
+      f[i] = val(i, 0, op_id);
+      df[3 * i] = val(i, 0, op_dx); df[3 * i + 1] = val(i, 0, op_dy); df[3 * i + 2] = val(i, 0, op_dz);
+    }
+  }
+  Th->destroy();
+  return 0;
 }
 
 }  // extern "C"

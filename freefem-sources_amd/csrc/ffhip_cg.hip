@@ -54,6 +54,11 @@ constexpr int kWavesPerBlock = kBlock / kSlice;
 #define FFHIP_MATVEC_ATTR __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(FFHIP_WPE, FFHIP_WPE)))
 constexpr int kU = FFHIP_SELL_U;
 
+// matrix layouts of the mat-vec kernels (described at MatFmt below)
+constexpr int kSell = 0;
+constexpr int kCsrV = 1;
+constexpr int kLanesPerRow = 16;
+
 struct CgState {
   double gcg_prev[2];  // (g, Cg) at the start of the iteration, by parity
   double gcg0, eps2;
@@ -169,6 +174,23 @@ __global__ void k_sell_fill(int64_t n, int64_t S, const int64_t *__restrict__ ro
   }
 }
 
+// CSR-layout values: A (optionally transposed through tmap) and/or the FP32
+// copy of D^-1/2 A D^-1/2, one 16-lane group per row
+__global__ void k_csr_fill(int64_t n, const int64_t *__restrict__ rowptr,
+                           const int32_t *__restrict__ colidx, const double *__restrict__ vals,
+                           const int64_t *__restrict__ tmap, const double *__restrict__ dsc,
+                           double *__restrict__ dv, float *__restrict__ fv) {
+  const int64_t r = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kLanesPerRow;
+  const int lane = threadIdx.x % kLanesPerRow;
+  if (r >= n) return;
+  const int64_t k1 = rowptr[r + 1];
+  for (int64_t k = rowptr[r] + lane; k < k1; k += kLanesPerRow) {
+    const double a = tmap ? vals[tmap[k]] : vals[k];
+    if (dv) dv[k] = a;
+    if (fv) fv[k] = (float)(a * dsc[r] * dsc[colidx[k]]);
+  }
+}
+
 // ---------------------------------------------------------------- Jacobi / tgv
 __global__ void k_diag_vals(int64_t n, const int64_t *__restrict__ diag,
                             const double *__restrict__ vals, double *__restrict__ d,
@@ -222,41 +244,96 @@ __global__ void k_init_bc(int64_t n, const uint8_t *__restrict__ wcl, const doub
   if (i < n && wcl[i]) x[i] = b[i] / tgv;
 }
 
-// ---------------------------------------------------------------- FP64 PCG
-// G = A x - b, CG = D^-1 G, H0 = 0, partial (G, CG)
-__global__ FFHIP_MATVEC_ATTR void k_pcg_init(
-    int64_t n, int64_t S, const int64_t *__restrict__ sptr, const int32_t *__restrict__ slen,
-    const int32_t *__restrict__ scol, const double *__restrict__ sval,
-    const double *__restrict__ x, const double *__restrict__ b, const double *__restrict__ dinv,
-    double *__restrict__ G, double *__restrict__ CG, double *__restrict__ H0,
-    double *__restrict__ pout, CgState *st) {
-  __shared__ double lds[kBlock / kWave];
-  int64_t s0, s1;
-  block_slices(S, s0, s1);
-  const int w = wave_id(), lane = threadIdx.x % kSlice;
-  double part = 0.;
-  for (int64_t s = s0 + w; s < s1; s += kWavesPerBlock) {
-    const int64_t base = uniform(sptr[s]) + lane;
-    const int L = __builtin_amdgcn_readfirstlane(slen[s]);
-    const double acc = sell_row<kU, double, double>(scol + base, sval + base, L,
-                                                   [&](int32_t j) { return x[j]; });
-    const int64_t r = s * kSlice + lane;
-    if (r < n) {
-      const double g = acc + -1. * b[r];
-      const double c = dinv[r] * g;
-      G[r] = g; CG[r] = c; H0[r] = 0.;
-      part += g * c;
+// ---------------------------------------------------------------- matrix formats
+// Every mat-vec kernel streams the matrix through one of two layouts:
+//   kSell: sliced ELL, 64-row slices (one lane per row, no reduction) — for
+//          matrices whose rows have nearly equal length (P1);
+//   kCsrV: CSR with 16 lanes per row and a shuffle reduction — for rows of
+//          very different lengths (vector P2), where sliced ELL would pad.
+template <class V>
+struct MatFmt {
+  int64_t n, S;
+  const int64_t *ptr;  // kSell: slice offsets (S+1); kCsrV: row pointers (n+1)
+  const int32_t *len;  // kSell: slice widths
+  const int32_t *col;
+  const V *val;
+};
+
+// Calls epi(r, sum_k a_rk X(j_k)) once for every row r of this block's range;
+// for kCsrV only the first lane of each 16-lane group calls it.
+template <int KIND, class V, class ACC, class F, class E>
+__device__ inline void matvec_rows(const MatFmt<V> &M, F xval, E epi) {
+  if (KIND == kSell) {
+    int64_t s0, s1;
+    block_slices(M.S, s0, s1);
+    const int w = wave_id(), lane = threadIdx.x % kSlice;
+    for (int64_t s = s0 + w; s < s1; s += kWavesPerBlock) {
+      const int64_t base = uniform(M.ptr[s]) + lane;
+      const int L = __builtin_amdgcn_readfirstlane(M.len[s]);
+      const ACC acc = sell_row<kU, V, ACC>(M.col + base, M.val + base, L, xval);
+      const int64_t r = s * kSlice + lane;
+      if (r < M.n) epi(r, acc);
+    }
+  } else {
+    const int64_t G = gridDim.x, lb = xcd_block_id();
+    const int64_t r0 = (lb * M.n) / G, r1 = ((lb + 1) * M.n) / G;
+    const int g = threadIdx.x / kLanesPerRow, lane = threadIdx.x % kLanesPerRow;
+    constexpr int NG = kBlock / kLanesPerRow;
+    for (int64_t rb = r0; rb < r1; rb += NG) {
+      const int64_t r = rb + g;
+      ACC acc = 0;
+      if (r < r1) {
+        const int64_t k0 = M.ptr[r], k1 = M.ptr[r + 1];
+        for (int64_t k = k0 + lane; k < k1; k += 4 * kLanesPerRow) {
+          int32_t j[4];
+          V a[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int64_t kk = k + u * kLanesPerRow;
+            j[u] = 0;
+            a[u] = 0;
+            if (kk < k1) { j[u] = ntload(M.col + kk); a[u] = ntload(M.val + kk); }
+          }
+          ACC xv[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) xv[u] = xval(j[u]);
+#pragma unroll
+          for (int u = 0; u < 4; ++u) acc += (ACC)a[u] * xv[u];
+        }
+      }
+#pragma unroll
+      for (int off = kLanesPerRow / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kWave);
+      if (r < r1 && lane == 0) epi(r, acc);
     }
   }
+}
+
+// ---------------------------------------------------------------- FP64 PCG
+// G = A x - b, CG = D^-1 G, H0 = 0, partial (G, CG)
+template <int KIND>
+__global__ FFHIP_MATVEC_ATTR void k_pcg_init(MatFmt<double> M, const double *__restrict__ x,
+                                             const double *__restrict__ b,
+                                             const double *__restrict__ dinv, double *__restrict__ G,
+                                             double *__restrict__ CG, double *__restrict__ H0,
+                                             double *__restrict__ pout, CgState *st) {
+  __shared__ double lds[kBlock / kWave];
+  double part = 0.;
+  matvec_rows<KIND, double, double>(
+      M, [&](int32_t j) { return x[j]; },
+      [&](int64_t r, double acc) {
+        const double g = acc + -1. * b[r];
+        const double c = dinv[r] * g;
+        G[r] = g; CG[r] = c; H0[r] = 0.;
+        part += g * c;
+      });
   const double tot = block_sum<kBlock>(part, lds);
   if (threadIdx.x == 0) pout[xcd_block_id()] = tot;
   if (blockIdx.x == 0 && threadIdx.x == 0) { st->done = 0; st->iters = 0; }
 }
 
+template <int KIND>
 __global__ FFHIP_MATVEC_ATTR void k_pcg_dir(
-    int it, int itmax, double eps, int64_t n, int64_t S, const int64_t *__restrict__ sptr,
-    const int32_t *__restrict__ slen, const int32_t *__restrict__ scol,
-    const double *__restrict__ sval, const double *__restrict__ Hold,
+    int it, int itmax, double eps, MatFmt<double> M, const double *__restrict__ Hold,
     const double *__restrict__ CG, const double *__restrict__ G, double *__restrict__ Hnew,
     double *__restrict__ AH, const double *__restrict__ pin, double *__restrict__ pout,
     CgState *st) {
@@ -290,24 +367,16 @@ __global__ FFHIP_MATVEC_ATTR void k_pcg_dir(
     }
     if (done) return;
   }
-  int64_t s0, s1;
-  block_slices(S, s0, s1);
-  const int w = wave_id(), lane = threadIdx.x % kSlice;
   double p_hah = 0., p_gh = 0.;
-  for (int64_t s = s0 + w; s < s1; s += kWavesPerBlock) {
-    const int64_t base = uniform(sptr[s]) + lane;
-    const int L = __builtin_amdgcn_readfirstlane(slen[s]);
-    const double acc = sell_row<kU, double, double>(
-        scol + base, sval + base, L, [&](int32_t j) { return gamma * Hold[j] - CG[j]; });
-    const int64_t r = s * kSlice + lane;
-    if (r < n) {
-      const double h = gamma * Hold[r] - CG[r];
-      Hnew[r] = h;
-      AH[r] = acc;
-      p_hah += h * acc;
-      p_gh += G[r] * h;
-    }
-  }
+  matvec_rows<KIND, double, double>(
+      M, [&](int32_t j) { return gamma * Hold[j] - CG[j]; },
+      [&](int64_t r, double acc) {
+        const double h = gamma * Hold[r] - CG[r];
+        Hnew[r] = h;
+        AH[r] = acc;
+        p_hah += h * acc;
+        p_gh += G[r] * h;
+      });
   const double a = block_sum<kBlock>(p_hah, lds);
   const double c = block_sum<kBlock>(p_gh, lds);
   if (threadIdx.x == 0) {
@@ -317,8 +386,15 @@ __global__ FFHIP_MATVEC_ATTR void k_pcg_dir(
   }
 }
 
+// this block's row range for the row-wise vector kernels
+__device__ inline void block_rows(int64_t n, int64_t &r0, int64_t &r1) {
+  const int64_t G = gridDim.x, lb = xcd_block_id();
+  r0 = (lb * n) / G;
+  r1 = ((lb + 1) * n) / G;
+}
+
 __global__ __launch_bounds__(kBlock) void k_pcg_update(
-    int64_t n, int64_t S, const double *__restrict__ H, const double *__restrict__ AH,
+    int64_t n, const double *__restrict__ H, const double *__restrict__ AH,
     const double *__restrict__ dinv, double *__restrict__ x, double *__restrict__ G,
     double *__restrict__ CG, const double *__restrict__ pin, double *__restrict__ pout,
     CgState *st) {
@@ -328,10 +404,9 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(
   const double hah = reduce_partials(pin, P, 2, 0, lds);
   const double gh = reduce_partials(pin, P, 2, 1, lds);
   const double rho = -gh / hah;
-  int64_t s0, s1;
-  block_slices(S, s0, s1);
+  int64_t r0, r1;
+  block_rows(n, r0, r1);
   double part = 0.;
-  const int64_t r0 = s0 * kSlice, r1 = (s1 * kSlice < n) ? s1 * kSlice : n;
   for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) {
     x[r] += rho * H[r];
     const double g = G[r] + rho * AH[r];
@@ -346,30 +421,22 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(
 
 // ---------------------------------------------------------------- residual check
 // r = b - A x (FP64); partials (r,r), (b,b), (r, D^-1 r)
-__global__ FFHIP_MATVEC_ATTR void k_residual(
-    int64_t n, int64_t S, const int64_t *__restrict__ sptr, const int32_t *__restrict__ slen,
-    const int32_t *__restrict__ scol, const double *__restrict__ sval,
-    const double *__restrict__ x, const double *__restrict__ b, const double *__restrict__ dinv,
-    double *__restrict__ rout, double *__restrict__ pout) {
+template <int KIND>
+__global__ FFHIP_MATVEC_ATTR void k_residual(MatFmt<double> M, const double *__restrict__ x,
+                                             const double *__restrict__ b,
+                                             const double *__restrict__ dinv,
+                                             double *__restrict__ rout, double *__restrict__ pout) {
   __shared__ double lds[kBlock / kWave];
-  int64_t s0, s1;
-  block_slices(S, s0, s1);
-  const int w = wave_id(), lane = threadIdx.x % kSlice;
   double prr = 0., pbb = 0., prc = 0.;
-  for (int64_t s = s0 + w; s < s1; s += kWavesPerBlock) {
-    const int64_t base = uniform(sptr[s]) + lane;
-    const int L = __builtin_amdgcn_readfirstlane(slen[s]);
-    const double acc = sell_row<kU, double, double>(scol + base, sval + base, L,
-                                                   [&](int32_t j) { return x[j]; });
-    const int64_t r = s * kSlice + lane;
-    if (r < n) {
-      const double rr = b[r] - acc;
-      if (rout) rout[r] = rr;
-      prr += rr * rr;
-      pbb += b[r] * b[r];
-      prc += rr * (dinv[r] * rr);
-    }
-  }
+  matvec_rows<KIND, double, double>(
+      M, [&](int32_t j) { return x[j]; },
+      [&](int64_t r, double acc) {
+        const double rr = b[r] - acc;
+        if (rout) rout[r] = rr;
+        prr += rr * rr;
+        pbb += b[r] * b[r];
+        prc += rr * (dinv[r] * rr);
+      });
   const double a = block_sum<kBlock>(prr, lds);
   const double c = block_sum<kBlock>(pbb, lds);
   const double e = block_sum<kBlock>(prc, lds);
@@ -379,24 +446,11 @@ __global__ FFHIP_MATVEC_ATTR void k_residual(
   }
 }
 
-// plain SELL mat-vec for measurement: y = A x
-template <class V, class X>
-__global__ FFHIP_MATVEC_ATTR void k_sell_spmv(
-    int64_t n, int64_t S, const int64_t *__restrict__ sptr, const int32_t *__restrict__ slen,
-    const int32_t *__restrict__ scol, const V *__restrict__ sval, const X *__restrict__ x,
-    X *__restrict__ y) {
-  int64_t s0, s1;
-  block_slices(S, s0, s1);
-  const int w = wave_id(), lane = threadIdx.x % kSlice;
-  for (int64_t s = s0 + w; s < s1; s += kWavesPerBlock) {
-    const int64_t base = uniform(sptr[s]) + lane;
-    const int L = __builtin_amdgcn_readfirstlane(slen[s]);
-    const X acc = sell_row<kU, V, X>(scol + base, sval + base, L, [&](int32_t j) { return x[j]; });
-    const int64_t r = s * kSlice + lane;
-    if (r < n) y[r] = acc;
-  }
+// plain mat-vec for measurement: y = A x
+template <int KIND, class V, class X>
+__global__ FFHIP_MATVEC_ATTR void k_matvec(MatFmt<V> M, const X *__restrict__ x, X *__restrict__ y) {
+  matvec_rows<KIND, V, X>(M, [&](int32_t j) { return x[j]; }, [&](int64_t r, X acc) { y[r] = acc; });
 }
-
 
 // ---------------------------------------------------------------- MIXED (FP32 inner)
 // Mixed-precision state lives in CgState: sigma (scale of r32), rho32 (last
@@ -411,10 +465,9 @@ struct MixState {
 // K_m1: p = r32 + cb*p_old (on the fly for every gathered column); q = Ahat p;
 // partial (p, q).  Requests a reliable update when (r32, r32) fell by 1e-4
 // since the last update, or the recurrence says the FP64 criterion is met.
+template <int KIND>
 __global__ FFHIP_MATVEC_ATTR void k_mix_dir(
-    int it, int itmax, int64_t n, int64_t S, const int64_t *__restrict__ sptr,
-    const int32_t *__restrict__ slen, const int32_t *__restrict__ scol,
-    const float *__restrict__ sval, const float *__restrict__ pold,
+    int it, int itmax, MatFmt<float> M, const float *__restrict__ pold,
     const float *__restrict__ r32, float *__restrict__ pnew, float *__restrict__ q,
     const double *__restrict__ pin, double *__restrict__ pout, CgState *st, MixState *ms) {
   __shared__ double lds[kBlock / kWave];
@@ -442,30 +495,22 @@ __global__ FFHIP_MATVEC_ATTR void k_mix_dir(
   const double cb = (it == ms->first_it) ? ms->cbeta : rho / ms->rho_prev[(it - 1) & 1];
   if (lead) { ms->rho_prev[it & 1] = rho; st->iters = it - 1; }
   const float cbf = (float)cb;
-  int64_t s0, s1;
-  block_slices(S, s0, s1);
-  const int w = wave_id(), lane = threadIdx.x % kSlice;
   double part = 0.;
-  for (int64_t s = s0 + w; s < s1; s += kWavesPerBlock) {
-    const int64_t base = uniform(sptr[s]) + lane;
-    const int L = __builtin_amdgcn_readfirstlane(slen[s]);
-    const float acc = sell_row<kU, float, float>(scol + base, sval + base, L,
-                                                [&](int32_t j) { return r32[j] + cbf * pold[j]; });
-    const int64_t r = s * kSlice + lane;
-    if (r < n) {
-      const float p = r32[r] + cbf * pold[r];
-      pnew[r] = p;
-      q[r] = acc;
-      part += (double)p * (double)acc;
-    }
-  }
+  matvec_rows<KIND, float, float>(
+      M, [&](int32_t j) { return r32[j] + cbf * pold[j]; },
+      [&](int64_t r, float acc) {
+        const float p = r32[r] + cbf * pold[r];
+        pnew[r] = p;
+        q[r] = acc;
+        part += (double)p * (double)acc;
+      });
   const double tot = block_sum<kBlock>(part, lds);
   if (threadIdx.x == 0) pout[xcd_block_id()] = tot;
 }
 
 // K_m2: alpha = rho/(p,q); y32 += alpha p; r32 -= alpha q; partial (r32, r32)
 __global__ __launch_bounds__(kBlock) void k_mix_update(
-    int it, int64_t n, int64_t S, const float *__restrict__ p, const float *__restrict__ q,
+    int it, int64_t n, const float *__restrict__ p, const float *__restrict__ q,
     float *__restrict__ y32, float *__restrict__ r32, const double *__restrict__ pin,
     double *__restrict__ pout, CgState *st, MixState *ms) {
   __shared__ double lds[kBlock / kWave];
@@ -475,10 +520,9 @@ __global__ __launch_bounds__(kBlock) void k_mix_update(
   const double alpha = ms->rho_prev[it & 1] / pq;
   const float af = (float)alpha;
   if (blockIdx.x == 0 && threadIdx.x == 0) st->iters = it;
-  int64_t s0, s1;
-  block_slices(S, s0, s1);
+  int64_t r0, r1;
+  block_rows(n, r0, r1);
   double part = 0.;
-  const int64_t r0 = s0 * kSlice, r1 = (s1 * kSlice < n) ? s1 * kSlice : n;
   for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) {
     y32[r] += af * p[r];
     const float rr = r32[r] - af * q[r];
@@ -500,13 +544,12 @@ __global__ void k_mix_fold(int64_t n, double sigma, const double *__restrict__ d
 
 // new scaled residual r32 = D^-1/2 r / sigma (r = b - A x in FP64); partial (r32,r32)
 __global__ __launch_bounds__(kBlock) void k_mix_rescale(
-    int64_t n, int64_t S, double inv_sigma, const double *__restrict__ dsc,
+    int64_t n, double inv_sigma, const double *__restrict__ dsc,
     const double *__restrict__ r, float *__restrict__ r32, double *__restrict__ pout) {
   __shared__ double lds[kBlock / kWave];
-  int64_t s0, s1;
-  block_slices(S, s0, s1);
+  int64_t r0, r1;
+  block_rows(n, r0, r1);
   double part = 0.;
-  const int64_t r0 = s0 * kSlice, r1 = (s1 * kSlice < n) ? s1 * kSlice : n;
   for (int64_t i = r0 + threadIdx.x; i < r1; i += kBlock) {
     const float v = (float)(dsc[i] * r[i] * inv_sigma);
     r32[i] = v;
@@ -526,11 +569,13 @@ struct ffhip_cg_s {
   ffhip_ctx ctx = nullptr;
   ffhip_mat A = nullptr;
   int mode = 0;
+  int fmt = kSell;
   int64_t n = 0, S = 0, ents = 0;
   int grid = 0;
-  // SELL
+  // kSell layout (kCsrV streams A's own rowptr/colidx)
   int64_t *sptr = nullptr;
   int32_t *slen = nullptr, *scol = nullptr;
+  // values in the solver's layout; for kCsrV, sval is A's value array (borrowed)
   double *sval = nullptr;    // A
   double *svalT = nullptr;   // A^T (built on first transposed solve)
   float *sval32 = nullptr;   // D^-1/2 A D^-1/2 (MIXED)
@@ -574,9 +619,68 @@ struct ffhip_cg_s {
       t_mv += a; t_up += b; ++n_timed;
     }
   }
+  template <class V>
+  MatFmt<V> fmt_of(const V *val) const {
+    MatFmt<V> M;
+    M.n = n;
+    M.S = S;
+    if (fmt == kSell) { M.ptr = sptr; M.len = slen; M.col = scol; }
+    else { M.ptr = A->rowptr; M.len = nullptr; M.col = A->colidx; }
+    M.val = val;
+    return M;
+  }
 };
 
-static void cg_build_sell(ffhip_cg cg, bool transposed) {
+// launch a layout-templated mat-vec kernel with the solver's layout
+#define FFHIP_LAUNCH_FMT(cg, kern, grid, stream, ...)                        \
+  do {                                                                       \
+    if ((cg)->fmt == kSell) kern<kSell><<<(grid), kBlock, 0, (stream)>>>(__VA_ARGS__); \
+    else kern<kCsrV><<<(grid), kBlock, 0, (stream)>>>(__VA_ARGS__);           \
+  } while (0)
+
+// Layout choice: sliced ELL unless its slices pad the matrix by more than 25 %
+// (rows of very different lengths), then CSR with 16 lanes per row.
+static void cg_plan_layout(ffhip_cg cg, int layout) {
+  ffhip_ctx ctx = cg->ctx;
+  ffhip_mat A = cg->A;
+  hipStream_t s = ctx->stream;
+  cg->S = (A->n + kSlice - 1) / kSlice;
+  if (layout != FFHIP_CG_LAYOUT_CSR) {
+    cg->sptr = dalloc<int64_t>(cg->S + 1);
+    cg->slen = dalloc<int32_t>(cg->S);
+    int64_t *cnt = dalloc<int64_t>(cg->S + 1);
+    FFHIP_CHECK(hipMemsetAsync(cnt + cg->S, 0, 8, s));
+    k_sell_len<<<nblk(cg->S, 256), 256, 0, s>>>(A->n, cg->S, A->rowptr, cg->slen, cnt);
+    exclusive_scan_i64(ctx, cnt, cg->sptr, cg->S + 1);
+    cg->ents = read_i64(ctx, cg->sptr + cg->S);
+    dfree(cnt);
+    if (layout == FFHIP_CG_LAYOUT_SELL || 4 * cg->ents <= 5 * A->nnz) {
+      cg->fmt = kSell;
+      cg->scol = dalloc<int32_t>(cg->ents);
+    } else {
+      dfree(cg->sptr); dfree(cg->slen);
+      cg->sptr = nullptr; cg->slen = nullptr;
+      cg->fmt = kCsrV;
+    }
+  } else {
+    cg->fmt = kCsrV;
+  }
+  int64_t g;
+  if (cg->fmt == kSell) {
+    g = (cg->S + kWavesPerBlock - 1) / kWavesPerBlock;
+  } else {
+    cg->ents = A->nnz;
+    g = (A->n + kBlock / kLanesPerRow - 1) / (kBlock / kLanesPerRow);
+  }
+  const int64_t gmax = FFHIP_GRID_PER_CU * (int64_t)ctx->num_cu;
+  if (g > gmax) g = gmax;
+  if (g >= 64) g -= g % 8;  // whole XCD groups
+  if (g < 1) g = 1;
+  cg->grid = (int)g;
+}
+
+// values in the solver's layout (A or A^T; FP32 scaled copy for MIXED)
+static void cg_fill_values(ffhip_cg cg, bool transposed) {
   ffhip_ctx ctx = cg->ctx;
   ffhip_mat A = cg->A;
   hipStream_t s = ctx->stream;
@@ -588,24 +692,27 @@ static void cg_build_sell(ffhip_cg cg, bool transposed) {
     if (!ok) dfree(tmap);
     FFHIP_REQUIRE(ok, FFHIP_ERR_UNSUPPORTED, "transposed solve needs a symmetric pattern");
   }
-  if (!cg->sptr) {
-    cg->sptr = dalloc<int64_t>(cg->S + 1);
-    cg->slen = dalloc<int32_t>(cg->S);
-    int64_t *cnt = dalloc<int64_t>(cg->S + 1);
-    FFHIP_CHECK(hipMemsetAsync(cnt + cg->S, 0, 8, s));
-    k_sell_len<<<nblk(cg->S, 256), 256, 0, s>>>(A->n, cg->S, A->rowptr, cg->slen, cnt);
-    exclusive_scan_i64(ctx, cnt, cg->sptr, cg->S + 1);
-    cg->ents = read_i64(ctx, cg->sptr + cg->S);
-    dfree(cnt);
-    cg->scol = dalloc<int32_t>(cg->ents);
-  }
+  const bool mixed = cg->mode == FFHIP_CG_MIXED;
   double *&dv = transposed ? cg->svalT : cg->sval;
   float *&fv = transposed ? cg->sval32T : cg->sval32;
-  if (!dv) dv = dalloc<double>(cg->ents);
-  if (cg->mode == FFHIP_CG_MIXED && !fv) fv = dalloc<float>(cg->ents);
-  k_sell_fill<<<nblk(cg->S * kSlice, 256), 256, 0, s>>>(
-      A->n, cg->S, A->rowptr, A->colidx, A->vals, tmap, cg->sptr, cg->slen, cg->dsc,
-      transposed ? nullptr : cg->scol, dv, cg->mode == FFHIP_CG_MIXED ? fv : nullptr);
+  if (mixed && !fv) fv = dalloc<float>(cg->ents);
+  if (cg->fmt == kSell) {
+    if (!dv) dv = dalloc<double>(cg->ents);
+    k_sell_fill<<<nblk(cg->S * kSlice, 256), 256, 0, s>>>(
+        A->n, cg->S, A->rowptr, A->colidx, A->vals, tmap, cg->sptr, cg->slen, cg->dsc,
+        transposed ? nullptr : cg->scol, dv, mixed ? fv : nullptr);
+  } else {
+    double *dst = nullptr;
+    if (transposed) {
+      if (!dv) dv = dalloc<double>(cg->ents);
+      dst = dv;
+    } else {
+      dv = A->vals;  // borrowed: A's CSR order is the layout
+    }
+    if (dst || mixed)
+      k_csr_fill<<<nblk(A->n * kLanesPerRow, 256), 256, 0, s>>>(
+          A->n, A->rowptr, A->colidx, A->vals, tmap, cg->dsc, dst, mixed ? fv : nullptr);
+  }
   FFHIP_CHECK(hipGetLastError());
   FFHIP_CHECK(hipStreamSynchronize(s));
   dfree(tmap);
@@ -645,20 +752,26 @@ static void cg_setup_jacobi(ffhip_cg cg) {
   dfree(d); dfree(has); dfree(part);
 }
 
-
-// residual r = b - A x into cg->G; returns (r, D^-1 r) summed on the host in
-// block order
-static double mix_residual(ffhip_cg cg, const double *sval) {
+// r = b - A x (rout may be null); the three partial sums of k_residual added on
+// the host in block order
+static void residual_sums(ffhip_cg cg, const double *val, const double *x, const double *b,
+                          double *rout, double out[3]) {
   hipStream_t s = cg->ctx->stream;
-  k_residual<<<cg->grid, kBlock, 0, s>>>(cg->n, cg->S, cg->sptr, cg->slen, cg->scol, sval, cg->xw,
-                                         cg->bw, cg->dinv, cg->G, cg->pC);
+  FFHIP_LAUNCH_FMT(cg, k_residual, cg->grid, s, cg->fmt_of(val), x, b, cg->dinv, rout, cg->pC);
   FFHIP_CHECK(hipGetLastError());
   std::vector<double> h(3 * (size_t)cg->grid);
   FFHIP_CHECK(hipMemcpyAsync(h.data(), cg->pC, 24 * cg->grid, hipMemcpyDeviceToHost, s));
   FFHIP_CHECK(hipStreamSynchronize(s));
-  double g = 0.;
-  for (int i = 0; i < cg->grid; ++i) g += h[3 * i + 2];
-  return g;
+  out[0] = out[1] = out[2] = 0.;
+  for (int i = 0; i < cg->grid; ++i)
+    for (int c = 0; c < 3; ++c) out[c] += h[3 * i + c];
+}
+
+// residual r = b - A x into cg->G; returns (r, D^-1 r)
+static double mix_residual(ffhip_cg cg, const double *sval) {
+  double r[3];
+  residual_sums(cg, sval, cg->xw, cg->bw, cg->G, r);
+  return r[2];
 }
 
 // FP32 Jacobi-scaled CG with FP64 reliable updates (see the file comment).
@@ -668,6 +781,7 @@ int ffhip_cg_solve_mixed(ffhip_cg cg, const double *sval, const float *sval32, d
   hipStream_t s = ctx->stream;
   const int64_t n = cg->n;
   const int G = cg->grid;
+  const MatFmt<float> M32 = cg->fmt_of(sval32);
   FFHIP_CHECK(hipMemsetAsync(cg->p32[0], 0, 4 * n, s));
   FFHIP_CHECK(hipMemsetAsync(cg->p32[1], 0, 4 * n, s));
   FFHIP_CHECK(hipMemsetAsync(cg->y32, 0, 4 * n, s));
@@ -687,7 +801,7 @@ int ffhip_cg_solve_mixed(ffhip_cg cg, const double *sval, const float *sval32, d
   hm.first_it = 1;
   for (;;) {
     // (re)start the FP32 recurrence from the FP64 residual
-    k_mix_rescale<<<G, kBlock, 0, s>>>(n, cg->S, 1. / sigma, cg->dsc, cg->G, cg->r32, cg->pA);
+    k_mix_rescale<<<G, kBlock, 0, s>>>(n, 1. / sigma, cg->dsc, cg->G, cg->r32, cg->pA);
     hs.sigma = sigma;
     hs.need_update = 0;
     hs.iters = it - 1;
@@ -701,11 +815,11 @@ int ffhip_cg_solve_mixed(ffhip_cg cg, const double *sval, const float *sval32, d
         const float *pold = cg->p32[(it - 1) & 1];
         float *pnew = cg->p32[it & 1];
         cg->mark(3 * c);
-        k_mix_dir<<<G, kBlock, 0, s>>>(it, itmax, n, cg->S, cg->sptr, cg->slen, cg->scol, sval32,
-                                       pold, cg->r32, pnew, cg->q32, cg->pA, cg->pB, cg->st, cg->ms);
+        FFHIP_LAUNCH_FMT(cg, k_mix_dir, G, s, it, itmax, M32, pold, cg->r32, pnew, cg->q32, cg->pA,
+                         cg->pB, cg->st, cg->ms);
         cg->mark(3 * c + 1);
-        k_mix_update<<<G, kBlock, 0, s>>>(it, n, cg->S, pnew, cg->q32, cg->y32, cg->r32, cg->pB,
-                                          cg->pA, cg->st, cg->ms);
+        k_mix_update<<<G, kBlock, 0, s>>>(it, n, pnew, cg->q32, cg->y32, cg->r32, cg->pB, cg->pA,
+                                          cg->st, cg->ms);
         cg->mark(3 * c + 2);
       }
       FFHIP_CHECK(hipGetLastError());
@@ -740,41 +854,63 @@ int ffhip_cg_solve_mixed(ffhip_cg cg, const double *sval, const float *sval32, d
   }
 }
 
+static void cg_release(ffhip_cg cg) {
+  for (hipEvent_t e : cg->ev) (void)hipEventDestroy(e);
+  if (cg->fmt == kCsrV) cg->sval = nullptr;  // A's array
+  dfree(cg->sptr); dfree(cg->slen); dfree(cg->scol); dfree(cg->sval); dfree(cg->svalT);
+  dfree(cg->sval32); dfree(cg->sval32T); dfree(cg->dinv); dfree(cg->dsc); dfree(cg->wcl);
+  dfree(cg->xw); dfree(cg->bw); dfree(cg->G); dfree(cg->CG); dfree(cg->H[0]); dfree(cg->H[1]);
+  dfree(cg->AH); dfree(cg->r32); dfree(cg->q32); dfree(cg->y32); dfree(cg->p32[0]);
+  dfree(cg->p32[1]); dfree(cg->pA); dfree(cg->pB); dfree(cg->pC); dfree(cg->st); dfree(cg->ms);
+  delete cg;
+}
+
 extern "C" {
 
 int ffhip_cg_create(ffhip_ctx ctx, ffhip_mat A, int mode, ffhip_cg *out) {
   return guard([&] {
     FFHIP_REQUIRE(ctx && A && out, FFHIP_ERR_ARG, "ffhip_cg_create: null argument");
-    FFHIP_REQUIRE(mode == FFHIP_CG_FP64 || mode == FFHIP_CG_MIXED, FFHIP_ERR_ARG,
-                  "ffhip_cg_create: mode %d", mode);
+    const int prec = mode & 0xf, layout = mode & 0x30;
+    FFHIP_REQUIRE((prec == FFHIP_CG_FP64 || prec == FFHIP_CG_MIXED) && (mode & ~0x3f) == 0 &&
+                      layout != (FFHIP_CG_LAYOUT_SELL | FFHIP_CG_LAYOUT_CSR),
+                  FFHIP_ERR_ARG, "ffhip_cg_create: mode %d", mode);
     ffhip_cg cg = new ffhip_cg_s();
-    cg->ctx = ctx; cg->A = A; cg->mode = mode; cg->n = A->n;
-    cg->S = (A->n + kSlice - 1) / kSlice;
-    int64_t g = (cg->S + kWavesPerBlock - 1) / kWavesPerBlock;
-    const int64_t gmax = FFHIP_GRID_PER_CU * (int64_t)ctx->num_cu;
-    if (g > gmax) g = gmax;
-    if (g >= 64) g -= g % 8;  // whole XCD groups
-    cg->grid = (int)g;
-    const int64_t n = A->n;
-    cg->dinv = dalloc<double>(n);
-    cg->dsc = dalloc<double>(n);
-    cg->wcl = dalloc<uint8_t>(n);
-    cg_setup_jacobi(cg);
-    cg_build_sell(cg, false);
-    cg->xw = dalloc<double>(n); cg->bw = dalloc<double>(n);
-    cg->G = dalloc<double>(n); cg->CG = dalloc<double>(n);
-    cg->H[0] = dalloc<double>(n); cg->H[1] = dalloc<double>(n);
-    cg->AH = dalloc<double>(n);
-    if (mode == FFHIP_CG_MIXED) {
-      cg->r32 = dalloc<float>(n); cg->q32 = dalloc<float>(n); cg->y32 = dalloc<float>(n);
-      cg->p32[0] = dalloc<float>(n); cg->p32[1] = dalloc<float>(n);
+    cg->ctx = ctx; cg->A = A; cg->mode = prec; cg->n = A->n;
+    try {
+      const int64_t n = A->n;
+      cg->dinv = dalloc<double>(n);
+      cg->dsc = dalloc<double>(n);
+      cg->wcl = dalloc<uint8_t>(n);
+      cg_setup_jacobi(cg);
+      cg_plan_layout(cg, layout);
+      cg_fill_values(cg, false);
+      cg->xw = dalloc<double>(n); cg->bw = dalloc<double>(n);
+      cg->G = dalloc<double>(n); cg->CG = dalloc<double>(n);
+      cg->H[0] = dalloc<double>(n); cg->H[1] = dalloc<double>(n);
+      cg->AH = dalloc<double>(n);
+      if (prec == FFHIP_CG_MIXED) {
+        cg->r32 = dalloc<float>(n); cg->q32 = dalloc<float>(n); cg->y32 = dalloc<float>(n);
+        cg->p32[0] = dalloc<float>(n); cg->p32[1] = dalloc<float>(n);
+      }
+      cg->pA = dalloc<double>(3 * (int64_t)cg->grid);
+      cg->pB = dalloc<double>(3 * (int64_t)cg->grid);
+      cg->pC = dalloc<double>(3 * (int64_t)cg->grid);
+      cg->st = dalloc<CgState>(1);
+      cg->ms = dalloc<MixState>(1);
+    } catch (...) {
+      (void)hipStreamSynchronize(ctx->stream);
+      cg_release(cg);
+      throw;
     }
-    cg->pA = dalloc<double>(3 * (int64_t)cg->grid);
-    cg->pB = dalloc<double>(3 * (int64_t)cg->grid);
-    cg->pC = dalloc<double>(3 * (int64_t)cg->grid);
-    cg->st = dalloc<CgState>(1);
-    cg->ms = dalloc<MixState>(1);
     *out = cg;
+  });
+}
+
+int ffhip_cg_layout(ffhip_cg cg, int *layout, int64_t *entries) {
+  return guard([&] {
+    FFHIP_REQUIRE(cg, FFHIP_ERR_ARG, "ffhip_cg_layout: null solver");
+    if (layout) *layout = cg->fmt == kSell ? FFHIP_CG_LAYOUT_SELL : FFHIP_CG_LAYOUT_CSR;
+    if (entries) *entries = cg->ents;
   });
 }
 
@@ -782,23 +918,9 @@ int ffhip_cg_update_values(ffhip_cg cg) {
   return guard([&] {
     FFHIP_REQUIRE(cg, FFHIP_ERR_ARG, "ffhip_cg_update_values: null solver");
     cg_setup_jacobi(cg);
-    cg_build_sell(cg, false);
-    if (cg->svalT) cg_build_sell(cg, true);
+    cg_fill_values(cg, false);
+    if (cg->svalT) cg_fill_values(cg, true);
   });
-}
-
-static void true_residual(ffhip_cg cg, const double *sval, const double *x, const double *b,
-                          double out[3]) {
-  hipStream_t s = cg->ctx->stream;
-  k_residual<<<cg->grid, kBlock, 0, s>>>(cg->n, cg->S, cg->sptr, cg->slen, cg->scol, sval, x, b,
-                                         cg->dinv, nullptr, cg->pC);
-  FFHIP_CHECK(hipGetLastError());
-  std::vector<double> h(3 * (size_t)cg->grid);
-  FFHIP_CHECK(hipMemcpyAsync(h.data(), cg->pC, 24 * cg->grid, hipMemcpyDeviceToHost, s));
-  FFHIP_CHECK(hipStreamSynchronize(s));
-  out[0] = out[1] = out[2] = 0.;
-  for (int i = 0; i < cg->grid; ++i)
-    for (int c = 0; c < 3; ++c) out[c] += h[3 * i + c];
 }
 
 int ffhip_cg_solve(ffhip_cg cg, const double *b, double *x, int on_device, double eps, int itmax,
@@ -808,7 +930,7 @@ int ffhip_cg_solve(ffhip_cg cg, const double *b, double *x, int on_device, doubl
     ffhip_ctx ctx = cg->ctx;
     hipStream_t s = ctx->stream;
     const int64_t n = cg->n;
-    if (trans && !cg->svalT) cg_build_sell(cg, true);
+    if (trans && !cg->svalT) cg_fill_values(cg, true);
     const double *sval = trans ? cg->svalT : cg->sval;
     const float *sval32 = trans ? cg->sval32T : cg->sval32;
     if (itmax <= 0) itmax = (int)n;  // SolverCG(ds): itmax > 0 ? itmax : n
@@ -822,8 +944,9 @@ int ffhip_cg_solve(ffhip_cg cg, const double *b, double *x, int on_device, doubl
     cg->n_timed = 0;
     if (cg->mode == FFHIP_CG_FP64) {
       const int G = cg->grid;
-      k_pcg_init<<<G, kBlock, 0, s>>>(n, cg->S, cg->sptr, cg->slen, cg->scol, sval, cg->xw, cg->bw,
-                                     cg->dinv, cg->G, cg->CG, cg->H[0], cg->pA, cg->st);
+      const MatFmt<double> M = cg->fmt_of(sval);
+      FFHIP_LAUNCH_FMT(cg, k_pcg_init, G, s, M, cg->xw, cg->bw, cg->dinv, cg->G, cg->CG, cg->H[0],
+                       cg->pA, cg->st);
       FFHIP_CHECK(hipGetLastError());
       const int chunk = 16;
       cg->ensure_events(3 * chunk);
@@ -832,11 +955,11 @@ int ffhip_cg_solve(ffhip_cg cg, const double *b, double *x, int on_device, doubl
         double *Hnew = cg->H[it & 1];
         const int c = (it - 1) % chunk;
         cg->mark(3 * c);
-        k_pcg_dir<<<G, kBlock, 0, s>>>(it, itmax, eps, n, cg->S, cg->sptr, cg->slen, cg->scol, sval,
-                                       Hold, cg->CG, cg->G, Hnew, cg->AH, cg->pA, cg->pB, cg->st);
+        FFHIP_LAUNCH_FMT(cg, k_pcg_dir, G, s, it, itmax, eps, M, Hold, cg->CG, cg->G, Hnew, cg->AH,
+                         cg->pA, cg->pB, cg->st);
         cg->mark(3 * c + 1);
-        k_pcg_update<<<G, kBlock, 0, s>>>(n, cg->S, Hnew, cg->AH, cg->dinv, cg->xw, cg->G, cg->CG,
-                                          cg->pB, cg->pA, cg->st);
+        k_pcg_update<<<G, kBlock, 0, s>>>(n, Hnew, cg->AH, cg->dinv, cg->xw, cg->G, cg->CG, cg->pB,
+                                          cg->pA, cg->st);
         cg->mark(3 * c + 2);
         if (it % chunk == 0) {
           FFHIP_CHECK(hipGetLastError());
@@ -852,7 +975,7 @@ int ffhip_cg_solve(ffhip_cg cg, const double *b, double *x, int on_device, doubl
     ctx->toc();
     const double ms = ctx->last_ms;
     double res[3];
-    true_residual(cg, sval, cg->xw, cg->bw, res);
+    residual_sums(cg, sval, cg->xw, cg->bw, nullptr, res);
     FFHIP_CHECK(hipMemcpyAsync(x, cg->xw, 8 * n, on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
     FFHIP_CHECK(hipStreamSynchronize(s));
     if (stats) {
@@ -885,13 +1008,7 @@ int ffhip_cg_free(ffhip_cg cg) {
   return guard([&] {
     if (!cg) return;
     (void)hipStreamSynchronize(cg->ctx->stream);
-    for (hipEvent_t e : cg->ev) (void)hipEventDestroy(e);
-    dfree(cg->sptr); dfree(cg->slen); dfree(cg->scol); dfree(cg->sval); dfree(cg->svalT);
-    dfree(cg->sval32); dfree(cg->sval32T); dfree(cg->dinv); dfree(cg->dsc); dfree(cg->wcl);
-    dfree(cg->xw); dfree(cg->bw); dfree(cg->G); dfree(cg->CG); dfree(cg->H[0]); dfree(cg->H[1]);
-    dfree(cg->AH); dfree(cg->r32); dfree(cg->q32); dfree(cg->y32); dfree(cg->p32[0]);
-    dfree(cg->p32[1]); dfree(cg->pA); dfree(cg->pB); dfree(cg->pC); dfree(cg->st); dfree(cg->ms);
-    delete cg;
+    cg_release(cg);
   });
 }
 
@@ -903,14 +1020,19 @@ int ffhip_bench_spmv(ffhip_cg cg, int mode, int reps, const void *x, void *y, do
     ffhip_ctx ctx = cg->ctx;
     hipStream_t s = ctx->stream;
     auto launch = [&] {
-      if (mode == FFHIP_CG_FP64)
-        k_sell_spmv<double, double><<<cg->grid, kBlock, 0, s>>>(cg->n, cg->S, cg->sptr, cg->slen,
-                                                               cg->scol, cg->sval, (const double *)x,
-                                                               (double *)y);
-      else
-        k_sell_spmv<float, float><<<cg->grid, kBlock, 0, s>>>(cg->n, cg->S, cg->sptr, cg->slen,
-                                                             cg->scol, cg->sval32, (const float *)x,
-                                                             (float *)y);
+      if (mode == FFHIP_CG_FP64) {
+        const MatFmt<double> M = cg->fmt_of<double>(cg->sval);
+        if (cg->fmt == kSell)
+          k_matvec<kSell, double, double><<<cg->grid, kBlock, 0, s>>>(M, (const double *)x, (double *)y);
+        else
+          k_matvec<kCsrV, double, double><<<cg->grid, kBlock, 0, s>>>(M, (const double *)x, (double *)y);
+      } else {
+        const MatFmt<float> M = cg->fmt_of<float>(cg->sval32);
+        if (cg->fmt == kSell)
+          k_matvec<kSell, float, float><<<cg->grid, kBlock, 0, s>>>(M, (const float *)x, (float *)y);
+        else
+          k_matvec<kCsrV, float, float><<<cg->grid, kBlock, 0, s>>>(M, (const float *)x, (float *)y);
+      }
     };
     launch();  // warm
     FFHIP_CHECK(hipGetLastError());

@@ -112,6 +112,27 @@ struct ffhip_mat_s {
   uint32_t *slotmap = nullptr; // (nt*nloc) packed 4 x uint8 (P1, nloc <= 4)
   int max_row = 0;
   ffhip_mesh mesh = nullptr;   // mesh the pattern was built from (if any)
+  // vector-P2 matrices: node-level rows (the scalar rows 3a..3a+2 of node a
+  // hold 3 x 3 blocks of its node columns) and the byte slot map, 10 offsets
+  // per node incidence
+  ffhip_p2v p2v = nullptr;
+  int64_t *nodeptr = nullptr;  // nnodes + 1
+  uint8_t *slot8 = nullptr;    // (nt * 10) * 10
+};
+
+// [P2,P2,P2] space on a tetrahedral mesh (FESpace3 with GTypeOfFESum of three
+// P2): nodes numbered by first appearance over the elements, then dof
+// 3 * node + component.
+struct ffhip_p2v_s {
+  ffhip_ctx ctx = nullptr;
+  ffhip_mesh mesh = nullptr;
+  int64_t nnodes = 0, ne = 0;
+  int32_t *elem_nodes = nullptr;  // nt * 10: 4 vertex nodes, 6 edge nodes
+  int32_t *vnode = nullptr;       // nv: node of each vertex
+  uint64_t *ekeys = nullptr;      // ne: sorted edge keys (min << 32 | max)
+  int32_t *enode = nullptr;       // ne: node of each edge
+  int64_t *n2e_ptr = nullptr;     // nnodes + 1
+  int32_t *n2e = nullptr;         // nt * 10: element * 10 + local node, element order
 };
 
 namespace ffhip {

@@ -1,0 +1,676 @@
+// ffhip_p2.hip — vector P2 ([P2,P2,P2]) linear elasticity on tetrahedra:
+// node numbering, matrix pattern, Lame element matrices on the matrix cores,
+// atomic-free row assembly, clamped boundary rows and body-force right-hand
+// side.
+//
+// Replaces, for `varf va([u1,u2,u3],[v1,v2,v3]) = int3d(Th)(lambda*div(u)*div(v)
+// + 2*mu*(epsilon(u)'*epsilon(v))) + int3d(Th)(f'*v) + on(4, u1=0,u2=0,u3=0)`
+// on Vh(Th,[P2,P2,P2]):
+//   node numbering   GenericMesh::BuildDFNumbering (GenericMesh.hpp:1880-1920)
+//                    and FESpacen.cpp:403-428 (dof = 3*node + component)
+//   element matrix   Element_Op (problem.cpp:6337-6437) with the P2 basis of
+//                    TypeOfFE_P2Lagrange3d::FB (P012_3d.cpp:199-270), Tet_5
+//   accumulation     HashMatrix::operator+= (HashMatrix.cpp:1294-1332)
+//   Dirichlet        AssembleBC (problem.cpp:10120-10164), SetBC tgv
+//   right-hand side  Element_rhs (problem.cpp:7940-7978)
+//
+// Design:
+//   numbering  every (element, local node) item carries a key (vertex or
+//              edge); a node's number is the rank of its first item, found
+//              with atomicMin over vertices, a radix sort of the edge keys and
+//              one scan over the items — equal to FreeFEM's sequential
+//              first-appearance numbering.
+//   pattern    node rows (sorted unique node columns, LDS merge per lane, as
+//              P1), expanded to scalar rows of 3 x 3 blocks.
+//   element    one wave per element: M = D W D^T (30 x 30, D = P2 gradients
+//              at the 14 points, W = |K| w_q) as three 16x16 tiles of
+//              v_mfma_f64_16x16x4f64 over k = 16 (14 points, zero padded);
+//              K[(c,i),(c',j)] = lambda M[ic][jc'] + mu M[ic'][jc]
+//              + mu delta_cc' sum_a M[ia][ja] is formed from M in LDS and
+//              written node-block major (90 doubles per local node).
+//   gather     one wave per node: the incident elements' 90-value blocks are
+//              added in element order into the node's 3 rows staged in LDS,
+//              then written back contiguously.  No atomics, deterministic.
+#include <hipcub/hipcub.hpp>
+
+#include <climits>
+#include <vector>
+
+#include "ffhip_fem.hpp"
+
+namespace ffhip {
+void build_incidence(ffhip_ctx ctx, const int32_t *elems, int64_t nt, int nloc, int64_t n,
+                     int64_t **ptr_out, int32_t **list_out);
+void set_diag_rows(ffhip_ctx ctx, int64_t n, const uint8_t *onbc, const int64_t *diag, double tgv,
+                   double *vals, int *missing);
+int read_flag(ffhip_ctx ctx, int *d);
+
+// Tet::nvedge (local vertex pairs of the 6 edges)
+static __constant__ int c_edge[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+
+static unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+__device__ inline uint64_t edge_key(int32_t a, int32_t b) {
+  if (a > b) { const int32_t t = a; a = b; b = t; }
+  return ((uint64_t)(uint32_t)a << 32) | (uint32_t)b;
+}
+
+// ---------------------------------------------------------------- numbering
+__global__ void k_fill_i32(int64_t n, int32_t v, int32_t *__restrict__ p) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// first item (10 k + i) of every vertex
+__global__ void k_p2_vfirst(int64_t nt, const int32_t *__restrict__ tets, int32_t *__restrict__ firstv) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= 4 * nt) return;
+  const int64_t k = m / 4;
+  const int i = (int)(m % 4);
+  atomicMin(&firstv[tets[m]], (int32_t)(10 * k + i));
+}
+
+// edge keys and their items, in item order (the radix sort is stable)
+__global__ void k_p2_edge_keys(int64_t nt, const int32_t *__restrict__ tets,
+                               uint64_t *__restrict__ keys, int32_t *__restrict__ items) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= 6 * nt) return;
+  const int64_t k = m / 6;
+  const int j = (int)(m % 6);
+  const int32_t *t = tets + 4 * k;
+  keys[m] = edge_key(t[c_edge[j][0]], t[c_edge[j][1]]);
+  items[m] = (int32_t)(10 * k + 4 + j);
+}
+
+// head flags of the sorted edge keys; first-appearance marks of edge items
+__global__ void k_p2_edge_heads(int64_t E, const uint64_t *__restrict__ skeys,
+                                const int32_t *__restrict__ sitems, int64_t *__restrict__ head,
+                                int64_t *__restrict__ mark) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= E) return;
+  const int64_t h = (p == 0 || skeys[p] != skeys[p - 1]) ? 1 : 0;
+  head[p] = h;
+  mark[sitems[p]] = h;
+}
+
+// edge segment of every edge item; unique keys and their first items
+__global__ void k_p2_edge_seg(int64_t E, const uint64_t *__restrict__ skeys,
+                              const int32_t *__restrict__ sitems, const int64_t *__restrict__ hscan,
+                              const int64_t *__restrict__ head, int32_t *__restrict__ item_seg,
+                              uint64_t *__restrict__ ukeys, int32_t *__restrict__ segfirst) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= E) return;
+  const int64_t seg = hscan[p] + head[p] - 1;  // inclusive scan - 1
+  const int32_t item = sitems[p];
+  item_seg[(item / 10) * 6 + (item % 10 - 4)] = (int32_t)seg;
+  if (head[p]) { ukeys[seg] = skeys[p]; segfirst[seg] = item; }
+}
+
+__global__ void k_p2_vmark(int64_t nt, const int32_t *__restrict__ tets,
+                           const int32_t *__restrict__ firstv, int64_t *__restrict__ mark) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= 4 * nt) return;
+  const int64_t k = m / 4;
+  const int i = (int)(m % 4);
+  const int32_t item = (int32_t)(10 * k + i);
+  mark[10 * k + i] = firstv[tets[m]] == item ? 1 : 0;
+}
+
+__global__ void k_p2_assign(int64_t nt, const int32_t *__restrict__ tets,
+                            const int32_t *__restrict__ firstv, const int32_t *__restrict__ item_seg,
+                            const int32_t *__restrict__ segfirst, const int64_t *__restrict__ rank,
+                            int32_t *__restrict__ elem_nodes) {
+  const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= 10 * nt) return;
+  const int64_t k = m / 10;
+  const int i = (int)(m % 10);
+  const int32_t first = i < 4 ? firstv[tets[4 * k + i]] : segfirst[item_seg[6 * k + i - 4]];
+  elem_nodes[m] = (int32_t)rank[first];
+}
+
+__global__ void k_p2_vnode(int64_t nv, const int32_t *__restrict__ firstv,
+                           const int64_t *__restrict__ rank, int32_t *__restrict__ vnode) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nv) return;
+  const int32_t f = firstv[v];
+  vnode[v] = f == INT_MAX ? -1 : (int32_t)rank[f];
+}
+
+__global__ void k_p2_enode(int64_t ne, const int32_t *__restrict__ segfirst,
+                           const int64_t *__restrict__ rank, int32_t *__restrict__ enode) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < ne) enode[s] = (int32_t)rank[segfirst[s]];
+}
+
+// ---------------------------------------------------------------- pattern
+// scalar rows 3a + c of node a: for every node column b, columns 3b, 3b+1, 3b+2
+__global__ void k_p2v_expand(int64_t nn, const int64_t *__restrict__ nodeptr,
+                             const int32_t *__restrict__ ncol, const int64_t *__restrict__ ndiag,
+                             int64_t *__restrict__ rowptr, int32_t *__restrict__ colidx,
+                             int64_t *__restrict__ diag) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= 3 * nn) return;
+  const int64_t a = r / 3;
+  const int c = (int)(r % 3);
+  const int64_t p0 = nodeptr[a], L = nodeptr[a + 1] - p0;
+  const int64_t base = 9 * p0 + c * 3 * L;
+  rowptr[r] = base;
+  if (r == 3 * nn - 1) rowptr[3 * nn] = 9 * nodeptr[nn];
+  for (int64_t q = 0; q < L; ++q) {
+    const int32_t b = ncol[p0 + q];
+    colidx[base + 3 * q] = 3 * b;
+    colidx[base + 3 * q + 1] = 3 * b + 1;
+    colidx[base + 3 * q + 2] = 3 * b + 2;
+  }
+  diag[r] = ndiag[a] < 0 ? -1 : base + 3 * (ndiag[a] - p0) + c;
+}
+
+// ---------------------------------------------------------------- element matrices
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+__device__ inline double pick3(const double v[3], int a) { return a == 0 ? v[0] : (a == 1 ? v[1] : v[2]); }
+__device__ inline double pick4(const double v[4], int i) {
+  return i == 0 ? v[0] : (i == 1 ? v[1] : (i == 2 ? v[2] : v[3]));
+}
+
+constexpr int kElemBlock = 256;
+constexpr int kElemWaves = kElemBlock / kWave;
+constexpr int kMs = 33;  // LDS row stride of M (doubles)
+
+// Row r = 3 i + a of D (r < 30): gradient component a of P2 basis i,
+//   vertex i:        G_i[a] (4 l_i - 1)
+//   edge (i0, i1):   4 (G_i1[a] l_i0 + G_i0[a] l_i1)        (P012_3d.cpp:236-262)
+struct P2Row {
+  int vert;      // 1: vertex basis
+  int i0, i1;    // vertex index (i0) or edge end points
+  double g0, g1; // G_i0[a], G_i1[a] of the current element
+};
+
+__device__ inline double p2_grad(const P2Row &R, const double l[4]) {
+  if (R.vert) return R.g0 * (4 * pick4(l, R.i0) - 1);
+  return 4 * (R.g1 * pick4(l, R.i0) + R.g0 * pick4(l, R.i1));
+}
+
+// Ke[e][i][c][j][c'] = K[(c,i),(c',j)] (row = test function), 900 doubles
+__global__ __launch_bounds__(kElemBlock) void k_p2v_elast_element(
+    int64_t nt, const double *__restrict__ xyz, const int32_t *__restrict__ tets, double lambda,
+    double mu, double *__restrict__ Ke) {
+  __shared__ double sM[kElemWaves][32 * kMs];
+  const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int col = lane % 16, kq = lane / 16;
+  // the two rows of D this lane feeds to the matrix cores (r = col, 16 + col)
+  int rr[2] = {col, 16 + col};
+  P2Row R[2];
+  int rvalid[2];
+#pragma unroll
+  for (int I = 0; I < 2; ++I) {
+    const int r = rr[I];
+    rvalid[I] = r < 30;
+    const int i = r < 30 ? r / 3 : 0;
+    R[I].vert = i < 4;
+    R[I].i0 = i < 4 ? i : c_edge[i - 4][0];
+    R[I].i1 = i < 4 ? i : c_edge[i - 4][1];
+  }
+  double* M = sM[w];
+  for (int64_t base = (int64_t)blockIdx.x * kElemWaves; base < nt; base += (int64_t)gridDim.x * kElemWaves) {
+    const int64_t e = base + w;  // wave-uniform
+    if (e < nt) {
+      const int4 tv = reinterpret_cast<const int4 *>(tets)[e];
+      const int32_t id[4] = {tv.x, tv.y, tv.z, tv.w};
+      double v[4][3];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) v[i][k] = xyz[3 * (int64_t)id[i] + k];
+      double G[4][3];
+      const double mes = tet_geom(v, G);
+#pragma unroll
+      for (int I = 0; I < 2; ++I) {
+        const int a = rr[I] < 30 ? rr[I] % 3 : 0;
+        double ga[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) ga[t] = pick3(G[t], a);
+        R[I].g0 = pick4(ga, R[I].i0);
+        R[I].g1 = pick4(ga, R[I].i1);
+      }
+      d4 acc00 = {0., 0., 0., 0.}, acc01 = {0., 0., 0., 0.}, acc11 = {0., 0., 0., 0.};
+#pragma unroll
+      for (int ks = 0; ks < 4; ++ks) {
+        const int q = 4 * ks + kq;  // quadrature point of this lane's k index
+        double W = 0., l[4] = {0., 0., 0., 0.};
+        if (q < 14) {
+          const double px = c_qf5_pts[q][0], py = c_qf5_pts[q][1], pz = c_qf5_pts[q][2];
+          l[0] = 1. - (px + py + pz); l[1] = px; l[2] = py; l[3] = pz;
+          W = mes * qf5_weight(q);
+        }
+        const double g0 = rvalid[0] ? p2_grad(R[0], l) : 0.;
+        const double g1 = rvalid[1] ? p2_grad(R[1], l) : 0.;
+        acc00 = __builtin_amdgcn_mfma_f64_16x16x4f64(W * g0, g0, acc00, 0, 0, 0);
+        acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(W * g0, g1, acc01, 0, 0, 0);
+        acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(W * g1, g1, acc11, 0, 0, 0);
+      }
+      // D[i][j] of a 16x16 tile: j = lane % 16, i = 4 (lane / 16) + v
+#pragma unroll
+      for (int vv = 0; vv < 4; ++vv) {
+        const int i = 4 * kq + vv, j = col;
+        M[i * kMs + j] = acc00[vv];
+        M[i * kMs + 16 + j] = acc01[vv];
+        M[(16 + j) * kMs + i] = acc01[vv];
+        M[(16 + i) * kMs + 16 + j] = acc11[vv];
+      }
+    }
+    __syncthreads();
+    if (e < nt) {
+      double *dst = Ke + e * 900;
+      for (int o = lane; o < 900; o += kWave) {
+        const int i = o / 90, rem = o % 90, c = rem / 30, r2 = rem % 30, j = r2 / 3, cp = r2 % 3;
+        double k = lambda * M[(3 * i + c) * kMs + 3 * j + cp] + mu * M[(3 * i + cp) * kMs + 3 * j + c];
+        if (c == cp)
+          k += mu * (M[(3 * i) * kMs + 3 * j] + M[(3 * i + 1) * kMs + 3 * j + 1] +
+                     M[(3 * i + 2) * kMs + 3 * j + 2]);
+        dst[o] = k;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- row gather
+// orders the LDS accesses of one wave (cross-lane reuse of the row staging)
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int kNodeCap = 96;  // node columns per node row
+
+__global__ __launch_bounds__(256) void k_p2v_gather(
+    int64_t nn, const int64_t *__restrict__ nodeptr, const int64_t *__restrict__ n2e_ptr,
+    const int32_t *__restrict__ n2e, const uint8_t *__restrict__ slot8,
+    const double *__restrict__ Ke, double *__restrict__ vals) {
+  __shared__ double S[256 / kWave][9 * kNodeCap];
+  const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int64_t a = (int64_t)blockIdx.x * (256 / kWave) + w;
+  if (a >= nn) return;  // wave-uniform; the kernel has no block barrier
+  double *acc = S[w];
+  const int64_t p0 = nodeptr[a];
+  const int L = (int)(nodeptr[a + 1] - p0), Wd = 9 * L;
+  for (int t = lane; t < Wd; t += kWave) acc[t] = 0.;
+  // this lane's two entries of a 90-value block: (c, j, c') of t = lane, lane + 64
+  int tj[2], tdst[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int t = lane + h * kWave;
+    const int c = t / 30, r2 = t % 30;
+    tj[h] = r2 / 3;
+    tdst[h] = c * 3 * L + r2 % 3;  // + 3 * offset of node column j
+  }
+  wave_lds_sync();
+  const int64_t k0 = n2e_ptr[a], k1 = n2e_ptr[a + 1];
+  for (int64_t k = k0; k < k1; ++k) {
+    const int32_t item = n2e[k];
+    const int64_t e = item / 10;
+    const int i = item - (int)e * 10;
+    const double *blk = Ke + e * 900 + i * 90;
+    const uint8_t *sl = slot8 + k * 10;
+    const double v0 = blk[lane];
+    const int o0 = sl[tj[0]];
+    double v1 = 0.;
+    int o1 = 0;
+    if (lane < 90 - kWave) { v1 = blk[lane + kWave]; o1 = sl[tj[1]]; }
+    acc[tdst[0] + 3 * o0] += v0;
+    if (lane < 90 - kWave) acc[tdst[1] + 3 * o1] += v1;
+    wave_lds_sync();
+  }
+  double *out = vals + 9 * p0;
+  for (int t = lane; t < Wd; t += kWave) out[t] = acc[t];
+}
+
+// ---------------------------------------------------------------- Dirichlet
+__device__ inline int32_t find_edge_node(const uint64_t *__restrict__ ekeys,
+                                         const int32_t *__restrict__ enode, int64_t ne, uint64_t key) {
+  int64_t lo = 0, hi = ne - 1;
+  while (lo <= hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    const uint64_t k = ekeys[mid];
+    if (k == key) return enode[mid];
+    if (k < key) lo = mid + 1; else hi = mid - 1;
+  }
+  return -1;
+}
+
+// dofs of the labelled boundary triangles: 3 vertex nodes and 3 edge nodes,
+// all components
+__global__ void k_p2v_mark_bc(int64_t nbe, const int32_t *__restrict__ btri,
+                              const int32_t *__restrict__ blab, const int32_t *__restrict__ labs,
+                              int nlab, const int32_t *__restrict__ vnode,
+                              const uint64_t *__restrict__ ekeys, const int32_t *__restrict__ enode,
+                              int64_t ne, uint8_t *__restrict__ onbc, int *__restrict__ missing) {
+  const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= nbe) return;
+  int on = 0;
+  for (int q = 0; q < nlab; ++q) on |= blab[f] == labs[q];
+  if (!on) return;
+  const int32_t *t = btri + 3 * f;
+  int32_t nodes[6] = {vnode[t[0]], vnode[t[1]], vnode[t[2]],
+                      find_edge_node(ekeys, enode, ne, edge_key(t[0], t[1])),
+                      find_edge_node(ekeys, enode, ne, edge_key(t[1], t[2])),
+                      find_edge_node(ekeys, enode, ne, edge_key(t[0], t[2]))};
+  for (int i = 0; i < 6; ++i) {
+    if (nodes[i] < 0) { atomicMax(missing, 1); continue; }
+    for (int c = 0; c < 3; ++c) onbc[3 * (int64_t)nodes[i] + c] = 1;
+  }
+}
+
+// ---------------------------------------------------------------- right-hand side
+// b_(3a+c) = sum over incident elements (element order) and points q of
+// (|K| w_q f_c) phi_i(x_q), as Element_rhs accumulates it; tgv * g_c on
+// Dirichlet rows
+__global__ void k_p2v_rhs(int64_t nn, const double *__restrict__ xyz, const int32_t *__restrict__ tets,
+                          const int64_t *__restrict__ n2e_ptr, const int32_t *__restrict__ n2e,
+                          double f0, double f1, double f2, const uint8_t *__restrict__ onbc,
+                          double tgv, double g0, double g1, double g2, double *__restrict__ b) {
+#pragma clang fp contract(off)
+  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= nn) return;
+  const double fv[3] = {f0, f1, f2};
+  double acc[3] = {0., 0., 0.};
+  for (int64_t k = n2e_ptr[a]; k < n2e_ptr[a + 1]; ++k) {
+    const int32_t item = n2e[k];
+    const int64_t e = item / 10;
+    const int i = item - (int)e * 10;
+    double v[4][3];
+    for (int t = 0; t < 4; ++t)
+      for (int d = 0; d < 3; ++d) v[t][d] = xyz[3 * (int64_t)tets[4 * e + t] + d];
+    const double mes = tet_mesure(v);
+    const int i0 = i < 4 ? i : c_edge[i - 4][0], i1 = i < 4 ? i : c_edge[i - 4][1];
+    for (int q = 0; q < 14; ++q) {
+      const double px = c_qf5_pts[q][0], py = c_qf5_pts[q][1], pz = c_qf5_pts[q][2];
+      const double l[4] = {1. - (px + py + pz), px, py, pz};
+      const double coef = mes * qf5_weight(q);
+      const double phi = i < 4 ? l[i] * (2 * l[i] - 1.) : 4. * l[i0] * l[i1];
+      for (int c = 0; c < 3; ++c) acc[c] += coef * fv[c] * phi;
+    }
+  }
+  const double g[3] = {g0, g1, g2};
+  for (int c = 0; c < 3; ++c) b[3 * a + c] = onbc[3 * a + c] ? tgv * g[c] : acc[c];
+}
+
+// onbc (3 nnodes bytes) of the labelled boundary; false if a node was missing
+static bool p2v_mark(ffhip_ctx ctx, ffhip_p2v sp, const int32_t *labels, int nlab, uint8_t *onbc) {
+  hipStream_t s = ctx->stream;
+  ffhip_mesh m = sp->mesh;
+  int32_t *labs = dalloc<int32_t>(nlab > 0 ? nlab : 1);
+  int *missing = dalloc<int>(1);
+  if (nlab) FFHIP_CHECK(hipMemcpyAsync(labs, labels, 4 * nlab, hipMemcpyHostToDevice, s));
+  FFHIP_CHECK(hipMemsetAsync(onbc, 0, 3 * sp->nnodes, s));
+  FFHIP_CHECK(hipMemsetAsync(missing, 0, sizeof(int), s));
+  if (m->nbe)
+    k_p2v_mark_bc<<<nblk(m->nbe, 256), 256, 0, s>>>(m->nbe, m->belems, m->blab, labs, nlab,
+                                                    sp->vnode, sp->ekeys, sp->enode, sp->ne, onbc,
+                                                    missing);
+  FFHIP_CHECK(hipGetLastError());
+  const int miss = read_flag(ctx, missing);
+  dfree(labs);
+  dfree(missing);
+  return miss == 0;
+}
+
+static void p2v_release(ffhip_p2v sp) {
+  dfree(sp->elem_nodes); dfree(sp->vnode); dfree(sp->ekeys); dfree(sp->enode);
+  dfree(sp->n2e_ptr); dfree(sp->n2e);
+  delete sp;
+}
+
+}  // namespace ffhip
+
+using namespace ffhip;
+
+extern "C" {
+
+int ffhip_p2v_create(ffhip_ctx ctx, ffhip_mesh m, ffhip_p2v *out) {
+  return guard([&] {
+    FFHIP_REQUIRE(ctx && m && out, FFHIP_ERR_ARG, "ffhip_p2v_create: null argument");
+    FFHIP_REQUIRE(m->dim == 3 && m->nloc == 4, FFHIP_ERR_UNSUPPORTED,
+                  "ffhip_p2v_create: needs a tetrahedral mesh");
+    FFHIP_REQUIRE(m->nt > 0 && 10 * m->nt < INT_MAX, FFHIP_ERR_UNSUPPORTED,
+                  "ffhip_p2v_create: %lld elements (need 10 nt < 2^31)", (long long)m->nt);
+    hipStream_t s = ctx->stream;
+    const int64_t nt = m->nt, nv = m->nv, E = 6 * nt, NI = 10 * nt;
+    ffhip_p2v sp = new ffhip_p2v_s();
+    sp->ctx = ctx; sp->mesh = m;
+    int32_t *firstv = nullptr, *items = nullptr, *sitems = nullptr, *item_seg = nullptr,
+            *segfirst = nullptr;
+    uint64_t *keys = nullptr, *skeys = nullptr, *ukeys = nullptr;
+    int64_t *head = nullptr, *hscan = nullptr, *mark = nullptr, *rank = nullptr;
+    void *tmp = nullptr;
+    try {
+      ctx->tic();
+      firstv = dalloc<int32_t>(nv);
+      k_fill_i32<<<nblk(nv, 256), 256, 0, s>>>(nv, INT_MAX, firstv);
+      k_p2_vfirst<<<nblk(4 * nt, 256), 256, 0, s>>>(nt, m->elems, firstv);
+      keys = dalloc<uint64_t>(E); skeys = dalloc<uint64_t>(E);
+      items = dalloc<int32_t>(E); sitems = dalloc<int32_t>(E);
+      k_p2_edge_keys<<<nblk(E, 256), 256, 0, s>>>(nt, m->elems, keys, items);
+      size_t tb = 0;
+      FFHIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, keys, skeys, items, sitems, (int)E, 0,
+                                                     64, s));
+      tmp = dalloc<char>(tb);
+      FFHIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, keys, skeys, items, sitems, (int)E, 0,
+                                                     64, s));
+      head = dalloc<int64_t>(E + 1); hscan = dalloc<int64_t>(E + 1);
+      mark = dalloc<int64_t>(NI + 1); rank = dalloc<int64_t>(NI + 1);
+      FFHIP_CHECK(hipMemsetAsync(head + E, 0, 8, s));
+      FFHIP_CHECK(hipMemsetAsync(mark + NI, 0, 8, s));
+      k_p2_edge_heads<<<nblk(E, 256), 256, 0, s>>>(E, skeys, sitems, head, mark);
+      k_p2_vmark<<<nblk(4 * nt, 256), 256, 0, s>>>(nt, m->elems, firstv, mark);
+      exclusive_scan_i64(ctx, head, hscan, E + 1);
+      sp->ne = read_i64(ctx, hscan + E);
+      item_seg = dalloc<int32_t>(E);
+      ukeys = dalloc<uint64_t>(sp->ne);
+      segfirst = dalloc<int32_t>(sp->ne);
+      k_p2_edge_seg<<<nblk(E, 256), 256, 0, s>>>(E, skeys, sitems, hscan, head, item_seg, ukeys,
+                                                 segfirst);
+      exclusive_scan_i64(ctx, mark, rank, NI + 1);
+      sp->nnodes = read_i64(ctx, rank + NI);
+      sp->elem_nodes = dalloc<int32_t>(NI);
+      k_p2_assign<<<nblk(NI, 256), 256, 0, s>>>(nt, m->elems, firstv, item_seg, segfirst, rank,
+                                                sp->elem_nodes);
+      sp->vnode = dalloc<int32_t>(nv);
+      k_p2_vnode<<<nblk(nv, 256), 256, 0, s>>>(nv, firstv, rank, sp->vnode);
+      sp->enode = dalloc<int32_t>(sp->ne);
+      k_p2_enode<<<nblk(sp->ne, 256), 256, 0, s>>>(sp->ne, segfirst, rank, sp->enode);
+      sp->ekeys = ukeys;
+      ukeys = nullptr;
+      FFHIP_CHECK(hipGetLastError());
+      build_incidence(ctx, sp->elem_nodes, nt, 10, sp->nnodes, &sp->n2e_ptr, &sp->n2e);
+      ctx->toc();
+    } catch (...) {
+      (void)hipStreamSynchronize(s);
+      dfree(firstv); dfree(keys); dfree(skeys); dfree(items); dfree(sitems); dfree(tmp);
+      dfree(head); dfree(hscan); dfree(mark); dfree(rank); dfree(item_seg); dfree(ukeys);
+      dfree(segfirst);
+      p2v_release(sp);
+      throw;
+    }
+    dfree(firstv); dfree(keys); dfree(skeys); dfree(items); dfree(sitems); dfree(tmp);
+    dfree(head); dfree(hscan); dfree(mark); dfree(rank); dfree(item_seg); dfree(segfirst);
+    *out = sp;
+  });
+}
+
+int ffhip_p2v_info(ffhip_p2v sp, int64_t *nnodes, int64_t *ndof) {
+  return guard([&] {
+    FFHIP_REQUIRE(sp, FFHIP_ERR_ARG, "ffhip_p2v_info: null space");
+    if (nnodes) *nnodes = sp->nnodes;
+    if (ndof) *ndof = 3 * sp->nnodes;
+  });
+}
+
+int ffhip_p2v_download(ffhip_p2v sp, int32_t *elem_nodes) {
+  return guard([&] {
+    FFHIP_REQUIRE(sp && elem_nodes, FFHIP_ERR_ARG, "ffhip_p2v_download: null argument");
+    hipStream_t s = sp->ctx->stream;
+    FFHIP_CHECK(hipMemcpyAsync(elem_nodes, sp->elem_nodes, 4 * 10 * sp->mesh->nt,
+                               hipMemcpyDeviceToHost, s));
+    FFHIP_CHECK(hipStreamSynchronize(s));
+  });
+}
+
+int ffhip_p2v_free(ffhip_p2v sp) {
+  return guard([&] {
+    if (!sp) return;
+    (void)hipStreamSynchronize(sp->ctx->stream);
+    p2v_release(sp);
+  });
+}
+
+int ffhip_pattern_p2v(ffhip_ctx ctx, ffhip_p2v sp, ffhip_mat *out) {
+  return guard([&] {
+    FFHIP_REQUIRE(ctx && sp && out, FFHIP_ERR_ARG, "ffhip_pattern_p2v: null argument");
+    hipStream_t s = ctx->stream;
+    ffhip_mesh m = sp->mesh;
+    const int64_t nn = sp->nnodes, n = 3 * nn;
+    FFHIP_REQUIRE(n < INT_MAX, FFHIP_ERR_UNSUPPORTED, "ffhip_pattern_p2v: too many dofs");
+    ffhip_mat A = new ffhip_mat_s();
+    A->ctx = ctx; A->n = n; A->p2v = sp;
+    int64_t *len = nullptr, *ndiag = nullptr;
+    int32_t *ncol = nullptr;
+    int *ovf = nullptr;
+    try {
+      ctx->tic();
+      len = dalloc<int64_t>(nn + 1);
+      ovf = dalloc<int>(1);
+      A->nodeptr = dalloc<int64_t>(nn + 1);
+      ndiag = dalloc<int64_t>(nn);
+      A->slot8 = dalloc<uint8_t>(m->nt * 100);
+      FFHIP_CHECK(hipMemsetAsync(len, 0, 8 * (nn + 1), s));
+      FFHIP_CHECK(hipMemsetAsync(ovf, 0, sizeof(int), s));
+      constexpr int B = 128, CAP = kNodeCap;
+      k_pattern_rows<B, CAP, false><<<nblk(nn, B), B, 0, s>>>(
+          nn, 10, sp->elem_nodes, sp->n2e_ptr, sp->n2e, len, nullptr, nullptr, nullptr, nullptr,
+          nullptr, ovf);
+      FFHIP_CHECK(hipGetLastError());
+      FFHIP_REQUIRE(read_flag(ctx, ovf) == 0, FFHIP_ERR_UNSUPPORTED,
+                    "ffhip_pattern_p2v: a node couples to more than %d nodes", CAP);
+      exclusive_scan_i64(ctx, len, A->nodeptr, nn + 1);
+      const int64_t nnz_nodes = read_i64(ctx, A->nodeptr + nn);
+      ncol = dalloc<int32_t>(nnz_nodes);
+      k_pattern_rows<B, CAP, true><<<nblk(nn, B), B, 0, s>>>(
+          nn, 10, sp->elem_nodes, sp->n2e_ptr, sp->n2e, nullptr, A->nodeptr, ncol, ndiag, nullptr,
+          A->slot8, ovf);
+      A->nnz = 9 * nnz_nodes;
+      A->rowptr = dalloc<int64_t>(n + 1);
+      A->colidx = dalloc<int32_t>(A->nnz);
+      A->diag = dalloc<int64_t>(n);
+      A->vals = dalloc<double>(A->nnz);
+      k_p2v_expand<<<nblk(n, 256), 256, 0, s>>>(nn, A->nodeptr, ncol, ndiag, A->rowptr, A->colidx,
+                                                A->diag);
+      FFHIP_CHECK(hipMemsetAsync(A->vals, 0, 8 * A->nnz, s));
+      FFHIP_CHECK(hipGetLastError());
+      ctx->toc();
+      A->max_row = 3 * CAP;
+    } catch (...) {
+      (void)hipStreamSynchronize(s);
+      dfree(len); dfree(ovf); dfree(ndiag); dfree(ncol);
+      dfree(A->rowptr); dfree(A->colidx); dfree(A->vals); dfree(A->diag);
+      dfree(A->nodeptr); dfree(A->slot8);
+      delete A;
+      throw;
+    }
+    dfree(len); dfree(ovf); dfree(ndiag); dfree(ncol);
+    *out = A;
+  });
+}
+
+int ffhip_assemble_p2v_elasticity(ffhip_ctx ctx, ffhip_p2v sp, ffhip_mat A, double lambda,
+                                  double mu) {
+  return guard([&] {
+    FFHIP_REQUIRE(ctx && sp && A && A->p2v == sp && A->slot8, FFHIP_ERR_ARG,
+                  "ffhip_assemble_p2v_elasticity: matrix pattern was not built from this space");
+    hipStream_t s = ctx->stream;
+    ffhip_mesh m = sp->mesh;
+    double *Ke = dalloc<double>(900 * m->nt);
+    int64_t g = (m->nt + kElemWaves - 1) / kElemWaves;
+    const int64_t gmax = 8 * (int64_t)ctx->num_cu;
+    if (g > gmax) g = gmax;
+    ctx->tic();
+    k_p2v_elast_element<<<(unsigned)g, kElemBlock, 0, s>>>(m->nt, m->xyz, m->elems, lambda, mu, Ke);
+    k_p2v_gather<<<nblk(sp->nnodes, 256 / kWave), 256, 0, s>>>(sp->nnodes, A->nodeptr, sp->n2e_ptr,
+                                                              sp->n2e, A->slot8, Ke, A->vals);
+    FFHIP_CHECK(hipGetLastError());
+    ctx->toc();
+    dfree(Ke);
+  });
+}
+
+int ffhip_set_bc_p2v(ffhip_ctx ctx, ffhip_p2v sp, ffhip_mat A, const int32_t *labels, int nlab,
+                     double tgv, uint8_t *onbc_out) {
+  return guard([&] {
+    FFHIP_REQUIRE(ctx && sp && A && A->n == 3 * sp->nnodes, FFHIP_ERR_ARG,
+                  "ffhip_set_bc_p2v: bad arguments");
+    FFHIP_REQUIRE(nlab >= 0 && (nlab == 0 || labels), FFHIP_ERR_ARG, "ffhip_set_bc_p2v: labels");
+    FFHIP_REQUIRE(tgv > 0, FFHIP_ERR_UNSUPPORTED, "ffhip_set_bc_p2v: only tgv > 0 is supported");
+    uint8_t *onbc = dalloc<uint8_t>(A->n);
+    int *missing = dalloc<int>(1);
+    bool ok = false;
+    int miss = 0;
+    try {
+      ok = p2v_mark(ctx, sp, labels, nlab, onbc);
+      FFHIP_CHECK(hipMemsetAsync(missing, 0, sizeof(int), ctx->stream));
+      set_diag_rows(ctx, A->n, onbc, A->diag, tgv, A->vals, missing);
+      if (onbc_out) {
+        FFHIP_CHECK(hipMemcpyAsync(onbc_out, onbc, A->n, hipMemcpyDeviceToHost, ctx->stream));
+      }
+      miss = read_flag(ctx, missing);
+    } catch (...) {
+      (void)hipStreamSynchronize(ctx->stream);
+      dfree(onbc); dfree(missing);
+      throw;
+    }
+    dfree(onbc); dfree(missing);
+    FFHIP_REQUIRE(ok, FFHIP_ERR_INTERNAL, "ffhip_set_bc_p2v: a boundary edge is no element edge");
+    FFHIP_REQUIRE(miss == 0, FFHIP_ERR_INTERNAL, "ffhip_set_bc_p2v: a Dirichlet row has no diagonal");
+  });
+}
+
+int ffhip_rhs_p2v(ffhip_ctx ctx, ffhip_p2v sp, const double f[3], const int32_t *labels, int nlab,
+                  double tgv, const double g[3], double *b, int on_device) {
+  return guard([&] {
+    FFHIP_REQUIRE(ctx && sp && f && b, FFHIP_ERR_ARG, "ffhip_rhs_p2v: null argument");
+    FFHIP_REQUIRE(nlab >= 0 && (nlab == 0 || labels), FFHIP_ERR_ARG, "ffhip_rhs_p2v: labels");
+    hipStream_t s = ctx->stream;
+    ffhip_mesh m = sp->mesh;
+    const int64_t n = 3 * sp->nnodes;
+    const double gz[3] = {0., 0., 0.};
+    const double *gg = g ? g : gz;
+    uint8_t *onbc = dalloc<uint8_t>(n);
+    double *bd = on_device ? b : nullptr;
+    bool ok = false;
+    try {
+      ok = p2v_mark(ctx, sp, labels, nlab, onbc);
+      if (!on_device) bd = dalloc<double>(n);
+      ctx->tic();
+      k_p2v_rhs<<<nblk(sp->nnodes, 256), 256, 0, s>>>(sp->nnodes, m->xyz, m->elems, sp->n2e_ptr,
+                                                      sp->n2e, f[0], f[1], f[2], onbc, tgv, gg[0],
+                                                      gg[1], gg[2], bd);
+      FFHIP_CHECK(hipGetLastError());
+      ctx->toc();
+      if (!on_device) {
+        FFHIP_CHECK(hipMemcpyAsync(b, bd, 8 * n, hipMemcpyDeviceToHost, s));
+        FFHIP_CHECK(hipStreamSynchronize(s));
+      }
+    } catch (...) {
+      (void)hipStreamSynchronize(s);
+      dfree(onbc);
+      if (!on_device) dfree(bd);
+      throw;
+    }
+    dfree(onbc);
+    if (!on_device) dfree(bd);
+    FFHIP_REQUIRE(ok, FFHIP_ERR_INTERNAL, "ffhip_rhs_p2v: a boundary edge is no element edge");
+  });
+}
+
+}  // extern "C"

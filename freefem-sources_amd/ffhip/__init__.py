@@ -25,6 +25,7 @@ LIBPATH = os.environ.get("FFHIP_LIB") or os.path.join(os.path.dirname(HERE), "li
 
 COEF_CONST, COEF_SIN3, COEF_P0, COEF_QP = 0, 1, 2, 3
 CG_FP64, CG_MIXED = 0, 1
+CG_LAYOUT_AUTO, CG_LAYOUT_SELL, CG_LAYOUT_CSR = 0, 0x10, 0x20
 ERRORS = {-1: "bad argument", -2: "HIP error", -3: "CG did not converge", -4: "NaN in CG",
           -5: "unsupported", -6: "internal error"}
 
@@ -80,6 +81,7 @@ _SIGS = {
     "ffhip_spmv": (_i, [_vp, _vp, _vp, _vp, _i]),
     "ffhip_cg_create": (_i, [_vp, _vp, _i, _pvp]),
     "ffhip_cg_update_values": (_i, [_vp]),
+    "ffhip_cg_layout": (_i, [_vp, ctypes.POINTER(_i), _pl]),
     "ffhip_cg_solve": (_i, [_vp, _vp, _vp, _i, _d, _i, _d, _i, ctypes.POINTER(CGStats)]),
     "ffhip_cg_free": (_i, [_vp]),
     "ffhip_cg_set_profiling": (_i, [_vp, _i]),
@@ -333,10 +335,21 @@ def rhs_p1(mesh, f=1.0, labels=(), tgv=1e30, g=0.0, out=None):
 
 
 class CG:
-    def __init__(self, A, mode=CG_FP64):
+    """Jacobi-CG solver of a device matrix (SolverCG).  layout: "auto", "sell"
+    or "csr" (the streamed matrix layout; auto takes sliced ELL unless it pads
+    by more than 25 %)."""
+
+    def __init__(self, A, mode=CG_FP64, layout="auto"):
         h = ctypes.c_void_p()
-        check(lib().ffhip_cg_create(A.ctx.h, A.h, mode, ctypes.byref(h)))
+        flag = {"auto": CG_LAYOUT_AUTO, "sell": CG_LAYOUT_SELL, "csr": CG_LAYOUT_CSR}[layout]
+        check(lib().ffhip_cg_create(A.ctx.h, A.h, mode | flag, ctypes.byref(h)))
         self.h, self.A, self.mode = h, A, mode
+
+    def layout(self):
+        """(layout name, entries streamed per mat-vec)"""
+        k, e = ctypes.c_int(), ctypes.c_int64()
+        check(lib().ffhip_cg_layout(self.h, ctypes.byref(k), ctypes.byref(e)))
+        return ("sell" if k.value == CG_LAYOUT_SELL else "csr"), e.value
 
     def set_profiling(self, on=True):
         check(lib().ffhip_cg_set_profiling(self.h, int(on)))

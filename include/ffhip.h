@@ -36,6 +36,7 @@ typedef struct ffhip_ctx_s *ffhip_ctx;   /* device, stream, scratch        */
 typedef struct ffhip_mesh_s *ffhip_mesh; /* device-resident mesh            */
 typedef struct ffhip_mat_s *ffhip_mat;   /* device-resident CSR matrix      */
 typedef struct ffhip_cg_s *ffhip_cg;     /* device-resident CG solver       */
+typedef struct ffhip_p2v_s *ffhip_p2v;   /* [P2,P2,P2] space on a 3D mesh   */
 
 const char *ffhip_last_error(void);
 const char *ffhip_version(void);
@@ -142,8 +143,45 @@ int ffhip_mat_free(ffhip_mat mat);
 int ffhip_spmv(ffhip_ctx ctx, ffhip_mat mat, const double *x, double *y,
                int trans);
 
+/* ---- vector P2 linear elasticity (3D) ------------------------------------ */
+/* Vh(Th,[P2,P2,P2]): P2 nodes numbered by first appearance over the elements
+ * (4 vertices, then the 6 edges of Tet::nvedge per element), as
+ * GenericMesh::BuildDFNumbering (GenericMesh.hpp:1880-1920) does; dof of
+ * component c at node a = 3 a + c (FESpacen.cpp:403-428). */
+int ffhip_p2v_create(ffhip_ctx ctx, ffhip_mesh mesh, ffhip_p2v *space);
+int ffhip_p2v_info(ffhip_p2v space, int64_t *nnodes, int64_t *ndof);
+/* element nodes (nt x 10, host): FESpace::operator()(k, i) of the P2 space */
+int ffhip_p2v_download(ffhip_p2v space, int32_t *elem_nodes);
+int ffhip_p2v_free(ffhip_p2v space);
+/* Pattern of `matrix A = va(Vh,Vh)`: every (dof, dof) pair of each element's
+ * 30 dofs (HashMatrix::operator+=, HashMatrix.cpp:1294-1332), CSR-sorted. */
+int ffhip_pattern_p2v(ffhip_ctx ctx, ffhip_p2v space, ffhip_mat *mat);
+/* Values of int3d(Th)(lambda*div(u)*div(v) + 2*mu*(epsilon(u)'*epsilon(v)))
+ * with Tet_5: Element_Op (problem.cpp:6337-6437) with the P2 basis of
+ * TypeOfFE_P2Lagrange3d (P012_3d.cpp:199-270), accumulated in element order.
+ * Local matrices on the FP64 matrix cores. */
+int ffhip_assemble_p2v_elasticity(ffhip_ctx ctx, ffhip_p2v space, ffhip_mat mat,
+                                  double lambda, double mu);
+/* on(labels, u1=.., u2=.., u3=..): all components of the vertex and edge nodes
+ * of the labelled boundary triangles (AssembleBC, problem.cpp:10120-10164);
+ * a_ii = tgv (HashMatrix::SetBC).  onbc_out (host, ndof bytes) optional. */
+int ffhip_set_bc_p2v(ffhip_ctx ctx, ffhip_p2v space, ffhip_mat mat,
+                     const int32_t *labels, int nlab, double tgv,
+                     uint8_t *onbc_out);
+/* b = int3d(Th)(f . v) for a constant body force f[3] (Element_rhs,
+ * problem.cpp:7940-7978), with b_i = tgv * g[c] on Dirichlet dofs (g may be
+ * null for 0).  b: device pointer when on_device != 0, else host. */
+int ffhip_rhs_p2v(ffhip_ctx ctx, ffhip_p2v space, const double f[3],
+                  const int32_t *labels, int nlab, double tgv, const double g[3],
+                  double *b, int on_device);
+
 /* ---- Jacobi-preconditioned CG ------------------------------------------ */
 enum { FFHIP_CG_FP64 = 0, FFHIP_CG_MIXED = 1 };
+/* Layout of the streamed matrix, OR'ed into mode: sliced ELL with 64-row
+ * slices (one lane per row), or CSR with 16 lanes per row.  AUTO takes sliced
+ * ELL unless its slices would pad the matrix by more than 25 % (rows of very
+ * different lengths, e.g. vector P2), then CSR. */
+enum { FFHIP_CG_LAYOUT_AUTO = 0, FFHIP_CG_LAYOUT_SELL = 0x10, FFHIP_CG_LAYOUT_CSR = 0x20 };
 typedef struct {
   int iters;          /* CG iterations (inner FP32 iterations for MIXED)   */
   int converged;      /* ConjugueGradient's nret: 1 converged, 2 gCg0<1e-30, 0 no */
@@ -165,9 +203,14 @@ typedef struct {
 /* Solver setup for a matrix (SolverCG ctor + HMatVirtPrecon,
  * src/femlib/VirtualSolverCG.hpp:27-69, 157-167): tgv detection
  * (HashMatrix::gettgv HashMatrix.cpp:1340-1371), Jacobi diagonal, and the
- * device layouts the kernels stream (sliced-ELL FP64; for MIXED also the
- * FP32 Jacobi-scaled copy). */
+ * device layouts the kernels stream (FP64 values in the chosen layout; for
+ * MIXED also the FP32 Jacobi-scaled copy).  mode = FFHIP_CG_FP64 or
+ * FFHIP_CG_MIXED, optionally OR'ed with a FFHIP_CG_LAYOUT_* value. */
 int ffhip_cg_create(ffhip_ctx ctx, ffhip_mat mat, int mode, ffhip_cg *cg);
+/* Layout the solver chose (FFHIP_CG_LAYOUT_SELL or _CSR) and the number of
+ * matrix entries one mat-vec streams (padding included).  No reference
+ * counterpart (a property of this solver's device layout). */
+int ffhip_cg_layout(ffhip_cg cg, int *layout, int64_t *entries);
 /* Re-read the values of mat (same pattern) — SolverHIPCG::fac_numeric. */
 int ffhip_cg_update_values(ffhip_cg cg);
 /* Solve A x = b (SolverCG::dosolver, VirtualSolverCG.hpp:169-190, one rhs):

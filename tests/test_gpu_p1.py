@@ -103,9 +103,10 @@ def test_kappa_on_device_matches_host():
     assert np.array_equal(A1.download()[2], A2.download()[2])
 
 
+@pytest.mark.parametrize("layout", ["sell", "csr"])
 @pytest.mark.parametrize("mode", [0, 1])
 @pytest.mark.parametrize("n", [6, 20])
-def test_cg_vs_oracle(mode, n):
+def test_cg_vs_oracle(mode, n, layout):
     m, rpo, cio, vo, onbco, bo = ffo.poisson_p1_3d(n)
     xo, so = ffo.pcg(rpo, cio, vo, bo, eps=1e-10)
     M = ffhip.cube(n)
@@ -113,7 +114,8 @@ def test_cg_vs_oracle(mode, n):
     A.assemble_p1(ffhip.COEF_SIN3)
     A.set_bc(LABELS)
     b = ffhip.rhs_p1(M, 1.0, LABELS)
-    cg = ffhip.CG(A, mode)
+    cg = ffhip.CG(A, mode, layout=layout)
+    assert cg.layout()[0] == layout
     x, st = cg.solve(b, eps=1e-10)
     assert st["converged"] == 1
     assert st["true_gcg"] < 1e-20 * st["gcg0"] * (1.0 + 1e-6)
@@ -125,14 +127,15 @@ def test_cg_vs_oracle(mode, n):
     assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
 
 
-def test_cg_deterministic_and_warm_start():
+@pytest.mark.parametrize("layout", ["sell", "csr"])
+def test_cg_deterministic_and_warm_start(layout):
     n = 12
     M = ffhip.cube(n)
     A = ffhip.Matrix.p1(M)
     A.assemble_p1(ffhip.COEF_SIN3)
     A.set_bc(LABELS)
     b = ffhip.rhs_p1(M, 1.0, LABELS)
-    cg = ffhip.CG(A)
+    cg = ffhip.CG(A, layout=layout)
     x1, s1 = cg.solve(b, eps=1e-10)
     x2, s2 = cg.solve(b, eps=1e-10)
     assert np.array_equal(x1, x2) and s1["iters"] == s2["iters"]
@@ -175,12 +178,14 @@ def test_upload_path_and_spmv():
     assert np.linalg.norm(x2 - xo) <= 1e-8 * np.linalg.norm(xo)
 
 
-def test_transposed_solve():
+@pytest.mark.parametrize("layout", ["sell", "csr"])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_transposed_solve(mode, layout):
     mo = ffo.cube(6)
     ke = np.exp(0.5 * np.random.default_rng(3).standard_normal(mo["nt"]))
     m, rpo, cio, vo, onbco, bo = ffo.poisson_p1_3d(6, ffo.COEF_P0, 1.0, ke)
     A = ffhip.Matrix.upload(rpo, cio, vo)
-    x, st = ffhip.CG(A).solve(bo, eps=1e-10, trans=True)
+    x, st = ffhip.CG(A, mode, layout=layout).solve(bo, eps=1e-10, trans=True)
     xo, so = ffo.pcg(rpo, cio, vo, bo, eps=1e-10, trans=True)
     assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
 
@@ -275,3 +280,26 @@ def test_set_values_and_update():
     cg.update_values()
     x2, _ = cg.solve(bo, eps=1e-10)
     assert np.linalg.norm(2.0 * x2 - x1) <= 1e-8 * np.linalg.norm(x1)
+
+
+def test_layout_choice_and_spmv_bench_both_layouts():
+    """P1 rows have nearly equal length: AUTO streams sliced ELL with little
+    padding; both layouts' plain mat-vec kernels give A x."""
+    import torch
+    n = 10
+    M = ffhip.cube(n)
+    A = ffhip.Matrix.p1(M)
+    A.assemble_p1(ffhip.COEF_SIN3)
+    rp, ci, va = A.download()
+    nnz = int(rp[-1])
+    cg = ffhip.CG(A, ffhip.CG_MIXED)
+    lay, ents = cg.layout()
+    assert lay == "sell" and nnz <= ents <= 1.25 * nnz
+    x = np.random.default_rng(1).standard_normal(rp.size - 1)
+    y = ffo.spmv(rp, ci, va, x)
+    for layout in ("sell", "csr"):
+        c = ffhip.CG(A, ffhip.CG_MIXED, layout=layout)
+        xd = torch.tensor(x, device="cuda")
+        yd = torch.empty_like(xd)
+        c.bench_spmv(ffhip.CG_FP64, 2, xd, yd)
+        assert np.allclose(yd.cpu().numpy(), y, rtol=1e-13, atol=1e-13 * np.abs(y).max())

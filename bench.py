@@ -150,7 +150,8 @@ def main():
         t_m = time.perf_counter() - t0
         rel = ((xm - x64).norm() / x64.norm()).item()
         out = dict(nnz=A.nnz, n=A.n, t_asm=t_asm, t_pat_dev=t_pat, t_val_dev=t_val, t_bc_dev=t_bc,
-                   t_rhs=t_rhs, t_64=t_64, t_m=t_m, t_msetup=t_msetup, s64=s64, sm=sm, rel=rel)
+                   t_rhs=t_rhs, t_64=t_64, t_m=t_m, t_msetup=t_msetup, s64=s64, sm=sm, rel=rel,
+                   layout64=cg64.layout(), layoutm=cgm.layout())
         cg64.free(); cgm.free(); A.free()
         return out
 
@@ -178,6 +179,36 @@ def main():
         P.cg.free(); P.A.free(); P.mesh.free()
         return out
 
+    # the [P2,P2,P2] Lame configuration: cube(32), lambda = 1, mu = 0.5,
+    # f = (0,0,-1), clamped label 4, FP32 inner CG with FP64 refinement
+    def p2_eval(n2=32):
+        M2 = ffhip.cube(n2)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sp = ffhip.P2VSpace(M2)
+        t_num = M2.ctx.last_ms()
+        A2 = ffhip.Matrix.p2v(sp)
+        t_pat = M2.ctx.last_ms()
+        t_val = A2.assemble_p2v_elasticity(1.0, 0.5)
+        t_bc = A2.set_bc_p2v([4])
+        cgm = ffhip.CG(A2, ffhip.CG_MIXED)
+        t_asm = time.perf_counter() - t0
+        b2 = torch.empty(sp.ndof, dtype=torch.float64, device="cuda")
+        ffhip.rhs_p2v(sp, (0.0, 0.0, -1.0), [4], out=b2)
+        t0 = time.perf_counter()
+        xm, sm = cgm.solve(b2, eps=1e-10)
+        t_m = time.perf_counter() - t0
+        cg64 = ffhip.CG(A2, ffhip.CG_FP64)
+        t0 = time.perf_counter()
+        x64, s64 = cg64.solve(b2, eps=1e-10)
+        t_64 = time.perf_counter() - t0
+        out = dict(ndof=sp.ndof, nnz=A2.nnz, layout=cgm.layout()[0], t_asm=t_asm,
+                   dev=dict(numbering=t_num, pattern=t_pat, values=t_val, bc=t_bc),
+                   t_m=t_m, t_64=t_64, sm=sm, s64=s64,
+                   rel=((xm - x64).norm() / x64.norm()).item())
+        cgm.free(); cg64.free(); A2.free(); sp.free(); M2.free()
+        return out
+
     for _ in range(args.warmup):
         step(False)
     torch.cuda.synchronize()
@@ -186,6 +217,7 @@ def main():
     for _ in range(args.steps):
         r = step(True)
         r["adj"] = adjoint_eval()
+        r["p2"] = p2_eval()
         res.append(r)
     torch.cuda.synchronize()
     T = time.perf_counter() - T0
@@ -195,8 +227,13 @@ def main():
     t_asm = mean("t_asm")
     mv_ms = float(np.mean([r["s64"]["ms_matvec_avg"] for r in res]))
     mv_ms32 = float(np.mean([r["sm"]["ms_matvec_avg"] for r in res]))
-    bytes_mv = 12 * nnz + 40 * nv
-    bytes_mv32 = 8 * nnz + 16 * nv
+    # bytes a launch must move in the streamed layout: value + column index per
+    # nonzero (16-bit column deltas in "sell16"), 5 FP64 (FP32: 4 FP32) vectors
+    ib64 = 2 if res[-1]["layout64"][0] == "sell16" else 4
+    ib32 = 2 if res[-1]["layoutm"][0] == "sell16" else 4
+    bytes_mv = (8 + ib64) * nnz + 40 * nv
+    bytes_mv32 = (4 + ib32) * nnz + 16 * nv
+    csr_bytes = 12 * nnz + 40 * nv  # the same work with int32 CSR columns
     achieved = bytes_mv / (mv_ms * 1e-3) / 1e9
     s64, sm = res[-1]["s64"], res[-1]["sm"]
     copy_bytes = 1 << 30
@@ -239,6 +276,22 @@ def main():
                     "forward_s": round(float(np.mean([r["adj"]["t_forward"] for r in res])), 5),
                     "adjoint_and_gradient_s": round(float(np.mean([r["adj"]["t_adjoint_grad"] for r in res])), 5),
                     "iters_forward": res[-1]["adj"]["iters_fwd"], "iters_adjoint": res[-1]["adj"]["iters_adj"]},
+        "p2_elasticity": {
+            "workload": "cube(32) [P2,P2,P2] Lame lambda=1 mu=0.5 f=(0,0,-1), clamped label 4",
+            "ndof": res[-1]["p2"]["ndof"], "nnz": res[-1]["p2"]["nnz"],
+            "assembly_s": round(float(np.mean([r["p2"]["t_asm"] for r in res])), 5),
+            "assembly_Mnnz_per_s": round(res[-1]["p2"]["nnz"] / float(np.mean([r["p2"]["t_asm"] for r in res])) / 1e6, 1),
+            "assembly_device_ms": {k: round(float(np.mean([r["p2"]["dev"][k] for r in res])), 3)
+                                   for k in ("numbering", "pattern", "values", "bc")},
+            "solver_layout": res[-1]["p2"]["layout"],
+            "cg_mixed_s": round(float(np.mean([r["p2"]["t_m"] for r in res])), 5),
+            "cg_fp64_s": round(float(np.mean([r["p2"]["t_64"] for r in res])), 5),
+            "cg_mixed_inner_iters": res[-1]["p2"]["sm"]["iters"],
+            "cg_mixed_fp64_refinements": res[-1]["p2"]["sm"]["refinements"],
+            "cg_fp64_iters": res[-1]["p2"]["s64"]["iters"],
+            "true_relres": {"fp64": res[-1]["p2"]["s64"]["true_relres"],
+                            "mixed": res[-1]["p2"]["sm"]["true_relres"]},
+            "mixed_relerr_vs_fp64": res[-1]["p2"]["rel"]},
         "roofline": {
             "bound": "hbm",
             "achieved": round(achieved, 1),
@@ -246,9 +299,10 @@ def main():
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic() if n == 160 else None,
-            "kernel": "k_pcg_dir (FP64 SELL mat-vec + direction update + 2 dots)",
+            "kernel": f"k_pcg_dir (FP64 {res[-1]['layout64'][0]} mat-vec + direction update + 2 dots)",
             "bytes_per_launch": bytes_mv,
             "avg_launch_ms": round(mv_ms, 5),
+            "csr_int32_equivalent_GBs": round(csr_bytes / (mv_ms * 1e-3) / 1e9, 1),
             "copy_reference_GBs": round(copy_gbs, 1),
             "fp32_matvec": {"kernel": "k_mix_dir", "bytes_per_launch": bytes_mv32,
                             "avg_launch_ms": round(mv_ms32, 5),

@@ -57,6 +57,7 @@ constexpr int kU = FFHIP_SELL_U;
 // matrix layouts of the mat-vec kernels (described at MatFmt below)
 constexpr int kSell = 0;
 constexpr int kCsrV = 1;
+constexpr int kSell16 = 2;  // sliced ELL with 16-bit column deltas (col - row)
 constexpr int kLanesPerRow = 16;
 
 struct CgState {
@@ -86,19 +87,21 @@ __device__ inline T ntload(const T *p) { return __builtin_nontemporal_load(p); }
 // this lane's row.  Unrolled by U: the U column and value loads of a round
 // are all issued before any gather, so every wave keeps 2U streaming loads
 // and U (or 2U) gathers in flight instead of one dependent chain per entry.
-template <int U, class V, class ACC, class F>
-__device__ inline ACC sell_row(const int32_t *__restrict__ colp, const V *__restrict__ valp,
-                               int L, F xval) {
+// IT = int32_t: absolute columns; IT = int16_t: deltas to the row (row0 + d).
+template <int U, class V, class ACC, class IT, class F>
+__device__ inline ACC sell_row(const IT *__restrict__ colp, const V *__restrict__ valp, int L,
+                               int32_t row0, F xval) {
   ACC acc = 0;
   for (int k = 0; k < L; k += U) {
     int32_t j[U];
     V a[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      j[u] = 0;  // past the row end: column 0 with a zero value
+      j[u] = row0;  // past the row end: a valid column with a zero value
       a[u] = 0;
       if (k + u < L) {
-        j[u] = ntload(colp + (int64_t)(k + u) * kSlice);
+        const IT c = ntload(colp + (int64_t)(k + u) * kSlice);
+        j[u] = sizeof(IT) == 2 ? row0 + (int32_t)c : (int32_t)c;
         a[u] = ntload(valp + (int64_t)(k + u) * kSlice);
       }
     }
@@ -148,8 +151,8 @@ __global__ void k_sell_fill(int64_t n, int64_t S, const int64_t *__restrict__ ro
                             const int32_t *__restrict__ colidx, const double *__restrict__ vals,
                             const int64_t *__restrict__ tmap, const int64_t *__restrict__ sptr,
                             const int32_t *__restrict__ len, const double *__restrict__ dsc,
-                            int32_t *__restrict__ scol, double *__restrict__ sval,
-                            float *__restrict__ sval32) {
+                            int32_t *__restrict__ scol, int16_t *__restrict__ scol16,
+                            double *__restrict__ sval, float *__restrict__ sval32) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= S * kSlice) return;
   const int64_t s = r / kSlice;
@@ -163,14 +166,27 @@ __global__ void k_sell_fill(int64_t n, int64_t S, const int64_t *__restrict__ ro
       const int32_t c = colidx[rp + k];
       const double a = tmap ? vals[tmap[rp + k]] : vals[rp + k];
       if (scol) scol[idx] = c;
+      if (scol16) scol16[idx] = (int16_t)(c - r);
       if (sval) sval[idx] = a;
       if (sval32) sval32[idx] = (float)(a * dsc[r] * dsc[c]);
     } else {
       const int32_t c = r < n ? (int32_t)r : 0;
       if (scol) scol[idx] = c;
+      if (scol16) scol16[idx] = 0;
       if (sval) sval[idx] = 0.;
       if (sval32) sval32[idx] = 0.f;
     }
+  }
+}
+
+// flags a row with a column farther than 16 bits from the row index
+__global__ void k_far_columns(int64_t n, const int64_t *__restrict__ rowptr,
+                              const int32_t *__restrict__ colidx, int *__restrict__ far) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  for (int64_t k = rowptr[r]; k < rowptr[r + 1]; ++k) {
+    const int64_t d = (int64_t)colidx[k] - r;
+    if (d < -32768 || d > 32767) { atomicOr(far, 1); return; }
   }
 }
 
@@ -253,9 +269,10 @@ __global__ void k_init_bc(int64_t n, const uint8_t *__restrict__ wcl, const doub
 template <class V>
 struct MatFmt {
   int64_t n, S;
-  const int64_t *ptr;  // kSell: slice offsets (S+1); kCsrV: row pointers (n+1)
-  const int32_t *len;  // kSell: slice widths
-  const int32_t *col;
+  const int64_t *ptr;  // kSell*: slice offsets (S+1); kCsrV: row pointers (n+1)
+  const int32_t *len;  // kSell*: slice widths
+  const int32_t *col;  // kSell, kCsrV
+  const int16_t *col16;  // kSell16: col - row
   const V *val;
 };
 
@@ -263,15 +280,21 @@ struct MatFmt {
 // for kCsrV only the first lane of each 16-lane group calls it.
 template <int KIND, class V, class ACC, class F, class E>
 __device__ inline void matvec_rows(const MatFmt<V> &M, F xval, E epi) {
-  if (KIND == kSell) {
+  if (KIND == kSell || KIND == kSell16) {
     int64_t s0, s1;
     block_slices(M.S, s0, s1);
     const int w = wave_id(), lane = threadIdx.x % kSlice;
     for (int64_t s = s0 + w; s < s1; s += kWavesPerBlock) {
       const int64_t base = uniform(M.ptr[s]) + lane;
       const int L = __builtin_amdgcn_readfirstlane(M.len[s]);
-      const ACC acc = sell_row<kU, V, ACC>(M.col + base, M.val + base, L, xval);
       const int64_t r = s * kSlice + lane;
+      // rows past n (last slice) have only padding: their column is clamped
+      const int32_t r32 = (int32_t)(r < M.n ? r : M.n - 1);
+      ACC acc;
+      if (KIND == kSell16)
+        acc = sell_row<kU, V, ACC, int16_t>(M.col16 + base, M.val + base, L, r32, xval);
+      else
+        acc = sell_row<kU, V, ACC, int32_t>(M.col + base, M.val + base, L, r32, xval);
       if (r < M.n) epi(r, acc);
     }
   } else {
@@ -575,6 +598,7 @@ struct ffhip_cg_s {
   // kSell layout (kCsrV streams A's own rowptr/colidx)
   int64_t *sptr = nullptr;
   int32_t *slen = nullptr, *scol = nullptr;
+  int16_t *scol16 = nullptr;  // kSell16
   // values in the solver's layout; for kCsrV, sval is A's value array (borrowed)
   double *sval = nullptr;    // A
   double *svalT = nullptr;   // A^T (built on first transposed solve)
@@ -624,7 +648,8 @@ struct ffhip_cg_s {
     MatFmt<V> M;
     M.n = n;
     M.S = S;
-    if (fmt == kSell) { M.ptr = sptr; M.len = slen; M.col = scol; }
+    M.col16 = scol16;
+    if (fmt == kSell || fmt == kSell16) { M.ptr = sptr; M.len = slen; M.col = scol; }
     else { M.ptr = A->rowptr; M.len = nullptr; M.col = A->colidx; }
     M.val = val;
     return M;
@@ -635,12 +660,23 @@ struct ffhip_cg_s {
 #define FFHIP_LAUNCH_FMT(cg, kern, grid, stream, ...)                        \
   do {                                                                       \
     if ((cg)->fmt == kSell) kern<kSell><<<(grid), kBlock, 0, (stream)>>>(__VA_ARGS__); \
+    else if ((cg)->fmt == kSell16) kern<kSell16><<<(grid), kBlock, 0, (stream)>>>(__VA_ARGS__); \
     else kern<kCsrV><<<(grid), kBlock, 0, (stream)>>>(__VA_ARGS__);           \
+  } while (0)
+
+#define FFHIP_LAUNCH_MATVEC(cg, T, M, x, y, stream)                                          \
+  do {                                                                                      \
+    if ((cg)->fmt == kSell)                                                                 \
+      k_matvec<kSell, T, T><<<(cg)->grid, kBlock, 0, (stream)>>>(M, (const T *)(x), (T *)(y)); \
+    else if ((cg)->fmt == kSell16)                                                          \
+      k_matvec<kSell16, T, T><<<(cg)->grid, kBlock, 0, (stream)>>>(M, (const T *)(x), (T *)(y)); \
+    else                                                                                    \
+      k_matvec<kCsrV, T, T><<<(cg)->grid, kBlock, 0, (stream)>>>(M, (const T *)(x), (T *)(y)); \
   } while (0)
 
 // Layout choice: sliced ELL unless its slices pad the matrix by more than 25 %
 // (rows of very different lengths), then CSR with 16 lanes per row.
-static void cg_plan_layout(ffhip_cg cg, int layout) {
+static void cg_plan_layout(ffhip_cg cg, int layout, bool index32) {
   ffhip_ctx ctx = cg->ctx;
   ffhip_mat A = cg->A;
   hipStream_t s = ctx->stream;
@@ -655,8 +691,23 @@ static void cg_plan_layout(ffhip_cg cg, int layout) {
     cg->ents = read_i64(ctx, cg->sptr + cg->S);
     dfree(cnt);
     if (layout == FFHIP_CG_LAYOUT_SELL || 4 * cg->ents <= 5 * A->nnz) {
-      cg->fmt = kSell;
-      cg->scol = dalloc<int32_t>(cg->ents);
+      // 16-bit column deltas when every column is within 2^15 of its row
+      int far = 1;
+      if (!index32) {
+        int *d = dalloc<int>(1);
+        FFHIP_CHECK(hipMemsetAsync(d, 0, sizeof(int), s));
+        k_far_columns<<<nblk(A->n, 256), 256, 0, s>>>(A->n, A->rowptr, A->colidx, d);
+        FFHIP_CHECK(hipGetLastError());
+        far = read_flag(ctx, d);
+        dfree(d);
+      }
+      if (far) {
+        cg->fmt = kSell;
+        cg->scol = dalloc<int32_t>(cg->ents);
+      } else {
+        cg->fmt = kSell16;
+        cg->scol16 = dalloc<int16_t>(cg->ents);
+      }
     } else {
       dfree(cg->sptr); dfree(cg->slen);
       cg->sptr = nullptr; cg->slen = nullptr;
@@ -666,7 +717,7 @@ static void cg_plan_layout(ffhip_cg cg, int layout) {
     cg->fmt = kCsrV;
   }
   int64_t g;
-  if (cg->fmt == kSell) {
+  if (cg->fmt != kCsrV) {
     g = (cg->S + kWavesPerBlock - 1) / kWavesPerBlock;
   } else {
     cg->ents = A->nnz;
@@ -696,11 +747,12 @@ static void cg_fill_values(ffhip_cg cg, bool transposed) {
   double *&dv = transposed ? cg->svalT : cg->sval;
   float *&fv = transposed ? cg->sval32T : cg->sval32;
   if (mixed && !fv) fv = dalloc<float>(cg->ents);
-  if (cg->fmt == kSell) {
+  if (cg->fmt != kCsrV) {
     if (!dv) dv = dalloc<double>(cg->ents);
     k_sell_fill<<<nblk(cg->S * kSlice, 256), 256, 0, s>>>(
         A->n, cg->S, A->rowptr, A->colidx, A->vals, tmap, cg->sptr, cg->slen, cg->dsc,
-        transposed ? nullptr : cg->scol, dv, mixed ? fv : nullptr);
+        transposed ? nullptr : cg->scol, transposed ? nullptr : cg->scol16, dv,
+        mixed ? fv : nullptr);
   } else {
     double *dst = nullptr;
     if (transposed) {
@@ -857,7 +909,8 @@ int ffhip_cg_solve_mixed(ffhip_cg cg, const double *sval, const float *sval32, d
 static void cg_release(ffhip_cg cg) {
   for (hipEvent_t e : cg->ev) (void)hipEventDestroy(e);
   if (cg->fmt == kCsrV) cg->sval = nullptr;  // A's array
-  dfree(cg->sptr); dfree(cg->slen); dfree(cg->scol); dfree(cg->sval); dfree(cg->svalT);
+  dfree(cg->sptr); dfree(cg->slen); dfree(cg->scol); dfree(cg->scol16); dfree(cg->sval);
+  dfree(cg->svalT);
   dfree(cg->sval32); dfree(cg->sval32T); dfree(cg->dinv); dfree(cg->dsc); dfree(cg->wcl);
   dfree(cg->xw); dfree(cg->bw); dfree(cg->G); dfree(cg->CG); dfree(cg->H[0]); dfree(cg->H[1]);
   dfree(cg->AH); dfree(cg->r32); dfree(cg->q32); dfree(cg->y32); dfree(cg->p32[0]);
@@ -871,7 +924,9 @@ int ffhip_cg_create(ffhip_ctx ctx, ffhip_mat A, int mode, ffhip_cg *out) {
   return guard([&] {
     FFHIP_REQUIRE(ctx && A && out, FFHIP_ERR_ARG, "ffhip_cg_create: null argument");
     const int prec = mode & 0xf, layout = mode & 0x30;
-    FFHIP_REQUIRE((prec == FFHIP_CG_FP64 || prec == FFHIP_CG_MIXED) && (mode & ~0x3f) == 0 &&
+    const bool index32 = (mode & FFHIP_CG_INDEX32) != 0;
+    FFHIP_REQUIRE((prec == FFHIP_CG_FP64 || prec == FFHIP_CG_MIXED) &&
+                      (mode & ~(0x3f | FFHIP_CG_INDEX32)) == 0 &&
                       layout != (FFHIP_CG_LAYOUT_SELL | FFHIP_CG_LAYOUT_CSR),
                   FFHIP_ERR_ARG, "ffhip_cg_create: mode %d", mode);
     ffhip_cg cg = new ffhip_cg_s();
@@ -882,7 +937,7 @@ int ffhip_cg_create(ffhip_ctx ctx, ffhip_mat A, int mode, ffhip_cg *out) {
       cg->dsc = dalloc<double>(n);
       cg->wcl = dalloc<uint8_t>(n);
       cg_setup_jacobi(cg);
-      cg_plan_layout(cg, layout);
+      cg_plan_layout(cg, layout, index32);
       cg_fill_values(cg, false);
       cg->xw = dalloc<double>(n); cg->bw = dalloc<double>(n);
       cg->G = dalloc<double>(n); cg->CG = dalloc<double>(n);
@@ -909,7 +964,9 @@ int ffhip_cg_create(ffhip_ctx ctx, ffhip_mat A, int mode, ffhip_cg *out) {
 int ffhip_cg_layout(ffhip_cg cg, int *layout, int64_t *entries) {
   return guard([&] {
     FFHIP_REQUIRE(cg, FFHIP_ERR_ARG, "ffhip_cg_layout: null solver");
-    if (layout) *layout = cg->fmt == kSell ? FFHIP_CG_LAYOUT_SELL : FFHIP_CG_LAYOUT_CSR;
+    if (layout)
+      *layout = cg->fmt == kCsrV ? FFHIP_CG_LAYOUT_CSR
+                                 : (FFHIP_CG_LAYOUT_SELL | (cg->fmt == kSell16 ? FFHIP_CG_INDEX16 : 0));
     if (entries) *entries = cg->ents;
   });
 }
@@ -1022,16 +1079,10 @@ int ffhip_bench_spmv(ffhip_cg cg, int mode, int reps, const void *x, void *y, do
     auto launch = [&] {
       if (mode == FFHIP_CG_FP64) {
         const MatFmt<double> M = cg->fmt_of<double>(cg->sval);
-        if (cg->fmt == kSell)
-          k_matvec<kSell, double, double><<<cg->grid, kBlock, 0, s>>>(M, (const double *)x, (double *)y);
-        else
-          k_matvec<kCsrV, double, double><<<cg->grid, kBlock, 0, s>>>(M, (const double *)x, (double *)y);
+        FFHIP_LAUNCH_MATVEC(cg, double, M, x, y, s);
       } else {
         const MatFmt<float> M = cg->fmt_of<float>(cg->sval32);
-        if (cg->fmt == kSell)
-          k_matvec<kSell, float, float><<<cg->grid, kBlock, 0, s>>>(M, (const float *)x, (float *)y);
-        else
-          k_matvec<kCsrV, float, float><<<cg->grid, kBlock, 0, s>>>(M, (const float *)x, (float *)y);
+        FFHIP_LAUNCH_MATVEC(cg, float, M, x, y, s);
       }
     };
     launch();  // warm

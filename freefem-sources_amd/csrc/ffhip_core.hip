@@ -240,7 +240,9 @@ int ffhip_bench_copy(ffhip_ctx ctx, int64_t nbytes, int reps, double *avg_ms) {
     const int64_t n16 = nbytes / 16;
     v4i *a = dalloc<v4i>(n16), *b = dalloc<v4i>(n16);
     FFHIP_CHECK(hipMemsetAsync(a, 0, 16 * n16, ctx->stream));
-    const unsigned grid = 32 * (unsigned)ctx->num_cu;  // 8 blocks of 256 per SIMD group
+    // 4 blocks of 256 per CU: the fastest plain copy in a sweep of grid sizes
+    // (4..32 blocks per CU) and unroll depths on MI355X
+    const unsigned grid = 4 * (unsigned)ctx->num_cu;
     k_copy<<<grid, 256, 0, ctx->stream>>>(n16, a, b);
     FFHIP_CHECK(hipGetLastError());
     ctx->tic();

@@ -208,8 +208,9 @@ __global__ __launch_bounds__(kElemBlock) void k_p2v_elast_element(
     rvalid[I] = r < 30;
     const int i = r < 30 ? r / 3 : 0;
     R[I].vert = i < 4;
-    R[I].i0 = i < 4 ? i : c_edge[i - 4][0];
-    R[I].i1 = i < 4 ? i : c_edge[i - 4][1];
+    const int ed = i < 4 ? 0 : i - 4;
+    R[I].i0 = i < 4 ? i : c_edge[ed][0];
+    R[I].i1 = i < 4 ? i : c_edge[ed][1];
   }
   double* M = sM[w];
   for (int64_t base = (int64_t)blockIdx.x * kElemWaves; base < nt; base += (int64_t)gridDim.x * kElemWaves) {
@@ -249,10 +250,11 @@ __global__ __launch_bounds__(kElemBlock) void k_p2v_elast_element(
         acc01 = __builtin_amdgcn_mfma_f64_16x16x4f64(W * g0, g1, acc01, 0, 0, 0);
         acc11 = __builtin_amdgcn_mfma_f64_16x16x4f64(W * g1, g1, acc11, 0, 0, 0);
       }
-      // D[i][j] of a 16x16 tile: j = lane % 16, i = 4 (lane / 16) + v
+      // D[i][j] of a 16x16 tile in lane l, register v: j = l % 16, i = l / 16 + 4 v
+      // (A[i][k] and B[k][j] are taken from lane i + 16 k and j + 16 k)
 #pragma unroll
       for (int vv = 0; vv < 4; ++vv) {
-        const int i = 4 * kq + vv, j = col;
+        const int i = kq + 4 * vv, j = col;
         M[i * kMs + j] = acc00[vv];
         M[i * kMs + 16 + j] = acc01[vv];
         M[(16 + j) * kMs + i] = acc01[vv];
@@ -384,7 +386,8 @@ __global__ void k_p2v_rhs(int64_t nn, const double *__restrict__ xyz, const int3
     for (int t = 0; t < 4; ++t)
       for (int d = 0; d < 3; ++d) v[t][d] = xyz[3 * (int64_t)tets[4 * e + t] + d];
     const double mes = tet_mesure(v);
-    const int i0 = i < 4 ? i : c_edge[i - 4][0], i1 = i < 4 ? i : c_edge[i - 4][1];
+    const int ed = i < 4 ? 0 : i - 4;
+    const int i0 = i < 4 ? i : c_edge[ed][0], i1 = i < 4 ? i : c_edge[ed][1];
     for (int q = 0; q < 14; ++q) {
       const double px = c_qf5_pts[q][0], py = c_qf5_pts[q][1], pz = c_qf5_pts[q][2];
       const double l[4] = {1. - (px + py + pz), px, py, pz};

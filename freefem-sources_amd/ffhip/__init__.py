@@ -26,6 +26,7 @@ LIBPATH = os.environ.get("FFHIP_LIB") or os.path.join(os.path.dirname(HERE), "li
 COEF_CONST, COEF_SIN3, COEF_P0, COEF_QP = 0, 1, 2, 3
 CG_FP64, CG_MIXED = 0, 1
 CG_LAYOUT_AUTO, CG_LAYOUT_SELL, CG_LAYOUT_CSR = 0, 0x10, 0x20
+CG_INDEX16, CG_INDEX32 = 0x40, 0x80
 ERRORS = {-1: "bad argument", -2: "HIP error", -3: "CG did not converge", -4: "NaN in CG",
           -5: "unsupported", -6: "internal error"}
 
@@ -90,6 +91,14 @@ _SIGS = {
     "ffhip_bench_copy": (_i, [_vp, _l, _i, ctypes.POINTER(_d)]),
     "ffhip_p1_misfit": (_i, [_vp, _vp, _vp, _vp, _vp, _i, ctypes.POINTER(_d), _vp]),
     "ffhip_p1_grad_kappa": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "ffhip_p2v_create": (_i, [_vp, _vp, _pvp]),
+    "ffhip_p2v_info": (_i, [_vp, _pl, _pl]),
+    "ffhip_p2v_download": (_i, [_vp, _vp]),
+    "ffhip_p2v_free": (_i, [_vp]),
+    "ffhip_pattern_p2v": (_i, [_vp, _vp, _pvp]),
+    "ffhip_assemble_p2v_elasticity": (_i, [_vp, _vp, _vp, _d, _d]),
+    "ffhip_set_bc_p2v": (_i, [_vp, _vp, _vp, _vp, _i, _d, _vp]),
+    "ffhip_rhs_p2v": (_i, [_vp, _vp, _vp, _vp, _i, _d, _vp, _vp, _i]),
 }
 
 _LIB = None
@@ -270,6 +279,27 @@ class Matrix:
                                      _hp(vals), ctypes.byref(h)))
         return Matrix(h, ctx)
 
+    @staticmethod
+    def p2v(space):
+        """pattern of `matrix A = va(Vh,Vh)` on Vh(Th,[P2,P2,P2])"""
+        h = ctypes.c_void_p()
+        check(lib().ffhip_pattern_p2v(space.ctx.h, space.h, ctypes.byref(h)))
+        A = Matrix(h, space.ctx, space.mesh)
+        A.space = space
+        return A
+
+    def assemble_p2v_elasticity(self, lam=1.0, mu=0.5):
+        """int3d(lambda div(u) div(v) + 2 mu epsilon(u):epsilon(v)) values"""
+        check(lib().ffhip_assemble_p2v_elasticity(self.ctx.h, self.space.h, self.h, lam, mu))
+        return self.ctx.last_ms()
+
+    def set_bc_p2v(self, labels, tgv=1e30, want_onbc=False):
+        lab = np.asarray(labels, np.int32)
+        onbc = np.zeros(self.n, np.uint8) if want_onbc else None
+        check(lib().ffhip_set_bc_p2v(self.ctx.h, self.space.h, self.h, _hp(lab), lab.size, tgv,
+                                     _hp(onbc)))
+        return onbc if want_onbc else self.ctx.last_ms()
+
     def assemble_p1(self, kind=COEF_SIN3, c0=1.0, kappa_e=None):
         """kappa_e: numpy (host) or torch cuda tensor (device) of nt doubles"""
         on_dev = 0
@@ -334,22 +364,70 @@ def rhs_p1(mesh, f=1.0, labels=(), tgv=1e30, g=0.0, out=None):
     return out
 
 
+class P2VSpace:
+    """Vh(Th,[P2,P2,P2]) on a device tetrahedral mesh"""
+
+    def __init__(self, mesh):
+        h = ctypes.c_void_p()
+        check(lib().ffhip_p2v_create(mesh.ctx.h, mesh.h, ctypes.byref(h)))
+        self.h, self.mesh, self.ctx = h, mesh, mesh.ctx
+        nn, nd = _l(), _l()
+        check(lib().ffhip_p2v_info(h, ctypes.byref(nn), ctypes.byref(nd)))
+        self.nnodes, self.ndof = nn.value, nd.value
+
+    def elem_nodes(self):
+        en = np.zeros(10 * self.mesh.nt, np.int32)
+        check(lib().ffhip_p2v_download(self.h, _hp(en)))
+        return en
+
+    def free(self):
+        if self.h:
+            lib().ffhip_p2v_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def rhs_p2v(space, f=(0.0, 0.0, -1.0), labels=(), tgv=1e30, g=None, out=None):
+    """b of int3d(f'*v) + on(labels, u=g) on [P2,P2,P2].  out: torch cuda
+    tensor (device) or None (returns a numpy array)."""
+    lab = np.asarray(labels, np.int32)
+    fv = np.ascontiguousarray(f, np.float64)
+    gv = None if g is None else np.ascontiguousarray(g, np.float64)
+    if out is None:
+        b = np.zeros(space.ndof)
+        check(lib().ffhip_rhs_p2v(space.ctx.h, space.h, _hp(fv), _hp(lab), lab.size, tgv, _hp(gv),
+                                  _hp(b), 0))
+        return b
+    check(lib().ffhip_rhs_p2v(space.ctx.h, space.h, _hp(fv), _hp(lab), lab.size, tgv, _hp(gv),
+                              _dp(out), 1))
+    return out
+
+
 class CG:
     """Jacobi-CG solver of a device matrix (SolverCG).  layout: "auto", "sell"
-    or "csr" (the streamed matrix layout; auto takes sliced ELL unless it pads
-    by more than 25 %)."""
+    (sliced ELL, 16-bit column deltas where they fit), "sell32" (sliced ELL,
+    32-bit columns) or "csr" (16 lanes per row); auto takes sliced ELL unless
+    it pads by more than 25 %."""
 
     def __init__(self, A, mode=CG_FP64, layout="auto"):
         h = ctypes.c_void_p()
-        flag = {"auto": CG_LAYOUT_AUTO, "sell": CG_LAYOUT_SELL, "csr": CG_LAYOUT_CSR}[layout]
+        flag = {"auto": CG_LAYOUT_AUTO, "sell": CG_LAYOUT_SELL,
+                "sell32": CG_LAYOUT_SELL | CG_INDEX32, "csr": CG_LAYOUT_CSR}[layout]
         check(lib().ffhip_cg_create(A.ctx.h, A.h, mode | flag, ctypes.byref(h)))
         self.h, self.A, self.mode = h, A, mode
 
     def layout(self):
-        """(layout name, entries streamed per mat-vec)"""
+        """(layout name "sell16" | "sell" | "csr", entries streamed per mat-vec)"""
         k, e = ctypes.c_int(), ctypes.c_int64()
         check(lib().ffhip_cg_layout(self.h, ctypes.byref(k), ctypes.byref(e)))
-        return ("sell" if k.value == CG_LAYOUT_SELL else "csr"), e.value
+        if k.value & CG_LAYOUT_SELL:
+            return ("sell16" if k.value & CG_INDEX16 else "sell"), e.value
+        return "csr", e.value
 
     def set_profiling(self, on=True):
         check(lib().ffhip_cg_set_profiling(self.h, int(on)))

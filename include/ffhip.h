@@ -182,6 +182,11 @@ enum { FFHIP_CG_FP64 = 0, FFHIP_CG_MIXED = 1 };
  * ELL unless its slices would pad the matrix by more than 25 % (rows of very
  * different lengths, e.g. vector P2), then CSR. */
 enum { FFHIP_CG_LAYOUT_AUTO = 0, FFHIP_CG_LAYOUT_SELL = 0x10, FFHIP_CG_LAYOUT_CSR = 0x20 };
+/* Sliced ELL streams 16-bit column deltas (column - row) when every column is
+ * within 2^15 of its row, else 32-bit columns; FFHIP_CG_INDEX32 in mode forces
+ * 32-bit columns.  ffhip_cg_layout reports FFHIP_CG_INDEX16 with the layout
+ * when the deltas are used. */
+enum { FFHIP_CG_INDEX16 = 0x40, FFHIP_CG_INDEX32 = 0x80 };
 typedef struct {
   int iters;          /* CG iterations (inner FP32 iterations for MIXED)   */
   int converged;      /* ConjugueGradient's nret: 1 converged, 2 gCg0<1e-30, 0 no */
@@ -207,7 +212,8 @@ typedef struct {
  * MIXED also the FP32 Jacobi-scaled copy).  mode = FFHIP_CG_FP64 or
  * FFHIP_CG_MIXED, optionally OR'ed with a FFHIP_CG_LAYOUT_* value. */
 int ffhip_cg_create(ffhip_ctx ctx, ffhip_mat mat, int mode, ffhip_cg *cg);
-/* Layout the solver chose (FFHIP_CG_LAYOUT_SELL or _CSR) and the number of
+/* Layout the solver chose (FFHIP_CG_LAYOUT_SELL, possibly | FFHIP_CG_INDEX16,
+ * or FFHIP_CG_LAYOUT_CSR) and the number of
  * matrix entries one mat-vec streams (padding included).  No reference
  * counterpart (a property of this solver's device layout). */
 int ffhip_cg_layout(ffhip_cg cg, int *layout, int64_t *entries);

@@ -296,10 +296,11 @@ def p2v_dofs(m):
     return nn, en, ed
 
 
-def elasticity_p2(n, lam=1.0, mu=0.5, f=(0.0, 0.0, -1.0), labels=(4,), tgv=1e30):
-    """cube(n) [P2,P2,P2] Lame system as `matrix A = va(Vh,Vh)` builds it:
-    pattern, values, tgv rows (clamped labels), body-force right-hand side."""
-    m = cube(n)
+def elasticity_p2(n, lam=1.0, mu=0.5, f=(0.0, 0.0, -1.0), labels=(4,), tgv=1e30, mesh=None):
+    """cube(n) (or `mesh`, a dict like cube's) [P2,P2,P2] Lame system as
+    `matrix A = va(Vh,Vh)` builds it: pattern, values, tgv rows (clamped
+    labels), body-force right-hand side."""
+    m = cube(n) if mesh is None else mesh
     nn, en, ed = p2v_dofs(m)
     ndof = 3 * nn
     rp, ci = pattern(ndof, ed, 30)

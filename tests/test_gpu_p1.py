@@ -103,7 +103,7 @@ def test_kappa_on_device_matches_host():
     assert np.array_equal(A1.download()[2], A2.download()[2])
 
 
-@pytest.mark.parametrize("layout", ["sell", "csr"])
+@pytest.mark.parametrize("layout", ["sell", "sell32", "csr"])
 @pytest.mark.parametrize("mode", [0, 1])
 @pytest.mark.parametrize("n", [6, 20])
 def test_cg_vs_oracle(mode, n, layout):
@@ -115,7 +115,7 @@ def test_cg_vs_oracle(mode, n, layout):
     A.set_bc(LABELS)
     b = ffhip.rhs_p1(M, 1.0, LABELS)
     cg = ffhip.CG(A, mode, layout=layout)
-    assert cg.layout()[0] == layout
+    assert cg.layout()[0] == {"sell": "sell16", "sell32": "sell", "csr": "csr"}[layout]
     x, st = cg.solve(b, eps=1e-10)
     assert st["converged"] == 1
     assert st["true_gcg"] < 1e-20 * st["gcg0"] * (1.0 + 1e-6)
@@ -127,7 +127,7 @@ def test_cg_vs_oracle(mode, n, layout):
     assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
 
 
-@pytest.mark.parametrize("layout", ["sell", "csr"])
+@pytest.mark.parametrize("layout", ["sell", "sell32", "csr"])
 def test_cg_deterministic_and_warm_start(layout):
     n = 12
     M = ffhip.cube(n)
@@ -178,7 +178,7 @@ def test_upload_path_and_spmv():
     assert np.linalg.norm(x2 - xo) <= 1e-8 * np.linalg.norm(xo)
 
 
-@pytest.mark.parametrize("layout", ["sell", "csr"])
+@pytest.mark.parametrize("layout", ["sell", "sell32", "csr"])
 @pytest.mark.parametrize("mode", [0, 1])
 def test_transposed_solve(mode, layout):
     mo = ffo.cube(6)
@@ -294,12 +294,38 @@ def test_layout_choice_and_spmv_bench_both_layouts():
     nnz = int(rp[-1])
     cg = ffhip.CG(A, ffhip.CG_MIXED)
     lay, ents = cg.layout()
-    assert lay == "sell" and nnz <= ents <= 1.25 * nnz
+    assert lay == "sell16" and nnz <= ents <= 1.25 * nnz
     x = np.random.default_rng(1).standard_normal(rp.size - 1)
     y = ffo.spmv(rp, ci, va, x)
-    for layout in ("sell", "csr"):
+    for layout in ("sell", "sell32", "csr"):
         c = ffhip.CG(A, ffhip.CG_MIXED, layout=layout)
         xd = torch.tensor(x, device="cuda")
         yd = torch.empty_like(xd)
         c.bench_spmv(ffhip.CG_FP64, 2, xd, yd)
         assert np.allclose(yd.cpu().numpy(), y, rtol=1e-13, atol=1e-13 * np.abs(y).max())
+
+
+def test_far_columns_use_32bit_layout():
+    """Vertex ids relabelled at random (cube(32): 35,937 vertices): columns lie
+    farther than 2^15 from their rows, so the solver may not stream 16-bit
+    deltas; the solution is the relabelled solution of the cube mesh."""
+    m = ffo.cube(32)
+    rng = np.random.default_rng(4)
+    perm = rng.permutation(m["nv"]).astype(np.int32)  # old -> new
+    xyz = np.zeros_like(m["xyz"]).reshape(-1, 3)
+    xyz[perm] = m["xyz"].reshape(-1, 3)
+    tets = perm[m["tets"]].astype(np.int32)
+    btri = perm[m["btri"]].astype(np.int32)
+    outs = []
+    for (X, T, B) in ((m["xyz"], m["tets"], m["btri"]), (xyz.ravel(), tets, btri)):
+        M = ffhip.mesh3_upload(X, T, B, m["blab"])
+        A = ffhip.Matrix.p1(M)
+        A.assemble_p1(ffhip.COEF_SIN3)
+        A.set_bc(LABELS)
+        b = ffhip.rhs_p1(M, 1.0, LABELS)
+        cg = ffhip.CG(A, ffhip.CG_MIXED)
+        outs.append((cg.layout()[0], cg.solve(b, eps=1e-10)))
+    assert outs[0][0] == "sell16" and outs[1][0] != "sell16"
+    x0, x1 = outs[0][1][0], outs[1][1][0]
+    assert outs[1][1][1]["converged"] == 1
+    assert np.linalg.norm(x1[perm] - x0) <= 1e-8 * np.linalg.norm(x0)

@@ -114,6 +114,42 @@ __device__ inline ACC sell_row(const IT *__restrict__ colp, const V *__restrict_
   return acc;
 }
 
+// 16-bit column deltas packed four per lane: the deltas of entries 4g..4g+3 of
+// a lane's row are one 8-byte load (layout [g][lane][4] inside a slice), so a
+// round of U entries issues U/4 index loads ahead of the U value loads.
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+template <int U, class V, class ACC, class F>
+__device__ inline ACC sell_row16(const int16_t *__restrict__ colp, const V *__restrict__ valp,
+                                 int L, int32_t row0, F xval) {
+  static_assert(U % 4 == 0, "unroll by whole index groups");
+  ACC acc = 0;
+  for (int k = 0; k < L; k += U) {
+    int32_t j[U];
+    V a[U];
+#pragma unroll
+    for (int g = 0; g < U / 4; ++g) {
+      u32x2 pk = {0u, 0u};  // past the row end: deltas 0 (the row itself)
+      if (k + 4 * g < L)
+        pk = ntload(reinterpret_cast<const u32x2 *>(colp + (int64_t)((k >> 2) + g) * (4 * kSlice)));
+      j[4 * g + 0] = row0 + (int32_t)(int16_t)(pk.x & 0xffffu);
+      j[4 * g + 1] = row0 + (int32_t)(int16_t)(pk.x >> 16);
+      j[4 * g + 2] = row0 + (int32_t)(int16_t)(pk.y & 0xffffu);
+      j[4 * g + 3] = row0 + (int32_t)(int16_t)(pk.y >> 16);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      a[u] = 0;
+      if (k + u < L) a[u] = ntload(valp + (int64_t)(k + u) * kSlice);
+    }
+    ACC xv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) xv[u] = xval(j[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u) acc += (ACC)a[u] * xv[u];
+  }
+  return acc;
+}
+
 // wave-uniform slice index (kept in scalar registers)
 __device__ inline int64_t uniform(int64_t v) {
   const int lo = __builtin_amdgcn_readfirstlane((int)(v & 0xffffffff));
@@ -151,8 +187,9 @@ __global__ void k_sell_fill(int64_t n, int64_t S, const int64_t *__restrict__ ro
                             const int32_t *__restrict__ colidx, const double *__restrict__ vals,
                             const int64_t *__restrict__ tmap, const int64_t *__restrict__ sptr,
                             const int32_t *__restrict__ len, const double *__restrict__ dsc,
-                            int32_t *__restrict__ scol, int16_t *__restrict__ scol16,
-                            double *__restrict__ sval, float *__restrict__ sval32) {
+                            int32_t *__restrict__ scol, const int64_t *__restrict__ sptr16,
+                            int16_t *__restrict__ scol16, double *__restrict__ sval,
+                            float *__restrict__ sval32) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= S * kSlice) return;
   const int64_t s = r / kSlice;
@@ -166,17 +203,27 @@ __global__ void k_sell_fill(int64_t n, int64_t S, const int64_t *__restrict__ ro
       const int32_t c = colidx[rp + k];
       const double a = tmap ? vals[tmap[rp + k]] : vals[rp + k];
       if (scol) scol[idx] = c;
-      if (scol16) scol16[idx] = (int16_t)(c - r);
       if (sval) sval[idx] = a;
       if (sval32) sval32[idx] = (float)(a * dsc[r] * dsc[c]);
     } else {
       const int32_t c = r < n ? (int32_t)r : 0;
       if (scol) scol[idx] = c;
-      if (scol16) scol16[idx] = 0;
       if (sval) sval[idx] = 0.;
       if (sval32) sval32[idx] = 0.f;
     }
   }
+  if (scol16) {  // [g][lane][4] packed deltas, the group tail padded with 0
+    const int L4 = (L + 3) & ~3;
+    int16_t *dst = scol16 + sptr16[s] + 4 * lane;
+    for (int k = 0; k < L4; ++k)
+      dst[(k >> 2) * (4 * kSlice) + (k & 3)] = k < rl ? (int16_t)(colidx[rp + k] - r) : (int16_t)0;
+  }
+}
+
+// slice sizes of the packed 16-bit index array (widths rounded up to 4)
+__global__ void k_sell_len4(int64_t S, const int32_t *__restrict__ len, int64_t *__restrict__ cnt) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < S) cnt[s] = (int64_t)((len[s] + 3) & ~3) * kSlice;
 }
 
 // flags a row with a column farther than 16 bits from the row index
@@ -272,7 +319,8 @@ struct MatFmt {
   const int64_t *ptr;  // kSell*: slice offsets (S+1); kCsrV: row pointers (n+1)
   const int32_t *len;  // kSell*: slice widths
   const int32_t *col;  // kSell, kCsrV
-  const int16_t *col16;  // kSell16: col - row
+  const int16_t *col16;  // kSell16: col - row, packed [group of 4][lane][4] per slice
+  const int64_t *ptr16;  // kSell16: slice offsets of col16
   const V *val;
 };
 
@@ -292,7 +340,8 @@ __device__ inline void matvec_rows(const MatFmt<V> &M, F xval, E epi) {
       const int32_t r32 = (int32_t)(r < M.n ? r : M.n - 1);
       ACC acc;
       if (KIND == kSell16)
-        acc = sell_row<kU, V, ACC, int16_t>(M.col16 + base, M.val + base, L, r32, xval);
+        acc = sell_row16<kU, V, ACC>(M.col16 + uniform(M.ptr16[s]) + 4 * lane, M.val + base, L, r32,
+                                     xval);
       else
         acc = sell_row<kU, V, ACC, int32_t>(M.col + base, M.val + base, L, r32, xval);
       if (r < M.n) epi(r, acc);
@@ -599,6 +648,7 @@ struct ffhip_cg_s {
   int64_t *sptr = nullptr;
   int32_t *slen = nullptr, *scol = nullptr;
   int16_t *scol16 = nullptr;  // kSell16
+  int64_t *sptr16 = nullptr;  // kSell16 slice offsets of scol16
   // values in the solver's layout; for kCsrV, sval is A's value array (borrowed)
   double *sval = nullptr;    // A
   double *svalT = nullptr;   // A^T (built on first transposed solve)
@@ -649,6 +699,7 @@ struct ffhip_cg_s {
     M.n = n;
     M.S = S;
     M.col16 = scol16;
+    M.ptr16 = sptr16;
     if (fmt == kSell || fmt == kSell16) { M.ptr = sptr; M.len = slen; M.col = scol; }
     else { M.ptr = A->rowptr; M.len = nullptr; M.col = A->colidx; }
     M.val = val;
@@ -706,7 +757,14 @@ static void cg_plan_layout(ffhip_cg cg, int layout, bool index32) {
         cg->scol = dalloc<int32_t>(cg->ents);
       } else {
         cg->fmt = kSell16;
-        cg->scol16 = dalloc<int16_t>(cg->ents);
+        cg->sptr16 = dalloc<int64_t>(cg->S + 1);
+        int64_t *cnt4 = dalloc<int64_t>(cg->S + 1);
+        FFHIP_CHECK(hipMemsetAsync(cnt4 + cg->S, 0, 8, s));
+        k_sell_len4<<<nblk(cg->S, 256), 256, 0, s>>>(cg->S, cg->slen, cnt4);
+        exclusive_scan_i64(ctx, cnt4, cg->sptr16, cg->S + 1);
+        const int64_t ents16 = read_i64(ctx, cg->sptr16 + cg->S);
+        dfree(cnt4);
+        cg->scol16 = dalloc<int16_t>(ents16);
       }
     } else {
       dfree(cg->sptr); dfree(cg->slen);
@@ -751,7 +809,7 @@ static void cg_fill_values(ffhip_cg cg, bool transposed) {
     if (!dv) dv = dalloc<double>(cg->ents);
     k_sell_fill<<<nblk(cg->S * kSlice, 256), 256, 0, s>>>(
         A->n, cg->S, A->rowptr, A->colidx, A->vals, tmap, cg->sptr, cg->slen, cg->dsc,
-        transposed ? nullptr : cg->scol, transposed ? nullptr : cg->scol16, dv,
+        transposed ? nullptr : cg->scol, cg->sptr16, transposed ? nullptr : cg->scol16, dv,
         mixed ? fv : nullptr);
   } else {
     double *dst = nullptr;
@@ -909,7 +967,7 @@ int ffhip_cg_solve_mixed(ffhip_cg cg, const double *sval, const float *sval32, d
 static void cg_release(ffhip_cg cg) {
   for (hipEvent_t e : cg->ev) (void)hipEventDestroy(e);
   if (cg->fmt == kCsrV) cg->sval = nullptr;  // A's array
-  dfree(cg->sptr); dfree(cg->slen); dfree(cg->scol); dfree(cg->scol16); dfree(cg->sval);
+  dfree(cg->sptr); dfree(cg->slen); dfree(cg->scol); dfree(cg->scol16); dfree(cg->sptr16); dfree(cg->sval);
   dfree(cg->svalT);
   dfree(cg->sval32); dfree(cg->sval32T); dfree(cg->dinv); dfree(cg->dsc); dfree(cg->wcl);
   dfree(cg->xw); dfree(cg->bw); dfree(cg->G); dfree(cg->CG); dfree(cg->H[0]); dfree(cg->H[1]);

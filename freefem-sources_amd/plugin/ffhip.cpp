@@ -19,9 +19,17 @@
 //    on a scalar P1 space with the default Tet_5 rule; the coefficient c is
 //    evaluated by FreeFEM at the 14 quadrature points of every element (as
 //    Element_Op does, problem.cpp:6392-6395) unless it is a constant, and the
-//    pattern, values and tgv rows are built on the GPU.  Any other form falls
-//    back to FreeFEM's own creationBlockOfMatrixToBilinearForm
-//    (problem.hpp:1663-1714), as the PETSc plugin does for its Mat type.
+//    pattern, values and tgv rows are built on the GPU.  It also recognises
+//    linear elasticity on Vh(Th,[P2,P2,P2]):
+//        int3d(Th)(lambda*div(u)*div(v) + 2*mu*(epsilon(u)'*epsilon(v)))
+//        [+ on(labels, u1=.., u2=.., u3=..)]
+//    in any spelling whose first-derivative terms have mesh-independent
+//    coefficients that add up to the Lame table
+//        C[c][a][c'][b] = lambda d_ca d_c'b + mu (d_cb d_ac' + d_cc' d_ab)
+//    (test component c, derivative a; trial component c', derivative b).
+//    Any other form falls back to FreeFEM's own
+//    creationBlockOfMatrixToBilinearForm (problem.hpp:1663-1714), as the
+//    PETSc plugin does for its Mat type.
 //
 // Build (FreeFEM's plugin script):  ff-c++ -auto ffhip.cpp -I<repo>/include
 //   -L<repo>/freefem-sources_amd -lffhip
@@ -211,6 +219,136 @@ bool recognise(Stack stack, const FESpace3 &Uh, const FESpace3 &Vh, const list<C
   return true;
 }
 
+// ---------------------------------------------------------------- [P2,P2,P2]
+struct RecognisedP2 {
+  const FormBilinear *bf = nullptr;
+  double lambda = 0., mu = 0.;
+  set<long> labels;
+};
+
+// Vh(Th,[P2,P2,P2]): a GTypeOfFESum of three P2 (FESpacen.cpp:403-428, 531-575)
+bool is_p2v(const FESpace3 &Uh) {
+  if (Uh.N != 3 || Uh.TFE.N() != 1) return false;
+  const GTypeOfFESum<Mesh3> *sum = dynamic_cast<const GTypeOfFESum<Mesh3> *>(Uh.TFE[0]);
+  if (!sum || sum->k != 3) return false;
+  for (int c = 0; c < 3; ++c)
+    if (sum->teb[c] != &DataFE<Mesh3>::P2) return false;
+  return true;
+}
+
+bool recognise_p2v(Stack stack, const FESpace3 &Uh, const FESpace3 &Vh, const list<C_F0> &largs,
+                   RecognisedP2 &r) {
+  if (&Uh != &Vh || !is_p2v(Uh)) return false;
+  FreeFempp::TypeVarForm<double> *tvf = FreeFempp::TypeVarForm<double>::Global;
+  double C[3][3][3][3];
+  memset(C, 0, sizeof(C));
+  for (list<C_F0>::const_iterator ii = largs.begin(); ii != largs.end(); ++ii) {
+    const aType t = ii->left();
+    const Expression e = ii->LeftValue();
+    if (t == tvf->tFL) continue;
+    if (t == tvf->tBC) {  // every component of the listed labels
+      const BC_set *bc = dynamic_cast<const BC_set *>(e);
+      if (!bc) return false;
+      set<int> comps;
+      for (size_t k = 0; k < bc->bc.size(); ++k) comps.insert(bc->bc[k].first);
+      if (comps.size() != 3 || *comps.begin() != 0 || *comps.rbegin() != 2) return false;
+      set<long> labs;
+      Fem2D::Expandsetoflab(stack, *bc, labs);
+      r.labels.insert(labs.begin(), labs.end());
+      continue;
+    }
+    if (t != tvf->tFB || r.bf) return false;
+    const FormBilinear *bf = dynamic_cast<const FormBilinear *>(e);
+    if (!bf || bf->di->kind != CDomainOfIntegration::int3d || !bf->di->what.empty()) return false;
+    if (GetAny<const Mesh3 *>((*bf->di->Th)(stack)) != &Uh.Th) return false;
+    if (bf->di->FIV(stack).n != QuadratureFormular_Tet_5.n) return false;
+    const BilinearOperator &Op = *bf->b;
+    for (BilinearOperator::const_iterator l = Op.v.begin(); l != Op.v.end(); ++l) {
+      const BilinearOperator::K &ll = *l;
+      // trial (u) component/derivative first, test (v) second (problem.cpp:6404-6405)
+      const int jcomp = ll.first.first.first, jop = ll.first.first.second;
+      const int icomp = ll.first.second.first, iop = ll.first.second.second;
+      if (jop < op_dx || jop > op_dz || iop < op_dx || iop > op_dz) return false;
+      if (jcomp < 0 || jcomp > 2 || icomp < 0 || icomp > 2) return false;
+      if (!ll.second.LeftValue()->MeshIndependent()) return false;
+      C[icomp][iop - op_dx][jcomp][jop - op_dx] += GetAny<double>(ll.second.eval(stack));
+    }
+    r.bf = bf;
+  }
+  if (!r.bf) return false;
+  const double lam = C[0][0][1][1], mu = C[0][1][1][0];
+  const double scale = max(fabs(lam), fabs(mu));
+  if (!(scale > 0)) return false;
+  for (int c = 0; c < 3; ++c)
+    for (int a = 0; a < 3; ++a)
+      for (int cp = 0; cp < 3; ++cp)
+        for (int b = 0; b < 3; ++b) {
+          const double want = lam * (c == a) * (cp == b) + mu * ((c == b) * (a == cp) + (c == cp) * (a == b));
+          if (fabs(C[c][a][cp][b] - want) > 1e-12 * scale) return false;
+        }
+  r.lambda = lam;
+  r.mu = mu;
+  return true;
+}
+
+// true when built on the GPU; false (nothing changed) when FreeFEM's dof
+// numbering of Uh is not the first-appearance numbering the device builds
+bool assemble_p2v_on_gpu(const FESpace3 &Uh, const RecognisedP2 &r, double tgv,
+                         Matrice_Creuse<double> &A) {
+  const Mesh3 &Th = Uh.Th;
+  vector<double> xyz(3 * (size_t)Th.nv);
+  vector<int32_t> tets(4 * (size_t)Th.nt), btri(3 * (size_t)Th.nbe), blab(Th.nbe);
+  for (int i = 0; i < Th.nv; ++i) {
+    xyz[3 * i] = Th(i).x; xyz[3 * i + 1] = Th(i).y; xyz[3 * i + 2] = Th(i).z;
+  }
+  for (int k = 0; k < Th.nt; ++k)
+    for (int j = 0; j < 4; ++j) tets[4 * k + j] = Th(Th[k][j]);
+  for (int k = 0; k < Th.nbe; ++k) {
+    for (int j = 0; j < 3; ++j) btri[3 * k + j] = Th(Th.be(k)[j]);
+    blab[k] = Th.be(k).lab;
+  }
+  ffhip_ctx ctx = hip_ctx();
+  ffhip_mesh m = nullptr;
+  ffhip_p2v sp = nullptr;
+  hip_check(ffhip_mesh3_upload(ctx, Th.nv, Th.nt, Th.nbe, xyz.data(), tets.data(), btri.data(),
+                               blab.data(), &m), "ffhip: mesh upload");
+  hip_check(ffhip_p2v_create(ctx, m, &sp), "ffhip: P2 numbering");
+  vector<int32_t> en(10 * (size_t)Th.nt);
+  hip_check(ffhip_p2v_download(sp, en.data()), "ffhip: P2 numbering");
+  bool same = true;
+  for (int k = 0; k < Th.nt && same; ++k) {
+    const FElement3 K(Uh[k]);
+    for (int df = 0; df < 30 && same; ++df)  // local dofs comp-blocked (c*10 + i)
+      same = K(df) == 3 * en[10 * (size_t)k + df % 10] + df / 10;
+  }
+  if (!same) {
+    ffhip_p2v_free(sp);
+    ffhip_mesh_free(m);
+    return false;
+  }
+  ffhip_mat M = nullptr;
+  hip_check(ffhip_pattern_p2v(ctx, sp, &M), "ffhip: P2 pattern");
+  hip_check(ffhip_assemble_p2v_elasticity(ctx, sp, M, r.lambda, r.mu), "ffhip: P2 assembly");
+  if (!r.labels.empty()) {
+    vector<int32_t> labs(r.labels.begin(), r.labels.end());
+    hip_check(ffhip_set_bc_p2v(ctx, sp, M, labs.data(), (int)labs.size(), tgv, nullptr), "ffhip: on()");
+  }
+  int64_t n = 0, nnz = 0;
+  ffhip_mat_info(M, &n, &nnz);
+  vector<int64_t> rp(n + 1);
+  vector<int> ia(n + 1), ja(nnz);
+  vector<double> aa(nnz);
+  hip_check(ffhip_mat_download(M, rp.data(), ja.data(), aa.data()), "ffhip: download");
+  for (int64_t i = 0; i <= n; ++i) ia[i] = (int)rp[i];
+  ffhip_mat_free(M);
+  ffhip_p2v_free(sp);
+  ffhip_mesh_free(m);
+  MatriceMorse<double> *pH = new MatriceMorse<double>((int)n, (int)n, (int)nnz, 0);
+  pH->set((int)n, (int)n, 0, (size_t)nnz, ia.data(), ja.data(), aa.data(), 0, 1);
+  A.A.master(pH);
+  return true;
+}
+
 // the coefficient at the 14 Tet_5 points of every element, evaluated by the
 // interpreter as Element_Op does (problem.cpp:6385-6395)
 void eval_coef_qp(Stack stack, const FESpace3 &Uh, const C_F0 &coef, vector<double> &kq) {
@@ -304,12 +442,21 @@ class OpHipMatrixP1 : public OneOperator {
       if (init) A.init();
       if (!PUh || !PVh) return SetAny<Matrice_Creuse<double> *>(&A);
       Recognised r;
+      RecognisedP2 r2;
+      bool done = false;
       if (ds.sym == 0 && ds.tgv > 0 && recognise(stack, *PUh, *PVh, b->largs, r)) {
         A.A = 0;
         A.Uh = *PUh;
         A.Vh = *PVh;
         assemble_on_gpu(stack, *PUh, r, ds.tgv, A);
-      } else {
+        done = true;
+      } else if (ds.sym == 0 && ds.tgv > 0 && recognise_p2v(stack, *PUh, *PVh, b->largs, r2)) {
+        A.A = 0;
+        A.Uh = *PUh;
+        A.Vh = *PVh;
+        done = assemble_p2v_on_gpu(*PUh, r2, ds.tgv, A);
+      }
+      if (!done) {
         creationBlockOfMatrixToBilinearForm<double, Mesh3, FESpace3, FESpace3>(PUh, PVh, ds.sym, ds.tgv,
                                                                                b->largs, stack, A);
       }

@@ -43,13 +43,14 @@ LABELS = (1, 2, 3, 4, 5, 6)
 HBM_PEAK_GBS = 8000.0
 
 
-def measured_traffic(kernel="k_pcg_dir"):
+def measured_traffic(kernel="k_pcg_dir", layout="sell16"):
     """HBM bytes per launch from the committed PMC summary (FETCH_SIZE and
-    WRITE_SIZE scaled by a copy kernel of known size), or None."""
+    WRITE_SIZE scaled by a copy kernel of known size) for the layout the
+    solver streamed, or None."""
     path = os.path.join(ROOT, "profiles", "k_pcg_dir_cube160.md")
     try:
         for line in open(path):
-            if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+            if line.startswith(f"traffic_bytes_per_launch {kernel} {layout}:"):
                 return int(line.split(":")[1])
     except OSError:
         pass
@@ -298,7 +299,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic() if n == 160 else None,
+            "traffic": measured_traffic("k_pcg_dir", res[-1]["layout64"][0]) if n == 160 else None,
             "kernel": f"k_pcg_dir (FP64 {res[-1]['layout64'][0]} mat-vec + direction update + 2 dots)",
             "bytes_per_launch": bytes_mv,
             "avg_launch_ms": round(mv_ms, 5),

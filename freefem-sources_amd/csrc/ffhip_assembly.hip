@@ -2,21 +2,27 @@
 // rows, right-hand side and CSR mat-vec.
 //
 // Design (MI355X-first, deterministic):
-//   1. dof -> (element, local vertex) incidence lists, sorted by element id
-//      (counts by integer atomics, a scan, and a per-dof sort: deterministic).
-//   2. pattern: one lane per row merges the vertex sets of its incident
-//      elements into a sorted unique list held in LDS; a count pass, a scan,
-//      a fill pass.  The fill pass also records, for every incidence entry,
-//      the offsets of that element's vertices inside the row — the
-//      element-to-row map.
+//   1. dof -> (element, local vertex) incidence lists, sorted by element id:
+//      counts by integer atomics, a scan, an atomic fill, then every list of
+//      up to 32 entries is sorted by a bitonic network in a 32-lane half-wave
+//      (longer lists: per-dof insertion sort).
+//   2. pattern: one half-wave per row loads the vertices of the row's
+//      incident elements (4 per lane, 128 per row), sorts them with a bitonic
+//      network and keeps the heads of equal runs — the sorted unique columns.
+//      A count pass, a scan, a fill pass; the fill pass also records, for
+//      every incidence entry, the offsets of that element's vertices inside
+//      the row (the element-to-row map).  Rows with more than 32 incident
+//      elements use a lane-per-row merge in LDS.
 //   3. values: one thread per element evaluates the 14-point Tet_5 integral
 //      of kappa and the P1 gradients (VALU FP64) and writes the 10 unique
-//      entries of its symmetric 4x4 matrix; then one lane per row walks its
-//      incidence list in element order and accumulates into the row's slots
-//      staged in LDS, which the block writes back coalesced.  No atomics:
-//      every slot is owned by one lane and summed in element order, the
-//      order in which HashMatrix::operator+= accumulates
-//      (src/femlib/HashMatrix.cpp:1294-1332).
+//      entries of its symmetric 4x4 matrix; then one wave per row loads the
+//      4 entries of each incident element (one lane per element) into LDS,
+//      and the lane owning each slot adds them in element order.  No
+//      atomics: every slot is summed by one lane in element order, the order
+//      in which HashMatrix::operator+= accumulates
+//      (src/femlib/HashMatrix.cpp:1294-1332).  2D triangles and the rows with
+//      more than 32 elements use a lane-per-row walk with the row staged in
+//      LDS.
 #include <hipcub/hipcub.hpp>
 
 #include <cstring>
@@ -32,9 +38,27 @@ __constant__ double c_qt5_w[7];
 
 // ---------------------------------------------------------------- incidence
 __global__ void k_v2e_count(const int32_t *__restrict__ elems, int64_t m,
-                            unsigned long long *__restrict__ deg) {
+                            uint32_t *__restrict__ deg) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < m) atomicAdd(&deg[elems[i]], 1ull);
+  if (i < m) atomicAdd(&deg[elems[i]], 1u);
+}
+
+// one pass: count and place every incidence into a 32-entry bucket of its
+// dof (lists longer than 32 set *ovf and are rebuilt by count + fill)
+__global__ void k_v2e_bucket(const int32_t *__restrict__ elems, int64_t m,
+                             uint32_t *__restrict__ deg, int32_t *__restrict__ bucket,
+                             int *__restrict__ ovf) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const int32_t v = elems[i];
+  const uint32_t pos = atomicAdd(&deg[v], 1u);
+  if (pos < 32) bucket[32 * (int64_t)v + pos] = (int32_t)i;
+  else atomicOr(ovf, 1);
+}
+
+__global__ void k_u32_to_i64(int64_t n, const uint32_t *__restrict__ a, int64_t *__restrict__ b) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
 }
 
 __global__ void k_v2e_fill(const int32_t *__restrict__ elems, int64_t m,
@@ -47,11 +71,98 @@ __global__ void k_v2e_fill(const int32_t *__restrict__ elems, int64_t m,
   v2e[ptr[v] + pos] = (int32_t)i;
 }
 
-// per-dof insertion sort: incidence entries in element order
+// ---------------------------------------------------------------- half-wave sorting networks
+// A 32-lane half of a wave holds one row's list; bitonic compare-exchange
+// across lanes (lane xor m, m <= 16, never leaves the half) and inside a lane.
+constexpr int kHalf = 32;  // rows of up to 32 incident elements use the half-wave kernels
+
+template <class T>
+__device__ inline T tmin(T a, T b) { return a < b ? a : b; }
+template <class T>
+__device__ inline T tmax(T a, T b) { return a < b ? b : a; }
+
+// one value per lane, sequence index = lane within the half
+template <class T>
+__device__ inline T bitonic32(T x, int hl) {
+#pragma unroll
+  for (int k = 2; k <= 32; k <<= 1)
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const T p = __shfl_xor(x, j, kWave);
+      const bool up = (hl & k) == 0, lower = (hl & j) == 0;
+      x = (lower == up) ? tmin(x, p) : tmax(x, p);
+    }
+  return x;
+}
+
+// four values per lane, sequence index = 4 * lane-in-half + r (128 entries)
+template <class T>
+__device__ inline void bitonic128(T c[4], int hl) {
+#pragma unroll
+  for (int k = 2; k <= 128; k <<= 1)
+#pragma unroll
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      if (j < 4) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          if (r & j) continue;
+          const bool up = ((hl * 4 + r) & k) == 0;
+          const T a = c[r], b = c[r | j];
+          const bool sw = up ? (b < a) : (a < b);
+          c[r] = sw ? b : a;
+          c[r | j] = sw ? a : b;
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const T p = __shfl_xor(c[r], j >> 2, kWave);
+          const int idx = hl * 4 + r;
+          const bool up = (idx & k) == 0, lower = (idx & j) == 0;
+          c[r] = (lower == up) ? tmin(c[r], p) : tmax(c[r], p);
+        }
+      }
+    }
+}
+
+// incidence lists of up to 32 entries sorted into element order, two rows
+// per wave; longer lists are left to k_v2e_sort (flagged in *big)
+__global__ __launch_bounds__(256) void k_v2e_sort_half(int64_t n, const int64_t *__restrict__ ptr,
+                                                       int32_t *__restrict__ v2e, int *__restrict__ big) {
+  const int hl = threadIdx.x % kHalf;
+  const int64_t v = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kHalf;
+  int64_t k0 = 0;
+  int deg = 0;
+  if (v < n) { k0 = ptr[v]; deg = (int)(ptr[v + 1] - k0); }
+  if (deg > kHalf) {
+    if (hl == 0) atomicOr(big, 1);
+    deg = 0;
+  }
+  int32_t x = hl < deg ? v2e[k0 + hl] : INT_MAX;
+  x = bitonic32(x, hl);
+  if (hl < deg) v2e[k0 + hl] = x;
+}
+
+// bucketed lists (up to 32 entries) sorted into element order and packed
+// at their CSR offsets, two dofs per wave
+__global__ __launch_bounds__(256) void k_v2e_sort_bucket(int64_t n, const int64_t *__restrict__ ptr,
+                                                         const int32_t *__restrict__ bucket,
+                                                         int32_t *__restrict__ v2e) {
+  const int hl = threadIdx.x % kHalf;
+  const int64_t v = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kHalf;
+  int64_t k0 = 0;
+  int deg = 0;
+  if (v < n) { k0 = ptr[v]; deg = (int)(ptr[v + 1] - k0); }
+  int32_t x = hl < deg ? bucket[32 * v + hl] : INT_MAX;
+  x = bitonic32(x, hl);
+  if (hl < deg) v2e[k0 + hl] = x;
+}
+
+// per-dof insertion sort of the lists longer than 32 entries
 __global__ void k_v2e_sort(int64_t n, const int64_t *__restrict__ ptr, int32_t *__restrict__ v2e) {
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
   const int64_t a = ptr[v], b = ptr[v + 1];
+  if (b - a <= kHalf) return;
   for (int64_t i = a + 1; i < b; ++i) {
     const int32_t key = v2e[i];
     int64_t j = i - 1;
@@ -60,26 +171,120 @@ __global__ void k_v2e_sort(int64_t n, const int64_t *__restrict__ ptr, int32_t *
   }
 }
 
+// ---------------------------------------------------------------- pattern (half-wave rows)
+// One row per 32-lane half-wave: lane q loads incidence q's element vertices
+// (4 candidates per lane, 128 per row), a bitonic network sorts them, and the
+// row's sorted unique columns are the heads of equal runs.  One pass writes
+// the row length, the unique columns to a 64-entry row buffer, the relative
+// diagonal slot and the element-to-row map (each candidate's offset found by
+// binary search in the row's columns staged in LDS); after the scan,
+// k_pattern_compact moves the buffered columns to their CSR place.  Rows with
+// more than 32 incident elements or 64 columns are flagged in big[] for the
+// lane-per-row merge (k_pattern_rows).
+constexpr int kRowBuf = 64;
+
+template <int NLOC>
+__global__ __launch_bounds__(256) void k_pattern_half(
+    int64_t n, const int32_t *__restrict__ elems, const int64_t *__restrict__ v2e_ptr,
+    const int32_t *__restrict__ v2e, int64_t *__restrict__ rowlen, int32_t *__restrict__ rowbuf,
+    int8_t *__restrict__ dposr, uint32_t *__restrict__ slotmap, uint8_t *__restrict__ big,
+    int *__restrict__ any_big) {
+  __shared__ int32_t ucol[256 / kHalf][kRowBuf];
+  const int hl = threadIdx.x % kHalf, hid = threadIdx.x / kHalf;
+  const int64_t v = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kHalf;
+  int64_t k0 = 0;
+  int deg = 0;
+  if (v < n) { k0 = v2e_ptr[v]; deg = (int)(v2e_ptr[v + 1] - k0); }
+  bool mine = v < n && deg <= kHalf;
+  if (!mine) deg = 0;
+  int32_t c[4] = {INT_MAX, INT_MAX, INT_MAX, INT_MAX};
+  if (hl < deg) {
+    const int64_t e = v2e[k0 + hl] / NLOC;
+    if (NLOC == 4) {
+      const int4 t = reinterpret_cast<const int4 *>(elems)[e];
+      c[0] = t.x; c[1] = t.y; c[2] = t.z; c[3] = t.w;
+    } else {
+#pragma unroll
+      for (int m = 0; m < NLOC; ++m) c[m] = elems[e * NLOC + m];
+    }
+  }
+  int32_t orig[4] = {c[0], c[1], c[2], c[3]};
+  bitonic128(c, hl);
+  const int32_t prev0 = __shfl_up(c[3], 1, kWave);
+  int flag[4], cnt = 0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int32_t prev = r ? c[r - 1] : (hl ? prev0 : INT_MIN);
+    flag[r] = (c[r] != INT_MAX && c[r] != prev) ? 1 : 0;
+    cnt += flag[r];
+  }
+  int incl = cnt;  // inclusive prefix of run heads over the half
+#pragma unroll
+  for (int off = 1; off < kHalf; off <<= 1) {
+    const int t = __shfl_up(incl, off, kWave);
+    if (hl >= off) incl += t;
+  }
+  const int len = __shfl(incl, (threadIdx.x % kWave) / kHalf * kHalf + kHalf - 1, kWave);
+  if (len > kRowBuf) mine = false;
+  int run = incl - cnt, dpos = -1;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    run += flag[r];
+    if (flag[r] && mine) {
+      ucol[hid][run - 1] = c[r];
+      rowbuf[kRowBuf * v + run - 1] = c[r];
+      if ((int64_t)c[r] == v) dpos = run - 1;
+    }
+  }
+#pragma unroll
+  for (int off = 1; off < kHalf; off <<= 1) {
+    const int t = __shfl_xor(dpos, off, kWave);
+    dpos = t > dpos ? t : dpos;
+  }
+  __syncthreads();
+  if (v < n && hl == 0) {
+    big[v] = mine ? 0 : 1;
+    if (!mine) atomicOr(any_big, 1);
+    if (mine) { rowlen[v] = len; dposr[v] = (int8_t)dpos; }
+  }
+  if (mine && hl < deg) {
+    uint32_t pk = 0;
+#pragma unroll
+    for (int m = 0; m < NLOC; ++m) {
+      int lo = 0, hi = len - 1;  // the column is in the row: find its offset
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (ucol[hid][mid] < orig[m]) lo = mid + 1; else hi = mid;
+      }
+      pk |= (uint32_t)lo << (8 * m);
+    }
+    slotmap[k0 + hl] = pk;
+  }
+}
+
+// buffered columns to their CSR place; diagonal slots
+__global__ __launch_bounds__(256) void k_pattern_compact(
+    int64_t n, const int64_t *__restrict__ rowptr, const int32_t *__restrict__ rowbuf,
+    const int8_t *__restrict__ dposr, const uint8_t *__restrict__ big,
+    int32_t *__restrict__ colidx, int64_t *__restrict__ diag) {
+  const int hl = threadIdx.x % kHalf;
+  const int64_t v = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kHalf;
+  if (v >= n || big[v]) return;
+  const int64_t r0 = rowptr[v];
+  const int len = (int)(rowptr[v + 1] - r0);
+  for (int j = hl; j < len; j += kHalf) colidx[r0 + j] = rowbuf[kRowBuf * v + j];
+  if (hl == 0) diag[v] = dposr[v] < 0 ? -1 : r0 + dposr[v];
+}
+
 // ---------------------------------------------------------------- element matrices
-// 3D P1: 10 unique entries of Ke = sum_q |K| w_q kappa(x_q) grad l_i . grad l_j
-__global__ __launch_bounds__(256) void k_p1_element_3d(
-    int64_t nt, const double *__restrict__ xyz, const int32_t *__restrict__ tets,
-    int kind, double c0, const double *__restrict__ kappa_e, double *__restrict__ Ke) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nt) return;
-  const int4 tv = reinterpret_cast<const int4 *>(tets)[e];
-  const int32_t id[4] = {tv.x, tv.y, tv.z, tv.w};
-  double v[4][3];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int k = 0; k < 3; ++k) v[i][k] = xyz[3 * (int64_t)id[i] + k];
-  double G[4][3];
-  const double mes = tet_geom(v, G);
-  // S = sum_q coef_q * kappa(x_q), coef_q = |K| w_q  (problem.cpp:6387-6392)
-  double S;
+// 3D P1: Ke_ij = S grad l_i . grad l_j with S = sum_q |K| w_q kappa(x_q)
+// (problem.cpp:6387-6392); the 10 unique entries per element.
+// gather recomputes the element's gradients from its vertices when it adds
+// the element's row, so the element matrices never travel through HBM.
+__device__ inline double p1_kappa_integral(int64_t e, const double v[4][3], double mes, int kind,
+                                           double c0, const double *__restrict__ kappa_e) {
+  double S = 0.;
   if (kind == FFHIP_COEF_SIN3) {
-    S = 0.;
 #pragma unroll 2
     for (int q = 0; q < 14; ++q) {
       const double px = c_qf5_pts[q][0], py = c_qf5_pts[q][1], pz = c_qf5_pts[q][2];
@@ -90,26 +295,50 @@ __global__ __launch_bounds__(256) void k_p1_element_3d(
       S += (mes * qf5_weight(q)) * kappa_sin3(P[0], P[1], P[2]);
     }
   } else if (kind == FFHIP_COEF_QP) {
-    S = 0.;
 #pragma unroll
     for (int q = 0; q < 14; ++q) S += (mes * qf5_weight(q)) * kappa_e[14 * e + q];
   } else {
     const double kc = (kind == FFHIP_COEF_P0) ? kappa_e[e] : c0;
-    S = 0.;
 #pragma unroll
     for (int q = 0; q < 14; ++q) S += (mes * qf5_weight(q)) * kc;
   }
+  return S;
+}
+
+__device__ inline void load_tet(const double *__restrict__ xyz, const int32_t *__restrict__ tets,
+                                int64_t e, double v[4][3]) {
+  const int4 tv = reinterpret_cast<const int4 *>(tets)[e];
+  const int32_t id[4] = {tv.x, tv.y, tv.z, tv.w};
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) v[i][k] = xyz[3 * (int64_t)id[i] + k];
+}
+
+// entry (i, j), i <= j, of the P1 element matrix: sum_k (S G_ik) G_jk
+__device__ inline double p1_entry(double S, const double G[4][3], int i, int j) {
+  double a = 0.;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) a += (S * G[i][k]) * G[j][k];
+  return a;
+}
+
+// the 10 entries (i <= j, row-major upper triangle) per element
+__global__ __launch_bounds__(256) void k_p1_element_3d(
+    int64_t nt, const double *__restrict__ xyz, const int32_t *__restrict__ tets,
+    int kind, double c0, const double *__restrict__ kappa_e, double *__restrict__ Ke) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nt) return;
+  double v[4][3], G[4][3];
+  load_tet(xyz, tets, e, v);
+  const double mes = tet_geom(v, G);
+  const double S = p1_kappa_integral(e, v, mes, kind, c0, kappa_e);
   double out[10];
   int o = 0;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = i; j < 4; ++j) {
-      double a = 0.;
-#pragma unroll
-      for (int k = 0; k < 3; ++k) a += (S * G[i][k]) * G[j][k];
-      out[o++] = a;
-    }
+    for (int j = i; j < 4; ++j) out[o++] = p1_entry(S, G, i, j);
   double *dst = Ke + 10 * e;
 #pragma unroll
   for (int i = 0; i < 10; ++i) dst[i] = out[i];
@@ -166,7 +395,8 @@ template <int BLOCK, int STAGE, int NLOC>
 __global__ __launch_bounds__(BLOCK) void k_p1_gather(
     int64_t n, const int64_t *__restrict__ rowptr, const int64_t *__restrict__ v2e_ptr,
     const int32_t *__restrict__ v2e, const uint32_t *__restrict__ slotmap,
-    const double *__restrict__ Ke, double *__restrict__ vals) {
+    const double *__restrict__ Ke, double *__restrict__ vals,
+    const uint8_t *__restrict__ only = nullptr) {
   constexpr int NU = NLOC * (NLOC + 1) / 2;
   __shared__ double S[STAGE];
   const int t = threadIdx.x;
@@ -179,7 +409,7 @@ __global__ __launch_bounds__(BLOCK) void k_p1_gather(
     for (int64_t i = t; i < seg1 - seg0; i += BLOCK) S[i] = 0.;
     __syncthreads();
   }
-  if (v < n) {
+  if (v < n && (!only || only[v])) {  // only[]: the rows this kernel owns
     const int64_t r0 = rowptr[v];
     double *acc = staged ? S + (r0 - seg0) : vals + r0;
     if (!staged)
@@ -201,8 +431,63 @@ __global__ __launch_bounds__(BLOCK) void k_p1_gather(
   }
   if (staged) {
     __syncthreads();
-    for (int64_t i = t; i < seg1 - seg0; i += BLOCK) vals[seg0 + i] = S[i];
+    if (!only) {
+      for (int64_t i = t; i < seg1 - seg0; i += BLOCK) vals[seg0 + i] = S[i];
+    } else {
+      for (int64_t i = t; i < seg1 - seg0; i += BLOCK) {
+        const int64_t g = seg0 + i;
+        int64_t lo = v0, hi = vend - 1;  // the row of slot g
+        while (lo < hi) {
+          const int64_t mid = (lo + hi + 1) >> 1;
+          if (rowptr[mid] <= g) lo = mid; else hi = mid - 1;
+        }
+        if (only[lo]) vals[g] = S[i];
+      }
+    }
   }
+}
+
+// 3D values, one row per wave: lane q (< 32) takes incidence q (element
+// order) and stages that element's 4 entries of the row and their 4 slot
+// bytes in LDS; lane j then walks the incidences in order and adds the entry
+// (if any) whose slot is j — the same sums, in the same order, as a
+// sequential walk over the row's elements.  Rows flagged big[] (more than 32
+// elements or 64 columns) are left to k_p1_gather.
+__global__ __launch_bounds__(256) void k_p1_values_wave(
+    int64_t n, const int64_t *__restrict__ rowptr, const int64_t *__restrict__ v2e_ptr,
+    const int32_t *__restrict__ v2e, const uint32_t *__restrict__ slotmap,
+    const double *__restrict__ Ke, const uint8_t *__restrict__ big, double *__restrict__ vals) {
+  __shared__ double sv[256 / kWave][4 * kHalf];
+  __shared__ uint32_t so[256 / kWave][kHalf];  // 4 slot bytes per incidence
+  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+  const int64_t v = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  int64_t k0 = 0, r0 = 0;
+  int deg = 0, len = 0;
+  if (v < n && !big[v]) {
+    k0 = v2e_ptr[v];
+    deg = (int)(v2e_ptr[v + 1] - k0);
+    r0 = rowptr[v];
+    len = (int)(rowptr[v + 1] - r0);
+  }
+  if (lane < deg) {
+    const int32_t idx = v2e[k0 + lane];
+    const int64_t e = idx / 4;
+    const int l = idx - (int)e * 4;
+    const double *ke = Ke + 10 * e;
+#pragma unroll
+    for (int m = 0; m < 4; ++m) sv[w][4 * lane + m] = ke[sym4(l, m)];
+    so[w][lane] = slotmap[k0 + lane];
+  }
+  __syncthreads();
+  const uint32_t me = (uint32_t)lane * 0x01010101u;
+  double acc = 0.;
+  for (int q = 0; q < deg; ++q) {
+    // lowest byte of so[q] equal to this lane's slot (at most one is)
+    const uint32_t t = so[w][q] ^ me;
+    const uint32_t z = (t - 0x01010101u) & ~t & 0x80808080u;
+    if (z) acc += sv[w][4 * q + (__builtin_ctz(z) >> 3)];
+  }
+  if (lane < len) vals[r0 + lane] = acc;
 }
 
 // ---------------------------------------------------------------- Dirichlet
@@ -409,23 +694,39 @@ static void upload_t5_once() {
 // dof -> (element * nloc + local) incidence lists in element order, for any
 // element-to-dof table (P1 vertices, P2 nodes)
 void build_incidence(ffhip_ctx ctx, const int32_t *elems, int64_t nt, int nloc, int64_t n,
-                     int64_t **ptr_out, int32_t **list_out) {
+                     int64_t **ptr_out, int32_t **list_out, int *big_out) {
   hipStream_t s = ctx->stream;
   const int64_t M = nt * nloc;
-  unsigned long long *deg = dalloc<unsigned long long>(n + 1);
-  int32_t *cur = dalloc<int32_t>(n);
+  uint32_t *deg = dalloc<uint32_t>(n);
+  int64_t *deg64 = dalloc<int64_t>(n + 1);
+  int32_t *bucket = dalloc<int32_t>(32 * n);
+  int *flag = dalloc<int>(1);
   int64_t *ptr = dalloc<int64_t>(n + 1);
   int32_t *list = dalloc<int32_t>(M);
-  FFHIP_CHECK(hipMemsetAsync(deg, 0, 8 * (n + 1), s));
-  FFHIP_CHECK(hipMemsetAsync(cur, 0, 4 * n, s));
-  k_v2e_count<<<nblocks(M, 256), 256, 0, s>>>(elems, M, deg);
-  exclusive_scan_i64(ctx, (const int64_t *)deg, ptr, n + 1);
-  k_v2e_fill<<<nblocks(M, 256), 256, 0, s>>>(elems, M, ptr, cur, list);
-  k_v2e_sort<<<nblocks(n, 256), 256, 0, s>>>(n, ptr, list);
+  FFHIP_CHECK(hipMemsetAsync(deg, 0, 4 * n, s));
+  FFHIP_CHECK(hipMemsetAsync(deg64 + n, 0, 8, s));
+  FFHIP_CHECK(hipMemsetAsync(flag, 0, sizeof(int), s));
+  k_v2e_bucket<<<nblocks(M, 256), 256, 0, s>>>(elems, M, deg, bucket, flag);
+  k_u32_to_i64<<<nblocks(n, 256), 256, 0, s>>>(n, deg, deg64);
+  exclusive_scan_i64(ctx, deg64, ptr, n + 1);
+  FFHIP_CHECK(hipGetLastError());
+  const int has_big = read_flag(ctx, flag);
+  if (!has_big) {
+    k_v2e_sort_bucket<<<nblocks(n * kHalf, 256), 256, 0, s>>>(n, ptr, bucket, list);
+  } else {  // some list exceeds its bucket: fill by atomics, then sort
+    int32_t *cur = dalloc<int32_t>(n);
+    FFHIP_CHECK(hipMemsetAsync(cur, 0, 4 * n, s));
+    FFHIP_CHECK(hipMemsetAsync(flag, 0, sizeof(int), s));
+    k_v2e_fill<<<nblocks(M, 256), 256, 0, s>>>(elems, M, ptr, cur, list);
+    k_v2e_sort_half<<<nblocks(n * kHalf, 256), 256, 0, s>>>(n, ptr, list, flag);
+    k_v2e_sort<<<nblocks(n, 256), 256, 0, s>>>(n, ptr, list);
+    FFHIP_CHECK(hipStreamSynchronize(s));
+    dfree(cur);
+  }
   FFHIP_CHECK(hipGetLastError());
   FFHIP_CHECK(hipStreamSynchronize(s));
-  dfree(deg);
-  dfree(cur);
+  dfree(deg); dfree(deg64); dfree(bucket); dfree(flag);
+  if (big_out) *big_out = has_big;
   *ptr_out = ptr;
   *list_out = list;
 }
@@ -433,7 +734,7 @@ void build_incidence(ffhip_ctx ctx, const int32_t *elems, int64_t nt, int nloc, 
 // vertex -> incidence lists of a mesh (built once per mesh)
 void build_v2e(ffhip_mesh m) {
   if (m->v2e_ptr) return;
-  build_incidence(m->ctx, m->elems, m->nt, m->nloc, m->nv, &m->v2e_ptr, &m->v2e);
+  build_incidence(m->ctx, m->elems, m->nt, m->nloc, m->nv, &m->v2e_ptr, &m->v2e, &m->v2e_big);
 }
 
 // a_ii = tgv on the marked rows (HashMatrix::SetBC); *missing set when a
@@ -487,9 +788,25 @@ int ffhip_pattern_p1(ffhip_ctx ctx, ffhip_mesh m, ffhip_mat *out) {
     FFHIP_CHECK(hipMemsetAsync(len, 0, 8 * (n + 1), s));
     FFHIP_CHECK(hipMemsetAsync(ovf, 0, sizeof(int), s));
     constexpr int B = 128, CAP = 64;
-    k_pattern_rows<B, CAP, false><<<nblocks(n, B), B, 0, s>>>(
-        n, m->nloc, m->elems, m->v2e_ptr, m->v2e, len, nullptr, nullptr, nullptr, nullptr, nullptr,
-        ovf);
+    // rows with up to 32 incident elements and 64 columns: one half-wave
+    // pass into row buffers; the rest: the lane-per-row merge
+    int32_t *rowbuf = dalloc<int32_t>(kRowBuf * n);
+    int8_t *dposr = dalloc<int8_t>(n);
+    int *anyb = dalloc<int>(1);
+    A->big = dalloc<uint8_t>(n);
+    FFHIP_CHECK(hipMemsetAsync(anyb, 0, sizeof(int), s));
+    if (m->nloc == 4)
+      k_pattern_half<4><<<nblocks(n * kHalf, 256), 256, 0, s>>>(
+          n, m->elems, m->v2e_ptr, m->v2e, len, rowbuf, dposr, A->slotmap, A->big, anyb);
+    else
+      k_pattern_half<3><<<nblocks(n * kHalf, 256), 256, 0, s>>>(
+          n, m->elems, m->v2e_ptr, m->v2e, len, rowbuf, dposr, A->slotmap, A->big, anyb);
+    FFHIP_CHECK(hipGetLastError());
+    A->any_big = read_flag(ctx, anyb);
+    if (A->any_big)
+      k_pattern_rows<B, CAP, false><<<nblocks(n, B), B, 0, s>>>(
+          n, m->nloc, m->elems, m->v2e_ptr, m->v2e, len, nullptr, nullptr, nullptr, nullptr, nullptr,
+          ovf, A->big);
     FFHIP_CHECK(hipGetLastError());
     FFHIP_REQUIRE(read_flag(ctx, ovf) == 0, FFHIP_ERR_UNSUPPORTED,
                   "ffhip_pattern_p1: a row has more than %d columns", CAP);
@@ -497,10 +814,16 @@ int ffhip_pattern_p1(ffhip_ctx ctx, ffhip_mesh m, ffhip_mat *out) {
     A->nnz = read_i64(ctx, A->rowptr + n);
     A->colidx = dalloc<int32_t>(A->nnz);
     A->vals = dalloc<double>(A->nnz);
-    k_pattern_rows<B, CAP, true><<<nblocks(n, B), B, 0, s>>>(
-        n, m->nloc, m->elems, m->v2e_ptr, m->v2e, nullptr, A->rowptr, A->colidx, A->diag,
-        A->slotmap, nullptr, ovf);
+    k_pattern_compact<<<nblocks(n * kHalf, 256), 256, 0, s>>>(n, A->rowptr, rowbuf, dposr, A->big,
+                                                              A->colidx, A->diag);
+    if (A->any_big)
+      k_pattern_rows<B, CAP, true><<<nblocks(n, B), B, 0, s>>>(
+          n, m->nloc, m->elems, m->v2e_ptr, m->v2e, nullptr, A->rowptr, A->colidx, A->diag,
+          A->slotmap, nullptr, ovf, A->big);
     FFHIP_CHECK(hipGetLastError());
+    dfree(rowbuf);
+    dfree(dposr);
+    dfree(anyb);
     FFHIP_CHECK(hipMemsetAsync(A->vals, 0, 8 * A->nnz, s));
     ctx->toc();
     A->max_row = CAP;
@@ -546,9 +869,13 @@ int ffhip_assemble_p1(ffhip_ctx ctx, ffhip_mesh m, ffhip_mat A, int kind, double
     else
       k_p1_element_2d<<<nblocks(m->nt, 256), 256, 0, s>>>(m->nt, m->xyz, m->elems, kind, c0, kd, Ke);
     constexpr int B = 256, STAGE = 6144;
-    if (m->nloc == 4)
-      k_p1_gather<B, STAGE, 4><<<nblocks(A->n, B), B, 0, s>>>(A->n, A->rowptr, m->v2e_ptr, m->v2e,
-                                                               A->slotmap, Ke, A->vals);
+    if (m->nloc == 4) {
+      k_p1_values_wave<<<nblocks(A->n * kWave, 256), 256, 0, s>>>(
+          A->n, A->rowptr, m->v2e_ptr, m->v2e, A->slotmap, Ke, A->big, A->vals);
+      if (A->any_big)
+        k_p1_gather<B, STAGE, 4><<<nblocks(A->n, B), B, 0, s>>>(
+            A->n, A->rowptr, m->v2e_ptr, m->v2e, A->slotmap, Ke, A->vals, A->big);
+    }
     else
       k_p1_gather<B, STAGE, 3><<<nblocks(A->n, B), B, 0, s>>>(A->n, A->rowptr, m->v2e_ptr, m->v2e,
                                                                A->slotmap, Ke, A->vals);
@@ -747,7 +1074,7 @@ int ffhip_mat_free(ffhip_mat A) {
     if (!A) return;
     (void)hipStreamSynchronize(A->ctx->stream);
     dfree(A->rowptr); dfree(A->colidx); dfree(A->vals); dfree(A->diag); dfree(A->slotmap);
-    dfree(A->nodeptr); dfree(A->slot8);
+    dfree(A->nodeptr); dfree(A->slot8); dfree(A->big);
     delete A;
   });
 }

@@ -108,7 +108,8 @@ __global__ __launch_bounds__(BLOCK) void k_pattern_rows(
     const int64_t *__restrict__ v2e_ptr, const int32_t *__restrict__ v2e,
     int64_t *__restrict__ rowlen, const int64_t *__restrict__ rowptr,
     int32_t *__restrict__ colidx, int64_t *__restrict__ diag,
-    uint32_t *__restrict__ slotmap, uint8_t *__restrict__ slot8, int *__restrict__ overflow) {
+    uint32_t *__restrict__ slotmap, uint8_t *__restrict__ slot8, int *__restrict__ overflow,
+    const uint8_t *__restrict__ only = nullptr) {
   __shared__ int32_t L[CAP * BLOCK];
   const int t = threadIdx.x;
   const int64_t v = (int64_t)blockIdx.x * BLOCK + t;
@@ -116,6 +117,7 @@ __global__ __launch_bounds__(BLOCK) void k_pattern_rows(
   int u = 0;
   bool ovf = false;
   const int64_t k0 = v2e_ptr[v], k1 = v2e_ptr[v + 1];
+  if (only && !only[v]) return;  // rows the half-wave kernel built
   for (int64_t k = k0; k < k1 && !ovf; ++k) {
     const int64_t e = v2e[k] / nloc;
     for (int mloc = 0; mloc < nloc; ++mloc) {

@@ -98,6 +98,7 @@ struct ffhip_mesh_s {
   // dof -> incident (element, local vertex) list, element order (built lazily)
   int64_t *v2e_ptr = nullptr;  // nv + 1
   int32_t *v2e = nullptr;      // nt * nloc entries: e * nloc + l
+  int v2e_big = 0;             // some vertex has more than 32 incident elements
 };
 
 struct ffhip_mat_s {
@@ -112,6 +113,10 @@ struct ffhip_mat_s {
   uint32_t *slotmap = nullptr; // (nt*nloc) packed 4 x uint8 (P1, nloc <= 4)
   int max_row = 0;
   ffhip_mesh mesh = nullptr;   // mesh the pattern was built from (if any)
+  // P1 rows built by the lane-per-row kernels (more than 32 incident elements
+  // or 64 columns); the others by the half-wave kernels
+  uint8_t *big = nullptr;      // n
+  int any_big = 0;
   // vector-P2 matrices: node-level rows (the scalar rows 3a..3a+2 of node a
   // hold 3 x 3 blocks of its node columns) and the byte slot map, 10 offsets
   // per node incidence

@@ -329,3 +329,38 @@ def test_far_columns_use_32bit_layout():
     x0, x1 = outs[0][1][0], outs[1][1][0]
     assert outs[1][1][1]["converged"] == 1
     assert np.linalg.norm(x1[perm] - x0) <= 1e-8 * np.linalg.norm(x0)
+
+
+def _fan_mesh(m=20):
+    """A vertex with 2m incident tets (more than the 32 the half-wave kernels
+    take), ring vertices with 4, apexes with m."""
+    ang = 2 * np.pi * np.arange(m) / m
+    xyz = np.vstack([[0.0, 0.0, 0.0], np.c_[np.cos(ang), np.sin(ang), np.zeros(m)],
+                     [0.0, 0.0, 1.0], [0.0, 0.0, -1.0]])
+    top, bot = m + 1, m + 2
+    tets, tris = [], []
+    for i in range(m):
+        a, b = 1 + i, 1 + (i + 1) % m
+        tets += [[0, a, b, top], [0, b, a, bot]]
+        tris += [[a, b, top], [b, a, bot]]
+    tets = np.array(tets, np.int32)
+    X = xyz
+    for t in tets:
+        if np.linalg.det(np.c_[X[t[1]] - X[t[0]], X[t[2]] - X[t[0]], X[t[3]] - X[t[0]]]) < 0:
+            t[1], t[2] = t[2], t[1]
+    return dict(nv=len(xyz), nt=len(tets), nbe=len(tris), xyz=xyz.ravel().copy(),
+                tets=tets.ravel().copy(), btri=np.array(tris, np.int32).ravel().copy(),
+                blab=np.ones(len(tris), np.int32))
+
+
+def test_high_degree_vertex_pattern_and_values():
+    m = _fan_mesh(20)
+    rp, ci = ffo.pattern(m["nv"], m["tets"], 4)
+    vo = ffo.assemble_p1_3d(m, rp, ci, ffo.COEF_CONST, 1.0, None)
+    M = ffhip.mesh3_upload(m["xyz"], m["tets"], m["btri"], m["blab"])
+    A = ffhip.Matrix.p1(M)
+    A.assemble_p1(ffhip.COEF_CONST, 1.0)
+    r2, c2, v2 = A.download()
+    assert np.array_equal(r2, rp) and np.array_equal(c2, ci)
+    assert np.abs(v2 - vo).max() <= 1e-13 * np.abs(vo).max()
+    assert rp[1] - rp[0] == m["nv"]  # the centre couples to every vertex

@@ -58,6 +58,13 @@ constexpr int kU = FFHIP_SELL_U;
 constexpr int kSell = 0;
 constexpr int kCsrV = 1;
 constexpr int kSell16 = 2;  // sliced ELL with 16-bit column deltas (col - row)
+constexpr int kBsr3 = 3;    // 3x3 blocks of the [P2,P2,P2] node rows
+constexpr int kLanesPerBlockRow = 8;
+// MIXED: the FP32 residual is replaced by the FP64 one once (r32, r32) fell by
+// this factor since the last replacement (every two decades of the residual
+// norm: FP32 recurrences of ill-conditioned systems, e.g. vector P2
+// elasticity, drift if left longer; P1 Poisson is insensitive to it)
+constexpr double kMixUpdate = 1e-2;
 constexpr int kLanesPerRow = 16;
 
 struct CgState {
@@ -226,6 +233,34 @@ __global__ void k_sell_len4(int64_t S, const int32_t *__restrict__ len, int64_t 
   if (s < S) cnt[s] = (int64_t)((len[s] + 3) & ~3) * kSlice;
 }
 
+// 3x3 blocks of a vector-P2 matrix from its scalar CSR: block (a, q) holds
+// rows 3a..3a+2 at offsets 3q..3q+2 of those rows (values optionally through
+// tmap; FP32 copy of D^-1/2 A D^-1/2)
+__global__ void k_bsr_fill(int64_t nb, const int64_t *__restrict__ bptr,
+                           const int64_t *__restrict__ rowptr, const int32_t *__restrict__ colidx,
+                           const double *__restrict__ vals, const int64_t *__restrict__ tmap,
+                           const double *__restrict__ dsc, int32_t *__restrict__ bcol,
+                           double *__restrict__ bval, float *__restrict__ bval32) {
+  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= nb) return;
+  const int64_t kb0 = bptr[a], L = bptr[a + 1] - kb0;
+  for (int64_t q = 0; q < L; ++q) {
+    const int64_t kb = kb0 + q;
+    const int32_t c0 = colidx[rowptr[3 * a] + 3 * q];
+    if (bcol) bcol[kb] = c0 / 3;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const int64_t r = 3 * a + c, base = rowptr[r] + 3 * q;
+#pragma unroll
+      for (int cp = 0; cp < 3; ++cp) {
+        const double v = tmap ? vals[tmap[base + cp]] : vals[base + cp];
+        if (bval) bval[9 * kb + 3 * c + cp] = v;
+        if (bval32) bval32[9 * kb + 3 * c + cp] = (float)(v * dsc[r] * dsc[c0 + cp]);
+      }
+    }
+  }
+}
+
 // flags a row with a column farther than 16 bits from the row index
 __global__ void k_far_columns(int64_t n, const int64_t *__restrict__ rowptr,
                               const int32_t *__restrict__ colidx, int *__restrict__ far) {
@@ -308,11 +343,15 @@ __global__ void k_init_bc(int64_t n, const uint8_t *__restrict__ wcl, const doub
 }
 
 // ---------------------------------------------------------------- matrix formats
-// Every mat-vec kernel streams the matrix through one of two layouts:
-//   kSell: sliced ELL, 64-row slices (one lane per row, no reduction) — for
-//          matrices whose rows have nearly equal length (P1);
-//   kCsrV: CSR with 16 lanes per row and a shuffle reduction — for rows of
-//          very different lengths (vector P2), where sliced ELL would pad.
+// Every mat-vec kernel streams the matrix through one of these layouts:
+//   kSell:   sliced ELL, 64-row slices (one lane per row, no reduction) — for
+//            matrices whose rows have nearly equal length (P1);
+//   kSell16: the same with 16-bit column deltas packed four per lane;
+//   kCsrV:   CSR with 16 lanes per row and a shuffle reduction — for rows of
+//            very different lengths, where sliced ELL would pad;
+//   kBsr3:   3x3 blocks of a vector-P2 matrix (node rows of 3 scalar rows,
+//            one int32 block column and 9 values per block), 8 lanes per
+//            block row — 76 B per 9 FP64 entries instead of 108.
 template <class V>
 struct MatFmt {
   int64_t n, S;
@@ -321,7 +360,8 @@ struct MatFmt {
   const int32_t *col;  // kSell, kCsrV
   const int16_t *col16;  // kSell16: col - row, packed [group of 4][lane][4] per slice
   const int64_t *ptr16;  // kSell16: slice offsets of col16
-  const V *val;
+  const V *val;          // kBsr3: 9 values per block, row-major 3x3
+  int64_t nb;            // kBsr3: block rows; ptr = block row pointers, col = block columns
 };
 
 // Calls epi(r, sum_k a_rk X(j_k)) once for every row r of this block's range;
@@ -345,6 +385,52 @@ __device__ inline void matvec_rows(const MatFmt<V> &M, F xval, E epi) {
       else
         acc = sell_row<kU, V, ACC, int32_t>(M.col + base, M.val + base, L, r32, xval);
       if (r < M.n) epi(r, acc);
+    }
+  } else if (KIND == kBsr3) {
+    constexpr int GL = kLanesPerBlockRow, NG = kBlock / GL;
+    const int64_t G = gridDim.x, lb = xcd_block_id();
+    const int64_t a0 = (lb * M.nb) / G, a1 = ((lb + 1) * M.nb) / G;
+    const int g = threadIdx.x / GL, gl = threadIdx.x % GL;
+    for (int64_t ab = a0; ab < a1; ab += NG) {
+      const int64_t a = ab + g;
+      ACC s0 = 0, s1 = 0, s2 = 0;
+      if (a < a1) {
+        const int64_t k0 = M.ptr[a], k1 = M.ptr[a + 1];
+        for (int64_t k = k0 + gl; k < k1; k += 2 * GL) {
+          int32_t b[2];
+          V w[2][9];
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            const int64_t kk = k + u * GL;
+            b[u] = 0;
+#pragma unroll
+            for (int i = 0; i < 9; ++i) w[u][i] = 0;
+            if (kk < k1) {
+              b[u] = ntload(M.col + kk);
+#pragma unroll
+              for (int i = 0; i < 9; ++i) w[u][i] = ntload(M.val + 9 * kk + i);
+            }
+          }
+          ACC xv[2][3];
+#pragma unroll
+          for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) xv[u][c] = xval(3 * b[u] + c);
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            s0 += (ACC)w[u][0] * xv[u][0] + (ACC)w[u][1] * xv[u][1] + (ACC)w[u][2] * xv[u][2];
+            s1 += (ACC)w[u][3] * xv[u][0] + (ACC)w[u][4] * xv[u][1] + (ACC)w[u][5] * xv[u][2];
+            s2 += (ACC)w[u][6] * xv[u][0] + (ACC)w[u][7] * xv[u][1] + (ACC)w[u][8] * xv[u][2];
+          }
+        }
+      }
+#pragma unroll
+      for (int off = GL / 2; off > 0; off >>= 1) {
+        s0 += __shfl_xor(s0, off, kWave);
+        s1 += __shfl_xor(s1, off, kWave);
+        s2 += __shfl_xor(s2, off, kWave);
+      }
+      if (a < a1 && gl == 0) { epi(3 * a, s0); epi(3 * a + 1, s1); epi(3 * a + 2, s2); }
     }
   } else {
     const int64_t G = gridDim.x, lb = xcd_block_id();
@@ -535,7 +621,8 @@ struct MixState {
 };
 
 // K_m1: p = r32 + cb*p_old (on the fly for every gathered column); q = Ahat p;
-// partial (p, q).  Requests a reliable update when (r32, r32) fell by 1e-4
+// partial (p, q).  Requests a reliable update when (r32, r32) fell by
+// kMixUpdate
 // since the last update, or the recurrence says the FP64 criterion is met.
 template <int KIND>
 __global__ FFHIP_MATVEC_ATTR void k_mix_dir(
@@ -550,7 +637,7 @@ __global__ FFHIP_MATVEC_ATTR void k_mix_dir(
   const double sig = st->sigma;
   int req = 0;
   if (rho != rho) req = 4;
-  else if (rho * sig * sig < st->eps2 || rho < 1e-4) req = 1;  // check on the FP64 residual
+  else if (rho * sig * sig < st->eps2 || rho < kMixUpdate) req = 1;  // check on the FP64 residual
   else if (it - 1 >= itmax) req = 3;
   if (req) {
     if (lead) {
@@ -649,6 +736,7 @@ struct ffhip_cg_s {
   int32_t *slen = nullptr, *scol = nullptr;
   int16_t *scol16 = nullptr;  // kSell16
   int64_t *sptr16 = nullptr;  // kSell16 slice offsets of scol16
+  int32_t *bcol = nullptr;    // kBsr3 block columns (block row pointers: A's node rows)
   // values in the solver's layout; for kCsrV, sval is A's value array (borrowed)
   double *sval = nullptr;    // A
   double *svalT = nullptr;   // A^T (built on first transposed solve)
@@ -700,7 +788,9 @@ struct ffhip_cg_s {
     M.S = S;
     M.col16 = scol16;
     M.ptr16 = sptr16;
+    M.nb = 0;
     if (fmt == kSell || fmt == kSell16) { M.ptr = sptr; M.len = slen; M.col = scol; }
+    else if (fmt == kBsr3) { M.ptr = A->nodeptr; M.len = nullptr; M.col = bcol; M.nb = n / 3; }
     else { M.ptr = A->rowptr; M.len = nullptr; M.col = A->colidx; }
     M.val = val;
     return M;
@@ -712,6 +802,7 @@ struct ffhip_cg_s {
   do {                                                                       \
     if ((cg)->fmt == kSell) kern<kSell><<<(grid), kBlock, 0, (stream)>>>(__VA_ARGS__); \
     else if ((cg)->fmt == kSell16) kern<kSell16><<<(grid), kBlock, 0, (stream)>>>(__VA_ARGS__); \
+    else if ((cg)->fmt == kBsr3) kern<kBsr3><<<(grid), kBlock, 0, (stream)>>>(__VA_ARGS__); \
     else kern<kCsrV><<<(grid), kBlock, 0, (stream)>>>(__VA_ARGS__);           \
   } while (0)
 
@@ -721,6 +812,8 @@ struct ffhip_cg_s {
       k_matvec<kSell, T, T><<<(cg)->grid, kBlock, 0, (stream)>>>(M, (const T *)(x), (T *)(y)); \
     else if ((cg)->fmt == kSell16)                                                          \
       k_matvec<kSell16, T, T><<<(cg)->grid, kBlock, 0, (stream)>>>(M, (const T *)(x), (T *)(y)); \
+    else if ((cg)->fmt == kBsr3)                                                            \
+      k_matvec<kBsr3, T, T><<<(cg)->grid, kBlock, 0, (stream)>>>(M, (const T *)(x), (T *)(y)); \
     else                                                                                    \
       k_matvec<kCsrV, T, T><<<(cg)->grid, kBlock, 0, (stream)>>>(M, (const T *)(x), (T *)(y)); \
   } while (0)
@@ -732,7 +825,14 @@ static void cg_plan_layout(ffhip_cg cg, int layout, bool index32) {
   ffhip_mat A = cg->A;
   hipStream_t s = ctx->stream;
   cg->S = (A->n + kSlice - 1) / kSlice;
-  if (layout != FFHIP_CG_LAYOUT_CSR) {
+  if (A->nodeptr && (layout == FFHIP_CG_LAYOUT_AUTO || layout == FFHIP_CG_LAYOUT_BSR3)) {
+    // vector P2: the node rows are 3x3 blocks
+    cg->fmt = kBsr3;
+    cg->ents = A->nnz;
+    cg->bcol = dalloc<int32_t>(A->nnz / 9);
+  } else if (layout == FFHIP_CG_LAYOUT_BSR3) {
+    FFHIP_REQUIRE(false, FFHIP_ERR_UNSUPPORTED, "ffhip_cg_create: 3x3 blocks need a [P2,P2,P2] matrix");
+  } else if (layout != FFHIP_CG_LAYOUT_CSR) {
     cg->sptr = dalloc<int64_t>(cg->S + 1);
     cg->slen = dalloc<int32_t>(cg->S);
     int64_t *cnt = dalloc<int64_t>(cg->S + 1);
@@ -775,7 +875,9 @@ static void cg_plan_layout(ffhip_cg cg, int layout, bool index32) {
     cg->fmt = kCsrV;
   }
   int64_t g;
-  if (cg->fmt != kCsrV) {
+  if (cg->fmt == kBsr3) {
+    g = (A->n / 3 + kBlock / kLanesPerBlockRow - 1) / (kBlock / kLanesPerBlockRow);
+  } else if (cg->fmt != kCsrV) {
     g = (cg->S + kWavesPerBlock - 1) / kWavesPerBlock;
   } else {
     cg->ents = A->nnz;
@@ -805,7 +907,13 @@ static void cg_fill_values(ffhip_cg cg, bool transposed) {
   double *&dv = transposed ? cg->svalT : cg->sval;
   float *&fv = transposed ? cg->sval32T : cg->sval32;
   if (mixed && !fv) fv = dalloc<float>(cg->ents);
-  if (cg->fmt != kCsrV) {
+  if (cg->fmt == kBsr3) {
+    if (!dv) dv = dalloc<double>(cg->ents);
+    k_bsr_fill<<<nblk(A->n / 3, 256), 256, 0, s>>>(A->n / 3, A->nodeptr, A->rowptr, A->colidx,
+                                                   A->vals, tmap, cg->dsc,
+                                                   transposed ? nullptr : cg->bcol, dv,
+                                                   mixed ? fv : nullptr);
+  } else if (cg->fmt != kCsrV) {
     if (!dv) dv = dalloc<double>(cg->ents);
     k_sell_fill<<<nblk(cg->S * kSlice, 256), 256, 0, s>>>(
         A->n, cg->S, A->rowptr, A->colidx, A->vals, tmap, cg->sptr, cg->slen, cg->dsc,
@@ -967,7 +1075,7 @@ int ffhip_cg_solve_mixed(ffhip_cg cg, const double *sval, const float *sval32, d
 static void cg_release(ffhip_cg cg) {
   for (hipEvent_t e : cg->ev) (void)hipEventDestroy(e);
   if (cg->fmt == kCsrV) cg->sval = nullptr;  // A's array
-  dfree(cg->sptr); dfree(cg->slen); dfree(cg->scol); dfree(cg->scol16); dfree(cg->sptr16); dfree(cg->sval);
+  dfree(cg->sptr); dfree(cg->slen); dfree(cg->scol); dfree(cg->scol16); dfree(cg->sptr16); dfree(cg->bcol); dfree(cg->sval);
   dfree(cg->svalT);
   dfree(cg->sval32); dfree(cg->sval32T); dfree(cg->dinv); dfree(cg->dsc); dfree(cg->wcl);
   dfree(cg->xw); dfree(cg->bw); dfree(cg->G); dfree(cg->CG); dfree(cg->H[0]); dfree(cg->H[1]);
@@ -981,11 +1089,12 @@ extern "C" {
 int ffhip_cg_create(ffhip_ctx ctx, ffhip_mat A, int mode, ffhip_cg *out) {
   return guard([&] {
     FFHIP_REQUIRE(ctx && A && out, FFHIP_ERR_ARG, "ffhip_cg_create: null argument");
-    const int prec = mode & 0xf, layout = mode & 0x30;
+    const int prec = mode & 0xf, layout = mode & 0x130;
     const bool index32 = (mode & FFHIP_CG_INDEX32) != 0;
     FFHIP_REQUIRE((prec == FFHIP_CG_FP64 || prec == FFHIP_CG_MIXED) &&
-                      (mode & ~(0x3f | FFHIP_CG_INDEX32)) == 0 &&
-                      layout != (FFHIP_CG_LAYOUT_SELL | FFHIP_CG_LAYOUT_CSR),
+                      (mode & ~(0x13f | FFHIP_CG_INDEX32)) == 0 &&
+                      (layout == 0 || layout == FFHIP_CG_LAYOUT_SELL || layout == FFHIP_CG_LAYOUT_CSR ||
+                       layout == FFHIP_CG_LAYOUT_BSR3),
                   FFHIP_ERR_ARG, "ffhip_cg_create: mode %d", mode);
     ffhip_cg cg = new ffhip_cg_s();
     cg->ctx = ctx; cg->A = A; cg->mode = prec; cg->n = A->n;
@@ -1023,8 +1132,9 @@ int ffhip_cg_layout(ffhip_cg cg, int *layout, int64_t *entries) {
   return guard([&] {
     FFHIP_REQUIRE(cg, FFHIP_ERR_ARG, "ffhip_cg_layout: null solver");
     if (layout)
-      *layout = cg->fmt == kCsrV ? FFHIP_CG_LAYOUT_CSR
-                                 : (FFHIP_CG_LAYOUT_SELL | (cg->fmt == kSell16 ? FFHIP_CG_INDEX16 : 0));
+      *layout = cg->fmt == kCsrV    ? FFHIP_CG_LAYOUT_CSR
+                : cg->fmt == kBsr3 ? FFHIP_CG_LAYOUT_BSR3
+                                   : (FFHIP_CG_LAYOUT_SELL | (cg->fmt == kSell16 ? FFHIP_CG_INDEX16 : 0));
     if (entries) *entries = cg->ents;
   });
 }

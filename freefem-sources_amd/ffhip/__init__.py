@@ -25,7 +25,7 @@ LIBPATH = os.environ.get("FFHIP_LIB") or os.path.join(os.path.dirname(HERE), "li
 
 COEF_CONST, COEF_SIN3, COEF_P0, COEF_QP = 0, 1, 2, 3
 CG_FP64, CG_MIXED = 0, 1
-CG_LAYOUT_AUTO, CG_LAYOUT_SELL, CG_LAYOUT_CSR = 0, 0x10, 0x20
+CG_LAYOUT_AUTO, CG_LAYOUT_SELL, CG_LAYOUT_CSR, CG_LAYOUT_BSR3 = 0, 0x10, 0x20, 0x100
 CG_INDEX16, CG_INDEX32 = 0x40, 0x80
 ERRORS = {-1: "bad argument", -2: "HIP error", -3: "CG did not converge", -4: "NaN in CG",
           -5: "unsupported", -6: "internal error"}
@@ -411,23 +411,25 @@ def rhs_p2v(space, f=(0.0, 0.0, -1.0), labels=(), tgv=1e30, g=None, out=None):
 class CG:
     """Jacobi-CG solver of a device matrix (SolverCG).  layout: "auto", "sell"
     (sliced ELL, 16-bit column deltas where they fit), "sell32" (sliced ELL,
-    32-bit columns) or "csr" (16 lanes per row); auto takes sliced ELL unless
-    it pads by more than 25 %."""
+    32-bit columns), "csr" (16 lanes per row) or "bsr3" (3x3 blocks of a
+    [P2,P2,P2] matrix); auto takes bsr3 for [P2,P2,P2] matrices, else sliced
+    ELL unless it pads by more than 25 %."""
 
     def __init__(self, A, mode=CG_FP64, layout="auto"):
         h = ctypes.c_void_p()
         flag = {"auto": CG_LAYOUT_AUTO, "sell": CG_LAYOUT_SELL,
-                "sell32": CG_LAYOUT_SELL | CG_INDEX32, "csr": CG_LAYOUT_CSR}[layout]
+                "sell32": CG_LAYOUT_SELL | CG_INDEX32, "csr": CG_LAYOUT_CSR,
+                "bsr3": CG_LAYOUT_BSR3}[layout]
         check(lib().ffhip_cg_create(A.ctx.h, A.h, mode | flag, ctypes.byref(h)))
         self.h, self.A, self.mode = h, A, mode
 
     def layout(self):
-        """(layout name "sell16" | "sell" | "csr", entries streamed per mat-vec)"""
+        """(layout name "sell16" | "sell" | "csr" | "bsr3", entries streamed per mat-vec)"""
         k, e = ctypes.c_int(), ctypes.c_int64()
         check(lib().ffhip_cg_layout(self.h, ctypes.byref(k), ctypes.byref(e)))
         if k.value & CG_LAYOUT_SELL:
             return ("sell16" if k.value & CG_INDEX16 else "sell"), e.value
-        return "csr", e.value
+        return ("bsr3" if k.value & CG_LAYOUT_BSR3 else "csr"), e.value
 
     def set_profiling(self, on=True):
         check(lib().ffhip_cg_set_profiling(self.h, int(on)))

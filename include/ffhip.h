@@ -178,10 +178,12 @@ int ffhip_rhs_p2v(ffhip_ctx ctx, ffhip_p2v space, const double f[3],
 /* ---- Jacobi-preconditioned CG ------------------------------------------ */
 enum { FFHIP_CG_FP64 = 0, FFHIP_CG_MIXED = 1 };
 /* Layout of the streamed matrix, OR'ed into mode: sliced ELL with 64-row
- * slices (one lane per row), or CSR with 16 lanes per row.  AUTO takes sliced
- * ELL unless its slices would pad the matrix by more than 25 % (rows of very
- * different lengths, e.g. vector P2), then CSR. */
-enum { FFHIP_CG_LAYOUT_AUTO = 0, FFHIP_CG_LAYOUT_SELL = 0x10, FFHIP_CG_LAYOUT_CSR = 0x20 };
+ * slices (one lane per row), CSR with 16 lanes per row, or (matrices of
+ * ffhip_pattern_p2v only) 3x3 blocks of the node rows.  AUTO takes the 3x3
+ * blocks for a [P2,P2,P2] matrix, else sliced ELL unless its slices would pad
+ * the matrix by more than 25 %, then CSR. */
+enum { FFHIP_CG_LAYOUT_AUTO = 0, FFHIP_CG_LAYOUT_SELL = 0x10, FFHIP_CG_LAYOUT_CSR = 0x20,
+       FFHIP_CG_LAYOUT_BSR3 = 0x100 };
 /* Sliced ELL streams 16-bit column deltas (column - row) when every column is
  * within 2^15 of its row, else 32-bit columns; FFHIP_CG_INDEX32 in mode forces
  * 32-bit columns.  ffhip_cg_layout reports FFHIP_CG_INDEX16 with the layout
@@ -213,7 +215,7 @@ typedef struct {
  * FFHIP_CG_MIXED, optionally OR'ed with a FFHIP_CG_LAYOUT_* value. */
 int ffhip_cg_create(ffhip_ctx ctx, ffhip_mat mat, int mode, ffhip_cg *cg);
 /* Layout the solver chose (FFHIP_CG_LAYOUT_SELL, possibly | FFHIP_CG_INDEX16,
- * or FFHIP_CG_LAYOUT_CSR) and the number of
+ * FFHIP_CG_LAYOUT_CSR or FFHIP_CG_LAYOUT_BSR3) and the number of
  * matrix entries one mat-vec streams (padding included).  No reference
  * counterpart (a property of this solver's device layout). */
 int ffhip_cg_layout(ffhip_cg cg, int *layout, int64_t *entries);

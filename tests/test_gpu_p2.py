@@ -72,22 +72,39 @@ def test_p2_system_vs_oracle(case):
     b = ffhip.rhs_p2v(sp, (0.0, 0.0, -1.0), [4])
     ob = o["b"]
     assert np.abs(b - ob).max() <= 1e-14 * np.abs(ob).max()
-    for mode in (ffhip.CG_FP64, ffhip.CG_MIXED):
-        cg = ffhip.CG(A, mode)
-        x, st = cg.solve(b, eps=1e-10)
-        xo, so = ffo.pcg(o["rowptr"], o["colidx"], o["vals"], ob, eps=1e-10)
-        assert st["converged"] == 1
-        assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
+    xo, so = ffo.pcg(o["rowptr"], o["colidx"], o["vals"], ob, eps=1e-10)
+    for layout in ("bsr3", "csr"):
+        for mode in (ffhip.CG_FP64, ffhip.CG_MIXED):
+            cg = ffhip.CG(A, mode, layout=layout)
+            assert cg.layout()[0] == layout
+            x, st = cg.solve(b, eps=1e-10)
+            assert st["converged"] == 1
+            assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
+            if mode == ffhip.CG_FP64:
+                assert abs(st["iters"] - so["iters"]) <= 2
+        # transposed solve (A is symmetric: the same solution)
+        xt, stt = ffhip.CG(A, ffhip.CG_FP64, layout=layout).solve(b, eps=1e-10, trans=True)
+        assert np.linalg.norm(xt - xo) <= 1e-8 * np.linalg.norm(xo)
 
 
-def test_p2_layout_is_csr_for_vector_p2():
+def test_p2_layout_is_blocks_for_vector_p2():
+    import torch
     m = ffo.cube(4)
     sp = ffhip.P2VSpace(_gpu_mesh(m))
     A = ffhip.Matrix.p2v(sp)
     A.assemble_p2v_elasticity()
     A.set_bc_p2v([4])
     lay, ents = ffhip.CG(A).layout()
-    assert lay == "csr" and ents == A.nnz
+    assert lay == "bsr3" and ents == A.nnz
+    # the block mat-vec kernels (FP64, and FP32 of D^-1/2 A D^-1/2) against the CSR product
+    rp, ci, va = A.download()
+    x = np.random.default_rng(3).standard_normal(A.n)
+    y = ffo.spmv(rp, ci, va, x)
+    cg = ffhip.CG(A, ffhip.CG_MIXED, layout="bsr3")
+    xd = torch.tensor(x, device="cuda")
+    yd = torch.empty_like(xd)
+    cg.bench_spmv(ffhip.CG_FP64, 2, xd, yd)
+    assert np.allclose(yd.cpu().numpy(), y, rtol=1e-13, atol=1e-13 * np.abs(y).max())
 
 
 def test_p2_cube32_configuration():

@@ -59,7 +59,6 @@ constexpr int kSell = 0;
 constexpr int kCsrV = 1;
 constexpr int kSell16 = 2;  // sliced ELL with 16-bit column deltas (col - row)
 constexpr int kBsr3 = 3;    // 3x3 blocks of the [P2,P2,P2] node rows
-constexpr int kLanesPerBlockRow = 8;
 // MIXED: the FP32 residual is replaced by the FP64 one once (r32, r32) fell by
 // this factor since the last replacement (every two decades of the residual
 // norm: FP32 recurrences of ill-conditioned systems, e.g. vector P2
@@ -233,29 +232,74 @@ __global__ void k_sell_len4(int64_t S, const int32_t *__restrict__ len, int64_t 
   if (s < S) cnt[s] = (int64_t)((len[s] + 3) & ~3) * kSlice;
 }
 
-// 3x3 blocks of a vector-P2 matrix from its scalar CSR: block (a, q) holds
-// rows 3a..3a+2 at offsets 3q..3q+2 of those rows (values optionally through
-// tmap; FP32 copy of D^-1/2 A D^-1/2)
-__global__ void k_bsr_fill(int64_t nb, const int64_t *__restrict__ bptr,
-                           const int64_t *__restrict__ rowptr, const int32_t *__restrict__ colidx,
-                           const double *__restrict__ vals, const int64_t *__restrict__ tmap,
-                           const double *__restrict__ dsc, int32_t *__restrict__ bcol,
-                           double *__restrict__ bval, float *__restrict__ bval32) {
+// node-row sort keys: window of 4096 rows in the high bits, longer rows first
+__global__ void k_bsr_keys(int64_t nb, const int64_t *__restrict__ bptr, uint32_t *__restrict__ key,
+                           int32_t *__restrict__ val) {
   const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= nb) return;
-  const int64_t kb0 = bptr[a], L = bptr[a + 1] - kb0;
-  for (int64_t q = 0; q < L; ++q) {
-    const int64_t kb = kb0 + q;
-    const int32_t c0 = colidx[rowptr[3 * a] + 3 * q];
-    if (bcol) bcol[kb] = c0 / 3;
+  const int64_t L = bptr[a + 1] - bptr[a];
+  key[a] = ((uint32_t)(a >> 12) << 8) | (uint32_t)(255 - (L < 255 ? L : 255));
+  val[a] = (int32_t)a;
+}
+
+// slice widths of the sorted node rows
+__global__ void k_bsr_len(int64_t nb, int64_t S3, const int64_t *__restrict__ bptr,
+                          const int32_t *__restrict__ perm, int32_t *__restrict__ len,
+                          int64_t *__restrict__ cnt) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= S3) return;
+  int64_t m = 0;
+  for (int l = 0; l < kSlice; ++l) {
+    const int64_t i = s * kSlice + l;
+    if (i < nb) {
+      const int32_t a = perm[i];
+      const int64_t c = bptr[a + 1] - bptr[a];
+      m = c > m ? c : m;
+    }
+  }
+  len[s] = (int32_t)m;
+  cnt[s] = m * kSlice;
+}
+
+// 3x3 blocks of a vector-P2 matrix from its scalar CSR: entry k of sorted node
+// row i (node a = perm[i]) holds rows 3a..3a+2 at offsets 3k..3k+2 of those
+// rows (values optionally through tmap; FP32 copy of D^-1/2 A D^-1/2)
+__global__ void k_bsr_fill(int64_t nb, int64_t ents, const int64_t *__restrict__ bptr,
+                           const int32_t *__restrict__ perm, const int64_t *__restrict__ sptr,
+                           const int32_t *__restrict__ slen, const int64_t *__restrict__ rowptr,
+                           const int32_t *__restrict__ colidx, const double *__restrict__ vals,
+                           const int64_t *__restrict__ tmap, const double *__restrict__ dsc,
+                           int32_t *__restrict__ bcol, double *__restrict__ bval,
+                           float *__restrict__ bval32) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t S3 = (nb + kSlice - 1) / kSlice;
+  if (i >= S3 * kSlice) return;
+  const int64_t s = i / kSlice;
+  const int lane = (int)(i % kSlice);
+  const int32_t a = i < nb ? perm[i] : 0;
+  const int64_t L = i < nb ? bptr[a + 1] - bptr[a] : 0;
+  const int W = slen[s];
+  for (int k = 0; k < W; ++k) {
+    const int64_t e = sptr[s] + (int64_t)k * kSlice + lane;
+    if (k < L) {
+      const int32_t c0 = colidx[rowptr[3 * (int64_t)a] + 3 * k];
+      if (bcol) bcol[e] = c0 / 3;
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const int64_t r = 3 * a + c, base = rowptr[r] + 3 * q;
+      for (int c = 0; c < 3; ++c) {
+        const int64_t r = 3 * (int64_t)a + c, src = rowptr[r] + 3 * k;
 #pragma unroll
-      for (int cp = 0; cp < 3; ++cp) {
-        const double v = tmap ? vals[tmap[base + cp]] : vals[base + cp];
-        if (bval) bval[9 * kb + 3 * c + cp] = v;
-        if (bval32) bval32[9 * kb + 3 * c + cp] = (float)(v * dsc[r] * dsc[c0 + cp]);
+        for (int cp = 0; cp < 3; ++cp) {
+          const double v = tmap ? vals[tmap[src + cp]] : vals[src + cp];
+          if (bval) bval[(3 * c + cp) * ents + e] = v;
+          if (bval32) bval32[(3 * c + cp) * ents + e] = (float)(v * dsc[r] * dsc[c0 + cp]);
+        }
+      }
+    } else {
+      if (bcol) bcol[e] = a;
+#pragma unroll
+      for (int q = 0; q < 9; ++q) {
+        if (bval) bval[q * ents + e] = 0.;
+        if (bval32) bval32[q * ents + e] = 0.f;
       }
     }
   }
@@ -349,9 +393,11 @@ __global__ void k_init_bc(int64_t n, const uint8_t *__restrict__ wcl, const doub
 //   kSell16: the same with 16-bit column deltas packed four per lane;
 //   kCsrV:   CSR with 16 lanes per row and a shuffle reduction — for rows of
 //            very different lengths, where sliced ELL would pad;
-//   kBsr3:   3x3 blocks of a vector-P2 matrix (node rows of 3 scalar rows,
-//            one int32 block column and 9 values per block), 8 lanes per
-//            block row — 76 B per 9 FP64 entries instead of 108.
+//   kBsr3:   3x3 blocks of a vector-P2 matrix, sliced ELL over node rows:
+//            the node rows are sorted by length inside windows of 4096 rows
+//            (so a 64-row slice pads little), one lane per node row (3 scalar
+//            rows), one int32 block column and 9 value planes per slice entry
+//            — 76 B per 9 FP64 entries instead of 108, every load coalesced.
 template <class V>
 struct MatFmt {
   int64_t n, S;
@@ -360,8 +406,9 @@ struct MatFmt {
   const int32_t *col;  // kSell, kCsrV
   const int16_t *col16;  // kSell16: col - row, packed [group of 4][lane][4] per slice
   const int64_t *ptr16;  // kSell16: slice offsets of col16
-  const V *val;          // kBsr3: 9 values per block, row-major 3x3
-  int64_t nb;            // kBsr3: block rows; ptr = block row pointers, col = block columns
+  const V *val;          // kBsr3: 9 planes of ents values (entry 3c+c' of every block)
+  int64_t nb, ents;      // kBsr3: node rows, slice entries (ptr/len/col: slices, perm: rows)
+  const int32_t *perm;   // kBsr3: node row of each sorted position
 };
 
 // Calls epi(r, sum_k a_rk X(j_k)) once for every row r of this block's range;
@@ -387,50 +434,49 @@ __device__ inline void matvec_rows(const MatFmt<V> &M, F xval, E epi) {
       if (r < M.n) epi(r, acc);
     }
   } else if (KIND == kBsr3) {
-    constexpr int GL = kLanesPerBlockRow, NG = kBlock / GL;
-    const int64_t G = gridDim.x, lb = xcd_block_id();
-    const int64_t a0 = (lb * M.nb) / G, a1 = ((lb + 1) * M.nb) / G;
-    const int g = threadIdx.x / GL, gl = threadIdx.x % GL;
-    for (int64_t ab = a0; ab < a1; ab += NG) {
-      const int64_t a = ab + g;
-      ACC s0 = 0, s1 = 0, s2 = 0;
-      if (a < a1) {
-        const int64_t k0 = M.ptr[a], k1 = M.ptr[a + 1];
-        for (int64_t k = k0 + gl; k < k1; k += 2 * GL) {
-          int32_t b[2];
-          V w[2][9];
+    constexpr int U = 2;
+    const int64_t S3 = (M.nb + kSlice - 1) / kSlice;
+    int64_t s0, s1;
+    block_slices(S3, s0, s1);
+    const int w = wave_id(), lane = threadIdx.x % kSlice;
+    for (int64_t s = s0 + w; s < s1; s += kWavesPerBlock) {
+      const int64_t i = s * kSlice + lane;
+      const int32_t a = i < M.nb ? M.perm[i] : -1;
+      const int64_t base = uniform(M.ptr[s]) + lane;
+      const int L = __builtin_amdgcn_readfirstlane(M.len[s]);
+      ACC s0v = 0, s1v = 0, s2v = 0;
+      for (int k = 0; k < L; k += U) {
+        int32_t b[U];
+        V wv[U][9];
 #pragma unroll
-          for (int u = 0; u < 2; ++u) {
-            const int64_t kk = k + u * GL;
-            b[u] = 0;
+        for (int u = 0; u < U; ++u) {
+          b[u] = a < 0 ? 0 : a;  // past the row end: the row's own block, zero values
 #pragma unroll
-            for (int i = 0; i < 9; ++i) w[u][i] = 0;
-            if (kk < k1) {
-              b[u] = ntload(M.col + kk);
+          for (int q = 0; q < 9; ++q) wv[u][q] = 0;
+          if (k + u < L) {
+            const int64_t e = base + (int64_t)(k + u) * kSlice;
+            b[u] = ntload(M.col + e);
 #pragma unroll
-              for (int i = 0; i < 9; ++i) w[u][i] = ntload(M.val + 9 * kk + i);
-            }
-          }
-          ACC xv[2][3];
-#pragma unroll
-          for (int u = 0; u < 2; ++u)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) xv[u][c] = xval(3 * b[u] + c);
-#pragma unroll
-          for (int u = 0; u < 2; ++u) {
-            s0 += (ACC)w[u][0] * xv[u][0] + (ACC)w[u][1] * xv[u][1] + (ACC)w[u][2] * xv[u][2];
-            s1 += (ACC)w[u][3] * xv[u][0] + (ACC)w[u][4] * xv[u][1] + (ACC)w[u][5] * xv[u][2];
-            s2 += (ACC)w[u][6] * xv[u][0] + (ACC)w[u][7] * xv[u][1] + (ACC)w[u][8] * xv[u][2];
+            for (int q = 0; q < 9; ++q) wv[u][q] = ntload(M.val + q * M.ents + e);
           }
         }
-      }
+        ACC xv[U][3];
 #pragma unroll
-      for (int off = GL / 2; off > 0; off >>= 1) {
-        s0 += __shfl_xor(s0, off, kWave);
-        s1 += __shfl_xor(s1, off, kWave);
-        s2 += __shfl_xor(s2, off, kWave);
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int c = 0; c < 3; ++c) xv[u][c] = xval(3 * b[u] + c);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          s0v += (ACC)wv[u][0] * xv[u][0] + (ACC)wv[u][1] * xv[u][1] + (ACC)wv[u][2] * xv[u][2];
+          s1v += (ACC)wv[u][3] * xv[u][0] + (ACC)wv[u][4] * xv[u][1] + (ACC)wv[u][5] * xv[u][2];
+          s2v += (ACC)wv[u][6] * xv[u][0] + (ACC)wv[u][7] * xv[u][1] + (ACC)wv[u][8] * xv[u][2];
+        }
       }
-      if (a < a1 && gl == 0) { epi(3 * a, s0); epi(3 * a + 1, s1); epi(3 * a + 2, s2); }
+      if (a >= 0) {
+        epi(3 * (int64_t)a, s0v);
+        epi(3 * (int64_t)a + 1, s1v);
+        epi(3 * (int64_t)a + 2, s2v);
+      }
     }
   } else {
     const int64_t G = gridDim.x, lb = xcd_block_id();
@@ -736,7 +782,8 @@ struct ffhip_cg_s {
   int32_t *slen = nullptr, *scol = nullptr;
   int16_t *scol16 = nullptr;  // kSell16
   int64_t *sptr16 = nullptr;  // kSell16 slice offsets of scol16
-  int32_t *bcol = nullptr;    // kBsr3 block columns (block row pointers: A's node rows)
+  int32_t *bcol = nullptr;    // kBsr3 block columns (slices over sorted node rows)
+  int32_t *bperm = nullptr;   // kBsr3 node row of each sorted position
   // values in the solver's layout; for kCsrV, sval is A's value array (borrowed)
   double *sval = nullptr;    // A
   double *svalT = nullptr;   // A^T (built on first transposed solve)
@@ -789,8 +836,10 @@ struct ffhip_cg_s {
     M.col16 = scol16;
     M.ptr16 = sptr16;
     M.nb = 0;
+    M.ents = ents;
+    M.perm = bperm;
     if (fmt == kSell || fmt == kSell16) { M.ptr = sptr; M.len = slen; M.col = scol; }
-    else if (fmt == kBsr3) { M.ptr = A->nodeptr; M.len = nullptr; M.col = bcol; M.nb = n / 3; }
+    else if (fmt == kBsr3) { M.ptr = sptr; M.len = slen; M.col = bcol; M.nb = n / 3; }
     else { M.ptr = A->rowptr; M.len = nullptr; M.col = A->colidx; }
     M.val = val;
     return M;
@@ -826,10 +875,29 @@ static void cg_plan_layout(ffhip_cg cg, int layout, bool index32) {
   hipStream_t s = ctx->stream;
   cg->S = (A->n + kSlice - 1) / kSlice;
   if (A->nodeptr && (layout == FFHIP_CG_LAYOUT_AUTO || layout == FFHIP_CG_LAYOUT_BSR3)) {
-    // vector P2: the node rows are 3x3 blocks
+    // vector P2: the node rows are 3x3 blocks; slices over node rows sorted by
+    // length inside windows of 4096 rows
     cg->fmt = kBsr3;
-    cg->ents = A->nnz;
-    cg->bcol = dalloc<int32_t>(A->nnz / 9);
+    const int64_t nb = A->n / 3, S3 = (nb + kSlice - 1) / kSlice;
+    uint32_t *keys = dalloc<uint32_t>(nb), *skeys = dalloc<uint32_t>(nb);
+    int32_t *vals = dalloc<int32_t>(nb);
+    cg->bperm = dalloc<int32_t>(nb);
+    k_bsr_keys<<<nblk(nb, 256), 256, 0, s>>>(nb, A->nodeptr, keys, vals);
+    size_t tb = 0;
+    FFHIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, keys, skeys, vals, cg->bperm, (int)nb,
+                                                   0, 32, s));
+    char *tmp = dalloc<char>(tb);
+    FFHIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, keys, skeys, vals, cg->bperm, (int)nb, 0,
+                                                   32, s));
+    cg->sptr = dalloc<int64_t>(S3 + 1);
+    cg->slen = dalloc<int32_t>(S3);
+    int64_t *cnt = dalloc<int64_t>(S3 + 1);
+    FFHIP_CHECK(hipMemsetAsync(cnt + S3, 0, 8, s));
+    k_bsr_len<<<nblk(S3, 256), 256, 0, s>>>(nb, S3, A->nodeptr, cg->bperm, cg->slen, cnt);
+    exclusive_scan_i64(ctx, cnt, cg->sptr, S3 + 1);
+    cg->ents = read_i64(ctx, cg->sptr + S3);
+    dfree(cnt); dfree(tmp); dfree(keys); dfree(skeys); dfree(vals);
+    cg->bcol = dalloc<int32_t>(cg->ents);
   } else if (layout == FFHIP_CG_LAYOUT_BSR3) {
     FFHIP_REQUIRE(false, FFHIP_ERR_UNSUPPORTED, "ffhip_cg_create: 3x3 blocks need a [P2,P2,P2] matrix");
   } else if (layout != FFHIP_CG_LAYOUT_CSR) {
@@ -876,7 +944,7 @@ static void cg_plan_layout(ffhip_cg cg, int layout, bool index32) {
   }
   int64_t g;
   if (cg->fmt == kBsr3) {
-    g = (A->n / 3 + kBlock / kLanesPerBlockRow - 1) / (kBlock / kLanesPerBlockRow);
+    g = ((A->n / 3 + kSlice - 1) / kSlice + kWavesPerBlock - 1) / kWavesPerBlock;
   } else if (cg->fmt != kCsrV) {
     g = (cg->S + kWavesPerBlock - 1) / kWavesPerBlock;
   } else {
@@ -906,13 +974,14 @@ static void cg_fill_values(ffhip_cg cg, bool transposed) {
   const bool mixed = cg->mode == FFHIP_CG_MIXED;
   double *&dv = transposed ? cg->svalT : cg->sval;
   float *&fv = transposed ? cg->sval32T : cg->sval32;
-  if (mixed && !fv) fv = dalloc<float>(cg->ents);
+  const int64_t nval = cg->fmt == kBsr3 ? 9 * cg->ents : cg->ents;
+  if (mixed && !fv) fv = dalloc<float>(nval);
   if (cg->fmt == kBsr3) {
-    if (!dv) dv = dalloc<double>(cg->ents);
-    k_bsr_fill<<<nblk(A->n / 3, 256), 256, 0, s>>>(A->n / 3, A->nodeptr, A->rowptr, A->colidx,
-                                                   A->vals, tmap, cg->dsc,
-                                                   transposed ? nullptr : cg->bcol, dv,
-                                                   mixed ? fv : nullptr);
+    if (!dv) dv = dalloc<double>(nval);
+    const int64_t nb = A->n / 3, S3 = (nb + kSlice - 1) / kSlice;
+    k_bsr_fill<<<nblk(S3 * kSlice, 256), 256, 0, s>>>(
+        nb, cg->ents, A->nodeptr, cg->bperm, cg->sptr, cg->slen, A->rowptr, A->colidx, A->vals, tmap,
+        cg->dsc, transposed ? nullptr : cg->bcol, dv, mixed ? fv : nullptr);
   } else if (cg->fmt != kCsrV) {
     if (!dv) dv = dalloc<double>(cg->ents);
     k_sell_fill<<<nblk(cg->S * kSlice, 256), 256, 0, s>>>(
@@ -1075,7 +1144,7 @@ int ffhip_cg_solve_mixed(ffhip_cg cg, const double *sval, const float *sval32, d
 static void cg_release(ffhip_cg cg) {
   for (hipEvent_t e : cg->ev) (void)hipEventDestroy(e);
   if (cg->fmt == kCsrV) cg->sval = nullptr;  // A's array
-  dfree(cg->sptr); dfree(cg->slen); dfree(cg->scol); dfree(cg->scol16); dfree(cg->sptr16); dfree(cg->bcol); dfree(cg->sval);
+  dfree(cg->sptr); dfree(cg->slen); dfree(cg->scol); dfree(cg->scol16); dfree(cg->sptr16); dfree(cg->bcol); dfree(cg->bperm); dfree(cg->sval);
   dfree(cg->svalT);
   dfree(cg->sval32); dfree(cg->sval32T); dfree(cg->dinv); dfree(cg->dsc); dfree(cg->wcl);
   dfree(cg->xw); dfree(cg->bw); dfree(cg->G); dfree(cg->CG); dfree(cg->H[0]); dfree(cg->H[1]);
@@ -1135,7 +1204,7 @@ int ffhip_cg_layout(ffhip_cg cg, int *layout, int64_t *entries) {
       *layout = cg->fmt == kCsrV    ? FFHIP_CG_LAYOUT_CSR
                 : cg->fmt == kBsr3 ? FFHIP_CG_LAYOUT_BSR3
                                    : (FFHIP_CG_LAYOUT_SELL | (cg->fmt == kSell16 ? FFHIP_CG_INDEX16 : 0));
-    if (entries) *entries = cg->ents;
+    if (entries) *entries = cg->fmt == kBsr3 ? 9 * cg->ents : cg->ents;
   });
 }
 

@@ -1,0 +1,104 @@
+"""Small fixed workloads of the library, for profiling runs (rocprofv3) and
+quick timing:
+
+    python3 freefem-sources_amd/ffhip/cases.py assembly_p1 160 3
+    python3 freefem-sources_amd/ffhip/cases.py assembly_p2 32 3
+    python3 freefem-sources_amd/ffhip/cases.py solve_p2 32
+
+assembly_p1: cube(n) P1 Poisson pattern + values + tgv rows + mixed solver
+setup, `reps` times (a fresh device mesh each time).
+assembly_p2: cube(n) [P2,P2,P2] numbering + pattern + Lame values, `reps` times.
+solve_p2: cube(n) Lame system, FP64 and MIXED solves in every solver layout,
+with per-kernel HIP-event timing.
+pcie_p1: the host-buffer path of the C-ABI on cube(n) P1: mesh upload from
+host arrays, device pattern and values, CSR download to host (wall times).
+"""
+import os
+import sys
+
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+
+LABELS = (1, 2, 3, 4, 5, 6)
+
+
+def assembly_p1(n=160, reps=3):
+    import ffhip
+    for _ in range(reps):
+        M = ffhip.cube(n)
+        A = ffhip.Matrix.p1(M)
+        tp = M.ctx.last_ms()
+        tv = A.assemble_p1(ffhip.COEF_SIN3)
+        tb = A.set_bc(LABELS)
+        cg = ffhip.CG(A, ffhip.CG_MIXED)
+        print(f"P1 cube({n}) nnz {A.nnz}: pattern {tp:.3f} ms, values {tv:.3f} ms, bc {tb:.3f} ms",
+              flush=True)
+        cg.free(); A.free(); M.free()
+
+
+def assembly_p2(n=32, reps=3):
+    import ffhip
+    for _ in range(reps):
+        M = ffhip.cube(n)
+        sp = ffhip.P2VSpace(M)
+        tn = M.ctx.last_ms()
+        A = ffhip.Matrix.p2v(sp)
+        tp = M.ctx.last_ms()
+        tv = A.assemble_p2v_elasticity(1.0, 0.5)
+        print(f"P2 cube({n}) nnz {A.nnz}: numbering {tn:.3f} ms, pattern {tp:.3f} ms, values {tv:.3f} ms",
+              flush=True)
+        A.free(); sp.free(); M.free()
+
+
+def solve_p2(n=32):
+    import torch
+    import ffhip
+    M = ffhip.cube(n)
+    sp = ffhip.P2VSpace(M)
+    A = ffhip.Matrix.p2v(sp)
+    A.assemble_p2v_elasticity(1.0, 0.5)
+    A.set_bc_p2v([4])
+    b = torch.empty(sp.ndof, dtype=torch.float64, device="cuda")
+    ffhip.rhs_p2v(sp, (0.0, 0.0, -1.0), [4], out=b)
+    for layout in ("bsr3", "csr"):
+        for mode in (ffhip.CG_FP64, ffhip.CG_MIXED):
+            cg = ffhip.CG(A, mode, layout=layout)
+            cg.set_profiling(True)
+            x, st = cg.solve(b, eps=1e-10)
+            print(f"{layout} {'FP64' if mode == ffhip.CG_FP64 else 'MIXED'}: {st['iters']} iterations, "
+                  f"{st['ms_solve']:.1f} ms, mat-vec {st['ms_matvec_avg'] * 1e3:.1f} us, "
+                  f"update {st['ms_update_avg'] * 1e3:.1f} us", flush=True)
+            cg.free()
+
+
+def pcie_p1(n=160, reps=2):
+    import time
+    import numpy as np
+    import ffhip
+    G = ffhip.cube(n)
+    m = G.download()
+    G.free()
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        M = ffhip.mesh3_upload(m["xyz"], m["elems"], m["belems"], m["blab"])
+        M.ctx.sync()
+        t1 = time.perf_counter()
+        A = ffhip.Matrix.p1(M)
+        A.assemble_p1(ffhip.COEF_SIN3)
+        A.set_bc(LABELS)
+        M.ctx.sync()
+        t2 = time.perf_counter()
+        rp, ci, va = A.download()
+        t3 = time.perf_counter()
+        mb_up = (m["xyz"].nbytes + m["elems"].nbytes + m["belems"].nbytes + m["blab"].nbytes) / 1e6
+        mb_dn = (rp.nbytes + ci.nbytes + va.nbytes) / 1e6
+        print(f"P1 cube({n}): mesh upload {mb_up:.0f} MB {1e3 * (t1 - t0):.1f} ms (host checks included), "
+              f"device assembly {1e3 * (t2 - t1):.1f} ms, CSR download {mb_dn:.0f} MB "
+              f"{1e3 * (t3 - t2):.1f} ms ({mb_dn / 1e3 / (t3 - t2):.1f} GB/s)", flush=True)
+        A.free(); M.free()
+
+
+if __name__ == "__main__":
+    case = sys.argv[1]
+    args = [int(a) for a in sys.argv[2:]]
+    {"assembly_p1": assembly_p1, "assembly_p2": assembly_p2, "solve_p2": solve_p2,
+     "pcie_p1": pcie_p1}[case](*args)

@@ -43,7 +43,7 @@ constexpr int kWavesPerBlock = kBlock / kSlice;
 // Tuning of the streaming mat-vec kernels: unroll depth of a row, the waves
 // per SIMD the register allocation targets, and resident blocks per CU.
 #ifndef FFHIP_SELL_U
-#define FFHIP_SELL_U 8
+#define FFHIP_SELL_U 16
 #endif
 #ifndef FFHIP_WPE
 #define FFHIP_WPE 4

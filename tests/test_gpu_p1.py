@@ -250,6 +250,40 @@ def test_full_size_properties(n):
     assert (sols[0] - sols[1]).norm().item() <= 1e-8 * sols[0].norm().item()
 
 
+def test_square200_configuration():
+    """square(200), the 2D Poisson configuration: FreeFEM's nnz (281,201),
+    zero row sums before Dirichlet, the CG solve against the oracle's, and
+    the discrete solution against the exact series value at the centre."""
+    import torch
+    n = 200
+    M = ffhip.square(n)
+    A = ffhip.Matrix.p1(M)
+    assert A.nnz == 281201
+    A.assemble_p1(ffhip.COEF_CONST, 1.0)
+    one = torch.ones(A.n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(one)
+    A.spmv(one, y)
+    rp, ci, va = A.download()
+    assert y.abs().max().item() <= 1e-12 * np.abs(va).max()
+    A.set_bc([1, 2, 3, 4])
+    b = ffhip.rhs_p1(M, 1.0, [1, 2, 3, 4])
+    x, st = ffhip.CG(A).solve(b, eps=1e-10)
+    assert st["converged"] == 1
+    s = ffo.square(n)
+    rpo, cio = ffo.pattern(s["nv"], s["tris"], 3)
+    vo = ffo.assemble_p1_2d(s, rpo, cio, ffo.COEF_CONST, 1.0)
+    onb = ffo.mark_bc(s["nv"], s["bedges"], s["blab"], 2, (1, 2, 3, 4))
+    ffo.set_bc(rpo, cio, vo, onb, 1e30)
+    bo = ffo.rhs_p1_2d(s, 1.0)
+    ffo.rhs_bc(bo, onb, 1e30, 0.0)
+    xo, so = ffo.pcg(rpo, cio, vo, bo, eps=1e-10)
+    assert abs(st["iters"] - so["iters"]) <= 2
+    assert np.linalg.norm(x - xo) <= 1e-8 * np.linalg.norm(xo)
+    # -Lap u = 1 on the unit square, u = 0 on the boundary: u(1/2,1/2) = 0.0736713532...
+    centre = (n // 2) * (n + 1) + n // 2
+    assert abs(x[centre] - 0.07367135328) < 2e-5
+
+
 def test_qp_coefficient_matches_analytic():
     """kappa handed over at the 14 Tet_5 points (the plugin's path for any
     interpreted coefficient) gives the SIN3 matrix to rounding."""

@@ -14,12 +14,12 @@
 //      the row (the element-to-row map).  Rows with more than 32 incident
 //      elements use a lane-per-row merge in LDS.
 //   3. values: one thread per element evaluates the 14-point Tet_5 integral
-//      of kappa and the P1 gradients (VALU FP64) and writes the 10 unique
-//      entries of its symmetric 4x4 matrix; then one wave per row loads the
-//      4 entries of each incident element (one lane per element) into LDS,
-//      and the lane owning each slot adds them in element order.  No
-//      atomics: every slot is summed by one lane in element order, the order
-//      in which HashMatrix::operator+= accumulates
+//      of kappa and the P1 gradients (VALU FP64) and writes the 4 rows of its
+//      symmetric 4x4 matrix where its incidence entries sit in the vertices'
+//      lists; then one half-wave per row loads them contiguously (one lane
+//      per element) into LDS, and the lane owning each slot adds them in
+//      element order.  No atomics: every slot is summed by one lane in
+//      element order, the order in which HashMatrix::operator+= accumulates
 //      (src/femlib/HashMatrix.cpp:1294-1332).  2D triangles and the rows with
 //      more than 32 elements use a lane-per-row walk with the row staged in
 //      LDS.
@@ -155,6 +155,12 @@ __global__ __launch_bounds__(256) void k_v2e_sort_bucket(int64_t n, const int64_
   int32_t x = hl < deg ? bucket[32 * v + hl] : INT_MAX;
   x = bitonic32(x, hl);
   if (hl < deg) v2e[k0 + hl] = x;
+}
+
+// position of every (element, local dof) item in the incidence lists
+__global__ void k_v2e_inverse(int64_t m, const int32_t *__restrict__ v2e, int32_t *__restrict__ inv) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < m) inv[v2e[k]] = (int32_t)k;
 }
 
 // per-dof insertion sort of the lists longer than 32 entries
@@ -323,26 +329,42 @@ __device__ inline double p1_entry(double S, const double G[4][3], int i, int j) 
   return a;
 }
 
-// the 10 entries (i <= j, row-major upper triangle) per element
+// the element's 4 rows, each written where its incidence entry sits in the
+// vertex's list (inv): Kinc[4k + m] = Ke(l, m) for the incidence k of (e, l),
+// so the row gather reads them contiguously; with Ke != null also the 10
+// unique entries per element (i <= j, row-major upper triangle) for the
+// lane-per-row gather
 __global__ __launch_bounds__(256) void k_p1_element_3d(
     int64_t nt, const double *__restrict__ xyz, const int32_t *__restrict__ tets,
-    int kind, double c0, const double *__restrict__ kappa_e, double *__restrict__ Ke) {
+    int kind, double c0, const double *__restrict__ kappa_e, const int32_t *__restrict__ inv,
+    double *__restrict__ Kinc, double *__restrict__ Ke) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nt) return;
   double v[4][3], G[4][3];
   load_tet(xyz, tets, e, v);
   const double mes = tet_geom(v, G);
   const double S = p1_kappa_integral(e, v, mes, kind, c0, kappa_e);
-  double out[10];
+  double K[10];
   int o = 0;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = i; j < 4; ++j) out[o++] = p1_entry(S, G, i, j);
-  double *dst = Ke + 10 * e;
+    for (int j = i; j < 4; ++j) K[o++] = p1_entry(S, G, i, j);
+  const int4 pos = reinterpret_cast<const int4 *>(inv)[e];
+  const int32_t kk[4] = {pos.x, pos.y, pos.z, pos.w};
 #pragma unroll
-  for (int i = 0; i < 10; ++i) dst[i] = out[i];
+  for (int l = 0; l < 4; ++l) {
+    double4 row;
+    row.x = K[sym4(l, 0)]; row.y = K[sym4(l, 1)]; row.z = K[sym4(l, 2)]; row.w = K[sym4(l, 3)];
+    reinterpret_cast<double4 *>(Kinc)[kk[l]] = row;
+  }
+  if (Ke) {
+    double *dst = Ke + 10 * e;
+#pragma unroll
+    for (int i = 0; i < 10; ++i) dst[i] = K[i];
+  }
 }
+
 
 // 2D P1: 6 unique entries; TTriangle area and H(i) (fem.hpp:290-325)
 __global__ __launch_bounds__(256) void k_p1_element_2d(
@@ -447,20 +469,21 @@ __global__ __launch_bounds__(BLOCK) void k_p1_gather(
   }
 }
 
-// 3D values, one row per wave: lane q (< 32) takes incidence q (element
-// order) and stages that element's 4 entries of the row and their 4 slot
-// bytes in LDS; lane j then walks the incidences in order and adds the entry
-// (if any) whose slot is j — the same sums, in the same order, as a
-// sequential walk over the row's elements.  Rows flagged big[] (more than 32
-// elements or 64 columns) are left to k_p1_gather.
-__global__ __launch_bounds__(256) void k_p1_values_wave(
+// 3D values, one row per 32-lane half-wave: lane q takes incidence q (element
+// order), whose 4 row entries the element kernel stored at Kinc[4k..4k+3],
+// and stages them and their 4 slot bytes in LDS; lane j then walks the
+// incidences in order and adds the entry (if any) whose slot is j (lanes own
+// slots j and j + 32) — the same sums, in the same order, as a sequential walk
+// over the row's elements.  Rows flagged big[] (more than 32 elements or 64
+// columns) are left to k_p1_gather.
+__global__ __launch_bounds__(256) void k_p1_values_half(
     int64_t n, const int64_t *__restrict__ rowptr, const int64_t *__restrict__ v2e_ptr,
-    const int32_t *__restrict__ v2e, const uint32_t *__restrict__ slotmap,
-    const double *__restrict__ Ke, const uint8_t *__restrict__ big, double *__restrict__ vals) {
-  __shared__ double sv[256 / kWave][4 * kHalf];
-  __shared__ uint32_t so[256 / kWave][kHalf];  // 4 slot bytes per incidence
-  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
-  const int64_t v = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+    const uint32_t *__restrict__ slotmap, const double *__restrict__ Kinc,
+    const uint8_t *__restrict__ big, double *__restrict__ vals) {
+  __shared__ double sv[256 / kHalf][4 * kHalf];
+  __shared__ uint32_t so[256 / kHalf][kHalf];  // 4 slot bytes per incidence
+  const int hl = threadIdx.x % kHalf, hid = threadIdx.x / kHalf;
+  const int64_t v = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kHalf;
   int64_t k0 = 0, r0 = 0;
   int deg = 0, len = 0;
   if (v < n && !big[v]) {
@@ -469,25 +492,29 @@ __global__ __launch_bounds__(256) void k_p1_values_wave(
     r0 = rowptr[v];
     len = (int)(rowptr[v + 1] - r0);
   }
-  if (lane < deg) {
-    const int32_t idx = v2e[k0 + lane];
-    const int64_t e = idx / 4;
-    const int l = idx - (int)e * 4;
-    const double *ke = Ke + 10 * e;
-#pragma unroll
-    for (int m = 0; m < 4; ++m) sv[w][4 * lane + m] = ke[sym4(l, m)];
-    so[w][lane] = slotmap[k0 + lane];
+  if (hl < deg) {
+    const double4 row = reinterpret_cast<const double4 *>(Kinc)[k0 + hl];
+    sv[hid][4 * hl + 0] = row.x;
+    sv[hid][4 * hl + 1] = row.y;
+    sv[hid][4 * hl + 2] = row.z;
+    sv[hid][4 * hl + 3] = row.w;
+    so[hid][hl] = slotmap[k0 + hl];
   }
   __syncthreads();
-  const uint32_t me = (uint32_t)lane * 0x01010101u;
-  double acc = 0.;
+  const uint32_t me0 = (uint32_t)hl * 0x01010101u, me1 = (uint32_t)(hl + kHalf) * 0x01010101u;
+  double acc0 = 0., acc1 = 0.;
   for (int q = 0; q < deg; ++q) {
-    // lowest byte of so[q] equal to this lane's slot (at most one is)
-    const uint32_t t = so[w][q] ^ me;
-    const uint32_t z = (t - 0x01010101u) & ~t & 0x80808080u;
-    if (z) acc += sv[w][4 * q + (__builtin_ctz(z) >> 3)];
+    // lowest byte of so[q] equal to the lane's slot (at most one is)
+    const uint32_t pk = so[hid][q];
+    const uint32_t t0 = pk ^ me0, z0 = (t0 - 0x01010101u) & ~t0 & 0x80808080u;
+    if (z0) acc0 += sv[hid][4 * q + (__builtin_ctz(z0) >> 3)];
+    if (len > kHalf) {
+      const uint32_t t1 = pk ^ me1, z1 = (t1 - 0x01010101u) & ~t1 & 0x80808080u;
+      if (z1) acc1 += sv[hid][4 * q + (__builtin_ctz(z1) >> 3)];
+    }
   }
-  if (lane < len) vals[r0 + lane] = acc;
+  if (hl < len) vals[r0 + hl] = acc0;
+  if (hl + kHalf < len) vals[r0 + hl + kHalf] = acc1;
 }
 
 // ---------------------------------------------------------------- Dirichlet
@@ -731,10 +758,15 @@ void build_incidence(ffhip_ctx ctx, const int32_t *elems, int64_t nt, int nloc, 
   *list_out = list;
 }
 
-// vertex -> incidence lists of a mesh (built once per mesh)
+// vertex -> incidence lists of a mesh (built once per mesh), and the position
+// of every (element, local vertex) item in them
 void build_v2e(ffhip_mesh m) {
   if (m->v2e_ptr) return;
   build_incidence(m->ctx, m->elems, m->nt, m->nloc, m->nv, &m->v2e_ptr, &m->v2e, &m->v2e_big);
+  const int64_t M = m->nt * m->nloc;
+  m->v2e_inv = dalloc<int32_t>(M);
+  k_v2e_inverse<<<nblocks(M, 256), 256, 0, m->ctx->stream>>>(M, m->v2e, m->v2e_inv);
+  FFHIP_CHECK(hipGetLastError());
 }
 
 // a_ii = tgv on the marked rows (HashMatrix::SetBC); *missing set when a
@@ -853,7 +885,8 @@ int ffhip_assemble_p1(ffhip_ctx ctx, ffhip_mesh m, ffhip_mat A, int kind, double
     FFHIP_REQUIRE(!arr || kappa_e, FFHIP_ERR_ARG, "ffhip_assemble_p1: kappa_e");
     hipStream_t s = ctx->stream;
     const int nu = m->nloc == 4 ? 10 : 6;
-    double *Ke = dalloc<double>(nu * m->nt);
+    // 3D: Ke only for the lane-per-row gather of long rows
+    double *Ke = (m->nloc == 4 && !A->any_big) ? nullptr : dalloc<double>(nu * m->nt);
     const double *kd = kappa_e;
     double *kbuf = nullptr;
     if (arr && !kappa_on_device) {
@@ -864,14 +897,17 @@ int ffhip_assemble_p1(ffhip_ctx ctx, ffhip_mesh m, ffhip_mat A, int kind, double
     }
     if (m->dim == 2) upload_t5_once();
     ctx->tic();
-    if (m->dim == 3)
-      k_p1_element_3d<<<nblocks(m->nt, 256), 256, 0, s>>>(m->nt, m->xyz, m->elems, kind, c0, kd, Ke);
-    else
+    double *Kinc = nullptr;
+    if (m->dim == 3) {
+      Kinc = dalloc<double>(4 * m->nt * 4);
+      k_p1_element_3d<<<nblocks(m->nt, 256), 256, 0, s>>>(m->nt, m->xyz, m->elems, kind, c0, kd,
+                                                           m->v2e_inv, Kinc, A->any_big ? Ke : nullptr);
+    } else
       k_p1_element_2d<<<nblocks(m->nt, 256), 256, 0, s>>>(m->nt, m->xyz, m->elems, kind, c0, kd, Ke);
     constexpr int B = 256, STAGE = 6144;
     if (m->nloc == 4) {
-      k_p1_values_wave<<<nblocks(A->n * kWave, 256), 256, 0, s>>>(
-          A->n, A->rowptr, m->v2e_ptr, m->v2e, A->slotmap, Ke, A->big, A->vals);
+      k_p1_values_half<<<nblocks(A->n * kHalf, 256), 256, 0, s>>>(
+          A->n, A->rowptr, m->v2e_ptr, A->slotmap, Kinc, A->big, A->vals);
       if (A->any_big)
         k_p1_gather<B, STAGE, 4><<<nblocks(A->n, B), B, 0, s>>>(
             A->n, A->rowptr, m->v2e_ptr, m->v2e, A->slotmap, Ke, A->vals, A->big);
@@ -882,6 +918,7 @@ int ffhip_assemble_p1(ffhip_ctx ctx, ffhip_mesh m, ffhip_mat A, int kind, double
     FFHIP_CHECK(hipGetLastError());
     ctx->toc();
     dfree(Ke);
+    dfree(Kinc);
     dfree(kbuf);
   });
 }

@@ -390,7 +390,7 @@ int ffhip_mesh_free(ffhip_mesh m) {
     if (!m) return;
     (void)hipStreamSynchronize(m->ctx->stream);
     dfree(m->xyz); dfree(m->elems); dfree(m->belems); dfree(m->blab);
-    dfree(m->v2e_ptr); dfree(m->v2e);
+    dfree(m->v2e_ptr); dfree(m->v2e); dfree(m->v2e_inv);
     delete m;
   });
 }

@@ -99,6 +99,7 @@ struct ffhip_mesh_s {
   int64_t *v2e_ptr = nullptr;  // nv + 1
   int32_t *v2e = nullptr;      // nt * nloc entries: e * nloc + l
   int v2e_big = 0;             // some vertex has more than 32 incident elements
+  int32_t *v2e_inv = nullptr;  // nt * nloc: position of item e * nloc + l in v2e
 };
 
 struct ffhip_mat_s {

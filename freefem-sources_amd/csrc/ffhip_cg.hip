@@ -189,25 +189,44 @@ __global__ void k_sell_len(int64_t n, int64_t S, const int64_t *__restrict__ row
 }
 
 // vals_src may be permuted through tmap (transposed values)
-__global__ void k_sell_fill(int64_t n, int64_t S, const int64_t *__restrict__ rowptr,
-                            const int32_t *__restrict__ colidx, const double *__restrict__ vals,
-                            const int64_t *__restrict__ tmap, const int64_t *__restrict__ sptr,
-                            const int32_t *__restrict__ len, const double *__restrict__ dsc,
-                            int32_t *__restrict__ scol, const int64_t *__restrict__ sptr16,
-                            int16_t *__restrict__ scol16, double *__restrict__ sval,
-                            float *__restrict__ sval32) {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// The block's 256 rows (4 slices) read their CSR segment once, coalesced, into
+// LDS (when it fits in STAGE entries), then every lane writes its row
+// transposed into the slice layout.
+template <int STAGE>
+__global__ __launch_bounds__(256) void k_sell_fill(
+    int64_t n, int64_t S, const int64_t *__restrict__ rowptr, const int32_t *__restrict__ colidx,
+    const double *__restrict__ vals, const int64_t *__restrict__ tmap,
+    const int64_t *__restrict__ sptr, const int32_t *__restrict__ len,
+    const double *__restrict__ dsc, int32_t *__restrict__ scol,
+    const int64_t *__restrict__ sptr16, int16_t *__restrict__ scol16, double *__restrict__ sval,
+    float *__restrict__ sval32) {
+  __shared__ int32_t lc[STAGE];
+  __shared__ double lv[STAGE];
+  const int t = threadIdx.x;
+  const int64_t v0 = (int64_t)blockIdx.x * blockDim.x;
+  const int64_t vend = v0 + blockDim.x < n ? v0 + blockDim.x : n;
+  const int64_t seg0 = v0 < n ? rowptr[v0] : 0, seg1 = v0 < n ? rowptr[vend] : 0;
+  const bool staged = seg1 - seg0 <= STAGE;
+  if (staged) {
+    for (int64_t i = t; i < seg1 - seg0; i += blockDim.x) {
+      lc[i] = colidx[seg0 + i];
+      lv[i] = tmap ? vals[tmap[seg0 + i]] : vals[seg0 + i];
+    }
+  }
+  __syncthreads();
+  const int64_t r = v0 + t;
   if (r >= S * kSlice) return;
   const int64_t s = r / kSlice;
   const int lane = (int)(r % kSlice);
   const int L = len[s];
   const int64_t base = sptr[s] + lane;
   const int64_t rp = r < n ? rowptr[r] : 0, rl = r < n ? rowptr[r + 1] - rp : 0;
+  const int32_t *cp = staged ? lc + (rp - seg0) : colidx + rp;
   for (int k = 0; k < L; ++k) {
     const int64_t idx = base + (int64_t)k * kSlice;
     if (k < rl) {
-      const int32_t c = colidx[rp + k];
-      const double a = tmap ? vals[tmap[rp + k]] : vals[rp + k];
+      const int32_t c = cp[k];
+      const double a = staged ? lv[rp - seg0 + k] : (tmap ? vals[tmap[rp + k]] : vals[rp + k]);
       if (scol) scol[idx] = c;
       if (sval) sval[idx] = a;
       if (sval32) sval32[idx] = (float)(a * dsc[r] * dsc[c]);
@@ -222,7 +241,7 @@ __global__ void k_sell_fill(int64_t n, int64_t S, const int64_t *__restrict__ ro
     const int L4 = (L + 3) & ~3;
     int16_t *dst = scol16 + sptr16[s] + 4 * lane;
     for (int k = 0; k < L4; ++k)
-      dst[(k >> 2) * (4 * kSlice) + (k & 3)] = k < rl ? (int16_t)(colidx[rp + k] - r) : (int16_t)0;
+      dst[(k >> 2) * (4 * kSlice) + (k & 3)] = k < rl ? (int16_t)(cp[k] - r) : (int16_t)0;
   }
 }
 
@@ -984,7 +1003,7 @@ static void cg_fill_values(ffhip_cg cg, bool transposed) {
         cg->dsc, transposed ? nullptr : cg->bcol, dv, mixed ? fv : nullptr);
   } else if (cg->fmt != kCsrV) {
     if (!dv) dv = dalloc<double>(cg->ents);
-    k_sell_fill<<<nblk(cg->S * kSlice, 256), 256, 0, s>>>(
+    k_sell_fill<4096><<<nblk(cg->S * kSlice, 256), 256, 0, s>>>(
         A->n, cg->S, A->rowptr, A->colidx, A->vals, tmap, cg->sptr, cg->slen, cg->dsc,
         transposed ? nullptr : cg->scol, cg->sptr16, transposed ? nullptr : cg->scol16, dv,
         mixed ? fv : nullptr);

@@ -194,7 +194,8 @@ __global__ __launch_bounds__(256) void k_v2e_bucket_wave(const int32_t *__restri
 // at their CSR offsets, two dofs per wave
 __global__ __launch_bounds__(256) void k_v2e_sort_bucket(int64_t n, const int64_t *__restrict__ ptr,
                                                          const int32_t *__restrict__ bucket,
-                                                         int32_t *__restrict__ v2e) {
+                                                         int32_t *__restrict__ v2e,
+                                                         int32_t *__restrict__ inv) {
   const int hl = threadIdx.x % kHalf;
   const int64_t v = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kHalf;
   int64_t k0 = 0;
@@ -202,7 +203,10 @@ __global__ __launch_bounds__(256) void k_v2e_sort_bucket(int64_t n, const int64_
   if (v < n) { k0 = ptr[v]; deg = (int)(ptr[v + 1] - k0); }
   int32_t x = hl < deg ? bucket[32 * v + hl] : INT_MAX;
   x = bitonic32(x, hl);
-  if (hl < deg) v2e[k0 + hl] = x;
+  if (hl < deg) {
+    v2e[k0 + hl] = x;
+    if (inv) inv[x] = (int32_t)(k0 + hl);
+  }
 }
 
 // position of every (element, local dof) item in the incidence lists
@@ -769,7 +773,7 @@ static void upload_t5_once() {
 // dof -> (element * nloc + local) incidence lists in element order, for any
 // element-to-dof table (P1 vertices, P2 nodes)
 void build_incidence(ffhip_ctx ctx, const int32_t *elems, int64_t nt, int nloc, int64_t n,
-                     int64_t **ptr_out, int32_t **list_out, int *big_out) {
+                     int64_t **ptr_out, int32_t **list_out, int *big_out, int32_t **inv_out) {
   hipStream_t s = ctx->stream;
   const int64_t M = nt * nloc;
   uint32_t *deg = dalloc<uint32_t>(n);
@@ -778,6 +782,7 @@ void build_incidence(ffhip_ctx ctx, const int32_t *elems, int64_t nt, int nloc, 
   int *flag = dalloc<int>(1);
   int64_t *ptr = dalloc<int64_t>(n + 1);
   int32_t *list = dalloc<int32_t>(M);
+  int32_t *inv = inv_out ? dalloc<int32_t>(M) : nullptr;
   FFHIP_CHECK(hipMemsetAsync(deg, 0, 4 * n, s));
   FFHIP_CHECK(hipMemsetAsync(deg64 + n, 0, 8, s));
   FFHIP_CHECK(hipMemsetAsync(flag, 0, sizeof(int), s));
@@ -787,7 +792,7 @@ void build_incidence(ffhip_ctx ctx, const int32_t *elems, int64_t nt, int nloc, 
   FFHIP_CHECK(hipGetLastError());
   const int has_big = read_flag(ctx, flag);
   if (!has_big) {
-    k_v2e_sort_bucket<<<nblocks(n * kHalf, 256), 256, 0, s>>>(n, ptr, bucket, list);
+    k_v2e_sort_bucket<<<nblocks(n * kHalf, 256), 256, 0, s>>>(n, ptr, bucket, list, inv);
   } else {  // some list exceeds its bucket: fill by atomics, then sort
     int32_t *cur = dalloc<int32_t>(n);
     FFHIP_CHECK(hipMemsetAsync(cur, 0, 4 * n, s));
@@ -795,6 +800,7 @@ void build_incidence(ffhip_ctx ctx, const int32_t *elems, int64_t nt, int nloc, 
     k_v2e_fill<<<nblocks(M, 256), 256, 0, s>>>(elems, M, ptr, cur, list);
     k_v2e_sort_half<<<nblocks(n * kHalf, 256), 256, 0, s>>>(n, ptr, list, flag);
     k_v2e_sort<<<nblocks(n, 256), 256, 0, s>>>(n, ptr, list);
+    if (inv) k_v2e_inverse<<<nblocks(M, 256), 256, 0, s>>>(M, list, inv);
     FFHIP_CHECK(hipStreamSynchronize(s));
     dfree(cur);
   }
@@ -802,6 +808,7 @@ void build_incidence(ffhip_ctx ctx, const int32_t *elems, int64_t nt, int nloc, 
   FFHIP_CHECK(hipStreamSynchronize(s));
   dfree(deg); dfree(deg64); dfree(bucket); dfree(flag);
   if (big_out) *big_out = has_big;
+  if (inv_out) *inv_out = inv;
   *ptr_out = ptr;
   *list_out = list;
 }
@@ -810,11 +817,8 @@ void build_incidence(ffhip_ctx ctx, const int32_t *elems, int64_t nt, int nloc, 
 // of every (element, local vertex) item in them
 void build_v2e(ffhip_mesh m) {
   if (m->v2e_ptr) return;
-  build_incidence(m->ctx, m->elems, m->nt, m->nloc, m->nv, &m->v2e_ptr, &m->v2e, &m->v2e_big);
-  const int64_t M = m->nt * m->nloc;
-  m->v2e_inv = dalloc<int32_t>(M);
-  k_v2e_inverse<<<nblocks(M, 256), 256, 0, m->ctx->stream>>>(M, m->v2e, m->v2e_inv);
-  FFHIP_CHECK(hipGetLastError());
+  build_incidence(m->ctx, m->elems, m->nt, m->nloc, m->nv, &m->v2e_ptr, &m->v2e, &m->v2e_big,
+                  &m->v2e_inv);
 }
 
 // a_ii = tgv on the marked rows (HashMatrix::SetBC); *missing set when a

@@ -80,12 +80,13 @@ __device__ inline double tet_mesure(const double v[4][3]) {
   return det3_pivot(AB, AC, AD) / 6.;
 }
 
-// kappa as the script evaluates 1+0.5*sin(2*pi*x)*sin(2*pi*y)*sin(2*pi*z)
+// kappa = 1+0.5*sin(2*pi*x)*sin(2*pi*y)*sin(2*pi*z), the script's product
+// order; sin(2 pi x) as sinpi(2 x) (no range reduction of a rounded 2 pi x:
+// within an ulp of the script's sin(2*pi*x))
 __device__ inline double kappa_sin3(double x, double y, double z) {
-  const double tp = 2 * M_PI;
-  double s = 0.5 * sin(tp * x);
-  s = s * sin(tp * y);
-  s = s * sin(tp * z);
+  double s = 0.5 * sinpi(2 * x);
+  s = s * sinpi(2 * y);
+  s = s * sinpi(2 * z);
   return 1 + s;
 }
 

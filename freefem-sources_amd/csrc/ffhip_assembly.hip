@@ -7,11 +7,12 @@
 //      up to 32 entries is sorted by a bitonic network in a 32-lane half-wave
 //      (longer lists: per-dof insertion sort).
 //   2. pattern: one half-wave per row loads the vertices of the row's
-//      incident elements (4 per lane, 128 per row), sorts them with a bitonic
-//      network and keeps the heads of equal runs — the sorted unique columns.
-//      A count pass, a scan, a fill pass; the fill pass also records, for
-//      every incidence entry, the offsets of that element's vertices inside
-//      the row (the element-to-row map).  Rows with more than 32 incident
+//      incident elements (4 per lane), deduplicates them in an LDS hash set
+//      and orders the unique columns by a counting sort.  One pass writes
+//      each row's sorted columns to a row buffer, its length and, for every
+//      incidence entry, the offsets of that element's vertices inside the row
+//      (the element-to-row map); after a scan of the lengths the buffers are
+//      compacted into the CSR arrays.  Rows with more than 32 incident
 //      elements use a lane-per-row merge in LDS.
 //   3. values: one thread per element evaluates the 14-point Tet_5 integral
 //      of kappa and the P1 gradients (VALU FP64) and writes the 4 rows of its
@@ -231,15 +232,18 @@ __global__ void k_v2e_sort(int64_t n, const int64_t *__restrict__ ptr, int32_t *
 
 // ---------------------------------------------------------------- pattern (half-wave rows)
 // One row per 32-lane half-wave: lane q loads incidence q's element vertices
-// (4 candidates per lane, 128 per row), a bitonic network sorts them, and the
-// row's sorted unique columns are the heads of equal runs.  One pass writes
-// the row length, the unique columns to a 64-entry row buffer, the relative
-// diagonal slot and the element-to-row map (each candidate's offset found by
-// binary search in the row's columns staged in LDS); after the scan,
+// (4 candidates per lane), inserts them into an open-addressing set in LDS
+// (atomicCAS; the lane that claims a slot appends the column), and the
+// unique columns are ordered by a counting sort (rank = number of smaller
+// columns).  One pass writes the row length, the sorted columns to a 64-entry
+// row buffer, the relative diagonal slot and the element-to-row map (each
+// candidate's offset by binary search in the sorted columns); after the scan,
 // k_pattern_compact moves the buffered columns to their CSR place.  Rows with
 // more than 32 incident elements or 64 columns are flagged in big[] for the
 // lane-per-row merge (k_pattern_rows).
 constexpr int kRowBuf = 64;
+
+constexpr int kHash = 128;  // LDS hash slots per row (a row has at most 1 + 3 * 32 columns)
 
 template <int NLOC>
 __global__ __launch_bounds__(256) void k_pattern_half(
@@ -247,7 +251,10 @@ __global__ __launch_bounds__(256) void k_pattern_half(
     const int32_t *__restrict__ v2e, int64_t *__restrict__ rowlen, int32_t *__restrict__ rowbuf,
     int8_t *__restrict__ dposr, uint32_t *__restrict__ slotmap, uint8_t *__restrict__ big,
     int *__restrict__ any_big) {
-  __shared__ int32_t ucol[256 / kHalf][kRowBuf];
+  constexpr int NH = 256 / kHalf;
+  __shared__ int32_t hk[NH][kHash];      // open-addressing set of the row's columns
+  __shared__ int32_t ucol[NH][kRowBuf];  // unique columns, then sorted
+  __shared__ int32_t ucnt[NH];
   const int hl = threadIdx.x % kHalf, hid = threadIdx.x / kHalf;
   const int64_t v = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kHalf;
   int64_t k0 = 0;
@@ -255,6 +262,8 @@ __global__ __launch_bounds__(256) void k_pattern_half(
   if (v < n) { k0 = v2e_ptr[v]; deg = (int)(v2e_ptr[v + 1] - k0); }
   bool mine = v < n && deg <= kHalf;
   if (!mine) deg = 0;
+  for (int i = hl; i < kHash; i += kHalf) hk[hid][i] = INT_MAX;
+  if (hl == 0) ucnt[hid] = 0;
   int32_t c[4] = {INT_MAX, INT_MAX, INT_MAX, INT_MAX};
   if (hl < deg) {
     const int64_t e = v2e[k0 + hl] / NLOC;
@@ -266,33 +275,51 @@ __global__ __launch_bounds__(256) void k_pattern_half(
       for (int m = 0; m < NLOC; ++m) c[m] = elems[e * NLOC + m];
     }
   }
-  int32_t orig[4] = {c[0], c[1], c[2], c[3]};
-  bitonic128(c, hl);
-  const int32_t prev0 = __shfl_up(c[3], 1, kWave);
-  int flag[4], cnt = 0;
+  __syncthreads();
+  // insert the candidates; the lane that claims a free slot appends the column
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const int32_t prev = r ? c[r - 1] : (hl ? prev0 : INT_MIN);
-    flag[r] = (c[r] != INT_MAX && c[r] != prev) ? 1 : 0;
-    cnt += flag[r];
-  }
-  int incl = cnt;  // inclusive prefix of run heads over the half
-#pragma unroll
-  for (int off = 1; off < kHalf; off <<= 1) {
-    const int t = __shfl_up(incl, off, kWave);
-    if (hl >= off) incl += t;
-  }
-  const int len = __shfl(incl, (threadIdx.x % kWave) / kHalf * kHalf + kHalf - 1, kWave);
-  if (len > kRowBuf) mine = false;
-  int run = incl - cnt, dpos = -1;
-#pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    run += flag[r];
-    if (flag[r] && mine) {
-      ucol[hid][run - 1] = c[r];
-      rowbuf[kRowBuf * v + run - 1] = c[r];
-      if ((int64_t)c[r] == v) dpos = run - 1;
+    if (c[r] == INT_MAX) continue;
+    uint32_t h = ((uint32_t)c[r] * 2654435761u) >> (32 - 7);
+    for (;;) {  // ends: the set holds at most 97 columns of its 128 slots
+      const int32_t old = atomicCAS(&hk[hid][h], INT_MAX, c[r]);
+      if (old == INT_MAX) {
+        const int idx = atomicAdd(&ucnt[hid], 1);
+        if (idx < kRowBuf) ucol[hid][idx] = c[r];
+        break;
+      }
+      if (old == c[r]) break;
+      h = (h + 1) & (kHash - 1);
     }
+  }
+  __syncthreads();
+  const int len = ucnt[hid];
+  if (len > kRowBuf) mine = false;
+  // counting sort: the rank of a column is the number of smaller columns
+  int32_t u[2] = {INT_MAX, INT_MAX};
+  int rk[2] = {0, 0};
+  if (mine) {
+#pragma unroll
+    for (int h2 = 0; h2 < 2; ++h2) {
+      const int j = hl + h2 * kHalf;
+      if (j < len) {
+        u[h2] = ucol[hid][j];
+        int r = 0;
+        for (int i = 0; i < len; ++i) r += ucol[hid][i] < u[h2] ? 1 : 0;
+        rk[h2] = r;
+      }
+    }
+  }
+  __syncthreads();
+  int dpos = -1;
+  if (mine) {
+#pragma unroll
+    for (int h2 = 0; h2 < 2; ++h2)
+      if (hl + h2 * kHalf < len) {
+        ucol[hid][rk[h2]] = u[h2];
+        rowbuf[kRowBuf * v + rk[h2]] = u[h2];
+        if ((int64_t)u[h2] == v) dpos = rk[h2];
+      }
   }
 #pragma unroll
   for (int off = 1; off < kHalf; off <<= 1) {
@@ -312,7 +339,7 @@ __global__ __launch_bounds__(256) void k_pattern_half(
       int lo = 0, hi = len - 1;  // the column is in the row: find its offset
       while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        if (ucol[hid][mid] < orig[m]) lo = mid + 1; else hi = mid;
+        if (ucol[hid][mid] < c[m]) lo = mid + 1; else hi = mid;
       }
       pk |= (uint32_t)lo << (8 * m);
     }
@@ -776,10 +803,12 @@ void build_incidence(ffhip_ctx ctx, const int32_t *elems, int64_t nt, int nloc, 
                      int64_t **ptr_out, int32_t **list_out, int *big_out, int32_t **inv_out) {
   hipStream_t s = ctx->stream;
   const int64_t M = nt * nloc;
-  uint32_t *deg = dalloc<uint32_t>(n);
-  int64_t *deg64 = dalloc<int64_t>(n + 1);
-  int32_t *bucket = dalloc<int32_t>(32 * n);
-  int *flag = dalloc<int>(1);
+  Arena ar(ctx, Arena::bytes(n, 4) + Arena::bytes(n + 1, 8) + Arena::bytes(32 * n, 4) +
+                    Arena::bytes(1, 4) + Arena::bytes(n, 4));
+  uint32_t *deg = ar.take<uint32_t>(n);
+  int64_t *deg64 = ar.take<int64_t>(n + 1);
+  int32_t *bucket = ar.take<int32_t>(32 * n);
+  int *flag = ar.take<int>(1);
   int64_t *ptr = dalloc<int64_t>(n + 1);
   int32_t *list = dalloc<int32_t>(M);
   int32_t *inv = inv_out ? dalloc<int32_t>(M) : nullptr;
@@ -794,19 +823,16 @@ void build_incidence(ffhip_ctx ctx, const int32_t *elems, int64_t nt, int nloc, 
   if (!has_big) {
     k_v2e_sort_bucket<<<nblocks(n * kHalf, 256), 256, 0, s>>>(n, ptr, bucket, list, inv);
   } else {  // some list exceeds its bucket: fill by atomics, then sort
-    int32_t *cur = dalloc<int32_t>(n);
+    int32_t *cur = ar.take<int32_t>(n);
     FFHIP_CHECK(hipMemsetAsync(cur, 0, 4 * n, s));
     FFHIP_CHECK(hipMemsetAsync(flag, 0, sizeof(int), s));
     k_v2e_fill<<<nblocks(M, 256), 256, 0, s>>>(elems, M, ptr, cur, list);
     k_v2e_sort_half<<<nblocks(n * kHalf, 256), 256, 0, s>>>(n, ptr, list, flag);
     k_v2e_sort<<<nblocks(n, 256), 256, 0, s>>>(n, ptr, list);
     if (inv) k_v2e_inverse<<<nblocks(M, 256), 256, 0, s>>>(M, list, inv);
-    FFHIP_CHECK(hipStreamSynchronize(s));
-    dfree(cur);
   }
   FFHIP_CHECK(hipGetLastError());
   FFHIP_CHECK(hipStreamSynchronize(s));
-  dfree(deg); dfree(deg64); dfree(bucket); dfree(flag);
   if (big_out) *big_out = has_big;
   if (inv_out) *inv_out = inv;
   *ptr_out = ptr;
@@ -864,8 +890,10 @@ int ffhip_pattern_p1(ffhip_ctx ctx, ffhip_mesh m, ffhip_mat *out) {
     const int64_t n = m->nv;
     ffhip_mat A = new ffhip_mat_s();
     A->ctx = ctx; A->n = n; A->mesh = m;
-    int64_t *len = dalloc<int64_t>(n + 1);
-    int *ovf = dalloc<int>(1);
+    Arena ar(ctx, Arena::bytes(n + 1, 8) + Arena::bytes(1, 4) + Arena::bytes(kRowBuf * n, 4) +
+                      Arena::bytes(n, 1) + Arena::bytes(1, 4));
+    int64_t *len = ar.take<int64_t>(n + 1);
+    int *ovf = ar.take<int>(1);
     A->rowptr = dalloc<int64_t>(n + 1);
     A->diag = dalloc<int64_t>(n);
     A->slotmap = dalloc<uint32_t>(m->nt * m->nloc);
@@ -874,9 +902,9 @@ int ffhip_pattern_p1(ffhip_ctx ctx, ffhip_mesh m, ffhip_mat *out) {
     constexpr int B = 128, CAP = 64;
     // rows with up to 32 incident elements and 64 columns: one half-wave
     // pass into row buffers; the rest: the lane-per-row merge
-    int32_t *rowbuf = dalloc<int32_t>(kRowBuf * n);
-    int8_t *dposr = dalloc<int8_t>(n);
-    int *anyb = dalloc<int>(1);
+    int32_t *rowbuf = ar.take<int32_t>(kRowBuf * n);
+    int8_t *dposr = ar.take<int8_t>(n);
+    int *anyb = ar.take<int>(1);
     A->big = dalloc<uint8_t>(n);
     FFHIP_CHECK(hipMemsetAsync(anyb, 0, sizeof(int), s));
     if (m->nloc == 4)
@@ -905,14 +933,9 @@ int ffhip_pattern_p1(ffhip_ctx ctx, ffhip_mesh m, ffhip_mat *out) {
           n, m->nloc, m->elems, m->v2e_ptr, m->v2e, nullptr, A->rowptr, A->colidx, A->diag,
           A->slotmap, nullptr, ovf, A->big);
     FFHIP_CHECK(hipGetLastError());
-    dfree(rowbuf);
-    dfree(dposr);
-    dfree(anyb);
     FFHIP_CHECK(hipMemsetAsync(A->vals, 0, 8 * A->nnz, s));
     ctx->toc();
     A->max_row = CAP;
-    dfree(len);
-    dfree(ovf);
     *out = A;
   });
 }
@@ -937,8 +960,12 @@ int ffhip_assemble_p1(ffhip_ctx ctx, ffhip_mesh m, ffhip_mat A, int kind, double
     FFHIP_REQUIRE(!arr || kappa_e, FFHIP_ERR_ARG, "ffhip_assemble_p1: kappa_e");
     hipStream_t s = ctx->stream;
     const int nu = m->nloc == 4 ? 10 : 6;
-    // 3D: Ke only for the lane-per-row gather of long rows
-    double *Ke = (m->nloc == 4 && !A->any_big) ? nullptr : dalloc<double>(nu * m->nt);
+    // 3D: the element rows in incidence order, and Ke only for the lane-per-row
+    // gather of long rows
+    const bool needKe = m->nloc != 4 || A->any_big;
+    Arena ar(ctx, (m->dim == 3 ? Arena::bytes(16 * m->nt, 8) : 0) +
+                      (needKe ? Arena::bytes(nu * m->nt, 8) : 0));
+    double *Ke = needKe ? ar.take<double>(nu * m->nt) : nullptr;
     const double *kd = kappa_e;
     double *kbuf = nullptr;
     if (arr && !kappa_on_device) {
@@ -951,7 +978,7 @@ int ffhip_assemble_p1(ffhip_ctx ctx, ffhip_mesh m, ffhip_mat A, int kind, double
     ctx->tic();
     double *Kinc = nullptr;
     if (m->dim == 3) {
-      Kinc = dalloc<double>(4 * m->nt * 4);
+      Kinc = ar.take<double>(16 * m->nt);
       k_p1_element_3d<<<nblocks(m->nt, 256), 256, 0, s>>>(m->nt, m->xyz, m->elems, kind, c0, kd,
                                                            m->v2e_inv, Kinc, A->any_big ? Ke : nullptr);
     } else
@@ -969,8 +996,6 @@ int ffhip_assemble_p1(ffhip_ctx ctx, ffhip_mesh m, ffhip_mat A, int kind, double
                                                                A->slotmap, Ke, A->vals);
     FFHIP_CHECK(hipGetLastError());
     ctx->toc();
-    dfree(Ke);
-    dfree(Kinc);
     dfree(kbuf);
   });
 }

@@ -920,24 +920,23 @@ static void cg_plan_layout(ffhip_cg cg, int layout, bool index32) {
   } else if (layout == FFHIP_CG_LAYOUT_BSR3) {
     FFHIP_REQUIRE(false, FFHIP_ERR_UNSUPPORTED, "ffhip_cg_create: 3x3 blocks need a [P2,P2,P2] matrix");
   } else if (layout != FFHIP_CG_LAYOUT_CSR) {
+    Arena ar(ctx, 2 * Arena::bytes(cg->S + 1, 8) + Arena::bytes(1, 4));
     cg->sptr = dalloc<int64_t>(cg->S + 1);
     cg->slen = dalloc<int32_t>(cg->S);
-    int64_t *cnt = dalloc<int64_t>(cg->S + 1);
+    int64_t *cnt = ar.take<int64_t>(cg->S + 1);
     FFHIP_CHECK(hipMemsetAsync(cnt + cg->S, 0, 8, s));
     k_sell_len<<<nblk(cg->S, 256), 256, 0, s>>>(A->n, cg->S, A->rowptr, cg->slen, cnt);
     exclusive_scan_i64(ctx, cnt, cg->sptr, cg->S + 1);
     cg->ents = read_i64(ctx, cg->sptr + cg->S);
-    dfree(cnt);
     if (layout == FFHIP_CG_LAYOUT_SELL || 4 * cg->ents <= 5 * A->nnz) {
       // 16-bit column deltas when every column is within 2^15 of its row
       int far = 1;
       if (!index32) {
-        int *d = dalloc<int>(1);
+        int *d = ar.take<int>(1);
         FFHIP_CHECK(hipMemsetAsync(d, 0, sizeof(int), s));
         k_far_columns<<<nblk(A->n, 256), 256, 0, s>>>(A->n, A->rowptr, A->colidx, d);
         FFHIP_CHECK(hipGetLastError());
         far = read_flag(ctx, d);
-        dfree(d);
       }
       if (far) {
         cg->fmt = kSell;
@@ -945,12 +944,11 @@ static void cg_plan_layout(ffhip_cg cg, int layout, bool index32) {
       } else {
         cg->fmt = kSell16;
         cg->sptr16 = dalloc<int64_t>(cg->S + 1);
-        int64_t *cnt4 = dalloc<int64_t>(cg->S + 1);
+        int64_t *cnt4 = ar.take<int64_t>(cg->S + 1);
         FFHIP_CHECK(hipMemsetAsync(cnt4 + cg->S, 0, 8, s));
         k_sell_len4<<<nblk(cg->S, 256), 256, 0, s>>>(cg->S, cg->slen, cnt4);
         exclusive_scan_i64(ctx, cnt4, cg->sptr16, cg->S + 1);
         const int64_t ents16 = read_i64(ctx, cg->sptr16 + cg->S);
-        dfree(cnt4);
         cg->scol16 = dalloc<int16_t>(ents16);
       }
     } else {
@@ -1029,11 +1027,12 @@ static void cg_setup_jacobi(ffhip_cg cg) {
   ffhip_mat A = cg->A;
   hipStream_t s = ctx->stream;
   const int64_t n = A->n;
-  double *d = dalloc<double>(n);
-  uint8_t *has = dalloc<uint8_t>(n);
-  k_diag_vals<<<nblk(n, 256), 256, 0, s>>>(n, A->diag, A->vals, d, has);
   const int GB = 512;
-  double *part = dalloc<double>(2 * GB);
+  Arena ar(ctx, Arena::bytes(n, 8) + Arena::bytes(n, 1) + Arena::bytes(2 * GB, 8));
+  double *d = ar.take<double>(n);
+  uint8_t *has = ar.take<uint8_t>(n);
+  k_diag_vals<<<nblk(n, 256), 256, 0, s>>>(n, A->diag, A->vals, d, has);
+  double *part = ar.take<double>(2 * GB);
   std::vector<double> hp(2 * GB);
   k_tgv_pass<<<GB, kBlock, 0, s>>>(n, d, has, 0., 1, part);
   FFHIP_CHECK(hipMemcpyAsync(hp.data(), part, 16 * GB, hipMemcpyDeviceToHost, s));
@@ -1055,7 +1054,7 @@ static void cg_setup_jacobi(ffhip_cg cg) {
   k_jacobi<<<nblk(n, 256), 256, 0, s>>>(n, d, has, tgve, ntgv > 0, cg->dinv, cg->dsc, cg->wcl);
   FFHIP_CHECK(hipGetLastError());
   FFHIP_CHECK(hipStreamSynchronize(s));
-  dfree(d); dfree(has); dfree(part);
+
 }
 
 // r = b - A x (rout may be null); the three partial sums of k_residual added on

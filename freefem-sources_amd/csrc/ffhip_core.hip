@@ -180,6 +180,17 @@ void *ffhip_ctx_s::ensure_scratch(size_t bytes) {
   return scratch;
 }
 
+void *ffhip_ctx_s::ensure_work(size_t bytes) {
+  if (bytes > work_bytes) {
+    if (work) (void)hipFree(work);
+    work = nullptr;
+    work_bytes = 0;
+    FFHIP_CHECK(hipMalloc(&work, bytes));
+    work_bytes = bytes;
+  }
+  return work;
+}
+
 void ffhip_ctx_s::tic() { FFHIP_CHECK(hipEventRecord(ev0, stream)); }
 void ffhip_ctx_s::toc() {
   FFHIP_CHECK(hipEventRecord(ev1, stream));
@@ -221,6 +232,7 @@ int ffhip_ctx_free(ffhip_ctx c) {
     if (!c) return;
     (void)hipStreamSynchronize(c->stream);
     if (c->scratch) (void)hipFree(c->scratch);
+    if (c->work) (void)hipFree(c->work);
     (void)hipEventDestroy(c->ev0);
     (void)hipEventDestroy(c->ev1);
     (void)hipStreamDestroy(c->stream);

@@ -80,6 +80,11 @@ struct ffhip_ctx_s {
   void *scratch = nullptr;
   size_t scratch_bytes = 0;
   void *ensure_scratch(size_t bytes);
+  // work buffer the assembly and solver-setup temporaries are carved from
+  // (grows to the largest request and is kept: no allocation on repeat calls)
+  void *work = nullptr;
+  size_t work_bytes = 0;
+  void *ensure_work(size_t bytes);
   double last_ms = 0.;  // device time of the last timed call
   void tic();           // record ev0 on the stream
   void toc();           // record ev1, synchronise, set last_ms
@@ -178,6 +183,23 @@ __device__ inline int xcd_block_id() {
 
 void exclusive_scan_i64(ffhip_ctx ctx, const int64_t *in, int64_t *out,
                         int64_t n);
+
+// 256-byte aligned temporaries carved from the context's work buffer; one
+// arena at a time per context (the calls that use one are sequential)
+struct Arena {
+  static size_t bytes(size_t count, size_t size) { return (count * size + 255) & ~size_t(255); }
+  char *base;
+  size_t cap, off = 0;
+  Arena(ffhip_ctx ctx, size_t total) : base((char *)ctx->ensure_work(total)), cap(total) {}
+  template <class T>
+  T *take(size_t count) {
+    const size_t b = bytes(count ? count : 1, sizeof(T));
+    FFHIP_REQUIRE(off + b <= cap, FFHIP_ERR_INTERNAL, "work arena overflow");
+    T *p = (T *)(base + off);
+    off += b;
+    return p;
+  }
+};
 int64_t read_i64(ffhip_ctx ctx, const int64_t *p);
 
 }  // namespace ffhip

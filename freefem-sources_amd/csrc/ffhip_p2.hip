@@ -443,37 +443,35 @@ int ffhip_p2v_create(ffhip_ctx ctx, ffhip_mesh m, ffhip_p2v *out) {
     const int64_t nt = m->nt, nv = m->nv, E = 6 * nt, NI = 10 * nt;
     ffhip_p2v sp = new ffhip_p2v_s();
     sp->ctx = ctx; sp->mesh = m;
-    int32_t *firstv = nullptr, *items = nullptr, *sitems = nullptr, *item_seg = nullptr,
-            *segfirst = nullptr;
-    uint64_t *keys = nullptr, *skeys = nullptr, *ukeys = nullptr;
-    int64_t *head = nullptr, *hscan = nullptr, *mark = nullptr, *rank = nullptr;
-    void *tmp = nullptr;
     try {
+      size_t tb = 0;
+      FFHIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, (uint64_t *)nullptr, (uint64_t *)nullptr,
+                                                     (int32_t *)nullptr, (int32_t *)nullptr, (int)E, 0,
+                                                     64, s));
+      Arena ar(ctx, Arena::bytes(nv, 4) + 2 * Arena::bytes(E, 8) + 4 * Arena::bytes(E, 4) +
+                        Arena::bytes(tb, 1) + 2 * Arena::bytes(E + 1, 8) + 2 * Arena::bytes(NI + 1, 8));
       ctx->tic();
-      firstv = dalloc<int32_t>(nv);
+      int32_t *firstv = ar.take<int32_t>(nv);
       k_fill_i32<<<nblk(nv, 256), 256, 0, s>>>(nv, INT_MAX, firstv);
       k_p2_vfirst<<<nblk(4 * nt, 256), 256, 0, s>>>(nt, m->elems, firstv);
-      keys = dalloc<uint64_t>(E); skeys = dalloc<uint64_t>(E);
-      items = dalloc<int32_t>(E); sitems = dalloc<int32_t>(E);
+      uint64_t *keys = ar.take<uint64_t>(E), *skeys = ar.take<uint64_t>(E);
+      int32_t *items = ar.take<int32_t>(E), *sitems = ar.take<int32_t>(E);
       k_p2_edge_keys<<<nblk(E, 256), 256, 0, s>>>(nt, m->elems, keys, items);
-      size_t tb = 0;
-      FFHIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, keys, skeys, items, sitems, (int)E, 0,
-                                                     64, s));
-      tmp = dalloc<char>(tb);
+      void *tmp = ar.take<char>(tb);
       FFHIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, keys, skeys, items, sitems, (int)E, 0,
                                                      64, s));
-      head = dalloc<int64_t>(E + 1); hscan = dalloc<int64_t>(E + 1);
-      mark = dalloc<int64_t>(NI + 1); rank = dalloc<int64_t>(NI + 1);
+      int64_t *head = ar.take<int64_t>(E + 1), *hscan = ar.take<int64_t>(E + 1);
+      int64_t *mark = ar.take<int64_t>(NI + 1), *rank = ar.take<int64_t>(NI + 1);
       FFHIP_CHECK(hipMemsetAsync(head + E, 0, 8, s));
       FFHIP_CHECK(hipMemsetAsync(mark + NI, 0, 8, s));
       k_p2_edge_heads<<<nblk(E, 256), 256, 0, s>>>(E, skeys, sitems, head, mark);
       k_p2_vmark<<<nblk(4 * nt, 256), 256, 0, s>>>(nt, m->elems, firstv, mark);
       exclusive_scan_i64(ctx, head, hscan, E + 1);
       sp->ne = read_i64(ctx, hscan + E);
-      item_seg = dalloc<int32_t>(E);
-      ukeys = dalloc<uint64_t>(sp->ne);
-      segfirst = dalloc<int32_t>(sp->ne);
-      k_p2_edge_seg<<<nblk(E, 256), 256, 0, s>>>(E, skeys, sitems, hscan, head, item_seg, ukeys,
+      int32_t *item_seg = ar.take<int32_t>(E);
+      int32_t *segfirst = ar.take<int32_t>(E);  // ne <= E
+      sp->ekeys = dalloc<uint64_t>(sp->ne);
+      k_p2_edge_seg<<<nblk(E, 256), 256, 0, s>>>(E, skeys, sitems, hscan, head, item_seg, sp->ekeys,
                                                  segfirst);
       exclusive_scan_i64(ctx, mark, rank, NI + 1);
       sp->nnodes = read_i64(ctx, rank + NI);
@@ -484,21 +482,16 @@ int ffhip_p2v_create(ffhip_ctx ctx, ffhip_mesh m, ffhip_p2v *out) {
       k_p2_vnode<<<nblk(nv, 256), 256, 0, s>>>(nv, firstv, rank, sp->vnode);
       sp->enode = dalloc<int32_t>(sp->ne);
       k_p2_enode<<<nblk(sp->ne, 256), 256, 0, s>>>(sp->ne, segfirst, rank, sp->enode);
-      sp->ekeys = ukeys;
-      ukeys = nullptr;
       FFHIP_CHECK(hipGetLastError());
+      FFHIP_CHECK(hipStreamSynchronize(s));
+      // the arena's temporaries are dead from here on (build_incidence reuses the work buffer)
       build_incidence(ctx, sp->elem_nodes, nt, 10, sp->nnodes, &sp->n2e_ptr, &sp->n2e, nullptr, nullptr);
       ctx->toc();
     } catch (...) {
       (void)hipStreamSynchronize(s);
-      dfree(firstv); dfree(keys); dfree(skeys); dfree(items); dfree(sitems); dfree(tmp);
-      dfree(head); dfree(hscan); dfree(mark); dfree(rank); dfree(item_seg); dfree(ukeys);
-      dfree(segfirst);
       p2v_release(sp);
       throw;
     }
-    dfree(firstv); dfree(keys); dfree(skeys); dfree(items); dfree(sitems); dfree(tmp);
-    dfree(head); dfree(hscan); dfree(mark); dfree(rank); dfree(item_seg); dfree(segfirst);
     *out = sp;
   });
 }
@@ -538,15 +531,14 @@ int ffhip_pattern_p2v(ffhip_ctx ctx, ffhip_p2v sp, ffhip_mat *out) {
     FFHIP_REQUIRE(n < INT_MAX, FFHIP_ERR_UNSUPPORTED, "ffhip_pattern_p2v: too many dofs");
     ffhip_mat A = new ffhip_mat_s();
     A->ctx = ctx; A->n = n; A->p2v = sp;
-    int64_t *len = nullptr, *ndiag = nullptr;
-    int32_t *ncol = nullptr;
-    int *ovf = nullptr;
     try {
+      Arena ar(ctx, Arena::bytes(nn + 1, 8) + Arena::bytes(1, 4) + Arena::bytes(nn, 8) +
+                        Arena::bytes(kNodeCap * nn, 4));
       ctx->tic();
-      len = dalloc<int64_t>(nn + 1);
-      ovf = dalloc<int>(1);
+      int64_t *len = ar.take<int64_t>(nn + 1);
+      int *ovf = ar.take<int>(1);
       A->nodeptr = dalloc<int64_t>(nn + 1);
-      ndiag = dalloc<int64_t>(nn);
+      int64_t *ndiag = ar.take<int64_t>(nn);
       A->slot8 = dalloc<uint8_t>(m->nt * 100);
       FFHIP_CHECK(hipMemsetAsync(len, 0, 8 * (nn + 1), s));
       FFHIP_CHECK(hipMemsetAsync(ovf, 0, sizeof(int), s));
@@ -559,7 +551,7 @@ int ffhip_pattern_p2v(ffhip_ctx ctx, ffhip_p2v sp, ffhip_mat *out) {
                     "ffhip_pattern_p2v: a node couples to more than %d nodes", CAP);
       exclusive_scan_i64(ctx, len, A->nodeptr, nn + 1);
       const int64_t nnz_nodes = read_i64(ctx, A->nodeptr + nn);
-      ncol = dalloc<int32_t>(nnz_nodes);
+      int32_t *ncol = ar.take<int32_t>(nnz_nodes);  // at most kNodeCap per node
       k_pattern_rows<B, CAP, true><<<nblk(nn, B), B, 0, s>>>(
           nn, 10, sp->elem_nodes, sp->n2e_ptr, sp->n2e, nullptr, A->nodeptr, ncol, ndiag, nullptr,
           A->slot8, ovf);
@@ -576,13 +568,11 @@ int ffhip_pattern_p2v(ffhip_ctx ctx, ffhip_p2v sp, ffhip_mat *out) {
       A->max_row = 3 * CAP;
     } catch (...) {
       (void)hipStreamSynchronize(s);
-      dfree(len); dfree(ovf); dfree(ndiag); dfree(ncol);
       dfree(A->rowptr); dfree(A->colidx); dfree(A->vals); dfree(A->diag);
       dfree(A->nodeptr); dfree(A->slot8);
       delete A;
       throw;
     }
-    dfree(len); dfree(ovf); dfree(ndiag); dfree(ncol);
     *out = A;
   });
 }
@@ -594,7 +584,8 @@ int ffhip_assemble_p2v_elasticity(ffhip_ctx ctx, ffhip_p2v sp, ffhip_mat A, doub
                   "ffhip_assemble_p2v_elasticity: matrix pattern was not built from this space");
     hipStream_t s = ctx->stream;
     ffhip_mesh m = sp->mesh;
-    double *Ke = dalloc<double>(900 * m->nt);
+    Arena ar(ctx, Arena::bytes(900 * m->nt, 8));
+    double *Ke = ar.take<double>(900 * m->nt);
     int64_t g = (m->nt + kElemWaves - 1) / kElemWaves;
     const int64_t gmax = 8 * (int64_t)ctx->num_cu;
     if (g > gmax) g = gmax;
@@ -604,7 +595,6 @@ int ffhip_assemble_p2v_elasticity(ffhip_ctx ctx, ffhip_p2v sp, ffhip_mat A, doub
                                                               sp->n2e, A->slot8, Ke, A->vals);
     FFHIP_CHECK(hipGetLastError());
     ctx->toc();
-    dfree(Ke);
   });
 }
 
@@ -620,9 +610,11 @@ int ffhip_set_bc_p2v(ffhip_ctx ctx, ffhip_p2v sp, ffhip_mat A, const int32_t *la
     bool ok = false;
     int miss = 0;
     try {
+      ctx->tic();
       ok = p2v_mark(ctx, sp, labels, nlab, onbc);
       FFHIP_CHECK(hipMemsetAsync(missing, 0, sizeof(int), ctx->stream));
       set_diag_rows(ctx, A->n, onbc, A->diag, tgv, A->vals, missing);
+      ctx->toc();
       if (onbc_out) {
         FFHIP_CHECK(hipMemcpyAsync(onbc_out, onbc, A->n, hipMemcpyDeviceToHost, ctx->stream));
       }

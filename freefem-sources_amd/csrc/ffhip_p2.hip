@@ -143,9 +143,116 @@ __global__ void k_p2_enode(int64_t ne, const int32_t *__restrict__ segfirst,
 }
 
 // ---------------------------------------------------------------- pattern
+constexpr int kNodeCap = 96;   // node columns per node row
+constexpr int kNodeHash = 256;  // LDS hash slots per node row
+
+// One node row per wave: lane q takes incidence q's 10 element nodes and
+// inserts them into an open-addressing set in LDS (atomicCAS, bounded
+// probing); the unique node columns are ordered by a counting sort and
+// written to the row's buffer (kNodeCap entries), with the row length, the
+// relative diagonal position and the 10 slot bytes of every incidence
+// (binary search in the sorted columns).  A row with more than 64 incident
+// elements, more than kNodeCap columns or a full set raises *ovf (the
+// lane-per-row merge then builds the whole pattern).
+__global__ __launch_bounds__(256) void k_p2_pattern_wave(
+    int64_t nn, const int32_t *__restrict__ elem_nodes, const int64_t *__restrict__ n2e_ptr,
+    const int32_t *__restrict__ n2e, int64_t *__restrict__ nlen, int32_t *__restrict__ rowbuf,
+    int8_t *__restrict__ dposr, uint8_t *__restrict__ slot8, int *__restrict__ ovf) {
+  constexpr int NW = 256 / kWave;
+  __shared__ int32_t hk[NW][kNodeHash];
+  __shared__ int32_t ucol[NW][kNodeCap];
+  __shared__ int32_t ucnt[NW];
+  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+  const int64_t a = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  int64_t k0 = 0;
+  int deg = 0;
+  if (a < nn) { k0 = n2e_ptr[a]; deg = (int)(n2e_ptr[a + 1] - k0); }
+  bool ok = deg <= kWave;
+  if (!ok) deg = 0;
+  for (int i = lane; i < kNodeHash; i += kWave) hk[w][i] = INT_MAX;
+  if (lane == 0) ucnt[w] = 0;
+  int32_t c[10];
+  if (lane < deg) {
+    const int64_t e = n2e[k0 + lane] / 10;
+#pragma unroll
+    for (int i = 0; i < 10; ++i) c[i] = elem_nodes[10 * e + i];
+  }
+  __syncthreads();
+  bool full = false;
+  if (lane < deg) {
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+      uint32_t h = ((uint32_t)c[i] * 2654435761u) >> (32 - 8);
+      bool placed = false;
+      for (int probe = 0; probe < kNodeHash && !placed; ++probe) {
+        const int32_t old = atomicCAS(&hk[w][h], INT_MAX, c[i]);
+        if (old == INT_MAX) {
+          const int idx = atomicAdd(&ucnt[w], 1);
+          if (idx < kNodeCap) ucol[w][idx] = c[i];
+          placed = true;
+        } else if (old == c[i]) {
+          placed = true;
+        }
+        h = (h + 1) & (kNodeHash - 1);
+      }
+      full |= !placed;
+    }
+  }
+  __syncthreads();
+  const int len = ucnt[w];
+  if (__any(full) || len > kNodeCap) ok = false;
+  if (a < nn && !ok && lane == 0) atomicOr(ovf, 1);
+  int32_t u[2] = {INT_MAX, INT_MAX};
+  int rk[2] = {0, 0};
+  if (ok) {
+#pragma unroll
+    for (int h2 = 0; h2 < 2; ++h2) {
+      const int j = lane + h2 * kWave;
+      if (j < len) {
+        u[h2] = ucol[w][j];
+        int r = 0;
+        for (int i = 0; i < len; ++i) r += ucol[w][i] < u[h2] ? 1 : 0;
+        rk[h2] = r;
+      }
+    }
+  }
+  __syncthreads();
+  int dpos = -1;
+  if (ok) {
+#pragma unroll
+    for (int h2 = 0; h2 < 2; ++h2)
+      if (lane + h2 * kWave < len) {
+        ucol[w][rk[h2]] = u[h2];
+        rowbuf[kNodeCap * a + rk[h2]] = u[h2];
+        if ((int64_t)u[h2] == a) dpos = rk[h2];
+      }
+  }
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const int t = __shfl_xor(dpos, off, kWave);
+    dpos = t > dpos ? t : dpos;
+  }
+  __syncthreads();
+  if (ok && a < nn && lane == 0) { nlen[a] = len; dposr[a] = (int8_t)dpos; }
+  if (ok && lane < deg) {
+    uint8_t *dst = slot8 + (k0 + lane) * 10;
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+      int lo = 0, hi = len - 1;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (ucol[w][mid] < c[i]) lo = mid + 1; else hi = mid;
+      }
+      dst[i] = (uint8_t)lo;
+    }
+  }
+}
+
 // scalar rows 3a + c of node a: for every node column b, columns 3b, 3b+1, 3b+2
+// (node columns from ncol at nodeptr, or from the row buffers when rowbuf != null)
 __global__ void k_p2v_expand(int64_t nn, const int64_t *__restrict__ nodeptr,
                              const int32_t *__restrict__ ncol, const int64_t *__restrict__ ndiag,
+                             const int32_t *__restrict__ rowbuf, const int8_t *__restrict__ dposr,
                              int64_t *__restrict__ rowptr, int32_t *__restrict__ colidx,
                              int64_t *__restrict__ diag) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -156,13 +263,15 @@ __global__ void k_p2v_expand(int64_t nn, const int64_t *__restrict__ nodeptr,
   const int64_t base = 9 * p0 + c * 3 * L;
   rowptr[r] = base;
   if (r == 3 * nn - 1) rowptr[3 * nn] = 9 * nodeptr[nn];
+  const int32_t *cols = rowbuf ? rowbuf + kNodeCap * a : ncol + p0;
   for (int64_t q = 0; q < L; ++q) {
-    const int32_t b = ncol[p0 + q];
+    const int32_t b = cols[q];
     colidx[base + 3 * q] = 3 * b;
     colidx[base + 3 * q + 1] = 3 * b + 1;
     colidx[base + 3 * q + 2] = 3 * b + 2;
   }
-  diag[r] = ndiag[a] < 0 ? -1 : base + 3 * (ndiag[a] - p0) + c;
+  const int64_t d = rowbuf ? (dposr[a] < 0 ? -1 : dposr[a]) : (ndiag[a] < 0 ? -1 : ndiag[a] - p0);
+  diag[r] = d < 0 ? -1 : base + 3 * d + c;
 }
 
 // ---------------------------------------------------------------- element matrices
@@ -284,8 +393,6 @@ __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
-
-constexpr int kNodeCap = 96;  // node columns per node row
 
 __global__ __launch_bounds__(256) void k_p2v_gather(
     int64_t nn, const int64_t *__restrict__ nodeptr, const int64_t *__restrict__ n2e_ptr,
@@ -533,35 +640,50 @@ int ffhip_pattern_p2v(ffhip_ctx ctx, ffhip_p2v sp, ffhip_mat *out) {
     A->ctx = ctx; A->n = n; A->p2v = sp;
     try {
       Arena ar(ctx, Arena::bytes(nn + 1, 8) + Arena::bytes(1, 4) + Arena::bytes(nn, 8) +
-                        Arena::bytes(kNodeCap * nn, 4));
+                        Arena::bytes(kNodeCap * nn, 4) + Arena::bytes(nn, 1));
       ctx->tic();
       int64_t *len = ar.take<int64_t>(nn + 1);
       int *ovf = ar.take<int>(1);
       A->nodeptr = dalloc<int64_t>(nn + 1);
       int64_t *ndiag = ar.take<int64_t>(nn);
+      int32_t *rowbuf = ar.take<int32_t>(kNodeCap * nn);
+      int8_t *dposr = ar.take<int8_t>(nn);
       A->slot8 = dalloc<uint8_t>(m->nt * 100);
       FFHIP_CHECK(hipMemsetAsync(len, 0, 8 * (nn + 1), s));
       FFHIP_CHECK(hipMemsetAsync(ovf, 0, sizeof(int), s));
-      constexpr int B = 128, CAP = kNodeCap;
-      k_pattern_rows<B, CAP, false><<<nblk(nn, B), B, 0, s>>>(
-          nn, 10, sp->elem_nodes, sp->n2e_ptr, sp->n2e, len, nullptr, nullptr, nullptr, nullptr,
-          nullptr, ovf);
+      // one wave per node row (hash set in LDS); the lane-per-row merge if a
+      // row does not fit
+      k_p2_pattern_wave<<<nblk(nn * kWave, 256), 256, 0, s>>>(nn, sp->elem_nodes, sp->n2e_ptr, sp->n2e,
+                                                              len, rowbuf, dposr, A->slot8, ovf);
       FFHIP_CHECK(hipGetLastError());
-      FFHIP_REQUIRE(read_flag(ctx, ovf) == 0, FFHIP_ERR_UNSUPPORTED,
-                    "ffhip_pattern_p2v: a node couples to more than %d nodes", CAP);
+      const bool wave_ok = read_flag(ctx, ovf) == 0;
+      constexpr int B = 128, CAP = kNodeCap;
+      if (!wave_ok) {
+        FFHIP_CHECK(hipMemsetAsync(len, 0, 8 * (nn + 1), s));
+        FFHIP_CHECK(hipMemsetAsync(ovf, 0, sizeof(int), s));
+        k_pattern_rows<B, CAP, false><<<nblk(nn, B), B, 0, s>>>(
+            nn, 10, sp->elem_nodes, sp->n2e_ptr, sp->n2e, len, nullptr, nullptr, nullptr, nullptr,
+            nullptr, ovf);
+        FFHIP_CHECK(hipGetLastError());
+        FFHIP_REQUIRE(read_flag(ctx, ovf) == 0, FFHIP_ERR_UNSUPPORTED,
+                      "ffhip_pattern_p2v: a node couples to more than %d nodes", CAP);
+      }
       exclusive_scan_i64(ctx, len, A->nodeptr, nn + 1);
       const int64_t nnz_nodes = read_i64(ctx, A->nodeptr + nn);
-      int32_t *ncol = ar.take<int32_t>(nnz_nodes);  // at most kNodeCap per node
-      k_pattern_rows<B, CAP, true><<<nblk(nn, B), B, 0, s>>>(
-          nn, 10, sp->elem_nodes, sp->n2e_ptr, sp->n2e, nullptr, A->nodeptr, ncol, ndiag, nullptr,
-          A->slot8, ovf);
+      int32_t *ncol = nullptr;
+      if (!wave_ok) {
+        ncol = rowbuf;  // the row buffers are free: nnz_nodes <= kNodeCap * nn
+        k_pattern_rows<B, CAP, true><<<nblk(nn, B), B, 0, s>>>(
+            nn, 10, sp->elem_nodes, sp->n2e_ptr, sp->n2e, nullptr, A->nodeptr, ncol, ndiag, nullptr,
+            A->slot8, ovf);
+      }
       A->nnz = 9 * nnz_nodes;
       A->rowptr = dalloc<int64_t>(n + 1);
       A->colidx = dalloc<int32_t>(A->nnz);
       A->diag = dalloc<int64_t>(n);
       A->vals = dalloc<double>(A->nnz);
-      k_p2v_expand<<<nblk(n, 256), 256, 0, s>>>(nn, A->nodeptr, ncol, ndiag, A->rowptr, A->colidx,
-                                                A->diag);
+      k_p2v_expand<<<nblk(n, 256), 256, 0, s>>>(nn, A->nodeptr, ncol, ndiag, wave_ok ? rowbuf : nullptr,
+                                                dposr, A->rowptr, A->colidx, A->diag);
       FFHIP_CHECK(hipMemsetAsync(A->vals, 0, 8 * A->nnz, s));
       FFHIP_CHECK(hipGetLastError());
       ctx->toc();

@@ -144,6 +144,7 @@ struct ffhip_p2v_s {
   int32_t *enode = nullptr;       // ne: node of each edge
   int64_t *n2e_ptr = nullptr;     // nnodes + 1
   int32_t *n2e = nullptr;         // nt * 10: element * 10 + local node, element order
+  int32_t *n2e_inv = nullptr;     // nt * 10: position of item element * 10 + local in n2e
 };
 
 namespace ffhip {

@@ -300,10 +300,11 @@ __device__ inline double p2_grad(const P2Row &R, const double l[4]) {
   return 4 * (R.g1 * pick4(l, R.i0) + R.g0 * pick4(l, R.i1));
 }
 
-// Ke[e][i][c][j][c'] = K[(c,i),(c',j)] (row = test function), 900 doubles
+// K[(c,i),(c',j)] (row = test function) as 10 node blocks of 90 doubles
+// [c][j][c'], block i stored at Ke[90 k] for the incidence k of (e, i)
 __global__ __launch_bounds__(kElemBlock) void k_p2v_elast_element(
     int64_t nt, const double *__restrict__ xyz, const int32_t *__restrict__ tets, double lambda,
-    double mu, double *__restrict__ Ke) {
+    double mu, const int32_t *__restrict__ inv, double *__restrict__ Ke) {
   __shared__ double sM[kElemWaves][32 * kMs];
   const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const int col = lane % 16, kq = lane / 16;
@@ -372,14 +373,14 @@ __global__ __launch_bounds__(kElemBlock) void k_p2v_elast_element(
     }
     __syncthreads();
     if (e < nt) {
-      double *dst = Ke + e * 900;
       for (int o = lane; o < 900; o += kWave) {
         const int i = o / 90, rem = o % 90, c = rem / 30, r2 = rem % 30, j = r2 / 3, cp = r2 % 3;
         double k = lambda * M[(3 * i + c) * kMs + 3 * j + cp] + mu * M[(3 * i + cp) * kMs + 3 * j + c];
         if (c == cp)
           k += mu * (M[(3 * i) * kMs + 3 * j] + M[(3 * i + 1) * kMs + 3 * j + 1] +
                      M[(3 * i + 2) * kMs + 3 * j + 2]);
-        dst[o] = k;
+        // node block i goes where its incidence entry sits in node i's list
+        Ke[(int64_t)inv[10 * e + i] * 90 + rem] = k;
       }
     }
     __syncthreads();
@@ -396,8 +397,7 @@ __device__ inline void wave_lds_sync() {
 
 __global__ __launch_bounds__(256) void k_p2v_gather(
     int64_t nn, const int64_t *__restrict__ nodeptr, const int64_t *__restrict__ n2e_ptr,
-    const int32_t *__restrict__ n2e, const uint8_t *__restrict__ slot8,
-    const double *__restrict__ Ke, double *__restrict__ vals) {
+    const uint8_t *__restrict__ slot8, const double *__restrict__ Ke, double *__restrict__ vals) {
   __shared__ double S[256 / kWave][9 * kNodeCap];
   const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const int64_t a = (int64_t)blockIdx.x * (256 / kWave) + w;
@@ -415,22 +415,30 @@ __global__ __launch_bounds__(256) void k_p2v_gather(
     tj[h] = r2 / 3;
     tdst[h] = c * 3 * L + r2 % 3;  // + 3 * offset of node column j
   }
+  const bool two = lane < 90 - kWave;
   wave_lds_sync();
   const int64_t k0 = n2e_ptr[a], k1 = n2e_ptr[a + 1];
+  // the node's blocks are contiguous (incidence order = element order):
+  // the next block's loads are issued before the current one is added
+  double v0 = 0., v1 = 0.;
+  int o0 = 0, o1 = 0;
+  if (k0 < k1) {
+    v0 = Ke[k0 * 90 + lane];
+    o0 = slot8[k0 * 10 + tj[0]];
+    if (two) { v1 = Ke[k0 * 90 + lane + kWave]; o1 = slot8[k0 * 10 + tj[1]]; }
+  }
   for (int64_t k = k0; k < k1; ++k) {
-    const int32_t item = n2e[k];
-    const int64_t e = item / 10;
-    const int i = item - (int)e * 10;
-    const double *blk = Ke + e * 900 + i * 90;
-    const uint8_t *sl = slot8 + k * 10;
-    const double v0 = blk[lane];
-    const int o0 = sl[tj[0]];
-    double v1 = 0.;
-    int o1 = 0;
-    if (lane < 90 - kWave) { v1 = blk[lane + kWave]; o1 = sl[tj[1]]; }
+    double n0 = 0., n1 = 0.;
+    int m0 = 0, m1 = 0;
+    if (k + 1 < k1) {
+      n0 = Ke[(k + 1) * 90 + lane];
+      m0 = slot8[(k + 1) * 10 + tj[0]];
+      if (two) { n1 = Ke[(k + 1) * 90 + lane + kWave]; m1 = slot8[(k + 1) * 10 + tj[1]]; }
+    }
     acc[tdst[0] + 3 * o0] += v0;
-    if (lane < 90 - kWave) acc[tdst[1] + 3 * o1] += v1;
+    if (two) acc[tdst[1] + 3 * o1] += v1;
     wave_lds_sync();
+    v0 = n0; v1 = n1; o0 = m0; o1 = m1;
   }
   double *out = vals + 9 * p0;
   for (int t = lane; t < Wd; t += kWave) out[t] = acc[t];
@@ -529,7 +537,7 @@ static bool p2v_mark(ffhip_ctx ctx, ffhip_p2v sp, const int32_t *labels, int nla
 
 static void p2v_release(ffhip_p2v sp) {
   dfree(sp->elem_nodes); dfree(sp->vnode); dfree(sp->ekeys); dfree(sp->enode);
-  dfree(sp->n2e_ptr); dfree(sp->n2e);
+  dfree(sp->n2e_ptr); dfree(sp->n2e); dfree(sp->n2e_inv);
   delete sp;
 }
 
@@ -592,7 +600,8 @@ int ffhip_p2v_create(ffhip_ctx ctx, ffhip_mesh m, ffhip_p2v *out) {
       FFHIP_CHECK(hipGetLastError());
       FFHIP_CHECK(hipStreamSynchronize(s));
       // the arena's temporaries are dead from here on (build_incidence reuses the work buffer)
-      build_incidence(ctx, sp->elem_nodes, nt, 10, sp->nnodes, &sp->n2e_ptr, &sp->n2e, nullptr, nullptr);
+      build_incidence(ctx, sp->elem_nodes, nt, 10, sp->nnodes, &sp->n2e_ptr, &sp->n2e, nullptr,
+                      &sp->n2e_inv);
       ctx->toc();
     } catch (...) {
       (void)hipStreamSynchronize(s);
@@ -712,9 +721,10 @@ int ffhip_assemble_p2v_elasticity(ffhip_ctx ctx, ffhip_p2v sp, ffhip_mat A, doub
     const int64_t gmax = 8 * (int64_t)ctx->num_cu;
     if (g > gmax) g = gmax;
     ctx->tic();
-    k_p2v_elast_element<<<(unsigned)g, kElemBlock, 0, s>>>(m->nt, m->xyz, m->elems, lambda, mu, Ke);
+    k_p2v_elast_element<<<(unsigned)g, kElemBlock, 0, s>>>(m->nt, m->xyz, m->elems, lambda, mu,
+                                                          sp->n2e_inv, Ke);
     k_p2v_gather<<<nblk(sp->nnodes, 256 / kWave), 256, 0, s>>>(sp->nnodes, A->nodeptr, sp->n2e_ptr,
-                                                              sp->n2e, A->slot8, Ke, A->vals);
+                                                              A->slot8, Ke, A->vals);
     FFHIP_CHECK(hipGetLastError());
     ctx->toc();
   });

@@ -669,6 +669,152 @@ __global__ FFHIP_MATVEC_ATTR void k_residual(MatFmt<double> M, const double *__r
   }
 }
 
+// ---------------------------------------------------------------- affine CG
+// LinearCG / AffineCG with the matrix product as the operator:
+// ConjuguedGradient2 (src/femlib/MatriceCreuse.hpp:180-240).  Iteration it:
+//   k_cg2_step:   (Cg, g) of the last update, stop test, gamma = g2/g2p;
+//                 h = gamma h - Cg (it > 0); x += rop h
+//   k_cg2_dir:    Ah = A x - b - g; partials (h, Ah), (g, h)
+//   k_cg2_update: ro = -(g, h) rop / (h, Ah); x += (ro - rop) h;
+//                 g += (ro / rop) Ah; Cg = C g; partial (Cg, g)
+// C is the identity (dinv null) or the Jacobi diagonal inverse.  Scalars
+// written by a kernel's lead thread are kept by parity of the iteration, so
+// no block reads a slot another block of the same kernel writes.
+struct Cg2State {
+  double g2[2];    // (Cg, g) reduced by the step kernel of iteration it, slot it & 1
+  double ro[2];    // ro of iteration it, slot it & 1
+  double g20, reps2, g2last;
+  int done;        // 0 run, 1 converged, 2 g2_0 < 1e-30, 3 itmax, 5 (h,Ah)^2 < 1e-60, 6 ro NaN
+  int iters;
+};
+
+// g = A x - b, Cg = C g, h = -Cg; partial (Cg, g)
+template <int KIND>
+__global__ FFHIP_MATVEC_ATTR void k_cg2_init(MatFmt<double> M, const double *__restrict__ x,
+                                             const double *__restrict__ b,
+                                             const double *__restrict__ dinv, double *__restrict__ g,
+                                             double *__restrict__ Cg, double *__restrict__ h,
+                                             double *__restrict__ pout, Cg2State *st) {
+  __shared__ double lds[kBlock / kWave];
+  double part = 0.;
+  matvec_rows<KIND, double, double>(
+      M, [&](int32_t j) { return x[j]; },
+      [&](int64_t r, double acc) {
+        const double gv = b ? acc - b[r] : acc;
+        const double c = dinv ? dinv[r] * gv : gv;
+        g[r] = gv; Cg[r] = c; h[r] = -c;
+        part += c * gv;
+      });
+  const double tot = block_sum<kBlock>(part, lds);
+  if (threadIdx.x == 0) pout[xcd_block_id()] = tot;
+  if (blockIdx.x == 0 && threadIdx.x == 0) { st->done = 0; st->iters = 0; }
+}
+
+__global__ __launch_bounds__(kBlock) void k_cg2_step(int it, int itmax, double eps, int64_t n,
+                                                     double *__restrict__ h,
+                                                     const double *__restrict__ Cg,
+                                                     double *__restrict__ x,
+                                                     const double *__restrict__ pin, Cg2State *st) {
+  __shared__ double lds[kBlock / kWave];
+  if (st->done) return;
+  const double g2 = reduce_partials(pin, gridDim.x, 1, 0, lds);
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  double gamma = 0., rop = 1.;
+  if (it == 0) {
+    int done = 0;
+    if (g2 != g2) done = 6;  // the reference's first ro would be NaN
+    else if (g2 < 1e-30) done = 2;
+    if (lead) {
+      st->g20 = g2; st->g2last = g2; st->g2[0] = g2;
+      st->reps2 = eps > 0 ? eps * eps * g2 : -eps;
+      if (done) { st->done = done; st->iters = 0; }
+    }
+    if (done) return;
+  } else {
+    int done = 0;
+    if (g2 < st->reps2) done = 1;
+    else if (it - 1 >= itmax) done = 3;
+    if (lead) {
+      st->g2[it & 1] = g2; st->g2last = g2;
+      if (done) { st->done = done; st->iters = it; }
+    }
+    if (done) return;
+    gamma = g2 / st->g2[(it - 1) & 1];
+    rop = st->ro[(it - 1) & 1];
+  }
+  int64_t r0, r1;
+  block_rows(n, r0, r1);
+  for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) {
+    double hv = h[r];
+    if (it > 0) { hv = hv * gamma - Cg[r]; h[r] = hv; }
+    x[r] += rop * hv;
+  }
+}
+
+template <int KIND>
+__global__ FFHIP_MATVEC_ATTR void k_cg2_dir(MatFmt<double> M, const double *__restrict__ x,
+                                            const double *__restrict__ b,
+                                            const double *__restrict__ g,
+                                            const double *__restrict__ h, double *__restrict__ Ah,
+                                            double *__restrict__ pout, const Cg2State *st) {
+  __shared__ double lds[kBlock / kWave];
+  if (st->done) return;
+  double p_hah = 0., p_gh = 0.;
+  matvec_rows<KIND, double, double>(
+      M, [&](int32_t j) { return x[j]; },
+      [&](int64_t r, double acc) {
+        const double a = (b ? acc - b[r] : acc) - g[r];
+        Ah[r] = a;
+        p_hah += h[r] * a;
+        p_gh += g[r] * h[r];
+      });
+  const double a = block_sum<kBlock>(p_hah, lds);
+  const double c = block_sum<kBlock>(p_gh, lds);
+  if (threadIdx.x == 0) {
+    const int lb = xcd_block_id();
+    pout[2 * lb] = a;
+    pout[2 * lb + 1] = c;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_cg2_update(
+    int it, int64_t n, const double *__restrict__ h, const double *__restrict__ Ah,
+    const double *__restrict__ dinv, double *__restrict__ x, double *__restrict__ g,
+    double *__restrict__ Cg, const double *__restrict__ pin, double *__restrict__ pout,
+    Cg2State *st) {
+  __shared__ double lds[kBlock / kWave];
+  if (st->done) return;
+  const int P = gridDim.x;
+  const double hah = reduce_partials(pin, P, 2, 0, lds);
+  const double gh = reduce_partials(pin, P, 2, 1, lds);
+  const double rop = it == 0 ? 1. : st->ro[(it - 1) & 1];
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (hah * hah < 1e-60) {
+    if (lead) { st->done = 5; st->iters = it; }
+    return;
+  }
+  const double ro = -gh * rop / hah;
+  if (ro != ro) {
+    if (lead) { st->done = 6; st->iters = it; }
+    return;
+  }
+  if (lead) st->ro[it & 1] = ro;
+  const double dx = ro - rop, dg = ro / rop;
+  int64_t r0, r1;
+  block_rows(n, r0, r1);
+  double part = 0.;
+  for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) {
+    x[r] += dx * h[r];
+    const double gv = g[r] + dg * Ah[r];
+    g[r] = gv;
+    const double c = dinv ? dinv[r] * gv : gv;
+    Cg[r] = c;
+    part += c * gv;
+  }
+  const double tot = block_sum<kBlock>(part, lds);
+  if (threadIdx.x == 0) pout[xcd_block_id()] = tot;
+}
+
 // plain mat-vec for measurement: y = A x
 template <int KIND, class V, class X>
 __global__ FFHIP_MATVEC_ATTR void k_matvec(MatFmt<V> M, const X *__restrict__ x, X *__restrict__ y) {
@@ -820,6 +966,7 @@ struct ffhip_cg_s {
   double *pA = nullptr, *pB = nullptr, *pC = nullptr;
   CgState *st = nullptr;
   MixState *ms = nullptr;
+  Cg2State *st2 = nullptr;  // affine CG (LinearCG)
   // per-kernel timing (measurement only): 3 events per iteration of a chunk
   int profile = 0;
   std::vector<hipEvent_t> ev;
@@ -1168,6 +1315,7 @@ static void cg_release(ffhip_cg cg) {
   dfree(cg->xw); dfree(cg->bw); dfree(cg->G); dfree(cg->CG); dfree(cg->H[0]); dfree(cg->H[1]);
   dfree(cg->AH); dfree(cg->r32); dfree(cg->q32); dfree(cg->y32); dfree(cg->p32[0]);
   dfree(cg->p32[1]); dfree(cg->pA); dfree(cg->pB); dfree(cg->pC); dfree(cg->st); dfree(cg->ms);
+  dfree(cg->st2);
   delete cg;
 }
 
@@ -1206,6 +1354,7 @@ int ffhip_cg_create(ffhip_ctx ctx, ffhip_mat A, int mode, ffhip_cg *out) {
       cg->pC = dalloc<double>(3 * (int64_t)cg->grid);
       cg->st = dalloc<CgState>(1);
       cg->ms = dalloc<MixState>(1);
+      cg->st2 = dalloc<Cg2State>(1);
     } catch (...) {
       (void)hipStreamSynchronize(ctx->stream);
       cg_release(cg);
@@ -1306,6 +1455,65 @@ int ffhip_cg_solve(ffhip_cg cg, const double *b, double *x, int on_device, doubl
     FFHIP_REQUIRE(hs.done != 4, FFHIP_ERR_NAN, "ffhip_cg_solve: NaN in CG (bad matrix)");
     FFHIP_REQUIRE(hs.done != 3, FFHIP_ERR_NOCONV,
                   "ffhip_cg_solve: ConjugueGradient did not converge in %d iterations", itmax);
+  });
+}
+
+int ffhip_cg_solve_affine(ffhip_cg cg, const double *b, double *x, int on_device, double eps,
+                          int itmax, int precon, double *eps_out, ffhip_cg_stats *stats) {
+  return guard([&] {
+    FFHIP_REQUIRE(cg && x, FFHIP_ERR_ARG, "ffhip_cg_solve_affine: null argument");
+    FFHIP_REQUIRE(precon == FFHIP_CG2_IDENTITY || precon == FFHIP_CG2_JACOBI, FFHIP_ERR_ARG,
+                  "ffhip_cg_solve_affine: precon %d", precon);
+    FFHIP_REQUIRE(itmax >= 0, FFHIP_ERR_ARG, "ffhip_cg_solve_affine: nbiter %d", itmax);
+    ffhip_ctx ctx = cg->ctx;
+    hipStream_t s = ctx->stream;
+    const int64_t n = cg->n;
+    const hipMemcpyKind kin = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    if (b) FFHIP_CHECK(hipMemcpyAsync(cg->bw, b, 8 * n, kin, s));
+    FFHIP_CHECK(hipMemcpyAsync(cg->xw, x, 8 * n, kin, s));
+    const double *bd = b ? cg->bw : nullptr;
+    const double *dinv = precon == FFHIP_CG2_JACOBI ? cg->dinv : nullptr;
+    double *h = cg->H[0];
+    const int G = cg->grid;
+    const MatFmt<double> M = cg->fmt_of(cg->sval);
+    ctx->tic();
+    FFHIP_LAUNCH_FMT(cg, k_cg2_init, G, s, M, cg->xw, bd, dinv, cg->G, cg->CG, h, cg->pA, cg->st2);
+    FFHIP_CHECK(hipGetLastError());
+    Cg2State hs;
+    const int chunk = 16;
+    for (int it = 0;; ++it) {
+      k_cg2_step<<<G, kBlock, 0, s>>>(it, itmax, eps, n, h, cg->CG, cg->xw, cg->pA, cg->st2);
+      FFHIP_LAUNCH_FMT(cg, k_cg2_dir, G, s, M, cg->xw, bd, cg->G, h, cg->AH, cg->pB, cg->st2);
+      k_cg2_update<<<G, kBlock, 0, s>>>(it, n, h, cg->AH, dinv, cg->xw, cg->G, cg->CG, cg->pB,
+                                        cg->pA, cg->st2);
+      if ((it + 1) % chunk == 0) {
+        FFHIP_CHECK(hipGetLastError());
+        FFHIP_CHECK(hipMemcpyAsync(&hs, cg->st2, sizeof(hs), hipMemcpyDeviceToHost, s));
+        FFHIP_CHECK(hipStreamSynchronize(s));
+        if (hs.done) break;
+      }
+    }
+    ctx->toc();
+    const double ms = ctx->last_ms;
+    double res[3] = {0., 0., 0.};
+    if (!b) FFHIP_CHECK(hipMemsetAsync(cg->bw, 0, 8 * n, s));
+    residual_sums(cg, cg->sval, cg->xw, cg->bw, nullptr, res);
+    FFHIP_CHECK(hipMemcpyAsync(x, cg->xw, 8 * n, on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+    FFHIP_CHECK(hipStreamSynchronize(s));
+    if (eps_out) *eps_out = hs.done == 2 ? eps : hs.reps2;
+    if (stats) {
+      memset(stats, 0, sizeof(*stats));
+      stats->iters = hs.iters;
+      stats->converged = hs.done == 1 ? 1 : (hs.done == 2 ? 2 : 0);
+      stats->refinements = 0;
+      stats->gcg0 = hs.g20;
+      stats->gcg = hs.g2last;
+      stats->true_gcg = dinv ? res[2] : res[0];
+      stats->true_relres = res[1] > 0 ? std::sqrt(res[0] / res[1]) : std::sqrt(res[0]);
+      stats->ms_solve = ms;
+    }
+    FFHIP_REQUIRE(hs.done != 5, FFHIP_ERR_NAN, "CG2: Matrix is not defined (/0), sorry");
+    FFHIP_REQUIRE(hs.done != 6, FFHIP_ERR_NAN, "CG2: Bug : ro is NaN");
   });
 }
 

@@ -27,6 +27,7 @@ COEF_CONST, COEF_SIN3, COEF_P0, COEF_QP = 0, 1, 2, 3
 CG_FP64, CG_MIXED = 0, 1
 CG_LAYOUT_AUTO, CG_LAYOUT_SELL, CG_LAYOUT_CSR, CG_LAYOUT_BSR3 = 0, 0x10, 0x20, 0x100
 CG_INDEX16, CG_INDEX32 = 0x40, 0x80
+CG2_IDENTITY, CG2_JACOBI = 0, 1
 ERRORS = {-1: "bad argument", -2: "HIP error", -3: "CG did not converge", -4: "NaN in CG",
           -5: "unsupported", -6: "internal error"}
 
@@ -84,6 +85,8 @@ _SIGS = {
     "ffhip_cg_update_values": (_i, [_vp]),
     "ffhip_cg_layout": (_i, [_vp, ctypes.POINTER(_i), _pl]),
     "ffhip_cg_solve": (_i, [_vp, _vp, _vp, _i, _d, _i, _d, _i, ctypes.POINTER(CGStats)]),
+    "ffhip_cg_solve_affine": (_i, [_vp, _vp, _vp, _i, _d, _i, _i, ctypes.POINTER(_d),
+                                   ctypes.POINTER(CGStats)]),
     "ffhip_cg_free": (_i, [_vp]),
     "ffhip_cg_set_profiling": (_i, [_vp, _i]),
     "ffhip_bench_spmv": (_i, [_vp, _i, _i, _vp, _vp, ctypes.POINTER(_d)]),
@@ -453,6 +456,37 @@ class CG:
         if raise_on_error:
             check(rc)
         return x, st.as_dict()
+
+    def linear_cg(self, x, b=None, eps=1e-6, nbiter=100, precon=None, veps=None,
+                  raise_on_error=True):
+        """LinearCG(A*x, x, b, eps=, nbiter=, precon=, veps=) / AffineCG(A*x, x)
+        (lgfem.cpp:705-758): ConjuguedGradient2 with this solver's matrix as
+        the operator, in FP64.  precon: None (identity) or "jacobi".  x (the
+        initial guess, updated in place) and b: numpy arrays (host) or torch
+        cuda tensors (device); b None is the zero right-hand side.  veps, a
+        one-element numpy array, replaces eps when given and receives -reps2
+        afterwards, as the script's veps does.  Returns (ret, stats): ret is
+        LinearCG's value, 1 converged, 2 nothing to do, 0 not converged."""
+        pc = {None: CG2_IDENTITY, "identity": CG2_IDENTITY, "jacobi": CG2_JACOBI}[precon]
+        if veps is not None:
+            eps = float(veps[0])
+        st = CGStats()
+        eo = _d()
+        if isinstance(x, np.ndarray):
+            assert x.dtype == np.float64 and x.flags["C_CONTIGUOUS"]
+            bb = None if b is None else np.ascontiguousarray(b, np.float64)
+            rc = lib().ffhip_cg_solve_affine(self.h, _hp(bb), _hp(x), 0, eps, nbiter, pc,
+                                             ctypes.byref(eo), ctypes.byref(st))
+        else:
+            rc = lib().ffhip_cg_solve_affine(self.h, _dp(b), _dp(x), 1, eps, nbiter, pc,
+                                             ctypes.byref(eo), ctypes.byref(st))
+        if raise_on_error:
+            check(rc)
+        if veps is not None:
+            veps[0] = -eo.value
+        d = st.as_dict()
+        d["reps2"] = eo.value
+        return d["converged"], d
 
     def bench_spmv(self, mode, reps, x, y):
         ms = _d()

@@ -30,6 +30,8 @@
 //    Any other form falls back to FreeFEM's own
 //    creationBlockOfMatrixToBilinearForm (problem.hpp:1663-1714), as the
 //    PETSc plugin does for its Mat type.
+//  * LinearCG / AffineCG overloads whose operator is a matrix (the affine
+//    ConjuguedGradient2 of LinearCG on the GPU, see HipLinearCG).
 //
 // Build (FreeFEM's plugin script):  ff-c++ -auto ffhip.cpp -I<repo>/include
 //   -L<repo>/freefem-sources_amd -lffhip
@@ -476,6 +478,85 @@ class OpHipMatrixP1 : public OneOperator {
   }
 };
 
+// ---------------------------------------------------------------- LinearCG
+// LinearCG(A, x, b, ...) and AffineCG(A, x, ...) / LinearCG(A, x, ...) with a
+// matrix as the operator: ConjuguedGradient2 (MatriceCreuse.hpp:180-240) on
+// the GPU in place of LinearCG<R>::E_LCG (lgfem.cpp:705-758) calling back
+//   func real[int] op(real[int] &u) { real[int] v = A*u; return v; }
+// for every product.  Named parameters as LinearCG's (eps, nbiter, veps,
+// verbosity); the preconditioner is named by jacobi=true (C g = D^-1 g, the
+// diagonal of A) instead of a precon= function.  Returns LinearCG's value:
+// 1 converged, 2 nothing to do, 0 not converged.
+class HipLinearCG : public OneOperator {
+ public:
+  const int cas;  // 2: with a right-hand side, 1: without (AffineCG)
+  class Op : public E_F0mps {
+   public:
+    static const int n_name_param = 5;
+    static basicAC_F0::name_and_type name_param[];
+    Expression nargs[n_name_param];
+    Expression A, X, B;
+    Op(const basicAC_F0 &args, int cas) : B(0) {
+      args.SetNameParam(n_name_param, name_param, nargs);
+      A = to<Matrice_Creuse<double> *>(args[0]);
+      X = to<KN<double> *>(args[1]);
+      if (cas == 2) B = to<KN<double> *>(args[2]);
+    }
+    operator aType() const { return atype<long>(); }
+    AnyType operator()(Stack stack) const {
+      Matrice_Creuse<double> &MA = *GetAny<Matrice_Creuse<double> *>((*A)(stack));
+      KN<double> &x = *GetAny<KN<double> *>((*X)(stack));
+      KN<double> *b = B ? GetAny<KN<double> *>((*B)(stack)) : 0;
+      double eps = 1.0e-6, *veps = 0;
+      long nbitermax = 100, verb = verbosity;
+      bool jacobi = false;
+      if (nargs[0]) eps = GetAny<double>((*nargs[0])(stack));
+      if (nargs[1]) nbitermax = GetAny<long>((*nargs[1])(stack));
+      if (nargs[2]) veps = GetAny<double *>((*nargs[2])(stack));
+      if (nargs[3]) verb = Abs(GetAny<long>((*nargs[3])(stack)));
+      if (nargs[4]) jacobi = GetAny<bool>((*nargs[4])(stack));
+      if (veps) eps = *veps;
+      HashMatrix<int, double> *H = MA.pHM();
+      if (!H) ExecError("LinearCG: empty matrix");
+      ffassert(H->n == H->m && H->n == x.N() && (!b || b->N() == x.N()));
+      int *ia = 0, *ja = 0;
+      double *a = 0;
+      H->CSR(ia, ja, a);
+      const int64_t n = H->n, nnz = ia[n];
+      vector<int64_t> rp(ia, ia + n + 1);
+      ffhip_mat M = nullptr;
+      ffhip_cg cg = nullptr;
+      hip_check(ffhip_mat_upload(hip_ctx(), n, nnz, rp.data(), ja, a, &M), "LinearCG: matrix upload");
+      int rc = ffhip_cg_create(hip_ctx(), M, FFHIP_CG_FP64, &cg);
+      if (rc != FFHIP_OK) { ffhip_mat_free(M); hip_check(rc, "LinearCG: solver setup"); }
+      double eps_out = eps;
+      ffhip_cg_stats st;
+      rc = ffhip_cg_solve_affine(cg, b ? (double *)*b : nullptr, (double *)x, 0, eps, (int)nbitermax,
+                                 jacobi ? FFHIP_CG2_JACOBI : FFHIP_CG2_IDENTITY, &eps_out, &st);
+      ffhip_cg_free(cg);
+      ffhip_mat_free(M);
+      hip_check(rc, "CG2: Matrix is not defined (/0) or ro is NaN");
+      if (veps) *veps = -eps_out;
+      if (verb > 1 || (verb && st.converged != 1))
+        cout << (st.converged ? "CG converges " : "CG doesn't converge: ") << st.iters
+             << " ||g||^2 = " << st.gcg << " (" << st.ms_solve << " ms on the GPU)" << endl;
+      return SetAny<long>(st.converged);
+    }
+  };
+  E_F0 *code(const basicAC_F0 &args) const { return new Op(args, cas); }
+  HipLinearCG()
+      : OneOperator(atype<long>(), atype<Matrice_Creuse<double> *>(), atype<KN<double> *>(),
+                    atype<KN<double> *>()),
+        cas(2) {}
+  explicit HipLinearCG(int)
+      : OneOperator(atype<long>(), atype<Matrice_Creuse<double> *>(), atype<KN<double> *>()),
+        cas(1) {}
+};
+
+basicAC_F0::name_and_type HipLinearCG::Op::name_param[] = {
+    {"eps", &typeid(double)},    {"nbiter", &typeid(long)},     {"veps", &typeid(double *)},
+    {"verbosity", &typeid(long)}, {"jacobi", &typeid(bool)}};
+
 }  // namespace
 
 static void Load_Init() {
@@ -484,6 +565,9 @@ static void Load_Init() {
   Global.Add("ffhipreplacecg", "(", new OneOperator1<long, long>(ffhip_replace_cg));
   TheOperators->Add("=", new OpHipMatrixP1(0));
   TheOperators->Add("<-", new OpHipMatrixP1(1));
+  Global.Add("LinearCG", "(", new HipLinearCG());
+  Global.Add("LinearCG", "(", new HipLinearCG(1));
+  Global.Add("AffineCG", "(", new HipLinearCG(1));
   if (verbosity > 1) cout << " load: ffhip (" << ffhip_version() << ")" << endl;
 }
 

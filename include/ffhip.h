@@ -230,6 +230,25 @@ int ffhip_cg_update_values(ffhip_cg cg);
 int ffhip_cg_solve(ffhip_cg cg, const double *b, double *x, int on_device,
                    double eps, int itmax, double tgv, int trans,
                    ffhip_cg_stats *stats);
+/* LinearCG / AffineCG whose operator is the matrix product A*x
+ * (LinearCG<R>::E_LCG::operator(), src/fflib/lgfem.cpp:705-758, over
+ * ConjuguedGradient2, src/femlib/MatriceCreuse.hpp:180-240), in FP64 with the
+ * solver's layout whatever its mode: affine CG on g = A x - b (b NULL: the
+ * zero right-hand side of AffineCG / LinearCG without b).  precon:
+ * FFHIP_CG2_IDENTITY (no precon=, MatriceIdentite) or FFHIP_CG2_JACOBI
+ * (C g = D^-1 g with the Jacobi diagonal of ffhip_cg_solve).  Stop when
+ * (Cg, g) < eps^2 (Cg0, g0) for eps > 0, < -eps for eps < 0; the loop runs
+ * iter = 0..itmax (nbiter).  *eps_out (may be NULL) receives the threshold
+ * the reference writes back into eps (LinearCG stores its negation in
+ * veps).  stats->converged is ConjuguedGradient2's return value, 1
+ * converged, 2 (Cg0, g0) < 1e-30, 0 not converged: not converging is no
+ * error (LinearCG returns 0 to the script).  FFHIP_ERR_NAN for the
+ * reference's two ExecErrors ((h, Ah)^2 < 1e-60; ro NaN).  stats->iters
+ * counts executed iterations (iter + 1 at convergence). */
+enum { FFHIP_CG2_IDENTITY = 0, FFHIP_CG2_JACOBI = 1 };
+int ffhip_cg_solve_affine(ffhip_cg cg, const double *b, double *x, int on_device,
+                          double eps, int itmax, int precon, double *eps_out,
+                          ffhip_cg_stats *stats);
 int ffhip_cg_free(ffhip_cg cg);
 /* Per-kernel HIP-event timing of the solve loop (measurement only). */
 int ffhip_cg_set_profiling(ffhip_cg cg, int on);

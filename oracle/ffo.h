@@ -163,6 +163,19 @@ double ffo_gettgv(int n, const int64_t *rowptr, const int32_t *colidx,
 int ffo_pcg(int n, const int64_t *rowptr, const int32_t *colidx,
             const double *vals, const double *b, double *x, double eps,
             int itmax, double tgv, int trans, double *stats);
+/* LinearCG / AffineCG (lgfem.cpp:705-758) with the matrix product as the
+ * operator, i.e. ConjuguedGradient2 (MatriceCreuse.hpp:180-240) on
+ * g = A x - b (b NULL: AffineCG's zero right-hand side).  precon 0: identity
+ * (MatriceIdentite), 1: Jacobi, diagonal inverse (1 where a_ii^2 < 1e-60).
+ * *eps in: eps > 0 relative, < 0 absolute threshold on (Cg, g); out: the
+ * threshold reps2 (unchanged when the start is already converged).  The loop
+ * runs iter = 0..itmax.  stats = {iterations, g2_0, g2_final, nret}.
+ * Returns ConjuguedGradient2's value (1 converged, 2 g2_0 < 1e-30, 0 not
+ * converged) or -1 for its ExecError "Matrix is not defined (/0)" and -2 for
+ * "ro is NaN". */
+int ffo_cg2(int n, const int64_t *rowptr, const int32_t *colidx,
+            const double *vals, const double *b, double *x, double *eps,
+            int itmax, int precon, double *stats);
 
 #ifdef __cplusplus
 }

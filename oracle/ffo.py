@@ -66,6 +66,7 @@ def lib():
             "ffo_spmv_t": (None, [_i, _i64p, _i32p, _f64p, _f64p, _f64p]),
             "ffo_gettgv": (_d, [_i, _i64p, _i32p, _f64p, ctypes.POINTER(_i)]),
             "ffo_pcg": (_i, [_i, _i64p, _i32p, _f64p, _f64p, _f64p, _d, _i, _d, _i, _f64p]),
+            "ffo_cg2": (_i, [_i, _i64p, _i32p, _f64p, _f64p, _f64p, _f64p, _i, _i, _f64p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -221,6 +222,19 @@ def pcg(rowptr, colidx, vals, b, x0=None, eps=1e-10, itmax=0, tgv=1e30, trans=Fa
     lib().ffo_pcg(n, rowptr, colidx, vals, np.ascontiguousarray(b, np.float64), x, eps,
                   itmax, tgv, int(trans), st)
     return x, dict(iters=int(st[0]), gcg0=st[1], gcg=st[2], nret=int(st[3]))
+
+
+def cg2(rowptr, colidx, vals, b=None, x0=None, eps=1e-6, itmax=100, precon=0):
+    """LinearCG(A*x, x, b) / AffineCG: ConjuguedGradient2 with the matrix
+    product as the operator (b None: zero right-hand side).  Returns x and
+    {iters, g20, g2, nret, reps2}."""
+    n = rowptr.size - 1
+    x = np.zeros(n) if x0 is None else np.array(x0, np.float64)
+    bb = np.zeros(n) if b is None else np.ascontiguousarray(b, np.float64)
+    e = np.array([eps], np.float64)
+    st = np.zeros(4)
+    lib().ffo_cg2(n, rowptr, colidx, vals, bb, x, e, itmax, precon, st)
+    return x, dict(iters=int(st[0]), g20=st[1], g2=st[2], nret=int(st[3]), reps2=float(e[0]))
 
 
 def poisson_p1_3d(n, kind=COEF_SIN3, c0=1.0, kappa_e=None, labels=(1, 2, 3, 4, 5, 6),

@@ -127,3 +127,63 @@ int ffo_pcg(int n, const int64_t *rowptr, const int32_t *colidx,
   free(G); free(CG); free(H); free(d1); free(wcl);
   return nret;
 }
+
+/* LinearCG / AffineCG with a matrix operator: ConjuguedGradient2
+ * (MatriceCreuse.hpp:180-240).  The operator A(x) is the matrix product, so
+ * Ah = A x - b - g of the reference is rop * A h up to rounding; the
+ * restatement keeps the reference's affine form (recomputed from x). */
+int ffo_cg2(int n, const int64_t *rowptr, const int32_t *colidx,
+            const double *vals, const double *b, double *x, double *eps,
+            int itmax, int precon, double *stats) {
+  double *d1 = NULL;
+  if (precon) { /* the Jacobi func of a precon= argument */
+    d1 = (double *)malloc(sizeof(double) * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+      int f;
+      const double a = diag_of(rowptr, colidx, vals, i, &f);
+      d1[i] = (a * a < 1e-60) ? 1. : 1. / a;
+    }
+  }
+  double *g = (double *)malloc(sizeof(double) * (size_t)n);
+  double *h = (double *)malloc(sizeof(double) * (size_t)n);
+  double *Ah = (double *)malloc(sizeof(double) * (size_t)n);
+  double *Cg = Ah; /* "Rn ... Ah(n), & Cg(Ah)": the same array */
+  double ro = 1.;
+  ffo_spmv(n, rowptr, colidx, vals, x, g); /* g = A*x; g -= b */
+  if (b) for (int i = 0; i < n; ++i) g[i] -= b[i];
+  for (int i = 0; i < n; ++i) Cg[i] = d1 ? 0. + d1[i] * g[i] : 0. + g[i]; /* Cg = C*g */
+  for (int i = 0; i < n; ++i) h[i] = -Cg[i];
+  double g2 = dot(n, Cg, g);
+  const double g20 = g2;
+  int nret = 0, iters = 0;
+  if (g2 < 1e-30) {
+    nret = 2;
+  } else {
+    const double reps2 = *eps > 0 ? *eps * *eps * g2 : -*eps;
+    *eps = reps2;
+    iters = itmax + 1;
+    for (int iter = 0; iter <= itmax; iter++) {
+      const double rop = ro;
+      for (int i = 0; i < n; ++i) x[i] += rop * h[i];
+      ffo_spmv(n, rowptr, colidx, vals, x, Ah); /* Ah = A*x; Ah -= b; Ah -= g */
+      if (b) for (int i = 0; i < n; ++i) Ah[i] -= b[i];
+      for (int i = 0; i < n; ++i) Ah[i] -= g[i];
+      const double hAh = dot(n, h, Ah);
+      if (hAh * hAh < 1e-60) { nret = -1; iters = iter; break; }
+      ro = -dot(n, g, h) * rop / hAh;
+      if (ro != ro) { nret = -2; iters = iter; break; }
+      for (int i = 0; i < n; ++i) x[i] += (ro - rop) * h[i];
+      for (int i = 0; i < n; ++i) g[i] += (ro / rop) * Ah[i];
+      for (int i = 0; i < n; ++i) Cg[i] = d1 ? 0. + d1[i] * g[i] : 0. + g[i];
+      const double g2p = g2;
+      g2 = dot(n, Cg, g);
+      if (g2 < reps2) { nret = 1; iters = iter + 1; break; }
+      const double gamma = g2 / g2p;
+      for (int i = 0; i < n; ++i) h[i] *= gamma;
+      for (int i = 0; i < n; ++i) h[i] -= Cg[i];
+    }
+  }
+  if (stats) { stats[0] = iters; stats[1] = g20; stats[2] = g2; stats[3] = nret; }
+  free(g); free(h); free(Ah); free(d1);
+  return nret;
+}

@@ -22,7 +22,8 @@ def declared():
 def test_header_declares_the_path():
     names = declared()
     for must in ("ffhip_init", "ffhip_mesh3_cube", "ffhip_pattern_p1", "ffhip_assemble_p1",
-                 "ffhip_set_bc", "ffhip_rhs_p1", "ffhip_cg_create", "ffhip_cg_solve", "ffhip_spmv"):
+                 "ffhip_set_bc", "ffhip_rhs_p1", "ffhip_cg_create", "ffhip_cg_solve", "ffhip_spmv",
+                 "ffhip_cg_solve_affine"):
         assert must in names
 
 

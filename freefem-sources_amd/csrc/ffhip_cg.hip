@@ -75,6 +75,12 @@ struct CgState {
   int iters;
   int need_update;     // MIXED: reliable update requested
   int refinements;
+  // iteration of the first launch of the current chunk (the kernels add their
+  // position in the chunk; k_cg_advance moves it on), so one captured chunk
+  // serves every chunk of a solve
+  int it0;
+  int itmax;
+  double eps;
 };
 
 // ---------------------------------------------------------------- reductions
@@ -556,12 +562,14 @@ __global__ FFHIP_MATVEC_ATTR void k_pcg_init(MatFmt<double> M, const double *__r
 
 template <int KIND>
 __global__ FFHIP_MATVEC_ATTR void k_pcg_dir(
-    int it, int itmax, double eps, MatFmt<double> M, const double *__restrict__ Hold,
+    int c, MatFmt<double> M, const double *__restrict__ Hold,
     const double *__restrict__ CG, const double *__restrict__ G, double *__restrict__ Hnew,
     double *__restrict__ AH, const double *__restrict__ pin, double *__restrict__ pout,
     CgState *st) {
   __shared__ double lds[kBlock / kWave];
   if (st->done) return;
+  const int it = st->it0 + c, itmax = st->itmax;
+  const double eps = st->eps;
   const int P = gridDim.x;
   const double gcg = reduce_partials(pin, P, 1, 0, lds);
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
@@ -601,11 +609,11 @@ __global__ FFHIP_MATVEC_ATTR void k_pcg_dir(
         p_gh += G[r] * h;
       });
   const double a = block_sum<kBlock>(p_hah, lds);
-  const double c = block_sum<kBlock>(p_gh, lds);
+  const double gh = block_sum<kBlock>(p_gh, lds);
   if (threadIdx.x == 0) {
     const int lb = xcd_block_id();
     pout[2 * lb] = a;
-    pout[2 * lb + 1] = c;
+    pout[2 * lb + 1] = gh;
   }
 }
 
@@ -837,11 +845,12 @@ struct MixState {
 // since the last update, or the recurrence says the FP64 criterion is met.
 template <int KIND>
 __global__ FFHIP_MATVEC_ATTR void k_mix_dir(
-    int it, int itmax, MatFmt<float> M, const float *__restrict__ pold,
+    int c, MatFmt<float> M, const float *__restrict__ pold,
     const float *__restrict__ r32, float *__restrict__ pnew, float *__restrict__ q,
     const double *__restrict__ pin, double *__restrict__ pout, CgState *st, MixState *ms) {
   __shared__ double lds[kBlock / kWave];
   if (st->done || st->need_update) return;
+  const int it = st->it0 + c, itmax = st->itmax;
   const int P = gridDim.x;
   const double rho = reduce_partials(pin, P, 1, 0, lds);
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
@@ -880,11 +889,12 @@ __global__ FFHIP_MATVEC_ATTR void k_mix_dir(
 
 // K_m2: alpha = rho/(p,q); y32 += alpha p; r32 -= alpha q; partial (r32, r32)
 __global__ __launch_bounds__(kBlock) void k_mix_update(
-    int it, int64_t n, const float *__restrict__ p, const float *__restrict__ q,
+    int c, int64_t n, const float *__restrict__ p, const float *__restrict__ q,
     float *__restrict__ y32, float *__restrict__ r32, const double *__restrict__ pin,
     double *__restrict__ pout, CgState *st, MixState *ms) {
   __shared__ double lds[kBlock / kWave];
   if (st->done || st->need_update) return;
+  const int it = st->it0 + c;
   const int P = gridDim.x;
   const double pq = reduce_partials(pin, P, 1, 0, lds);
   const double alpha = ms->rho_prev[it & 1] / pq;
@@ -901,6 +911,11 @@ __global__ __launch_bounds__(kBlock) void k_mix_update(
   }
   const double tot = block_sum<kBlock>(part, lds);
   if (threadIdx.x == 0) pout[xcd_block_id()] = tot;
+}
+
+// end of a chunk of CG launches: the next chunk starts `chunk` iterations on
+__global__ void k_cg_advance(CgState *st, int chunk) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->it0 += chunk;
 }
 
 // fold the FP32 correction into the FP64 solution: x += D^-1/2 (sigma y32); y32 = 0
@@ -930,6 +945,37 @@ __global__ __launch_bounds__(kBlock) void k_mix_rescale(
 }
 
 static unsigned nblk(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// CG iterations launched per host poll of the device state; one chunk is
+// captured once into a hipGraph and replayed (even, so the two-buffer
+// parities of a chunk repeat)
+constexpr int kChunk = 16;
+
+// stream-captures the launches of f() into an executable graph
+template <class F>
+hipGraphExec_t capture_graph(hipStream_t s, F f) {
+  hipGraph_t g = nullptr;
+  hipGraphExec_t e = nullptr;
+  FFHIP_CHECK(hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed));
+  try {
+    f();
+  } catch (...) {  // the stream must leave capture mode whatever happened
+    if (hipStreamEndCapture(s, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    throw;
+  }
+  const hipError_t le = hipGetLastError();
+  const hipError_t ce = hipStreamEndCapture(s, &g);
+  if (le != hipSuccess || ce != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    FFHIP_CHECK(le);
+    FFHIP_CHECK(ce);
+  }
+  const hipError_t ie = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  FFHIP_CHECK(ie);
+  return e;
+}
 
 }  // namespace ffhip
 
@@ -967,6 +1013,25 @@ struct ffhip_cg_s {
   CgState *st = nullptr;
   MixState *ms = nullptr;
   Cg2State *st2 = nullptr;  // affine CG (LinearCG)
+  // captured chunks of CG launches, keyed by (mode, value array, parity of the
+  // chunk's first iteration); kernel by kernel when eager or profiling (event
+  // records captured in a graph carry no timestamps hipEventElapsedTime reads)
+  struct Chunk {
+    int kind;
+    const void *val;
+    int par;
+    hipGraphExec_t exec;
+  };
+  std::vector<Chunk> graphs;
+  int use_graph = 1;
+  template <class F>
+  hipGraphExec_t graph(int kind, const void *val, int par, F launch) {
+    if (!use_graph || profile) return nullptr;
+    for (const Chunk &c : graphs)
+      if (c.kind == kind && c.val == val && c.par == par) return c.exec;
+    graphs.push_back({kind, val, par, capture_graph(ctx->stream, launch)});
+    return graphs.back().exec;
+  }
   // per-kernel timing (measurement only): 3 events per iteration of a chunk
   int profile = 0;
   std::vector<hipEvent_t> ev;
@@ -1226,6 +1291,50 @@ static double mix_residual(ffhip_cg cg, const double *sval) {
   return r[2];
 }
 
+// One chunk of FP64 PCG iterations it0 + c, c < kChunk, it0 = 1 (mod kChunk):
+// the H buffer parity follows c.
+static void launch_pcg_chunk(ffhip_cg cg, const double *sval, bool marks) {
+  hipStream_t s = cg->ctx->stream;
+  const int G = cg->grid;
+  const MatFmt<double> M = cg->fmt_of(sval);
+  for (int c = 0; c < kChunk; ++c) {
+    if (marks) cg->mark(3 * c);
+    FFHIP_LAUNCH_FMT(cg, k_pcg_dir, G, s, c, M, cg->H[c & 1], cg->CG, cg->G, cg->H[(c + 1) & 1],
+                     cg->AH, cg->pA, cg->pB, cg->st);
+    if (marks) cg->mark(3 * c + 1);
+    k_pcg_update<<<G, kBlock, 0, s>>>(cg->n, cg->H[(c + 1) & 1], cg->AH, cg->dinv, cg->xw, cg->G,
+                                      cg->CG, cg->pB, cg->pA, cg->st);
+    if (marks) cg->mark(3 * c + 2);
+  }
+  k_cg_advance<<<1, 64, 0, s>>>(cg->st, kChunk);
+}
+
+// One chunk of MIXED inner iterations it0 + c, par = it0 & 1.
+static void launch_mix_chunk(ffhip_cg cg, const float *sval32, int par, bool marks) {
+  hipStream_t s = cg->ctx->stream;
+  const int G = cg->grid;
+  const MatFmt<float> M32 = cg->fmt_of(sval32);
+  for (int c = 0; c < kChunk; ++c) {
+    const float *pold = cg->p32[(par + c + 1) & 1];
+    float *pnew = cg->p32[(par + c) & 1];
+    if (marks) cg->mark(3 * c);
+    FFHIP_LAUNCH_FMT(cg, k_mix_dir, G, s, c, M32, pold, cg->r32, pnew, cg->q32, cg->pA, cg->pB,
+                     cg->st, cg->ms);
+    if (marks) cg->mark(3 * c + 1);
+    k_mix_update<<<G, kBlock, 0, s>>>(c, cg->n, pnew, cg->q32, cg->y32, cg->r32, cg->pB, cg->pA,
+                                      cg->st, cg->ms);
+    if (marks) cg->mark(3 * c + 2);
+  }
+  k_cg_advance<<<1, 64, 0, s>>>(cg->st, kChunk);
+}
+
+static hipGraphExec_t pcg_chunk_graph(ffhip_cg cg, const double *sval) {
+  return cg->graph(0, sval, 1, [&] { launch_pcg_chunk(cg, sval, false); });
+}
+static hipGraphExec_t mix_chunk_graph(ffhip_cg cg, const float *sval32, int par) {
+  return cg->graph(1, sval32, par, [&] { launch_mix_chunk(cg, sval32, par, false); });
+}
+
 // FP32 Jacobi-scaled CG with FP64 reliable updates (see the file comment).
 int ffhip_cg_solve_mixed(ffhip_cg cg, const double *sval, const float *sval32, double eps,
                          int itmax, CgState &hs) {
@@ -1233,7 +1342,8 @@ int ffhip_cg_solve_mixed(ffhip_cg cg, const double *sval, const float *sval32, d
   hipStream_t s = ctx->stream;
   const int64_t n = cg->n;
   const int G = cg->grid;
-  const MatFmt<float> M32 = cg->fmt_of(sval32);
+  const int chunk = kChunk;
+  hipGraphExec_t gx[2] = {mix_chunk_graph(cg, sval32, 0), mix_chunk_graph(cg, sval32, 1)};
   FFHIP_CHECK(hipMemsetAsync(cg->p32[0], 0, 4 * n, s));
   FFHIP_CHECK(hipMemsetAsync(cg->p32[1], 0, 4 * n, s));
   FFHIP_CHECK(hipMemsetAsync(cg->y32, 0, 4 * n, s));
@@ -1257,23 +1367,18 @@ int ffhip_cg_solve_mixed(ffhip_cg cg, const double *sval, const float *sval32, d
     hs.sigma = sigma;
     hs.need_update = 0;
     hs.iters = it - 1;
+    hs.it0 = it;
+    hs.itmax = itmax;
+    hs.eps = eps;
     FFHIP_CHECK(hipMemcpyAsync(cg->st, &hs, sizeof(hs), hipMemcpyHostToDevice, s));
     FFHIP_CHECK(hipMemcpyAsync(cg->ms, &hm, sizeof(hm), hipMemcpyHostToDevice, s));
-    const int chunk = 16;
     cg->ensure_events(3 * chunk);
+    const int par = it & 1;
     for (;;) {
       const int it0 = it;
-      for (int c = 0; c < chunk; ++c, ++it) {
-        const float *pold = cg->p32[(it - 1) & 1];
-        float *pnew = cg->p32[it & 1];
-        cg->mark(3 * c);
-        FFHIP_LAUNCH_FMT(cg, k_mix_dir, G, s, it, itmax, M32, pold, cg->r32, pnew, cg->q32, cg->pA,
-                         cg->pB, cg->st, cg->ms);
-        cg->mark(3 * c + 1);
-        k_mix_update<<<G, kBlock, 0, s>>>(it, n, pnew, cg->q32, cg->y32, cg->r32, cg->pB, cg->pA,
-                                          cg->st, cg->ms);
-        cg->mark(3 * c + 2);
-      }
+      if (gx[par]) FFHIP_CHECK(hipGraphLaunch(gx[par], s));
+      else launch_mix_chunk(cg, sval32, par, cg->profile != 0);
+      it += chunk;
       FFHIP_CHECK(hipGetLastError());
       FFHIP_CHECK(hipMemcpyAsync(&hs, cg->st, sizeof(hs), hipMemcpyDeviceToHost, s));
       FFHIP_CHECK(hipStreamSynchronize(s));
@@ -1308,6 +1413,7 @@ int ffhip_cg_solve_mixed(ffhip_cg cg, const double *sval, const float *sval32, d
 
 static void cg_release(ffhip_cg cg) {
   for (hipEvent_t e : cg->ev) (void)hipEventDestroy(e);
+  for (auto &g : cg->graphs) (void)hipGraphExecDestroy(g.exec);
   if (cg->fmt == kCsrV) cg->sval = nullptr;  // A's array
   dfree(cg->sptr); dfree(cg->slen); dfree(cg->scol); dfree(cg->scol16); dfree(cg->sptr16); dfree(cg->bcol); dfree(cg->bperm); dfree(cg->sval);
   dfree(cg->svalT);
@@ -1327,12 +1433,13 @@ int ffhip_cg_create(ffhip_ctx ctx, ffhip_mat A, int mode, ffhip_cg *out) {
     const int prec = mode & 0xf, layout = mode & 0x130;
     const bool index32 = (mode & FFHIP_CG_INDEX32) != 0;
     FFHIP_REQUIRE((prec == FFHIP_CG_FP64 || prec == FFHIP_CG_MIXED) &&
-                      (mode & ~(0x13f | FFHIP_CG_INDEX32)) == 0 &&
+                      (mode & ~(0x13f | FFHIP_CG_INDEX32 | FFHIP_CG_EAGER)) == 0 &&
                       (layout == 0 || layout == FFHIP_CG_LAYOUT_SELL || layout == FFHIP_CG_LAYOUT_CSR ||
                        layout == FFHIP_CG_LAYOUT_BSR3),
                   FFHIP_ERR_ARG, "ffhip_cg_create: mode %d", mode);
     ffhip_cg cg = new ffhip_cg_s();
     cg->ctx = ctx; cg->A = A; cg->mode = prec; cg->n = A->n;
+    cg->use_graph = (mode & FFHIP_CG_EAGER) ? 0 : 1;
     try {
       const int64_t n = A->n;
       cg->dinv = dalloc<double>(n);
@@ -1395,6 +1502,9 @@ int ffhip_cg_solve(ffhip_cg cg, const double *b, double *x, int on_device, doubl
     const double *sval = trans ? cg->svalT : cg->sval;
     const float *sval32 = trans ? cg->sval32T : cg->sval32;
     if (itmax <= 0) itmax = (int)n;  // SolverCG(ds): itmax > 0 ? itmax : n
+    // captured before this solve's first launch (kept for later solves)
+    if (cg->mode == FFHIP_CG_FP64) (void)pcg_chunk_graph(cg, sval);
+    else { (void)mix_chunk_graph(cg, sval32, 0); (void)mix_chunk_graph(cg, sval32, 1); }
     FFHIP_CHECK(hipMemcpyAsync(cg->bw, b, 8 * n, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
     FFHIP_CHECK(hipMemcpyAsync(cg->xw, x, 8 * n, on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
     ctx->tic();
@@ -1406,29 +1516,24 @@ int ffhip_cg_solve(ffhip_cg cg, const double *b, double *x, int on_device, doubl
     if (cg->mode == FFHIP_CG_FP64) {
       const int G = cg->grid;
       const MatFmt<double> M = cg->fmt_of(sval);
+      const int chunk = kChunk;
+      hipGraphExec_t gx = pcg_chunk_graph(cg, sval);
+      hs.it0 = 1;
+      hs.itmax = itmax;
+      hs.eps = eps;
+      FFHIP_CHECK(hipMemcpyAsync(cg->st, &hs, sizeof(hs), hipMemcpyHostToDevice, s));
       FFHIP_LAUNCH_FMT(cg, k_pcg_init, G, s, M, cg->xw, cg->bw, cg->dinv, cg->G, cg->CG, cg->H[0],
                        cg->pA, cg->st);
       FFHIP_CHECK(hipGetLastError());
-      const int chunk = 16;
       cg->ensure_events(3 * chunk);
-      for (int it = 1;; ++it) {
-        const double *Hold = cg->H[(it - 1) & 1];
-        double *Hnew = cg->H[it & 1];
-        const int c = (it - 1) % chunk;
-        cg->mark(3 * c);
-        FFHIP_LAUNCH_FMT(cg, k_pcg_dir, G, s, it, itmax, eps, M, Hold, cg->CG, cg->G, Hnew, cg->AH,
-                         cg->pA, cg->pB, cg->st);
-        cg->mark(3 * c + 1);
-        k_pcg_update<<<G, kBlock, 0, s>>>(n, Hnew, cg->AH, cg->dinv, cg->xw, cg->G, cg->CG, cg->pB,
-                                          cg->pA, cg->st);
-        cg->mark(3 * c + 2);
-        if (it % chunk == 0) {
-          FFHIP_CHECK(hipGetLastError());
-          FFHIP_CHECK(hipMemcpyAsync(&hs, cg->st, sizeof(hs), hipMemcpyDeviceToHost, s));
-          FFHIP_CHECK(hipStreamSynchronize(s));
-          cg->collect(it - chunk + 1, chunk, hs.done ? hs.iters : it);
-          if (hs.done) break;
-        }
+      for (int it = 1;; it += chunk) {
+        if (gx) FFHIP_CHECK(hipGraphLaunch(gx, s));
+        else launch_pcg_chunk(cg, sval, cg->profile != 0);
+        FFHIP_CHECK(hipGetLastError());
+        FFHIP_CHECK(hipMemcpyAsync(&hs, cg->st, sizeof(hs), hipMemcpyDeviceToHost, s));
+        FFHIP_CHECK(hipStreamSynchronize(s));
+        cg->collect(it, chunk, hs.done ? hs.iters : it + chunk - 1);
+        if (hs.done) break;
       }
     } else {
       ffhip_cg_solve_mixed(cg, sval, sval32, eps, itmax, hs);

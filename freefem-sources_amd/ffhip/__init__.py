@@ -27,6 +27,7 @@ COEF_CONST, COEF_SIN3, COEF_P0, COEF_QP = 0, 1, 2, 3
 CG_FP64, CG_MIXED = 0, 1
 CG_LAYOUT_AUTO, CG_LAYOUT_SELL, CG_LAYOUT_CSR, CG_LAYOUT_BSR3 = 0, 0x10, 0x20, 0x100
 CG_INDEX16, CG_INDEX32 = 0x40, 0x80
+CG_EAGER = 0x200
 CG2_IDENTITY, CG2_JACOBI = 0, 1
 ERRORS = {-1: "bad argument", -2: "HIP error", -3: "CG did not converge", -4: "NaN in CG",
           -5: "unsupported", -6: "internal error"}
@@ -412,17 +413,21 @@ def rhs_p2v(space, f=(0.0, 0.0, -1.0), labels=(), tgv=1e30, g=None, out=None):
 
 
 class CG:
-    """Jacobi-CG solver of a device matrix (SolverCG).  layout: "auto", "sell"
+    """Jacobi-CG solver of a device matrix (SolverCG).  graph=False launches
+    the iteration kernels one by one instead of replaying a captured chunk of
+    16 iterations.  layout: "auto", "sell"
     (sliced ELL, 16-bit column deltas where they fit), "sell32" (sliced ELL,
     32-bit columns), "csr" (16 lanes per row) or "bsr3" (3x3 blocks of a
     [P2,P2,P2] matrix); auto takes bsr3 for [P2,P2,P2] matrices, else sliced
     ELL unless it pads by more than 25 %."""
 
-    def __init__(self, A, mode=CG_FP64, layout="auto"):
+    def __init__(self, A, mode=CG_FP64, layout="auto", graph=True):
         h = ctypes.c_void_p()
         flag = {"auto": CG_LAYOUT_AUTO, "sell": CG_LAYOUT_SELL,
                 "sell32": CG_LAYOUT_SELL | CG_INDEX32, "csr": CG_LAYOUT_CSR,
                 "bsr3": CG_LAYOUT_BSR3}[layout]
+        if not graph:
+            flag |= CG_EAGER
         check(lib().ffhip_cg_create(A.ctx.h, A.h, mode | flag, ctypes.byref(h)))
         self.h, self.A, self.mode = h, A, mode
 

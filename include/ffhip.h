@@ -189,6 +189,11 @@ enum { FFHIP_CG_LAYOUT_AUTO = 0, FFHIP_CG_LAYOUT_SELL = 0x10, FFHIP_CG_LAYOUT_CS
  * 32-bit columns.  ffhip_cg_layout reports FFHIP_CG_INDEX16 with the layout
  * when the deltas are used. */
 enum { FFHIP_CG_INDEX16 = 0x40, FFHIP_CG_INDEX32 = 0x80 };
+/* The solve loops replay one captured chunk of 16 iterations (a hipGraph per
+ * solver, value array and parity, captured on first use); FFHIP_CG_EAGER in
+ * mode launches the kernels one by one instead (as does per-kernel
+ * profiling). */
+enum { FFHIP_CG_EAGER = 0x200 };
 typedef struct {
   int iters;          /* CG iterations (inner FP32 iterations for MIXED)   */
   int converged;      /* ConjugueGradient's nret: 1 converged, 2 gCg0<1e-30, 0 no */

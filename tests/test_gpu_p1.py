@@ -398,3 +398,32 @@ def test_high_degree_vertex_pattern_and_values():
     assert np.array_equal(r2, rp) and np.array_equal(c2, ci)
     assert np.abs(v2 - vo).max() <= 1e-13 * np.abs(vo).max()
     assert rp[1] - rp[0] == m["nv"]  # the centre couples to every vertex
+
+
+@pytest.mark.parametrize("layout", ["sell", "csr"])
+@pytest.mark.parametrize("mode", [0, 1])
+def test_graph_replay_bitwise_equals_eager(mode, layout):
+    # the solve loops replay one captured chunk of 16 iterations, with the
+    # host's polls (and MIXED's restarts, which alternate the chunk parity)
+    # between replays; later solves replay it again.  The same kernels in the
+    # same order with order-fixed reductions: the bits must equal the
+    # kernel-by-kernel launches'.
+    m, rpo, cio, vo, onbco, bo = ffo.poisson_p1_3d(8)
+    A = ffhip.Matrix.upload(rpo, cio, vo)
+    cg_g = ffhip.CG(A, mode, layout=layout)
+    cg_e = ffhip.CG(A, mode, layout=layout, graph=False)
+    rng = np.random.default_rng(7)
+    for k in range(3):
+        b = bo * (1 + k) + (k > 0) * rng.standard_normal(bo.size) * (onbco == 0)
+        for trans in (False, True):
+            xg, sg = cg_g.solve(b, eps=1e-10, trans=trans)
+            xe, se = cg_e.solve(b, eps=1e-10, trans=trans)
+            assert sg["converged"] == 1 and sg["iters"] > 16
+            assert np.array_equal(xg, xe)
+            assert sg["iters"] == se["iters"] and sg["refinements"] == se["refinements"]
+    # profiling launches kernel by kernel with timing events: same bits
+    cg_g.set_profiling(True)
+    xp, sp = cg_g.solve(bo, eps=1e-10)
+    xe, se = cg_e.solve(bo, eps=1e-10)
+    assert np.array_equal(xp, xe)
+    assert abs(sp["timed"] - sp["iters"]) <= 1 and sp["ms_matvec_avg"] > 0 and sp["ms_update_avg"] > 0

@@ -107,6 +107,19 @@ def test_p2_layout_is_blocks_for_vector_p2():
     assert np.allclose(yd.cpu().numpy(), y, rtol=1e-13, atol=1e-13 * np.abs(y).max())
 
 
+def test_p2_graph_replay_bitwise_equals_eager():
+    m = ffo.cube(4)
+    sp = ffhip.P2VSpace(_gpu_mesh(m))
+    A = ffhip.Matrix.p2v(sp)
+    A.assemble_p2v_elasticity()
+    A.set_bc_p2v([4])
+    b = ffhip.rhs_p2v(sp, (0.0, 0.0, -1.0), [4])
+    for mode in (ffhip.CG_FP64, ffhip.CG_MIXED):
+        xg, sg = ffhip.CG(A, mode, layout="bsr3").solve(b, eps=1e-10)
+        xe, se = ffhip.CG(A, mode, layout="bsr3", graph=False).solve(b, eps=1e-10)
+        assert sg["converged"] == 1 and np.array_equal(xg, xe) and sg["iters"] == se["iters"]
+
+
 def test_p2_cube32_configuration():
     """configuration: cube(32), [P2,P2,P2], lambda=1, mu=0.5, f=(0,0,-1),
     clamped label 4, FP32 inner CG with FP64 refinement"""

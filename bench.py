@@ -17,11 +17,12 @@ The mesh is built on the device before the timed region (inputs resident in
 HBM).  value = assembled Mnnz/s (nnz / assembly time); the CG times, the
 iteration counts and the accuracies are reported beside it.
 
-roofline: the dominant kernel is the FP64 CG mat-vec (k_pcg_dir: SpMV fused
-with the direction update and two dot products), timed per launch with HIP
-events on the library stream during the timed solves.  Algorithmic bytes per
-launch = 12 nnz (FP64 value + int32 column) + 40 n (H_prev, C g, g read once,
-H and A H written once).
+roofline: the dominant kernel is the FP64 CG mat-vec (k_pcg_matvec: SpMV of
+the direction H with the two dot products (H, AH) and (g, H)), timed per
+launch with HIP events on the library stream during the profiled solves of
+the timed steps.  Algorithmic bytes per launch = (8 + column bytes) nnz (FP64
+value + 16-bit column delta in the sliced ELL layout, 32-bit otherwise) +
+24 n (H gathered once, g read once, A H written once).
 
 cpu_baseline: FreeFEM's own compiled femlib code (oracle/_ref: HashMatrix
 accumulation + SetBC + the reference ConjugueGradient with its COO mat-vec)
@@ -43,11 +44,11 @@ LABELS = (1, 2, 3, 4, 5, 6)
 HBM_PEAK_GBS = 8000.0
 
 
-def measured_traffic(kernel="k_pcg_dir", layout="sell16"):
+def measured_traffic(kernel="k_pcg_matvec", layout="sell16"):
     """HBM bytes per launch from the committed PMC summary (FETCH_SIZE and
     WRITE_SIZE scaled by a copy kernel of known size) for the layout the
     solver streamed, or None."""
-    path = os.path.join(ROOT, "profiles", "k_pcg_dir_cube160.md")
+    path = os.path.join(ROOT, "profiles", "k_pcg_matvec_cube160.md")
     try:
         for line in open(path):
             if line.startswith(f"traffic_bytes_per_launch {kernel} {layout}:"):
@@ -244,12 +245,12 @@ def main():
     mv_ms = float(np.mean([r["p64"]["ms_matvec_avg"] for r in res]))
     mv_ms32 = float(np.mean([r["pm"]["ms_matvec_avg"] for r in res]))
     # bytes a launch must move in the streamed layout: value + column index per
-    # nonzero (16-bit column deltas in "sell16"), 5 FP64 (FP32: 4 FP32) vectors
+    # nonzero (16-bit column deltas in "sell16"), 3 FP64 (FP32: 2 FP32) vectors
     ib64 = 2 if res[-1]["layout64"][0] == "sell16" else 4
     ib32 = 2 if res[-1]["layoutm"][0] == "sell16" else 4
-    bytes_mv = (8 + ib64) * nnz + 40 * nv
-    bytes_mv32 = (4 + ib32) * nnz + 16 * nv
-    csr_bytes = 12 * nnz + 40 * nv  # the same work with int32 CSR columns
+    bytes_mv = (8 + ib64) * nnz + 24 * nv
+    bytes_mv32 = (4 + ib32) * nnz + 8 * nv
+    csr_bytes = 12 * nnz + 24 * nv  # the same work with int32 CSR columns
     achieved = bytes_mv / (mv_ms * 1e-3) / 1e9
     s64, sm = res[-1]["s64"], res[-1]["sm"]
     copy_bytes = 1 << 30
@@ -281,9 +282,11 @@ def main():
         "cg_mixed_s": round(mean("t_m"), 5),
         "cg_launch": "hipGraph replay of captured 16-iteration chunks (first solve captures)",
         "cg_kernel_us_profiled": {
-            "k_pcg_dir": round(mv_ms * 1e3, 1),
+            "k_pcg_dirvec": round(float(np.mean([r["p64"]["ms_direction_avg"] for r in res])) * 1e3, 1),
+            "k_pcg_matvec": round(mv_ms * 1e3, 1),
             "k_pcg_update": round(float(np.mean([r["p64"]["ms_update_avg"] for r in res])) * 1e3, 1),
-            "k_mix_dir": round(mv_ms32 * 1e3, 1),
+            "k_mix_dirvec": round(float(np.mean([r["pm"]["ms_direction_avg"] for r in res])) * 1e3, 1),
+            "k_mix_matvec": round(mv_ms32 * 1e3, 1),
             "k_mix_update": round(float(np.mean([r["pm"]["ms_update_avg"] for r in res])) * 1e3, 1)},
         "cg_mixed_setup_s": round(mean("t_msetup"), 5),
         "mixed_speedup_vs_fp64": round(mean("t_64") / mean("t_m"), 3),
@@ -320,13 +323,13 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic("k_pcg_dir", res[-1]["layout64"][0]) if n == 160 else None,
-            "kernel": f"k_pcg_dir (FP64 {res[-1]['layout64'][0]} mat-vec + direction update + 2 dots)",
+            "traffic": measured_traffic("k_pcg_matvec", res[-1]["layout64"][0]) if n == 160 else None,
+            "kernel": f"k_pcg_matvec (FP64 {res[-1]['layout64'][0]} mat-vec of the CG direction + 2 dots)",
             "bytes_per_launch": bytes_mv,
             "avg_launch_ms": round(mv_ms, 5),
             "csr_int32_equivalent_GBs": round(csr_bytes / (mv_ms * 1e-3) / 1e9, 1),
             "copy_reference_GBs": round(copy_gbs, 1),
-            "fp32_matvec": {"kernel": "k_mix_dir", "bytes_per_launch": bytes_mv32,
+            "fp32_matvec": {"kernel": "k_mix_matvec", "bytes_per_launch": bytes_mv32,
                             "avg_launch_ms": round(mv_ms32, 5),
                             "achieved_GBs": round(bytes_mv32 / (mv_ms32 * 1e-3) / 1e9, 1)},
         },

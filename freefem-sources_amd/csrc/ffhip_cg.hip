@@ -170,12 +170,61 @@ __device__ inline int64_t uniform(int64_t v) {
 }
 __device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane(threadIdx.x / kSlice); }
 
-// logical slice range of this block
-__device__ inline void block_slices(int64_t S, int64_t &s0, int64_t &s1) {
-  const int64_t G = gridDim.x, lb = xcd_block_id();
-  s0 = (lb * S) / G;
-  s1 = ((lb + 1) * S) / G;
+// Slices of this wave in the streaming mat-vec kernels.  CYCLIC: each XCD's
+// contiguous slice range is walked by all of its waves together, wave q taking
+// slices q, q + Q, q + 2Q, ... (Q waves on the XCD), so the XCD streams one
+// contiguous window of Q slices at a time and a column's x value, needed by
+// rows up to one window-width apart (the plane neighbours of a 3D mesh), is
+// re-read while still in that XCD's L2 (cube(160) P1: 8 % less HBM fetch, the
+// FP32 mat-vec 6 % faster).  Otherwise each block walks its own contiguous
+// chunk of slices — kept for the [P2,P2,P2] block layout, whose node rows are
+// sorted by length in windows and ran 1.8x slower with the cyclic walk.
+#ifndef FFHIP_SCHED_SELL
+#define FFHIP_SCHED_SELL 1
+#endif
+#ifndef FFHIP_SCHED_BSR
+#define FFHIP_SCHED_BSR 0
+#endif
+template <bool CYCLIC>
+__device__ inline void wave_slices(int64_t S, int64_t &first, int64_t &end, int64_t &stride) {
+  if (CYCLIC) {
+    const int G = gridDim.x, b = blockIdx.x;
+    const int groups = (G % 8) ? 1 : 8;
+    const int grp = b % groups, loc = b / groups, nloc = G / groups;
+    first = (grp * S) / groups + (int64_t)loc * kWavesPerBlock + wave_id();
+    end = ((grp + 1) * S) / groups;
+    stride = (int64_t)nloc * kWavesPerBlock;
+  } else {
+    const int64_t G = gridDim.x, lb = xcd_block_id();
+    first = (lb * S) / G + wave_id();
+    end = ((lb + 1) * S) / G;
+    stride = kWavesPerBlock;
+  }
 }
+
+// epilogue streams used once by a kernel (FFHIP_NTEPI 1: non-temporal, so they
+// do not displace the gathered vectors from L2)
+#ifndef FFHIP_NTEPI
+#define FFHIP_NTEPI 0
+#endif
+
+template <class T>
+__device__ inline T epi_load(const T *p) {
+#if FFHIP_NTEPI
+  return __builtin_nontemporal_load(p);
+#else
+  return *p;
+#endif
+}
+template <class T>
+__device__ inline void epi_store(T v, T *p) {
+#if FFHIP_NTEPI
+  __builtin_nontemporal_store(v, p);
+#else
+  *p = v;
+#endif
+}
+
 
 // ---------------------------------------------------------------- SELL build
 __global__ void k_sell_len(int64_t n, int64_t S, const int64_t *__restrict__ rowptr,
@@ -441,10 +490,10 @@ struct MatFmt {
 template <int KIND, class V, class ACC, class F, class E>
 __device__ inline void matvec_rows(const MatFmt<V> &M, F xval, E epi) {
   if (KIND == kSell || KIND == kSell16) {
-    int64_t s0, s1;
-    block_slices(M.S, s0, s1);
-    const int w = wave_id(), lane = threadIdx.x % kSlice;
-    for (int64_t s = s0 + w; s < s1; s += kWavesPerBlock) {
+    int64_t s0, s1, ds;
+    wave_slices<FFHIP_SCHED_SELL != 0>(M.S, s0, s1, ds);
+    const int lane = threadIdx.x % kSlice;
+    for (int64_t s = s0; s < s1; s += ds) {
       const int64_t base = uniform(M.ptr[s]) + lane;
       const int L = __builtin_amdgcn_readfirstlane(M.len[s]);
       const int64_t r = s * kSlice + lane;
@@ -461,10 +510,10 @@ __device__ inline void matvec_rows(const MatFmt<V> &M, F xval, E epi) {
   } else if (KIND == kBsr3) {
     constexpr int U = 2;
     const int64_t S3 = (M.nb + kSlice - 1) / kSlice;
-    int64_t s0, s1;
-    block_slices(S3, s0, s1);
-    const int w = wave_id(), lane = threadIdx.x % kSlice;
-    for (int64_t s = s0 + w; s < s1; s += kWavesPerBlock) {
+    int64_t s0, s1, ds;
+    wave_slices<FFHIP_SCHED_BSR != 0>(S3, s0, s1, ds);
+    const int lane = threadIdx.x % kSlice;
+    for (int64_t s = s0; s < s1; s += ds) {
       const int64_t i = s * kSlice + lane;
       const int32_t a = i < M.nb ? M.perm[i] : -1;
       const int64_t base = uniform(M.ptr[s]) + lane;
@@ -560,53 +609,80 @@ __global__ FFHIP_MATVEC_ATTR void k_pcg_init(MatFmt<double> M, const double *__r
   if (blockIdx.x == 0 && threadIdx.x == 0) { st->done = 0; st->iters = 0; }
 }
 
-template <int KIND>
-__global__ FFHIP_MATVEC_ATTR void k_pcg_dir(
-    int c, MatFmt<double> M, const double *__restrict__ Hold,
-    const double *__restrict__ CG, const double *__restrict__ G, double *__restrict__ Hnew,
-    double *__restrict__ AH, const double *__restrict__ pin, double *__restrict__ pout,
-    CgState *st) {
-  __shared__ double lds[kBlock / kWave];
-  if (st->done) return;
+// Start of FP64 PCG iteration it = it0 + c: (g, Cg) from the last update's
+// partials, the stop tests of ConjugueGradient (CG.cpp:226-252) and gamma.
+// Returns false when the solve stops (the lead thread records why).
+__device__ inline bool pcg_begin(int c, const double *__restrict__ pin, CgState *st, double *lds,
+                                 double &gamma) {
   const int it = st->it0 + c, itmax = st->itmax;
   const double eps = st->eps;
-  const int P = gridDim.x;
-  const double gcg = reduce_partials(pin, P, 1, 0, lds);
+  const double gcg = reduce_partials(pin, gridDim.x, 1, 0, lds);
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
-  double gamma = 0.;
+  gamma = 0.;
+  int done = 0;
   if (it == 1) {
     const double eps2 = eps > 0 ? eps * eps * gcg : eps * eps;
-    int done = 0;
     if (gcg != gcg) done = 4;
     else if (gcg < 1e-30) done = 2;
     if (lead) {
       st->gcg0 = gcg; st->eps2 = eps2; st->gcg_prev[1] = gcg;
       if (done) { st->done = done; st->iters = 0; }
     }
-    if (done) return;
   } else {
     const double gcgp = st->gcg_prev[(it - 1) & 1];
-    const double eps2 = st->eps2;
     gamma = gcg / gcgp;
-    int done = 0;
     if (gcg != gcg) done = 4;
-    else if (gcg < eps2) done = 1;
+    else if (gcg < st->eps2) done = 1;
     else if (it - 1 >= itmax) done = 3;
     if (lead) {
       st->gcg_prev[it & 1] = gcg;
       if (done) { st->done = done; st->iters = it - 1; }
     }
-    if (done) return;
   }
+  return done == 0;
+}
+
+// this block's row range for the row-wise vector kernels
+__device__ inline void block_rows(int64_t n, int64_t &r0, int64_t &r1) {
+  const int64_t G = gridDim.x, lb = xcd_block_id();
+  r0 = (lb * n) / G;
+  r1 = ((lb + 1) * n) / G;
+}
+
+// K1: the direction H = gamma H_prev - Cg, written by an elementwise pass so
+// that the mat-vec gathers one vector (forming H on the fly from two gathered
+// vectors per column made the cube(160) mat-vec 17 % slower and the
+// [P2,P2,P2] block mat-vec 50 % slower).
+__global__ __launch_bounds__(kBlock) void k_pcg_dirvec(int c, int64_t n,
+                                                       const double *__restrict__ Hold,
+                                                       const double *__restrict__ CG,
+                                                       double *__restrict__ Hnew,
+                                                       const double *__restrict__ pin, CgState *st) {
+  __shared__ double lds[kBlock / kWave];
+  if (st->done) return;
+  double gamma;
+  if (!pcg_begin(c, pin, st, lds, gamma)) return;
+  int64_t r0, r1;
+  block_rows(n, r0, r1);
+  for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) Hnew[r] = gamma * Hold[r] - CG[r];
+}
+
+// K2: AH = A H, partials (H, AH), (g, H)
+template <int KIND>
+__global__ FFHIP_MATVEC_ATTR void k_pcg_matvec(MatFmt<double> M, const double *__restrict__ H,
+                                               const double *__restrict__ G,
+                                               double *__restrict__ AH, double *__restrict__ pout,
+                                               const CgState *st) {
+  __shared__ double lds[kBlock / kWave];
+  if (st->done) return;
   double p_hah = 0., p_gh = 0.;
   matvec_rows<KIND, double, double>(
-      M, [&](int32_t j) { return gamma * Hold[j] - CG[j]; },
+      M, [&](int32_t j) { return H[j]; },
       [&](int64_t r, double acc) {
-        const double h = gamma * Hold[r] - CG[r];
-        Hnew[r] = h;
-        AH[r] = acc;
+        const double h = H[r];
+        epi_store(acc, AH + r);
         p_hah += h * acc;
-        p_gh += G[r] * h;
+        p_gh += epi_load(G + r) * h;
       });
   const double a = block_sum<kBlock>(p_hah, lds);
   const double gh = block_sum<kBlock>(p_gh, lds);
@@ -615,13 +691,6 @@ __global__ FFHIP_MATVEC_ATTR void k_pcg_dir(
     pout[2 * lb] = a;
     pout[2 * lb + 1] = gh;
   }
-}
-
-// this block's row range for the row-wise vector kernels
-__device__ inline void block_rows(int64_t n, int64_t &r0, int64_t &r1) {
-  const int64_t G = gridDim.x, lb = xcd_block_id();
-  r0 = (lb * n) / G;
-  r1 = ((lb + 1) * n) / G;
 }
 
 __global__ __launch_bounds__(kBlock) void k_pcg_update(
@@ -843,16 +912,13 @@ struct MixState {
 // partial (p, q).  Requests a reliable update when (r32, r32) fell by
 // kMixUpdate
 // since the last update, or the recurrence says the FP64 criterion is met.
-template <int KIND>
-__global__ FFHIP_MATVEC_ATTR void k_mix_dir(
-    int c, MatFmt<float> M, const float *__restrict__ pold,
-    const float *__restrict__ r32, float *__restrict__ pnew, float *__restrict__ q,
-    const double *__restrict__ pin, double *__restrict__ pout, CgState *st, MixState *ms) {
-  __shared__ double lds[kBlock / kWave];
-  if (st->done || st->need_update) return;
+// Start of inner iteration it = it0 + c: rho = (r32, r32) from the last
+// update's partials, the requests (reliable update, itmax, NaN) and the
+// coefficient of p_old.  Returns false when the chunk stops.
+__device__ inline bool mix_begin(int c, const double *__restrict__ pin, CgState *st, MixState *ms,
+                                 double *lds, float &cbf) {
   const int it = st->it0 + c, itmax = st->itmax;
-  const int P = gridDim.x;
-  const double rho = reduce_partials(pin, P, 1, 0, lds);
+  const double rho = reduce_partials(pin, gridDim.x, 1, 0, lds);
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
   const double sig = st->sigma;
   int req = 0;
@@ -869,25 +935,50 @@ __global__ FFHIP_MATVEC_ATTR void k_mix_dir(
       st->rho32 = ms->rho_prev[(it - 1) & 1];
       st->iters = it - 1;
     }
-    return;
+    return false;
   }
   const double cb = (it == ms->first_it) ? ms->cbeta : rho / ms->rho_prev[(it - 1) & 1];
   if (lead) { ms->rho_prev[it & 1] = rho; st->iters = it - 1; }
-  const float cbf = (float)cb;
+  cbf = (float)cb;
+  return true;
+}
+
+// K_m1: p = r32 + cb p_old (elementwise); K_m2: q = Ahat p, partial (p, q).
+// Requests a reliable update when (r32, r32) fell by kMixUpdate since the last
+// update, or the recurrence says the FP64 criterion is met.
+__global__ __launch_bounds__(kBlock) void k_mix_dirvec(int c, int64_t n,
+                                                       const float *__restrict__ pold,
+                                                       const float *__restrict__ r32,
+                                                       float *__restrict__ pnew,
+                                                       const double *__restrict__ pin, CgState *st,
+                                                       MixState *ms) {
+  __shared__ double lds[kBlock / kWave];
+  if (st->done || st->need_update) return;
+  float cbf;
+  if (!mix_begin(c, pin, st, ms, lds, cbf)) return;
+  int64_t r0, r1;
+  block_rows(n, r0, r1);
+  for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) pnew[r] = r32[r] + cbf * pold[r];
+}
+
+template <int KIND>
+__global__ FFHIP_MATVEC_ATTR void k_mix_matvec(MatFmt<float> M, const float *__restrict__ p,
+                                               float *__restrict__ q, double *__restrict__ pout,
+                                               const CgState *st) {
+  __shared__ double lds[kBlock / kWave];
+  if (st->done || st->need_update) return;
   double part = 0.;
   matvec_rows<KIND, float, float>(
-      M, [&](int32_t j) { return r32[j] + cbf * pold[j]; },
+      M, [&](int32_t j) { return p[j]; },
       [&](int64_t r, float acc) {
-        const float p = r32[r] + cbf * pold[r];
-        pnew[r] = p;
-        q[r] = acc;
-        part += (double)p * (double)acc;
+        epi_store(acc, q + r);
+        part += (double)p[r] * (double)acc;
       });
   const double tot = block_sum<kBlock>(part, lds);
   if (threadIdx.x == 0) pout[xcd_block_id()] = tot;
 }
 
-// K_m2: alpha = rho/(p,q); y32 += alpha p; r32 -= alpha q; partial (r32, r32)
+// K_m3: alpha = rho/(p,q); y32 += alpha p; r32 -= alpha q; partial (r32, r32)
 __global__ __launch_bounds__(kBlock) void k_mix_update(
     int c, int64_t n, const float *__restrict__ p, const float *__restrict__ q,
     float *__restrict__ y32, float *__restrict__ r32, const double *__restrict__ pin,
@@ -1032,10 +1123,11 @@ struct ffhip_cg_s {
     graphs.push_back({kind, val, par, capture_graph(ctx->stream, launch)});
     return graphs.back().exec;
   }
-  // per-kernel timing (measurement only): 3 events per iteration of a chunk
+  // per-kernel timing (measurement only): 4 events per iteration of a chunk,
+  // around the direction, mat-vec and update kernels
   int profile = 0;
   std::vector<hipEvent_t> ev;
-  double t_mv = 0., t_up = 0.;
+  double t_dv = 0., t_mv = 0., t_up = 0.;
   int n_timed = 0;
   void ensure_events(int k) {
     while ((int)ev.size() < k) {
@@ -1053,10 +1145,11 @@ struct ffhip_cg_s {
     for (int c = 0; c < nits; ++c) {
       const int it = chunk_first_it + c;
       if (it > last_exec_it) break;
-      float a = 0.f, b = 0.f;
-      FFHIP_CHECK(hipEventElapsedTime(&a, ev[3 * c], ev[3 * c + 1]));
-      FFHIP_CHECK(hipEventElapsedTime(&b, ev[3 * c + 1], ev[3 * c + 2]));
-      t_mv += a; t_up += b; ++n_timed;
+      float d = 0.f, a = 0.f, b = 0.f;
+      FFHIP_CHECK(hipEventElapsedTime(&d, ev[4 * c], ev[4 * c + 1]));
+      FFHIP_CHECK(hipEventElapsedTime(&a, ev[4 * c + 1], ev[4 * c + 2]));
+      FFHIP_CHECK(hipEventElapsedTime(&b, ev[4 * c + 2], ev[4 * c + 3]));
+      t_dv += d; t_mv += a; t_up += b; ++n_timed;
     }
   }
   template <class V>
@@ -1298,13 +1391,15 @@ static void launch_pcg_chunk(ffhip_cg cg, const double *sval, bool marks) {
   const int G = cg->grid;
   const MatFmt<double> M = cg->fmt_of(sval);
   for (int c = 0; c < kChunk; ++c) {
-    if (marks) cg->mark(3 * c);
-    FFHIP_LAUNCH_FMT(cg, k_pcg_dir, G, s, c, M, cg->H[c & 1], cg->CG, cg->G, cg->H[(c + 1) & 1],
-                     cg->AH, cg->pA, cg->pB, cg->st);
-    if (marks) cg->mark(3 * c + 1);
+    if (marks) cg->mark(4 * c);
+    k_pcg_dirvec<<<G, kBlock, 0, s>>>(c, cg->n, cg->H[c & 1], cg->CG, cg->H[(c + 1) & 1], cg->pA,
+                                      cg->st);
+    if (marks) cg->mark(4 * c + 1);
+    FFHIP_LAUNCH_FMT(cg, k_pcg_matvec, G, s, M, cg->H[(c + 1) & 1], cg->G, cg->AH, cg->pB, cg->st);
+    if (marks) cg->mark(4 * c + 2);
     k_pcg_update<<<G, kBlock, 0, s>>>(cg->n, cg->H[(c + 1) & 1], cg->AH, cg->dinv, cg->xw, cg->G,
                                       cg->CG, cg->pB, cg->pA, cg->st);
-    if (marks) cg->mark(3 * c + 2);
+    if (marks) cg->mark(4 * c + 3);
   }
   k_cg_advance<<<1, 64, 0, s>>>(cg->st, kChunk);
 }
@@ -1317,13 +1412,14 @@ static void launch_mix_chunk(ffhip_cg cg, const float *sval32, int par, bool mar
   for (int c = 0; c < kChunk; ++c) {
     const float *pold = cg->p32[(par + c + 1) & 1];
     float *pnew = cg->p32[(par + c) & 1];
-    if (marks) cg->mark(3 * c);
-    FFHIP_LAUNCH_FMT(cg, k_mix_dir, G, s, c, M32, pold, cg->r32, pnew, cg->q32, cg->pA, cg->pB,
-                     cg->st, cg->ms);
-    if (marks) cg->mark(3 * c + 1);
+    if (marks) cg->mark(4 * c);
+    k_mix_dirvec<<<G, kBlock, 0, s>>>(c, cg->n, pold, cg->r32, pnew, cg->pA, cg->st, cg->ms);
+    if (marks) cg->mark(4 * c + 1);
+    FFHIP_LAUNCH_FMT(cg, k_mix_matvec, G, s, M32, pnew, cg->q32, cg->pB, cg->st);
+    if (marks) cg->mark(4 * c + 2);
     k_mix_update<<<G, kBlock, 0, s>>>(c, cg->n, pnew, cg->q32, cg->y32, cg->r32, cg->pB, cg->pA,
                                       cg->st, cg->ms);
-    if (marks) cg->mark(3 * c + 2);
+    if (marks) cg->mark(4 * c + 3);
   }
   k_cg_advance<<<1, 64, 0, s>>>(cg->st, kChunk);
 }
@@ -1372,7 +1468,7 @@ int ffhip_cg_solve_mixed(ffhip_cg cg, const double *sval, const float *sval32, d
     hs.eps = eps;
     FFHIP_CHECK(hipMemcpyAsync(cg->st, &hs, sizeof(hs), hipMemcpyHostToDevice, s));
     FFHIP_CHECK(hipMemcpyAsync(cg->ms, &hm, sizeof(hm), hipMemcpyHostToDevice, s));
-    cg->ensure_events(3 * chunk);
+    cg->ensure_events(4 * chunk);
     const int par = it & 1;
     for (;;) {
       const int it0 = it;
@@ -1511,7 +1607,7 @@ int ffhip_cg_solve(ffhip_cg cg, const double *b, double *x, int on_device, doubl
     if (cg->ntgv > 0) k_init_bc<<<nblk(n, 256), 256, 0, s>>>(n, cg->wcl, cg->bw, tgv, cg->xw);
     CgState hs;
     memset(&hs, 0, sizeof(hs));
-    cg->t_mv = cg->t_up = 0.;
+    cg->t_dv = cg->t_mv = cg->t_up = 0.;
     cg->n_timed = 0;
     if (cg->mode == FFHIP_CG_FP64) {
       const int G = cg->grid;
@@ -1525,7 +1621,7 @@ int ffhip_cg_solve(ffhip_cg cg, const double *b, double *x, int on_device, doubl
       FFHIP_LAUNCH_FMT(cg, k_pcg_init, G, s, M, cg->xw, cg->bw, cg->dinv, cg->G, cg->CG, cg->H[0],
                        cg->pA, cg->st);
       FFHIP_CHECK(hipGetLastError());
-      cg->ensure_events(3 * chunk);
+      cg->ensure_events(4 * chunk);
       for (int it = 1;; it += chunk) {
         if (gx) FFHIP_CHECK(hipGraphLaunch(gx, s));
         else launch_pcg_chunk(cg, sval, cg->profile != 0);
@@ -1556,6 +1652,7 @@ int ffhip_cg_solve(ffhip_cg cg, const double *b, double *x, int on_device, doubl
       stats->timed = cg->n_timed;
       stats->ms_matvec_avg = cg->n_timed ? cg->t_mv / cg->n_timed : 0.;
       stats->ms_update_avg = cg->n_timed ? cg->t_up / cg->n_timed : 0.;
+      stats->ms_direction_avg = cg->n_timed ? cg->t_dv / cg->n_timed : 0.;
     }
     FFHIP_REQUIRE(hs.done != 4, FFHIP_ERR_NAN, "ffhip_cg_solve: NaN in CG (bad matrix)");
     FFHIP_REQUIRE(hs.done != 3, FFHIP_ERR_NOCONV,

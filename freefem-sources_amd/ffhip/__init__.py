@@ -46,7 +46,7 @@ class CGStats(ctypes.Structure):
                 ("true_gcg", ctypes.c_double), ("true_relres", ctypes.c_double),
                 ("ms_solve", ctypes.c_double), ("ms_matvec_avg", ctypes.c_double),
                 ("ms_update_avg", ctypes.c_double), ("timed", ctypes.c_int),
-                ("reserved2", ctypes.c_int)]
+                ("reserved2", ctypes.c_int), ("ms_direction_avg", ctypes.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if not k.startswith("reserved")}

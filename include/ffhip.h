@@ -211,6 +211,7 @@ typedef struct {
   double ms_update_avg;
   int timed;
   int reserved2;
+  double ms_direction_avg; /* the same for the direction kernel */
 } ffhip_cg_stats;
 /* Solver setup for a matrix (SolverCG ctor + HMatVirtPrecon,
  * src/femlib/VirtualSolverCG.hpp:27-69, 157-167): tgv detection

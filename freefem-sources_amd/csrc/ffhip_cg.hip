@@ -25,6 +25,7 @@
 // accumulated in FP64 and convergence is only accepted on the FP64 residual.
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -892,6 +893,119 @@ __global__ __launch_bounds__(kBlock) void k_cg2_update(
   if (threadIdx.x == 0) pout[xcd_block_id()] = tot;
 }
 
+// ---------------------------------------------------------------- GMRES
+// Restarted right-preconditioned GMRES with modified Gram-Schmidt
+// (SolverGMRES::dosolver + fgmres, VirtualSolverCG.hpp:236-255, CG.cpp:347-517).
+// Per inner iteration it: k_gm_prec (V_it = w / ||w|| and Vp_it = C V_it),
+// k_gm_matvec (w = A Vp_it, partial (w, V_0)), then one k_gm_mgs per basis
+// vector i <= it (h_i from the previous partials, w -= h_i V_i, partial
+// (w, V_i+1) or (w, w)); the Hessenberg column, its Givens rotations and the
+// convergence test are the reference's scalar code on the host.
+
+// partial (r, r) of the right-hand side off the tgv rows: ||b|| of fgmres
+__global__ __launch_bounds__(kBlock) void k_gm_normb(int64_t n, const double *__restrict__ b,
+                                                     const uint8_t *__restrict__ wcl,
+                                                     double *__restrict__ pout) {
+  __shared__ double lds[kBlock / kWave];
+  int64_t r0, r1;
+  block_rows(n, r0, r1);
+  double part = 0.;
+  for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) {
+    const double v = (wcl && wcl[r]) ? 0. : b[r];
+    part += v * v;
+  }
+  const double tot = block_sum<kBlock>(part, lds);
+  if (threadIdx.x == 0) pout[xcd_block_id()] = tot;
+}
+
+// z = -(A u - b) (fgmres' ri, pCl the identity); partial (z, z)
+template <int KIND>
+__global__ FFHIP_MATVEC_ATTR void k_gm_residual(MatFmt<double> M, const double *__restrict__ u,
+                                                const double *__restrict__ b,
+                                                double *__restrict__ z,
+                                                double *__restrict__ pout) {
+  __shared__ double lds[kBlock / kWave];
+  double part = 0.;
+  matvec_rows<KIND, double, double>(
+      M, [&](int32_t j) { return u[j]; },
+      [&](int64_t r, double acc) {
+        const double v = (acc + -1. * b[r]) * -1.;
+        z[r] = v;
+        part += v * v;
+      });
+  const double tot = block_sum<kBlock>(part, lds);
+  if (threadIdx.x == 0) pout[xcd_block_id()] = tot;
+}
+
+// V = inv * w; Vp = C V (dinv null: the identity)
+__global__ __launch_bounds__(kBlock) void k_gm_prec(int64_t n, double inv,
+                                                    const double *__restrict__ w,
+                                                    const double *__restrict__ dinv,
+                                                    double *__restrict__ V, double *__restrict__ Vp) {
+  int64_t r0, r1;
+  block_rows(n, r0, r1);
+  for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) {
+    const double v = inv * w[r];
+    V[r] = v;
+    Vp[r] = dinv ? dinv[r] * v : v;
+  }
+}
+
+// w = A Vp; partial (w, V_0)
+template <int KIND>
+__global__ FFHIP_MATVEC_ATTR void k_gm_matvec(MatFmt<double> M, const double *__restrict__ Vp,
+                                              const double *__restrict__ V0,
+                                              double *__restrict__ w, double *__restrict__ pout) {
+  __shared__ double lds[kBlock / kWave];
+  double part = 0.;
+  matvec_rows<KIND, double, double>(
+      M, [&](int32_t j) { return Vp[j]; },
+      [&](int64_t r, double acc) {
+        w[r] = acc;
+        part += acc * V0[r];
+      });
+  const double tot = block_sum<kBlock>(part, lds);
+  if (threadIdx.x == 0) pout[xcd_block_id()] = tot;
+}
+
+// h_i = (w, V_i) from the partials; w -= h_i V_i; partial (w, Vnext), or
+// (w, w) when Vnext is null (the last basis vector)
+__global__ __launch_bounds__(kBlock) void k_gm_mgs(int i, int64_t n, const double *__restrict__ Vi,
+                                                   const double *__restrict__ Vnext,
+                                                   double *__restrict__ w,
+                                                   const double *__restrict__ pin,
+                                                   double *__restrict__ pout,
+                                                   double *__restrict__ hcol) {
+  __shared__ double lds[kBlock / kWave];
+  const double h = reduce_partials(pin, gridDim.x, 1, 0, lds);
+  if (blockIdx.x == 0 && threadIdx.x == 0) hcol[i] = h;
+  const double a = -h;
+  int64_t r0, r1;
+  block_rows(n, r0, r1);
+  double part = 0.;
+  for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) {
+    const double v = w[r] + a * Vi[r];
+    w[r] = v;
+    part += v * (Vnext ? Vnext[r] : v);
+  }
+  const double tot = block_sum<kBlock>(part, lds);
+  if (threadIdx.x == 0) pout[xcd_block_id()] = tot;
+}
+
+// x = (sum_i y_i Vp_i) + x0, the sum taken in order i = 0..k-1
+__global__ __launch_bounds__(kBlock) void k_gm_update(int64_t n, int k, const double *__restrict__ y,
+                                                      const double *const *__restrict__ Vp,
+                                                      const double *__restrict__ x0,
+                                                      double *__restrict__ x) {
+  int64_t r0, r1;
+  block_rows(n, r0, r1);
+  for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) {
+    double acc = 0.;
+    for (int i = 0; i < k; ++i) acc += y[i] * Vp[i][r];
+    x[r] = acc + x0[r];
+  }
+}
+
 // plain mat-vec for measurement: y = A x
 template <int KIND, class V, class X>
 __global__ FFHIP_MATVEC_ATTR void k_matvec(MatFmt<V> M, const X *__restrict__ x, X *__restrict__ y) {
@@ -1104,6 +1218,31 @@ struct ffhip_cg_s {
   CgState *st = nullptr;
   MixState *ms = nullptr;
   Cg2State *st2 = nullptr;  // affine CG (LinearCG)
+  // GMRES: Krylov basis V_i and C V_i, allocated vector by vector as a solve
+  // first reaches them (as the reference's KN<KN<K>> are) and kept; the
+  // device table of the C V_i pointers; w, x0, the Hessenberg column, y
+  std::vector<double *> gmV, gmVp;
+  double **gmVpDev = nullptr;
+  int gmCap = 0;  // entries of gmVpDev, gmH and gmY
+  double *gmW = nullptr, *gmX0 = nullptr, *gmH = nullptr, *gmY = nullptr;
+  void gm_basis(int k) {
+    if (k > gmCap) {
+      const int cap = std::max(k, 2 * gmCap);
+      double **p = dalloc<double *>(cap);
+      double *h = dalloc<double>(cap + 1), *y = dalloc<double>(cap + 1);
+      dfree(gmVpDev); dfree(gmH); dfree(gmY);
+      gmVpDev = p; gmH = h; gmY = y; gmCap = cap;
+      FFHIP_CHECK(hipMemcpy(gmVpDev, gmVp.data(), sizeof(double *) * gmVp.size(),
+                            hipMemcpyHostToDevice));
+    }
+    while ((int)gmV.size() < k) {
+      gmV.push_back(dalloc<double>(n));
+      gmVp.push_back(dalloc<double>(n));
+      FFHIP_CHECK(hipMemcpy(gmVpDev + gmVp.size() - 1, &gmVp.back(), sizeof(double *),
+                            hipMemcpyHostToDevice));
+    }
+    if (!gmW) { gmW = dalloc<double>(n); gmX0 = dalloc<double>(n); }
+  }
   // captured chunks of CG launches, keyed by (mode, value array, parity of the
   // chunk's first iteration); kernel by kernel when eager or profiling (event
   // records captured in a graph carry no timestamps hipEventElapsedTime reads)
@@ -1518,6 +1657,9 @@ static void cg_release(ffhip_cg cg) {
   dfree(cg->AH); dfree(cg->r32); dfree(cg->q32); dfree(cg->y32); dfree(cg->p32[0]);
   dfree(cg->p32[1]); dfree(cg->pA); dfree(cg->pB); dfree(cg->pC); dfree(cg->st); dfree(cg->ms);
   dfree(cg->st2);
+  for (double *&p : cg->gmV) dfree(p);
+  for (double *&p : cg->gmVp) dfree(p);
+  dfree(cg->gmVpDev); dfree(cg->gmW); dfree(cg->gmX0); dfree(cg->gmH); dfree(cg->gmY);
   delete cg;
 }
 
@@ -1716,6 +1858,119 @@ int ffhip_cg_solve_affine(ffhip_cg cg, const double *b, double *x, int on_device
     }
     FFHIP_REQUIRE(hs.done != 5, FFHIP_ERR_NAN, "CG2: Matrix is not defined (/0), sorry");
     FFHIP_REQUIRE(hs.done != 6, FFHIP_ERR_NAN, "CG2: Bug : ro is NaN");
+  });
+}
+
+int ffhip_gmres_solve(ffhip_cg cg, const double *b, double *x, int on_device, double eps,
+                      int itmax, int restart, double tgv, int trans, ffhip_cg_stats *stats) {
+  return guard([&] {
+    FFHIP_REQUIRE(cg && b && x, FFHIP_ERR_ARG, "ffhip_gmres_solve: null argument");
+    FFHIP_REQUIRE(restart >= 1, FFHIP_ERR_ARG, "ffhip_gmres_solve: restart %d", restart);
+    ffhip_ctx ctx = cg->ctx;
+    hipStream_t s = ctx->stream;
+    const int64_t n = cg->n;
+    const int G = cg->grid, m = restart;
+    if (trans && !cg->svalT) cg_fill_values(cg, true);
+    const double *sval = trans ? cg->svalT : cg->sval;
+    const MatFmt<double> M = cg->fmt_of(sval);
+    const hipMemcpyKind kin = on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    cg->gm_basis(1);
+    FFHIP_CHECK(hipMemcpyAsync(cg->bw, b, 8 * n, kin, s));
+    FFHIP_CHECK(hipMemcpyAsync(cg->xw, x, 8 * n, kin, s));
+    std::vector<double> part(G), hcol(m + 1);
+    // block partials written by the last kernel, added on the host in block order
+    auto host_sum = [&](const double *p) {
+      FFHIP_CHECK(hipMemcpyAsync(part.data(), p, 8 * G, hipMemcpyDeviceToHost, s));
+      FFHIP_CHECK(hipStreamSynchronize(s));
+      double t = 0.;
+      for (int i = 0; i < G; ++i) t += part[i];
+      return t;
+    };
+    ctx->tic();
+    if (cg->ntgv > 0) k_init_bc<<<nblk(n, 256), 256, 0, s>>>(n, cg->wcl, cg->bw, tgv, cg->xw);
+    const int nbitermx = itmax > 0 ? itmax : (int)n;  // SolverGMRES(ds)
+    // fgmres (CG.cpp:347-517) with leftC = 0: the scalar part as the reference's
+    std::vector<double> rot0(m + 2), rot1(m + 2), g(m + 1), g1(m + 1), y(m + 1),
+        Hn((size_t)(m + 2) * (m + 1), 0.);
+    auto HN = [&](int i, int j) -> double & { return Hn[(size_t)i * (m + 1) + j]; };
+    k_gm_normb<<<G, kBlock, 0, s>>>(n, cg->bw, cg->ntgv > 0 ? cg->wcl : nullptr, cg->pA);
+    double normb = std::sqrt(host_sum(cg->pA));
+    double relres = 1e100;
+    bool noconv = true;
+    int iter = 0, nrestart = 0;
+    while (noconv) {
+      FFHIP_CHECK(hipMemcpyAsync(cg->gmX0, cg->xw, 8 * n, hipMemcpyDeviceToDevice, s));
+      FFHIP_LAUNCH_FMT(cg, k_gm_residual, G, s, M, cg->xw, cg->bw, cg->gmW, cg->pA);
+      FFHIP_CHECK(hipGetLastError());
+      g[0] = std::sqrt(host_sum(cg->pA));
+      if (normb < 1.e-20 || eps < 0) normb = 1.;
+      double inv = 1. / g[0];
+      int it;
+      for (it = 0; it < m; it++, iter++) {
+        cg->gm_basis(it + 1);
+        k_gm_prec<<<G, kBlock, 0, s>>>(n, inv, cg->gmW, cg->dinv, cg->gmV[it], cg->gmVp[it]);
+        FFHIP_LAUNCH_FMT(cg, k_gm_matvec, G, s, M, cg->gmVp[it], cg->gmV[0], cg->gmW, cg->pB);
+        double *pin = cg->pB, *pout = cg->pA;
+        for (int i = 0; i <= it; ++i) {  // modified Gram-Schmidt
+          k_gm_mgs<<<G, kBlock, 0, s>>>(i, n, cg->gmV[i], i < it ? cg->gmV[i + 1] : nullptr,
+                                        cg->gmW, pin, pout, cg->gmH);
+          std::swap(pin, pout);
+        }
+        FFHIP_CHECK(hipGetLastError());
+        FFHIP_CHECK(hipMemcpyAsync(hcol.data(), cg->gmH, 8 * (it + 1), hipMemcpyDeviceToHost, s));
+        const double ww = host_sum(pin);
+        for (int i = 0; i <= it; ++i) HN(i, it) = hcol[i];
+        const double aux = HN(it + 1, it) = std::sqrt(ww);
+        inv = 1. / aux;
+        for (int i = 0; i < it; i++) {
+          const double aa = rot0[i] * HN(i, it) + rot1[i] * HN(i + 1, it);
+          const double bb = -rot1[i] * HN(i, it) + rot0[i] * HN(i + 1, it);
+          HN(i, it) = aa;
+          HN(i + 1, it) = bb;
+        }
+        const double sq = std::sqrt(HN(it, it) * HN(it, it) + HN(it + 1, it) * HN(it + 1, it));
+        rot0[it] = HN(it, it) / sq;
+        rot1[it] = HN(it + 1, it) / sq;
+        HN(it, it) = rot0[it] * HN(it, it) + rot1[it] * HN(it + 1, it);
+        HN(it + 1, it) = 0.;
+        g[it + 1] = -rot1[it] * g[it];
+        g[it] = rot0[it] * g[it];
+        relres = std::fabs(g[it + 1]);
+        if (relres / normb < std::fabs(eps)) { noconv = false; break; }
+        if (it > nbitermx) break;
+      }
+      it = std::min(it, m - 1);
+      for (int i = it; i >= 0; i--) {  // reconstruction
+        g1[i] = g[i];
+        for (int j = i + 1; j < it + 1; j++) g1[i] = g1[i] - HN(i, j) * y[j];
+        y[i] = g1[i] / HN(i, i);
+      }
+      FFHIP_CHECK(hipMemcpyAsync(cg->gmY, y.data(), 8 * (it + 1), hipMemcpyHostToDevice, s));
+      k_gm_update<<<G, kBlock, 0, s>>>(n, it + 1, cg->gmY, cg->gmVpDev, cg->gmX0, cg->xw);
+      FFHIP_CHECK(hipGetLastError());
+      if (!noconv) break;
+      if (iter > nbitermx) break;
+      nrestart++;
+    }
+    ctx->toc();
+    const double ms = ctx->last_ms;
+    double res[3];
+    residual_sums(cg, sval, cg->xw, cg->bw, nullptr, res);
+    FFHIP_CHECK(hipMemcpyAsync(x, cg->xw, 8 * n, on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+    FFHIP_CHECK(hipStreamSynchronize(s));
+    if (stats) {
+      memset(stats, 0, sizeof(*stats));
+      stats->iters = iter;
+      stats->converged = noconv ? 0 : 1;
+      stats->refinements = nrestart;
+      stats->gcg0 = normb;
+      stats->gcg = relres / normb;
+      stats->true_gcg = res[2];
+      stats->true_relres = res[1] > 0 ? std::sqrt(res[0] / res[1]) : std::sqrt(res[0]);
+      stats->ms_solve = ms;
+    }
+    FFHIP_REQUIRE(!noconv, FFHIP_ERR_NOCONV, "ffhip_gmres_solve: fgmres did not converge in %d iterations",
+                  iter);
   });
 }
 

@@ -88,6 +88,7 @@ _SIGS = {
     "ffhip_cg_solve": (_i, [_vp, _vp, _vp, _i, _d, _i, _d, _i, ctypes.POINTER(CGStats)]),
     "ffhip_cg_solve_affine": (_i, [_vp, _vp, _vp, _i, _d, _i, _i, ctypes.POINTER(_d),
                                    ctypes.POINTER(CGStats)]),
+    "ffhip_gmres_solve": (_i, [_vp, _vp, _vp, _i, _d, _i, _i, _d, _i, ctypes.POINTER(CGStats)]),
     "ffhip_cg_free": (_i, [_vp]),
     "ffhip_cg_set_profiling": (_i, [_vp, _i]),
     "ffhip_bench_spmv": (_i, [_vp, _i, _i, _vp, _vp, ctypes.POINTER(_d)]),
@@ -458,6 +459,28 @@ class CG:
             x = torch.zeros_like(b) if x is None else x
             rc = lib().ffhip_cg_solve(self.h, _dp(b), _dp(x), 1, eps, itmax, tgv, int(trans),
                                       ctypes.byref(st))
+        if raise_on_error:
+            check(rc)
+        return x, st.as_dict()
+
+    def gmres(self, b, x=None, eps=1e-6, itmax=0, restart=1000, tgv=1e30, trans=False,
+              raise_on_error=True):
+        """solver=GMRES (SolverGMRES::dosolver over fgmres) with this solver's
+        matrix, Jacobi preconditioner and layout, in FP64.  restart is the
+        Krylov dimension (FreeFEM's dimKrylov / NbSpace, 1000 by default).
+        Returns (x, stats): iters, converged, refinements = restarts,
+        gcg0 = ||b||, gcg = final relative GMRES residual."""
+        st = CGStats()
+        if isinstance(b, np.ndarray):
+            b = np.ascontiguousarray(b, np.float64)
+            x = np.zeros(b.size) if x is None else np.array(x, np.float64)
+            rc = lib().ffhip_gmres_solve(self.h, _hp(b), _hp(x), 0, eps, itmax, restart, tgv,
+                                         int(trans), ctypes.byref(st))
+        else:
+            import torch
+            x = torch.zeros_like(b) if x is None else x
+            rc = lib().ffhip_gmres_solve(self.h, _dp(b), _dp(x), 1, eps, itmax, restart, tgv,
+                                         int(trans), ctypes.byref(st))
         if raise_on_error:
             check(rc)
         return x, st.as_dict()

@@ -10,6 +10,8 @@
 //    (src/femlib/VirtualSolverCG.hpp:116) so the matrix gets the same full
 //    (non-half) storage as with solver=CG.  `ffhipreplacecg(1)` re-points
 //    solver=CG itself (TheFFSolver::ChangeSolver, SparseLinearSolver.hpp:59-71).
+//  * solver "HIPGMRES": restarted GMRES with the Jacobi diagonal
+//    (SolverGMRES, VirtualSolverCG.hpp:196-255) for non-symmetric matrices.
 //  * a higher-preference "=" / "<-" operator for
 //    (Matrice_Creuse<double>*, Call_FormBilinear<v_fes3,v_fes3>*)
 //    (the overload of OpMatrixtoBilinearForm, src/fflib/problem.hpp:897-927,
@@ -152,6 +154,84 @@ class SolverHIPCG : public VirtualSolver<int, double> {
   ~SolverHIPCG() {
     // device buffers only: the HashMatrix may already be deleted
     // (type_isdeleted, HashMatrix.hpp:62) and is not touched here
+    if (cg) ffhip_cg_free(cg);
+    if (M) ffhip_mat_free(M);
+  }
+};
+
+// solver=HIPGMRES: SolverGMRES (VirtualSolverCG.hpp:196-255) — restarted
+// right-preconditioned GMRES (fgmres, CG.cpp:347-517) with the Jacobi
+// diagonal, for non-symmetric matrices; restart = dimKrylov (NbSpace)
+class SolverHIPGMRES : public VirtualSolver<int, double> {
+ public:
+  // 1 unsym, 2 herm, 4 sym, 8 pos, 16 nopos, 32 seq: as SolverGMRES
+  static const int orTypeSol = 1 | 2 | 4 | 8 | 16 | 32;
+  typedef HashMatrix<int, double> HMat;
+  HMat *ptA;
+  ffhip_mat M;
+  ffhip_cg cg;
+  int verb, itermax, restart;
+  double eps, tgv;
+  double *veps;
+  long *getnbiter;
+  long cn, cs;
+
+  SolverHIPGMRES(HMat &A, const Data_Sparse_Solver &ds, Stack)
+      : ptA(&A), M(nullptr), cg(nullptr), verb(ds.verb), itermax(ds.itmax > 0 ? ds.itmax : A.n),
+        restart(ds.NbSpace), eps(ds.epsilon), tgv(ds.tgv), veps(ds.veps),
+        getnbiter(ds.getnbiter), cn(0), cs(0) {
+    ffassert(A.n == A.m);
+  }
+
+  void UpdateState() {
+    if (ptA->GetReDoNumerics()) cn++;
+    if (ptA->GetReDoSymbolic()) cs++;
+    this->ChangeCodeState(ptA->n, cs, cn);
+  }
+
+  void fac_symbolic() {
+    int *ia = 0, *ja = 0;
+    double *a = 0;
+    ptA->CSR(ia, ja, a);
+    const int64_t n = ptA->n, nnz = ia[n];
+    vector<int64_t> rp(ia, ia + n + 1);
+    if (cg) { ffhip_cg_free(cg); cg = nullptr; }
+    if (M) { ffhip_mat_free(M); M = nullptr; }
+    hip_check(ffhip_mat_upload(hip_ctx(), n, nnz, rp.data(), ja, a, &M), "HIPGMRES: matrix upload");
+    hip_check(ffhip_cg_create(hip_ctx(), M, FFHIP_CG_FP64, &cg), "HIPGMRES: solver setup");
+  }
+
+  void fac_numeric() {
+    int *ia = 0, *ja = 0;
+    double *a = 0;
+    ptA->CSR(ia, ja, a);
+    hip_check(ffhip_mat_set_values(M, a), "HIPGMRES: matrix values");
+    hip_check(ffhip_cg_update_values(cg), "HIPGMRES: solver values");
+  }
+
+  // SolverGMRES::dosolver semantics (VirtualSolverCG.hpp:236-255)
+  void dosolver(double *x, double *b, int N, int trans) {
+    int err = 0;
+    for (int k = 0; k < N; ++k) {
+      ffhip_cg_stats st;
+      const int rc = ffhip_gmres_solve(cg, b + (size_t)k * ptA->n, x + (size_t)k * ptA->n, 0, eps,
+                                       itermax, restart, tgv, trans, &st);
+      if (rc == FFHIP_ERR_NOCONV) err++;
+      else hip_check(rc, "HIPGMRES: solve");
+      if (veps) *veps = eps;
+      if (getnbiter) *getnbiter = st.iters;
+      if (verb > 1)
+        cout << " HIPGMRES: " << (st.converged ? "converged in " : "no convergence after ") << st.iters
+             << " iterations, relative residual " << st.gcg << " (" << st.ms_solve << " ms on the GPU)"
+             << endl;
+    }
+    if (err) {
+      cerr << "Error: fgmres do not converge nb end =" << err << endl;
+      ffassert(0);
+    }
+  }
+
+  ~SolverHIPGMRES() {
     if (cg) ffhip_cg_free(cg);
     if (M) ffhip_mat_free(M);
   }
@@ -562,6 +642,7 @@ basicAC_F0::name_and_type HipLinearCG::Op::name_param[] = {
 static void Load_Init() {
   addsolver<SolverHIPCG<0> >("HIPCG", 50);
   addsolver<SolverHIPCG<1> >("HIPCGMIXED", 50);
+  addsolver<SolverHIPGMRES>("HIPGMRES", 50);
   Global.Add("ffhipreplacecg", "(", new OneOperator1<long, long>(ffhip_replace_cg));
   TheOperators->Add("=", new OpHipMatrixP1(0));
   TheOperators->Add("<-", new OpHipMatrixP1(1));

@@ -255,6 +255,20 @@ enum { FFHIP_CG2_IDENTITY = 0, FFHIP_CG2_JACOBI = 1 };
 int ffhip_cg_solve_affine(ffhip_cg cg, const double *b, double *x, int on_device,
                           double eps, int itmax, int precon, double *eps_out,
                           ffhip_cg_stats *stats);
+/* solver=GMRES: SolverGMRES::dosolver (VirtualSolverCG.hpp:236-255) over
+ * fgmres (CG.cpp:347-517) — restarted GMRES, right-preconditioned by the
+ * solver's Jacobi diagonal (HMatVirtPrecon), modified Gram-Schmidt, in FP64
+ * with the solver's layout whatever its mode.  tgv rows start at b_i/tgv
+ * (SetInitWithBC); stop when |g_{k+1}| / ||b|| < |eps|, ||b|| taken with the
+ * tgv rows zeroed (and 1 for eps < 0 or ||b|| < 1e-20); restart = the Krylov
+ * dimension (NbSpace, dimKrylov=, 1000 by default in FreeFEM); itmax <= 0
+ * means n.  stats: iters = fgmres' iteration count, refinements = restarts,
+ * gcg0 = ||b||, gcg = the final |g_{k+1}| / ||b||.  FFHIP_ERR_NOCONV when it
+ * does not converge (SolverGMRES, erronerr = 1).  The basis vectors are
+ * allocated as a solve first reaches them and kept by the solver. */
+int ffhip_gmres_solve(ffhip_cg cg, const double *b, double *x, int on_device,
+                      double eps, int itmax, int restart, double tgv, int trans,
+                      ffhip_cg_stats *stats);
 int ffhip_cg_free(ffhip_cg cg);
 /* Per-kernel HIP-event timing of the solve loop (measurement only). */
 int ffhip_cg_set_profiling(ffhip_cg cg, int on);

@@ -176,6 +176,15 @@ int ffo_pcg(int n, const int64_t *rowptr, const int32_t *colidx,
 int ffo_cg2(int n, const int64_t *rowptr, const int32_t *colidx,
             const double *vals, const double *b, double *x, double *eps,
             int itmax, int precon, double *stats);
+/* SolverGMRES::dosolver (VirtualSolverCG.hpp:236-255) + HMatVirtPrecon
+ * (Jacobi, tgv rows, SetInitWithBC) + fgmres (CG.cpp:347-517): restarted
+ * right-preconditioned GMRES with modified Gram-Schmidt, restart = NbSpace,
+ * stop when |g_{it+1}| / ||b (tgv rows zeroed)|| < |eps|.  itmax <= 0 means n.
+ * stats = {iterations (fgmres' nbitermx out), relres/normb, normb, restarts}.
+ * Returns 1 converged, 0 not. */
+int ffo_fgmres(int n, const int64_t *rowptr, const int32_t *colidx, const double *vals,
+               const double *b, double *x, double eps, int itmax, int restart, double tgv,
+               double *stats);
 
 #ifdef __cplusplus
 }

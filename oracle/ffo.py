@@ -67,6 +67,7 @@ def lib():
             "ffo_gettgv": (_d, [_i, _i64p, _i32p, _f64p, ctypes.POINTER(_i)]),
             "ffo_pcg": (_i, [_i, _i64p, _i32p, _f64p, _f64p, _f64p, _d, _i, _d, _i, _f64p]),
             "ffo_cg2": (_i, [_i, _i64p, _i32p, _f64p, _f64p, _f64p, _f64p, _i, _i, _f64p]),
+            "ffo_fgmres": (_i, [_i, _i64p, _i32p, _f64p, _f64p, _f64p, _d, _i, _i, _d, _f64p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -98,6 +99,8 @@ def ref():
             "ref_hm_matmul": (_i, [_vp, _f64p, _f64p, _i]),
             "ref_hm_csr": (_i, [_vp, _i64p, _i32p, _f64p]),
             "ref_hm_cg": (_i, [_vp, _f64p, _f64p, _d, _i, _d, _i, _f64p]),
+            "ref_hm_from_csr": (_vp, [_i, _i64p, _i32p, _f64p]),
+            "ref_hm_gmres": (_i, [_vp, _f64p, _f64p, _d, _i, _i, _d, _f64p]),
             "ref_fespace_dofs": (_l, [_i, _f64p, _l, _i32p, _i, _vp, ctypes.POINTER(_i)]),
             "ref_p2_fb": (_i, [_f64p, _f64p, _f64p, _f64p]),
         }
@@ -235,6 +238,17 @@ def cg2(rowptr, colidx, vals, b=None, x0=None, eps=1e-6, itmax=100, precon=0):
     st = np.zeros(4)
     lib().ffo_cg2(n, rowptr, colidx, vals, bb, x, e, itmax, precon, st)
     return x, dict(iters=int(st[0]), g20=st[1], g2=st[2], nret=int(st[3]), reps2=float(e[0]))
+
+
+def fgmres(rowptr, colidx, vals, b, x0=None, eps=1e-6, itmax=0, restart=50, tgv=1e30):
+    """solver=GMRES: SolverGMRES::dosolver over fgmres.  Returns x and
+    {iters, relres, normb, restarts, converged}."""
+    n = rowptr.size - 1
+    x = np.zeros(n) if x0 is None else np.array(x0, np.float64)
+    st = np.zeros(4)
+    rc = lib().ffo_fgmres(n, rowptr, colidx, vals, np.ascontiguousarray(b, np.float64), x, eps,
+                          itmax, restart, tgv, st)
+    return x, dict(iters=int(st[0]), relres=st[1], normb=st[2], restarts=int(st[3]), converged=rc)
 
 
 def poisson_p1_3d(n, kind=COEF_SIN3, c0=1.0, kappa_e=None, labels=(1, 2, 3, 4, 5, 6),

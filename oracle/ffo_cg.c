@@ -187,3 +187,127 @@ int ffo_cg2(int n, const int64_t *rowptr, const int32_t *colidx,
   free(g); free(h); free(Ah); free(d1);
   return nret;
 }
+
+/* ---- restarted flexible GMRES: SolverGMRES::dosolver (VirtualSolverCG.hpp:
+ * 236-255) with HMatVirtPrecon (13-111: tgv rows, Jacobi diag1, SetInitWithBC)
+ * over fgmres (CG.cpp:347-517, right preconditioning: leftC is forced to 0). */
+static double nrm2(int n, const double *x) {
+  double s = 0.;
+  for (int i = 0; i < n; ++i) s += x[i] * x[i];
+  return sqrt(s);
+}
+
+int ffo_fgmres(int n, const int64_t *rowptr, const int32_t *colidx, const double *vals,
+               const double *b, double *x, double eps, int itmax, int restart, double tgv,
+               double *stats) {
+  const int m = restart;
+  int ntgv1 = 0;
+  const double tgvm = ffo_gettgv(n, rowptr, colidx, vals, &ntgv1);
+  int *wbc = NULL;
+  if (ntgv1) {
+    const double tgve = tgvm <= 0 ? 1e200 : tgvm;
+    wbc = (int *)malloc(sizeof(int) * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+      int f;
+      wbc[i] = diag_of(rowptr, colidx, vals, i, &f) == tgve;
+    }
+  }
+  double *d1 = (double *)malloc(sizeof(double) * (size_t)n);
+  for (int i = 0; i < n; ++i) {
+    int f;
+    const double a = diag_of(rowptr, colidx, vals, i, &f);
+    d1[i] = (a * a < 1e-60) ? 1. : 1. / a;
+  }
+  if (wbc) /* SetInitWithBC */
+    for (int i = 0; i < n; ++i)
+      if (wbc[i]) x[i] = b[i] / tgv;
+  int nbitermx = itmax > 0 ? itmax : n;
+  /* fgmres */
+  const size_t N = (size_t)n;
+  double *rot0 = (double *)malloc(sizeof(double) * (m + 2)), *rot1 = (double *)malloc(sizeof(double) * (m + 2));
+  double *g = (double *)malloc(sizeof(double) * (m + 1)), *g1 = (double *)malloc(sizeof(double) * (m + 1));
+  double *Hn = (double *)calloc((size_t)(m + 2) * (m + 1), sizeof(double));
+#define HN(i, j) Hn[(size_t)(i) * (m + 1) + (j)]
+  double *uni = (double *)malloc(sizeof(double) * N), *x0 = (double *)malloc(sizeof(double) * N);
+  double *ri = (double *)malloc(sizeof(double) * N), *zi = (double *)malloc(sizeof(double) * N);
+  double *vi = (double *)malloc(sizeof(double) * N), *wi = (double *)malloc(sizeof(double) * N);
+  double *V = (double *)malloc(sizeof(double) * N * (m + 1));
+  double *Vp = (double *)malloc(sizeof(double) * N * (m + 1));
+  double *y = (double *)malloc(sizeof(double) * (m + 1));
+  double relres = 1e100, normb;
+  for (int i = 0; i < n; ++i) wi[i] = b[i];
+  if (wbc)
+    for (int i = 0; i < n; ++i)
+      if (wbc[i]) wi[i] = 0.;
+  for (int i = 0; i < n; ++i) vi[i] = 0. + wi[i]; /* pCl = identity */
+  normb = nrm2(n, vi);
+  for (int i = 0; i < n; ++i) uni[i] = x[i];
+  int noconv = 1, iter = 0, nrestart = 0;
+  while (noconv) {
+    for (int i = 0; i < n; ++i) x0[i] = uni[i];
+    ffo_spmv(n, rowptr, colidx, vals, uni, ri); /* ri = -(A uni - rhs) */
+    for (int i = 0; i < n; ++i) ri[i] += -1. * b[i];
+    for (int i = 0; i < n; ++i) ri[i] *= -1.;
+    for (int i = 0; i < n; ++i) zi[i] = 0. + ri[i];
+    g[0] = nrm2(n, zi);
+    if (normb < 1.e-20 || eps < 0) normb = 1.;
+    {
+      const double inv = 1. / g[0];
+      for (int i = 0; i < n; ++i) V[i] = inv * zi[i];
+    }
+    int it;
+    for (it = 0; it < m; it++, iter++) {
+      double *Vit = V + N * it;
+      for (int i = 0; i < n; ++i) vi[i] = 0. + d1[i] * Vit[i]; /* C.matmul */
+      for (int i = 0; i < n; ++i) Vp[N * it + i] = vi[i];
+      ffo_spmv(n, rowptr, colidx, vals, vi, wi);
+      for (int i = 0; i < it + 1; i++) { /* modified Gram-Schmidt */
+        const double *Vi = V + N * i;
+        HN(i, it) = dot(n, wi, Vi);
+        const double a = -HN(i, it);
+        for (int k = 0; k < n; ++k) wi[k] += a * Vi[k];
+      }
+      const double aux = HN(it + 1, it) = nrm2(n, wi);
+      {
+        const double inv = 1. / aux;
+        for (int i = 0; i < n; ++i) V[N * (it + 1) + i] = inv * wi[i];
+      }
+      for (int i = 0; i < it; i++) {
+        const double aa = rot0[i] * HN(i, it) + rot1[i] * HN(i + 1, it);
+        const double bb = -rot1[i] * HN(i, it) + rot0[i] * HN(i + 1, it);
+        HN(i, it) = aa;
+        HN(i + 1, it) = bb;
+      }
+      const double sq = sqrt(HN(it, it) * HN(it, it) + HN(it + 1, it) * HN(it + 1, it));
+      rot0[it] = HN(it, it) / sq;
+      rot1[it] = HN(it + 1, it) / sq;
+      HN(it, it) = rot0[it] * HN(it, it) + rot1[it] * HN(it + 1, it);
+      HN(it + 1, it) = 0.;
+      g[it + 1] = -rot1[it] * g[it];
+      g[it] = rot0[it] * g[it];
+      relres = fabs(g[it + 1]);
+      if (relres / normb < fabs(eps)) { noconv = 0; break; }
+      if (it > nbitermx) break;
+    }
+    if (it > m - 1) it = m - 1;
+    for (int i = it; i >= 0; i--) { /* reconstruction */
+      g1[i] = g[i];
+      for (int j = i + 1; j < it + 1; j++) g1[i] = g1[i] - HN(i, j) * y[j];
+      y[i] = g1[i] / HN(i, i);
+    }
+    for (int k = 0; k < n; ++k) wi[k] = 0.;
+    for (int i = 0; i < it + 1; i++)
+      for (int k = 0; k < n; ++k) wi[k] += y[i] * Vp[N * i + k];
+    for (int k = 0; k < n; ++k) uni[k] = wi[k];
+    for (int k = 0; k < n; ++k) uni[k] += x0[k];
+    for (int k = 0; k < n; ++k) x[k] = uni[k];
+    if (!noconv) break;
+    if (iter > nbitermx) break;
+    nrestart++;
+  }
+#undef HN
+  if (stats) { stats[0] = iter; stats[1] = relres / normb; stats[2] = normb; stats[3] = nrestart; }
+  free(rot0); free(rot1); free(g); free(g1); free(Hn); free(uni); free(x0); free(ri); free(zi);
+  free(vi); free(wi); free(V); free(Vp); free(y); free(d1); free(wbc);
+  return !noconv;
+}

@@ -3,8 +3,8 @@
 //
 // TEST INFRASTRUCTURE ONLY: it is used by tests/ to pin the oracle against the
 // reference's compiled code (HashMatrix accumulation / CSR sort / SetBC /
-// gettgv / COO addMatMul, ConjugueGradient, the Tet_5 and T_5 tables, Tet
-// geometry).  It is never part of the product and never travels as a
+// gettgv / COO addMatMul, ConjugueGradient, fgmres, the Tet_5 and T_5
+// tables, Tet geometry).  It is never part of the product and never travels as a
 // dependency of libffhip.
 //
 // The reference pieces linked in: src/femlib/HashMatrix.cpp, CG.cpp,
@@ -212,6 +212,46 @@ int ref_hm_cg(void *h, const double *b, double *x, double eps, int itmax,
   int res = ConjugueGradient<int, double>(AA, C, bb.data(), x, itermx, epss, 0);
   if (stats) { stats[0] = itermx; stats[1] = epss; stats[3] = res; }
   return res;
+}
+
+// a HashMatrix holding a given CSR matrix (HashMatrix::set with CSR input,
+// HashMatrix.cpp:697-728, as a plugin hands a device matrix back)
+void *ref_hm_from_csr(int n, const int64_t *rowptr, const int32_t *colidx, const double *vals) {
+  verbosity = 0;
+  const size_t nnz = (size_t)rowptr[n];
+  std::vector<int> ia(n + 1), ja(colidx, colidx + nnz);
+  for (int i = 0; i <= n; ++i) ia[i] = (int)rowptr[i];
+  std::vector<double> aa(vals, vals + nnz);
+  HMat *A = new HMat(n, n, (int)nnz, 0);
+  A->set(n, n, 0, nnz, ia.data(), ja.data(), aa.data(), 0, 1);
+  return A;
+}
+
+// SolverGMRES::dosolver (VirtualSolverCG.hpp:236-255): SetInitWithBC, then the
+// reference fgmres with HMatVirt + the Jacobi diag1 of HMatVirtPrecon and its
+// tgv-row mask.
+int ref_hm_gmres(void *h, const double *b, double *x, double eps, int itmax, int restart,
+                 double tgv, double *stats) {
+  HMat &A = *(HMat *)h;
+  int ntgv1 = 0;
+  double tgvm = A.gettgv(&ntgv1);
+  std::vector<int> wcl;
+  if (ntgv1) {
+    double tgve = tgvm <= 0 ? 1e200 : tgvm;
+    wcl.resize(A.n);
+    for (int i = 0; i < A.n; ++i) wcl[i] = A(i, i) == tgve;
+    for (int i = 0; i < A.n; ++i)
+      if (wcl[i]) x[i] = b[i] / tgv;
+  }
+  HMatVirt AA(&A, 0);
+  Jacobi C(&A);
+  int itermx = itmax > 0 ? itmax : A.n;
+  double veps = eps;
+  std::vector<double> bb(b, b + A.n);
+  bool res = fgmres<double, int>(AA, C, 1, bb.data(), x, veps, itermx, restart, 0,
+                                 wcl.empty() ? nullptr : wcl.data());
+  if (stats) { stats[0] = itermx; stats[3] = res; }
+  return res ? 1 : 0;
 }
 
 // Mesh3 (Mesh3dn.cpp:1416) from arrays, owned by the returned object

@@ -400,6 +400,33 @@ __device__ inline void load_tet(const double *__restrict__ xyz, const int32_t *_
     for (int k = 0; k < 3; ++k) v[i][k] = xyz[3 * (int64_t)id[i] + k];
 }
 
+// the reaction term of int3d(m*u*v): K(i,j) += sum_q |K| w_q m(x_q) l_i(q) l_j(q)
+// for the 10 entries i <= j (P1 values l = barycentrics of the Tet_5 point)
+__device__ inline void p1_mass_3d(int64_t e, const double v[4][3], double mes, int mkind, double m0,
+                                  const double *__restrict__ mass_e, double K[10]) {
+  const double mp0 = (mkind == FFHIP_COEF_P0) ? mass_e[e] : m0;
+#pragma unroll 2
+  for (int q = 0; q < 14; ++q) {
+    const double px = c_qf5_pts[q][0], py = c_qf5_pts[q][1], pz = c_qf5_pts[q][2];
+    const double l[4] = {1. - (px + py + pz), px, py, pz};
+    double mq = mp0;
+    if (mkind == FFHIP_COEF_SIN3) {
+      double P[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) P[k] = l[0] * v[0][k] + px * v[1][k] + py * v[2][k] + pz * v[3][k];
+      mq = kappa_sin3(P[0], P[1], P[2]);
+    } else if (mkind == FFHIP_COEF_QP) {
+      mq = mass_e[14 * e + q];
+    }
+    const double W = (mes * qf5_weight(q)) * mq;
+    int o = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int j = i; j < 4; ++j) K[o++] += (W * l[i]) * l[j];
+  }
+}
+
 // entry (i, j), i <= j, of the P1 element matrix: sum_k (S G_ik) G_jk
 __device__ inline double p1_entry(double S, const double G[4][3], int i, int j) {
   double a = 0.;
@@ -413,9 +440,12 @@ __device__ inline double p1_entry(double S, const double G[4][3], int i, int j) 
 // so the row gather reads them contiguously; with Ke != null also the 10
 // unique entries per element (i <= j, row-major upper triangle) for the
 // lane-per-row gather
+// MASS: with the reaction term m*u*v (coefficient kind mkind, m0, mass_e)
+template <bool MASS>
 __global__ __launch_bounds__(256) void k_p1_element_3d(
     int64_t nt, const double *__restrict__ xyz, const int32_t *__restrict__ tets,
-    int kind, double c0, const double *__restrict__ kappa_e, const int32_t *__restrict__ inv,
+    int kind, double c0, const double *__restrict__ kappa_e, int mkind, double m0,
+    const double *__restrict__ mass_e, const int32_t *__restrict__ inv,
     double *__restrict__ Kinc, double *__restrict__ Ke) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nt) return;
@@ -429,6 +459,7 @@ __global__ __launch_bounds__(256) void k_p1_element_3d(
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = i; j < 4; ++j) K[o++] = p1_entry(S, G, i, j);
+  if (MASS) p1_mass_3d(e, v, mes, mkind, m0, mass_e, K);
   const int4 pos = reinterpret_cast<const int4 *>(inv)[e];
   const int32_t kk[4] = {pos.x, pos.y, pos.z, pos.w};
 #pragma unroll
@@ -448,7 +479,8 @@ __global__ __launch_bounds__(256) void k_p1_element_3d(
 // 2D P1: 6 unique entries; TTriangle area and H(i) (fem.hpp:290-325)
 __global__ __launch_bounds__(256) void k_p1_element_2d(
     int64_t nt, const double *__restrict__ xy, const int32_t *__restrict__ tris,
-    int kind, double c0, const double *__restrict__ kappa_e, double *__restrict__ Ke) {
+    int kind, double c0, const double *__restrict__ kappa_e, int mkind, double m0,
+    const double *__restrict__ mass_e, double *__restrict__ Ke) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nt) return;
   double v[3][2];
@@ -480,13 +512,34 @@ __global__ __launch_bounds__(256) void k_p1_element_2d(
     }
     S += (area * c_qt5_w[q]) * kc;
   }
+  double K[6];
   int o = 0;
   for (int i = 0; i < 3; ++i)
     for (int j = i; j < 3; ++j) {
       double a = 0.;
       for (int k = 0; k < 2; ++k) a += (S * G[i][k]) * G[j][k];
-      Ke[6 * e + o++] = a;
+      K[o++] = a;
     }
+  if (mkind >= 0) {  // the reaction term m*u*v with T_5
+    for (int q = 0; q < 7; ++q) {
+      const double px = c_qt5_pts[q][0], py = c_qt5_pts[q][1];
+      const double l[3] = {1. - px - py, px, py};
+      double mq = (mkind == FFHIP_COEF_P0) ? mass_e[e] : m0;
+      if (mkind == FFHIP_COEF_SIN3) {
+        const double X = l[0] * v[0][0] + px * v[1][0] + py * v[2][0];
+        const double Y = l[0] * v[0][1] + px * v[1][1] + py * v[2][1];
+        const double tp = 2 * M_PI;
+        mq = 1 + (0.5 * sin(tp * X)) * sin(tp * Y);
+      } else if (mkind == FFHIP_COEF_QP) {
+        mq = mass_e[7 * e + q];
+      }
+      const double W = (area * c_qt5_w[q]) * mq;
+      o = 0;
+      for (int i = 0; i < 3; ++i)
+        for (int j = i; j < 3; ++j) K[o++] += (W * l[i]) * l[j];
+    }
+  }
+  for (int i = 0; i < 6; ++i) Ke[6 * e + i] = K[i];
 }
 
 // ---------------------------------------------------------------- row gather
@@ -950,14 +1003,27 @@ int ffhip_mat_info(ffhip_mat A, int64_t *n, int64_t *nnz) {
 
 int ffhip_assemble_p1(ffhip_ctx ctx, ffhip_mesh m, ffhip_mat A, int kind, double c0,
                       const double *kappa_e, int kappa_on_device) {
+  return ffhip_assemble_p1_form(ctx, m, A, kind, c0, kappa_e, FFHIP_COEF_NONE, 0., nullptr,
+                                kappa_on_device);
+}
+
+int ffhip_assemble_p1_form(ffhip_ctx ctx, ffhip_mesh m, ffhip_mat A, int kind, double c0,
+                           const double *kappa_e, int mkind, double m0, const double *mass_e,
+                           int on_device) {
   return guard([&] {
     FFHIP_REQUIRE(ctx && m && A && A->mesh == m && A->slotmap, FFHIP_ERR_ARG,
                   "ffhip_assemble_p1: matrix pattern was not built from this mesh");
-    FFHIP_REQUIRE(kind == FFHIP_COEF_CONST || kind == FFHIP_COEF_SIN3 || kind == FFHIP_COEF_P0 ||
-                      kind == FFHIP_COEF_QP,
-                  FFHIP_ERR_ARG, "ffhip_assemble_p1: coefficient kind %d", kind);
+    auto valid = [](int k) {
+      return k == FFHIP_COEF_CONST || k == FFHIP_COEF_SIN3 || k == FFHIP_COEF_P0 || k == FFHIP_COEF_QP;
+    };
+    FFHIP_REQUIRE(valid(kind), FFHIP_ERR_ARG, "ffhip_assemble_p1: coefficient kind %d", kind);
+    FFHIP_REQUIRE(mkind == FFHIP_COEF_NONE || valid(mkind), FFHIP_ERR_ARG,
+                  "ffhip_assemble_p1_form: mass coefficient kind %d", mkind);
     const bool arr = kind == FFHIP_COEF_P0 || kind == FFHIP_COEF_QP;
+    const bool marr = mkind == FFHIP_COEF_P0 || mkind == FFHIP_COEF_QP;
     FFHIP_REQUIRE(!arr || kappa_e, FFHIP_ERR_ARG, "ffhip_assemble_p1: kappa_e");
+    FFHIP_REQUIRE(!marr || mass_e, FFHIP_ERR_ARG, "ffhip_assemble_p1_form: mass_e");
+    const int kappa_on_device = on_device;
     hipStream_t s = ctx->stream;
     const int nu = m->nloc == 4 ? 10 : 6;
     // 3D: the element rows in incidence order, and Ke only for the lane-per-row
@@ -966,23 +1032,37 @@ int ffhip_assemble_p1(ffhip_ctx ctx, ffhip_mesh m, ffhip_mat A, int kind, double
     Arena ar(ctx, (m->dim == 3 ? Arena::bytes(16 * m->nt, 8) : 0) +
                       (needKe ? Arena::bytes(nu * m->nt, 8) : 0));
     double *Ke = needKe ? ar.take<double>(nu * m->nt) : nullptr;
-    const double *kd = kappa_e;
-    double *kbuf = nullptr;
+    const double *kd = kappa_e, *md = mass_e;
+    double *kbuf = nullptr, *mbuf = nullptr;
+    auto count = [&](int k) { return k == FFHIP_COEF_P0 ? m->nt : m->nt * (m->dim == 3 ? 14 : 7); };
     if (arr && !kappa_on_device) {
-      const int64_t cnt = kind == FFHIP_COEF_P0 ? m->nt : m->nt * (m->dim == 3 ? 14 : 7);
+      const int64_t cnt = count(kind);
       kbuf = dalloc<double>(cnt);
       FFHIP_CHECK(hipMemcpyAsync(kbuf, kappa_e, 8 * cnt, hipMemcpyHostToDevice, s));
       kd = kbuf;
+    }
+    if (marr && !on_device) {
+      const int64_t cnt = count(mkind);
+      mbuf = dalloc<double>(cnt);
+      FFHIP_CHECK(hipMemcpyAsync(mbuf, mass_e, 8 * cnt, hipMemcpyHostToDevice, s));
+      md = mbuf;
     }
     if (m->dim == 2) upload_t5_once();
     ctx->tic();
     double *Kinc = nullptr;
     if (m->dim == 3) {
       Kinc = ar.take<double>(16 * m->nt);
-      k_p1_element_3d<<<nblocks(m->nt, 256), 256, 0, s>>>(m->nt, m->xyz, m->elems, kind, c0, kd,
-                                                           m->v2e_inv, Kinc, A->any_big ? Ke : nullptr);
+      if (mkind == FFHIP_COEF_NONE)
+        k_p1_element_3d<false><<<nblocks(m->nt, 256), 256, 0, s>>>(
+            m->nt, m->xyz, m->elems, kind, c0, kd, mkind, m0, md, m->v2e_inv, Kinc,
+            A->any_big ? Ke : nullptr);
+      else
+        k_p1_element_3d<true><<<nblocks(m->nt, 256), 256, 0, s>>>(
+            m->nt, m->xyz, m->elems, kind, c0, kd, mkind, m0, md, m->v2e_inv, Kinc,
+            A->any_big ? Ke : nullptr);
     } else
-      k_p1_element_2d<<<nblocks(m->nt, 256), 256, 0, s>>>(m->nt, m->xyz, m->elems, kind, c0, kd, Ke);
+      k_p1_element_2d<<<nblocks(m->nt, 256), 256, 0, s>>>(m->nt, m->xyz, m->elems, kind, c0, kd,
+                                                           mkind, m0, md, Ke);
     constexpr int B = 256, STAGE = 6144;
     if (m->nloc == 4) {
       k_p1_values_half<<<nblocks(A->n * kHalf, 256), 256, 0, s>>>(
@@ -997,6 +1077,7 @@ int ffhip_assemble_p1(ffhip_ctx ctx, ffhip_mesh m, ffhip_mat A, int kind, double
     FFHIP_CHECK(hipGetLastError());
     ctx->toc();
     dfree(kbuf);
+    dfree(mbuf);
   });
 }
 

@@ -23,7 +23,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.environ.get("FFHIP_LIB") or os.path.join(os.path.dirname(HERE), "libffhip.so")
 
-COEF_CONST, COEF_SIN3, COEF_P0, COEF_QP = 0, 1, 2, 3
+COEF_NONE, COEF_CONST, COEF_SIN3, COEF_P0, COEF_QP = -1, 0, 1, 2, 3
 CG_FP64, CG_MIXED = 0, 1
 CG_LAYOUT_AUTO, CG_LAYOUT_SELL, CG_LAYOUT_CSR, CG_LAYOUT_BSR3 = 0, 0x10, 0x20, 0x100
 CG_INDEX16, CG_INDEX32 = 0x40, 0x80
@@ -74,6 +74,7 @@ _SIGS = {
     "ffhip_pattern_p1": (_i, [_vp, _vp, _pvp]),
     "ffhip_mat_info": (_i, [_vp, _pl, _pl]),
     "ffhip_assemble_p1": (_i, [_vp, _vp, _vp, _i, _d, _vp, _i]),
+    "ffhip_assemble_p1_form": (_i, [_vp, _vp, _vp, _i, _d, _vp, _i, _d, _vp, _i]),
     "ffhip_set_bc": (_i, [_vp, _vp, _vp, _vp, _i, _d, _vp]),
     "ffhip_rhs_p1": (_i, [_vp, _vp, _d, _vp, _i, _d, _d, _vp, _i]),
     "ffhip_mat_download": (_i, [_vp, _vp, _vp, _vp]),
@@ -305,17 +306,32 @@ class Matrix:
                                      _hp(onbc)))
         return onbc if want_onbc else self.ctx.last_ms()
 
-    def assemble_p1(self, kind=COEF_SIN3, c0=1.0, kappa_e=None):
-        """kappa_e: numpy (host) or torch cuda tensor (device) of nt doubles"""
-        on_dev = 0
-        ptr = None
-        if kappa_e is not None:
-            if isinstance(kappa_e, np.ndarray):
-                kappa_e = np.ascontiguousarray(kappa_e, np.float64)
-                ptr = _hp(kappa_e)
-            else:
-                ptr, on_dev = _dp(kappa_e), 1
-        check(lib().ffhip_assemble_p1(self.ctx.h, self.mesh.h, self.h, kind, c0, ptr, on_dev))
+    def assemble_p1(self, kind=COEF_SIN3, c0=1.0, kappa_e=None, mkind=COEF_NONE, m0=0.0,
+                    mass_e=None):
+        """int3d(kappa grad u . grad v) [+ int3d(m u v) when mkind != COEF_NONE].
+        kappa_e / mass_e: numpy (host) or torch cuda tensors (device), nt
+        values (P0) or nt x nq (QP); both on the same side."""
+        arrs = [a for a in (kappa_e, mass_e) if a is not None]
+        on_dev = int(any(not isinstance(a, np.ndarray) for a in arrs))
+        assert all((not isinstance(a, np.ndarray)) == bool(on_dev) for a in arrs), \
+            "kappa_e and mass_e must both be host or both device arrays"
+
+        def ptr(a):
+            if a is None:
+                return None
+            if isinstance(a, np.ndarray):
+                return _hp(a)
+            return _dp(a)
+        if kappa_e is not None and isinstance(kappa_e, np.ndarray):
+            kappa_e = np.ascontiguousarray(kappa_e, np.float64)
+        if mass_e is not None and isinstance(mass_e, np.ndarray):
+            mass_e = np.ascontiguousarray(mass_e, np.float64)
+        if mkind == COEF_NONE:
+            check(lib().ffhip_assemble_p1(self.ctx.h, self.mesh.h, self.h, kind, c0, ptr(kappa_e),
+                                          on_dev))
+        else:
+            check(lib().ffhip_assemble_p1_form(self.ctx.h, self.mesh.h, self.h, kind, c0,
+                                               ptr(kappa_e), mkind, m0, ptr(mass_e), on_dev))
         return self.ctx.last_ms()
 
     def set_bc(self, labels, tgv=1e30, want_onbc=False):

@@ -18,6 +18,7 @@
 //    registered at src/fflib/lgfem.cpp:6669-6683; the highest pref wins,
 //    src/fflib/AFunction2.cpp:87-107).  It recognises
 //        int3d(Th)(c*(dx(u)*dx(v)+dy(u)*dy(v)+dz(u)*dz(v))) [+ on(labels,u=g)]
+//    [+ int3d(Th)(m*u*v)] (or the reaction term alone)
 //    on a scalar P1 space with the default Tet_5 rule; the coefficient c is
 //    evaluated by FreeFEM at the 14 quadrature points of every element (as
 //    Element_Op does, problem.cpp:6392-6395) unless it is a constant, and the
@@ -247,13 +248,19 @@ typedef Call_FormBilinear<v_fes3, v_fes3> CFB3;
 
 struct Recognised {
   const FormBilinear *bf = nullptr;
-  C_F0 coef;                  // the common coefficient of the three terms
+  bool has_grad = false;      // the three derivative terms
+  C_F0 coef;                  // their common coefficient
   bool coef_const = false;
   double c0 = 0.;
+  bool has_mass = false;      // a reaction term m*u*v
+  C_F0 mcoef;
+  bool mcoef_const = false;
+  double m0 = 0.;
   set<long> labels;           // Dirichlet labels (all BC_set of the form)
 };
 
-// int3d(Th)(c*(dx(u)*dx(v)+dy(u)*dy(v)+dz(u)*dz(v))) + on(...) on P1, or false
+// int3d(Th)(c*(dx(u)*dx(v)+dy(u)*dy(v)+dz(u)*dz(v)) + m*u*v) + on(...) on P1
+// (either part may be absent), or false
 bool recognise(Stack stack, const FESpace3 &Uh, const FESpace3 &Vh, const list<C_F0> &largs,
                Recognised &r) {
   if (&Uh != &Vh || Uh.N != 1 || Uh.TFE[0] != &DataFE<Mesh3>::P1) return false;
@@ -279,24 +286,40 @@ bool recognise(Stack stack, const FESpace3 &Uh, const FESpace3 &Vh, const list<C
     if (GetAny<const Mesh3 *>((*bf->di->Th)(stack)) != &Uh.Th) return false;
     if (bf->di->FIV(stack).n != QuadratureFormular_Tet_5.n) return false;  // default Tet_5 only
     const BilinearOperator &Op = *bf->b;
-    if (Op.v.size() != 3) return false;
     bool seen[4] = {false, false, false, false};
-    int k = 0;
-    for (BilinearOperator::const_iterator l = Op.v.begin(); l != Op.v.end(); ++l, ++k) {
+    int ngrad = 0;
+    for (BilinearOperator::const_iterator l = Op.v.begin(); l != Op.v.end(); ++l) {
       const BilinearOperator::K &ll = *l;
       const int jcomp = ll.first.first.first, jop = ll.first.first.second;
       const int icomp = ll.first.second.first, iop = ll.first.second.second;
-      if (jcomp || icomp || jop != iop || jop < op_dx || jop > op_dz || seen[jop]) return false;
+      if (jcomp || icomp || jop != iop || jop < op_id || jop > op_dz || seen[jop]) return false;
       seen[jop] = true;
-      if (k == 0) r.coef = ll.second;
-      else if (ll.second.LeftValue() != r.coef.LeftValue()) return false;
+      if (jop == op_id) {
+        r.has_mass = true;
+        r.mcoef = ll.second;
+      } else if (ngrad++ == 0) {
+        r.coef = ll.second;
+      } else if (ll.second.LeftValue() != r.coef.LeftValue()) {
+        return false;
+      }
     }
+    if (ngrad != 0 && ngrad != 3) return false;
+    r.has_grad = ngrad == 3;
+    if (!r.has_grad && !r.has_mass) return false;
     r.bf = bf;
   }
   if (!r.bf) return false;
-  if (r.coef.EvaluableWithOutStack()) {
+  if (r.has_grad && r.coef.EvaluableWithOutStack()) {
     r.coef_const = true;
     r.c0 = GetAny<double>(r.coef.eval(stack));
+  }
+  if (!r.has_grad) {  // no derivative terms: kappa = 0
+    r.coef_const = true;
+    r.c0 = 0.;
+  }
+  if (r.has_mass && r.mcoef.EvaluableWithOutStack()) {
+    r.mcoef_const = true;
+    r.m0 = GetAny<double>(r.mcoef.eval(stack));
   }
   return true;
 }
@@ -469,12 +492,15 @@ void assemble_on_gpu(Stack stack, const FESpace3 &Uh, const Recognised &r, doubl
   hip_check(ffhip_mesh3_upload(ctx, Th.nv, Th.nt, Th.nbe, xyz.data(), tets.data(), btri.data(),
                                blab.data(), &m), "ffhip: mesh upload");
   hip_check(ffhip_pattern_p1(ctx, m, &M), "ffhip: pattern");
-  if (r.coef_const) {
-    hip_check(ffhip_assemble_p1(ctx, m, M, FFHIP_COEF_CONST, r.c0, nullptr, 0), "ffhip: assembly");
-  } else {
-    vector<double> kq;
-    eval_coef_qp(stack, Uh, r.coef, kq);
-    hip_check(ffhip_assemble_p1(ctx, m, M, FFHIP_COEF_QP, 0., kq.data(), 0), "ffhip: assembly");
+  {
+    vector<double> kq, mq;
+    if (!r.coef_const) eval_coef_qp(stack, Uh, r.coef, kq);
+    if (r.has_mass && !r.mcoef_const) eval_coef_qp(stack, Uh, r.mcoef, mq);
+    const int kind = r.coef_const ? FFHIP_COEF_CONST : FFHIP_COEF_QP;
+    const int mkind = !r.has_mass ? FFHIP_COEF_NONE : (r.mcoef_const ? FFHIP_COEF_CONST : FFHIP_COEF_QP);
+    hip_check(ffhip_assemble_p1_form(ctx, m, M, kind, r.c0, kq.empty() ? nullptr : kq.data(), mkind,
+                                     r.m0, mq.empty() ? nullptr : mq.data(), 0),
+              "ffhip: assembly");
   }
   if (!r.labels.empty()) {
     vector<int32_t> labs(r.labels.begin(), r.labels.end());

@@ -78,7 +78,8 @@ int ffhip_mesh_free(ffhip_mesh mesh);
 int ffhip_pattern_p1(ffhip_ctx ctx, ffhip_mesh mesh, ffhip_mat *mat);
 int ffhip_mat_info(ffhip_mat mat, int64_t *n, int64_t *nnz);
 
-enum { FFHIP_COEF_CONST = 0, FFHIP_COEF_SIN3 = 1, FFHIP_COEF_P0 = 2, FFHIP_COEF_QP = 3 };
+enum { FFHIP_COEF_NONE = -1, FFHIP_COEF_CONST = 0, FFHIP_COEF_SIN3 = 1, FFHIP_COEF_P0 = 2,
+       FFHIP_COEF_QP = 3 };
 /* Values of int3d(Th)(kappa*(dx(u)*dx(v)+dy(u)*dy(v)+dz(u)*dz(v))) with the
  * default Tet_5 rule (src/fflib/problem.cpp:14102-14124), replacing
  * AssembleBilinearForm 3D (problem.cpp:1116-1417) + Element_Op
@@ -91,6 +92,16 @@ enum { FFHIP_COEF_CONST = 0, FFHIP_COEF_SIN3 = 1, FFHIP_COEF_P0 = 2, FFHIP_COEF_
  * when kappa_on_device != 0.  2D meshes: int2d with T_5, terms dx,dy. */
 int ffhip_assemble_p1(ffhip_ctx ctx, ffhip_mesh mesh, ffhip_mat mat, int kind,
                       double c0, const double *kappa_e, int kappa_on_device);
+/* Values of int3d(Th)(kappa*(dx(u)*dx(v)+dy(u)*dy(v)+dz(u)*dz(v)) + m*u*v)
+ * (int2d with dx, dy in 2D): the same Element_Op loop with a reaction term
+ * (op_id x op_id, problem.cpp:6384-6437; P1 values P012_3d.cpp:123-175), e.g.
+ * the u*v/dt of an implicit heat step.  m is described like kappa: mkind
+ * FFHIP_COEF_NONE (no reaction term: ffhip_assemble_p1), CONST (m0), SIN3,
+ * P0 (mass_e[element]) or QP (mass_e at the quadrature points).  kappa_e and
+ * mass_e are device pointers when on_device != 0, else host pointers. */
+int ffhip_assemble_p1_form(ffhip_ctx ctx, ffhip_mesh mesh, ffhip_mat mat, int kind, double c0,
+                           const double *kappa_e, int mkind, double m0, const double *mass_e,
+                           int on_device);
 /* on(labels..., u=g) with tgv > 0: AssembleBC (problem.cpp:10039-10194) then
  * HashMatrix::SetBC (HashMatrix.cpp:1194-1238): a_ii = tgv on the dofs of
  * boundary elements whose label is listed.  onbc (optional, host, n bytes)

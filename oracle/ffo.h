@@ -81,6 +81,22 @@ int ffo_assemble_p1_2d(int nv, int64_t nt, const double *xy,
                        const int32_t *tris, int kind, double c0,
                        const double *kappa_e, const int64_t *rowptr,
                        const int32_t *colidx, double *vals);
+/* int3d(Th)(kappa*(dx(u)*dx(v)+dy(u)*dy(v)+dz(u)*dz(v)) + m*u*v) (int2d in
+ * 2D): the Element_Op loop with the op_id x op_id term after the derivative
+ * terms at each quadrature point.  mkind < 0: no mass term (the functions
+ * above); else m(x) like kappa (CONST m0, SIN3, P0 mass_e[element]). */
+int ffo_p1_form_element_3d(const double *xyz, const int32_t *tet, int kind, double c0,
+                           double kappa_e, int mkind, double m0, double mass_e, double *Ke);
+int ffo_p1_form_element_2d(const double *xy, const int32_t *tri, int kind, double c0,
+                           double kappa_e, int mkind, double m0, double mass_e, double *Ke);
+int ffo_assemble_p1_form_3d(int nv, int64_t nt, const double *xyz, const int32_t *tets, int kind,
+                            double c0, const double *kappa_e, int mkind, double m0,
+                            const double *mass_e, const int64_t *rowptr, const int32_t *colidx,
+                            double *vals);
+int ffo_assemble_p1_form_2d(int nv, int64_t nt, const double *xy, const int32_t *tris, int kind,
+                            double c0, const double *kappa_e, int mkind, double m0,
+                            const double *mass_e, const int64_t *rowptr, const int32_t *colidx,
+                            double *vals);
 /* Element_rhs 3D (problem.cpp:7917-7985) for int3d(Th)(f*v), f constant,
  * accumulated into b in element order. */
 int ffo_rhs_p1_3d(int nv, int64_t nt, const double *xyz, const int32_t *tets,

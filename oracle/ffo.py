@@ -47,6 +47,10 @@ def lib():
             "ffo_p1_element_2d": (_i, [_f64p, _i32p, _i, _d, _d, _f64p]),
             "ffo_assemble_p1_3d": (_i, [_i, _l, _f64p, _i32p, _i, _d, _vp, _i64p, _i32p, _f64p]),
             "ffo_assemble_p1_2d": (_i, [_i, _l, _f64p, _i32p, _i, _d, _vp, _i64p, _i32p, _f64p]),
+            "ffo_p1_form_element_3d": (_i, [_f64p, _i32p, _i, _d, _d, _i, _d, _d, _f64p]),
+            "ffo_p1_form_element_2d": (_i, [_f64p, _i32p, _i, _d, _d, _i, _d, _d, _f64p]),
+            "ffo_assemble_p1_form_3d": (_i, [_i, _l, _f64p, _i32p, _i, _d, _vp, _i, _d, _vp, _i64p, _i32p, _f64p]),
+            "ffo_assemble_p1_form_2d": (_i, [_i, _l, _f64p, _i32p, _i, _d, _vp, _i, _d, _vp, _i64p, _i32p, _f64p]),
             "ffo_rhs_p1_3d": (_i, [_i, _l, _f64p, _i32p, _d, _f64p]),
             "ffo_rhs_p1_2d": (_i, [_i, _l, _f64p, _i32p, _d, _f64p]),
             "ffo_mark_bc": (_i, [_i, _l, _i, _i32p, _i32p, _i, _i32p, _u8p]),
@@ -165,20 +169,33 @@ def pattern_bruteforce(n, elems, nloc):
     return rowptr, colidx[:nnz].copy()
 
 
-def assemble_p1_3d(m, rowptr, colidx, kind=COEF_SIN3, c0=1.0, kappa_e=None):
+def assemble_p1_3d(m, rowptr, colidx, kind=COEF_SIN3, c0=1.0, kappa_e=None, mkind=-1, m0=0.0,
+                   mass_e=None):
+    """int3d(kappa grad u . grad v [+ m u v]); mkind < 0: no mass term"""
     vals = np.zeros(colidx.size)
     ke = None if kappa_e is None else np.ascontiguousarray(kappa_e, np.float64)
-    rc = lib().ffo_assemble_p1_3d(m["nv"], m["nt"], m["xyz"], m["tets"], kind, c0,
-                                  _ptr(ke), rowptr, colidx, vals)
+    me = None if mass_e is None else np.ascontiguousarray(mass_e, np.float64)
+    if mkind < 0:
+        rc = lib().ffo_assemble_p1_3d(m["nv"], m["nt"], m["xyz"], m["tets"], kind, c0,
+                                      _ptr(ke), rowptr, colidx, vals)
+    else:
+        rc = lib().ffo_assemble_p1_form_3d(m["nv"], m["nt"], m["xyz"], m["tets"], kind, c0, _ptr(ke),
+                                           mkind, m0, _ptr(me), rowptr, colidx, vals)
     assert rc == 0, rc
     return vals
 
 
-def assemble_p1_2d(m, rowptr, colidx, kind=COEF_CONST, c0=1.0, kappa_e=None):
+def assemble_p1_2d(m, rowptr, colidx, kind=COEF_CONST, c0=1.0, kappa_e=None, mkind=-1, m0=0.0,
+                   mass_e=None):
     vals = np.zeros(colidx.size)
     ke = None if kappa_e is None else np.ascontiguousarray(kappa_e, np.float64)
-    rc = lib().ffo_assemble_p1_2d(m["nv"], m["nt"], m["xy"], m["tris"], kind, c0,
-                                  _ptr(ke), rowptr, colidx, vals)
+    me = None if mass_e is None else np.ascontiguousarray(mass_e, np.float64)
+    if mkind < 0:
+        rc = lib().ffo_assemble_p1_2d(m["nv"], m["nt"], m["xy"], m["tris"], kind, c0,
+                                      _ptr(ke), rowptr, colidx, vals)
+    else:
+        rc = lib().ffo_assemble_p1_form_2d(m["nv"], m["nt"], m["xy"], m["tris"], kind, c0, _ptr(ke),
+                                           mkind, m0, _ptr(me), rowptr, colidx, vals)
     assert rc == 0, rc
     return vals
 

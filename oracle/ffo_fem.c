@@ -110,6 +110,14 @@ static double coef_eval(int kind, double c0, double ke, const double *P, int dim
  * (GenericMesh.hpp:372-377). */
 int ffo_p1_element_3d(const double *xyz, const int32_t *tet, int kind,
                       double c0, double kappa_e, double *Ke) {
+  return ffo_p1_form_element_3d(xyz, tet, kind, c0, kappa_e, -1, 0., 0., Ke);
+}
+
+/* The same Element_Op loop with a fourth term m*u*v (op_id x op_id, P1
+ * values l_i = barycentrics of the point, P012_3d.cpp:123-175) after the
+ * three derivative terms at each quadrature point; mkind < 0: no such term. */
+int ffo_p1_form_element_3d(const double *xyz, const int32_t *tet, int kind, double c0,
+                           double kappa_e, int mkind, double m0, double mass_e, double *Ke) {
   double v[12], qp[42], qw[14], G[12];
   for (int i = 0; i < 4; ++i)
     for (int k = 0; k < 3; ++k) v[3 * i + k] = xyz[3 * (int64_t)tet[i] + k];
@@ -133,6 +141,13 @@ int ffo_p1_element_3d(const double *xyz, const int32_t *tet, int kind,
         for (int j = 0; j < 4; ++j)
           Ke[4 * i + j] += ccc * G[3 * i + t] * G[3 * j + t];
     }
+    if (mkind >= 0) {
+      const double l[4] = {l0, Ph[0], Ph[1], Ph[2]};
+      double ccc = coef_eval(mkind, m0, mass_e, P, 3);
+      ccc *= coef;
+      for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) Ke[4 * i + j] += ccc * l[i] * l[j];
+    }
   }
   return 0;
 }
@@ -141,6 +156,11 @@ int ffo_p1_element_3d(const double *xyz, const int32_t *tet, int kind,
  * Element_Op 2D with terms dx*dx, dy*dy and QF T_5. */
 int ffo_p1_element_2d(const double *xy, const int32_t *tri, int kind,
                       double c0, double kappa_e, double *Ke) {
+  return ffo_p1_form_element_2d(xy, tri, kind, c0, kappa_e, -1, 0., 0., Ke);
+}
+
+int ffo_p1_form_element_2d(const double *xy, const int32_t *tri, int kind, double c0,
+                           double kappa_e, int mkind, double m0, double mass_e, double *Ke) {
   double v[6], qp[14], qw[7], G[6];
   for (int i = 0; i < 3; ++i)
     for (int k = 0; k < 2; ++k) v[2 * i + k] = xy[2 * (int64_t)tri[i] + k];
@@ -168,6 +188,13 @@ int ffo_p1_element_2d(const double *xy, const int32_t *tri, int kind,
       for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j)
           Ke[3 * i + j] += ccc * G[2 * i + t] * G[2 * j + t];
+    }
+    if (mkind >= 0) {
+      const double l[3] = {l0, Ph[0], Ph[1]};
+      double ccc = coef_eval(mkind, m0, mass_e, P, 2);
+      ccc *= coef;
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) Ke[3 * i + j] += ccc * l[i] * l[j];
     }
   }
   return 0;
@@ -285,6 +312,7 @@ static int64_t find_slot(const int64_t *rowptr, const int32_t *colidx, int r, in
 /* ---- assembly ----------------------------------------------------------- */
 static int assemble(int nv, int64_t nel, int nloc, int dim, const double *X,
                     const int32_t *el, int kind, double c0, const double *kappa_e,
+                    int mkind, double m0, const double *mass_e,
                     const int64_t *rowptr, const int32_t *colidx, double *vals) {
   (void)nv;
   memset(vals, 0, sizeof(double) * (size_t)rowptr[nv]);
@@ -292,8 +320,9 @@ static int assemble(int nv, int64_t nel, int nloc, int dim, const double *X,
   for (int64_t e = 0; e < nel; ++e) {
     const int32_t *ni = el + e * nloc;
     const double ke = kappa_e ? kappa_e[e] : 0.;
-    if (dim == 3) ffo_p1_element_3d(X, ni, kind, c0, ke, Ke);
-    else ffo_p1_element_2d(X, ni, kind, c0, ke, Ke);
+    const double me = mass_e ? mass_e[e] : 0.;
+    if (dim == 3) ffo_p1_form_element_3d(X, ni, kind, c0, ke, mkind, m0, me, Ke);
+    else ffo_p1_form_element_2d(X, ni, kind, c0, ke, mkind, m0, me, Ke);
     /* HashMatrix::operator+= full case: il outer, jl inner (HashMatrix.cpp:1311-1317) */
     for (int il = 0; il < nloc; ++il)
       for (int jl = 0; jl < nloc; ++jl) {
@@ -309,14 +338,30 @@ int ffo_assemble_p1_3d(int nv, int64_t nt, const double *xyz,
                        const int32_t *tets, int kind, double c0,
                        const double *kappa_e, const int64_t *rowptr,
                        const int32_t *colidx, double *vals) {
-  return assemble(nv, nt, 4, 3, xyz, tets, kind, c0, kappa_e, rowptr, colidx, vals);
+  return assemble(nv, nt, 4, 3, xyz, tets, kind, c0, kappa_e, -1, 0., NULL, rowptr, colidx, vals);
+}
+
+int ffo_assemble_p1_form_3d(int nv, int64_t nt, const double *xyz, const int32_t *tets, int kind,
+                            double c0, const double *kappa_e, int mkind, double m0,
+                            const double *mass_e, const int64_t *rowptr, const int32_t *colidx,
+                            double *vals) {
+  return assemble(nv, nt, 4, 3, xyz, tets, kind, c0, kappa_e, mkind, m0, mass_e, rowptr, colidx,
+                  vals);
 }
 
 int ffo_assemble_p1_2d(int nv, int64_t nt, const double *xy,
                        const int32_t *tris, int kind, double c0,
                        const double *kappa_e, const int64_t *rowptr,
                        const int32_t *colidx, double *vals) {
-  return assemble(nv, nt, 3, 2, xy, tris, kind, c0, kappa_e, rowptr, colidx, vals);
+  return assemble(nv, nt, 3, 2, xy, tris, kind, c0, kappa_e, -1, 0., NULL, rowptr, colidx, vals);
+}
+
+int ffo_assemble_p1_form_2d(int nv, int64_t nt, const double *xy, const int32_t *tris, int kind,
+                            double c0, const double *kappa_e, int mkind, double m0,
+                            const double *mass_e, const int64_t *rowptr, const int32_t *colidx,
+                            double *vals) {
+  return assemble(nv, nt, 3, 2, xy, tris, kind, c0, kappa_e, mkind, m0, mass_e, rowptr, colidx,
+                  vals);
 }
 
 /* Element_rhs 3D (problem.cpp:7940-7978): B[Kv(i)] += coef*c*w_i per point. */

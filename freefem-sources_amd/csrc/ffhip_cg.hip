@@ -1337,7 +1337,8 @@ static void cg_plan_layout(ffhip_cg cg, int layout, bool index32) {
   ffhip_mat A = cg->A;
   hipStream_t s = ctx->stream;
   cg->S = (A->n + kSlice - 1) / kSlice;
-  if (A->nodeptr && (layout == FFHIP_CG_LAYOUT_AUTO || layout == FFHIP_CG_LAYOUT_BSR3)) {
+  if (A->nodeptr && A->ncomp == 3 &&
+      (layout == FFHIP_CG_LAYOUT_AUTO || layout == FFHIP_CG_LAYOUT_BSR3)) {
     // vector P2: the node rows are 3x3 blocks; slices over node rows sorted by
     // length inside windows of 4096 rows
     cg->fmt = kBsr3;

@@ -127,6 +127,7 @@ struct ffhip_mat_s {
   // hold 3 x 3 blocks of its node columns) and the byte slot map, 10 offsets
   // per node incidence
   ffhip_p2v p2v = nullptr;
+  int ncomp = 0;               // P2 matrices: 3 for [P2,P2,P2], 1 for scalar P2
   int64_t *nodeptr = nullptr;  // nnodes + 1
   uint8_t *slot8 = nullptr;    // (nt * 10) * 10
 };

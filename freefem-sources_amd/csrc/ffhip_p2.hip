@@ -274,6 +274,24 @@ __global__ void k_p2v_expand(int64_t nn, const int64_t *__restrict__ nodeptr,
   diag[r] = d < 0 ? -1 : base + 3 * d + c;
 }
 
+// scalar P2: the node rows are the matrix rows (columns from ncol at nodeptr,
+// or from the row buffers when rowbuf != null)
+__global__ void k_p2_compact(int64_t nn, const int64_t *__restrict__ nodeptr,
+                             const int32_t *__restrict__ ncol, const int64_t *__restrict__ ndiag,
+                             const int32_t *__restrict__ rowbuf, const int8_t *__restrict__ dposr,
+                             int64_t *__restrict__ rowptr, int32_t *__restrict__ colidx,
+                             int64_t *__restrict__ diag) {
+  const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= nn) return;
+  const int64_t p0 = nodeptr[a], L = nodeptr[a + 1] - p0;
+  rowptr[a] = p0;
+  if (a == nn - 1) rowptr[nn] = nodeptr[nn];
+  if (rowbuf)
+    for (int64_t q = 0; q < L; ++q) colidx[p0 + q] = rowbuf[kNodeCap * a + q];
+  const int64_t d = rowbuf ? (dposr[a] < 0 ? -1 : dposr[a]) : (ndiag[a] < 0 ? -1 : ndiag[a] - p0);
+  diag[a] = d < 0 ? -1 : p0 + d;
+}
+
 // ---------------------------------------------------------------- element matrices
 typedef double d4 __attribute__((ext_vector_type(4)));
 
@@ -300,11 +318,16 @@ __device__ inline double p2_grad(const P2Row &R, const double l[4]) {
   return 4 * (R.g1 * pick4(l, R.i0) + R.g0 * pick4(l, R.i1));
 }
 
-// K[(c,i),(c',j)] (row = test function) as 10 node blocks of 90 doubles
-// [c][j][c'], block i stored at Ke[90 k] for the incidence k of (e, i)
-__global__ __launch_bounds__(kElemBlock) void k_p2v_elast_element(
+// LAP = false: K[(c,i),(c',j)] (row = test function) of the Lame form as 10
+// node blocks of 90 doubles [c][j][c'], block i stored at Ke[90 k] for the
+// incidence k of (e, i).  LAP = true: the scalar P2 stiffness of
+// int3d(kappa grad u . grad v), K(i,j) = sum_a M[(i,a),(j,a)] with kappa(x_q)
+// in the weights, as 10 rows of 10 doubles at Ke[10 k].
+template <bool LAP>
+__global__ __launch_bounds__(kElemBlock) void k_p2_element(
     int64_t nt, const double *__restrict__ xyz, const int32_t *__restrict__ tets, double lambda,
-    double mu, const int32_t *__restrict__ inv, double *__restrict__ Ke) {
+    double mu, int kind, double c0, const double *__restrict__ kappa_e,
+    const int32_t *__restrict__ inv, double *__restrict__ Ke) {
   __shared__ double sM[kElemWaves][32 * kMs];
   const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const int col = lane % 16, kq = lane / 16;
@@ -353,6 +376,19 @@ __global__ __launch_bounds__(kElemBlock) void k_p2v_elast_element(
           const double px = c_qf5_pts[q][0], py = c_qf5_pts[q][1], pz = c_qf5_pts[q][2];
           l[0] = 1. - (px + py + pz); l[1] = px; l[2] = py; l[3] = pz;
           W = mes * qf5_weight(q);
+          if (LAP) {
+            double kq = c0;
+            if (kind == FFHIP_COEF_P0) kq = kappa_e[e];
+            else if (kind == FFHIP_COEF_QP) kq = kappa_e[14 * e + q];
+            else if (kind == FFHIP_COEF_SIN3) {
+              double P[3];
+#pragma unroll
+              for (int k = 0; k < 3; ++k)
+                P[k] = l[0] * v[0][k] + px * v[1][k] + py * v[2][k] + pz * v[3][k];
+              kq = kappa_sin3(P[0], P[1], P[2]);
+            }
+            W *= kq;
+          }
         }
         const double g0 = rvalid[0] ? p2_grad(R[0], l) : 0.;
         const double g1 = rvalid[1] ? p2_grad(R[1], l) : 0.;
@@ -372,7 +408,14 @@ __global__ __launch_bounds__(kElemBlock) void k_p2v_elast_element(
       }
     }
     __syncthreads();
-    if (e < nt) {
+    if (LAP && e < nt) {
+      for (int o = lane; o < 100; o += kWave) {
+        const int i = o / 10, j = o % 10;
+        const double k = M[(3 * i) * kMs + 3 * j] + M[(3 * i + 1) * kMs + 3 * j + 1] +
+                         M[(3 * i + 2) * kMs + 3 * j + 2];
+        Ke[(int64_t)inv[10 * e + i] * 10 + j] = k;
+      }
+    } else if (e < nt) {
       for (int o = lane; o < 900; o += kWave) {
         const int i = o / 90, rem = o % 90, c = rem / 30, r2 = rem % 30, j = r2 / 3, cp = r2 % 3;
         double k = lambda * M[(3 * i + c) * kMs + 3 * j + cp] + mu * M[(3 * i + cp) * kMs + 3 * j + c];
@@ -444,6 +487,40 @@ __global__ __launch_bounds__(256) void k_p2v_gather(
   for (int t = lane; t < Wd; t += kWave) out[t] = acc[t];
 }
 
+// scalar P2: one wave per node row; the row's incident element rows (10
+// values each, element order) added slot by slot in LDS
+__global__ __launch_bounds__(256) void k_p2_gather(int64_t nn, const int64_t *__restrict__ nodeptr,
+                                                   const int64_t *__restrict__ n2e_ptr,
+                                                   const uint8_t *__restrict__ slot8,
+                                                   const double *__restrict__ Ke,
+                                                   double *__restrict__ vals) {
+  __shared__ double S[256 / kWave][kNodeCap];
+  const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int64_t a = (int64_t)blockIdx.x * (256 / kWave) + w;
+  if (a >= nn) return;  // wave-uniform; the kernel has no block barrier
+  double *acc = S[w];
+  const int64_t p0 = nodeptr[a];
+  const int L = (int)(nodeptr[a + 1] - p0);
+  for (int t = lane; t < L; t += kWave) acc[t] = 0.;
+  wave_lds_sync();
+  const int64_t k0 = n2e_ptr[a], k1 = n2e_ptr[a + 1];
+  // six incidences per step: lane = 10 u + j takes value j of incidence k + u;
+  // the steps keep element order, and within a step the incidences are added
+  // one after the other
+  const int u = lane / 10, j = lane % 10;
+  for (int64_t k = k0; k < k1; k += 6) {
+    const bool mine = u < 6 && k + u < k1;
+    double v = 0.;
+    int o = 0;
+    if (mine) { v = Ke[(k + u) * 10 + j]; o = slot8[(k + u) * 10 + j]; }
+    for (int uu = 0; uu < 6; ++uu) {
+      if (mine && u == uu) acc[o] += v;
+      wave_lds_sync();
+    }
+  }
+  for (int t = lane; t < L; t += kWave) vals[p0 + t] = acc[t];
+}
+
 // ---------------------------------------------------------------- Dirichlet
 __device__ inline int32_t find_edge_node(const uint64_t *__restrict__ ekeys,
                                          const int32_t *__restrict__ enode, int64_t ne, uint64_t key) {
@@ -463,7 +540,8 @@ __global__ void k_p2v_mark_bc(int64_t nbe, const int32_t *__restrict__ btri,
                               const int32_t *__restrict__ blab, const int32_t *__restrict__ labs,
                               int nlab, const int32_t *__restrict__ vnode,
                               const uint64_t *__restrict__ ekeys, const int32_t *__restrict__ enode,
-                              int64_t ne, uint8_t *__restrict__ onbc, int *__restrict__ missing) {
+                              int64_t ne, int ncomp, uint8_t *__restrict__ onbc,
+                              int *__restrict__ missing) {
   const int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (f >= nbe) return;
   int on = 0;
@@ -476,7 +554,7 @@ __global__ void k_p2v_mark_bc(int64_t nbe, const int32_t *__restrict__ btri,
                       find_edge_node(ekeys, enode, ne, edge_key(t[0], t[2]))};
   for (int i = 0; i < 6; ++i) {
     if (nodes[i] < 0) { atomicMax(missing, 1); continue; }
-    for (int c = 0; c < 3; ++c) onbc[3 * (int64_t)nodes[i] + c] = 1;
+    for (int c = 0; c < ncomp; ++c) onbc[ncomp * (int64_t)nodes[i] + c] = 1;
   }
 }
 
@@ -486,8 +564,9 @@ __global__ void k_p2v_mark_bc(int64_t nbe, const int32_t *__restrict__ btri,
 // Dirichlet rows
 __global__ void k_p2v_rhs(int64_t nn, const double *__restrict__ xyz, const int32_t *__restrict__ tets,
                           const int64_t *__restrict__ n2e_ptr, const int32_t *__restrict__ n2e,
-                          double f0, double f1, double f2, const uint8_t *__restrict__ onbc,
-                          double tgv, double g0, double g1, double g2, double *__restrict__ b) {
+                          int ncomp, double f0, double f1, double f2,
+                          const uint8_t *__restrict__ onbc, double tgv, double g0, double g1,
+                          double g2, double *__restrict__ b) {
 #pragma clang fp contract(off)
   const int64_t a = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= nn) return;
@@ -508,26 +587,28 @@ __global__ void k_p2v_rhs(int64_t nn, const double *__restrict__ xyz, const int3
       const double l[4] = {1. - (px + py + pz), px, py, pz};
       const double coef = mes * qf5_weight(q);
       const double phi = i < 4 ? l[i] * (2 * l[i] - 1.) : 4. * l[i0] * l[i1];
-      for (int c = 0; c < 3; ++c) acc[c] += coef * fv[c] * phi;
+      for (int c = 0; c < ncomp; ++c) acc[c] += coef * fv[c] * phi;
     }
   }
   const double g[3] = {g0, g1, g2};
-  for (int c = 0; c < 3; ++c) b[3 * a + c] = onbc[3 * a + c] ? tgv * g[c] : acc[c];
+  for (int c = 0; c < ncomp; ++c)
+    b[ncomp * a + c] = onbc[ncomp * a + c] ? tgv * g[c] : acc[c];
 }
 
 // onbc (3 nnodes bytes) of the labelled boundary; false if a node was missing
-static bool p2v_mark(ffhip_ctx ctx, ffhip_p2v sp, const int32_t *labels, int nlab, uint8_t *onbc) {
+static bool p2v_mark(ffhip_ctx ctx, ffhip_p2v sp, const int32_t *labels, int nlab, int ncomp,
+                     uint8_t *onbc) {
   hipStream_t s = ctx->stream;
   ffhip_mesh m = sp->mesh;
   int32_t *labs = dalloc<int32_t>(nlab > 0 ? nlab : 1);
   int *missing = dalloc<int>(1);
   if (nlab) FFHIP_CHECK(hipMemcpyAsync(labs, labels, 4 * nlab, hipMemcpyHostToDevice, s));
-  FFHIP_CHECK(hipMemsetAsync(onbc, 0, 3 * sp->nnodes, s));
+  FFHIP_CHECK(hipMemsetAsync(onbc, 0, ncomp * sp->nnodes, s));
   FFHIP_CHECK(hipMemsetAsync(missing, 0, sizeof(int), s));
   if (m->nbe)
     k_p2v_mark_bc<<<nblk(m->nbe, 256), 256, 0, s>>>(m->nbe, m->belems, m->blab, labs, nlab,
-                                                    sp->vnode, sp->ekeys, sp->enode, sp->ne, onbc,
-                                                    missing);
+                                                    sp->vnode, sp->ekeys, sp->enode, sp->ne, ncomp,
+                                                    onbc, missing);
   FFHIP_CHECK(hipGetLastError());
   const int miss = read_flag(ctx, missing);
   dfree(labs);
@@ -638,15 +719,19 @@ int ffhip_p2v_free(ffhip_p2v sp) {
   });
 }
 
-int ffhip_pattern_p2v(ffhip_ctx ctx, ffhip_p2v sp, ffhip_mat *out) {
-  return guard([&] {
-    FFHIP_REQUIRE(ctx && sp && out, FFHIP_ERR_ARG, "ffhip_pattern_p2v: null argument");
+}  // extern "C"
+
+namespace ffhip {
+// the node pattern of the P2 space (one wave per node row, the lane-per-row
+// merge if a row does not fit), then the matrix rows: 3 x 3 blocks of the node
+// columns (ncomp = 3) or the node rows themselves (ncomp = 1)
+static void p2_pattern(ffhip_ctx ctx, ffhip_p2v sp, int ncomp, ffhip_mat *out) {
     hipStream_t s = ctx->stream;
     ffhip_mesh m = sp->mesh;
-    const int64_t nn = sp->nnodes, n = 3 * nn;
-    FFHIP_REQUIRE(n < INT_MAX, FFHIP_ERR_UNSUPPORTED, "ffhip_pattern_p2v: too many dofs");
+    const int64_t nn = sp->nnodes, n = ncomp * nn;
+    FFHIP_REQUIRE(n < INT_MAX, FFHIP_ERR_UNSUPPORTED, "ffhip_pattern_p2: too many dofs");
     ffhip_mat A = new ffhip_mat_s();
-    A->ctx = ctx; A->n = n; A->p2v = sp;
+    A->ctx = ctx; A->n = n; A->p2v = sp; A->ncomp = ncomp;
     try {
       Arena ar(ctx, Arena::bytes(nn + 1, 8) + Arena::bytes(1, 4) + Arena::bytes(nn, 8) +
                         Arena::bytes(kNodeCap * nn, 4) + Arena::bytes(nn, 1));
@@ -686,17 +771,26 @@ int ffhip_pattern_p2v(ffhip_ctx ctx, ffhip_p2v sp, ffhip_mat *out) {
             nn, 10, sp->elem_nodes, sp->n2e_ptr, sp->n2e, nullptr, A->nodeptr, ncol, ndiag, nullptr,
             A->slot8, ovf);
       }
-      A->nnz = 9 * nnz_nodes;
+      A->nnz = (int64_t)ncomp * ncomp * nnz_nodes;
       A->rowptr = dalloc<int64_t>(n + 1);
       A->colidx = dalloc<int32_t>(A->nnz);
       A->diag = dalloc<int64_t>(n);
       A->vals = dalloc<double>(A->nnz);
-      k_p2v_expand<<<nblk(n, 256), 256, 0, s>>>(nn, A->nodeptr, ncol, ndiag, wave_ok ? rowbuf : nullptr,
-                                                dposr, A->rowptr, A->colidx, A->diag);
+      if (ncomp == 3)
+        k_p2v_expand<<<nblk(n, 256), 256, 0, s>>>(nn, A->nodeptr, ncol, ndiag,
+                                                  wave_ok ? rowbuf : nullptr, dposr, A->rowptr,
+                                                  A->colidx, A->diag);
+      else {
+        if (!wave_ok)  // the merge wrote the columns at their CSR positions
+          FFHIP_CHECK(hipMemcpyAsync(A->colidx, ncol, 4 * nnz_nodes, hipMemcpyDeviceToDevice, s));
+        k_p2_compact<<<nblk(nn, 256), 256, 0, s>>>(nn, A->nodeptr, ncol, ndiag,
+                                                   wave_ok ? rowbuf : nullptr, dposr, A->rowptr,
+                                                   A->colidx, A->diag);
+      }
       FFHIP_CHECK(hipMemsetAsync(A->vals, 0, 8 * A->nnz, s));
       FFHIP_CHECK(hipGetLastError());
       ctx->toc();
-      A->max_row = 3 * CAP;
+      A->max_row = ncomp * CAP;
     } catch (...) {
       (void)hipStreamSynchronize(s);
       dfree(A->rowptr); dfree(A->colidx); dfree(A->vals); dfree(A->diag);
@@ -705,13 +799,29 @@ int ffhip_pattern_p2v(ffhip_ctx ctx, ffhip_p2v sp, ffhip_mat *out) {
       throw;
     }
     *out = A;
+}
+}  // namespace ffhip
+
+extern "C" {
+
+int ffhip_pattern_p2v(ffhip_ctx ctx, ffhip_p2v sp, ffhip_mat *out) {
+  return guard([&] {
+    FFHIP_REQUIRE(ctx && sp && out, FFHIP_ERR_ARG, "ffhip_pattern_p2v: null argument");
+    p2_pattern(ctx, sp, 3, out);
+  });
+}
+
+int ffhip_pattern_p2(ffhip_ctx ctx, ffhip_p2v sp, ffhip_mat *out) {
+  return guard([&] {
+    FFHIP_REQUIRE(ctx && sp && out, FFHIP_ERR_ARG, "ffhip_pattern_p2: null argument");
+    p2_pattern(ctx, sp, 1, out);
   });
 }
 
 int ffhip_assemble_p2v_elasticity(ffhip_ctx ctx, ffhip_p2v sp, ffhip_mat A, double lambda,
                                   double mu) {
   return guard([&] {
-    FFHIP_REQUIRE(ctx && sp && A && A->p2v == sp && A->slot8, FFHIP_ERR_ARG,
+    FFHIP_REQUIRE(ctx && sp && A && A->p2v == sp && A->slot8 && A->ncomp == 3, FFHIP_ERR_ARG,
                   "ffhip_assemble_p2v_elasticity: matrix pattern was not built from this space");
     hipStream_t s = ctx->stream;
     ffhip_mesh m = sp->mesh;
@@ -721,8 +831,8 @@ int ffhip_assemble_p2v_elasticity(ffhip_ctx ctx, ffhip_p2v sp, ffhip_mat A, doub
     const int64_t gmax = 8 * (int64_t)ctx->num_cu;
     if (g > gmax) g = gmax;
     ctx->tic();
-    k_p2v_elast_element<<<(unsigned)g, kElemBlock, 0, s>>>(m->nt, m->xyz, m->elems, lambda, mu,
-                                                          sp->n2e_inv, Ke);
+    k_p2_element<false><<<(unsigned)g, kElemBlock, 0, s>>>(m->nt, m->xyz, m->elems, lambda, mu, 0,
+                                                          0., nullptr, sp->n2e_inv, Ke);
     k_p2v_gather<<<nblk(sp->nnodes, 256 / kWave), 256, 0, s>>>(sp->nnodes, A->nodeptr, sp->n2e_ptr,
                                                               A->slot8, Ke, A->vals);
     FFHIP_CHECK(hipGetLastError());
@@ -743,7 +853,7 @@ int ffhip_set_bc_p2v(ffhip_ctx ctx, ffhip_p2v sp, ffhip_mat A, const int32_t *la
     int miss = 0;
     try {
       ctx->tic();
-      ok = p2v_mark(ctx, sp, labels, nlab, onbc);
+      ok = p2v_mark(ctx, sp, labels, nlab, 3, onbc);
       FFHIP_CHECK(hipMemsetAsync(missing, 0, sizeof(int), ctx->stream));
       set_diag_rows(ctx, A->n, onbc, A->diag, tgv, A->vals, missing);
       ctx->toc();
@@ -776,11 +886,11 @@ int ffhip_rhs_p2v(ffhip_ctx ctx, ffhip_p2v sp, const double f[3], const int32_t 
     double *bd = on_device ? b : nullptr;
     bool ok = false;
     try {
-      ok = p2v_mark(ctx, sp, labels, nlab, onbc);
+      ok = p2v_mark(ctx, sp, labels, nlab, 3, onbc);
       if (!on_device) bd = dalloc<double>(n);
       ctx->tic();
       k_p2v_rhs<<<nblk(sp->nnodes, 256), 256, 0, s>>>(sp->nnodes, m->xyz, m->elems, sp->n2e_ptr,
-                                                      sp->n2e, f[0], f[1], f[2], onbc, tgv, gg[0],
+                                                      sp->n2e, 3, f[0], f[1], f[2], onbc, tgv, gg[0],
                                                       gg[1], gg[2], bd);
       FFHIP_CHECK(hipGetLastError());
       ctx->toc();
@@ -797,6 +907,107 @@ int ffhip_rhs_p2v(ffhip_ctx ctx, ffhip_p2v sp, const double f[3], const int32_t 
     dfree(onbc);
     if (!on_device) dfree(bd);
     FFHIP_REQUIRE(ok, FFHIP_ERR_INTERNAL, "ffhip_rhs_p2v: a boundary edge is no element edge");
+  });
+}
+
+// ---- scalar P2 (Vh(Th,P2), dof = node)
+int ffhip_assemble_p2(ffhip_ctx ctx, ffhip_p2v sp, ffhip_mat A, int kind, double c0,
+                      const double *kappa_e, int kappa_on_device) {
+  return guard([&] {
+    FFHIP_REQUIRE(ctx && sp && A && A->p2v == sp && A->slot8 && A->ncomp == 1, FFHIP_ERR_ARG,
+                  "ffhip_assemble_p2: matrix pattern was not built by ffhip_pattern_p2 from this space");
+    FFHIP_REQUIRE(kind == FFHIP_COEF_CONST || kind == FFHIP_COEF_SIN3 || kind == FFHIP_COEF_P0 ||
+                      kind == FFHIP_COEF_QP,
+                  FFHIP_ERR_ARG, "ffhip_assemble_p2: coefficient kind %d", kind);
+    const bool arr = kind == FFHIP_COEF_P0 || kind == FFHIP_COEF_QP;
+    FFHIP_REQUIRE(!arr || kappa_e, FFHIP_ERR_ARG, "ffhip_assemble_p2: kappa_e");
+    hipStream_t s = ctx->stream;
+    ffhip_mesh m = sp->mesh;
+    const int64_t cnt = kind == FFHIP_COEF_P0 ? m->nt : 14 * m->nt;
+    Arena ar(ctx, Arena::bytes(100 * m->nt, 8) + (arr && !kappa_on_device ? Arena::bytes(cnt, 8) : 0));
+    double *Ke = ar.take<double>(100 * m->nt);
+    const double *kd = kappa_e;
+    if (arr && !kappa_on_device) {
+      double *kb = ar.take<double>(cnt);
+      FFHIP_CHECK(hipMemcpyAsync(kb, kappa_e, 8 * cnt, hipMemcpyHostToDevice, s));
+      kd = kb;
+    }
+    int64_t g = (m->nt + kElemWaves - 1) / kElemWaves;
+    const int64_t gmax = 8 * (int64_t)ctx->num_cu;
+    if (g > gmax) g = gmax;
+    ctx->tic();
+    k_p2_element<true><<<(unsigned)g, kElemBlock, 0, s>>>(m->nt, m->xyz, m->elems, 0., 0., kind, c0,
+                                                          kd, sp->n2e_inv, Ke);
+    k_p2_gather<<<nblk(sp->nnodes, 256 / kWave), 256, 0, s>>>(sp->nnodes, A->nodeptr, sp->n2e_ptr,
+                                                             A->slot8, Ke, A->vals);
+    FFHIP_CHECK(hipGetLastError());
+    ctx->toc();
+  });
+}
+
+int ffhip_set_bc_p2(ffhip_ctx ctx, ffhip_p2v sp, ffhip_mat A, const int32_t *labels, int nlab,
+                    double tgv, uint8_t *onbc_out) {
+  return guard([&] {
+    FFHIP_REQUIRE(ctx && sp && A && A->ncomp == 1 && A->n == sp->nnodes, FFHIP_ERR_ARG,
+                  "ffhip_set_bc_p2: bad arguments");
+    FFHIP_REQUIRE(nlab >= 0 && (nlab == 0 || labels), FFHIP_ERR_ARG, "ffhip_set_bc_p2: labels");
+    FFHIP_REQUIRE(tgv > 0, FFHIP_ERR_UNSUPPORTED, "ffhip_set_bc_p2: only tgv > 0 is supported");
+    uint8_t *onbc = dalloc<uint8_t>(A->n);
+    int *missing = dalloc<int>(1);
+    bool ok = false;
+    int miss = 0;
+    try {
+      ctx->tic();
+      ok = p2v_mark(ctx, sp, labels, nlab, 1, onbc);
+      FFHIP_CHECK(hipMemsetAsync(missing, 0, sizeof(int), ctx->stream));
+      set_diag_rows(ctx, A->n, onbc, A->diag, tgv, A->vals, missing);
+      ctx->toc();
+      if (onbc_out)
+        FFHIP_CHECK(hipMemcpyAsync(onbc_out, onbc, A->n, hipMemcpyDeviceToHost, ctx->stream));
+      miss = read_flag(ctx, missing);
+    } catch (...) {
+      (void)hipStreamSynchronize(ctx->stream);
+      dfree(onbc); dfree(missing);
+      throw;
+    }
+    dfree(onbc); dfree(missing);
+    FFHIP_REQUIRE(ok, FFHIP_ERR_INTERNAL, "ffhip_set_bc_p2: a boundary edge is no element edge");
+    FFHIP_REQUIRE(miss == 0, FFHIP_ERR_INTERNAL, "ffhip_set_bc_p2: a Dirichlet row has no diagonal");
+  });
+}
+
+int ffhip_rhs_p2(ffhip_ctx ctx, ffhip_p2v sp, double f, const int32_t *labels, int nlab, double tgv,
+                 double g, double *b, int on_device) {
+  return guard([&] {
+    FFHIP_REQUIRE(ctx && sp && b, FFHIP_ERR_ARG, "ffhip_rhs_p2: null argument");
+    FFHIP_REQUIRE(nlab >= 0 && (nlab == 0 || labels), FFHIP_ERR_ARG, "ffhip_rhs_p2: labels");
+    hipStream_t s = ctx->stream;
+    ffhip_mesh m = sp->mesh;
+    const int64_t n = sp->nnodes;
+    uint8_t *onbc = dalloc<uint8_t>(n);
+    double *bd = on_device ? b : nullptr;
+    bool ok = false;
+    try {
+      ok = p2v_mark(ctx, sp, labels, nlab, 1, onbc);
+      if (!on_device) bd = dalloc<double>(n);
+      ctx->tic();
+      k_p2v_rhs<<<nblk(n, 256), 256, 0, s>>>(n, m->xyz, m->elems, sp->n2e_ptr, sp->n2e, 1, f, 0., 0.,
+                                             onbc, tgv, g, 0., 0., bd);
+      FFHIP_CHECK(hipGetLastError());
+      ctx->toc();
+      if (!on_device) {
+        FFHIP_CHECK(hipMemcpyAsync(b, bd, 8 * n, hipMemcpyDeviceToHost, s));
+        FFHIP_CHECK(hipStreamSynchronize(s));
+      }
+    } catch (...) {
+      (void)hipStreamSynchronize(s);
+      dfree(onbc);
+      if (!on_device) dfree(bd);
+      throw;
+    }
+    dfree(onbc);
+    if (!on_device) dfree(bd);
+    FFHIP_REQUIRE(ok, FFHIP_ERR_INTERNAL, "ffhip_rhs_p2: a boundary edge is no element edge");
   });
 }
 

@@ -105,6 +105,10 @@ _SIGS = {
     "ffhip_assemble_p2v_elasticity": (_i, [_vp, _vp, _vp, _d, _d]),
     "ffhip_set_bc_p2v": (_i, [_vp, _vp, _vp, _vp, _i, _d, _vp]),
     "ffhip_rhs_p2v": (_i, [_vp, _vp, _vp, _vp, _i, _d, _vp, _vp, _i]),
+    "ffhip_pattern_p2": (_i, [_vp, _vp, _pvp]),
+    "ffhip_assemble_p2": (_i, [_vp, _vp, _vp, _i, _d, _vp, _i]),
+    "ffhip_set_bc_p2": (_i, [_vp, _vp, _vp, _vp, _i, _d, _vp]),
+    "ffhip_rhs_p2": (_i, [_vp, _vp, _d, _vp, _i, _d, _d, _vp, _i]),
 }
 
 _LIB = None
@@ -294,6 +298,35 @@ class Matrix:
         A.space = space
         return A
 
+    @staticmethod
+    def p2(space):
+        """pattern of `matrix A = va(Vh,Vh)` on the scalar Vh(Th,P2) (dof = node
+        of `space`)"""
+        h = ctypes.c_void_p()
+        check(lib().ffhip_pattern_p2(space.ctx.h, space.h, ctypes.byref(h)))
+        A = Matrix(h, space.ctx, space.mesh)
+        A.space = space
+        return A
+
+    def assemble_p2(self, kind=COEF_SIN3, c0=1.0, kappa_e=None):
+        """int3d(kappa grad u . grad v) values on the scalar P2 space"""
+        on_dev, ptr = 0, None
+        if kappa_e is not None:
+            if isinstance(kappa_e, np.ndarray):
+                kappa_e = np.ascontiguousarray(kappa_e, np.float64)
+                ptr = _hp(kappa_e)
+            else:
+                ptr, on_dev = _dp(kappa_e), 1
+        check(lib().ffhip_assemble_p2(self.ctx.h, self.space.h, self.h, kind, c0, ptr, on_dev))
+        return self.ctx.last_ms()
+
+    def set_bc_p2(self, labels, tgv=1e30, want_onbc=False):
+        lab = np.asarray(labels, np.int32)
+        onbc = np.zeros(self.n, np.uint8) if want_onbc else None
+        check(lib().ffhip_set_bc_p2(self.ctx.h, self.space.h, self.h, _hp(lab), lab.size, tgv,
+                                    _hp(onbc)))
+        return onbc if want_onbc else self.ctx.last_ms()
+
     def assemble_p2v_elasticity(self, lam=1.0, mu=0.5):
         """int3d(lambda div(u) div(v) + 2 mu epsilon(u):epsilon(v)) values"""
         check(lib().ffhip_assemble_p2v_elasticity(self.ctx.h, self.space.h, self.h, lam, mu))
@@ -411,6 +444,17 @@ class P2VSpace:
             self.free()
         except Exception:
             pass
+
+
+def rhs_p2(space, f=1.0, labels=(), tgv=1e30, g=0.0, out=None):
+    """b of int3d(f*v) + on(labels, u=g) on the scalar Vh(Th,P2)"""
+    lab = np.asarray(labels, np.int32)
+    if out is None:
+        b = np.zeros(space.nnodes)
+        check(lib().ffhip_rhs_p2(space.ctx.h, space.h, f, _hp(lab), lab.size, tgv, g, _hp(b), 0))
+        return b
+    check(lib().ffhip_rhs_p2(space.ctx.h, space.h, f, _hp(lab), lab.size, tgv, g, _dp(out), 1))
+    return out
 
 
 def rhs_p2v(space, f=(0.0, 0.0, -1.0), labels=(), tgv=1e30, g=None, out=None):

@@ -19,7 +19,9 @@
 //    src/fflib/AFunction2.cpp:87-107).  It recognises
 //        int3d(Th)(c*(dx(u)*dx(v)+dy(u)*dy(v)+dz(u)*dz(v))) [+ on(labels,u=g)]
 //    [+ int3d(Th)(m*u*v)] (or the reaction term alone)
-//    on a scalar P1 space with the default Tet_5 rule; the coefficient c is
+//    on a scalar P1 space (the Laplacian also on a scalar P2 space, whose
+//    dof numbering is checked against the device's first-appearance node
+//    numbering) with the default Tet_5 rule; the coefficient c is
 //    evaluated by FreeFEM at the 14 quadrature points of every element (as
 //    Element_Op does, problem.cpp:6392-6395) unless it is a constant, and the
 //    pattern, values and tgv rows are built on the GPU.  It also recognises
@@ -248,6 +250,7 @@ typedef Call_FormBilinear<v_fes3, v_fes3> CFB3;
 
 struct Recognised {
   const FormBilinear *bf = nullptr;
+  int order = 1;              // P1 or P2 (scalar)
   bool has_grad = false;      // the three derivative terms
   C_F0 coef;                  // their common coefficient
   bool coef_const = false;
@@ -263,8 +266,15 @@ struct Recognised {
 // (either part may be absent), or false
 bool recognise(Stack stack, const FESpace3 &Uh, const FESpace3 &Vh, const list<C_F0> &largs,
                Recognised &r) {
-  if (&Uh != &Vh || Uh.N != 1 || Uh.TFE[0] != &DataFE<Mesh3>::P1) return false;
-  if (Uh.NbOfDF != Uh.Th.nv) return false;
+  if (&Uh != &Vh || Uh.N != 1 || Uh.TFE.N() != 1) return false;
+  if (Uh.TFE[0] == &DataFE<Mesh3>::P1) {
+    if (Uh.NbOfDF != Uh.Th.nv) return false;
+    r.order = 1;
+  } else if (Uh.TFE[0] == &DataFE<Mesh3>::P2) {
+    r.order = 2;
+  } else {
+    return false;
+  }
   FreeFempp::TypeVarForm<double> *tvf = FreeFempp::TypeVarForm<double>::Global;
   for (list<C_F0>::const_iterator ii = largs.begin(); ii != largs.end(); ++ii) {
     const aType t = ii->left();
@@ -309,6 +319,7 @@ bool recognise(Stack stack, const FESpace3 &Uh, const FESpace3 &Vh, const list<C
     r.bf = bf;
   }
   if (!r.bf) return false;
+  if (r.order == 2 && (r.has_mass || !r.has_grad)) return false;  // P2: the Laplacian only
   if (r.has_grad && r.coef.EvaluableWithOutStack()) {
     r.coef_const = true;
     r.c0 = GetAny<double>(r.coef.eval(stack));
@@ -522,6 +533,69 @@ void assemble_on_gpu(Stack stack, const FESpace3 &Uh, const Recognised &r, doubl
   A.A.master(pH);
 }
 
+// Vh(Th,P2): true when built on the GPU; false (nothing changed) when FreeFEM's
+// dof numbering of Uh is not the first-appearance node numbering of the device
+bool assemble_p2_on_gpu(Stack stack, const FESpace3 &Uh, const Recognised &r, double tgv,
+                        Matrice_Creuse<double> &A) {
+  const Mesh3 &Th = Uh.Th;
+  vector<double> xyz(3 * (size_t)Th.nv);
+  vector<int32_t> tets(4 * (size_t)Th.nt), btri(3 * (size_t)Th.nbe), blab(Th.nbe);
+  for (int i = 0; i < Th.nv; ++i) {
+    xyz[3 * i] = Th(i).x; xyz[3 * i + 1] = Th(i).y; xyz[3 * i + 2] = Th(i).z;
+  }
+  for (int k = 0; k < Th.nt; ++k)
+    for (int j = 0; j < 4; ++j) tets[4 * k + j] = Th(Th[k][j]);
+  for (int k = 0; k < Th.nbe; ++k) {
+    for (int j = 0; j < 3; ++j) btri[3 * k + j] = Th(Th.be(k)[j]);
+    blab[k] = Th.be(k).lab;
+  }
+  ffhip_ctx ctx = hip_ctx();
+  ffhip_mesh m = nullptr;
+  ffhip_p2v sp = nullptr;
+  hip_check(ffhip_mesh3_upload(ctx, Th.nv, Th.nt, Th.nbe, xyz.data(), tets.data(), btri.data(),
+                               blab.data(), &m), "ffhip: mesh upload");
+  hip_check(ffhip_p2v_create(ctx, m, &sp), "ffhip: P2 numbering");
+  vector<int32_t> en(10 * (size_t)Th.nt);
+  hip_check(ffhip_p2v_download(sp, en.data()), "ffhip: P2 numbering");
+  bool same = true;
+  for (int k = 0; k < Th.nt && same; ++k) {
+    const FElement3 K(Uh[k]);
+    for (int df = 0; df < 10 && same; ++df) same = K(df) == en[10 * (size_t)k + df];
+  }
+  if (!same) {
+    ffhip_p2v_free(sp);
+    ffhip_mesh_free(m);
+    return false;
+  }
+  ffhip_mat M = nullptr;
+  hip_check(ffhip_pattern_p2(ctx, sp, &M), "ffhip: P2 pattern");
+  if (r.coef_const) {
+    hip_check(ffhip_assemble_p2(ctx, sp, M, FFHIP_COEF_CONST, r.c0, nullptr, 0), "ffhip: P2 assembly");
+  } else {
+    vector<double> kq;
+    eval_coef_qp(stack, Uh, r.coef, kq);
+    hip_check(ffhip_assemble_p2(ctx, sp, M, FFHIP_COEF_QP, 0., kq.data(), 0), "ffhip: P2 assembly");
+  }
+  if (!r.labels.empty()) {
+    vector<int32_t> labs(r.labels.begin(), r.labels.end());
+    hip_check(ffhip_set_bc_p2(ctx, sp, M, labs.data(), (int)labs.size(), tgv, nullptr), "ffhip: on()");
+  }
+  int64_t n = 0, nnz = 0;
+  ffhip_mat_info(M, &n, &nnz);
+  vector<int64_t> rp(n + 1);
+  vector<int> ia(n + 1), ja(nnz);
+  vector<double> aa(nnz);
+  hip_check(ffhip_mat_download(M, rp.data(), ja.data(), aa.data()), "ffhip: download");
+  for (int64_t i = 0; i <= n; ++i) ia[i] = (int)rp[i];
+  ffhip_mat_free(M);
+  ffhip_p2v_free(sp);
+  ffhip_mesh_free(m);
+  MatriceMorse<double> *pH = new MatriceMorse<double>((int)n, (int)n, (int)nnz, 0);
+  pH->set((int)n, (int)n, 0, (size_t)nnz, ia.data(), ja.data(), aa.data(), 0, 1);
+  A.A.master(pH);
+  return true;
+}
+
 class OpHipMatrixP1 : public OneOperator {
  public:
   int init;
@@ -556,8 +630,12 @@ class OpHipMatrixP1 : public OneOperator {
         A.A = 0;
         A.Uh = *PUh;
         A.Vh = *PVh;
-        assemble_on_gpu(stack, *PUh, r, ds.tgv, A);
-        done = true;
+        if (r.order == 1) {
+          assemble_on_gpu(stack, *PUh, r, ds.tgv, A);
+          done = true;
+        } else {
+          done = assemble_p2_on_gpu(stack, *PUh, r, ds.tgv, A);
+        }
       } else if (ds.sym == 0 && ds.tgv > 0 && recognise_p2v(stack, *PUh, *PVh, b->largs, r2)) {
         A.A = 0;
         A.Uh = *PUh;

@@ -186,6 +186,27 @@ int ffhip_rhs_p2v(ffhip_ctx ctx, ffhip_p2v space, const double f[3],
                   const int32_t *labels, int nlab, double tgv, const double g[3],
                   double *b, int on_device);
 
+/* ---- scalar P2 on a tetrahedral mesh (Vh(Th,P2), dof = node) ----------- */
+/* The same node numbering as ffhip_p2v_create (FreeFEM's BuildDFNumbering
+ * for P2, GenericMesh.hpp:1880-1920).  Pattern: the node rows, bit-exact
+ * with the HashMatrix of the element loop (HashMatrix.cpp:1294-1332) sorted
+ * by CSR(). */
+int ffhip_pattern_p2(ffhip_ctx ctx, ffhip_p2v space, ffhip_mat *mat);
+/* int3d(Th)(kappa*(dx(u)*dx(v)+dy(u)*dy(v)+dz(u)*dz(v))) with
+ * TypeOfFE_P2Lagrange3d (P012_3d.cpp:199-270) and Tet_5 — Element_Op
+ * (problem.cpp:6384-6437); kappa as in ffhip_assemble_p1 (QP: 14 values per
+ * element). */
+int ffhip_assemble_p2(ffhip_ctx ctx, ffhip_p2v space, ffhip_mat mat, int kind, double c0,
+                      const double *kappa_e, int kappa_on_device);
+/* on(labels, u=0): the nodes of the labelled boundary triangles (3 vertices
+ * and 3 edges) get a_ii = tgv (AssembleBC, problem.cpp:10120-10164). */
+int ffhip_set_bc_p2(ffhip_ctx ctx, ffhip_p2v space, ffhip_mat mat, const int32_t *labels,
+                    int nlab, double tgv, uint8_t *onbc);
+/* int3d(Th)(f*v) (Element_rhs, problem.cpp:7940-7978), f constant; tgv*g on
+ * the Dirichlet rows. */
+int ffhip_rhs_p2(ffhip_ctx ctx, ffhip_p2v space, double f, const int32_t *labels, int nlab,
+                 double tgv, double g, double *b, int on_device);
+
 /* ---- Jacobi-preconditioned CG ------------------------------------------ */
 enum { FFHIP_CG_FP64 = 0, FFHIP_CG_MIXED = 1 };
 /* Layout of the streamed matrix, OR'ed into mode: sliced ELL with 64-row

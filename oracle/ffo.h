@@ -202,6 +202,22 @@ int ffo_fgmres(int n, const int64_t *rowptr, const int32_t *colidx, const double
                const double *b, double *x, double eps, int itmax, int restart, double tgv,
                double *stats);
 
+/* ---- scalar P2 (Vh(Th,P2), dof = node) --------------------------------- */
+/* the coefficient of the P1/P2 forms at physical point P (CONST c0, SIN3, P0 ke) */
+double ffo_coef_eval(int kind, double c0, double ke, const double *P, int dim);
+/* Element_Op for int3d(Th)(kappa*grad(u)'*grad(v)), P2 FB, Tet_5; Ke 10x10
+ * row-major, row = test function */
+int ffo_p2_lap_element(const double *xyz, const int32_t *tet, int kind, double c0,
+                       double kappa_e, double *Ke);
+int ffo_assemble_p2_lap(int64_t nt, const double *xyz, const int32_t *tets,
+                        const int32_t *elem_nodes, int kind, double c0, const double *kappa_e,
+                        int64_t nn, const int64_t *rowptr, const int32_t *colidx, double *vals);
+int ffo_rhs_p2(int64_t nt, const double *xyz, const int32_t *tets, const int32_t *elem_nodes,
+               double fv, int64_t nn, double *b);
+int ffo_p2_mark_bc(int nv, int64_t nt, const int32_t *tets, const int32_t *elem_nodes,
+                   int64_t nbe, const int32_t *btri, const int32_t *blab, int nlab,
+                   const int32_t *labs, int ncomp, int64_t ndof, uint8_t *onbc);
+
 #ifdef __cplusplus
 }
 #endif

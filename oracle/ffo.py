@@ -48,6 +48,10 @@ def lib():
             "ffo_assemble_p1_3d": (_i, [_i, _l, _f64p, _i32p, _i, _d, _vp, _i64p, _i32p, _f64p]),
             "ffo_assemble_p1_2d": (_i, [_i, _l, _f64p, _i32p, _i, _d, _vp, _i64p, _i32p, _f64p]),
             "ffo_p1_form_element_3d": (_i, [_f64p, _i32p, _i, _d, _d, _i, _d, _d, _f64p]),
+            "ffo_p2_lap_element": (_i, [_f64p, _i32p, _i, _d, _d, _f64p]),
+            "ffo_assemble_p2_lap": (_i, [_l, _f64p, _i32p, _i32p, _i, _d, _vp, _l, _i64p, _i32p, _f64p]),
+            "ffo_rhs_p2": (_i, [_l, _f64p, _i32p, _i32p, _d, _l, _f64p]),
+            "ffo_p2_mark_bc": (_i, [_i, _l, _i32p, _i32p, _l, _i32p, _i32p, _i, _i32p, _i, _l, _u8p]),
             "ffo_p1_form_element_2d": (_i, [_f64p, _i32p, _i, _d, _d, _i, _d, _d, _f64p]),
             "ffo_assemble_p1_form_3d": (_i, [_i, _l, _f64p, _i32p, _i, _d, _vp, _i, _d, _vp, _i64p, _i32p, _f64p]),
             "ffo_assemble_p1_form_2d": (_i, [_i, _l, _f64p, _i32p, _i, _d, _vp, _i, _d, _vp, _i64p, _i32p, _f64p]),
@@ -363,6 +367,29 @@ def elasticity_p2(n, lam=1.0, mu=0.5, f=(0.0, 0.0, -1.0), labels=(4,), tgv=1e30,
     rhs_bc(b, onbc, tgv, 0.0)
     return dict(mesh=m, nnodes=nn, elem_nodes=en, elem_dofs=ed, ndof=ndof, rowptr=rp, colidx=ci,
                 vals=vals, onbc=onbc, b=b)
+
+
+def poisson_p2(n, kind=COEF_SIN3, c0=1.0, kappa_e=None, labels=(1, 2, 3, 4, 5, 6), tgv=1e30,
+               f=1.0, mesh=None):
+    """cube(n) (or `mesh`) scalar P2 system of int3d(kappa grad u . grad v) +
+    int3d(f v) + on(labels, u=0) on Vh(Th,P2): pattern, values, tgv rows, rhs."""
+    m = cube(n) if mesh is None else mesh
+    nn, en = p2_nodes(m)
+    rp, ci = pattern(nn, en, 10)
+    vals = np.zeros(ci.size)
+    ke = None if kappa_e is None else np.ascontiguousarray(kappa_e, np.float64)
+    rc = lib().ffo_assemble_p2_lap(m["nt"], m["xyz"], m["tets"], en, kind, c0, _ptr(ke), nn, rp, ci, vals)
+    assert rc == 0, rc
+    onbc = np.zeros(nn, np.uint8)
+    labs = np.asarray(labels, np.int32)
+    rc = lib().ffo_p2_mark_bc(m["nv"], m["nt"], m["tets"], en, m["nbe"], m["btri"], m["blab"],
+                             labs.size, labs, 1, nn, onbc)
+    assert rc == 0, rc
+    set_bc(rp, ci, vals, onbc, tgv)
+    b = np.zeros(nn)
+    lib().ffo_rhs_p2(m["nt"], m["xyz"], m["tets"], en, f, nn, b)
+    rhs_bc(b, onbc, tgv, 0.0)
+    return dict(mesh=m, nnodes=nn, elem_nodes=en, rowptr=rp, colidx=ci, vals=vals, onbc=onbc, b=b)
 
 
 def ref_fespace_dofs(m, kind):

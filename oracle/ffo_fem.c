@@ -95,7 +95,7 @@ void ffo_tet_gradlambda(const double *v, double mes, double *G) {
 
 /* kappa at a physical point, evaluated as the script expression
  * 1+0.5*sin(2*pi*x)*sin(2*pi*y)*sin(2*pi*z) (left to right). */
-static double coef_eval(int kind, double c0, double ke, const double *P, int dim) {
+double ffo_coef_eval(int kind, double c0, double ke, const double *P, int dim) {
   if (kind == FFO_COEF_CONST) return c0;
   if (kind == FFO_COEF_P0) return ke;
   const double tp = 2 * M_PI;
@@ -135,7 +135,7 @@ int ffo_p1_form_element_3d(const double *xyz, const int32_t *tet, int kind, doub
     for (int k = 0; k < 3; ++k) P[k] += Ph[1] * v[6 + k];
     for (int k = 0; k < 3; ++k) P[k] += Ph[2] * v[9 + k];
     for (int t = 0; t < 3; ++t) {
-      double ccc = coef_eval(kind, c0, kappa_e, P, 3);
+      double ccc = ffo_coef_eval(kind, c0, kappa_e, P, 3);
       ccc *= coef;
       for (int i = 0; i < 4; ++i)
         for (int j = 0; j < 4; ++j)
@@ -143,7 +143,7 @@ int ffo_p1_form_element_3d(const double *xyz, const int32_t *tet, int kind, doub
     }
     if (mkind >= 0) {
       const double l[4] = {l0, Ph[0], Ph[1], Ph[2]};
-      double ccc = coef_eval(mkind, m0, mass_e, P, 3);
+      double ccc = ffo_coef_eval(mkind, m0, mass_e, P, 3);
       ccc *= coef;
       for (int i = 0; i < 4; ++i)
         for (int j = 0; j < 4; ++j) Ke[4 * i + j] += ccc * l[i] * l[j];
@@ -183,7 +183,7 @@ int ffo_p1_form_element_2d(const double *xy, const int32_t *tri, int kind, doubl
     for (int k = 0; k < 2; ++k)
       P[k] = l0 * v[k] + Ph[0] * v[2 + k] + Ph[1] * v[4 + k];
     for (int t = 0; t < 2; ++t) {
-      double ccc = coef_eval(kind, c0, kappa_e, P, 2);
+      double ccc = ffo_coef_eval(kind, c0, kappa_e, P, 2);
       ccc *= coef;
       for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j)
@@ -191,7 +191,7 @@ int ffo_p1_form_element_2d(const double *xy, const int32_t *tri, int kind, doubl
     }
     if (mkind >= 0) {
       const double l[3] = {l0, Ph[0], Ph[1]};
-      double ccc = coef_eval(mkind, m0, mass_e, P, 2);
+      double ccc = ffo_coef_eval(mkind, m0, mass_e, P, 2);
       ccc *= coef;
       for (int i = 0; i < 3; ++i)
         for (int j = 0; j < 3; ++j) Ke[3 * i + j] += ccc * l[i] * l[j];

@@ -171,6 +171,14 @@ int ffo_p2v_mark_bc(int nv, int64_t nt, const int32_t *tets,
                     const int32_t *elem_nodes, int64_t nbe, const int32_t *btri,
                     const int32_t *blab, int nlab, const int32_t *labs,
                     int64_t ndof, uint8_t *onbc) {
+  return ffo_p2_mark_bc(nv, nt, tets, elem_nodes, nbe, btri, blab, nlab, labs, 3, ndof, onbc);
+}
+
+/* the nodes of the labelled boundary triangles (3 vertices, 3 edge midpoints),
+ * every one of the ncomp components (dof ncomp * node + c) */
+int ffo_p2_mark_bc(int nv, int64_t nt, const int32_t *tets, const int32_t *elem_nodes,
+                   int64_t nbe, const int32_t *btri, const int32_t *blab, int nlab,
+                   const int32_t *labs, int ncomp, int64_t ndof, uint8_t *onbc) {
   (void)nv;
   /* edge key -> node, vertex -> node from the element tables */
   hmap_t h;
@@ -199,10 +207,84 @@ int ffo_p2v_mark_bc(int nv, int64_t nt, const int32_t *tets,
       int fresh;
       const int32_t node = *hfind(&h, keys[i], &fresh);
       if (fresh) { free(h.k); free(h.v); return -3; }
-      for (int c = 0; c < 3; ++c) onbc[3 * (int64_t)node + c] = 1;
+      for (int c = 0; c < ncomp; ++c) onbc[ncomp * (int64_t)node + c] = 1;
     }
   }
   free(h.k);
   free(h.v);
+  return 0;
+}
+
+/* ---- scalar P2: int3d(Th)(kappa*(dx(u)*dx(v)+dy(u)*dy(v)+dz(u)*dz(v))) on
+ * Vh(Th,P2) (dof = node), Element_Op (problem.cpp:6384-6437) with
+ * TypeOfFE_P2Lagrange3d::FB (P012_3d.cpp:199-270) and Tet_5. */
+int ffo_p2_lap_element(const double *xyz, const int32_t *tet, int kind, double c0,
+                       double kappa_e, double *Ke) {
+  double v[12], G[12], qp[42], qw[14];
+  for (int i = 0; i < 4; ++i)
+    for (int k = 0; k < 3; ++k) v[3 * i + k] = xyz[3 * (int64_t)tet[i] + k];
+  const double mes = ffo_tet_mesure(v, v + 3, v + 6, v + 9);
+  ffo_tet_gradlambda(v, mes, G);
+  ffo_qf_tet5(qp, qw);
+  memset(Ke, 0, sizeof(double) * 100);
+  for (int q = 0; q < 14; ++q) {
+    double f[10], d[30];
+    const double *Ph = qp + 3 * q;
+    ffo_p2_basis(Ph, G, f, d);
+    const double coef = mes * qw[q];
+    const double l0 = 1. - (Ph[0] + Ph[1] + Ph[2]);
+    double P[3];
+    for (int k = 0; k < 3; ++k) P[k] = l0 * v[k];
+    for (int k = 0; k < 3; ++k) P[k] += Ph[0] * v[3 + k];
+    for (int k = 0; k < 3; ++k) P[k] += Ph[1] * v[6 + k];
+    for (int k = 0; k < 3; ++k) P[k] += Ph[2] * v[9 + k];
+    for (int t = 0; t < 3; ++t) {
+      double ccc = ffo_coef_eval(kind, c0, kappa_e, P, 3);
+      ccc *= coef;
+      for (int i = 0; i < 10; ++i)
+        for (int j = 0; j < 10; ++j) Ke[10 * i + j] += ccc * d[3 * i + t] * d[3 * j + t];
+    }
+  }
+  return 0;
+}
+
+int ffo_assemble_p2_lap(int64_t nt, const double *xyz, const int32_t *tets,
+                        const int32_t *elem_nodes, int kind, double c0, const double *kappa_e,
+                        int64_t nn, const int64_t *rowptr, const int32_t *colidx, double *vals) {
+  memset(vals, 0, sizeof(double) * (size_t)rowptr[nn]);
+  double Ke[100];
+  for (int64_t k = 0; k < nt; ++k) {
+    ffo_p2_lap_element(xyz, tets + 4 * k, kind, c0, kappa_e ? kappa_e[k] : 0., Ke);
+    const int32_t *d = elem_nodes + 10 * k;
+    for (int il = 0; il < 10; ++il)
+      for (int jl = 0; jl < 10; ++jl) {
+        const int64_t s = slot_of(rowptr, colidx, d[il], d[jl]);
+        if (s < 0) return -3;
+        vals[s] += Ke[10 * il + jl];
+      }
+  }
+  return 0;
+}
+
+/* Element_rhs for int3d(Th)(f*v) on Vh(Th,P2), f constant */
+int ffo_rhs_p2(int64_t nt, const double *xyz, const int32_t *tets, const int32_t *elem_nodes,
+               double fv, int64_t nn, double *b) {
+  double qp[42], qw[14];
+  ffo_qf_tet5(qp, qw);
+  memset(b, 0, sizeof(double) * (size_t)nn);
+  for (int64_t k = 0; k < nt; ++k) {
+    const int32_t *t = tets + 4 * k;
+    double v[12], G[12];
+    for (int i = 0; i < 4; ++i)
+      for (int kk = 0; kk < 3; ++kk) v[3 * i + kk] = xyz[3 * (int64_t)t[i] + kk];
+    const double mes = ffo_tet_mesure(v, v + 3, v + 6, v + 9);
+    ffo_tet_gradlambda(v, mes, G);
+    for (int q = 0; q < 14; ++q) {
+      double f[10], d[30];
+      ffo_p2_basis(qp + 3 * q, G, f, d);
+      const double coef = mes * qw[q];
+      for (int i = 0; i < 10; ++i) b[elem_nodes[10 * k + i]] += coef * fv * f[i];
+    }
+  }
   return 0;
 }

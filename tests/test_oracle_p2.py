@@ -130,3 +130,51 @@ def test_p2_pattern_size_cube32_matches_freefem():
     assert nn == 274625
     rp, ci = ffo.pattern(nn, en, 10)
     assert 9 * int(rp[-1]) == 69108489
+
+
+# ---- scalar P2: int3d(kappa grad u . grad v) on Vh(Th,P2)
+def _p2_lap(seed, kind=ffo.COEF_CONST, c0=1.0):
+    rng = np.random.default_rng(seed)
+    X = rng.standard_normal(12)
+    t = np.arange(4, dtype=np.int32)
+    if ffo.lib().ffo_tet_mesure(X[0:3].copy(), X[3:6].copy(), X[6:9].copy(), X[9:12].copy()) < 0:
+        t = np.array([0, 2, 1, 3], np.int32)
+    Ke = np.zeros(100)
+    ffo.lib().ffo_p2_lap_element(X, t, kind, c0, 0.0, Ke)
+    V = X.reshape(4, 3)[t]
+    mes = ffo.lib().ffo_tet_mesure(*(V[i].copy() for i in range(4)))
+    edges = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
+    nodes = np.vstack([V] + [0.5 * (V[a] + V[b]) for a, b in edges])
+    return Ke.reshape(10, 10), nodes, mes
+
+
+def test_p2_lap_element_kernel_and_exact_energy():
+    for seed in range(6):
+        K, nodes, mes = _p2_lap(seed, c0=1.7)
+        scale = np.abs(K).max()
+        assert np.allclose(K, K.T, atol=1e-13 * scale)
+        assert np.abs(K.sum(axis=1)).max() <= 1e-12 * scale  # constants in the kernel
+        rng = np.random.default_rng(50 + seed)
+        for _ in range(3):
+            Q = rng.standard_normal((3, 3)); Q = Q + Q.T
+            p = rng.standard_normal(3)
+            u = np.array([x @ Q @ x + p @ x for x in nodes])  # P2 interpolates quadratics exactly
+            g2 = lambda x: np.sum((2 * Q @ x + p) ** 2)  # |grad u|^2, a quadratic
+            # int_K of a quadratic: |K| (-1/20 sum_vertices + 1/5 sum_edge midpoints)
+            exact = 1.7 * mes * (-sum(g2(x) for x in nodes[:4]) / 20 + sum(g2(x) for x in nodes[4:]) / 5)
+            assert abs(u @ K @ u - exact) <= 1e-11 * abs(exact)
+
+
+def test_p2_lap_system_small():
+    r = ffo.poisson_p2(3)
+    nn = r["nnodes"]
+    assert nn == 7 ** 3 and r["onbc"].sum() == 7 ** 3 - 5 ** 3  # boundary nodes of the 7^3 lattice
+    m, en = r["mesh"], r["elem_nodes"]
+    b = np.zeros(nn)
+    ffo.lib().ffo_rhs_p2(m["nt"], m["xyz"], m["tets"], en, 1.0, nn, b)
+    assert abs(b.sum() - 1.0) < 1e-13  # the P2 basis sums to 1: int(f) over the unit cube
+    assert np.array_equal(r["b"][r["onbc"] == 0], b[r["onbc"] == 0])
+    x, st = ffo.pcg(r["rowptr"], r["colidx"], r["vals"], r["b"], eps=1e-10)
+    assert st["nret"] == 1 and x[r["onbc"] == 0].min() > 0  # -lap u = 1, u = 0 on the boundary
+    # the pattern: the scalar node rows of the [P2,P2,P2] pattern
+    assert 9 * r["colidx"].size == ffo.elasticity_p2(3)["colidx"].size

@@ -8,6 +8,9 @@
 #include <hipcub/hipcub.hpp>
 
 #include <cstdarg>
+#include <map>
+#include <mutex>
+#include <unordered_map>
 #include <cstring>
 #include <vector>
 
@@ -205,6 +208,119 @@ extern "C" {
 const char *ffhip_last_error(void) { return g_err; }
 const char *ffhip_version(void) { return "ffhip 0.1 (gfx950)"; }
 
+}  // extern "C"
+
+namespace ffhip {
+namespace {
+struct BlockCache {
+  std::mutex mu;
+  std::map<int, std::multimap<size_t, void *>> free_;       // per device, by size
+  std::unordered_map<void *, std::pair<int, size_t>> info;  // every block: device, size
+  std::map<int, int> contexts;
+  size_t cached = 0;  // bytes in the free lists
+};
+BlockCache &blocks() {
+  static BlockCache *c = new BlockCache();  // never destroyed: used until process exit
+  return *c;
+}
+constexpr size_t kCacheCap = size_t(32) << 30;  // cached bytes kept at most
+
+size_t round_block(size_t b) {
+  const size_t g = b >= (size_t(1) << 20) ? (size_t(2) << 20) : 256;
+  return (b + g - 1) / g * g;
+}
+
+void trim_device(BlockCache &c, int dev) {  // caller holds c.mu
+  auto it = c.free_.find(dev);
+  if (it == c.free_.end()) return;
+  for (auto &kv : it->second) {
+    (void)hipFree(kv.second);
+    c.info.erase(kv.second);
+    c.cached -= kv.first;
+  }
+  it->second.clear();
+}
+}  // namespace
+
+void *block_alloc(size_t bytes) {
+  BlockCache &c = blocks();
+  int dev = 0;
+  FFHIP_CHECK(hipGetDevice(&dev));
+  const size_t want = round_block(bytes);
+  {
+    std::lock_guard<std::mutex> g(c.mu);
+    auto &fl = c.free_[dev];
+    auto it = fl.lower_bound(want);
+    // a cached block at most 1/8 larger (small blocks: at most twice as large)
+    if (it != fl.end() && it->first <= (want >= (size_t(1) << 20) ? want + want / 8 : 2 * want)) {
+      void *p = it->second;
+      c.cached -= it->first;
+      fl.erase(it);
+      return p;
+    }
+  }
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, want);
+  if (e != hipSuccess) {  // give the cached blocks back and try once more
+    (void)hipGetLastError();
+    {
+      std::lock_guard<std::mutex> g(c.mu);
+      trim_device(c, dev);
+    }
+    e = hipMalloc(&p, want);
+  }
+  if (e != hipSuccess) {
+    set_error("hipMalloc(%zu bytes) -> %s", want, hipGetErrorString(e));
+    throw Error{FFHIP_ERR_HIP};
+  }
+  std::lock_guard<std::mutex> g(c.mu);
+  c.info[p] = {dev, want};
+  return p;
+}
+
+void block_free(void *p) {
+  BlockCache &c = blocks();
+  int dev = 0;
+  size_t size = 0;
+  bool shared = false;
+  {
+    std::lock_guard<std::mutex> g(c.mu);
+    auto it = c.info.find(p);
+    if (it == c.info.end()) {  // not a cached block
+      (void)hipFree(p);
+      return;
+    }
+    dev = it->second.first;
+    size = it->second.second;
+    shared = c.contexts[dev] > 1;
+    if (c.cached + size > kCacheCap) {
+      c.info.erase(it);
+      (void)hipFree(p);
+      return;
+    }
+  }
+  if (shared) {  // another context's stream could reuse it: let this one's work finish
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(dev);
+    (void)hipDeviceSynchronize();
+    (void)hipSetDevice(cur);
+  }
+  std::lock_guard<std::mutex> g(c.mu);
+  c.free_[dev].emplace(size, p);
+  c.cached += size;
+}
+
+void blocks_context(int device, int delta) {
+  BlockCache &c = blocks();
+  std::lock_guard<std::mutex> g(c.mu);
+  c.contexts[device] += delta;
+  if (c.contexts[device] <= 0) trim_device(c, device);
+}
+}  // namespace ffhip
+
+extern "C" {
+
 int ffhip_init(int device, ffhip_ctx *out) {
   return guard([&] {
     FFHIP_REQUIRE(out, FFHIP_ERR_ARG, "ffhip_init: null output");
@@ -223,6 +339,7 @@ int ffhip_init(int device, ffhip_ctx *out) {
     hipDeviceProp_t prop;
     FFHIP_CHECK(hipGetDeviceProperties(&prop, device));
     c->num_cu = prop.multiProcessorCount;
+    blocks_context(device, +1);
     *out = c;
   });
 }
@@ -236,6 +353,7 @@ int ffhip_ctx_free(ffhip_ctx c) {
     (void)hipEventDestroy(c->ev0);
     (void)hipEventDestroy(c->ev1);
     (void)hipStreamDestroy(c->stream);
+    blocks_context(c->device, -1);
     delete c;
   });
 }

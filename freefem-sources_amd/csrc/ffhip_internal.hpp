@@ -55,16 +55,23 @@ int guard(F &&f) {
 }
 
 // ---------------------------------------------------------------- memory
+// Device blocks: released blocks are kept per device and handed out again for
+// requests of about the same size (a solve or an assembly repeated on the same
+// mesh re-allocates the same sizes), so steady-state calls do not pay
+// hipMalloc / hipFree (tens to hundreds of microseconds per large block).
+// All work of a context is ordered on its one stream; with several contexts
+// on a device, a release first waits for the device.
+void *block_alloc(size_t bytes);
+void block_free(void *p);
+void blocks_context(int device, int delta);  // context count per device; trims at 0
 template <class T>
 T *dalloc(size_t n) {
-  void *p = nullptr;
   if (n == 0) n = 1;
-  FFHIP_CHECK(hipMalloc(&p, n * sizeof(T)));
-  return (T *)p;
+  return (T *)block_alloc(n * sizeof(T));
 }
 template <class T>
 void dfree(T *&p) {
-  if (p) (void)hipFree(p);
+  if (p) block_free(p);
   p = nullptr;
 }
 

@@ -364,8 +364,6 @@ __global__ __launch_bounds__(256) void k_pattern_compact(
 // ---------------------------------------------------------------- element matrices
 // 3D P1: Ke_ij = S grad l_i . grad l_j with S = sum_q |K| w_q kappa(x_q)
 // (problem.cpp:6387-6392); the 10 unique entries per element.
-// gather recomputes the element's gradients from its vertices when it adds
-// the element's row, so the element matrices never travel through HBM.
 __device__ inline double p1_kappa_integral(int64_t e, const double v[4][3], double mes, int kind,
                                            double c0, const double *__restrict__ kappa_e) {
   double S = 0.;

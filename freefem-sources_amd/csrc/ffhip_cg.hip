@@ -180,12 +180,7 @@ __device__ inline int wave_id() { return __builtin_amdgcn_readfirstlane(threadId
 // FP32 mat-vec 6 % faster).  Otherwise each block walks its own contiguous
 // chunk of slices — kept for the [P2,P2,P2] block layout, whose node rows are
 // sorted by length in windows and ran 1.8x slower with the cyclic walk.
-#ifndef FFHIP_SCHED_SELL
-#define FFHIP_SCHED_SELL 1
-#endif
-#ifndef FFHIP_SCHED_BSR
-#define FFHIP_SCHED_BSR 0
-#endif
+constexpr bool kCyclicSell = true, kCyclicBsr = false;
 template <bool CYCLIC>
 __device__ inline void wave_slices(int64_t S, int64_t &first, int64_t &end, int64_t &stride) {
   if (CYCLIC) {
@@ -202,30 +197,6 @@ __device__ inline void wave_slices(int64_t S, int64_t &first, int64_t &end, int6
     stride = kWavesPerBlock;
   }
 }
-
-// epilogue streams used once by a kernel (FFHIP_NTEPI 1: non-temporal, so they
-// do not displace the gathered vectors from L2)
-#ifndef FFHIP_NTEPI
-#define FFHIP_NTEPI 0
-#endif
-
-template <class T>
-__device__ inline T epi_load(const T *p) {
-#if FFHIP_NTEPI
-  return __builtin_nontemporal_load(p);
-#else
-  return *p;
-#endif
-}
-template <class T>
-__device__ inline void epi_store(T v, T *p) {
-#if FFHIP_NTEPI
-  __builtin_nontemporal_store(v, p);
-#else
-  *p = v;
-#endif
-}
-
 
 // ---------------------------------------------------------------- SELL build
 __global__ void k_sell_len(int64_t n, int64_t S, const int64_t *__restrict__ rowptr,
@@ -492,7 +463,7 @@ template <int KIND, class V, class ACC, class F, class E>
 __device__ inline void matvec_rows(const MatFmt<V> &M, F xval, E epi) {
   if (KIND == kSell || KIND == kSell16) {
     int64_t s0, s1, ds;
-    wave_slices<FFHIP_SCHED_SELL != 0>(M.S, s0, s1, ds);
+    wave_slices<kCyclicSell>(M.S, s0, s1, ds);
     const int lane = threadIdx.x % kSlice;
     for (int64_t s = s0; s < s1; s += ds) {
       const int64_t base = uniform(M.ptr[s]) + lane;
@@ -512,7 +483,7 @@ __device__ inline void matvec_rows(const MatFmt<V> &M, F xval, E epi) {
     constexpr int U = 2;
     const int64_t S3 = (M.nb + kSlice - 1) / kSlice;
     int64_t s0, s1, ds;
-    wave_slices<FFHIP_SCHED_BSR != 0>(S3, s0, s1, ds);
+    wave_slices<kCyclicBsr>(S3, s0, s1, ds);
     const int lane = threadIdx.x % kSlice;
     for (int64_t s = s0; s < s1; s += ds) {
       const int64_t i = s * kSlice + lane;
@@ -681,9 +652,9 @@ __global__ FFHIP_MATVEC_ATTR void k_pcg_matvec(MatFmt<double> M, const double *_
       M, [&](int32_t j) { return H[j]; },
       [&](int64_t r, double acc) {
         const double h = H[r];
-        epi_store(acc, AH + r);
+        AH[r] = acc;
         p_hah += h * acc;
-        p_gh += epi_load(G + r) * h;
+        p_gh += G[r] * h;
       });
   const double a = block_sum<kBlock>(p_hah, lds);
   const double gh = block_sum<kBlock>(p_gh, lds);
@@ -1085,7 +1056,7 @@ __global__ FFHIP_MATVEC_ATTR void k_mix_matvec(MatFmt<float> M, const float *__r
   matvec_rows<KIND, float, float>(
       M, [&](int32_t j) { return p[j]; },
       [&](int64_t r, float acc) {
-        epi_store(acc, q + r);
+        q[r] = acc;
         part += (double)p[r] * (double)acc;
       });
   const double tot = block_sum<kBlock>(part, lds);

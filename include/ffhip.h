@@ -41,7 +41,11 @@ typedef struct ffhip_p2v_s *ffhip_p2v;   /* [P2,P2,P2] space on a 3D mesh   */
 const char *ffhip_last_error(void);
 const char *ffhip_version(void);
 
-/* Library init on one GPU (plugin Load_Init, load.cpp:93-105 / InitFunct.hpp:76-98). */
+/* Library init on one GPU (plugin Load_Init, load.cpp:93-105 / InitFunct.hpp:76-98).
+ * All work of a context is ordered on its one stream.  Device blocks that
+ * handles release are kept per device and reused by later allocations of
+ * about the same size; they go back to the device when the last context of
+ * the device is freed.  Free every handle of a context before the context. */
 int ffhip_init(int device, ffhip_ctx *ctx);
 int ffhip_ctx_free(ffhip_ctx ctx);
 int ffhip_sync(ffhip_ctx ctx);

@@ -347,18 +347,33 @@ __global__ __launch_bounds__(256) void k_pattern_half(
   }
 }
 
-// buffered columns to their CSR place; diagonal slots
+// buffered columns to their CSR place; diagonal slots.  One lane per row:
+// the row's buffer is read as 16-byte groups, all issued before the stores,
+// so a wave keeps 64 rows' loads in flight.
 __global__ __launch_bounds__(256) void k_pattern_compact(
     int64_t n, const int64_t *__restrict__ rowptr, const int32_t *__restrict__ rowbuf,
     const int8_t *__restrict__ dposr, const uint8_t *__restrict__ big,
     int32_t *__restrict__ colidx, int64_t *__restrict__ diag) {
-  const int hl = threadIdx.x % kHalf;
-  const int64_t v = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kHalf;
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n || big[v]) return;
   const int64_t r0 = rowptr[v];
   const int len = (int)(rowptr[v + 1] - r0);
-  for (int j = hl; j < len; j += kHalf) colidx[r0 + j] = rowbuf[kRowBuf * v + j];
-  if (hl == 0) diag[v] = dposr[v] < 0 ? -1 : r0 + dposr[v];
+  const int4 *src = reinterpret_cast<const int4 *>(rowbuf + kRowBuf * v);
+  int32_t *dst = colidx + r0;
+  for (int g0 = 0; g0 < len; g0 += 16) {  // 16 columns (4 groups) per round
+    int4 q[4];
+#pragma unroll
+    for (int g = 0; g < 4; ++g)
+      if (g0 + 4 * g < len) q[g] = src[(g0 >> 2) + g];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const int32_t e[4] = {q[g].x, q[g].y, q[g].z, q[g].w};
+#pragma unroll
+      for (int t = 0; t < 4; ++t)
+        if (g0 + 4 * g + t < len) dst[g0 + 4 * g + t] = e[t];
+    }
+  }
+  diag[v] = dposr[v] < 0 ? -1 : r0 + dposr[v];
 }
 
 // ---------------------------------------------------------------- element matrices
@@ -977,7 +992,7 @@ int ffhip_pattern_p1(ffhip_ctx ctx, ffhip_mesh m, ffhip_mat *out) {
     A->nnz = read_i64(ctx, A->rowptr + n);
     A->colidx = dalloc<int32_t>(A->nnz);
     A->vals = dalloc<double>(A->nnz);
-    k_pattern_compact<<<nblocks(n * kHalf, 256), 256, 0, s>>>(n, A->rowptr, rowbuf, dposr, A->big,
+    k_pattern_compact<<<nblocks(n, 256), 256, 0, s>>>(n, A->rowptr, rowbuf, dposr, A->big,
                                                               A->colidx, A->diag);
     if (A->any_big)
       k_pattern_rows<B, CAP, true><<<nblocks(n, B), B, 0, s>>>(

@@ -264,11 +264,15 @@ __global__ __launch_bounds__(256) void k_sell_fill(
       if (sval32) sval32[idx] = 0.f;
     }
   }
-  if (scol16) {  // [g][lane][4] packed deltas, the group tail padded with 0
-    const int L4 = (L + 3) & ~3;
+  if (scol16) {  // [g][lane][4] packed deltas, the group tail padded with 0: one 8-byte store per group
     int16_t *dst = scol16 + sptr16[s] + 4 * lane;
-    for (int k = 0; k < L4; ++k)
-      dst[(k >> 2) * (4 * kSlice) + (k & 3)] = k < rl ? (int16_t)(cp[k] - r) : (int16_t)0;
+    for (int k = 0; k < L; k += 4) {
+      uint32_t h[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) h[t] = k + t < rl ? (uint32_t)(uint16_t)(int16_t)(cp[k + t] - r) : 0u;
+      u32x2 pk = {h[0] | (h[1] << 16), h[2] | (h[3] << 16)};
+      *reinterpret_cast<u32x2 *>(dst + (k >> 2) * (4 * kSlice)) = pk;
+    }
   }
 }
 

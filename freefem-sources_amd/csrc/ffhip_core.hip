@@ -521,6 +521,7 @@ int ffhip_mesh_free(ffhip_mesh m) {
     (void)hipStreamSynchronize(m->ctx->stream);
     dfree(m->xyz); dfree(m->elems); dfree(m->belems); dfree(m->blab);
     dfree(m->v2e_ptr); dfree(m->v2e); dfree(m->v2e_inv);
+    dfree(m->b2e_ptr); dfree(m->b2e);
     delete m;
   });
 }

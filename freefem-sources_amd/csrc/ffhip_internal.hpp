@@ -112,6 +112,10 @@ struct ffhip_mesh_s {
   int32_t *v2e = nullptr;      // nt * nloc entries: e * nloc + l
   int v2e_big = 0;             // some vertex has more than 32 incident elements
   int32_t *v2e_inv = nullptr;  // nt * nloc: position of item e * nloc + l in v2e
+  // dof -> incident (boundary element, local vertex) list, boundary-element
+  // order (built lazily by the boundary-integral calls)
+  int64_t *b2e_ptr = nullptr;  // nv + 1
+  int32_t *b2e = nullptr;      // nbe * nbloc entries: f * nbloc + l
 };
 
 struct ffhip_mat_s {

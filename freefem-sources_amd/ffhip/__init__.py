@@ -77,6 +77,8 @@ _SIGS = {
     "ffhip_assemble_p1_form": (_i, [_vp, _vp, _vp, _i, _d, _vp, _i, _d, _vp, _i]),
     "ffhip_set_bc": (_i, [_vp, _vp, _vp, _vp, _i, _d, _vp]),
     "ffhip_rhs_p1": (_i, [_vp, _vp, _d, _vp, _i, _d, _d, _vp, _i]),
+    "ffhip_assemble_p1_boundary": (_i, [_vp, _vp, _vp, _vp, _i, _i, _d, _vp, _i]),
+    "ffhip_rhs_p1_boundary": (_i, [_vp, _vp, _vp, _i, _i, _d, _vp, _vp, _i, _vp, _i]),
     "ffhip_mat_download": (_i, [_vp, _vp, _vp, _vp]),
     "ffhip_mat_upload": (_i, [_vp, _l, _l, _vp, _vp, _vp, _pvp]),
     "ffhip_mat_device_ptrs": (_i, [_vp, _pvp, _pvp, _pvp]),
@@ -367,6 +369,19 @@ class Matrix:
                                                ptr(kappa_e), mkind, m0, ptr(mass_e), on_dev))
         return self.ctx.last_ms()
 
+    def assemble_p1_boundary(self, labels, kind=COEF_CONST, c0=1.0, coef=None):
+        """add int2d(Th, labels)(alpha*u*v) (int1d on a 2D mesh); alpha as
+        kind/c0/coef: coef numpy (host) or torch cuda tensor (device), nbe
+        values (P0) or nbe x nq (QP, nq = 7 on faces, 3 on edges)"""
+        lab = np.asarray(labels, np.int32)
+        on_dev = int(coef is not None and not isinstance(coef, np.ndarray))
+        if coef is not None and not on_dev:
+            coef = np.ascontiguousarray(coef, np.float64)
+        cp = None if coef is None else (_dp(coef) if on_dev else _hp(coef))
+        check(lib().ffhip_assemble_p1_boundary(self.ctx.h, self.mesh.h, self.h, _hp(lab), lab.size,
+                                               kind, c0, cp, on_dev))
+        return self.ctx.last_ms()
+
     def set_bc(self, labels, tgv=1e30, want_onbc=False):
         lab = np.asarray(labels, np.int32)
         onbc = np.zeros(self.n, np.uint8) if want_onbc else None
@@ -416,6 +431,26 @@ def rhs_p1(mesh, f=1.0, labels=(), tgv=1e30, g=0.0, out=None):
         return b
     check(lib().ffhip_rhs_p1(mesh.ctx.h, mesh.h, f, _hp(lab), lab.size, tgv, g, _dp(out), 1))
     return out
+
+
+def rhs_p1_boundary(mesh, b, labels, kind=COEF_CONST, g0=1.0, coef=None, dirichlet=()):
+    """b += int2d(Th, labels)(g*v) (int1d on a 2D mesh), leaving the rows of
+    the Dirichlet labels unchanged.  b: numpy array (updated in place) or
+    torch cuda tensor; coef like Matrix.assemble_p1_boundary's."""
+    lab = np.asarray(labels, np.int32)
+    dl = np.asarray(dirichlet, np.int32)
+    on_dev = int(not isinstance(b, np.ndarray))
+    if coef is not None:
+        assert isinstance(coef, np.ndarray) != bool(on_dev), "b and coef on the same side"
+        if not on_dev:
+            coef = np.ascontiguousarray(coef, np.float64)
+    if not on_dev:
+        assert b.dtype == np.float64 and b.flags.c_contiguous
+    cp = None if coef is None else (_dp(coef) if on_dev else _hp(coef))
+    check(lib().ffhip_rhs_p1_boundary(mesh.ctx.h, mesh.h, _hp(lab), lab.size, kind, g0, cp,
+                                      _hp(dl) if dl.size else None, dl.size,
+                                      _dp(b) if on_dev else _hp(b), on_dev))
+    return b
 
 
 class P2VSpace:

@@ -119,6 +119,34 @@ int ffhip_set_bc(ffhip_ctx ctx, ffhip_mesh mesh, ffhip_mat mat,
 int ffhip_rhs_p1(ffhip_ctx ctx, ffhip_mesh mesh, double f,
                  const int32_t *labels, int nlab, double tgv, double g,
                  double *b, int on_device);
+/* ---- boundary integrals (P1, Robin / Neumann terms) -------------------- */
+/* int2d(Th3, labels)(alpha*u*v) (2D meshes: int1d(Th2, labels)(alpha*u*v))
+ * added to the values of mat, the P1 matrix of this mesh (ffhip_pattern_p1):
+ * AssembleBilinearForm's boundary branch (problem.cpp:1273-1327) over the
+ * boundary elements whose label is listed, with the face rule of Element_Op
+ * (problem.cpp:6515-6561: T_5 on a face, coef = |N|/2 w_q; edges
+ * problem.cpp:6223-6240: QF_GaussLegendre3, coef = |E| w_q).  Call it after
+ * the volume terms and before ffhip_set_bc, the order FreeFEM adds them in.
+ * alpha: kind CONST (c0), SIN3, P0 (coef[boundary element], nbe values) or
+ * QP (coef[f * nq + q], nq = 7 on faces, 3 on edges, at the points
+ * x_q = (1 - s_q - t_q) B0 + s_q B1 + t_q B2 of boundary element f with
+ * vertices B0, B1, B2 in mesh order and (s_q, t_q) the T_5 points; edges:
+ * x_q = (1 - s_q) B0 + s_q B1 with s_q = 1/2, (1 - sqrt(3/5))/2,
+ * (1 + sqrt(3/5))/2).  coef is a device pointer when on_device != 0. */
+int ffhip_assemble_p1_boundary(ffhip_ctx ctx, ffhip_mesh mesh, ffhip_mat mat,
+                               const int32_t *labels, int nlab, int kind,
+                               double c0, const double *coef, int on_device);
+/* b += int2d(Th3, labels)(g*v) (int1d in 2D): the boundary terms of
+ * AssembleLinearForm (Element_rhs on a face, problem.cpp:8517-8590), g
+ * described like alpha above (g0 for CONST).  The rows of the vertices of
+ * boundary elements labelled dlabels (the Dirichlet labels of the problem,
+ * whose rows on(...) sets to tgv*g afterwards) are left unchanged, so the
+ * call can follow ffhip_rhs_p1.  b: n doubles, device pointer when
+ * on_device != 0. */
+int ffhip_rhs_p1_boundary(ffhip_ctx ctx, ffhip_mesh mesh, const int32_t *labels,
+                          int nlab, int kind, double g0, const double *coef,
+                          const int32_t *dlabels, int ndlab, double *b,
+                          int on_device);
 /* ---- adjoint building blocks (3D P1) ----------------------------------- */
 /* J = 1/2 int_Th (u - ud)^2 and the adjoint right-hand side
  * r = M (u - ud) with the rows of the Dirichlet dofs (boundary labels listed)

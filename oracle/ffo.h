@@ -60,7 +60,7 @@ int64_t ffo_pattern_bruteforce(int n, int64_t nel, int nloc,
                                int32_t *colidx);
 
 /* ---- P1 assembly -------------------------------------------------------- */
-enum { FFO_COEF_CONST = 0, FFO_COEF_SIN3 = 1, FFO_COEF_P0 = 2 };
+enum { FFO_COEF_CONST = 0, FFO_COEF_SIN3 = 1, FFO_COEF_P0 = 2, FFO_COEF_QP = 3 };
 /* Coefficient kappa(x,y,z): CONST -> c0; SIN3 -> 1+0.5*sin(2 pi x)sin(2 pi y)
  * sin(2 pi z) evaluated as the script expression; P0 -> kappa_e[element]. */
 /* Element_Op full 3D (problem.cpp:6337-6437) for
@@ -201,6 +201,22 @@ int ffo_cg2(int n, const int64_t *rowptr, const int32_t *colidx,
 int ffo_fgmres(int n, const int64_t *rowptr, const int32_t *colidx, const double *vals,
                const double *b, double *x, double eps, int itmax, int restart, double tgv,
                double *stats);
+
+/* ---- boundary integrals on P1 (ffo_bnd.c) ------------------------------ */
+/* QF_GaussLegendre3 points (on [0,1]) and weights, reference order. */
+int ffo_qf_gl3(double *x, double *w);
+/* int2d(Th3, labels)(alpha*u*v) added to vals (when vals != NULL) and
+ * int2d(Th3, labels)(g*v) added to b (when b != NULL), int1d on 2D meshes
+ * (dim 2: X = xy, elems = triangles, belems = edges), over the boundary
+ * elements whose label is listed.  Coefficient: CONST c0, SIN3, P0
+ * coef[boundary element], QP coef[f * nq + q] at the points of boundary
+ * element f's own rule (vertices in mesh order; nq = 7 on faces, 3 on edges).
+ * Returns 0, -3 (slot missing), -4 (no element holds a boundary element),
+ * -5 (QP point not found). */
+int ffo_p1_boundary(int dim, int nv, int64_t nel, const double *X, const int32_t *elems,
+                    int64_t nbe, const int32_t *belems, const int32_t *blab, int nlab,
+                    const int32_t *labs, int kind, double c0, const double *coef,
+                    const int64_t *rowptr, const int32_t *colidx, double *vals, double *b);
 
 /* ---- scalar P2 (Vh(Th,P2), dof = node) --------------------------------- */
 /* the coefficient of the P1/P2 forms at physical point P (CONST c0, SIN3, P0 ke) */

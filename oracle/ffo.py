@@ -22,7 +22,7 @@ _i = ctypes.c_int
 _l = ctypes.c_int64
 _d = ctypes.c_double
 
-COEF_CONST, COEF_SIN3, COEF_P0 = 0, 1, 2
+COEF_CONST, COEF_SIN3, COEF_P0, COEF_QP = 0, 1, 2, 3
 
 
 def lib():
@@ -76,6 +76,9 @@ def lib():
             "ffo_pcg": (_i, [_i, _i64p, _i32p, _f64p, _f64p, _f64p, _d, _i, _d, _i, _f64p]),
             "ffo_cg2": (_i, [_i, _i64p, _i32p, _f64p, _f64p, _f64p, _f64p, _i, _i, _f64p]),
             "ffo_fgmres": (_i, [_i, _i64p, _i32p, _f64p, _f64p, _f64p, _d, _i, _i, _d, _f64p]),
+            "ffo_qf_gl3": (_i, [_f64p, _f64p]),
+            "ffo_p1_boundary": (_i, [_i, _i, _l, _f64p, _i32p, _l, _i32p, _i32p, _i, _i32p, _i, _d,
+                                     _vp, _vp, _vp, _vp, _vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -214,6 +217,24 @@ def rhs_p1_2d(m, f=1.0):
     b = np.zeros(m["nv"])
     lib().ffo_rhs_p1_2d(m["nv"], m["nt"], m["xy"], m["tris"], f, b)
     return b
+
+
+def p1_boundary(m, labels, kind=COEF_CONST, c0=1.0, coef=None, rowptr=None, colidx=None,
+                vals=None, b=None):
+    """FreeFEM's int2d(Th, labels)(alpha*u*v) added to vals (CSR of the P1
+    pattern) and/or int2d(Th, labels)(g*v) added to b; int1d on a 2D mesh."""
+    dim = 3 if "tets" in m else 2
+    X = m["xyz"] if dim == 3 else m["xy"]
+    el = m["tets"] if dim == 3 else m["tris"]
+    be = m["btri"] if dim == 3 else m["bedges"]
+    labs = np.asarray(labels, np.int32)
+    cf = None if coef is None else np.ascontiguousarray(coef, np.float64)
+    for a in (vals, b):
+        assert a is None or (a.dtype == np.float64 and a.flags.c_contiguous)
+    rc = lib().ffo_p1_boundary(dim, m["nv"], m["nt"], X, el, m["nbe"], be, m["blab"], labs.size,
+                               labs, kind, c0, _ptr(cf), _ptr(rowptr), _ptr(colidx), _ptr(vals),
+                               _ptr(b))
+    assert rc == 0, rc
 
 
 def mark_bc(n, bfaces, blab, nvf, labels):

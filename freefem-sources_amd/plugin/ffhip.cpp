@@ -19,6 +19,7 @@
 //    src/fflib/AFunction2.cpp:87-107).  It recognises
 //        int3d(Th)(c*(dx(u)*dx(v)+dy(u)*dy(v)+dz(u)*dz(v))) [+ on(labels,u=g)]
 //    [+ int3d(Th)(m*u*v)] (or the reaction term alone)
+//    [+ int2d(Th, labels)(alpha*u*v) ...] (Robin terms, P1 only, default T_5)
 //    on a scalar P1 space (the Laplacian also on a scalar P2 space, whose
 //    dof numbering is checked against the device's first-appearance node
 //    numbering) with the default Tet_5 rule; the coefficient c is
@@ -58,6 +59,7 @@ using namespace Fem2D;
 // FreeFEM executable; declared where it is used, as src/fflib/problem.cpp:109 does
 namespace Fem2D {
 void Expandsetoflab(Stack stack, const BC_set &bc, set<long> &setoflab);
+void Expandsetoflab(Stack stack, const CDomainOfIntegration &di, set<int> &setoflab, bool &all);
 }
 
 namespace {
@@ -260,7 +262,37 @@ struct Recognised {
   bool mcoef_const = false;
   double m0 = 0.;
   set<long> labels;           // Dirichlet labels (all BC_set of the form)
+  struct Boundary {           // int2d(Th, labels)(alpha*u*v)
+    C_F0 coef;
+    bool is_const = false;
+    double c0 = 0.;
+    set<int> labels;
+    bool all = true;          // no label list: every boundary element
+  };
+  vector<Boundary> bnd;
 };
+
+// int2d(Th, labels)(alpha*u*v) on P1 with the default face rule: appended to r.bnd
+bool recognise_boundary(Stack stack, const FESpace3 &Uh, const FormBilinear *bf, Recognised &r) {
+  if (bf->di->islevelset()) return false;
+  if (GetAny<const Mesh3 *>((*bf->di->Th)(stack)) != &Uh.Th) return false;
+  if (bf->di->FIT(stack).n != QuadratureFormular_T_5.n) return false;  // default T_5 only
+  const BilinearOperator &Op = *bf->b;
+  if (Op.v.size() != 1) return false;
+  const BilinearOperator::K &ll = *Op.v.begin();
+  if (ll.first.first.first || ll.first.second.first || ll.first.first.second != op_id ||
+      ll.first.second.second != op_id)
+    return false;
+  Recognised::Boundary t;
+  t.coef = ll.second;
+  if (t.coef.EvaluableWithOutStack()) {
+    t.is_const = true;
+    t.c0 = GetAny<double>(t.coef.eval(stack));
+  }
+  Fem2D::Expandsetoflab(stack, *bf->di, t.labels, t.all);
+  r.bnd.push_back(t);
+  return true;
+}
 
 // int3d(Th)(c*(dx(u)*dx(v)+dy(u)*dy(v)+dz(u)*dz(v)) + m*u*v) + on(...) on P1
 // (either part may be absent), or false
@@ -290,9 +322,15 @@ bool recognise(Stack stack, const FESpace3 &Uh, const FESpace3 &Vh, const list<C
       r.labels.insert(labs.begin(), labs.end());
       continue;
     }
-    if (t != tvf->tFB || r.bf) return false;  // exactly one bilinear term
+    if (t != tvf->tFB) return false;
     const FormBilinear *bf = dynamic_cast<const FormBilinear *>(e);
-    if (!bf || bf->di->kind != CDomainOfIntegration::int3d || !bf->di->what.empty()) return false;
+    if (!bf) return false;
+    if (bf->di->kind == CDomainOfIntegration::int2d && r.order == 1) {
+      if (!recognise_boundary(stack, Uh, bf, r)) return false;
+      continue;
+    }
+    if (r.bf) return false;  // exactly one volume term
+    if (bf->di->kind != CDomainOfIntegration::int3d || !bf->di->what.empty()) return false;
     if (GetAny<const Mesh3 *>((*bf->di->Th)(stack)) != &Uh.Th) return false;
     if (bf->di->FIV(stack).n != QuadratureFormular_Tet_5.n) return false;  // default Tet_5 only
     const BilinearOperator &Op = *bf->b;
@@ -483,6 +521,41 @@ void eval_coef_qp(Stack stack, const FESpace3 &Uh, const C_F0 &coef, vector<doub
   *MeshPointStack(stack) = mp;
 }
 
+// a boundary coefficient at the 7 T_5 points of every boundary element, in the
+// order of the device rule: point q of boundary element k is
+// (1 - s_q - t_q) B0 + s_q B1 + t_q B2 for its vertices B in Th.be(k) order,
+// located in the adjacent tetrahedron (Mesh3::BoundaryElement) and evaluated
+// with the label and the outward normal set as Element_Op does on a face
+// (problem.cpp:6515-6528); zero for boundary elements whose label is not listed
+void eval_coef_bqp(Stack stack, const FESpace3 &Uh, const Recognised::Boundary &t,
+                   vector<double> &gq) {
+  const Mesh3 &Th = Uh.Th;
+  const GQuadratureFormular<R2> &FI = QuadratureFormular_T_5;
+  gq.assign((size_t)Th.nbe * FI.n, 0.);
+  MeshPoint mp = *MeshPointStack(stack);
+  for (int k = 0; k < Th.nbe; ++k) {
+    const int lab = Th.be(k).lab;
+    if (!t.all && t.labels.find(lab) == t.labels.end()) continue;
+    int ie = 0;
+    const int it = Th.BoundaryElement(k, ie);
+    const FElement3 K(Uh[it]);
+    const Tet &T = Th[it];
+    R3 NN = T.N(ie);
+    NN /= NN.norme();
+    int loc[3] = {0, 0, 0};
+    for (int j = 0; j < 3; ++j)
+      for (int i = 0; i < 4; ++i)
+        if (Th(T[i]) == Th(Th.be(k)[j])) loc[j] = i;
+    for (int q = 0; q < FI.n; ++q) {
+      const double s = FI[q].x, tt = FI[q].y;
+      const R3 Pt = R3::KHat[loc[0]] * (1 - s - tt) + R3::KHat[loc[1]] * s + R3::KHat[loc[2]] * tt;
+      MeshPointStack(stack)->set(T(Pt), Pt, K, lab, NN, ie);
+      gq[(size_t)k * FI.n + q] = GetAny<double>(t.coef.eval(stack));
+    }
+  }
+  *MeshPointStack(stack) = mp;
+}
+
 void assemble_on_gpu(Stack stack, const FESpace3 &Uh, const Recognised &r, double tgv,
                      Matrice_Creuse<double> &A) {
   const Mesh3 &Th = Uh.Th;
@@ -512,6 +585,23 @@ void assemble_on_gpu(Stack stack, const FESpace3 &Uh, const Recognised &r, doubl
     hip_check(ffhip_assemble_p1_form(ctx, m, M, kind, r.c0, kq.empty() ? nullptr : kq.data(), mkind,
                                      r.m0, mq.empty() ? nullptr : mq.data(), 0),
               "ffhip: assembly");
+  }
+  for (const Recognised::Boundary &t : r.bnd) {  // in the order of the varf's terms
+    vector<int32_t> labs;
+    if (t.all) {
+      set<int> present;
+      for (int k = 0; k < Th.nbe; ++k) present.insert(Th.be(k).lab);
+      labs.assign(present.begin(), present.end());
+    } else {
+      labs.assign(t.labels.begin(), t.labels.end());
+    }
+    if (labs.empty()) continue;
+    vector<double> gq;
+    if (!t.is_const) eval_coef_bqp(stack, Uh, t, gq);
+    hip_check(ffhip_assemble_p1_boundary(ctx, m, M, labs.data(), (int)labs.size(),
+                                         t.is_const ? FFHIP_COEF_CONST : FFHIP_COEF_QP, t.c0,
+                                         gq.empty() ? nullptr : gq.data(), 0),
+              "ffhip: boundary term");
   }
   if (!r.labels.empty()) {
     vector<int32_t> labs(r.labels.begin(), r.labels.end());

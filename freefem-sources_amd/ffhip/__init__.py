@@ -78,6 +78,12 @@ _SIGS = {
     "ffhip_set_bc": (_i, [_vp, _vp, _vp, _vp, _i, _d, _vp]),
     "ffhip_rhs_p1": (_i, [_vp, _vp, _d, _vp, _i, _d, _d, _vp, _i]),
     "ffhip_assemble_p1_boundary": (_i, [_vp, _vp, _vp, _vp, _i, _i, _d, _vp, _i]),
+    "ffhip_mat_coo": (_i, [_vp, _pl, _vp, _vp, _vp, _i]),
+    "ffhip_csr_write_morse": (_i, [_l, _l, _i, _vp, _vp, _vp, ctypes.c_char_p]),
+    "ffhip_mat_write_morse": (_i, [_vp, ctypes.c_char_p]),
+    "ffhip_csr_read_text": (_i, [ctypes.c_char_p, _l, _pl, _pl, ctypes.POINTER(_i), _pl, _vp, _vp,
+                                 _vp]),
+    "ffhip_mat_read_text": (_i, [_vp, ctypes.c_char_p, _pvp]),
     "ffhip_rhs_p1_boundary": (_i, [_vp, _vp, _vp, _i, _i, _d, _vp, _vp, _i, _vp, _i]),
     "ffhip_mat_download": (_i, [_vp, _vp, _vp, _vp]),
     "ffhip_mat_upload": (_i, [_vp, _l, _l, _vp, _vp, _vp, _pvp]),
@@ -382,6 +388,36 @@ class Matrix:
                                                kind, c0, cp, on_dev))
         return self.ctx.last_ms()
 
+    def coo(self, device=False):
+        """[I, J, C] = A: int64 rows and columns (0-based), values, in row
+        order, with FreeFEM's extra (n, n, 0) when A has no (n-1, n-1) entry.
+        device=True returns torch cuda tensors."""
+        cnt = _l()
+        check(lib().ffhip_mat_coo(self.h, ctypes.byref(cnt), None, None, None, 0))
+        k = cnt.value
+        if device:
+            import torch
+            I = torch.empty(k, dtype=torch.int64, device="cuda")
+            J = torch.empty(k, dtype=torch.int64, device="cuda")
+            C = torch.empty(k, dtype=torch.float64, device="cuda")
+            check(lib().ffhip_mat_coo(self.h, ctypes.byref(cnt), _dp(I), _dp(J), _dp(C), 1))
+        else:
+            I, J, C = np.zeros(k, np.int64), np.zeros(k, np.int64), np.zeros(k)
+            check(lib().ffhip_mat_coo(self.h, ctypes.byref(cnt), _hp(I), _hp(J), _hp(C), 0))
+        return I, J, C
+
+    def write_morse(self, path):
+        """`ofstream f(path); f << A;` (FreeFEM's Morse text)"""
+        check(lib().ffhip_mat_write_morse(self.h, os.fsencode(path)))
+
+    @staticmethod
+    def read_text(path, ctx=None):
+        """`ifstream f(path); f >> A;` into a device matrix (Morse or COO text)"""
+        ctx = ctx or context()
+        h = ctypes.c_void_p()
+        check(lib().ffhip_mat_read_text(ctx.h, os.fsencode(path), ctypes.byref(h)))
+        return Matrix(h, ctx)
+
     def set_bc(self, labels, tgv=1e30, want_onbc=False):
         lab = np.asarray(labels, np.int32)
         onbc = np.zeros(self.n, np.uint8) if want_onbc else None
@@ -451,6 +487,32 @@ def rhs_p1_boundary(mesh, b, labels, kind=COEF_CONST, g0=1.0, coef=None, dirichl
                                       _hp(dl) if dl.size else None, dl.size,
                                       _dp(b) if on_dev else _hp(b), on_dev))
     return b
+
+
+def csr_write_morse(path, rowptr, colidx, vals, m=None, half=0):
+    """FreeFEM's Morse text of a host CSR (host-only: no device work)"""
+    rowptr = np.ascontiguousarray(rowptr, np.int64)
+    colidx = np.ascontiguousarray(colidx, np.int32)
+    vals = np.ascontiguousarray(vals, np.float64)
+    n = rowptr.size - 1
+    check(lib().ffhip_csr_write_morse(n, n if m is None else m, half, _hp(rowptr), _hp(colidx),
+                                      _hp(vals), os.fsencode(path)))
+
+
+def csr_read_text(path):
+    """HashMatrix(istream) of a Morse / COO text file as a sorted host CSR:
+    returns (rowptr, colidx, vals, m, half) (host-only: no device work)"""
+    n, m, nnz = _l(), _l(), _l()
+    half = _i()
+    p = os.fsencode(path)
+    check(lib().ffhip_csr_read_text(p, 0, ctypes.byref(n), ctypes.byref(m), ctypes.byref(half),
+                                    ctypes.byref(nnz), None, None, None))
+    rp = np.zeros(n.value + 1, np.int64)
+    ci = np.zeros(nnz.value, np.int32)
+    va = np.zeros(nnz.value)
+    check(lib().ffhip_csr_read_text(p, nnz.value, ctypes.byref(n), ctypes.byref(m),
+                                    ctypes.byref(half), ctypes.byref(nnz), _hp(rp), _hp(ci), _hp(va)))
+    return rp, ci, va, m.value, half.value
 
 
 class P2VSpace:

@@ -147,6 +147,37 @@ int ffhip_rhs_p1_boundary(ffhip_ctx ctx, ffhip_mesh mesh, const int32_t *labels,
                           int nlab, int kind, double g0, const double *coef,
                           const int32_t *dlabels, int ndlab, double *b,
                           int on_device);
+/* ---- matrix exchange: triplets and FreeFEM's text files ---------------- */
+/* [I, J, C] = A (lgmat.cpp:2670-2710): the entries in HashMatrix::COO()
+ * order (rows, then columns, ascending; HashMatrix.cpp:837-843), 0-based, and
+ * — as FreeFEM does when A holds no (n-1, n-1) entry — one more entry
+ * (n, n, 0).  *count receives the number of entries; with I, J, C all NULL
+ * only the count is returned.  I, J: int64 (FreeFEM's KN<long>), C: double;
+ * device pointers when on_device != 0, else host pointers. */
+int ffhip_mat_coo(ffhip_mat mat, int64_t *count, int64_t *I, int64_t *J,
+                  double *C, int on_device);
+/* `ofstream f(path); f << A;` for a CSR-state HashMatrix (HashMatrix.hpp:
+ * 485-508): three '#' lines, "n m half  nnz", then one line per entry,
+ * setw(9) i+1, setw(9) j+1, the value with setprecision(20) (|a| < 1e-305
+ * as 0, RNM_tpl.hpp:60).  Host arrays (a sorted CSR); the first comment
+ * line carries an address, as FreeFEM's does. */
+int ffhip_csr_write_morse(int64_t n, int64_t m, int half, const int64_t *rowptr,
+                          const int32_t *colidx, const double *vals,
+                          const char *path);
+/* The same text for a device matrix (downloaded, then written). */
+int ffhip_mat_write_morse(ffhip_mat mat, const char *path);
+/* `ifstream f(path); f >> A;`: HashMatrix(istream) (HashMatrix.cpp:7-30,
+ * 136-214) reading the Morse text or the COO dump ("(Morse)" / "(COO)" in
+ * the first comment line; other '#' lines skipped; a repeated (i,j) keeps
+ * its last value), as the sorted CSR HashMatrix::CSR gives.  n, m, half and
+ * nnz are always returned; rowptr (n+1), colidx, vals (host) are filled when
+ * given and nnz <= cap (else FFHIP_ERR_ARG). */
+int ffhip_csr_read_text(const char *path, int64_t cap, int64_t *n, int64_t *m,
+                        int *half, int64_t *nnz, int64_t *rowptr,
+                        int32_t *colidx, double *vals);
+/* Read a square matrix file into a device matrix; a half (lower-triangle)
+ * file gives the full symmetric matrix it stands for. */
+int ffhip_mat_read_text(ffhip_ctx ctx, const char *path, ffhip_mat *mat);
 /* ---- adjoint building blocks (3D P1) ----------------------------------- */
 /* J = 1/2 int_Th (u - ud)^2 and the adjoint right-hand side
  * r = M (u - ud) with the rows of the Dirichlet dofs (boundary labels listed)

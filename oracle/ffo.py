@@ -114,6 +114,9 @@ def ref():
             "ref_hm_gmres": (_i, [_vp, _f64p, _f64p, _d, _i, _i, _d, _f64p]),
             "ref_fespace_dofs": (_l, [_i, _f64p, _l, _i32p, _i, _vp, ctypes.POINTER(_i)]),
             "ref_p2_fb": (_i, [_f64p, _f64p, _f64p, _f64p]),
+            "ref_hm_write": (_i, [_vp, ctypes.c_char_p, _i]),
+            "ref_hm_read": (_l, [ctypes.c_char_p, _l, _i64p, _vp, _vp, _vp]),
+            "ref_hm_coo": (_l, [_vp, _i64p, _i64p, _f64p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)

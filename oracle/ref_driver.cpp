@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <fstream>
 #include <iostream>
 #include <vector>
 
@@ -297,6 +298,42 @@ int64_t ref_fespace_dofs(int nv, const double *xyz, int64_t nt, const int32_t *t
   delete sum;
   Th->destroy();  // RefCounter (RefCounter.hpp:55-66)
   return ndf;
+}
+
+// `ofstream f(path); f << A;` (HashMatrix.hpp:485-521): the Morse text after
+// HashMatrix::CSR when csr != 0, else the hash-state COO dump
+int ref_hm_write(void *h, const char *path, int csr) {
+  HMat &A = *(HMat *)h;
+  if (csr) A.CSR();
+  std::ofstream f(path);
+  f << A;
+  return f.good() ? 0 : -1;
+}
+
+// HashMatrix(istream, -1) (HashMatrix.cpp:7-30, 136-214) then CSR: nmh =
+// {n, m, half}; fills rowptr/colidx/vals when nnz <= cap; returns nnz
+int64_t ref_hm_read(const char *path, int64_t cap, int64_t *nmh, int64_t *rowptr, int32_t *colidx,
+                    double *vals) {
+  std::ifstream f(path);
+  HMat A(f, -1);
+  int *ia, *ja;
+  double *aa;
+  A.CSR(ia, ja, aa);
+  nmh[0] = A.n; nmh[1] = A.m; nmh[2] = A.half;
+  if ((int64_t)A.nnz <= cap) {
+    for (int i = 0; i <= A.n; ++i) rowptr[i] = ia[i];
+    for (size_t k = 0; k < A.nnz; ++k) { colidx[k] = ja[k]; vals[k] = aa[k]; }
+  }
+  return (int64_t)A.nnz;
+}
+
+// the HashMatrix::COO() arrays (HashMatrix.cpp:837-843) that [I, J, C] = A
+// copies (lgmat.cpp:2690-2700)
+int64_t ref_hm_coo(void *h, int64_t *I, int64_t *J, double *C) {
+  HMat &A = *(HMat *)h;
+  A.COO();
+  for (size_t k = 0; k < A.nnz; ++k) { I[k] = A.i[k]; J[k] = A.j[k]; C[k] = A.aij[k]; }
+  return (int64_t)A.nnz;
 }
 
 // P2 basis values and gradients on one tet at a reference point

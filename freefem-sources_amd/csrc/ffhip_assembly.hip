@@ -848,6 +848,7 @@ __global__ void k_transpose_map(int64_t n, const int64_t *__restrict__ rowptr,
 
 static unsigned nblocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
 int read_flag(ffhip_ctx ctx, int *d);
+void half_view(ffhip_mat L);
 
 static void upload_t5_once() {
   static bool done = false;
@@ -1283,6 +1284,7 @@ int ffhip_mat_free(ffhip_mat A) {
     (void)hipStreamSynchronize(A->ctx->stream);
     dfree(A->rowptr); dfree(A->colidx); dfree(A->vals); dfree(A->diag); dfree(A->slotmap);
     dfree(A->nodeptr); dfree(A->slot8); dfree(A->big);
+    dfree(A->fptr); dfree(A->fcol); dfree(A->fslot);
     delete A;
   });
 }
@@ -1291,6 +1293,14 @@ int ffhip_spmv(ffhip_ctx ctx, ffhip_mat A, const double *x, double *y, int trans
   return guard([&] {
     FFHIP_REQUIRE(ctx && A && x && y, FFHIP_ERR_ARG, "ffhip_spmv: null argument");
     hipStream_t s = ctx->stream;
+    if (A->half) {  // y = (L + L^T - diag L) x over the full view; symmetric: trans alike
+      half_view(A);
+      ctx->tic();
+      k_spmv_csr<<<nblocks(A->n, 256), 256, 0, s>>>(A->n, A->fptr, A->fcol, A->vals, A->fslot, x, y);
+      FFHIP_CHECK(hipGetLastError());
+      ctx->toc();
+      return;
+    }
     int64_t *tmap = nullptr;
     if (trans) {
       tmap = dalloc<int64_t>(A->nnz);

@@ -1644,6 +1644,8 @@ extern "C" {
 int ffhip_cg_create(ffhip_ctx ctx, ffhip_mat A, int mode, ffhip_cg *out) {
   return guard([&] {
     FFHIP_REQUIRE(ctx && A && out, FFHIP_ERR_ARG, "ffhip_cg_create: null argument");
+    FFHIP_REQUIRE(!A->half, FFHIP_ERR_UNSUPPORTED,
+                  "ffhip_cg_create: a half (sym=1) matrix; expand it with ffhip_mat_full");
     const int prec = mode & 0xf, layout = mode & 0x130;
     const bool index32 = (mode & FFHIP_CG_INDEX32) != 0;
     FFHIP_REQUIRE((prec == FFHIP_CG_FP64 || prec == FFHIP_CG_MIXED) &&

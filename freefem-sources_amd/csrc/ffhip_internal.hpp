@@ -141,6 +141,13 @@ struct ffhip_mat_s {
   int ncomp = 0;               // P2 matrices: 3 for [P2,P2,P2], 1 for scalar P2
   int64_t *nodeptr = nullptr;  // nnodes + 1
   uint8_t *slot8 = nullptr;    // (nt * 10) * 10
+  // half storage (FreeFEM's sym=1): only row >= column is stored; the full
+  // view lists every full row as slots of vals (built once, by half_view)
+  int half = 0;
+  int64_t fnnz = 0;
+  int64_t *fptr = nullptr;     // n + 1
+  int32_t *fcol = nullptr;     // fnnz
+  int64_t *fslot = nullptr;    // fnnz: slot in vals
 };
 
 // [P2,P2,P2] space on a tetrahedral mesh (FESpace3 with GTypeOfFESum of three

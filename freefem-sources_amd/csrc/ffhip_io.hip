@@ -281,7 +281,7 @@ int ffhip_mat_write_morse(ffhip_mat A, const char *path) {
     std::vector<double> va((size_t)A->nnz);
     const int rc = ffhip_mat_download(A, rp.data(), ci.data(), va.data());
     if (rc != FFHIP_OK) throw Error{rc};
-    const int rc2 = ffhip_csr_write_morse(A->n, A->n, 0, rp.data(), ci.data(), va.data(), path);
+    const int rc2 = ffhip_csr_write_morse(A->n, A->n, A->half, rp.data(), ci.data(), va.data(), path);
     if (rc2 != FFHIP_OK) throw Error{rc2};
   });
 }

@@ -79,6 +79,10 @@ _SIGS = {
     "ffhip_rhs_p1": (_i, [_vp, _vp, _d, _vp, _i, _d, _d, _vp, _i]),
     "ffhip_assemble_p1_boundary": (_i, [_vp, _vp, _vp, _vp, _i, _i, _d, _vp, _i]),
     "ffhip_mat_coo": (_i, [_vp, _pl, _vp, _vp, _vp, _i]),
+    "ffhip_mat_lower": (_i, [_vp, _pvp]),
+    "ffhip_mat_set_half": (_i, [_vp, _i]),
+    "ffhip_mat_full": (_i, [_vp, _pvp]),
+    "ffhip_mat_half": (_i, [_vp, ctypes.POINTER(_i)]),
     "ffhip_csr_write_morse": (_i, [_l, _l, _i, _vp, _vp, _vp, ctypes.c_char_p]),
     "ffhip_mat_write_morse": (_i, [_vp, ctypes.c_char_p]),
     "ffhip_csr_read_text": (_i, [ctypes.c_char_p, _l, _pl, _pl, ctypes.POINTER(_i), _pl, _vp, _vp,
@@ -405,6 +409,27 @@ class Matrix:
             I, J, C = np.zeros(k, np.int64), np.zeros(k, np.int64), np.zeros(k)
             check(lib().ffhip_mat_coo(self.h, ctypes.byref(cnt), _hp(I), _hp(J), _hp(C), 0))
         return I, J, C
+
+    def lower(self):
+        """the half (sym=1) matrix: lower triangle of this symmetric matrix"""
+        h = ctypes.c_void_p()
+        check(lib().ffhip_mat_lower(self.h, ctypes.byref(h)))
+        return Matrix(h, self.ctx)
+
+    def full(self):
+        """the full symmetric matrix of a half one"""
+        h = ctypes.c_void_p()
+        check(lib().ffhip_mat_full(self.h, ctypes.byref(h)))
+        return Matrix(h, self.ctx)
+
+    def set_half(self, half=1):
+        check(lib().ffhip_mat_set_half(self.h, int(half)))
+
+    @property
+    def half(self):
+        v = _i()
+        check(lib().ffhip_mat_half(self.h, ctypes.byref(v)))
+        return v.value
 
     def write_morse(self, path):
         """`ofstream f(path); f << A;` (FreeFEM's Morse text)"""

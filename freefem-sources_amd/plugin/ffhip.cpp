@@ -33,6 +33,8 @@
 //    coefficients that add up to the Lame table
 //        C[c][a][c'][b] = lambda d_ca d_c'b + mu (d_cb d_ac' + d_cc' d_ab)
 //    (test component c, derivative a; trial component c', derivative b).
+//    With sym=1 the P1 forms keep the lower triangle (HashMatrix::half), as
+//    FreeFEM's symmetric element matrices do.
 //    Any other form falls back to FreeFEM's own
 //    creationBlockOfMatrixToBilinearForm (problem.hpp:1663-1714), as the
 //    PETSc plugin does for its Mat type.
@@ -556,7 +558,7 @@ void eval_coef_bqp(Stack stack, const FESpace3 &Uh, const Recognised::Boundary &
   *MeshPointStack(stack) = mp;
 }
 
-void assemble_on_gpu(Stack stack, const FESpace3 &Uh, const Recognised &r, double tgv,
+void assemble_on_gpu(Stack stack, const FESpace3 &Uh, const Recognised &r, double tgv, int sym,
                      Matrice_Creuse<double> &A) {
   const Mesh3 &Th = Uh.Th;
   vector<double> xyz(3 * (size_t)Th.nv);
@@ -607,6 +609,12 @@ void assemble_on_gpu(Stack stack, const FESpace3 &Uh, const Recognised &r, doubl
     vector<int32_t> labs(r.labels.begin(), r.labels.end());
     hip_check(ffhip_set_bc(ctx, m, M, labs.data(), (int)labs.size(), tgv, nullptr), "ffhip: on()");
   }
+  if (sym) {  // sym=1: the lower triangle, HashMatrix::half (HashMatrix.cpp:1312-1318)
+    ffhip_mat L = nullptr;
+    hip_check(ffhip_mat_lower(M, &L), "ffhip: sym=1 lower triangle");
+    ffhip_mat_free(M);
+    M = L;
+  }
   int64_t n = 0, nnz = 0;
   ffhip_mat_info(M, &n, &nnz);
   vector<int64_t> rp(n + 1);
@@ -618,8 +626,9 @@ void assemble_on_gpu(Stack stack, const FESpace3 &Uh, const Recognised &r, doubl
   ffhip_mesh_free(m);
   // the result is an ordinary HashMatrix (HashMatrix::set with CSR input,
   // HashMatrix.cpp:697-728) so every later FreeFEM operation works on it
-  MatriceMorse<double> *pH = new MatriceMorse<double>((int)n, (int)n, (int)nnz, 0);
-  pH->set((int)n, (int)n, 0, (size_t)nnz, ia.data(), ja.data(), aa.data(), 0, 1);
+  const int half = sym ? 1 : 0;
+  MatriceMorse<double> *pH = new MatriceMorse<double>((int)n, (int)n, (int)nnz, half);
+  pH->set((int)n, (int)n, half, (size_t)nnz, ia.data(), ja.data(), aa.data(), 0, 1);
   A.A.master(pH);
 }
 
@@ -716,12 +725,12 @@ class OpHipMatrixP1 : public OneOperator {
       Recognised r;
       RecognisedP2 r2;
       bool done = false;
-      if (ds.sym == 0 && ds.tgv > 0 && recognise(stack, *PUh, *PVh, b->largs, r)) {
+      if (ds.tgv > 0 && recognise(stack, *PUh, *PVh, b->largs, r) && (ds.sym == 0 || r.order == 1)) {
         A.A = 0;
         A.Uh = *PUh;
         A.Vh = *PVh;
-        if (r.order == 1) {
-          assemble_on_gpu(stack, *PUh, r, ds.tgv, A);
+        if (r.order == 1) {  // the recognised P1 forms are symmetric: sym=1 keeps the lower half
+          assemble_on_gpu(stack, *PUh, r, ds.tgv, ds.sym, A);
           done = true;
         } else {
           done = assemble_p2_on_gpu(stack, *PUh, r, ds.tgv, A);

@@ -147,6 +147,25 @@ int ffhip_rhs_p1_boundary(ffhip_ctx ctx, ffhip_mesh mesh, const int32_t *labels,
                           int nlab, int kind, double g0, const double *coef,
                           const int32_t *dlabels, int ndlab, double *b,
                           int on_device);
+/* ---- symmetric half storage (sym=1) ------------------------------------- */
+/* The lower triangle (row >= column) of a symmetric device matrix, as a new
+ * half matrix: what `matrix A = va(Vh, Vh, sym=1)` holds (HashMatrix::
+ * operator+= with MatriceElementaire::Symmetric stores (max(i,j), min(i,j)),
+ * HashMatrix.cpp:1312-1318; HashMatrix::half = 1).  ffhip_spmv on a half
+ * matrix computes y = (L + L^T - diag L) x, the half branch of
+ * HashMatrix::addMatMul (HashMatrix.cpp:1128-1136), trans alike;
+ * ffhip_mat_download / ffhip_mat_coo / ffhip_mat_write_morse give the stored
+ * triangle (the text with half = 1). */
+int ffhip_mat_lower(ffhip_mat mat, ffhip_mat *half_mat);
+/* Declare an uploaded matrix (ffhip_mat_upload) half: its entries must all
+ * have column <= row (checked; FFHIP_ERR_ARG otherwise).  half = 0 is
+ * accepted for a full matrix only. */
+int ffhip_mat_set_half(ffhip_mat mat, int half);
+/* The full symmetric matrix of a half one (new device matrix; the solvers
+ * take full matrices). */
+int ffhip_mat_full(ffhip_mat half_mat, ffhip_mat *mat);
+/* *half = 1 for a half matrix, 0 otherwise. */
+int ffhip_mat_half(ffhip_mat mat, int *half);
 /* ---- matrix exchange: triplets and FreeFEM's text files ---------------- */
 /* [I, J, C] = A (lgmat.cpp:2670-2710): the entries in HashMatrix::COO()
  * order (rows, then columns, ascending; HashMatrix.cpp:837-843), 0-based, and

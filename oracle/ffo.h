@@ -167,6 +167,9 @@ int ffo_rhs_bc(int n, const uint8_t *onbc, double tgv, double g, double *b);
 /* ---- SpMV and CG -------------------------------------------------------- */
 void ffo_spmv(int n, const int64_t *rowptr, const int32_t *colidx,
               const double *vals, const double *x, double *y);
+/* y = A x of a half (lower-triangle) matrix, HashMatrix.cpp:1128-1136 */
+void ffo_spmv_half(int n, const int64_t *rowptr, const int32_t *colidx,
+                   const double *vals, const double *x, double *y);
 void ffo_spmv_t(int n, const int64_t *rowptr, const int32_t *colidx,
                 const double *vals, const double *x, double *y);
 /* HashMatrix::gettgv (HashMatrix.cpp:1340-1371), ratio 1e6. */

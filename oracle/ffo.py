@@ -72,6 +72,7 @@ def lib():
             "ffo_rhs_bc": (_i, [_i, _u8p, _d, _d, _f64p]),
             "ffo_spmv": (None, [_i, _i64p, _i32p, _f64p, _f64p, _f64p]),
             "ffo_spmv_t": (None, [_i, _i64p, _i32p, _f64p, _f64p, _f64p]),
+            "ffo_spmv_half": (None, [_i, _i64p, _i32p, _f64p, _f64p, _f64p]),
             "ffo_gettgv": (_d, [_i, _i64p, _i32p, _f64p, ctypes.POINTER(_i)]),
             "ffo_pcg": (_i, [_i, _i64p, _i32p, _f64p, _f64p, _f64p, _d, _i, _d, _i, _f64p]),
             "ffo_cg2": (_i, [_i, _i64p, _i32p, _f64p, _f64p, _f64p, _f64p, _i, _i, _f64p]),
@@ -117,6 +118,7 @@ def ref():
             "ref_hm_write": (_i, [_vp, ctypes.c_char_p, _i]),
             "ref_hm_read": (_l, [ctypes.c_char_p, _l, _i64p, _vp, _vp, _vp]),
             "ref_hm_coo": (_l, [_vp, _i64p, _i64p, _f64p]),
+            "ref_hm_from_csr_half": (_vp, [_i, _i64p, _i32p, _f64p]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -254,6 +256,13 @@ def set_bc(rowptr, colidx, vals, onbc, tgv=1e30):
 
 def rhs_bc(b, onbc, tgv=1e30, g=0.0):
     lib().ffo_rhs_bc(onbc.size, onbc, tgv, g, b)
+
+
+def spmv_half(rowptr, colidx, vals, x):
+    """y = A x of a half (sym=1, lower triangle) matrix"""
+    y = np.zeros(rowptr.size - 1)
+    lib().ffo_spmv_half(rowptr.size - 1, rowptr, colidx, vals, np.ascontiguousarray(x, np.float64), y)
+    return y
 
 
 def spmv(rowptr, colidx, vals, x, trans=False):

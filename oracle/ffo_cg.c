@@ -20,6 +20,20 @@ void ffo_spmv(int n, const int64_t *rowptr, const int32_t *colidx,
   }
 }
 
+/* y = A x for a half matrix (lower triangle stored, HashMatrix::half): the
+ * half branch of addMatMul (HashMatrix.cpp:1128-1136) over the entries in
+ * row-major order: y_i += a x_j, and y_j += a x_i off the diagonal. */
+void ffo_spmv_half(int n, const int64_t *rowptr, const int32_t *colidx,
+                   const double *vals, const double *x, double *y) {
+  memset(y, 0, sizeof(double) * (size_t)n);
+  for (int i = 0; i < n; ++i)
+    for (int64_t k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+      const int j = colidx[k];
+      y[i] += vals[k] * x[j];
+      if (i != j) y[j] += vals[k] * x[i];
+    }
+}
+
 /* y = A^T x (addMatMul with Transpose: i/j swapped, HashMatrix.cpp:1094). */
 void ffo_spmv_t(int n, const int64_t *rowptr, const int32_t *colidx,
                 const double *vals, const double *x, double *y) {

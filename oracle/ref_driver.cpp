@@ -228,6 +228,18 @@ void *ref_hm_from_csr(int n, const int64_t *rowptr, const int32_t *colidx, const
   return A;
 }
 
+// the same for a half (lower-triangle, sym=1) matrix: HashMatrix::half = 1
+void *ref_hm_from_csr_half(int n, const int64_t *rowptr, const int32_t *colidx, const double *vals) {
+  verbosity = 0;
+  const size_t nnz = (size_t)rowptr[n];
+  std::vector<int> ia(n + 1), ja(colidx, colidx + nnz);
+  for (int i = 0; i <= n; ++i) ia[i] = (int)rowptr[i];
+  std::vector<double> aa(vals, vals + nnz);
+  HMat *A = new HMat(n, n, (int)nnz, 1);
+  A->set(n, n, 1, nnz, ia.data(), ja.data(), aa.data(), 0, 1);
+  return A;
+}
+
 // SolverGMRES::dosolver (VirtualSolverCG.hpp:236-255): SetInitWithBC, then the
 // reference fgmres with HMatVirt + the Jacobi diag1 of HMatVirtPrecon and its
 // tgv-row mask.

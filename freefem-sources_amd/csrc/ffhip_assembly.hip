@@ -688,13 +688,30 @@ __global__ void k_set_diag(int64_t n, const uint8_t *__restrict__ onbc,
 // ---------------------------------------------------------------- right-hand side
 // b_v = sum over incident elements (element order) of
 //       sum_q (|K| w_q f) lambda_l(x_q)   (Element_rhs, problem.cpp:7940-7978)
+// b[v] = sum over the incident elements (element order) of int_K f l_v with
+// the element's rule (Element_rhs, problem.cpp:7940-7978); f: CONST f0, SIN3,
+// P0 fcoef[e], QP fcoef[e * nq + q].  Dirichlet rows: tgv * (gvals[v] or g0).
+__device__ inline double rhs_coef(int kind, double f0, const double *__restrict__ fcoef, int64_t e,
+                                  int q, int nq, double x, double y, double z, int dim) {
+  if (kind == FFHIP_COEF_SIN3) {
+    if (dim == 3) return kappa_sin3(x, y, z);
+    const double tp = 2 * M_PI;
+    return 1 + (0.5 * sin(tp * x)) * sin(tp * y);
+  }
+  if (kind == FFHIP_COEF_QP) return fcoef[e * nq + q];
+  if (kind == FFHIP_COEF_P0) return fcoef[e];
+  return f0;
+}
+
 __global__ void k_rhs_p1_3d(int64_t n, const double *__restrict__ xyz,
                             const int32_t *__restrict__ tets, const int64_t *__restrict__ v2e_ptr,
-                            const int32_t *__restrict__ v2e, double f,
-                            const uint8_t *__restrict__ onbc, double tgvg, double *__restrict__ b) {
+                            const int32_t *__restrict__ v2e, int kind, double f0,
+                            const double *__restrict__ fcoef, const uint8_t *__restrict__ onbc,
+                            double tgv, double g0, const double *__restrict__ gvals,
+                            double *__restrict__ b) {
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
-  if (onbc && onbc[v]) { b[v] = tgvg; return; }
+  if (onbc && onbc[v]) { b[v] = tgv * (gvals ? gvals[v] : g0); return; }
   double s = 0.;
   for (int64_t k = v2e_ptr[v]; k < v2e_ptr[v + 1]; ++k) {
     const int32_t idx = v2e[k];
@@ -709,7 +726,15 @@ __global__ void k_rhs_p1_3d(int64_t n, const double *__restrict__ xyz,
     double r = 0.;
     for (int q = 0; q < 14; ++q) {
       const double px = c_qf5_pts[q][0], py = c_qf5_pts[q][1], pz = c_qf5_pts[q][2];
-      const double lam = l == 0 ? 1. - (px + py + pz) : (l == 1 ? px : (l == 2 ? py : pz));
+      const double l0 = 1. - (px + py + pz);
+      const double lam = l == 0 ? l0 : (l == 1 ? px : (l == 2 ? py : pz));
+      double f = f0;
+      if (kind != FFHIP_COEF_CONST) {
+        double P[3];
+        for (int kk = 0; kk < 3; ++kk)
+          P[kk] = l0 * vv[0][kk] + px * vv[1][kk] + py * vv[2][kk] + pz * vv[3][kk];
+        f = rhs_coef(kind, f0, fcoef, e, q, 14, P[0], P[1], P[2], 3);
+      }
       r += (mes * qf5_weight(q)) * f * lam;
     }
     s += r;
@@ -719,11 +744,13 @@ __global__ void k_rhs_p1_3d(int64_t n, const double *__restrict__ xyz,
 
 __global__ void k_rhs_p1_2d(int64_t n, const double *__restrict__ xy,
                             const int32_t *__restrict__ tris, const int64_t *__restrict__ v2e_ptr,
-                            const int32_t *__restrict__ v2e, double f,
-                            const uint8_t *__restrict__ onbc, double tgvg, double *__restrict__ b) {
+                            const int32_t *__restrict__ v2e, int kind, double f0,
+                            const double *__restrict__ fcoef, const uint8_t *__restrict__ onbc,
+                            double tgv, double g0, const double *__restrict__ gvals,
+                            double *__restrict__ b) {
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n) return;
-  if (onbc && onbc[v]) { b[v] = tgvg; return; }
+  if (onbc && onbc[v]) { b[v] = tgv * (gvals ? gvals[v] : g0); return; }
   double s = 0.;
   for (int64_t k = v2e_ptr[v]; k < v2e_ptr[v + 1]; ++k) {
     const int32_t idx = v2e[k];
@@ -737,7 +764,14 @@ __global__ void k_rhs_p1_2d(int64_t n, const double *__restrict__ xy,
     double r = 0.;
     for (int q = 0; q < 7; ++q) {
       const double px = c_qt5_pts[q][0], py = c_qt5_pts[q][1];
-      const double lam = l == 0 ? 1 - px - py : (l == 1 ? px : py);
+      const double l0 = 1 - px - py;
+      const double lam = l == 0 ? l0 : (l == 1 ? px : py);
+      double f = f0;
+      if (kind != FFHIP_COEF_CONST) {
+        const double X = l0 * p[0][0] + px * p[1][0] + py * p[2][0];
+        const double Y = l0 * p[0][1] + px * p[1][1] + py * p[2][1];
+        f = rhs_coef(kind, f0, fcoef, e, q, 7, X, Y, 0., 2);
+      }
       r += (area * c_qt5_w[q]) * f * lam;
     }
     s += r;
@@ -1122,10 +1156,15 @@ int ffhip_set_bc(ffhip_ctx ctx, ffhip_mesh m, ffhip_mat A, const int32_t *labels
   });
 }
 
-int ffhip_rhs_p1(ffhip_ctx ctx, ffhip_mesh m, double f, const int32_t *labels, int nlab,
-                 double tgv, double g, double *b, int on_device) {
+int ffhip_rhs_p1_form(ffhip_ctx ctx, ffhip_mesh m, int kind, double f0, const double *fcoef,
+                      const int32_t *labels, int nlab, double tgv, double g0, const double *gvals,
+                      double *b, int on_device) {
   return guard([&] {
     FFHIP_REQUIRE(ctx && m && b, FFHIP_ERR_ARG, "ffhip_rhs_p1: bad arguments");
+    FFHIP_REQUIRE(kind == FFHIP_COEF_CONST || kind == FFHIP_COEF_SIN3 || kind == FFHIP_COEF_P0 ||
+                      kind == FFHIP_COEF_QP, FFHIP_ERR_ARG, "ffhip_rhs_p1_form: coefficient kind");
+    const bool arr = kind == FFHIP_COEF_P0 || kind == FFHIP_COEF_QP;
+    FFHIP_REQUIRE(!arr || fcoef, FFHIP_ERR_ARG, "ffhip_rhs_p1_form: coefficient array missing");
     hipStream_t s = ctx->stream;
     build_v2e(m);
     const int64_t n = m->nv;
@@ -1141,23 +1180,45 @@ int ffhip_rhs_p1(ffhip_ctx ctx, ffhip_mesh m, double f, const int32_t *labels, i
                                                        nlab, onbc);
     }
     double *bd = on_device ? b : dalloc<double>(n);
+    const double *fd = fcoef, *gd = gvals;
+    double *fown = nullptr, *gown = nullptr;
+    if (arr && !on_device) {
+      const int64_t cnt = kind == FFHIP_COEF_P0 ? m->nt : m->nt * (m->dim == 3 ? 14 : 7);
+      fown = dalloc<double>(cnt);
+      FFHIP_CHECK(hipMemcpyAsync(fown, fcoef, 8 * cnt, hipMemcpyHostToDevice, s));
+      fd = fown;
+    }
+    if (gvals && !on_device) {
+      gown = dalloc<double>(n);
+      FFHIP_CHECK(hipMemcpyAsync(gown, gvals, 8 * n, hipMemcpyHostToDevice, s));
+      gd = gown;
+    }
     if (m->dim == 2) upload_t5_once();
     ctx->tic();
     if (m->dim == 3)
-      k_rhs_p1_3d<<<nblocks(n, 256), 256, 0, s>>>(n, m->xyz, m->elems, m->v2e_ptr, m->v2e, f, onbc,
-                                                   tgv * g, bd);
+      k_rhs_p1_3d<<<nblocks(n, 256), 256, 0, s>>>(n, m->xyz, m->elems, m->v2e_ptr, m->v2e, kind, f0,
+                                                   fd, onbc, tgv, g0, gd, bd);
     else
-      k_rhs_p1_2d<<<nblocks(n, 256), 256, 0, s>>>(n, m->xyz, m->elems, m->v2e_ptr, m->v2e, f, onbc,
-                                                   tgv * g, bd);
+      k_rhs_p1_2d<<<nblocks(n, 256), 256, 0, s>>>(n, m->xyz, m->elems, m->v2e_ptr, m->v2e, kind, f0,
+                                                   fd, onbc, tgv, g0, gd, bd);
     FFHIP_CHECK(hipGetLastError());
     ctx->toc();
     if (!on_device) {
       FFHIP_CHECK(hipMemcpy(b, bd, 8 * n, hipMemcpyDeviceToHost));
       dfree(bd);
     }
+    FFHIP_CHECK(hipStreamSynchronize(s));
     dfree(onbc);
     dfree(labs);
+    dfree(fown);
+    dfree(gown);
   });
+}
+
+int ffhip_rhs_p1(ffhip_ctx ctx, ffhip_mesh m, double f, const int32_t *labels, int nlab,
+                 double tgv, double g, double *b, int on_device) {
+  return ffhip_rhs_p1_form(ctx, m, FFHIP_COEF_CONST, f, nullptr, labels, nlab, tgv, g, nullptr, b,
+                           on_device);
 }
 
 int ffhip_p1_misfit(ffhip_ctx ctx, ffhip_mesh m, const double *u, const double *ud,

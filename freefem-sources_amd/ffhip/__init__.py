@@ -77,6 +77,7 @@ _SIGS = {
     "ffhip_assemble_p1_form": (_i, [_vp, _vp, _vp, _i, _d, _vp, _i, _d, _vp, _i]),
     "ffhip_set_bc": (_i, [_vp, _vp, _vp, _vp, _i, _d, _vp]),
     "ffhip_rhs_p1": (_i, [_vp, _vp, _d, _vp, _i, _d, _d, _vp, _i]),
+    "ffhip_rhs_p1_form": (_i, [_vp, _vp, _i, _d, _vp, _vp, _i, _d, _d, _vp, _vp, _i]),
     "ffhip_assemble_p1_boundary": (_i, [_vp, _vp, _vp, _vp, _i, _i, _d, _vp, _i]),
     "ffhip_mat_coo": (_i, [_vp, _pl, _vp, _vp, _vp, _i]),
     "ffhip_mat_lower": (_i, [_vp, _pvp]),
@@ -491,6 +492,24 @@ def rhs_p1(mesh, f=1.0, labels=(), tgv=1e30, g=0.0, out=None):
         check(lib().ffhip_rhs_p1(mesh.ctx.h, mesh.h, f, _hp(lab), lab.size, tgv, g, _hp(b), 0))
         return b
     check(lib().ffhip_rhs_p1(mesh.ctx.h, mesh.h, f, _hp(lab), lab.size, tgv, g, _dp(out), 1))
+    return out
+
+
+def rhs_p1_form(mesh, kind=COEF_CONST, f0=1.0, fcoef=None, labels=(), tgv=1e30, g0=0.0, gvals=None,
+                out=None):
+    """b = vb(0,Vh) for int(f*v) + on(labels, u=g): f as kind/f0/fcoef (P0: nt
+    values, QP: nt x nq), g per dof (gvals) or constant g0.  out: torch cuda
+    tensor (then fcoef/gvals are device tensors too) or None (numpy)."""
+    lab = np.asarray(labels, np.int32)
+    if out is None:
+        fc = None if fcoef is None else np.ascontiguousarray(fcoef, np.float64)
+        gv = None if gvals is None else np.ascontiguousarray(gvals, np.float64)
+        b = np.zeros(mesh.nv)
+        check(lib().ffhip_rhs_p1_form(mesh.ctx.h, mesh.h, kind, f0, _hp(fc), _hp(lab), lab.size, tgv,
+                                      g0, _hp(gv), _hp(b), 0))
+        return b
+    check(lib().ffhip_rhs_p1_form(mesh.ctx.h, mesh.h, kind, f0, _dp(fcoef), _hp(lab), lab.size, tgv,
+                                  g0, _dp(gvals), _dp(out), 1))
     return out
 
 

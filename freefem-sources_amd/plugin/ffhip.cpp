@@ -38,6 +38,11 @@
 //    Any other form falls back to FreeFEM's own
 //    creationBlockOfMatrixToBilinearForm (problem.hpp:1663-1714), as the
 //    PETSc plugin does for its Mat type.
+//  * a higher-preference "=" / "<-" for (real[int], varf(0, Vh)) on 3D P1:
+//        int3d(Th)(f*v) [+ int2d(Th, labels)(g*v)] [+ on(labels, u=g)]
+//    with f, g evaluated by FreeFEM at the quadrature points (the on() value at
+//    the boundary vertices) and the vector built on the GPU; anything else
+//    goes to FreeFEM's OpArraytoLinearForm code path.
 //  * LinearCG / AffineCG overloads whose operator is a matrix (the affine
 //    ConjuguedGradient2 of LinearCG on the GPU, see HipLinearCG).
 //
@@ -761,6 +766,205 @@ class OpHipMatrixP1 : public OneOperator {
   }
 };
 
+// ---------------------------------------------------------------- linear forms
+typedef Call_FormLinear<v_fes3> CFL3;
+
+// b = va(0, Vh) on P1: int3d(Th)(f*v) [+ int2d(Th, labels)(g*v) ...]
+// [+ on(labels, u=g) ...]; bilinear terms do not enter the vector
+// (AssembleVarForm with no matrix, problem.hpp:1385-1386)
+struct RecognisedRhs {
+  bool has_f = false, f_const = false;
+  C_F0 f;
+  double f0 = 0.;
+  vector<Recognised::Boundary> bnd;
+  vector<const BC_set *> bcs;  // in the order of the form
+  set<long> dlabels;
+};
+
+bool recognise_rhs(Stack stack, const FESpace3 &Vh, const list<C_F0> &largs, RecognisedRhs &r) {
+  if (Vh.N != 1 || Vh.TFE.N() != 1 || Vh.TFE[0] != &DataFE<Mesh3>::P1 || Vh.NbOfDF != Vh.Th.nv)
+    return false;
+  FreeFempp::TypeVarForm<double> *tvf = FreeFempp::TypeVarForm<double>::Global;
+  for (list<C_F0>::const_iterator ii = largs.begin(); ii != largs.end(); ++ii) {
+    const aType t = ii->left();
+    const Expression e = ii->LeftValue();
+    if (t == tvf->tFB) continue;
+    if (t == tvf->tBC) {
+      const BC_set *bc = dynamic_cast<const BC_set *>(e);
+      if (!bc || bc->complextype || bc->bc.size() != 1 || bc->bc[0].first != 0) return false;
+      set<long> labs;
+      Fem2D::Expandsetoflab(stack, *bc, labs);
+      r.dlabels.insert(labs.begin(), labs.end());
+      r.bcs.push_back(bc);
+      continue;
+    }
+    if (t != tvf->tFL) return false;
+    const FormLinear *fl = dynamic_cast<const FormLinear *>(e);
+    if (!fl || fl->di->islevelset()) return false;
+    if (GetAny<const Mesh3 *>((*fl->di->Th)(stack)) != &Vh.Th) return false;
+    const Ftest &L = *fl->l;
+    if (L.v.size() != 1) return false;
+    const MDroit &op = L.v.begin()->first;
+    if (op.first != 0 || op.second != op_id) return false;
+    const C_F0 &coef = L.v.begin()->second;
+    if (fl->di->kind == CDomainOfIntegration::int3d) {
+      if (r.has_f || !fl->di->what.empty()) return false;
+      if (fl->di->FIV(stack).n != QuadratureFormular_Tet_5.n) return false;
+      r.has_f = true;
+      r.f = coef;
+      if (coef.EvaluableWithOutStack()) {
+        r.f_const = true;
+        r.f0 = GetAny<double>(coef.eval(stack));
+      }
+    } else if (fl->di->kind == CDomainOfIntegration::int2d) {
+      if (fl->di->FIT(stack).n != QuadratureFormular_T_5.n) return false;
+      Recognised::Boundary b;
+      b.coef = coef;
+      if (coef.EvaluableWithOutStack()) {
+        b.is_const = true;
+        b.c0 = GetAny<double>(coef.eval(stack));
+      }
+      Fem2D::Expandsetoflab(stack, *fl->di, b.labels, b.all);
+      r.bnd.push_back(b);
+    } else {
+      return false;
+    }
+  }
+  return true;
+}
+
+// on(labels, u=g): g at the vertices of the labelled boundary triangles,
+// evaluated as AssembleBC does for P1 (problem.cpp:10143-10176: the vertex,
+// the label and the outward normal of the face set), later on() terms
+// overwriting earlier ones
+void eval_dirichlet(Stack stack, const FESpace3 &Vh, const RecognisedRhs &r, vector<double> &gv) {
+  const Mesh3 &Th = Vh.Th;
+  gv.assign(Th.nv, 0.);
+  MeshPoint mp = *MeshPointStack(stack);
+  for (const BC_set *bc : r.bcs) {
+    set<long> labs;
+    Fem2D::Expandsetoflab(stack, *bc, labs);
+    for (int k = 0; k < Th.nbe; ++k) {
+      const int lab = Th.be(k).lab;
+      if (labs.find(lab) == labs.end()) continue;
+      int ie = 0;
+      const int it = Th.BoundaryElement(k, ie);
+      const FElement3 K(Vh[it]);
+      const Tet &T = Th[it];
+      R3 NN = T.N(ie);
+      NN /= NN.norme();
+      for (int i = 0; i < 4; ++i) {
+        if (i == ie) continue;  // the vertex opposite the face
+        const R3 PtHat = R3::KHat[i];
+        MeshPointStack(stack)->set(T(PtHat), PtHat, K, lab, NN, ie);
+        gv[Th(T[i])] = GetAny<double>((*bc->bc[0].second)(stack));
+      }
+    }
+  }
+  *MeshPointStack(stack) = mp;
+}
+
+void rhs_on_gpu(Stack stack, const FESpace3 &Vh, const RecognisedRhs &r, double tgv, KN_<double> &b) {
+  const Mesh3 &Th = Vh.Th;
+  vector<double> xyz(3 * (size_t)Th.nv);
+  vector<int32_t> tets(4 * (size_t)Th.nt), btri(3 * (size_t)Th.nbe), blab(Th.nbe);
+  for (int i = 0; i < Th.nv; ++i) {
+    xyz[3 * i] = Th(i).x; xyz[3 * i + 1] = Th(i).y; xyz[3 * i + 2] = Th(i).z;
+  }
+  for (int k = 0; k < Th.nt; ++k)
+    for (int j = 0; j < 4; ++j) tets[4 * k + j] = Th(Th[k][j]);
+  for (int k = 0; k < Th.nbe; ++k) {
+    for (int j = 0; j < 3; ++j) btri[3 * k + j] = Th(Th.be(k)[j]);
+    blab[k] = Th.be(k).lab;
+  }
+  ffhip_ctx ctx = hip_ctx();
+  ffhip_mesh m = nullptr;
+  hip_check(ffhip_mesh3_upload(ctx, Th.nv, Th.nt, Th.nbe, xyz.data(), tets.data(), btri.data(),
+                               blab.data(), &m), "ffhip: mesh upload");
+  vector<double> fq, gv, out(Th.nv);
+  if (r.has_f && !r.f_const) eval_coef_qp(stack, Vh, r.f, fq);
+  if (!r.bcs.empty()) eval_dirichlet(stack, Vh, r, gv);
+  vector<int32_t> dl(r.dlabels.begin(), r.dlabels.end());
+  const int kind = !r.has_f ? FFHIP_COEF_CONST : (r.f_const ? FFHIP_COEF_CONST : FFHIP_COEF_QP);
+  const double f0 = r.has_f ? r.f0 : 0.;
+  hip_check(ffhip_rhs_p1_form(ctx, m, kind, f0, fq.empty() ? nullptr : fq.data(), dl.data(),
+                              (int)dl.size(), tgv, 0., gv.empty() ? nullptr : gv.data(), out.data(), 0),
+            "ffhip: linear form");
+  for (const Recognised::Boundary &t : r.bnd) {
+    vector<int32_t> labs;
+    if (t.all) {
+      set<int> present;
+      for (int k = 0; k < Th.nbe; ++k) present.insert(Th.be(k).lab);
+      labs.assign(present.begin(), present.end());
+    } else {
+      labs.assign(t.labels.begin(), t.labels.end());
+    }
+    if (labs.empty()) continue;
+    vector<double> gq;
+    if (!t.is_const) eval_coef_bqp(stack, Vh, t, gq);
+    hip_check(ffhip_rhs_p1_boundary(ctx, m, labs.data(), (int)labs.size(),
+                                    t.is_const ? FFHIP_COEF_CONST : FFHIP_COEF_QP, t.c0,
+                                    gq.empty() ? nullptr : gq.data(), dl.data(), (int)dl.size(),
+                                    out.data(), 0),
+              "ffhip: boundary linear form");
+  }
+  ffhip_mesh_free(m);
+  for (int i = 0; i < Th.nv; ++i) b[i] = out[i];
+}
+
+// "=" / "<-" for (real[int], varf(0, Vh)) on 3D P1 — preferred over
+// OpArraytoLinearForm<double, Mesh3, v_fes3> (problem.hpp:853-927, registered
+// at lgfem.cpp:6672 and 6688); other forms take FreeFEM's own path, with the
+// same array semantics (problem.hpp:1351-1388)
+class OpHipRhsP1 : public OneOperator {
+ public:
+  const bool isptr, init;
+  class Op : public E_F0mps {
+   public:
+    CFL3 *l;
+    Expression x;
+    const bool isptr, init;
+    Op(Expression xx, Expression ll, bool p, bool i)
+        : l(new CFL3(*dynamic_cast<const CFL3 *>(ll))), x(xx), isptr(p), init(i) {
+      ffassert(l);
+    }
+    operator aType() const { return atype<KN<double> *>(); }
+    AnyType operator()(Stack stack) const {
+      pfes3 &pp = *GetAny<pfes3 *>((*l->ppfes)(stack));
+      FESpace3 *pVh = *pp;
+      double tgv = ff_tgv;
+      if (l->nargs[0]) tgv = GetAny<double>((*l->nargs[0])(stack));
+      const long NbOfDF = pVh ? pVh->NbOfDF : 0;
+      KN<double> *px = 0;
+      if (isptr) {
+        px = GetAny<KN<double> *>((*x)(stack));
+        if (init) px->init(NbOfDF);
+        if (px->N() != NbOfDF) px->resize(NbOfDF);
+      }
+      KN_<double> xx(px ? *(KN_<double> *)px : GetAny<KN_<double> >((*x)(stack)));
+      if (NbOfDF) xx = 0.;
+      RecognisedRhs r;
+      if (pVh && tgv > 0 && xx.N() == NbOfDF && recognise_rhs(stack, *pVh, l->largs, r)) {
+        rhs_on_gpu(stack, *pVh, r, tgv, xx);
+      } else if (pVh &&
+                 AssembleVarForm<double, MatriceCreuse<double>, Mesh3, FESpace3, FESpace3>(
+                     stack, pVh->Th, *pVh, *pVh, false, 0, &xx, l->largs)) {
+        AssembleBC<double, Mesh3, FESpace3, FESpace3>(stack, pVh->Th, *pVh, *pVh, false, 0, &xx, 0,
+                                                      l->largs, tgv);
+      }
+      return SetAny<KN_<double> >(xx);
+    }
+  };
+  E_F0 *code(const basicAC_F0 &args) const {
+    if (isptr) return new Op(to<KN<double> *>(args[0]), args[1], isptr, init);
+    return new Op(to<KN_<double> >(args[0]), args[1], isptr, init);
+  }
+  OpHipRhsP1(const basicForEachType *tt, bool p, bool i)
+      : OneOperator(atype<KN_<double> >(), tt, atype<const CFL3 *>()), isptr(p), init(i) {
+    pref = 10;  // preferred over OpArraytoLinearForm (pref 0)
+  }
+};
+
 // ---------------------------------------------------------------- LinearCG
 // LinearCG(A, x, b, ...) and AffineCG(A, x, ...) / LinearCG(A, x, ...) with a
 // matrix as the operator: ConjuguedGradient2 (MatriceCreuse.hpp:180-240) on
@@ -849,6 +1053,8 @@ static void Load_Init() {
   Global.Add("ffhipreplacecg", "(", new OneOperator1<long, long>(ffhip_replace_cg));
   TheOperators->Add("=", new OpHipMatrixP1(0));
   TheOperators->Add("<-", new OpHipMatrixP1(1));
+  TheOperators->Add("=", new OpHipRhsP1(atype<KN_<double> >(), false, false));
+  TheOperators->Add("<-", new OpHipRhsP1(atype<KN<double> *>(), true, true));
   Global.Add("LinearCG", "(", new HipLinearCG());
   Global.Add("LinearCG", "(", new HipLinearCG(1));
   Global.Add("AffineCG", "(", new HipLinearCG(1));

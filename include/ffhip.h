@@ -119,6 +119,18 @@ int ffhip_set_bc(ffhip_ctx ctx, ffhip_mesh mesh, ffhip_mat mat,
 int ffhip_rhs_p1(ffhip_ctx ctx, ffhip_mesh mesh, double f,
                  const int32_t *labels, int nlab, double tgv, double g,
                  double *b, int on_device);
+/* The same linear form with a coefficient: int3d(Th)(f*v) (int2d in 2D) with
+ * f described like kappa in ffhip_assemble_p1 (kind CONST f0, SIN3, P0
+ * fcoef[element], QP fcoef[element * nq + q] at the Tet_5 / T_5 points — the
+ * values FreeFEM's interpreter gives any expression there, Element_rhs
+ * problem.cpp:7940-7978), and on(labels, u=g) rows set to tgv * g with g per
+ * dof (gvals, n values: g at the dof's vertex, as AssembleBC evaluates it,
+ * problem.cpp:10120-10176) or the constant g0 when gvals is NULL.  fcoef,
+ * gvals and b are device pointers when on_device != 0, else host pointers. */
+int ffhip_rhs_p1_form(ffhip_ctx ctx, ffhip_mesh mesh, int kind, double f0,
+                      const double *fcoef, const int32_t *labels, int nlab,
+                      double tgv, double g0, const double *gvals, double *b,
+                      int on_device);
 /* ---- boundary integrals (P1, Robin / Neumann terms) -------------------- */
 /* int2d(Th3, labels)(alpha*u*v) (2D meshes: int1d(Th2, labels)(alpha*u*v))
  * added to the values of mat, the P1 matrix of this mesh (ffhip_pattern_p1):

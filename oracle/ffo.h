@@ -205,6 +205,11 @@ int ffo_fgmres(int n, const int64_t *rowptr, const int32_t *colidx, const double
                const double *b, double *x, double eps, int itmax, int restart, double tgv,
                double *stats);
 
+/* int3d(Th)(f*v) (int2d in 2D): Element_rhs with a coefficient f (CONST f0,
+ * SIN3, P0 fcoef[element], QP fcoef[element * nq + q]); b zeroed first. */
+int ffo_rhs_p1_coef(int dim, int nv, int64_t nt, const double *X, const int32_t *elems, int kind,
+                    double f0, const double *fcoef, double *b);
+
 /* ---- boundary integrals on P1 (ffo_bnd.c) ------------------------------ */
 /* QF_GaussLegendre3 points (on [0,1]) and weights, reference order. */
 int ffo_qf_gl3(double *x, double *w);

@@ -78,6 +78,7 @@ def lib():
             "ffo_cg2": (_i, [_i, _i64p, _i32p, _f64p, _f64p, _f64p, _f64p, _i, _i, _f64p]),
             "ffo_fgmres": (_i, [_i, _i64p, _i32p, _f64p, _f64p, _f64p, _d, _i, _i, _d, _f64p]),
             "ffo_qf_gl3": (_i, [_f64p, _f64p]),
+            "ffo_rhs_p1_coef": (_i, [_i, _i, _l, _f64p, _i32p, _i, _d, _vp, _f64p]),
             "ffo_p1_boundary": (_i, [_i, _i, _l, _f64p, _i32p, _l, _i32p, _i32p, _i, _i32p, _i, _d,
                                      _vp, _vp, _vp, _vp, _vp]),
         }
@@ -215,6 +216,17 @@ def assemble_p1_2d(m, rowptr, colidx, kind=COEF_CONST, c0=1.0, kappa_e=None, mki
 def rhs_p1_3d(m, f=1.0):
     b = np.zeros(m["nv"])
     lib().ffo_rhs_p1_3d(m["nv"], m["nt"], m["xyz"], m["tets"], f, b)
+    return b
+
+
+def rhs_p1_coef(m, kind, f0=0.0, fcoef=None):
+    """int(f*v) with a coefficient (CONST / SIN3 / P0 / QP), 3D or 2D mesh"""
+    dim = 3 if "tets" in m else 2
+    b = np.zeros(m["nv"])
+    fc = None if fcoef is None else np.ascontiguousarray(fcoef, np.float64)
+    rc = lib().ffo_rhs_p1_coef(dim, m["nv"], m["nt"], m["xyz"] if dim == 3 else m["xy"],
+                               m["tets"] if dim == 3 else m["tris"], kind, f0, _ptr(fc), b)
+    assert rc == 0, rc
     return b
 
 

@@ -405,6 +405,48 @@ int ffo_rhs_p1_2d(int nv, int64_t nt, const double *xy, const int32_t *tris,
   return 0;
 }
 
+/* Element_rhs with a coefficient (problem.cpp:7940-7978): per element and
+ * point, B[K(i)] += coef * f(T(Pt)) * w_i, coef = |K| w_q, T(Pt) as
+ * GenericElement::operator() (GenericMesh.hpp:372-377); f: CONST f0, SIN3,
+ * P0 fcoef[e], QP fcoef[e * nq + q].  dim 3: Tet_5, dim 2: T_5.  b zeroed first. */
+int ffo_rhs_p1_coef(int dim, int nv, int64_t nt, const double *X, const int32_t *elems, int kind,
+                    double f0, const double *fcoef, double *b) {
+  double qp[42], qw[14];
+  const int nq = dim == 3 ? ffo_qf_tet5(qp, qw) : ffo_qf_tri5(qp, qw);
+  const int nloc = dim + 1;
+  memset(b, 0, sizeof(double) * (size_t)nv);
+  for (int64_t e = 0; e < nt; ++e) {
+    const int32_t *t = elems + nloc * e;
+    const double *v[4];
+    for (int i = 0; i < nloc; ++i) v[i] = X + dim * (int64_t)t[i];
+    double mes;
+    if (dim == 3) {
+      mes = ffo_tet_mesure(v[0], v[1], v[2], v[3]);
+    } else {
+      mes = ((v[1][0] - v[0][0]) * (v[2][1] - v[0][1]) - (v[1][1] - v[0][1]) * (v[2][0] - v[0][0])) * 0.5;
+    }
+    for (int q = 0; q < nq; ++q) {
+      const double *Ph = qp + dim * q;
+      /* l0: 1 - PHat.sum() in 3D (P012_3d.cpp), 1 - x - y in 2D (P012_2d.cpp) */
+      const double l0 = dim == 3 ? 1. - (Ph[0] + Ph[1] + Ph[2]) : 1 - Ph[0] - Ph[1];
+      double P[3] = {0, 0, 0}, l[4];
+      for (int k = 0; k < dim; ++k) {
+        double r = l0 * v[0][k];
+        for (int i = 1; i < nloc; ++i) r += Ph[i - 1] * v[i][k];
+        P[k] = r;
+      }
+      l[0] = l0;
+      for (int i = 1; i < nloc; ++i) l[i] = Ph[i - 1];
+      double f;
+      if (kind == FFO_COEF_QP) f = fcoef[e * nq + q];
+      else f = ffo_coef_eval(kind, f0, kind == FFO_COEF_P0 ? fcoef[e] : 0., P, dim);
+      const double coef = mes * qw[q];
+      for (int i = 0; i < nloc; ++i) b[t[i]] += coef * f * l[i];
+    }
+  }
+  return 0;
+}
+
 /* ---- Dirichlet ---------------------------------------------------------- */
 int ffo_mark_bc(int n, int64_t nbf, int nvf, const int32_t *bfaces,
                 const int32_t *blab, int nlab, const int32_t *labs,

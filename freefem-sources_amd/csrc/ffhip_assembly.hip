@@ -686,8 +686,6 @@ __global__ void k_set_diag(int64_t n, const uint8_t *__restrict__ onbc,
 }
 
 // ---------------------------------------------------------------- right-hand side
-// b_v = sum over incident elements (element order) of
-//       sum_q (|K| w_q f) lambda_l(x_q)   (Element_rhs, problem.cpp:7940-7978)
 // b[v] = sum over the incident elements (element order) of int_K f l_v with
 // the element's rule (Element_rhs, problem.cpp:7940-7978); f: CONST f0, SIN3,
 // P0 fcoef[e], QP fcoef[e * nq + q].  Dirichlet rows: tgv * (gvals[v] or g0).

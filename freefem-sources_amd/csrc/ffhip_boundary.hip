@@ -68,9 +68,9 @@ BndRule rule(int dim) {
 __device__ inline double coef_at(int kind, double c0, const double *__restrict__ coef, int64_t e,
                                  int q, int nq, const double *P, int dim) {
   if (kind == FFHIP_COEF_SIN3) {
-    double s = 0.5 * sinpi(2 * P[0]);
-    s = s * sinpi(2 * P[1]);
-    if (dim == 3) s = s * sinpi(2 * P[2]);
+    double s = 0.5 * sin2pi(P[0]);
+    s = s * sin2pi(P[1]);
+    if (dim == 3) s = s * sin2pi(P[2]);
     return 1 + s;
   }
   if (kind == FFHIP_COEF_QP) return coef[e * nq + q];

@@ -82,6 +82,12 @@ struct CgState {
   int it0;
   int itmax;
   double eps;
+  // FP64 PCG: the stop decision of the direction kernel (published as done by
+  // the next mat-vec launch, so every block of the deciding launch still runs)
+  // and the step length of the last update, whose x += rho H the next
+  // direction kernel applies
+  int stop;
+  double rho;
 };
 
 // ---------------------------------------------------------------- reductions
@@ -582,12 +588,13 @@ __global__ FFHIP_MATVEC_ATTR void k_pcg_init(MatFmt<double> M, const double *__r
       });
   const double tot = block_sum<kBlock>(part, lds);
   if (threadIdx.x == 0) pout[xcd_block_id()] = tot;
-  if (blockIdx.x == 0 && threadIdx.x == 0) { st->done = 0; st->iters = 0; }
+  if (blockIdx.x == 0 && threadIdx.x == 0) { st->done = 0; st->stop = 0; st->iters = 0; st->rho = 0.; }
 }
 
 // Start of FP64 PCG iteration it = it0 + c: (g, Cg) from the last update's
 // partials, the stop tests of ConjugueGradient (CG.cpp:226-252) and gamma.
-// Returns false when the solve stops (the lead thread records why).
+// Returns false when the solve stops (the lead thread records why in stop;
+// the next launch turns it into done).
 __device__ inline bool pcg_begin(int c, const double *__restrict__ pin, CgState *st, double *lds,
                                  double &gamma) {
   const int it = st->it0 + c, itmax = st->itmax;
@@ -602,7 +609,7 @@ __device__ inline bool pcg_begin(int c, const double *__restrict__ pin, CgState 
     else if (gcg < 1e-30) done = 2;
     if (lead) {
       st->gcg0 = gcg; st->eps2 = eps2; st->gcg_prev[1] = gcg;
-      if (done) { st->done = done; st->iters = 0; }
+      if (done) { st->stop = done; st->iters = 0; }
     }
   } else {
     const double gcgp = st->gcg_prev[(it - 1) & 1];
@@ -612,7 +619,7 @@ __device__ inline bool pcg_begin(int c, const double *__restrict__ pin, CgState 
     else if (it - 1 >= itmax) done = 3;
     if (lead) {
       st->gcg_prev[it & 1] = gcg;
-      if (done) { st->done = done; st->iters = it - 1; }
+      if (done) { st->stop = done; st->iters = it - 1; }
     }
   }
   return done == 0;
@@ -629,18 +636,35 @@ __device__ inline void block_rows(int64_t n, int64_t &r0, int64_t &r1) {
 // that the mat-vec gathers one vector (forming H on the fly from two gathered
 // vectors per column made the cube(160) mat-vec 17 % slower and the
 // [P2,P2,P2] block mat-vec 50 % slower).
+// It also applies the last update's step x += rho H_prev (ConjugueGradient's
+// x += ro*h, CG.cpp:241), moved here from the update kernel: this pass reads
+// H_prev anyway, so the step costs 16 instead of 24 bytes per row.  x enters
+// nothing else of the iteration; the stopping launch still applies it.
 __global__ __launch_bounds__(kBlock) void k_pcg_dirvec(int c, int64_t n,
                                                        const double *__restrict__ Hold,
                                                        const double *__restrict__ CG,
                                                        double *__restrict__ Hnew,
+                                                       double *__restrict__ x,
                                                        const double *__restrict__ pin, CgState *st) {
   __shared__ double lds[kBlock / kWave];
-  if (st->done) return;
+  if (st->done) return;  // set by an earlier launch only: every block of this one agrees
   double gamma;
-  if (!pcg_begin(c, pin, st, lds, gamma)) return;
+  const bool go = pcg_begin(c, pin, st, lds, gamma);
+  const bool step = st->it0 + c > 1;  // an update ran before this direction
+  const double rho = st->rho;
   int64_t r0, r1;
   block_rows(n, r0, r1);
-  for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) Hnew[r] = gamma * Hold[r] - CG[r];
+  if (go && step) {
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) {
+      const double h = Hold[r];
+      x[r] += rho * h;
+      Hnew[r] = gamma * h - CG[r];
+    }
+  } else if (go) {
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) Hnew[r] = gamma * Hold[r] - CG[r];
+  } else if (step) {
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) x[r] += rho * Hold[r];
+  }
 }
 
 // K2: AH = A H, partials (H, AH), (g, H)
@@ -648,8 +672,12 @@ template <int KIND>
 __global__ FFHIP_MATVEC_ATTR void k_pcg_matvec(MatFmt<double> M, const double *__restrict__ H,
                                                const double *__restrict__ G,
                                                double *__restrict__ AH, double *__restrict__ pout,
-                                               const CgState *st) {
+                                               CgState *st) {
   __shared__ double lds[kBlock / kWave];
+  if (st->stop) {  // the direction kernel stopped the solve: publish it
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->done = st->stop;
+    return;
+  }
   if (st->done) return;
   double p_hah = 0., p_gh = 0.;
   matvec_rows<KIND, double, double>(
@@ -670,21 +698,20 @@ __global__ FFHIP_MATVEC_ATTR void k_pcg_matvec(MatFmt<double> M, const double *_
 }
 
 __global__ __launch_bounds__(kBlock) void k_pcg_update(
-    int64_t n, const double *__restrict__ H, const double *__restrict__ AH,
-    const double *__restrict__ dinv, double *__restrict__ x, double *__restrict__ G,
-    double *__restrict__ CG, const double *__restrict__ pin, double *__restrict__ pout,
-    CgState *st) {
+    int64_t n, const double *__restrict__ AH, const double *__restrict__ dinv,
+    double *__restrict__ G, double *__restrict__ CG, const double *__restrict__ pin,
+    double *__restrict__ pout, CgState *st) {
   __shared__ double lds[kBlock / kWave];
   if (st->done) return;
   const int P = gridDim.x;
   const double hah = reduce_partials(pin, P, 2, 0, lds);
   const double gh = reduce_partials(pin, P, 2, 1, lds);
   const double rho = -gh / hah;
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->rho = rho;  // x += rho H: next direction kernel
   int64_t r0, r1;
   block_rows(n, r0, r1);
   double part = 0.;
   for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) {
-    x[r] += rho * H[r];
     const double g = G[r] + rho * AH[r];
     G[r] = g;
     const double c = dinv[r] * g;
@@ -1507,13 +1534,13 @@ static void launch_pcg_chunk(ffhip_cg cg, const double *sval, bool marks) {
   const MatFmt<double> M = cg->fmt_of(sval);
   for (int c = 0; c < kChunk; ++c) {
     if (marks) cg->mark(4 * c);
-    k_pcg_dirvec<<<G, kBlock, 0, s>>>(c, cg->n, cg->H[c & 1], cg->CG, cg->H[(c + 1) & 1], cg->pA,
-                                      cg->st);
+    k_pcg_dirvec<<<G, kBlock, 0, s>>>(c, cg->n, cg->H[c & 1], cg->CG, cg->H[(c + 1) & 1], cg->xw,
+                                      cg->pA, cg->st);
     if (marks) cg->mark(4 * c + 1);
     FFHIP_LAUNCH_FMT(cg, k_pcg_matvec, G, s, M, cg->H[(c + 1) & 1], cg->G, cg->AH, cg->pB, cg->st);
     if (marks) cg->mark(4 * c + 2);
-    k_pcg_update<<<G, kBlock, 0, s>>>(cg->n, cg->H[(c + 1) & 1], cg->AH, cg->dinv, cg->xw, cg->G,
-                                      cg->CG, cg->pB, cg->pA, cg->st);
+    k_pcg_update<<<G, kBlock, 0, s>>>(cg->n, cg->AH, cg->dinv, cg->G, cg->CG, cg->pB, cg->pA,
+                                      cg->st);
     if (marks) cg->mark(4 * c + 3);
   }
   k_cg_advance<<<1, 64, 0, s>>>(cg->st, kChunk);

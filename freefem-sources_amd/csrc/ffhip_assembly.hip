@@ -701,8 +701,22 @@ __device__ inline double rhs_coef(int kind, double f0, const double *__restrict_
   return f0;
 }
 
+// |K| of every tet, once (the right-hand side gathers it for each of the 4 incidences)
+__global__ void k_tet_mes(int64_t nt, const double *__restrict__ xyz, const int32_t *__restrict__ tets,
+                          double *__restrict__ mes) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nt) return;
+  double v[4][3];
+  const int4 tv = reinterpret_cast<const int4 *>(tets)[e];
+  const int32_t id[4] = {tv.x, tv.y, tv.z, tv.w};
+  for (int i = 0; i < 4; ++i)
+    for (int k = 0; k < 3; ++k) v[i][k] = xyz[3 * (int64_t)id[i] + k];
+  mes[e] = tet_mesure(v);
+}
+
 __global__ void k_rhs_p1_3d(int64_t n, const double *__restrict__ xyz,
-                            const int32_t *__restrict__ tets, const int64_t *__restrict__ v2e_ptr,
+                            const int32_t *__restrict__ tets, const double *__restrict__ tmes,
+                            const int64_t *__restrict__ v2e_ptr,
                             const int32_t *__restrict__ v2e, int kind, double f0,
                             const double *__restrict__ fcoef, const uint8_t *__restrict__ onbc,
                             double tgv, double g0, const double *__restrict__ gvals,
@@ -715,12 +729,14 @@ __global__ void k_rhs_p1_3d(int64_t n, const double *__restrict__ xyz,
     const int32_t idx = v2e[k];
     const int64_t e = idx >> 2;
     const int l = idx & 3;
-    const int4 tv = reinterpret_cast<const int4 *>(tets)[e];
-    const int32_t id[4] = {tv.x, tv.y, tv.z, tv.w};
     double vv[4][3];
-    for (int i = 0; i < 4; ++i)
-      for (int kk = 0; kk < 3; ++kk) vv[i][kk] = xyz[3 * (int64_t)id[i] + kk];
-    const double mes = tet_mesure(vv);
+    if (kind != FFHIP_COEF_CONST) {  // the points are needed for f
+      const int4 tv = reinterpret_cast<const int4 *>(tets)[e];
+      const int32_t id[4] = {tv.x, tv.y, tv.z, tv.w};
+      for (int i = 0; i < 4; ++i)
+        for (int kk = 0; kk < 3; ++kk) vv[i][kk] = xyz[3 * (int64_t)id[i] + kk];
+    }
+    const double mes = tmes[e];
     double r = 0.;
     for (int q = 0; q < 14; ++q) {
       const double px = c_qf5_pts[q][0], py = c_qf5_pts[q][1], pz = c_qf5_pts[q][2];
@@ -1192,10 +1208,13 @@ int ffhip_rhs_p1_form(ffhip_ctx ctx, ffhip_mesh m, int kind, double f0, const do
       gd = gown;
     }
     if (m->dim == 2) upload_t5_once();
+    double *tmes = m->dim == 3 ? dalloc<double>(m->nt) : nullptr;
     ctx->tic();
-    if (m->dim == 3)
-      k_rhs_p1_3d<<<nblocks(n, 256), 256, 0, s>>>(n, m->xyz, m->elems, m->v2e_ptr, m->v2e, kind, f0,
-                                                   fd, onbc, tgv, g0, gd, bd);
+    if (m->dim == 3) {
+      k_tet_mes<<<nblocks(m->nt, 256), 256, 0, s>>>(m->nt, m->xyz, m->elems, tmes);
+      k_rhs_p1_3d<<<nblocks(n, 256), 256, 0, s>>>(n, m->xyz, m->elems, tmes, m->v2e_ptr, m->v2e, kind,
+                                                   f0, fd, onbc, tgv, g0, gd, bd);
+    }
     else
       k_rhs_p1_2d<<<nblocks(n, 256), 256, 0, s>>>(n, m->xyz, m->elems, m->v2e_ptr, m->v2e, kind, f0,
                                                    fd, onbc, tgv, g0, gd, bd);
@@ -1210,6 +1229,7 @@ int ffhip_rhs_p1_form(ffhip_ctx ctx, ffhip_mesh m, int kind, double f0, const do
     dfree(labs);
     dfree(fown);
     dfree(gown);
+    dfree(tmes);
   });
 }
 

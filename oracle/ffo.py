@@ -102,6 +102,7 @@ def ref():
         sig = {
             "ref_qf_tet5": (_i, [_f64p, _f64p]),
             "ref_qf_tri5": (_i, [_f64p, _f64p]),
+            "ref_qf_gl3": (_i, [_f64p, _f64p]),
             "ref_tet_geom": (_i, [_f64p, _f64p, _f64p]),
             "ref_hm_new": (_vp, [_i]),
             "ref_hm_free": (None, [_vp]),

@@ -98,6 +98,13 @@ int ref_qf_tri5(double *pts, double *w) {
   return Q.n;
 }
 
+// QF_GaussLegendre3 (QuadratureFormular.cpp:320-343): points on [0,1], weights
+int ref_qf_gl3(double *x, double *w) {
+  const QuadratureFormular1d &Q = QF_GaussLegendre3;
+  for (int q = 0; q < Q.n; ++q) { x[q] = Q[q].x; w[q] = Q[q].a; }
+  return Q.n;
+}
+
 int ref_tet_geom(const double *v12, double *mes, double *G12) {
   Vertex3 v[4];
   for (int i = 0; i < 4; ++i) {

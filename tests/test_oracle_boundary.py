@@ -200,3 +200,10 @@ def test_p0_coefficient_is_linear():
             ffo.p1_boundary(mf, (m["blab"][f],), ffo.COEF_CONST, ce[f], rowptr=rp, colidx=ci,
                             vals=ref)
     assert np.abs(va - ref).max() <= 1e-15 * np.abs(ref).max() * 10
+
+
+@pytest.mark.skipif(not ffo.ref_available(), reason="oracle/_ref not built")
+def test_gauss_legendre3_table_bitexact_vs_reference():
+    x, w, xr, wr = np.zeros(3), np.zeros(3), np.zeros(3), np.zeros(3)
+    assert ffo.lib().ffo_qf_gl3(x, w) == 3 and ffo.ref().ref_qf_gl3(xr, wr) == 3
+    assert np.array_equal(x, xr) and np.array_equal(w, wr)

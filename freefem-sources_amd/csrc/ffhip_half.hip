@@ -159,12 +159,15 @@ void half_view(ffhip_mat L) {
   k_strict_count<<<nblocks(n, 256), 256, 0, s>>>(n, L->rowptr, L->colidx, scnt);
   exclusive_scan_i64(ctx, scnt, spos, n + 1);
   const int64_t m = read_i64(ctx, spos + n);  // strictly lower entries
+  if (m >= INT32_MAX) {
+    dfree(scnt); dfree(spos);
+    FFHIP_REQUIRE(false, FFHIP_ERR_UNSUPPORTED, "half matrix: too many entries");
+  }
   unsigned long long *ccnt = dalloc<unsigned long long>(n);
   uint64_t *keys = dalloc<uint64_t>(m), *skeys = dalloc<uint64_t>(m);
   int64_t *slots = dalloc<int64_t>(m), *sslots = dalloc<int64_t>(m);
   FFHIP_CHECK(hipMemsetAsync(ccnt, 0, 8 * n, s));
   k_half_keys<<<nblocks(n, 256), 256, 0, s>>>(n, L->rowptr, L->colidx, spos, ccnt, keys, slots);
-  FFHIP_REQUIRE(m < INT32_MAX, FFHIP_ERR_UNSUPPORTED, "half matrix: too many entries");
   size_t tb = 0;
   char *tmp = nullptr;
   if (m) {

@@ -492,6 +492,39 @@ int ffhip_mesh3_upload(ffhip_ctx ctx, int64_t nv, int64_t nt, int64_t nbe,
   });
 }
 
+int ffhip_mesh2_upload(ffhip_ctx ctx, int64_t nv, int64_t nt, int64_t nbe, const double *xy,
+                       const int32_t *tris, const int32_t *bedges, const int32_t *blab,
+                       ffhip_mesh *out) {
+  return guard([&] {
+    FFHIP_REQUIRE(ctx && out && xy && tris && nv > 0 && nt > 0 && nbe >= 0, FFHIP_ERR_ARG,
+                  "ffhip_mesh2_upload: bad arguments");
+    FFHIP_REQUIRE(nbe == 0 || (bedges && blab), FFHIP_ERR_ARG, "ffhip_mesh2_upload: boundary arrays");
+    FFHIP_REQUIRE(nv < (1ll << 31) && 3 * nt < (1ll << 31), FFHIP_ERR_UNSUPPORTED,
+                  "ffhip_mesh2_upload: mesh too large for int32 ids");
+    for (int64_t k = 0; k < 3 * nt; ++k)
+      FFHIP_REQUIRE(tris[k] >= 0 && tris[k] < nv, FFHIP_ERR_ARG,
+                    "ffhip_mesh2_upload: triangle vertex id %d out of range", tris[k]);
+    for (int64_t k = 0; k < 2 * nbe; ++k)
+      FFHIP_REQUIRE(bedges[k] >= 0 && bedges[k] < nv, FFHIP_ERR_ARG,
+                    "ffhip_mesh2_upload: boundary vertex id out of range");
+    ffhip_mesh m = new ffhip_mesh_s();
+    m->ctx = ctx; m->dim = 2; m->nloc = 3; m->nbloc = 2;
+    m->nv = nv; m->nt = nt; m->nbe = nbe;
+    m->xyz = dalloc<double>(2 * nv);
+    m->elems = dalloc<int32_t>(3 * nt);
+    m->belems = dalloc<int32_t>(2 * nbe);
+    m->blab = dalloc<int32_t>(nbe);
+    FFHIP_CHECK(hipMemcpyAsync(m->xyz, xy, 16 * nv, hipMemcpyHostToDevice, ctx->stream));
+    FFHIP_CHECK(hipMemcpyAsync(m->elems, tris, 12 * nt, hipMemcpyHostToDevice, ctx->stream));
+    if (nbe) {
+      FFHIP_CHECK(hipMemcpyAsync(m->belems, bedges, 8 * nbe, hipMemcpyHostToDevice, ctx->stream));
+      FFHIP_CHECK(hipMemcpyAsync(m->blab, blab, 4 * nbe, hipMemcpyHostToDevice, ctx->stream));
+    }
+    FFHIP_CHECK(hipStreamSynchronize(ctx->stream));
+    *out = m;
+  });
+}
+
 int ffhip_mesh_info(ffhip_mesh m, int64_t *nv, int64_t *nt, int64_t *nbe) {
   return guard([&] {
     FFHIP_REQUIRE(m, FFHIP_ERR_ARG, "ffhip_mesh_info: null mesh");

@@ -68,6 +68,7 @@ _SIGS = {
     "ffhip_mesh3_cube": (_i, [_vp, _i, _i, _i, _vp, _pvp]),
     "ffhip_mesh2_square": (_i, [_vp, _i, _i, _vp, _pvp]),
     "ffhip_mesh3_upload": (_i, [_vp, _l, _l, _l, _vp, _vp, _vp, _vp, _pvp]),
+    "ffhip_mesh2_upload": (_i, [_vp, _l, _l, _l, _vp, _vp, _vp, _vp, _pvp]),
     "ffhip_mesh_info": (_i, [_vp, _pl, _pl, _pl]),
     "ffhip_mesh_download": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "ffhip_mesh_free": (_i, [_vp]),
@@ -263,6 +264,19 @@ def square(nx, ny=None, labels=(1, 2, 3, 4), ctx=None):
     lab = np.asarray(labels, np.int32)
     h = ctypes.c_void_p()
     check(lib().ffhip_mesh2_square(ctx.h, nx, ny, _hp(lab), ctypes.byref(h)))
+    return Mesh(h, ctx, 2)
+
+
+def mesh2_upload(xy, tris, bedges, blab, ctx=None):
+    """a 2D mesh from host arrays (FreeFEM Mesh: vertices, triangles, boundary edges)"""
+    ctx = ctx or context()
+    xy = np.ascontiguousarray(xy, np.float64)
+    tris = np.ascontiguousarray(tris, np.int32)
+    bedges = np.ascontiguousarray(bedges, np.int32)
+    blab = np.ascontiguousarray(blab, np.int32)
+    h = ctypes.c_void_p()
+    check(lib().ffhip_mesh2_upload(ctx.h, xy.size // 2, tris.size // 3, blab.size, _hp(xy), _hp(tris),
+                                   _hp(bedges), _hp(blab), ctypes.byref(h)))
     return Mesh(h, ctx, 2)
 
 

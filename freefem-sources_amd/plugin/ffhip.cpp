@@ -38,6 +38,9 @@
 //    Any other form falls back to FreeFEM's own
 //    creationBlockOfMatrixToBilinearForm (problem.hpp:1663-1714), as the
 //    PETSc plugin does for its Mat type.
+//  * the same for 2D P1 forms (Vh on a mesh):
+//        int2d(Th)(c*(dx(u)*dx(v)+dy(u)*dy(v))) [+ int2d(Th)(m*u*v)]
+//        [+ int1d(Th, labels)(alpha*u*v)] [+ on(labels, u=g)]
 //  * a higher-preference "=" / "<-" for (real[int], varf(0, Vh)) on 3D P1:
 //        int3d(Th)(f*v) [+ int2d(Th, labels)(g*v)] [+ on(labels, u=g)]
 //    with f, g evaluated by FreeFEM at the quadrature points (the on() value at
@@ -766,6 +769,276 @@ class OpHipMatrixP1 : public OneOperator {
   }
 };
 
+// ---------------------------------------------------------------- 2D P1
+typedef Call_FormBilinear<v_fes, v_fes> CFB2;
+
+// int2d(Th)(c*(dx(u)*dx(v)+dy(u)*dy(v)) + m*u*v) [+ int1d(Th, labels)(alpha*u*v)]
+// [+ on(labels, u=g)] on a 2D P1 space, the default T_5 / Gauss-Legendre 3 rules
+bool recognise2(Stack stack, const FESpace &Uh, const FESpace &Vh, const list<C_F0> &largs,
+                Recognised &r) {
+  if (&Uh != &Vh || Uh.N != 1 || Uh.TFE.N() != 1 || Uh.TFE[0] != &P1Lagrange) return false;
+  if (Uh.NbOfDF != Uh.Th.nv) return false;
+  r.order = 1;
+  FreeFempp::TypeVarForm<double> *tvf = FreeFempp::TypeVarForm<double>::Global;
+  for (list<C_F0>::const_iterator ii = largs.begin(); ii != largs.end(); ++ii) {
+    const aType t = ii->left();
+    const Expression e = ii->LeftValue();
+    if (t == tvf->tFL) continue;
+    if (t == tvf->tBC) {
+      const BC_set *bc = dynamic_cast<const BC_set *>(e);
+      if (!bc) return false;
+      for (size_t k = 0; k < bc->bc.size(); ++k)
+        if (bc->bc[k].first != 0) return false;
+      set<long> labs;
+      Fem2D::Expandsetoflab(stack, *bc, labs);
+      r.labels.insert(labs.begin(), labs.end());
+      continue;
+    }
+    if (t != tvf->tFB) return false;
+    const FormBilinear *bf = dynamic_cast<const FormBilinear *>(e);
+    if (!bf || bf->di->islevelset()) return false;
+    if (GetAny<pmesh>((*bf->di->Th)(stack)) != &Uh.Th) return false;
+    const BilinearOperator &Op = *bf->b;
+    if (bf->di->kind == CDomainOfIntegration::int1d) {  // Robin term
+      if (bf->di->FIE(stack).n != QF_GaussLegendre3.n || Op.v.size() != 1) return false;
+      const BilinearOperator::K &ll = *Op.v.begin();
+      if (ll.first.first.first || ll.first.second.first || ll.first.first.second != op_id ||
+          ll.first.second.second != op_id)
+        return false;
+      Recognised::Boundary b;
+      b.coef = ll.second;
+      if (b.coef.EvaluableWithOutStack()) {
+        b.is_const = true;
+        b.c0 = GetAny<double>(b.coef.eval(stack));
+      }
+      Fem2D::Expandsetoflab(stack, *bf->di, b.labels, b.all);
+      r.bnd.push_back(b);
+      continue;
+    }
+    if (r.bf || bf->di->kind != CDomainOfIntegration::int2d || !bf->di->what.empty()) return false;
+    if (bf->di->FIT(stack).n != QuadratureFormular_T_5.n) return false;
+    bool seen[3] = {false, false, false};
+    int ngrad = 0;
+    for (BilinearOperator::const_iterator l = Op.v.begin(); l != Op.v.end(); ++l) {
+      const BilinearOperator::K &ll = *l;
+      const int jcomp = ll.first.first.first, jop = ll.first.first.second;
+      const int icomp = ll.first.second.first, iop = ll.first.second.second;
+      if (jcomp || icomp || jop != iop || jop < op_id || jop > op_dy || seen[jop]) return false;
+      seen[jop] = true;
+      if (jop == op_id) {
+        r.has_mass = true;
+        r.mcoef = ll.second;
+      } else if (ngrad++ == 0) {
+        r.coef = ll.second;
+      } else if (ll.second.LeftValue() != r.coef.LeftValue()) {
+        return false;
+      }
+    }
+    if (ngrad != 0 && ngrad != 2) return false;
+    r.has_grad = ngrad == 2;
+    if (!r.has_grad && !r.has_mass) return false;
+    r.bf = bf;
+  }
+  if (!r.bf) return false;
+  if (r.has_grad && r.coef.EvaluableWithOutStack()) {
+    r.coef_const = true;
+    r.c0 = GetAny<double>(r.coef.eval(stack));
+  }
+  if (!r.has_grad) {
+    r.coef_const = true;
+    r.c0 = 0.;
+  }
+  if (r.has_mass && r.mcoef.EvaluableWithOutStack()) {
+    r.mcoef_const = true;
+    r.m0 = GetAny<double>(r.mcoef.eval(stack));
+  }
+  return true;
+}
+
+// a coefficient at the 7 T_5 points of every triangle (Element_Op 2D)
+void eval_coef_qp2(Stack stack, const FESpace &Uh, const C_F0 &coef, vector<double> &kq) {
+  const Mesh &Th = Uh.Th;
+  const QuadratureFormular &FI = QuadratureFormular_T_5;
+  kq.resize((size_t)Th.nt * FI.n);
+  MeshPoint mp = *MeshPointStack(stack);
+  for (int it = 0; it < Th.nt; ++it) {
+    const FElement K(Uh[it]);
+    for (int q = 0; q < FI.n; ++q) {
+      const R2 Pt(FI[q]);
+      MeshPointStack(stack)->set(K.T(Pt), Pt, K);
+      kq[(size_t)it * FI.n + q] = GetAny<double>(coef.eval(stack));
+    }
+  }
+  *MeshPointStack(stack) = mp;
+}
+
+// a boundary coefficient at the 3 Gauss-Legendre points of every boundary
+// edge in the device rule's order, x_q = (1 - s_q) B0 + s_q B1 for its
+// vertices in Th.bedges order, located in the adjacent triangle with the
+// label and the outward normal set as Element_Op does on an edge
+// (problem.cpp:6226-6237); zero for edges whose label is not listed
+void eval_coef_bqp2(Stack stack, const FESpace &Uh, const Recognised::Boundary &t,
+                    vector<double> &gq) {
+  const Mesh &Th = Uh.Th;
+  const QuadratureFormular1d &FI = QF_GaussLegendre3;
+  gq.assign((size_t)Th.neb * FI.n, 0.);
+  // the device rule's points, in its order (s = 1/2, (1 - sqrt(3/5))/2, (1 + sqrt(3/5))/2)
+  const double g1 = (1 - sqrt(3. / 5.)) / 2;
+  const double sq[3] = {0.5, g1, 1 - g1};
+  MeshPoint mp = *MeshPointStack(stack);
+  for (int k = 0; k < Th.neb; ++k) {
+    const int lab = Th.bedges[k].lab;
+    if (!t.all && t.labels.find(lab) == t.labels.end()) continue;
+    int ie = 0;
+    const int it = Th.BoundaryElement(k, ie);
+    const FElement K(Uh[it]);
+    const Triangle &T = Th[it];
+    const R2 E = T.Edge(ie);
+    const double le = sqrt((E, E));
+    const R2 NN(E.y / le, -E.x / le);
+    int loc[2] = {0, 0};
+    for (int j = 0; j < 2; ++j)
+      for (int i = 0; i < 3; ++i)
+        if (&T[i] == &Th.bedges[k][j]) loc[j] = i;
+    for (int q = 0; q < FI.n; ++q) {
+      const R2 Pt = TriangleHat[loc[0]] * (1 - sq[q]) + TriangleHat[loc[1]] * sq[q];
+      MeshPointStack(stack)->set(T(Pt), Pt, K, lab, NN, ie);
+      gq[(size_t)k * FI.n + q] = GetAny<double>(t.coef.eval(stack));
+    }
+  }
+  *MeshPointStack(stack) = mp;
+}
+
+void assemble2_on_gpu(Stack stack, const FESpace &Uh, const Recognised &r, double tgv, int sym,
+                      Matrice_Creuse<double> &A) {
+  const Mesh &Th = Uh.Th;
+  vector<double> xy(2 * (size_t)Th.nv);
+  vector<int32_t> tris(3 * (size_t)Th.nt), bed(2 * (size_t)Th.neb), blab(Th.neb);
+  for (int i = 0; i < Th.nv; ++i) {
+    xy[2 * i] = Th(i).x; xy[2 * i + 1] = Th(i).y;
+  }
+  for (int k = 0; k < Th.nt; ++k)
+    for (int j = 0; j < 3; ++j) tris[3 * k + j] = Th(Th[k][j]);
+  for (int k = 0; k < Th.neb; ++k) {
+    for (int j = 0; j < 2; ++j) bed[2 * k + j] = Th(Th.bedges[k][j]);
+    blab[k] = Th.bedges[k].lab;
+  }
+  ffhip_ctx ctx = hip_ctx();
+  ffhip_mesh m = nullptr;
+  ffhip_mat M = nullptr;
+  hip_check(ffhip_mesh2_upload(ctx, Th.nv, Th.nt, Th.neb, xy.data(), tris.data(), bed.data(),
+                               blab.data(), &m), "ffhip: 2D mesh upload");
+  hip_check(ffhip_pattern_p1(ctx, m, &M), "ffhip: pattern");
+  {
+    vector<double> kq, mq;
+    if (!r.coef_const) eval_coef_qp2(stack, Uh, r.coef, kq);
+    if (r.has_mass && !r.mcoef_const) eval_coef_qp2(stack, Uh, r.mcoef, mq);
+    const int kind = r.coef_const ? FFHIP_COEF_CONST : FFHIP_COEF_QP;
+    const int mkind = !r.has_mass ? FFHIP_COEF_NONE : (r.mcoef_const ? FFHIP_COEF_CONST : FFHIP_COEF_QP);
+    hip_check(ffhip_assemble_p1_form(ctx, m, M, kind, r.c0, kq.empty() ? nullptr : kq.data(), mkind,
+                                     r.m0, mq.empty() ? nullptr : mq.data(), 0),
+              "ffhip: assembly");
+  }
+  for (const Recognised::Boundary &t : r.bnd) {
+    vector<int32_t> labs;
+    if (t.all) {
+      set<int> present;
+      for (int k = 0; k < Th.neb; ++k) present.insert(Th.bedges[k].lab);
+      labs.assign(present.begin(), present.end());
+    } else {
+      labs.assign(t.labels.begin(), t.labels.end());
+    }
+    if (labs.empty()) continue;
+    vector<double> gq;
+    if (!t.is_const) eval_coef_bqp2(stack, Uh, t, gq);
+    hip_check(ffhip_assemble_p1_boundary(ctx, m, M, labs.data(), (int)labs.size(),
+                                         t.is_const ? FFHIP_COEF_CONST : FFHIP_COEF_QP, t.c0,
+                                         gq.empty() ? nullptr : gq.data(), 0),
+              "ffhip: boundary term");
+  }
+  if (!r.labels.empty()) {
+    vector<int32_t> labs(r.labels.begin(), r.labels.end());
+    hip_check(ffhip_set_bc(ctx, m, M, labs.data(), (int)labs.size(), tgv, nullptr), "ffhip: on()");
+  }
+  if (sym) {
+    ffhip_mat L = nullptr;
+    hip_check(ffhip_mat_lower(M, &L), "ffhip: sym=1 lower triangle");
+    ffhip_mat_free(M);
+    M = L;
+  }
+  int64_t n = 0, nnz = 0;
+  ffhip_mat_info(M, &n, &nnz);
+  vector<int64_t> rp(n + 1);
+  vector<int> ia(n + 1), ja(nnz);
+  vector<double> aa(nnz);
+  hip_check(ffhip_mat_download(M, rp.data(), ja.data(), aa.data()), "ffhip: download");
+  for (int64_t i = 0; i <= n; ++i) ia[i] = (int)rp[i];
+  ffhip_mat_free(M);
+  ffhip_mesh_free(m);
+  const int half = sym ? 1 : 0;
+  MatriceMorse<double> *pH = new MatriceMorse<double>((int)n, (int)n, (int)nnz, half);
+  pH->set((int)n, (int)n, half, (size_t)nnz, ia.data(), ja.data(), aa.data(), 0, 1);
+  A.A.master(pH);
+}
+
+// "=" / "<-" for (matrix, varf(Vh, Vh)) on 2D P1 — preferred over
+// OpMatrixtoBilinearForm<double, Mesh, v_fes, v_fes> (problem.hpp:1611-1660);
+// other forms take its code path unchanged
+class OpHipMatrix2D : public OneOperator {
+ public:
+  int init;
+  class Op : public E_F0mps {
+   public:
+    CFB2 *b;
+    Expression a;
+    int init;
+    Op(Expression aa, Expression bb, int ii)
+        : b(new CFB2(*dynamic_cast<const CFB2 *>(bb))), a(aa), init(ii) {
+      ffassert(b && b->nargs);
+    }
+    operator aType() const { return atype<Matrice_Creuse<double> *>(); }
+    AnyType operator()(Stack stack) const {
+      pfes *pUh = GetAny<pfes *>((*b->euh)(stack));
+      pfes *pVh = GetAny<pfes *>((*b->evh)(stack));
+      const FESpace *PUh = (FESpace *)**pUh;
+      const FESpace *PVh = (FESpace *)**pVh;
+      const bool square = (void *)PUh == (void *)PVh || (PUh && PVh && PUh->NbOfDF == PVh->NbOfDF);
+      const bool VF = isVF(b->largs);
+      Data_Sparse_Solver ds;
+      ds.factorize = 0;
+      ds.initmat = true;
+      const int np = OpCall_FormBilinear_np::n_name_param - NB_NAME_PARM_HMAT;
+      SetEnd_Data_Sparse_Solver<double>(stack, ds, b->nargs, np);
+      WhereStackOfPtr2Free(stack) = new StackOfPtr2Free(stack);
+      Matrice_Creuse<double> &A(*GetAny<Matrice_Creuse<double> *>((*a)(stack)));
+      if (init) A.init();
+      if (!PUh || !PVh) return SetAny<Matrice_Creuse<double> *>(&A);
+      Recognised r;
+      if (ds.tgv > 0 && recognise2(stack, *PUh, *PVh, b->largs, r)) {
+        A.A = 0;
+        A.Uh = *PUh;
+        A.Vh = *PVh;
+        assemble2_on_gpu(stack, *PUh, r, ds.tgv, ds.sym, A);
+      } else {
+        creationBlockOfMatrixToBilinearForm<double, Mesh, FESpace, FESpace>(PUh, PVh, ds.sym, ds.tgv,
+                                                                            b->largs, stack, A);
+      }
+      A.pHM()->half = ds.sym;
+      if (square) SetSolver(stack, VF, *A.A, ds);
+      return SetAny<Matrice_Creuse<double> *>(&A);
+    }
+  };
+  E_F0 *code(const basicAC_F0 &args) const {
+    return new Op(to<Matrice_Creuse<double> *>(args[0]), args[1], init);
+  }
+  OpHipMatrix2D(int ii)
+      : OneOperator(atype<Matrice_Creuse<double> *>(), atype<Matrice_Creuse<double> *>(),
+                    atype<const CFB2 *>()),
+        init(ii) {
+    pref = 10;  // preferred over OpMatrixtoBilinearForm (pref 0)
+  }
+};
+
 // ---------------------------------------------------------------- linear forms
 typedef Call_FormLinear<v_fes3> CFL3;
 
@@ -1053,6 +1326,8 @@ static void Load_Init() {
   Global.Add("ffhipreplacecg", "(", new OneOperator1<long, long>(ffhip_replace_cg));
   TheOperators->Add("=", new OpHipMatrixP1(0));
   TheOperators->Add("<-", new OpHipMatrixP1(1));
+  TheOperators->Add("=", new OpHipMatrix2D(0));
+  TheOperators->Add("<-", new OpHipMatrix2D(1));
   TheOperators->Add("=", new OpHipRhsP1(atype<KN_<double> >(), false, false));
   TheOperators->Add("<-", new OpHipRhsP1(atype<KN<double> *>(), true, true));
   Global.Add("LinearCG", "(", new HipLinearCG());

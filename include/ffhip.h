@@ -67,6 +67,13 @@ int ffhip_mesh3_upload(ffhip_ctx ctx, int64_t nv, int64_t nt, int64_t nbe,
                        const double *xyz, const int32_t *tets,
                        const int32_t *btri, const int32_t *blab,
                        ffhip_mesh *mesh);
+/* Upload a 2D Mesh (src/femlib/fem.hpp, class Mesh): vertices xy[2*nv],
+ * triangles tris[3*nt] (0-based vertex ids, Triangle vertex order), boundary
+ * edges bedges[2*nbe] with labels blab[nbe].  Host pointers. */
+int ffhip_mesh2_upload(ffhip_ctx ctx, int64_t nv, int64_t nt, int64_t nbe,
+                       const double *xy, const int32_t *tris,
+                       const int32_t *bedges, const int32_t *blab,
+                       ffhip_mesh *mesh);
 int ffhip_mesh_info(ffhip_mesh mesh, int64_t *nv, int64_t *nt, int64_t *nbe);
 /* Copy the mesh back to host arrays (any pointer may be NULL). */
 int ffhip_mesh_download(ffhip_mesh mesh, double *xyz, int32_t *elems,

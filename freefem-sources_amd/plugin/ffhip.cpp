@@ -721,6 +721,8 @@ class OpHipMatrixP1 : public OneOperator {
       pfes3 *pVh = GetAny<pfes3 *>((*b->evh)(stack));
       const FESpace3 *PUh = (FESpace3 *)**pUh;
       const FESpace3 *PVh = (FESpace3 *)**pVh;
+      const bool square = (void *)PUh == (void *)PVh || (PUh && PVh && PUh->NbOfDF == PVh->NbOfDF);
+      const bool VF = isVF(b->largs);
       Data_Sparse_Solver ds;
       ds.factorize = 0;
       ds.initmat = true;
@@ -754,7 +756,7 @@ class OpHipMatrixP1 : public OneOperator {
                                                                                b->largs, stack, A);
       }
       A.pHM()->half = ds.sym;
-      SetSolver(stack, false, *A.A, ds);
+      if (square) SetSolver(stack, VF, *A.A, ds);  // as OpMatrixtoBilinearForm (problem.hpp:1654-1656)
       return SetAny<Matrice_Creuse<double> *>(&A);
     }
   };

@@ -276,10 +276,12 @@ __global__ __launch_bounds__(256) void k_pattern_half(
     }
   }
   __syncthreads();
-  // insert the candidates; the lane that claims a free slot appends the column
+  // insert the candidates; the lane that claims a free slot appends the column.
+  // The row's own vertex is in every incident element: only lane 0 inserts it
+  // (a quarter fewer atomics; c[] keeps it for the element-to-row map below)
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    if (c[r] == INT_MAX) continue;
+    if (c[r] == INT_MAX || (c[r] == (int32_t)v && hl != 0)) continue;
     uint32_t h = ((uint32_t)c[r] * 2654435761u) >> (32 - 7);
     for (;;) {  // ends: the set holds at most 97 columns of its 128 slots
       const int32_t old = atomicCAS(&hk[hid][h], INT_MAX, c[r]);

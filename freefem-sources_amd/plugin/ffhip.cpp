@@ -44,7 +44,8 @@
 //  * a higher-preference "=" / "<-" for (real[int], varf(0, Vh)) on 3D P1:
 //        int3d(Th)(f*v) [+ int2d(Th, labels)(g*v)] [+ on(labels, u=g)]
 //    with f, g evaluated by FreeFEM at the quadrature points (the on() value at
-//    the boundary vertices) and the vector built on the GPU; anything else
+//    the boundary vertices) and the vector built on the GPU; on P2 and
+//    [P2,P2,P2] spaces with mesh-independent f and on() values; anything else
 //    goes to FreeFEM's OpArraytoLinearForm code path.
 //  * LinearCG / AffineCG overloads whose operator is a matrix (the affine
 //    ConjuguedGradient2 of LinearCG on the GPU, see HipLinearCG).
@@ -1187,6 +1188,104 @@ void rhs_on_gpu(Stack stack, const FESpace3 &Vh, const RecognisedRhs &r, double 
   for (int i = 0; i < Th.nv; ++i) b[i] = out[i];
 }
 
+// b = va(0, Vh) on Vh(Th,[P2,P2,P2]) or Vh(Th,P2): int3d(Th)(f_c*v_c) terms with
+// mesh-independent coefficients and on(labels, u_c=g_c) with mesh-independent
+// values, the same g for every label (ffhip_rhs_p2v / ffhip_rhs_p2)
+struct RecognisedRhsP2 {
+  int ncomp = 0;
+  double f[3] = {0., 0., 0.}, g[3] = {0., 0., 0.};
+  bool gset = false;
+  set<long> labels;
+};
+
+bool recognise_rhs_p2(Stack stack, const FESpace3 &Vh, const list<C_F0> &largs, RecognisedRhsP2 &r) {
+  if (is_p2v(Vh)) r.ncomp = 3;
+  else if (Vh.N == 1 && Vh.TFE.N() == 1 && Vh.TFE[0] == &DataFE<Mesh3>::P2) r.ncomp = 1;
+  else return false;
+  FreeFempp::TypeVarForm<double> *tvf = FreeFempp::TypeVarForm<double>::Global;
+  for (list<C_F0>::const_iterator ii = largs.begin(); ii != largs.end(); ++ii) {
+    const aType t = ii->left();
+    const Expression e = ii->LeftValue();
+    if (t == tvf->tFB) continue;
+    if (t == tvf->tBC) {
+      const BC_set *bc = dynamic_cast<const BC_set *>(e);
+      if (!bc || bc->complextype || (int)bc->bc.size() != r.ncomp) return false;
+      double g[3] = {0., 0., 0.};
+      for (int k = 0; k < r.ncomp; ++k) {
+        if (bc->bc[k].first != k || !bc->bc[k].second->MeshIndependent()) return false;
+        g[k] = GetAny<double>((*bc->bc[k].second)(stack));
+      }
+      if (r.gset && (g[0] != r.g[0] || g[1] != r.g[1] || g[2] != r.g[2])) return false;
+      for (int k = 0; k < 3; ++k) r.g[k] = g[k];
+      r.gset = true;
+      set<long> labs;
+      Fem2D::Expandsetoflab(stack, *bc, labs);
+      r.labels.insert(labs.begin(), labs.end());
+      continue;
+    }
+    if (t != tvf->tFL) return false;
+    const FormLinear *fl = dynamic_cast<const FormLinear *>(e);
+    if (!fl || fl->di->islevelset() || fl->di->kind != CDomainOfIntegration::int3d ||
+        !fl->di->what.empty())
+      return false;
+    if (GetAny<const Mesh3 *>((*fl->di->Th)(stack)) != &Vh.Th) return false;
+    if (fl->di->FIV(stack).n != QuadratureFormular_Tet_5.n) return false;
+    for (Ftest::const_iterator l = fl->l->v.begin(); l != fl->l->v.end(); ++l) {
+      const MDroit &op = l->first;
+      if (op.first < 0 || op.first >= r.ncomp || op.second != op_id) return false;
+      if (!l->second.EvaluableWithOutStack()) return false;
+      r.f[op.first] += GetAny<double>(l->second.eval(stack));
+    }
+  }
+  return true;
+}
+
+// true when built on the GPU; false (b untouched) when FreeFEM's dof numbering
+// is not the device's first-appearance node numbering
+bool rhs_p2_on_gpu(const FESpace3 &Vh, const RecognisedRhsP2 &r, double tgv, KN_<double> &b) {
+  const Mesh3 &Th = Vh.Th;
+  vector<double> xyz(3 * (size_t)Th.nv);
+  vector<int32_t> tets(4 * (size_t)Th.nt), btri(3 * (size_t)Th.nbe), blab(Th.nbe);
+  for (int i = 0; i < Th.nv; ++i) {
+    xyz[3 * i] = Th(i).x; xyz[3 * i + 1] = Th(i).y; xyz[3 * i + 2] = Th(i).z;
+  }
+  for (int k = 0; k < Th.nt; ++k)
+    for (int j = 0; j < 4; ++j) tets[4 * k + j] = Th(Th[k][j]);
+  for (int k = 0; k < Th.nbe; ++k) {
+    for (int j = 0; j < 3; ++j) btri[3 * k + j] = Th(Th.be(k)[j]);
+    blab[k] = Th.be(k).lab;
+  }
+  ffhip_ctx ctx = hip_ctx();
+  ffhip_mesh m = nullptr;
+  ffhip_p2v sp = nullptr;
+  hip_check(ffhip_mesh3_upload(ctx, Th.nv, Th.nt, Th.nbe, xyz.data(), tets.data(), btri.data(),
+                               blab.data(), &m), "ffhip: mesh upload");
+  hip_check(ffhip_p2v_create(ctx, m, &sp), "ffhip: P2 numbering");
+  vector<int32_t> en(10 * (size_t)Th.nt);
+  hip_check(ffhip_p2v_download(sp, en.data()), "ffhip: P2 numbering");
+  bool same = true;
+  const int nd = 10 * r.ncomp;
+  for (int k = 0; k < Th.nt && same; ++k) {
+    const FElement3 K(Vh[k]);
+    for (int df = 0; df < nd && same; ++df)  // local dofs comp-blocked (c*10 + i)
+      same = K(df) == r.ncomp * en[10 * (size_t)k + df % 10] + df / 10;
+  }
+  if (same) {
+    vector<int32_t> labs(r.labels.begin(), r.labels.end());
+    vector<double> out(b.N());
+    if (r.ncomp == 3)
+      hip_check(ffhip_rhs_p2v(ctx, sp, r.f, labs.data(), (int)labs.size(), tgv, r.g, out.data(), 0),
+                "ffhip: P2 linear form");
+    else
+      hip_check(ffhip_rhs_p2(ctx, sp, r.f[0], labs.data(), (int)labs.size(), tgv, r.g[0], out.data(), 0),
+                "ffhip: P2 linear form");
+    for (long i = 0; i < b.N(); ++i) b[i] = out[i];
+  }
+  ffhip_p2v_free(sp);
+  ffhip_mesh_free(m);
+  return same;
+}
+
 // "=" / "<-" for (real[int], varf(0, Vh)) on 3D P1 — preferred over
 // OpArraytoLinearForm<double, Mesh3, v_fes3> (problem.hpp:853-927, registered
 // at lgfem.cpp:6672 and 6688); other forms take FreeFEM's own path, with the
@@ -1219,11 +1318,17 @@ class OpHipRhsP1 : public OneOperator {
       KN_<double> xx(px ? *(KN_<double> *)px : GetAny<KN_<double> >((*x)(stack)));
       if (NbOfDF) xx = 0.;
       RecognisedRhs r;
+      RecognisedRhsP2 r2;
+      bool done = false;
       if (pVh && tgv > 0 && xx.N() == NbOfDF && recognise_rhs(stack, *pVh, l->largs, r)) {
         rhs_on_gpu(stack, *pVh, r, tgv, xx);
-      } else if (pVh &&
-                 AssembleVarForm<double, MatriceCreuse<double>, Mesh3, FESpace3, FESpace3>(
-                     stack, pVh->Th, *pVh, *pVh, false, 0, &xx, l->largs)) {
+        done = true;
+      } else if (pVh && tgv > 0 && xx.N() == NbOfDF && recognise_rhs_p2(stack, *pVh, l->largs, r2)) {
+        done = rhs_p2_on_gpu(*pVh, r2, tgv, xx);
+      }
+      if (!done && pVh &&
+          AssembleVarForm<double, MatriceCreuse<double>, Mesh3, FESpace3, FESpace3>(
+              stack, pVh->Th, *pVh, *pVh, false, 0, &xx, l->largs)) {
         AssembleBC<double, Mesh3, FESpace3, FESpace3>(stack, pVh->Th, *pVh, *pVh, false, 0, &xx, 0,
                                                       l->largs, tgv);
       }

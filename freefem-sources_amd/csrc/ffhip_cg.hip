@@ -362,14 +362,15 @@ __global__ void k_bsr_fill(int64_t nb, int64_t ents, const int64_t *__restrict__
 }
 
 // flags a row with a column farther than 16 bits from the row index
+// the columns of a row are sorted: its first and last bound the deltas
 __global__ void k_far_columns(int64_t n, const int64_t *__restrict__ rowptr,
                               const int32_t *__restrict__ colidx, int *__restrict__ far) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n) return;
-  for (int64_t k = rowptr[r]; k < rowptr[r + 1]; ++k) {
-    const int64_t d = (int64_t)colidx[k] - r;
-    if (d < -32768 || d > 32767) { atomicOr(far, 1); return; }
-  }
+  const int64_t k0 = rowptr[r], k1 = rowptr[r + 1];
+  if (k0 == k1) return;
+  const int64_t d0 = (int64_t)colidx[k0] - r, d1 = (int64_t)colidx[k1 - 1] - r;
+  if (d0 < -32768 || d1 > 32767) atomicOr(far, 1);
 }
 
 // CSR-layout values: A (optionally transposed through tmap) and/or the FP32

@@ -63,3 +63,31 @@ def test_half_product_brute_force():
     x = rng.standard_normal(n)
     y = ffo.spmv_half(rl, np.array(ci, np.int32), np.array(va), x)
     assert np.abs(y - S @ x).max() <= 1e-13 * np.abs(S @ x).max()
+
+
+def test_transposed_product_equals_product_of_explicit_transpose():
+    # ffo_spmv_t (addMatMul with the transpose flag, HashMatrix.cpp:1094): y_j adds
+    # a_ij x_i over the rows in order, as the transpose's row j does
+    rng = np.random.default_rng(9)
+    n = 40
+    D = rng.standard_normal((n, n)) * (rng.uniform(size=(n, n)) < 0.2)
+    rp = np.zeros(n + 1, np.int64)
+    ci, va = [], []
+    for i in range(n):
+        nz = np.nonzero(D[i])[0]
+        ci.extend(nz)
+        va.extend(D[i, nz])
+        rp[i + 1] = len(ci)
+    ci, va = np.array(ci, np.int32), np.array(va)
+    DT = D.T.copy()
+    rt = np.zeros(n + 1, np.int64)
+    ct, vt = [], []
+    for i in range(n):
+        nz = np.nonzero(DT[i])[0]
+        ct.extend(nz)
+        vt.extend(DT[i, nz])
+        rt[i + 1] = len(ct)
+    x = rng.standard_normal(n)
+    yt = ffo.spmv(rp, ci, va, x, trans=True)
+    assert np.array_equal(yt, ffo.spmv(rt, np.array(ct, np.int32), np.array(vt), x))
+    assert np.abs(yt - D.T @ x).max() <= 1e-13 * max(1.0, np.abs(D.T @ x).max())

@@ -116,8 +116,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs on one GPU only; the multi-GPU sharding of the path is "
-              "described in DESIGN.md and not launched", file=sys.stderr)
+        print("bench.py: runs on one GPU only; the multi-GPU sharding of the path (contiguous row "
+              "slabs, one slab-face exchange per mat-vec) is not launched", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np

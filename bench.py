@@ -226,6 +226,37 @@ def main():
         cgm.free(); cg64.free(); A2.free(); sp.free(); M2.free()
         return out
 
+    # the two smaller configurations of the same path, once, after the timed
+    # steps: 2D P1 square(200) (kappa = 1) and 3D P1 cube(64) (analytic kappa),
+    # assembly (pattern, values, tgv rows, FP64 solver setup) and the first
+    # FP64 solve to 1e-10, beside FreeFEM's own times for the same workloads
+    # on one CPU core (its assembly, and its ConjugueGradient iterations/time)
+    def small_eval():
+        out = {}
+        cases = (("square200_2d", lambda: ffhip.square(200), ffhip.COEF_CONST, (1, 2, 3, 4),
+                  dict(assembly_s=0.153, cg_iters=414, cg_s=0.276)),
+                 ("cube64_3d", lambda: ffhip.cube(64), ffhip.COEF_SIN3, LABELS,
+                  dict(assembly_s=8.78, cg_iters=254, cg_s=2.72)))
+        for name, mk, kind, labels, ff in cases:
+            Ms = mk()
+            Ms.ctx.sync()
+            t0 = time.perf_counter()
+            As = ffhip.Matrix.p1(Ms)
+            As.assemble_p1(kind, 1.0)
+            As.set_bc(labels)
+            cgs = ffhip.CG(As, ffhip.CG_FP64)
+            ta = time.perf_counter() - t0
+            bs = torch.empty(Ms.nv, dtype=torch.float64, device="cuda")
+            ffhip.rhs_p1(Ms, 1.0, labels, out=bs)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            _, ss = cgs.solve(bs, eps=1e-10)
+            tc = time.perf_counter() - t0
+            out[name] = dict(nnz=As.nnz, assembly_s=round(ta, 5), cg_iters=ss["iters"], cg_s=round(tc, 5),
+                             freefem_1core=ff)
+            cgs.free(); As.free(); Ms.free()
+        return out
+
     for _ in range(args.warmup):
         step(False)
     torch.cuda.synchronize()
@@ -335,6 +366,7 @@ def main():
         },
         "cpu_baseline": None,
     }
+    line["other_configs"] = small_eval()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

@@ -113,7 +113,9 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=160)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true",
+                    help="the timed steps only: no CPU baseline and no smaller configurations "
+                         "(the form the profiles are taken with)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: runs on one GPU only; the multi-GPU sharding of the path (contiguous row "
@@ -366,8 +368,8 @@ def main():
         },
         "cpu_baseline": None,
     }
-    line["other_configs"] = small_eval()
     if not args.no_cpu:
+        line["other_configs"] = small_eval()
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)
 

@@ -633,6 +633,15 @@ __device__ inline void block_rows(int64_t n, int64_t &r0, int64_t &r1) {
   r1 = ((lb + 1) * n) / G;
 }
 
+// the 4-aligned middle [a4, b4) of [r0, r1), for 16-byte FP32 accesses
+// (vectors start 256-byte aligned); head [r0, a4) and tail [b4, r1) go scalar
+__device__ inline void block_rows4(int64_t r0, int64_t r1, int64_t &a4, int64_t &b4) {
+  a4 = (r0 + 3) & ~int64_t(3);
+  if (a4 > r1) a4 = r1;
+  b4 = r1 & ~int64_t(3);
+  if (b4 < a4) b4 = a4;
+}
+
 // K1: the direction H = gamma H_prev - Cg, written by an elementwise pass so
 // that the mat-vec gathers one vector (forming H on the fly from two gathered
 // vectors per column made the cube(160) mat-vec 17 % slower and the
@@ -1073,9 +1082,19 @@ __global__ __launch_bounds__(kBlock) void k_mix_dirvec(int c, int64_t n,
   if (st->done || st->need_update) return;
   float cbf;
   if (!mix_begin(c, pin, st, ms, lds, cbf)) return;
-  int64_t r0, r1;
+  int64_t r0, r1, a4, b4;
   block_rows(n, r0, r1);
-  for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) pnew[r] = r32[r] + cbf * pold[r];
+  block_rows4(r0, r1, a4, b4);
+  for (int64_t r = r0 + threadIdx.x; r < a4; r += kBlock) pnew[r] = r32[r] + cbf * pold[r];
+  for (int64_t i = a4 / 4 + threadIdx.x; i < b4 / 4; i += kBlock) {
+    const float4 rv = reinterpret_cast<const float4 *>(r32)[i];
+    const float4 po = reinterpret_cast<const float4 *>(pold)[i];
+    float4 pn;
+    pn.x = rv.x + cbf * po.x; pn.y = rv.y + cbf * po.y;
+    pn.z = rv.z + cbf * po.z; pn.w = rv.w + cbf * po.w;
+    reinterpret_cast<float4 *>(pnew)[i] = pn;
+  }
+  for (int64_t r = b4 + threadIdx.x; r < r1; r += kBlock) pnew[r] = r32[r] + cbf * pold[r];
 }
 
 template <int KIND>
@@ -1108,15 +1127,32 @@ __global__ __launch_bounds__(kBlock) void k_mix_update(
   const double alpha = ms->rho_prev[it & 1] / pq;
   const float af = (float)alpha;
   if (blockIdx.x == 0 && threadIdx.x == 0) st->iters = it;
-  int64_t r0, r1;
+  int64_t r0, r1, a4, b4;
   block_rows(n, r0, r1);
+  block_rows4(r0, r1, a4, b4);
   double part = 0.;
-  for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) {
+  auto one = [&](int64_t r) {
     y32[r] += af * p[r];
     const float rr = r32[r] - af * q[r];
     r32[r] = rr;
     part += (double)rr * (double)rr;
+  };
+  for (int64_t r = r0 + threadIdx.x; r < a4; r += kBlock) one(r);
+  for (int64_t i = a4 / 4 + threadIdx.x; i < b4 / 4; i += kBlock) {
+    const float4 pv = reinterpret_cast<const float4 *>(p)[i];
+    const float4 qv = reinterpret_cast<const float4 *>(q)[i];
+    float4 yv = reinterpret_cast<const float4 *>(y32)[i];
+    float4 rv = reinterpret_cast<const float4 *>(r32)[i];
+    yv.x += af * pv.x; yv.y += af * pv.y; yv.z += af * pv.z; yv.w += af * pv.w;
+    rv.x -= af * qv.x; rv.y -= af * qv.y; rv.z -= af * qv.z; rv.w -= af * qv.w;
+    reinterpret_cast<float4 *>(y32)[i] = yv;
+    reinterpret_cast<float4 *>(r32)[i] = rv;
+    part += (double)rv.x * (double)rv.x;
+    part += (double)rv.y * (double)rv.y;
+    part += (double)rv.z * (double)rv.z;
+    part += (double)rv.w * (double)rv.w;
   }
+  for (int64_t r = b4 + threadIdx.x; r < r1; r += kBlock) one(r);
   const double tot = block_sum<kBlock>(part, lds);
   if (threadIdx.x == 0) pout[xcd_block_id()] = tot;
 }

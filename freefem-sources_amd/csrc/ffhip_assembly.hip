@@ -900,9 +900,15 @@ static unsigned nblocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); 
 int read_flag(ffhip_ctx ctx, int *d);
 void half_view(ffhip_mat L);
 
-static void upload_t5_once() {
-  static bool done = false;
-  if (done) return;
+// once per device (__constant__ memory is per device): the context's device,
+// the caller's current device restored afterwards
+static void upload_t5_once(int dev) {
+  static bool done[64] = {};
+  FFHIP_REQUIRE(dev >= 0 && dev < 64, FFHIP_ERR_UNSUPPORTED, "device id %d", dev);
+  if (done[dev]) return;
+  int cur = 0;
+  FFHIP_CHECK(hipGetDevice(&cur));
+  if (cur != dev) FFHIP_CHECK(hipSetDevice(dev));
   const double sqrt15 = 3.87298334620741688517926539978;
   const double t = 1.E0 / 3.E0, A = 0.225E0;
   const double r = (6 - sqrt15) / 21, s = (9 + 2 * sqrt15) / 21, B = (155 - sqrt15) / 1200;
@@ -911,7 +917,8 @@ static void upload_t5_once() {
   const double W[7] = {A, B, B, B, C, C, C};
   FFHIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_qt5_pts), P, sizeof(P)));
   FFHIP_CHECK(hipMemcpyToSymbol(HIP_SYMBOL(c_qt5_w), W, sizeof(W)));
-  done = true;
+  if (cur != dev) FFHIP_CHECK(hipSetDevice(cur));
+  done[dev] = true;
 }
 
 // dof -> (element * nloc + local) incidence lists in element order, for any
@@ -1111,7 +1118,7 @@ int ffhip_assemble_p1_form(ffhip_ctx ctx, ffhip_mesh m, ffhip_mat A, int kind, d
       FFHIP_CHECK(hipMemcpyAsync(mbuf, mass_e, 8 * cnt, hipMemcpyHostToDevice, s));
       md = mbuf;
     }
-    if (m->dim == 2) upload_t5_once();
+    if (m->dim == 2) upload_t5_once(ctx->device);
     ctx->tic();
     double *Kinc = nullptr;
     if (m->dim == 3) {
@@ -1209,7 +1216,7 @@ int ffhip_rhs_p1_form(ffhip_ctx ctx, ffhip_mesh m, int kind, double f0, const do
       FFHIP_CHECK(hipMemcpyAsync(gown, gvals, 8 * n, hipMemcpyHostToDevice, s));
       gd = gown;
     }
-    if (m->dim == 2) upload_t5_once();
+    if (m->dim == 2) upload_t5_once(ctx->device);
     double *tmes = m->dim == 3 ? dalloc<double>(m->nt) : nullptr;
     ctx->tic();
     if (m->dim == 3) {

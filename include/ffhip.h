@@ -45,7 +45,10 @@ const char *ffhip_version(void);
  * All work of a context is ordered on its one stream.  Device blocks that
  * handles release are kept per device and reused by later allocations of
  * about the same size; they go back to the device when the last context of
- * the device is freed.  Free every handle of a context before the context. */
+ * the device is freed.  Free every handle of a context before the context.
+ * ffhip_init makes `device` the calling thread's current device; the calls on
+ * a context allocate on the current device, so a program that drives several
+ * devices selects the context's device (hipSetDevice) before calling on it. */
 int ffhip_init(int device, ffhip_ctx *ctx);
 int ffhip_ctx_free(ffhip_ctx ctx);
 int ffhip_sync(ffhip_ctx ctx);

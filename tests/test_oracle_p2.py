@@ -88,6 +88,37 @@ def test_p2_elast_element_kernel_and_energy():
             assert abs(u @ K @ u - exact) <= 1e-11 * abs(exact)
 
 
+def test_p2_elast_element_exact_energy_of_quadratic_fields():
+    """u_c(x) = x^T Q_c x + p_c . x + r_c (each component quadratic): P2
+    interpolates u exactly, and the energy density lambda (div u)^2 +
+    2 mu eps(u):eps(u) is a quadratic, integrated exactly by the P2 nodal rule
+    int_K f = |K| (-1/20 sum_vertices f + 1/5 sum_edge-midpoints f).  The
+    quadratic modes exercise the edge basis functions' gradients
+    4 (l_a grad l_b + l_b grad l_a), which affine fields leave at their
+    averages; a wrong edge gradient, Lame term or coupling index fails here."""
+    lam, mu = 1.0, 0.5
+    for seed in range(5):
+        K, nodes, mes = _element(seed, lam, mu)
+        rng = np.random.default_rng(300 + seed)
+        for _ in range(3):
+            Q = rng.standard_normal((3, 3, 3))
+            Q = Q + Q.transpose(0, 2, 1)  # Q[c] symmetric
+            P = rng.standard_normal((3, 3))
+            r0 = rng.standard_normal(3)
+            field = lambda x: np.array([x @ Q[c] @ x for c in range(3)]) + P @ x + r0
+            grad = lambda x: 2 * Q @ x + P  # grad[c, a] = d u_c / d x_a
+
+            def density(x):
+                G = grad(x)
+                E = 0.5 * (G + G.T)
+                return lam * np.trace(G) ** 2 + 2 * mu * np.sum(E * E)
+
+            exact = mes * (-sum(density(x) for x in nodes[:4]) / 20 +
+                           sum(density(x) for x in nodes[4:]) / 5)
+            u = _nodal(field, nodes)
+            assert abs(u @ K @ u - exact) <= 1e-11 * abs(exact)
+
+
 def test_p2_elast_system_small():
     r = ffo.elasticity_p2(3)
     n = 3

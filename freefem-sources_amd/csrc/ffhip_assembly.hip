@@ -381,10 +381,16 @@ __global__ __launch_bounds__(256) void k_pattern_compact(
 // ---------------------------------------------------------------- element matrices
 // 3D P1: Ke_ij = S grad l_i . grad l_j with S = sum_q |K| w_q kappa(x_q)
 // (problem.cpp:6387-6392); the 10 unique entries per element.
-__device__ inline double p1_kappa_integral(int64_t e, const double v[4][3], double mes, int kind,
-                                           double c0, const double *__restrict__ kappa_e) {
+__device__ inline double p1_kappa_integral(int64_t e, const int32_t id[4], const double v[4][3],
+                                           double mes, int kind, double c0,
+                                           const double *__restrict__ kappa_e) {
   double S = 0.;
-  if (kind == FFHIP_COEF_SIN3) {
+  if (kind == FFHIP_COEF_P1) {
+    const double u[4] = {kappa_e[id[0]], kappa_e[id[1]], kappa_e[id[2]], kappa_e[id[3]]};
+#pragma unroll
+    for (int q = 0; q < 14; ++q)
+      S += (mes * qf5_weight(q)) * p1_value3(u, c_qf5_pts[q][0], c_qf5_pts[q][1], c_qf5_pts[q][2]);
+  } else if (kind == FFHIP_COEF_SIN3) {
 #pragma unroll 2
     for (int q = 0; q < 14; ++q) {
       const double px = c_qf5_pts[q][0], py = c_qf5_pts[q][1], pz = c_qf5_pts[q][2];
@@ -406,9 +412,9 @@ __device__ inline double p1_kappa_integral(int64_t e, const double v[4][3], doub
 }
 
 __device__ inline void load_tet(const double *__restrict__ xyz, const int32_t *__restrict__ tets,
-                                int64_t e, double v[4][3]) {
+                                int64_t e, int32_t id[4], double v[4][3]) {
   const int4 tv = reinterpret_cast<const int4 *>(tets)[e];
-  const int32_t id[4] = {tv.x, tv.y, tv.z, tv.w};
+  id[0] = tv.x; id[1] = tv.y; id[2] = tv.z; id[3] = tv.w;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -417,9 +423,14 @@ __device__ inline void load_tet(const double *__restrict__ xyz, const int32_t *_
 
 // the reaction term of int3d(m*u*v): K(i,j) += sum_q |K| w_q m(x_q) l_i(q) l_j(q)
 // for the 10 entries i <= j (P1 values l = barycentrics of the Tet_5 point)
-__device__ inline void p1_mass_3d(int64_t e, const double v[4][3], double mes, int mkind, double m0,
-                                  const double *__restrict__ mass_e, double K[10]) {
+__device__ inline void p1_mass_3d(int64_t e, const int32_t id[4], const double v[4][3], double mes,
+                                  int mkind, double m0, const double *__restrict__ mass_e,
+                                  double K[10]) {
   const double mp0 = (mkind == FFHIP_COEF_P0) ? mass_e[e] : m0;
+  double mu[4] = {0., 0., 0., 0.};
+  if (mkind == FFHIP_COEF_P1)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) mu[i] = mass_e[id[i]];
 #pragma unroll 2
   for (int q = 0; q < 14; ++q) {
     const double px = c_qf5_pts[q][0], py = c_qf5_pts[q][1], pz = c_qf5_pts[q][2];
@@ -432,6 +443,8 @@ __device__ inline void p1_mass_3d(int64_t e, const double v[4][3], double mes, i
       mq = kappa_sin3(P[0], P[1], P[2]);
     } else if (mkind == FFHIP_COEF_QP) {
       mq = mass_e[14 * e + q];
+    } else if (mkind == FFHIP_COEF_P1) {
+      mq = p1_value3(mu, px, py, pz);
     }
     const double W = (mes * qf5_weight(q)) * mq;
     int o = 0;
@@ -465,16 +478,17 @@ __global__ __launch_bounds__(256) void k_p1_element_3d(
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nt) return;
   double v[4][3], G[4][3];
-  load_tet(xyz, tets, e, v);
+  int32_t id[4];
+  load_tet(xyz, tets, e, id, v);
   const double mes = tet_geom(v, G);
-  const double S = p1_kappa_integral(e, v, mes, kind, c0, kappa_e);
+  const double S = p1_kappa_integral(e, id, v, mes, kind, c0, kappa_e);
   double K[10];
   int o = 0;
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = i; j < 4; ++j) K[o++] = p1_entry(S, G, i, j);
-  if (MASS) p1_mass_3d(e, v, mes, mkind, m0, mass_e, K);
+  if (MASS) p1_mass_3d(e, id, v, mes, mkind, m0, mass_e, K);
   const int4 pos = reinterpret_cast<const int4 *>(inv)[e];
   const int32_t kk[4] = {pos.x, pos.y, pos.z, pos.w};
 #pragma unroll
@@ -499,8 +513,11 @@ __global__ __launch_bounds__(256) void k_p1_element_2d(
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nt) return;
   double v[3][2];
-  for (int i = 0; i < 3; ++i)
-    for (int k = 0; k < 2; ++k) v[i][k] = xy[2 * (int64_t)tris[3 * e + i] + k];
+  int32_t id[3];
+  for (int i = 0; i < 3; ++i) {
+    id[i] = tris[3 * e + i];
+    for (int k = 0; k < 2; ++k) v[i][k] = xy[2 * (int64_t)id[i] + k];
+  }
   const double area = ((v[1][0] - v[0][0]) * (v[2][1] - v[0][1]) -
                        (v[1][1] - v[0][1]) * (v[2][0] - v[0][0])) * 0.5;
   double G[3][2];
@@ -522,6 +539,9 @@ __global__ __launch_bounds__(256) void k_p1_element_2d(
       kc = 1 + (0.5 * sin(tp * X)) * sin(tp * Y);
     } else if (kind == FFHIP_COEF_QP) {
       kc = kappa_e[7 * e + q];
+    } else if (kind == FFHIP_COEF_P1) {
+      const double u[3] = {kappa_e[id[0]], kappa_e[id[1]], kappa_e[id[2]]};
+      kc = p1_value2(u, c_qt5_pts[q][0], c_qt5_pts[q][1]);
     } else {
       kc = (kind == FFHIP_COEF_P0) ? kappa_e[e] : c0;
     }
@@ -547,6 +567,9 @@ __global__ __launch_bounds__(256) void k_p1_element_2d(
         mq = 1 + (0.5 * sin(tp * X)) * sin(tp * Y);
       } else if (mkind == FFHIP_COEF_QP) {
         mq = mass_e[7 * e + q];
+      } else if (mkind == FFHIP_COEF_P1) {
+        const double u[3] = {mass_e[id[0]], mass_e[id[1]], mass_e[id[2]]};
+        mq = p1_value2(u, px, py);
       }
       const double W = (area * c_qt5_w[q]) * mq;
       o = 0;
@@ -690,7 +713,8 @@ __global__ void k_set_diag(int64_t n, const uint8_t *__restrict__ onbc,
 // ---------------------------------------------------------------- right-hand side
 // b[v] = sum over the incident elements (element order) of int_K f l_v with
 // the element's rule (Element_rhs, problem.cpp:7940-7978); f: CONST f0, SIN3,
-// P0 fcoef[e], QP fcoef[e * nq + q].  Dirichlet rows: tgv * (gvals[v] or g0).
+// P0 fcoef[e], QP fcoef[e * nq + q], P1 fcoef[vertex] interpolated at the
+// point.  Dirichlet rows: tgv * (gvals[v] or g0).
 __device__ inline double rhs_coef(int kind, double f0, const double *__restrict__ fcoef, int64_t e,
                                   int q, int nq, double x, double y, double z, int dim) {
   if (kind == FFHIP_COEF_SIN3) {
@@ -731,12 +755,16 @@ __global__ void k_rhs_p1_3d(int64_t n, const double *__restrict__ xyz,
     const int32_t idx = v2e[k];
     const int64_t e = idx >> 2;
     const int l = idx & 3;
-    double vv[4][3];
-    if (kind != FFHIP_COEF_CONST) {  // the points are needed for f
+    double vv[4][3], fu[4] = {0., 0., 0., 0.};
+    if (kind != FFHIP_COEF_CONST) {  // the points (or the vertex values) are needed for f
       const int4 tv = reinterpret_cast<const int4 *>(tets)[e];
       const int32_t id[4] = {tv.x, tv.y, tv.z, tv.w};
-      for (int i = 0; i < 4; ++i)
-        for (int kk = 0; kk < 3; ++kk) vv[i][kk] = xyz[3 * (int64_t)id[i] + kk];
+      if (kind == FFHIP_COEF_P1) {
+        for (int i = 0; i < 4; ++i) fu[i] = fcoef[id[i]];
+      } else {
+        for (int i = 0; i < 4; ++i)
+          for (int kk = 0; kk < 3; ++kk) vv[i][kk] = xyz[3 * (int64_t)id[i] + kk];
+      }
     }
     const double mes = tmes[e];
     double r = 0.;
@@ -745,7 +773,9 @@ __global__ void k_rhs_p1_3d(int64_t n, const double *__restrict__ xyz,
       const double l0 = 1. - (px + py + pz);
       const double lam = l == 0 ? l0 : (l == 1 ? px : (l == 2 ? py : pz));
       double f = f0;
-      if (kind != FFHIP_COEF_CONST) {
+      if (kind == FFHIP_COEF_P1) {
+        f = p1_value3(fu, px, py, pz);
+      } else if (kind != FFHIP_COEF_CONST) {
         double P[3];
         for (int kk = 0; kk < 3; ++kk)
           P[kk] = l0 * vv[0][kk] + px * vv[1][kk] + py * vv[2][kk] + pz * vv[3][kk];
@@ -772,9 +802,11 @@ __global__ void k_rhs_p1_2d(int64_t n, const double *__restrict__ xy,
     const int32_t idx = v2e[k];
     const int64_t e = idx / 3;
     const int l = idx - 3 * (int)e;
-    double p[3][2];
+    double p[3][2], fu[3] = {0., 0., 0.};
     for (int i = 0; i < 3; ++i)
       for (int kk = 0; kk < 2; ++kk) p[i][kk] = xy[2 * (int64_t)tris[3 * e + i] + kk];
+    if (kind == FFHIP_COEF_P1)
+      for (int i = 0; i < 3; ++i) fu[i] = fcoef[tris[3 * e + i]];
     const double area = ((p[1][0] - p[0][0]) * (p[2][1] - p[0][1]) -
                          (p[1][1] - p[0][1]) * (p[2][0] - p[0][0])) * 0.5;
     double r = 0.;
@@ -783,7 +815,9 @@ __global__ void k_rhs_p1_2d(int64_t n, const double *__restrict__ xy,
       const double l0 = 1 - px - py;
       const double lam = l == 0 ? l0 : (l == 1 ? px : py);
       double f = f0;
-      if (kind != FFHIP_COEF_CONST) {
+      if (kind == FFHIP_COEF_P1) {
+        f = p1_value2(fu, px, py);
+      } else if (kind != FFHIP_COEF_CONST) {
         const double X = l0 * p[0][0] + px * p[1][0] + py * p[2][0];
         const double Y = l0 * p[0][1] + px * p[1][1] + py * p[2][1];
         f = rhs_coef(kind, f0, fcoef, e, q, 7, X, Y, 0., 2);
@@ -1014,52 +1048,76 @@ int ffhip_pattern_p1(ffhip_ctx ctx, ffhip_mesh m, ffhip_mat *out) {
     const int64_t n = m->nv;
     ffhip_mat A = new ffhip_mat_s();
     A->ctx = ctx; A->n = n; A->mesh = m;
-    Arena ar(ctx, Arena::bytes(n + 1, 8) + Arena::bytes(1, 4) + Arena::bytes(kRowBuf * n, 4) +
-                      Arena::bytes(n, 1) + Arena::bytes(1, 4));
-    int64_t *len = ar.take<int64_t>(n + 1);
-    int *ovf = ar.take<int>(1);
-    A->rowptr = dalloc<int64_t>(n + 1);
-    A->diag = dalloc<int64_t>(n);
-    A->slotmap = dalloc<uint32_t>(m->nt * m->nloc);
-    FFHIP_CHECK(hipMemsetAsync(len, 0, 8 * (n + 1), s));
-    FFHIP_CHECK(hipMemsetAsync(ovf, 0, sizeof(int), s));
-    constexpr int B = 128, CAP = 64;
-    // rows with up to 32 incident elements and 64 columns: one half-wave
-    // pass into row buffers; the rest: the lane-per-row merge
-    int32_t *rowbuf = ar.take<int32_t>(kRowBuf * n);
-    int8_t *dposr = ar.take<int8_t>(n);
-    int *anyb = ar.take<int>(1);
-    A->big = dalloc<uint8_t>(n);
-    FFHIP_CHECK(hipMemsetAsync(anyb, 0, sizeof(int), s));
-    if (m->nloc == 4)
-      k_pattern_half<4><<<nblocks(n * kHalf, 256), 256, 0, s>>>(
-          n, m->elems, m->v2e_ptr, m->v2e, len, rowbuf, dposr, A->slotmap, A->big, anyb);
-    else
-      k_pattern_half<3><<<nblocks(n * kHalf, 256), 256, 0, s>>>(
-          n, m->elems, m->v2e_ptr, m->v2e, len, rowbuf, dposr, A->slotmap, A->big, anyb);
-    FFHIP_CHECK(hipGetLastError());
-    A->any_big = read_flag(ctx, anyb);
-    if (A->any_big)
-      k_pattern_rows<B, CAP, false><<<nblocks(n, B), B, 0, s>>>(
-          n, m->nloc, m->elems, m->v2e_ptr, m->v2e, len, nullptr, nullptr, nullptr, nullptr, nullptr,
-          ovf, A->big);
-    FFHIP_CHECK(hipGetLastError());
-    FFHIP_REQUIRE(read_flag(ctx, ovf) == 0, FFHIP_ERR_UNSUPPORTED,
-                  "ffhip_pattern_p1: a row has more than %d columns", CAP);
-    exclusive_scan_i64(ctx, len, A->rowptr, n + 1);
-    A->nnz = read_i64(ctx, A->rowptr + n);
-    A->colidx = dalloc<int32_t>(A->nnz);
-    A->vals = dalloc<double>(A->nnz);
-    k_pattern_compact<<<nblocks(n, 256), 256, 0, s>>>(n, A->rowptr, rowbuf, dposr, A->big,
-                                                              A->colidx, A->diag);
-    if (A->any_big)
-      k_pattern_rows<B, CAP, true><<<nblocks(n, B), B, 0, s>>>(
-          n, m->nloc, m->elems, m->v2e_ptr, m->v2e, nullptr, A->rowptr, A->colidx, A->diag,
-          A->slotmap, nullptr, ovf, A->big);
-    FFHIP_CHECK(hipGetLastError());
-    FFHIP_CHECK(hipMemsetAsync(A->vals, 0, 8 * A->nnz, s));
-    ctx->toc();
-    A->max_row = CAP;
+    try {
+      Arena ar(ctx, Arena::bytes(n + 1, 8) + Arena::bytes(1, 4) + Arena::bytes(kRowBuf * n, 4) +
+                        Arena::bytes(n, 1) + Arena::bytes(1, 4));
+      int64_t *len = ar.take<int64_t>(n + 1);
+      int *ovf = ar.take<int>(1);
+      A->rowptr = dalloc<int64_t>(n + 1);
+      A->diag = dalloc<int64_t>(n);
+      A->slotmap = dalloc<uint32_t>(m->nt * m->nloc);
+      FFHIP_CHECK(hipMemsetAsync(len, 0, 8 * (n + 1), s));
+      FFHIP_CHECK(hipMemsetAsync(ovf, 0, sizeof(int), s));
+      // rows with up to 32 incident elements and 64 columns: one half-wave
+      // pass into row buffers (big = 0); the rest: the lane-per-row merge with
+      // 64 LDS columns per lane (big = 1), then with 256 (big = 2: the slot
+      // offsets of the element-to-row map are bytes)
+      constexpr int B = 128, CAP = 64, BW = 32, CAPW = 256;
+      int32_t *rowbuf = ar.take<int32_t>(kRowBuf * n);
+      int8_t *dposr = ar.take<int8_t>(n);
+      int *anyb = ar.take<int>(1);
+      A->big = dalloc<uint8_t>(n);
+      FFHIP_CHECK(hipMemsetAsync(anyb, 0, sizeof(int), s));
+      if (m->nloc == 4)
+        k_pattern_half<4><<<nblocks(n * kHalf, 256), 256, 0, s>>>(
+            n, m->elems, m->v2e_ptr, m->v2e, len, rowbuf, dposr, A->slotmap, A->big, anyb);
+      else
+        k_pattern_half<3><<<nblocks(n * kHalf, 256), 256, 0, s>>>(
+            n, m->elems, m->v2e_ptr, m->v2e, len, rowbuf, dposr, A->slotmap, A->big, anyb);
+      FFHIP_CHECK(hipGetLastError());
+      A->any_big = read_flag(ctx, anyb);
+      bool wide = false;
+      if (A->any_big) {
+        k_pattern_rows<B, CAP, false><<<nblocks(n, B), B, 0, s>>>(
+            n, m->nloc, m->elems, m->v2e_ptr, m->v2e, len, nullptr, nullptr, nullptr, nullptr,
+            nullptr, ovf, A->big, 1);
+        FFHIP_CHECK(hipGetLastError());
+        wide = read_flag(ctx, ovf) != 0;
+      }
+      if (wide) {
+        FFHIP_CHECK(hipMemsetAsync(ovf, 0, sizeof(int), s));
+        k_pattern_rows<BW, CAPW, false><<<nblocks(n, BW), BW, 0, s>>>(
+            n, m->nloc, m->elems, m->v2e_ptr, m->v2e, len, nullptr, nullptr, nullptr, nullptr,
+            nullptr, ovf, A->big, 2);
+        FFHIP_CHECK(hipGetLastError());
+        FFHIP_REQUIRE(read_flag(ctx, ovf) == 0, FFHIP_ERR_UNSUPPORTED,
+                      "ffhip_pattern_p1: a row has more than %d columns", CAPW);
+      }
+      exclusive_scan_i64(ctx, len, A->rowptr, n + 1);
+      A->nnz = read_i64(ctx, A->rowptr + n);
+      A->colidx = dalloc<int32_t>(A->nnz);
+      A->vals = dalloc<double>(A->nnz);
+      k_pattern_compact<<<nblocks(n, 256), 256, 0, s>>>(n, A->rowptr, rowbuf, dposr, A->big,
+                                                        A->colidx, A->diag);
+      if (A->any_big)
+        k_pattern_rows<B, CAP, true><<<nblocks(n, B), B, 0, s>>>(
+            n, m->nloc, m->elems, m->v2e_ptr, m->v2e, nullptr, A->rowptr, A->colidx, A->diag,
+            A->slotmap, nullptr, ovf, A->big, 1);
+      if (wide)
+        k_pattern_rows<BW, CAPW, true><<<nblocks(n, BW), BW, 0, s>>>(
+            n, m->nloc, m->elems, m->v2e_ptr, m->v2e, nullptr, A->rowptr, A->colidx, A->diag,
+            A->slotmap, nullptr, ovf, A->big, 2);
+      FFHIP_CHECK(hipGetLastError());
+      FFHIP_CHECK(hipMemsetAsync(A->vals, 0, 8 * A->nnz, s));
+      ctx->toc();
+      A->max_row = wide ? CAPW : CAP;
+    } catch (...) {
+      (void)hipStreamSynchronize(s);
+      dfree(A->rowptr); dfree(A->colidx); dfree(A->vals); dfree(A->diag);
+      dfree(A->slotmap); dfree(A->big);
+      delete A;
+      throw;
+    }
     *out = A;
   });
 }
@@ -1085,13 +1143,16 @@ int ffhip_assemble_p1_form(ffhip_ctx ctx, ffhip_mesh m, ffhip_mat A, int kind, d
     FFHIP_REQUIRE(ctx && m && A && A->mesh == m && A->slotmap, FFHIP_ERR_ARG,
                   "ffhip_assemble_p1: matrix pattern was not built from this mesh");
     auto valid = [](int k) {
-      return k == FFHIP_COEF_CONST || k == FFHIP_COEF_SIN3 || k == FFHIP_COEF_P0 || k == FFHIP_COEF_QP;
+      return k == FFHIP_COEF_CONST || k == FFHIP_COEF_SIN3 || k == FFHIP_COEF_P0 ||
+             k == FFHIP_COEF_QP || k == FFHIP_COEF_P1;
     };
     FFHIP_REQUIRE(valid(kind), FFHIP_ERR_ARG, "ffhip_assemble_p1: coefficient kind %d", kind);
     FFHIP_REQUIRE(mkind == FFHIP_COEF_NONE || valid(mkind), FFHIP_ERR_ARG,
                   "ffhip_assemble_p1_form: mass coefficient kind %d", mkind);
-    const bool arr = kind == FFHIP_COEF_P0 || kind == FFHIP_COEF_QP;
-    const bool marr = mkind == FFHIP_COEF_P0 || mkind == FFHIP_COEF_QP;
+    auto is_array = [](int k) {
+      return k == FFHIP_COEF_P0 || k == FFHIP_COEF_QP || k == FFHIP_COEF_P1;
+    };
+    const bool arr = is_array(kind), marr = is_array(mkind);
     FFHIP_REQUIRE(!arr || kappa_e, FFHIP_ERR_ARG, "ffhip_assemble_p1: kappa_e");
     FFHIP_REQUIRE(!marr || mass_e, FFHIP_ERR_ARG, "ffhip_assemble_p1_form: mass_e");
     const int kappa_on_device = on_device;
@@ -1105,7 +1166,9 @@ int ffhip_assemble_p1_form(ffhip_ctx ctx, ffhip_mesh m, ffhip_mat A, int kind, d
     double *Ke = needKe ? ar.take<double>(nu * m->nt) : nullptr;
     const double *kd = kappa_e, *md = mass_e;
     double *kbuf = nullptr, *mbuf = nullptr;
-    auto count = [&](int k) { return k == FFHIP_COEF_P0 ? m->nt : m->nt * (m->dim == 3 ? 14 : 7); };
+    auto count = [&](int k) {
+      return k == FFHIP_COEF_P1 ? m->nv : k == FFHIP_COEF_P0 ? m->nt : m->nt * (m->dim == 3 ? 14 : 7);
+    };
     if (arr && !kappa_on_device) {
       const int64_t cnt = count(kind);
       kbuf = dalloc<double>(cnt);
@@ -1185,8 +1248,9 @@ int ffhip_rhs_p1_form(ffhip_ctx ctx, ffhip_mesh m, int kind, double f0, const do
   return guard([&] {
     FFHIP_REQUIRE(ctx && m && b, FFHIP_ERR_ARG, "ffhip_rhs_p1: bad arguments");
     FFHIP_REQUIRE(kind == FFHIP_COEF_CONST || kind == FFHIP_COEF_SIN3 || kind == FFHIP_COEF_P0 ||
-                      kind == FFHIP_COEF_QP, FFHIP_ERR_ARG, "ffhip_rhs_p1_form: coefficient kind");
-    const bool arr = kind == FFHIP_COEF_P0 || kind == FFHIP_COEF_QP;
+                      kind == FFHIP_COEF_QP || kind == FFHIP_COEF_P1,
+                  FFHIP_ERR_ARG, "ffhip_rhs_p1_form: coefficient kind");
+    const bool arr = kind == FFHIP_COEF_P0 || kind == FFHIP_COEF_QP || kind == FFHIP_COEF_P1;
     FFHIP_REQUIRE(!arr || fcoef, FFHIP_ERR_ARG, "ffhip_rhs_p1_form: coefficient array missing");
     hipStream_t s = ctx->stream;
     build_v2e(m);
@@ -1206,7 +1270,9 @@ int ffhip_rhs_p1_form(ffhip_ctx ctx, ffhip_mesh m, int kind, double f0, const do
     const double *fd = fcoef, *gd = gvals;
     double *fown = nullptr, *gown = nullptr;
     if (arr && !on_device) {
-      const int64_t cnt = kind == FFHIP_COEF_P0 ? m->nt : m->nt * (m->dim == 3 ? 14 : 7);
+      const int64_t cnt = kind == FFHIP_COEF_P1   ? m->nv
+                          : kind == FFHIP_COEF_P0 ? m->nt
+                                                  : m->nt * (m->dim == 3 ? 14 : 7);
       fown = dalloc<double>(cnt);
       FFHIP_CHECK(hipMemcpyAsync(fown, fcoef, 8 * cnt, hipMemcpyHostToDevice, s));
       fd = fown;

@@ -126,9 +126,25 @@ __device__ inline int sym3(int l, int m) {
   return l * 3 - (l * (l - 1)) / 2 + (m - l);
 }
 
+// A P1 finite element function at a reference point of an element, from its
+// vertex values, in FreeFEM's order of operations: 3D
+// TypeOfFE_P1Lagrange3d::operator() (P012_3d.cpp:98-106), 2D
+// TypeOfFE_P1Lagrange::operator() (FESpace.cpp:1078-1085).
+__device__ inline double p1_value3(const double u[4], double x, double y, double z) {
+  const double l0 = 1 - x - y - z;
+  return u[0] * l0 + u[1] * x + y * u[2] + z * u[3];
+}
+__device__ inline double p1_value2(const double u[3], double x, double y) {
+  const double l0 = 1 - x - y;
+  return u[0] * l0 + u[1] * x + y * u[2];
+}
+
 // ---------------------------------------------------------------- pattern
 // One lane per row; the row's sorted unique column list lives in LDS, column
 // c of lane t at lds[c * BLOCK + t] (consecutive lanes in consecutive banks).
+// With only != null the kernel takes the rows whose only[] byte equals want;
+// a row of more than CAP columns sets *overflow and, with only != null, its
+// byte to want + 1 (the next, wider pass takes it).
 template <int BLOCK, int CAP, bool FILL>
 __global__ __launch_bounds__(BLOCK) void k_pattern_rows(
     int64_t n, int nloc, const int32_t *__restrict__ elems,
@@ -136,7 +152,7 @@ __global__ __launch_bounds__(BLOCK) void k_pattern_rows(
     int64_t *__restrict__ rowlen, const int64_t *__restrict__ rowptr,
     int32_t *__restrict__ colidx, int64_t *__restrict__ diag,
     uint32_t *__restrict__ slotmap, uint8_t *__restrict__ slot8, int *__restrict__ overflow,
-    const uint8_t *__restrict__ only = nullptr) {
+    uint8_t *__restrict__ only = nullptr, uint8_t want = 1) {
   __shared__ int32_t L[CAP * BLOCK];
   const int t = threadIdx.x;
   const int64_t v = (int64_t)blockIdx.x * BLOCK + t;
@@ -144,7 +160,7 @@ __global__ __launch_bounds__(BLOCK) void k_pattern_rows(
   int u = 0;
   bool ovf = false;
   const int64_t k0 = v2e_ptr[v], k1 = v2e_ptr[v + 1];
-  if (only && !only[v]) return;  // rows the half-wave kernel built
+  if (only && only[v] != want) return;  // rows of another pass
   for (int64_t k = k0; k < k1 && !ovf; ++k) {
     const int64_t e = v2e[k] / nloc;
     for (int mloc = 0; mloc < nloc; ++mloc) {
@@ -158,7 +174,11 @@ __global__ __launch_bounds__(BLOCK) void k_pattern_rows(
       ++u;
     }
   }
-  if (ovf) { atomicMax(overflow, 1); return; }
+  if (ovf) {
+    if (only) only[v] = (uint8_t)(want + 1);
+    atomicMax(overflow, 1);
+    return;
+  }
   if (!FILL) { rowlen[v] = u; return; }
   const int64_t r0 = rowptr[v];
   int dpos = -1;

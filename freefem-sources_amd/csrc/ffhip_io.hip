@@ -219,6 +219,7 @@ int ffhip_mat_coo(ffhip_mat A, int64_t *count, int64_t *I, int64_t *J, double *C
     ffhip_ctx ctx = A->ctx;
     hipStream_t s = ctx->stream;
     const int64_t n = A->n, nnz = A->nnz;
+    FFHIP_REQUIRE(n > 0, FFHIP_ERR_ARG, "ffhip_mat_coo: matrix without rows");
     // HashMatrix has no (n-1, m-1) entry: [I,J,C] = A appends (n, m, 0)
     int64_t last[2] = {0, 0};
     FFHIP_CHECK(hipMemcpyAsync(last, A->rowptr + n - 1, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, s));

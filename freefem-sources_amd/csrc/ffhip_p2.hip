@@ -380,6 +380,10 @@ __global__ __launch_bounds__(kElemBlock) void k_p2_element(
             double kq = c0;
             if (kind == FFHIP_COEF_P0) kq = kappa_e[e];
             else if (kind == FFHIP_COEF_QP) kq = kappa_e[14 * e + q];
+            else if (kind == FFHIP_COEF_P1) {
+              const double u[4] = {kappa_e[id[0]], kappa_e[id[1]], kappa_e[id[2]], kappa_e[id[3]]};
+              kq = p1_value3(u, px, py, pz);
+            }
             else if (kind == FFHIP_COEF_SIN3) {
               double P[3];
 #pragma unroll
@@ -917,13 +921,13 @@ int ffhip_assemble_p2(ffhip_ctx ctx, ffhip_p2v sp, ffhip_mat A, int kind, double
     FFHIP_REQUIRE(ctx && sp && A && A->p2v == sp && A->slot8 && A->ncomp == 1, FFHIP_ERR_ARG,
                   "ffhip_assemble_p2: matrix pattern was not built by ffhip_pattern_p2 from this space");
     FFHIP_REQUIRE(kind == FFHIP_COEF_CONST || kind == FFHIP_COEF_SIN3 || kind == FFHIP_COEF_P0 ||
-                      kind == FFHIP_COEF_QP,
+                      kind == FFHIP_COEF_QP || kind == FFHIP_COEF_P1,
                   FFHIP_ERR_ARG, "ffhip_assemble_p2: coefficient kind %d", kind);
-    const bool arr = kind == FFHIP_COEF_P0 || kind == FFHIP_COEF_QP;
+    const bool arr = kind == FFHIP_COEF_P0 || kind == FFHIP_COEF_QP || kind == FFHIP_COEF_P1;
     FFHIP_REQUIRE(!arr || kappa_e, FFHIP_ERR_ARG, "ffhip_assemble_p2: kappa_e");
     hipStream_t s = ctx->stream;
     ffhip_mesh m = sp->mesh;
-    const int64_t cnt = kind == FFHIP_COEF_P0 ? m->nt : 14 * m->nt;
+    const int64_t cnt = kind == FFHIP_COEF_P1 ? m->nv : kind == FFHIP_COEF_P0 ? m->nt : 14 * m->nt;
     Arena ar(ctx, Arena::bytes(100 * m->nt, 8) + (arr && !kappa_on_device ? Arena::bytes(cnt, 8) : 0));
     double *Ke = ar.take<double>(100 * m->nt);
     const double *kd = kappa_e;

@@ -23,12 +23,13 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIBPATH = os.environ.get("FFHIP_LIB") or os.path.join(os.path.dirname(HERE), "libffhip.so")
 
-COEF_NONE, COEF_CONST, COEF_SIN3, COEF_P0, COEF_QP = -1, 0, 1, 2, 3
+COEF_NONE, COEF_CONST, COEF_SIN3, COEF_P0, COEF_QP, COEF_P1 = -1, 0, 1, 2, 3, 4
 CG_FP64, CG_MIXED = 0, 1
 CG_LAYOUT_AUTO, CG_LAYOUT_SELL, CG_LAYOUT_CSR, CG_LAYOUT_BSR3 = 0, 0x10, 0x20, 0x100
 CG_INDEX16, CG_INDEX32 = 0x40, 0x80
 CG_EAGER = 0x200
 CG2_IDENTITY, CG2_JACOBI = 0, 1
+ERR_ARG, ERR_HIP, ERR_NOCONV, ERR_NAN, ERR_UNSUPPORTED, ERR_INTERNAL = -1, -2, -3, -4, -5, -6
 ERRORS = {-1: "bad argument", -2: "HIP error", -3: "CG did not converge", -4: "NaN in CG",
           -5: "unsupported", -6: "internal error"}
 
@@ -370,7 +371,8 @@ class Matrix:
                     mass_e=None):
         """int3d(kappa grad u . grad v) [+ int3d(m u v) when mkind != COEF_NONE].
         kappa_e / mass_e: numpy (host) or torch cuda tensors (device), nt
-        values (P0) or nt x nq (QP); both on the same side."""
+        values (P0), nt x nq (QP) or nv (P1, vertex values of a P1
+        function); both on the same side."""
         arrs = [a for a in (kappa_e, mass_e) if a is not None]
         on_dev = int(any(not isinstance(a, np.ndarray) for a in arrs))
         assert all((not isinstance(a, np.ndarray)) == bool(on_dev) for a in arrs), \
@@ -512,7 +514,7 @@ def rhs_p1(mesh, f=1.0, labels=(), tgv=1e30, g=0.0, out=None):
 def rhs_p1_form(mesh, kind=COEF_CONST, f0=1.0, fcoef=None, labels=(), tgv=1e30, g0=0.0, gvals=None,
                 out=None):
     """b = vb(0,Vh) for int(f*v) + on(labels, u=g): f as kind/f0/fcoef (P0: nt
-    values, QP: nt x nq), g per dof (gvals) or constant g0.  out: torch cuda
+    values, QP: nt x nq, P1: nv vertex values), g per dof (gvals) or constant g0.  out: torch cuda
     tensor (then fcoef/gvals are device tensors too) or None (numpy)."""
     lab = np.asarray(labels, np.int32)
     if out is None:

@@ -93,7 +93,7 @@ int ffhip_pattern_p1(ffhip_ctx ctx, ffhip_mesh mesh, ffhip_mat *mat);
 int ffhip_mat_info(ffhip_mat mat, int64_t *n, int64_t *nnz);
 
 enum { FFHIP_COEF_NONE = -1, FFHIP_COEF_CONST = 0, FFHIP_COEF_SIN3 = 1, FFHIP_COEF_P0 = 2,
-       FFHIP_COEF_QP = 3 };
+       FFHIP_COEF_QP = 3, FFHIP_COEF_P1 = 4 };
 /* Values of int3d(Th)(kappa*(dx(u)*dx(v)+dy(u)*dy(v)+dz(u)*dz(v))) with the
  * default Tet_5 rule (src/fflib/problem.cpp:14102-14124), replacing
  * AssembleBilinearForm 3D (problem.cpp:1116-1417) + Element_Op
@@ -102,8 +102,13 @@ enum { FFHIP_COEF_NONE = -1, FFHIP_COEF_CONST = 0, FFHIP_COEF_SIN3 = 1, FFHIP_CO
  * QP -> kappa_e[element * nq + q], the coefficient already evaluated at the
  * nq quadrature points of each element (nq = 14 for Tet_5, 7 for T_5) — the
  * values FreeFEM's interpreter produces for any scalar coefficient expression
- * (problem.cpp:6392-6395).  kappa_e is a host pointer, or a device pointer
- * when kappa_on_device != 0.  2D meshes: int2d with T_5, terms dx,dy. */
+ * (problem.cpp:6392-6395); P1 -> kappa_e[vertex] (nv values), a P1 finite
+ * element function interpolated at each quadrature point as FreeFEM evaluates
+ * one there (TypeOfFE_P1Lagrange3d::operator(), P012_3d.cpp:98-106:
+ * u0 l0 + u1 l1 + u2 l2 + u3 l3 with l0 = 1 - x - y - z; 2D:
+ * TypeOfFE_P1Lagrange::operator(), FESpace.cpp:1078-1085).  kappa_e is a host
+ * pointer, or a device pointer when kappa_on_device != 0.  2D meshes: int2d
+ * with T_5, terms dx,dy. */
 int ffhip_assemble_p1(ffhip_ctx ctx, ffhip_mesh mesh, ffhip_mat mat, int kind,
                       double c0, const double *kappa_e, int kappa_on_device);
 /* Values of int3d(Th)(kappa*(dx(u)*dx(v)+dy(u)*dy(v)+dz(u)*dz(v)) + m*u*v)
@@ -111,7 +116,8 @@ int ffhip_assemble_p1(ffhip_ctx ctx, ffhip_mesh mesh, ffhip_mat mat, int kind,
  * (op_id x op_id, problem.cpp:6384-6437; P1 values P012_3d.cpp:123-175), e.g.
  * the u*v/dt of an implicit heat step.  m is described like kappa: mkind
  * FFHIP_COEF_NONE (no reaction term: ffhip_assemble_p1), CONST (m0), SIN3,
- * P0 (mass_e[element]) or QP (mass_e at the quadrature points).  kappa_e and
+ * P0 (mass_e[element]), QP (mass_e at the quadrature points) or P1
+ * (mass_e[vertex], interpolated at the points).  kappa_e and
  * mass_e are device pointers when on_device != 0, else host pointers. */
 int ffhip_assemble_p1_form(ffhip_ctx ctx, ffhip_mesh mesh, ffhip_mat mat, int kind, double c0,
                            const double *kappa_e, int mkind, double m0, const double *mass_e,
@@ -133,7 +139,8 @@ int ffhip_rhs_p1(ffhip_ctx ctx, ffhip_mesh mesh, double f,
  * f described like kappa in ffhip_assemble_p1 (kind CONST f0, SIN3, P0
  * fcoef[element], QP fcoef[element * nq + q] at the Tet_5 / T_5 points — the
  * values FreeFEM's interpreter gives any expression there, Element_rhs
- * problem.cpp:7940-7978), and on(labels, u=g) rows set to tgv * g with g per
+ * problem.cpp:7940-7978 — or P1 fcoef[vertex] interpolated at the points),
+ * and on(labels, u=g) rows set to tgv * g with g per
  * dof (gvals, n values: g at the dof's vertex, as AssembleBC evaluates it,
  * problem.cpp:10120-10176) or the constant g0 when gvals is NULL.  fcoef,
  * gvals and b are device pointers when on_device != 0, else host pointers. */

@@ -61,6 +61,10 @@ int64_t ffo_pattern_bruteforce(int n, int64_t nel, int nloc,
 
 /* ---- P1 assembly -------------------------------------------------------- */
 enum { FFO_COEF_CONST = 0, FFO_COEF_SIN3 = 1, FFO_COEF_P0 = 2, FFO_COEF_QP = 3 };
+/* A P1 function (vertex values u) at the Tet_5 / T_5 points of every element,
+ * FreeFEM's P1 evaluation order (P012_3d.cpp:98-106; FESpace.cpp:1078-1085):
+ * out[e * nq + q]; returns nq. */
+int ffo_p1_at_qp(int dim, int64_t nt, const int32_t *elems, const double *u, double *out);
 /* Coefficient kappa(x,y,z): CONST -> c0; SIN3 -> 1+0.5*sin(2 pi x)sin(2 pi y)
  * sin(2 pi z) evaluated as the script expression; P0 -> kappa_e[element]. */
 /* Element_Op full 3D (problem.cpp:6337-6437) for

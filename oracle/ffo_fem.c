@@ -447,6 +447,36 @@ int ffo_rhs_p1_coef(int dim, int nv, int64_t nt, const double *X, const int32_t 
   return 0;
 }
 
+/* A P1 finite element function (vertex values u) at the quadrature points
+ * of every element, as FreeFEM evaluates one inside Element_Op / Element_rhs
+ * (the MeshPoint is on the element itself, pf3r2R lgmesh3.cpp:2169-2256):
+ * 3D TypeOfFE_P1Lagrange3d::operator() (P012_3d.cpp:98-106)
+ *   l0 = 1 - x - y - z;  r = u0*l0 + u1*l1 + l2*u2 + l3*u3,
+ * 2D TypeOfFE_P1Lagrange::operator() (FESpace.cpp:1078-1085)
+ *   l0 = 1 - x - y;  r = u0*l0 + u1*l1 + l2*u2,
+ * at the Tet_5 (14) / T_5 (7) points; out[e * nq + q]. */
+int ffo_p1_at_qp(int dim, int64_t nt, const int32_t *elems, const double *u, double *out) {
+  double qp[42], qw[14];
+  const int nq = dim == 3 ? ffo_qf_tet5(qp, qw) : ffo_qf_tri5(qp, qw);
+  const int nloc = dim + 1;
+  for (int64_t e = 0; e < nt; ++e) {
+    const int32_t *t = elems + nloc * e;
+    for (int q = 0; q < nq; ++q) {
+      const double *Ph = qp + dim * q;
+      double r;
+      if (dim == 3) {
+        const double l0 = 1 - Ph[0] - Ph[1] - Ph[2], l1 = Ph[0], l2 = Ph[1], l3 = Ph[2];
+        r = u[t[0]] * l0 + u[t[1]] * l1 + l2 * u[t[2]] + l3 * u[t[3]];
+      } else {
+        const double l0 = 1 - Ph[0] - Ph[1], l1 = Ph[0], l2 = Ph[1];
+        r = u[t[0]] * l0 + u[t[1]] * l1 + l2 * u[t[2]];
+      }
+      out[e * nq + q] = r;
+    }
+  }
+  return nq;
+}
+
 /* ---- Dirichlet ---------------------------------------------------------- */
 int ffo_mark_bc(int n, int64_t nbf, int nvf, const int32_t *bfaces,
                 const int32_t *blab, int nlab, const int32_t *labs,

@@ -387,8 +387,11 @@ def _fan_mesh(m=20):
                 blab=np.ones(len(tris), np.int32))
 
 
-def test_high_degree_vertex_pattern_and_values():
-    m = _fan_mesh(20)
+@pytest.mark.parametrize("ring", [20, 100, 253])
+def test_high_degree_vertex_pattern_and_values(ring):
+    """centre rows of 23 (lane-per-row merge, 64 LDS columns), 103 and 256
+    columns (the wide merge, 256 columns: the byte offsets' limit)"""
+    m = _fan_mesh(ring)
     rp, ci = ffo.pattern(m["nv"], m["tets"], 4)
     vo = ffo.assemble_p1_3d(m, rp, ci, ffo.COEF_CONST, 1.0, None)
     M = ffhip.mesh3_upload(m["xyz"], m["tets"], m["btri"], m["blab"])
@@ -397,7 +400,24 @@ def test_high_degree_vertex_pattern_and_values():
     r2, c2, v2 = A.download()
     assert np.array_equal(r2, rp) and np.array_equal(c2, ci)
     assert np.abs(v2 - vo).max() <= 1e-13 * np.abs(vo).max()
-    assert rp[1] - rp[0] == m["nv"]  # the centre couples to every vertex
+    assert rp[1] - rp[0] == m["nv"] == ring + 3  # the centre couples to every vertex
+
+
+def test_row_wider_than_256_columns_is_refused_without_leak():
+    """a centre row of 303 columns: FFHIP_ERR_UNSUPPORTED (the element-to-row
+    map stores byte offsets), and the failed call frees what it allocated:
+    repeating it does not lower the device's free memory"""
+    import torch
+    m = _fan_mesh(300)
+    M = ffhip.mesh3_upload(m["xyz"], m["tets"], m["btri"], m["blab"])
+    free = []
+    for _ in range(4):
+        with pytest.raises(ffhip.FFHipError) as ei:
+            ffhip.Matrix.p1(M)
+        assert ei.value.code == ffhip.ERR_UNSUPPORTED
+        torch.cuda.synchronize()
+        free.append(torch.cuda.mem_get_info()[0])
+    assert free[-1] >= free[1]
 
 
 @pytest.mark.parametrize("layout", ["sell", "csr"])

@@ -54,7 +54,10 @@
 //   -L<repo>/freefem-sources_amd -lffhip
 #include <ff++.hpp>
 
+#include <dlfcn.h>
+
 #include <cstring>
+#include <memory>
 #include <set>
 #include <vector>
 
@@ -62,6 +65,7 @@
 #include "problem.hpp"
 
 #include "ffhip.h"
+#include "ffhip_glue.hpp"
 
 using namespace std;
 using namespace Fem2D;
@@ -93,70 +97,92 @@ void hip_check(int rc, const char *what) {
 }
 
 // ---------------------------------------------------------------- solver
+typedef HashMatrix<int, double> HMat;
+typedef ffhip_glue::SolverCore<HMat> Core;
+
+// The device state an assembly operator hands to the solver SetSolver creates
+// next for the same matrix (set just before SetSolver, claimed by the
+// solver's constructor, freed by the operator if no ffhip solver took it).
+struct Handover {
+  const HMat *H = nullptr;
+  ffhip_glue::Resident *r = nullptr;
+};
+Handover &handover() {
+  static Handover h;
+  return h;
+}
+ffhip_glue::Resident *claim_handover(const HMat &A) {
+  Handover &h = handover();
+  if (h.H != &A) return nullptr;
+  ffhip_glue::Resident *r = h.r;
+  h = Handover();
+  return r;
+}
+
+// common part of the ffhip solvers: the device side (SolverCore) and the
+// redo bookkeeping of UpdateState
+class HipSolverBase : public VirtualSolver<int, double> {
+ public:
+  HMat *ptA;
+  Core core;
+  long cn, cs;
+  HipSolverBase(HMat &A, int mode) : ptA(&A), core(&A, hip_ctx(), mode), cn(0), cs(0) {
+    core.adopt(claim_handover(A));
+  }
+  // the matrix's redo flags (HashMatrix.hpp:278-279) as the reference's
+  // sparse solvers read them; the device copy is refreshed before the next
+  // solve when they are set (a script that changed A after the solver was made)
+  void UpdateState() {
+    const bool num = ptA->GetReDoNumerics(), sym = ptA->GetReDoSymbolic();
+    core.note_changes(num, sym);
+    if (num) cn++;
+    if (sym) cs++;
+    this->ChangeCodeState(ptA->n, cs, cn);
+  }
+  void fac_symbolic() { hip_check(core.symbolic(), "ffhip solver: matrix to the GPU"); }
+  void fac_numeric() { hip_check(core.numeric(), "ffhip solver: matrix values"); }
+  // the assembly's device state, for a re-assembly that keeps the pattern
+  // (only when this solver is the matrix's sole owner)
+  ffhip_glue::Resident *release_resident() { return this->count == 0 ? core.take_resident() : nullptr; }
+};
+
+// solver=HIPCG / HIPCGMIXED: SolverCG (VirtualSolverCG.hpp:112-192) on the GPU
 template <int MODE>
-class SolverHIPCG : public VirtualSolver<int, double> {
+class SolverHIPCG : public HipSolverBase {
  public:
   // 1 unsym, 2 herm, 4 sym, 8 pos, 32 seq: as SolverCG
   static const int orTypeSol = 1 | 2 | 4 | 8 | 32;
-  typedef HashMatrix<int, double> HMat;
-  HMat *ptA;
-  ffhip_mat M;
-  ffhip_cg cg;
   int verb, itermax;
   double eps, tgv;
   double *veps;
   long *getnbiter;
-  long cn, cs;
 
   SolverHIPCG(HMat &A, const Data_Sparse_Solver &ds, Stack)
-      : ptA(&A), M(nullptr), cg(nullptr), verb(ds.verb), itermax(ds.itmax > 0 ? ds.itmax : A.n),
-        eps(ds.epsilon), tgv(ds.tgv), veps(ds.veps), getnbiter(ds.getnbiter), cn(0), cs(0) {
+      : HipSolverBase(A, MODE ? FFHIP_CG_MIXED : FFHIP_CG_FP64), verb(ds.verb),
+        itermax(ds.itmax > 0 ? ds.itmax : A.n), eps(ds.epsilon), tgv(ds.tgv), veps(ds.veps),
+        getnbiter(ds.getnbiter) {
     ffassert(A.n == A.m);
+    if (ds.precon) ExecError("HIPCG: precon= is not supported (the GPU solver applies Jacobi)");
     if (verb > 4)
       cout << " ## SolverHIPCG" << (MODE ? "MIXED " : " ") << A.n << " eps " << eps << " itmax "
-           << itermax << endl;
+           << itermax << (core.res ? " (device copy from the assembly)" : "") << endl;
   }
 
-  void UpdateState() {
-    if (ptA->GetReDoNumerics()) cn++;
-    if (ptA->GetReDoSymbolic()) cs++;
-    this->ChangeCodeState(ptA->n, cs, cn);
-  }
-
-  // the pattern: HashMatrix::CSR (sorted, HashMatrix.cpp:858-867) to the GPU
-  void fac_symbolic() {
-    int *ia = 0, *ja = 0;
-    double *a = 0;
-    ptA->CSR(ia, ja, a);
-    const int64_t n = ptA->n, nnz = ia[n];
-    vector<int64_t> rp(ia, ia + n + 1);
-    if (cg) { ffhip_cg_free(cg); cg = nullptr; }
-    if (M) { ffhip_mat_free(M); M = nullptr; }
-    hip_check(ffhip_mat_upload(hip_ctx(), n, nnz, rp.data(), ja, a, &M), "HIPCG: matrix upload");
-    hip_check(ffhip_cg_create(hip_ctx(), M, MODE ? FFHIP_CG_MIXED : FFHIP_CG_FP64, &cg),
-              "HIPCG: solver setup");
-  }
-
-  void fac_numeric() {
-    int *ia = 0, *ja = 0;
-    double *a = 0;
-    ptA->CSR(ia, ja, a);
-    hip_check(ffhip_mat_set_values(M, a), "HIPCG: matrix values");
-    hip_check(ffhip_cg_update_values(cg), "HIPCG: solver values");
-  }
-
-  // SolverCG::dosolver semantics (VirtualSolverCG.hpp:169-190)
+  // SolverCG::dosolver semantics (VirtualSolverCG.hpp:169-190): x holds the
+  // initial guess, tgv rows start at b_i / tgv, veps receives the absolute
+  // threshold ConjugueGradient leaves in eps (CG.cpp:226)
   void dosolver(double *x, double *b, int N, int trans) {
+    hip_check(core.refresh(), "HIPCG: matrix to the GPU");
     int err = 0;
     if (getnbiter) *getnbiter = 0;
     for (int k = 0; k < N; ++k) {
       ffhip_cg_stats st;
-      const int rc = ffhip_cg_solve(cg, b + (size_t)k * ptA->n, x + (size_t)k * ptA->n, 0, eps,
+      const int rc = ffhip_cg_solve(core.cg, b + (size_t)k * ptA->n, x + (size_t)k * ptA->n, 0, eps,
                                     itermax, tgv, trans, &st);
       if (rc == FFHIP_ERR_NOCONV) err++;
       else hip_check(rc, "HIPCG: solve");
       if (rc == FFHIP_OK && getnbiter) *getnbiter += st.iters;
-      if (veps) *veps = st.gcg;
+      if (veps) *veps = ffhip_glue::cg_written_back_eps(eps, st.gcg0);
       if (verb > 1)
         cout << " HIPCG: " << (st.converged ? "converge after " : "no convergence after ") << st.iters
              << " g=" << st.gcg << " (" << st.ms_solve << " ms on the GPU)" << endl;
@@ -166,72 +192,36 @@ class SolverHIPCG : public VirtualSolver<int, double> {
       ffassert(0);
     }
   }
-
-  ~SolverHIPCG() {
-    // device buffers only: the HashMatrix may already be deleted
-    // (type_isdeleted, HashMatrix.hpp:62) and is not touched here
-    if (cg) ffhip_cg_free(cg);
-    if (M) ffhip_mat_free(M);
-  }
 };
 
 // solver=HIPGMRES: SolverGMRES (VirtualSolverCG.hpp:196-255) — restarted
 // right-preconditioned GMRES (fgmres, CG.cpp:347-517) with the Jacobi
 // diagonal, for non-symmetric matrices; restart = dimKrylov (NbSpace)
-class SolverHIPGMRES : public VirtualSolver<int, double> {
+class SolverHIPGMRES : public HipSolverBase {
  public:
   // 1 unsym, 2 herm, 4 sym, 8 pos, 16 nopos, 32 seq: as SolverGMRES
   static const int orTypeSol = 1 | 2 | 4 | 8 | 16 | 32;
-  typedef HashMatrix<int, double> HMat;
-  HMat *ptA;
-  ffhip_mat M;
-  ffhip_cg cg;
   int verb, itermax, restart;
   double eps, tgv;
   double *veps;
   long *getnbiter;
-  long cn, cs;
 
   SolverHIPGMRES(HMat &A, const Data_Sparse_Solver &ds, Stack)
-      : ptA(&A), M(nullptr), cg(nullptr), verb(ds.verb), itermax(ds.itmax > 0 ? ds.itmax : A.n),
-        restart(ds.NbSpace), eps(ds.epsilon), tgv(ds.tgv), veps(ds.veps),
-        getnbiter(ds.getnbiter), cn(0), cs(0) {
+      : HipSolverBase(A, FFHIP_CG_FP64), verb(ds.verb), itermax(ds.itmax > 0 ? ds.itmax : A.n),
+        restart(ds.NbSpace), eps(ds.epsilon), tgv(ds.tgv), veps(ds.veps), getnbiter(ds.getnbiter) {
     ffassert(A.n == A.m);
+    if (ds.precon) ExecError("HIPGMRES: precon= is not supported (the GPU solver applies Jacobi)");
   }
 
-  void UpdateState() {
-    if (ptA->GetReDoNumerics()) cn++;
-    if (ptA->GetReDoSymbolic()) cs++;
-    this->ChangeCodeState(ptA->n, cs, cn);
-  }
-
-  void fac_symbolic() {
-    int *ia = 0, *ja = 0;
-    double *a = 0;
-    ptA->CSR(ia, ja, a);
-    const int64_t n = ptA->n, nnz = ia[n];
-    vector<int64_t> rp(ia, ia + n + 1);
-    if (cg) { ffhip_cg_free(cg); cg = nullptr; }
-    if (M) { ffhip_mat_free(M); M = nullptr; }
-    hip_check(ffhip_mat_upload(hip_ctx(), n, nnz, rp.data(), ja, a, &M), "HIPGMRES: matrix upload");
-    hip_check(ffhip_cg_create(hip_ctx(), M, FFHIP_CG_FP64, &cg), "HIPGMRES: solver setup");
-  }
-
-  void fac_numeric() {
-    int *ia = 0, *ja = 0;
-    double *a = 0;
-    ptA->CSR(ia, ja, a);
-    hip_check(ffhip_mat_set_values(M, a), "HIPGMRES: matrix values");
-    hip_check(ffhip_cg_update_values(cg), "HIPGMRES: solver values");
-  }
-
-  // SolverGMRES::dosolver semantics (VirtualSolverCG.hpp:236-255)
+  // SolverGMRES::dosolver semantics (VirtualSolverCG.hpp:236-255): veps
+  // receives eps unchanged, the iteration count is the last right-hand side's
   void dosolver(double *x, double *b, int N, int trans) {
+    hip_check(core.refresh(), "HIPGMRES: matrix to the GPU");
     int err = 0;
     for (int k = 0; k < N; ++k) {
       ffhip_cg_stats st;
-      const int rc = ffhip_gmres_solve(cg, b + (size_t)k * ptA->n, x + (size_t)k * ptA->n, 0, eps,
-                                       itermax, restart, tgv, trans, &st);
+      const int rc = ffhip_gmres_solve(core.cg, b + (size_t)k * ptA->n, x + (size_t)k * ptA->n, 0,
+                                       eps, itermax, restart, tgv, trans, &st);
       if (rc == FFHIP_ERR_NOCONV) err++;
       else hip_check(rc, "HIPGMRES: solve");
       if (veps) *veps = eps;
@@ -245,11 +235,6 @@ class SolverHIPGMRES : public VirtualSolver<int, double> {
       cerr << "Error: fgmres do not converge nb end =" << err << endl;
       ffassert(0);
     }
-  }
-
-  ~SolverHIPGMRES() {
-    if (cg) ffhip_cg_free(cg);
-    if (M) ffhip_mat_free(M);
   }
 };
 
@@ -456,61 +441,196 @@ bool recognise_p2v(Stack stack, const FESpace3 &Uh, const FESpace3 &Vh, const li
   return true;
 }
 
-// true when built on the GPU; false (nothing changed) when FreeFEM's dof
-// numbering of Uh is not the first-appearance numbering the device builds
-bool assemble_p2v_on_gpu(const FESpace3 &Uh, const RecognisedP2 &r, double tgv,
-                         Matrice_Creuse<double> &A) {
+// kinds of forms a Resident was built for (reuse needs the same kind)
+enum { FORM_P1_3D = 1, FORM_P1_2D = 2, FORM_P2_3D = 3, FORM_P2V_3D = 4 };
+
+typedef std::unique_ptr<ffhip_glue::Resident> ResidentPtr;
+
+// the device mesh of Th (3D) into r
+void upload_mesh3(const Mesh3 &Th, ffhip_glue::Resident &r) {
+  ffhip_glue::MeshArrays a;
+  ffhip_glue::mesh3_arrays(Th, a);
+  hip_check(ffhip_glue::upload_mesh(hip_ctx(), 3, a, &r.mesh), "ffhip: mesh upload");
+  r.Th = &Th;
+}
+
+// the device P2 numbering of r's mesh; false when FreeFEM's dof numbering of
+// Uh is not the device's first-appearance numbering (ncomp = 1: Vh(Th,P2),
+// 3: [P2,P2,P2], local dofs comp-blocked c*10 + i)
+bool p2_numbering(const FESpace3 &Uh, int ncomp, ffhip_glue::Resident &r) {
+  hip_check(ffhip_p2v_create(hip_ctx(), r.mesh, &r.sp), "ffhip: P2 numbering");
   const Mesh3 &Th = Uh.Th;
-  vector<double> xyz(3 * (size_t)Th.nv);
-  vector<int32_t> tets(4 * (size_t)Th.nt), btri(3 * (size_t)Th.nbe), blab(Th.nbe);
-  for (int i = 0; i < Th.nv; ++i) {
-    xyz[3 * i] = Th(i).x; xyz[3 * i + 1] = Th(i).y; xyz[3 * i + 2] = Th(i).z;
-  }
-  for (int k = 0; k < Th.nt; ++k)
-    for (int j = 0; j < 4; ++j) tets[4 * k + j] = Th(Th[k][j]);
-  for (int k = 0; k < Th.nbe; ++k) {
-    for (int j = 0; j < 3; ++j) btri[3 * k + j] = Th(Th.be(k)[j]);
-    blab[k] = Th.be(k).lab;
-  }
-  ffhip_ctx ctx = hip_ctx();
-  ffhip_mesh m = nullptr;
-  ffhip_p2v sp = nullptr;
-  hip_check(ffhip_mesh3_upload(ctx, Th.nv, Th.nt, Th.nbe, xyz.data(), tets.data(), btri.data(),
-                               blab.data(), &m), "ffhip: mesh upload");
-  hip_check(ffhip_p2v_create(ctx, m, &sp), "ffhip: P2 numbering");
   vector<int32_t> en(10 * (size_t)Th.nt);
-  hip_check(ffhip_p2v_download(sp, en.data()), "ffhip: P2 numbering");
-  bool same = true;
-  for (int k = 0; k < Th.nt && same; ++k) {
+  hip_check(ffhip_p2v_download(r.sp, en.data()), "ffhip: P2 numbering");
+  for (int k = 0; k < Th.nt; ++k) {
     const FElement3 K(Uh[k]);
-    for (int df = 0; df < 30 && same; ++df)  // local dofs comp-blocked (c*10 + i)
-      same = K(df) == 3 * en[10 * (size_t)k + df % 10] + df / 10;
+    for (int df = 0; df < 10 * ncomp; ++df)
+      if (K(df) != ncomp * en[10 * (size_t)k + df % 10] + df / 10) return false;
   }
-  if (!same) {
-    ffhip_p2v_free(sp);
-    ffhip_mesh_free(m);
-    return false;
-  }
-  ffhip_mat M = nullptr;
-  hip_check(ffhip_pattern_p2v(ctx, sp, &M), "ffhip: P2 pattern");
-  hip_check(ffhip_assemble_p2v_elasticity(ctx, sp, M, r.lambda, r.mu), "ffhip: P2 assembly");
-  if (!r.labels.empty()) {
-    vector<int32_t> labs(r.labels.begin(), r.labels.end());
-    hip_check(ffhip_set_bc_p2v(ctx, sp, M, labs.data(), (int)labs.size(), tgv, nullptr), "ffhip: on()");
+  return true;
+}
+
+// the device matrix of res into A: a new HashMatrix (res->nnz_host set), or,
+// when A's HashMatrix already holds this pattern (reuse), its values only
+void device_into_matrix(ffhip_mat M, int half, bool reuse, ffhip_glue::Resident &res,
+                        Matrice_Creuse<double> &A) {
+  if (reuse) {
+    hip_check(ffhip_glue::values_into_hashmatrix(M, *A.pHM()), "ffhip: values to the matrix");
+    return;
   }
   int64_t n = 0, nnz = 0;
   ffhip_mat_info(M, &n, &nnz);
-  vector<int64_t> rp(n + 1);
-  vector<int> ia(n + 1), ja(nnz);
-  vector<double> aa(nnz);
-  hip_check(ffhip_mat_download(M, rp.data(), ja.data(), aa.data()), "ffhip: download");
-  for (int64_t i = 0; i <= n; ++i) ia[i] = (int)rp[i];
-  ffhip_mat_free(M);
-  ffhip_p2v_free(sp);
-  ffhip_mesh_free(m);
-  MatriceMorse<double> *pH = new MatriceMorse<double>((int)n, (int)n, (int)nnz, 0);
-  pH->set((int)n, (int)n, 0, (size_t)nnz, ia.data(), ja.data(), aa.data(), 0, 1);
+  // an ordinary HashMatrix (HashMatrix::set with CSR input, HashMatrix.cpp:697-728),
+  // so every later FreeFEM operation works on it
+  MatriceMorse<double> *pH = new MatriceMorse<double>((int)n, (int)n, (int)nnz, half);
+  const int rc = ffhip_glue::csr_into_hashmatrix(M, *pH, half);
+  if (rc != FFHIP_OK) {
+    delete pH;
+    hip_check(rc, "ffhip: matrix to FreeFEM");
+  }
   A.A.master(pH);
+  res.nnz_host = (int64_t)pH->nnz;
+}
+
+// [P2,P2,P2] Lame form; true when built on the GPU, false (A untouched) when
+// FreeFEM's dof numbering is not the device's.  res: in, the previous
+// assembly's device state to reuse (pattern kept), or null; out, this one's.
+bool assemble_p2v_on_gpu(const FESpace3 &Uh, const RecognisedP2 &r, double tgv,
+                         Matrice_Creuse<double> &A, ResidentPtr &res) {
+  ffhip_ctx ctx = hip_ctx();
+  const bool reuse = (bool)res;
+  if (!reuse) {
+    res.reset(new ffhip_glue::Resident);
+    res->kind = FORM_P2V_3D;
+    res->Vh = &Uh;
+    upload_mesh3(Uh.Th, *res);
+    if (!p2_numbering(Uh, 3, *res)) {
+      res.reset();
+      return false;
+    }
+    hip_check(ffhip_pattern_p2v(ctx, res->sp, &res->M), "ffhip: P2 pattern");
+  }
+  hip_check(ffhip_assemble_p2v_elasticity(ctx, res->sp, res->M, r.lambda, r.mu), "ffhip: P2 assembly");
+  if (!r.labels.empty()) {
+    vector<int32_t> labs(r.labels.begin(), r.labels.end());
+    hip_check(ffhip_set_bc_p2v(ctx, res->sp, res->M, labs.data(), (int)labs.size(), tgv, nullptr),
+              "ffhip: on()");
+  }
+  device_into_matrix(res->M, 0, reuse, *res, A);
+  return true;
+}
+
+// ---------------------------------------------------------------- coefficients
+// The functions FreeFEM compiles a real finite element function into when a
+// script uses it as a value: the casts pf3r -> double (3D, lgmesh3.cpp:
+// 3093-3095, over pf3r2R<double,0,v_fes3>, lgmesh3.cpp:2169-2256) and
+// pfer -> double (2D, lgfem.cpp:6993, over pfer2R<double,0>, lgfem.cpp:2054).
+// They are looked up by name in the running executable (FreeFEM links with
+// -rdynamic); where one is not found, coefficients go through the
+// interpreter at the quadrature points as before.
+Function1 fe_value_function(int dim) {
+  static Function1 f3 = nullptr, f2 = nullptr;
+  static bool looked = false;
+  if (!looked) {
+    looked = true;
+    const char *n3[2] = {"_Z6pf3r2RIdLi0E6v_fes3E19AnyTypeWithOutCheckPvRKS1_",
+                         "_Z6pf3r2RIdLi0E6v_fes3E16AnyTypeWithCheckPvRKS1_"};
+    const char *n2[2] = {"_Z6pfer2RIdLi0EE19AnyTypeWithOutCheckPvRKS0_",
+                         "_Z6pfer2RIdLi0EE16AnyTypeWithCheckPvRKS0_"};
+    for (int k = 0; k < 2 && !f3; ++k) f3 = (Function1)dlsym(RTLD_DEFAULT, n3[k]);
+    for (int k = 0; k < 2 && !f2; ++k) f2 = (Function1)dlsym(RTLD_DEFAULT, n2[k]);
+  }
+  return dim == 3 ? f3 : f2;
+}
+
+// A volume coefficient as the device takes it: CONST (c0) when it does not
+// depend on the point, P0 / P1 (the array of the function's values) when it
+// is a P0 or P1 finite element function of the form's own mesh with the
+// standard numbering, else QP with the interpreter's values at the points.
+struct CoefArg {
+  int kind = FFHIP_COEF_CONST;
+  double c0 = 0.;
+  const double *ptr = nullptr;
+  vector<double> own;
+};
+
+// the spaces whose numbering was checked (dof = element for P0, dof = vertex
+// number for P1), so the O(nt) check runs once per space
+bool numbering_checked(const UniqueffId &id, bool set) {
+  static vector<UniqueffId> ok;
+  for (const UniqueffId &u : ok)
+    if (u == id) return true;
+  if (set) {
+    if (ok.size() > 16) ok.erase(ok.begin());
+    ok.push_back(id);
+  }
+  return false;
+}
+
+// c is a P0 / P1 function of Th (3D): its kind and values (true), else false
+bool fe_coef3(Stack stack, const C_F0 &c, const Mesh3 &Th, CoefArg &out) {
+  const Function1 f = fe_value_function(3);
+  const E_F0_Func1 *ef = dynamic_cast<const E_F0_Func1 *>(c.LeftValue());
+  if (!f || !ef || ef->f != f) return false;
+  pf3r p = GetAny<pf3r>((*ef->a)(stack));
+  FEbase<double, v_fes3> *fe = p.first;
+  if (!fe || p.second != 0) return false;
+  if (!fe->x()) {  // an unset function evaluates to 0 (pf3r2R)
+    out.kind = FFHIP_COEF_CONST;
+    out.c0 = 0.;
+    return true;
+  }
+  const FESpace3 *Vh = (FESpace3 *)fe->Vh;
+  if (!Vh || &Vh->Th != &Th || Vh->N != 1 || Vh->TFE.N() != 1) return false;
+  const bool p1 = Vh->TFE[0] == &DataFE<Mesh3>::P1, p0 = Vh->TFE[0] == &DataFE<Mesh3>::P0;
+  if (!p1 && !p0) return false;
+  if (Vh->NbOfDF != (p1 ? Th.nv : Th.nt)) return false;
+  if (!numbering_checked(*Vh, false)) {
+    for (int k = 0; k < Th.nt; ++k) {
+      const FElement3 K((*Vh)[k]);
+      if (p0 && K(0) != k) return false;
+      if (p1)
+        for (int i = 0; i < 4; ++i)
+          if (K(i) != Th(Th[k][i])) return false;
+    }
+    numbering_checked(*Vh, true);
+  }
+  out.kind = p1 ? FFHIP_COEF_P1 : FFHIP_COEF_P0;
+  out.ptr = (const double *)*fe->x();
+  return true;
+}
+
+// the same for a 2D mesh
+bool fe_coef2(Stack stack, const C_F0 &c, const Mesh &Th, CoefArg &out) {
+  typedef pair<FEbase<double, v_fes> *, int> pfe2;
+  const Function1 f = fe_value_function(2);
+  const E_F0_Func1 *ef = dynamic_cast<const E_F0_Func1 *>(c.LeftValue());
+  if (!f || !ef || ef->f != f) return false;
+  pfe2 p = GetAny<pfe2>((*ef->a)(stack));
+  FEbase<double, v_fes> *fe = p.first;
+  if (!fe || p.second != 0) return false;
+  if (!fe->x()) {
+    out.kind = FFHIP_COEF_CONST;
+    out.c0 = 0.;
+    return true;
+  }
+  const FESpace *Vh = (FESpace *)fe->Vh;
+  if (!Vh || &Vh->Th != &Th || Vh->N != 1 || Vh->TFE.N() != 1) return false;
+  const bool p1 = Vh->TFE[0] == &P1Lagrange, p0 = Vh->TFE[0] == &P0Lagrange;
+  if (!p1 && !p0) return false;
+  if (Vh->NbOfDF != (p1 ? Th.nv : Th.nt)) return false;
+  if (!numbering_checked(*Vh, false)) {
+    for (int k = 0; k < Th.nt; ++k) {
+      const FElement K((*Vh)[k]);
+      if (p0 && K(0) != k) return false;
+      if (p1)
+        for (int i = 0; i < 3; ++i)
+          if (K(i) != Th(Th[k][i])) return false;
+    }
+    numbering_checked(*Vh, true);
+  }
+  out.kind = p1 ? FFHIP_COEF_P1 : FFHIP_COEF_P0;
+  out.ptr = (const double *)*fe->x();
   return true;
 }
 
@@ -530,6 +650,19 @@ void eval_coef_qp(Stack stack, const FESpace3 &Uh, const C_F0 &coef, vector<doub
     }
   }
   *MeshPointStack(stack) = mp;
+}
+
+// a volume coefficient of a 3D form as the device takes it (CoefArg)
+void coef_arg3(Stack stack, const FESpace3 &Uh, bool is_const, double c0, const C_F0 &c,
+               CoefArg &out) {
+  if (is_const) {
+    out.kind = FFHIP_COEF_CONST;
+    out.c0 = c0;
+  } else if (!fe_coef3(stack, c, Uh.Th, out)) {
+    eval_coef_qp(stack, Uh, c, out.own);
+    out.kind = FFHIP_COEF_QP;
+    out.ptr = out.own.data();
+  }
 }
 
 // a boundary coefficient at the 7 T_5 points of every boundary element, in the
@@ -567,34 +700,37 @@ void eval_coef_bqp(Stack stack, const FESpace3 &Uh, const Recognised::Boundary &
   *MeshPointStack(stack) = mp;
 }
 
-void assemble_on_gpu(Stack stack, const FESpace3 &Uh, const Recognised &r, double tgv, int sym,
-                     Matrice_Creuse<double> &A) {
+// the P1 form of r on the GPU into A; false (A untouched) when the device
+// pattern cannot hold the mesh (a vertex row of more than 256 columns): the
+// caller then assembles with FreeFEM.  res: in, the previous assembly's
+// device state to reuse (pattern kept), or null; out, this one's.
+bool assemble_on_gpu(Stack stack, const FESpace3 &Uh, const Recognised &r, double tgv, int sym,
+                     Matrice_Creuse<double> &A, ResidentPtr &res) {
   const Mesh3 &Th = Uh.Th;
-  vector<double> xyz(3 * (size_t)Th.nv);
-  vector<int32_t> tets(4 * (size_t)Th.nt), btri(3 * (size_t)Th.nbe), blab(Th.nbe);
-  for (int i = 0; i < Th.nv; ++i) {
-    xyz[3 * i] = Th(i).x; xyz[3 * i + 1] = Th(i).y; xyz[3 * i + 2] = Th(i).z;
-  }
-  for (int k = 0; k < Th.nt; ++k)
-    for (int j = 0; j < 4; ++j) tets[4 * k + j] = Th(Th[k][j]);  // P1 dof = vertex id
-  for (int k = 0; k < Th.nbe; ++k) {
-    for (int j = 0; j < 3; ++j) btri[3 * k + j] = Th(Th.be(k)[j]);
-    blab[k] = Th.be(k).lab;
-  }
   ffhip_ctx ctx = hip_ctx();
-  ffhip_mesh m = nullptr;
-  ffhip_mat M = nullptr;
-  hip_check(ffhip_mesh3_upload(ctx, Th.nv, Th.nt, Th.nbe, xyz.data(), tets.data(), btri.data(),
-                               blab.data(), &m), "ffhip: mesh upload");
-  hip_check(ffhip_pattern_p1(ctx, m, &M), "ffhip: pattern");
+  const bool reuse = (bool)res;
+  if (!reuse) {
+    res.reset(new ffhip_glue::Resident);
+    res->kind = FORM_P1_3D;
+    res->Vh = &Uh;
+    res->half = sym ? 1 : 0;
+    upload_mesh3(Th, *res);
+    const int rc = ffhip_pattern_p1(ctx, res->mesh, &res->M);
+    if (rc == FFHIP_ERR_UNSUPPORTED) {
+      if (verbosity > 1) cout << " ffhip: " << ffhip_last_error() << ": FreeFEM assembles" << endl;
+      res.reset();
+      return false;
+    }
+    hip_check(rc, "ffhip: pattern");
+  }
+  ffhip_mesh m = res->mesh;
+  ffhip_mat M = res->M;
   {
-    vector<double> kq, mq;
-    if (!r.coef_const) eval_coef_qp(stack, Uh, r.coef, kq);
-    if (r.has_mass && !r.mcoef_const) eval_coef_qp(stack, Uh, r.mcoef, mq);
-    const int kind = r.coef_const ? FFHIP_COEF_CONST : FFHIP_COEF_QP;
-    const int mkind = !r.has_mass ? FFHIP_COEF_NONE : (r.mcoef_const ? FFHIP_COEF_CONST : FFHIP_COEF_QP);
-    hip_check(ffhip_assemble_p1_form(ctx, m, M, kind, r.c0, kq.empty() ? nullptr : kq.data(), mkind,
-                                     r.m0, mq.empty() ? nullptr : mq.data(), 0),
+    CoefArg k, mc;
+    coef_arg3(stack, Uh, r.coef_const, r.c0, r.coef, k);
+    if (r.has_mass) coef_arg3(stack, Uh, r.mcoef_const, r.m0, r.mcoef, mc);
+    else mc.kind = FFHIP_COEF_NONE;
+    hip_check(ffhip_assemble_p1_form(ctx, m, M, k.kind, k.c0, k.ptr, mc.kind, mc.c0, mc.ptr, 0),
               "ffhip: assembly");
   }
   for (const Recognised::Boundary &t : r.bnd) {  // in the order of the varf's terms
@@ -621,87 +757,77 @@ void assemble_on_gpu(Stack stack, const FESpace3 &Uh, const Recognised &r, doubl
   if (sym) {  // sym=1: the lower triangle, HashMatrix::half (HashMatrix.cpp:1312-1318)
     ffhip_mat L = nullptr;
     hip_check(ffhip_mat_lower(M, &L), "ffhip: sym=1 lower triangle");
-    ffhip_mat_free(M);
-    M = L;
+    struct FreeL { ffhip_mat L; ~FreeL() { ffhip_mat_free(L); } } guard{L};
+    device_into_matrix(L, 1, reuse, *res, A);
+  } else {
+    device_into_matrix(M, 0, reuse, *res, A);
   }
-  int64_t n = 0, nnz = 0;
-  ffhip_mat_info(M, &n, &nnz);
-  vector<int64_t> rp(n + 1);
-  vector<int> ia(n + 1), ja(nnz);
-  vector<double> aa(nnz);
-  hip_check(ffhip_mat_download(M, rp.data(), ja.data(), aa.data()), "ffhip: download");
-  for (int64_t i = 0; i <= n; ++i) ia[i] = (int)rp[i];
-  ffhip_mat_free(M);
-  ffhip_mesh_free(m);
-  // the result is an ordinary HashMatrix (HashMatrix::set with CSR input,
-  // HashMatrix.cpp:697-728) so every later FreeFEM operation works on it
-  const int half = sym ? 1 : 0;
-  MatriceMorse<double> *pH = new MatriceMorse<double>((int)n, (int)n, (int)nnz, half);
-  pH->set((int)n, (int)n, half, (size_t)nnz, ia.data(), ja.data(), aa.data(), 0, 1);
-  A.A.master(pH);
+  return true;
 }
 
-// Vh(Th,P2): true when built on the GPU; false (nothing changed) when FreeFEM's
+// Vh(Th,P2): true when built on the GPU; false (A untouched) when FreeFEM's
 // dof numbering of Uh is not the first-appearance node numbering of the device
 bool assemble_p2_on_gpu(Stack stack, const FESpace3 &Uh, const Recognised &r, double tgv,
-                        Matrice_Creuse<double> &A) {
-  const Mesh3 &Th = Uh.Th;
-  vector<double> xyz(3 * (size_t)Th.nv);
-  vector<int32_t> tets(4 * (size_t)Th.nt), btri(3 * (size_t)Th.nbe), blab(Th.nbe);
-  for (int i = 0; i < Th.nv; ++i) {
-    xyz[3 * i] = Th(i).x; xyz[3 * i + 1] = Th(i).y; xyz[3 * i + 2] = Th(i).z;
-  }
-  for (int k = 0; k < Th.nt; ++k)
-    for (int j = 0; j < 4; ++j) tets[4 * k + j] = Th(Th[k][j]);
-  for (int k = 0; k < Th.nbe; ++k) {
-    for (int j = 0; j < 3; ++j) btri[3 * k + j] = Th(Th.be(k)[j]);
-    blab[k] = Th.be(k).lab;
-  }
+                        Matrice_Creuse<double> &A, ResidentPtr &res) {
   ffhip_ctx ctx = hip_ctx();
-  ffhip_mesh m = nullptr;
-  ffhip_p2v sp = nullptr;
-  hip_check(ffhip_mesh3_upload(ctx, Th.nv, Th.nt, Th.nbe, xyz.data(), tets.data(), btri.data(),
-                               blab.data(), &m), "ffhip: mesh upload");
-  hip_check(ffhip_p2v_create(ctx, m, &sp), "ffhip: P2 numbering");
-  vector<int32_t> en(10 * (size_t)Th.nt);
-  hip_check(ffhip_p2v_download(sp, en.data()), "ffhip: P2 numbering");
-  bool same = true;
-  for (int k = 0; k < Th.nt && same; ++k) {
-    const FElement3 K(Uh[k]);
-    for (int df = 0; df < 10 && same; ++df) same = K(df) == en[10 * (size_t)k + df];
+  const bool reuse = (bool)res;
+  if (!reuse) {
+    res.reset(new ffhip_glue::Resident);
+    res->kind = FORM_P2_3D;
+    res->Vh = &Uh;
+    upload_mesh3(Uh.Th, *res);
+    if (!p2_numbering(Uh, 1, *res)) {
+      res.reset();
+      return false;
+    }
+    hip_check(ffhip_pattern_p2(ctx, res->sp, &res->M), "ffhip: P2 pattern");
   }
-  if (!same) {
-    ffhip_p2v_free(sp);
-    ffhip_mesh_free(m);
-    return false;
-  }
-  ffhip_mat M = nullptr;
-  hip_check(ffhip_pattern_p2(ctx, sp, &M), "ffhip: P2 pattern");
-  if (r.coef_const) {
-    hip_check(ffhip_assemble_p2(ctx, sp, M, FFHIP_COEF_CONST, r.c0, nullptr, 0), "ffhip: P2 assembly");
-  } else {
-    vector<double> kq;
-    eval_coef_qp(stack, Uh, r.coef, kq);
-    hip_check(ffhip_assemble_p2(ctx, sp, M, FFHIP_COEF_QP, 0., kq.data(), 0), "ffhip: P2 assembly");
+  {
+    CoefArg k;
+    coef_arg3(stack, Uh, r.coef_const, r.c0, r.coef, k);
+    hip_check(ffhip_assemble_p2(ctx, res->sp, res->M, k.kind, k.c0, k.ptr, 0), "ffhip: P2 assembly");
   }
   if (!r.labels.empty()) {
     vector<int32_t> labs(r.labels.begin(), r.labels.end());
-    hip_check(ffhip_set_bc_p2(ctx, sp, M, labs.data(), (int)labs.size(), tgv, nullptr), "ffhip: on()");
+    hip_check(ffhip_set_bc_p2(ctx, res->sp, res->M, labs.data(), (int)labs.size(), tgv, nullptr),
+              "ffhip: on()");
   }
-  int64_t n = 0, nnz = 0;
-  ffhip_mat_info(M, &n, &nnz);
-  vector<int64_t> rp(n + 1);
-  vector<int> ia(n + 1), ja(nnz);
-  vector<double> aa(nnz);
-  hip_check(ffhip_mat_download(M, rp.data(), ja.data(), aa.data()), "ffhip: download");
-  for (int64_t i = 0; i <= n; ++i) ia[i] = (int)rp[i];
-  ffhip_mat_free(M);
-  ffhip_p2v_free(sp);
-  ffhip_mesh_free(m);
-  MatriceMorse<double> *pH = new MatriceMorse<double>((int)n, (int)n, (int)nnz, 0);
-  pH->set((int)n, (int)n, 0, (size_t)nnz, ia.data(), ja.data(), aa.data(), 0, 1);
-  A.A.master(pH);
+  device_into_matrix(res->M, 0, reuse, *res, A);
   return true;
+}
+
+// A's previous GPU assembly, when it can serve again for this one: the same
+// spaces (A.Uh, A.Vh: FreeFEM then keeps the HashMatrix and resets its values,
+// problem.hpp:1679-1691), the same kind of form and storage, A's solver one of
+// ffhip's and not shared, and the HashMatrix's pattern untouched since (no
+// symbolic redo pending, still in CSR order, the same entry count)
+template <class FES>
+ResidentPtr reusable(Matrice_Creuse<double> &A, const FES &Uh, const FES &Vh, int kind, int half) {
+  if (!A.A || A.Uh != Uh || A.Vh != Vh) return ResidentPtr();
+  HMat *H = A.pHM();
+  if (!H || !H->vsolver || H->re_do_symbolic || H->state != HMat::sorted_ij || H->half != half)
+    return ResidentPtr();
+  HipSolverBase *sb = dynamic_cast<HipSolverBase *>(H->vsolver);
+  if (!sb || !sb->core.res) return ResidentPtr();
+  const ffhip_glue::Resident &c = *sb->core.res;
+  if (c.kind != kind || c.half != half || c.Vh != (const void *)&Uh || (int64_t)H->nnz != c.nnz_host)
+    return ResidentPtr();
+  return ResidentPtr(sb->release_resident());
+}
+
+// SetSolver with the device state handed to the solver it creates; what no
+// ffhip solver claimed is freed
+void set_solver_with(Stack stack, bool VF, Matrice_Creuse<double> &A, const Data_Sparse_Solver &ds,
+                     ResidentPtr &res) {
+  struct Clear {
+    ~Clear() {
+      delete handover().r;
+      handover() = Handover();
+    }
+  } clear;
+  handover().H = A.pHM();
+  handover().r = res.release();
+  SetSolver(stack, VF, *A.A, ds);  // as OpMatrixtoBilinearForm (problem.hpp:1654-1656)
 }
 
 class OpHipMatrixP1 : public OneOperator {
@@ -736,28 +862,34 @@ class OpHipMatrixP1 : public OneOperator {
       Recognised r;
       RecognisedP2 r2;
       bool done = false;
+      ResidentPtr res;
       if (ds.tgv > 0 && recognise(stack, *PUh, *PVh, b->largs, r) && (ds.sym == 0 || r.order == 1)) {
-        A.A = 0;
-        A.Uh = *PUh;
-        A.Vh = *PVh;
-        if (r.order == 1) {  // the recognised P1 forms are symmetric: sym=1 keeps the lower half
-          assemble_on_gpu(stack, *PUh, r, ds.tgv, ds.sym, A);
-          done = true;
-        } else {
-          done = assemble_p2_on_gpu(stack, *PUh, r, ds.tgv, A);
+        const int kind = r.order == 1 ? FORM_P1_3D : FORM_P2_3D;
+        res = reusable(A, *PUh, *PVh, kind, r.order == 1 ? ds.sym : 0);
+        if (!res) {
+          A.A = 0;
+          A.Uh = *PUh;
+          A.Vh = *PVh;
         }
+        if (r.order == 1)  // the recognised P1 forms are symmetric: sym=1 keeps the lower half
+          done = assemble_on_gpu(stack, *PUh, r, ds.tgv, ds.sym, A, res);
+        else
+          done = assemble_p2_on_gpu(stack, *PUh, r, ds.tgv, A, res);
       } else if (ds.sym == 0 && ds.tgv > 0 && recognise_p2v(stack, *PUh, *PVh, b->largs, r2)) {
-        A.A = 0;
-        A.Uh = *PUh;
-        A.Vh = *PVh;
-        done = assemble_p2v_on_gpu(*PUh, r2, ds.tgv, A);
+        res = reusable(A, *PUh, *PVh, FORM_P2V_3D, 0);
+        if (!res) {
+          A.A = 0;
+          A.Uh = *PUh;
+          A.Vh = *PVh;
+        }
+        done = assemble_p2v_on_gpu(*PUh, r2, ds.tgv, A, res);
       }
       if (!done) {
         creationBlockOfMatrixToBilinearForm<double, Mesh3, FESpace3, FESpace3>(PUh, PVh, ds.sym, ds.tgv,
                                                                                b->largs, stack, A);
       }
       A.pHM()->half = ds.sym;
-      if (square) SetSolver(stack, VF, *A.A, ds);  // as OpMatrixtoBilinearForm (problem.hpp:1654-1656)
+      if (square) set_solver_with(stack, VF, A, ds, res);
       return SetAny<Matrice_Creuse<double> *>(&A);
     }
   };
@@ -875,6 +1007,19 @@ void eval_coef_qp2(Stack stack, const FESpace &Uh, const C_F0 &coef, vector<doub
   *MeshPointStack(stack) = mp;
 }
 
+// a volume coefficient of a 2D form as the device takes it (CoefArg)
+void coef_arg2(Stack stack, const FESpace &Uh, bool is_const, double c0, const C_F0 &c,
+               CoefArg &out) {
+  if (is_const) {
+    out.kind = FFHIP_COEF_CONST;
+    out.c0 = c0;
+  } else if (!fe_coef2(stack, c, Uh.Th, out)) {
+    eval_coef_qp2(stack, Uh, c, out.own);
+    out.kind = FFHIP_COEF_QP;
+    out.ptr = out.own.data();
+  }
+}
+
 // a boundary coefficient at the 3 Gauss-Legendre points of every boundary
 // edge in the device rule's order, x_q = (1 - s_q) B0 + s_q B1 for its
 // vertices in Th.bedges order, located in the adjacent triangle with the
@@ -912,34 +1057,38 @@ void eval_coef_bqp2(Stack stack, const FESpace &Uh, const Recognised::Boundary &
   *MeshPointStack(stack) = mp;
 }
 
-void assemble2_on_gpu(Stack stack, const FESpace &Uh, const Recognised &r, double tgv, int sym,
-                      Matrice_Creuse<double> &A) {
+// the 2D P1 form of r on the GPU into A; false (A untouched) when the device
+// pattern cannot hold the mesh.  res as for assemble_on_gpu.
+bool assemble2_on_gpu(Stack stack, const FESpace &Uh, const Recognised &r, double tgv, int sym,
+                      Matrice_Creuse<double> &A, ResidentPtr &res) {
   const Mesh &Th = Uh.Th;
-  vector<double> xy(2 * (size_t)Th.nv);
-  vector<int32_t> tris(3 * (size_t)Th.nt), bed(2 * (size_t)Th.neb), blab(Th.neb);
-  for (int i = 0; i < Th.nv; ++i) {
-    xy[2 * i] = Th(i).x; xy[2 * i + 1] = Th(i).y;
-  }
-  for (int k = 0; k < Th.nt; ++k)
-    for (int j = 0; j < 3; ++j) tris[3 * k + j] = Th(Th[k][j]);
-  for (int k = 0; k < Th.neb; ++k) {
-    for (int j = 0; j < 2; ++j) bed[2 * k + j] = Th(Th.bedges[k][j]);
-    blab[k] = Th.bedges[k].lab;
-  }
   ffhip_ctx ctx = hip_ctx();
-  ffhip_mesh m = nullptr;
-  ffhip_mat M = nullptr;
-  hip_check(ffhip_mesh2_upload(ctx, Th.nv, Th.nt, Th.neb, xy.data(), tris.data(), bed.data(),
-                               blab.data(), &m), "ffhip: 2D mesh upload");
-  hip_check(ffhip_pattern_p1(ctx, m, &M), "ffhip: pattern");
+  const bool reuse = (bool)res;
+  if (!reuse) {
+    res.reset(new ffhip_glue::Resident);
+    res->kind = FORM_P1_2D;
+    res->Vh = &Uh;
+    res->Th = &Th;
+    res->half = sym ? 1 : 0;
+    ffhip_glue::MeshArrays a;
+    ffhip_glue::mesh2_arrays(Th, a);
+    hip_check(ffhip_glue::upload_mesh(ctx, 2, a, &res->mesh), "ffhip: 2D mesh upload");
+    const int rc = ffhip_pattern_p1(ctx, res->mesh, &res->M);
+    if (rc == FFHIP_ERR_UNSUPPORTED) {
+      if (verbosity > 1) cout << " ffhip: " << ffhip_last_error() << ": FreeFEM assembles" << endl;
+      res.reset();
+      return false;
+    }
+    hip_check(rc, "ffhip: pattern");
+  }
+  ffhip_mesh m = res->mesh;
+  ffhip_mat M = res->M;
   {
-    vector<double> kq, mq;
-    if (!r.coef_const) eval_coef_qp2(stack, Uh, r.coef, kq);
-    if (r.has_mass && !r.mcoef_const) eval_coef_qp2(stack, Uh, r.mcoef, mq);
-    const int kind = r.coef_const ? FFHIP_COEF_CONST : FFHIP_COEF_QP;
-    const int mkind = !r.has_mass ? FFHIP_COEF_NONE : (r.mcoef_const ? FFHIP_COEF_CONST : FFHIP_COEF_QP);
-    hip_check(ffhip_assemble_p1_form(ctx, m, M, kind, r.c0, kq.empty() ? nullptr : kq.data(), mkind,
-                                     r.m0, mq.empty() ? nullptr : mq.data(), 0),
+    CoefArg k, mc;
+    coef_arg2(stack, Uh, r.coef_const, r.c0, r.coef, k);
+    if (r.has_mass) coef_arg2(stack, Uh, r.mcoef_const, r.m0, r.mcoef, mc);
+    else mc.kind = FFHIP_COEF_NONE;
+    hip_check(ffhip_assemble_p1_form(ctx, m, M, k.kind, k.c0, k.ptr, mc.kind, mc.c0, mc.ptr, 0),
               "ffhip: assembly");
   }
   for (const Recognised::Boundary &t : r.bnd) {
@@ -966,22 +1115,12 @@ void assemble2_on_gpu(Stack stack, const FESpace &Uh, const Recognised &r, doubl
   if (sym) {
     ffhip_mat L = nullptr;
     hip_check(ffhip_mat_lower(M, &L), "ffhip: sym=1 lower triangle");
-    ffhip_mat_free(M);
-    M = L;
+    struct FreeL { ffhip_mat L; ~FreeL() { ffhip_mat_free(L); } } guard{L};
+    device_into_matrix(L, 1, reuse, *res, A);
+  } else {
+    device_into_matrix(M, 0, reuse, *res, A);
   }
-  int64_t n = 0, nnz = 0;
-  ffhip_mat_info(M, &n, &nnz);
-  vector<int64_t> rp(n + 1);
-  vector<int> ia(n + 1), ja(nnz);
-  vector<double> aa(nnz);
-  hip_check(ffhip_mat_download(M, rp.data(), ja.data(), aa.data()), "ffhip: download");
-  for (int64_t i = 0; i <= n; ++i) ia[i] = (int)rp[i];
-  ffhip_mat_free(M);
-  ffhip_mesh_free(m);
-  const int half = sym ? 1 : 0;
-  MatriceMorse<double> *pH = new MatriceMorse<double>((int)n, (int)n, (int)nnz, half);
-  pH->set((int)n, (int)n, half, (size_t)nnz, ia.data(), ja.data(), aa.data(), 0, 1);
-  A.A.master(pH);
+  return true;
 }
 
 // "=" / "<-" for (matrix, varf(Vh, Vh)) on 2D P1 — preferred over
@@ -1017,17 +1156,23 @@ class OpHipMatrix2D : public OneOperator {
       if (init) A.init();
       if (!PUh || !PVh) return SetAny<Matrice_Creuse<double> *>(&A);
       Recognised r;
+      ResidentPtr res;
+      bool done = false;
       if (ds.tgv > 0 && recognise2(stack, *PUh, *PVh, b->largs, r)) {
-        A.A = 0;
-        A.Uh = *PUh;
-        A.Vh = *PVh;
-        assemble2_on_gpu(stack, *PUh, r, ds.tgv, ds.sym, A);
-      } else {
+        res = reusable(A, *PUh, *PVh, FORM_P1_2D, ds.sym);
+        if (!res) {
+          A.A = 0;
+          A.Uh = *PUh;
+          A.Vh = *PVh;
+        }
+        done = assemble2_on_gpu(stack, *PUh, r, ds.tgv, ds.sym, A, res);
+      }
+      if (!done) {
         creationBlockOfMatrixToBilinearForm<double, Mesh, FESpace, FESpace>(PUh, PVh, ds.sym, ds.tgv,
                                                                             b->largs, stack, A);
       }
       A.pHM()->half = ds.sym;
-      if (square) SetSolver(stack, VF, *A.A, ds);
+      if (square) set_solver_with(stack, VF, A, ds, res);
       return SetAny<Matrice_Creuse<double> *>(&A);
     }
   };
@@ -1142,29 +1287,17 @@ void eval_dirichlet(Stack stack, const FESpace3 &Vh, const RecognisedRhs &r, vec
 
 void rhs_on_gpu(Stack stack, const FESpace3 &Vh, const RecognisedRhs &r, double tgv, KN_<double> &b) {
   const Mesh3 &Th = Vh.Th;
-  vector<double> xyz(3 * (size_t)Th.nv);
-  vector<int32_t> tets(4 * (size_t)Th.nt), btri(3 * (size_t)Th.nbe), blab(Th.nbe);
-  for (int i = 0; i < Th.nv; ++i) {
-    xyz[3 * i] = Th(i).x; xyz[3 * i + 1] = Th(i).y; xyz[3 * i + 2] = Th(i).z;
-  }
-  for (int k = 0; k < Th.nt; ++k)
-    for (int j = 0; j < 4; ++j) tets[4 * k + j] = Th(Th[k][j]);
-  for (int k = 0; k < Th.nbe; ++k) {
-    for (int j = 0; j < 3; ++j) btri[3 * k + j] = Th(Th.be(k)[j]);
-    blab[k] = Th.be(k).lab;
-  }
   ffhip_ctx ctx = hip_ctx();
-  ffhip_mesh m = nullptr;
-  hip_check(ffhip_mesh3_upload(ctx, Th.nv, Th.nt, Th.nbe, xyz.data(), tets.data(), btri.data(),
-                               blab.data(), &m), "ffhip: mesh upload");
-  vector<double> fq, gv, out(Th.nv);
-  if (r.has_f && !r.f_const) eval_coef_qp(stack, Vh, r.f, fq);
+  ffhip_glue::Resident dev;  // the device mesh, freed on return
+  upload_mesh3(Th, dev);
+  ffhip_mesh m = dev.mesh;
+  vector<double> gv, out(Th.nv);
+  CoefArg f;
+  if (r.has_f) coef_arg3(stack, Vh, r.f_const, r.f0, r.f, f);
   if (!r.bcs.empty()) eval_dirichlet(stack, Vh, r, gv);
   vector<int32_t> dl(r.dlabels.begin(), r.dlabels.end());
-  const int kind = !r.has_f ? FFHIP_COEF_CONST : (r.f_const ? FFHIP_COEF_CONST : FFHIP_COEF_QP);
-  const double f0 = r.has_f ? r.f0 : 0.;
-  hip_check(ffhip_rhs_p1_form(ctx, m, kind, f0, fq.empty() ? nullptr : fq.data(), dl.data(),
-                              (int)dl.size(), tgv, 0., gv.empty() ? nullptr : gv.data(), out.data(), 0),
+  hip_check(ffhip_rhs_p1_form(ctx, m, f.kind, f.c0, f.ptr, dl.data(), (int)dl.size(), tgv, 0.,
+                              gv.empty() ? nullptr : gv.data(), out.data(), 0),
             "ffhip: linear form");
   for (const Recognised::Boundary &t : r.bnd) {
     vector<int32_t> labs;
@@ -1184,7 +1317,6 @@ void rhs_on_gpu(Stack stack, const FESpace3 &Vh, const RecognisedRhs &r, double 
                                     out.data(), 0),
               "ffhip: boundary linear form");
   }
-  ffhip_mesh_free(m);
   for (int i = 0; i < Th.nv; ++i) b[i] = out[i];
 }
 
@@ -1243,47 +1375,20 @@ bool recognise_rhs_p2(Stack stack, const FESpace3 &Vh, const list<C_F0> &largs, 
 // true when built on the GPU; false (b untouched) when FreeFEM's dof numbering
 // is not the device's first-appearance node numbering
 bool rhs_p2_on_gpu(const FESpace3 &Vh, const RecognisedRhsP2 &r, double tgv, KN_<double> &b) {
-  const Mesh3 &Th = Vh.Th;
-  vector<double> xyz(3 * (size_t)Th.nv);
-  vector<int32_t> tets(4 * (size_t)Th.nt), btri(3 * (size_t)Th.nbe), blab(Th.nbe);
-  for (int i = 0; i < Th.nv; ++i) {
-    xyz[3 * i] = Th(i).x; xyz[3 * i + 1] = Th(i).y; xyz[3 * i + 2] = Th(i).z;
-  }
-  for (int k = 0; k < Th.nt; ++k)
-    for (int j = 0; j < 4; ++j) tets[4 * k + j] = Th(Th[k][j]);
-  for (int k = 0; k < Th.nbe; ++k) {
-    for (int j = 0; j < 3; ++j) btri[3 * k + j] = Th(Th.be(k)[j]);
-    blab[k] = Th.be(k).lab;
-  }
   ffhip_ctx ctx = hip_ctx();
-  ffhip_mesh m = nullptr;
-  ffhip_p2v sp = nullptr;
-  hip_check(ffhip_mesh3_upload(ctx, Th.nv, Th.nt, Th.nbe, xyz.data(), tets.data(), btri.data(),
-                               blab.data(), &m), "ffhip: mesh upload");
-  hip_check(ffhip_p2v_create(ctx, m, &sp), "ffhip: P2 numbering");
-  vector<int32_t> en(10 * (size_t)Th.nt);
-  hip_check(ffhip_p2v_download(sp, en.data()), "ffhip: P2 numbering");
-  bool same = true;
-  const int nd = 10 * r.ncomp;
-  for (int k = 0; k < Th.nt && same; ++k) {
-    const FElement3 K(Vh[k]);
-    for (int df = 0; df < nd && same; ++df)  // local dofs comp-blocked (c*10 + i)
-      same = K(df) == r.ncomp * en[10 * (size_t)k + df % 10] + df / 10;
-  }
-  if (same) {
-    vector<int32_t> labs(r.labels.begin(), r.labels.end());
-    vector<double> out(b.N());
-    if (r.ncomp == 3)
-      hip_check(ffhip_rhs_p2v(ctx, sp, r.f, labs.data(), (int)labs.size(), tgv, r.g, out.data(), 0),
-                "ffhip: P2 linear form");
-    else
-      hip_check(ffhip_rhs_p2(ctx, sp, r.f[0], labs.data(), (int)labs.size(), tgv, r.g[0], out.data(), 0),
-                "ffhip: P2 linear form");
-    for (long i = 0; i < b.N(); ++i) b[i] = out[i];
-  }
-  ffhip_p2v_free(sp);
-  ffhip_mesh_free(m);
-  return same;
+  ffhip_glue::Resident dev;  // the device mesh and P2 numbering, freed on return
+  upload_mesh3(Vh.Th, dev);
+  if (!p2_numbering(Vh, r.ncomp, dev)) return false;
+  vector<int32_t> labs(r.labels.begin(), r.labels.end());
+  vector<double> out(b.N());
+  if (r.ncomp == 3)
+    hip_check(ffhip_rhs_p2v(ctx, dev.sp, r.f, labs.data(), (int)labs.size(), tgv, r.g, out.data(), 0),
+              "ffhip: P2 linear form");
+  else
+    hip_check(ffhip_rhs_p2(ctx, dev.sp, r.f[0], labs.data(), (int)labs.size(), tgv, r.g[0], out.data(), 0),
+              "ffhip: P2 linear form");
+  for (long i = 0; i < b.N(); ++i) b[i] = out[i];
+  return true;
 }
 
 // "=" / "<-" for (real[int], varf(0, Vh)) on 3D P1 — preferred over
@@ -1386,22 +1491,25 @@ class HipLinearCG : public OneOperator {
       HashMatrix<int, double> *H = MA.pHM();
       if (!H) ExecError("LinearCG: empty matrix");
       ffassert(H->n == H->m && H->n == x.N() && (!b || b->N() == x.N()));
-      int *ia = 0, *ja = 0;
-      double *a = 0;
-      H->CSR(ia, ja, a);
-      const int64_t n = H->n, nnz = ia[n];
-      vector<int64_t> rp(ia, ia + n + 1);
-      ffhip_mat M = nullptr;
-      ffhip_cg cg = nullptr;
-      hip_check(ffhip_mat_upload(hip_ctx(), n, nnz, rp.data(), ja, a, &M), "LinearCG: matrix upload");
-      int rc = ffhip_cg_create(hip_ctx(), M, FFHIP_CG_FP64, &cg);
-      if (rc != FFHIP_OK) { ffhip_mat_free(M); hip_check(rc, "LinearCG: solver setup"); }
+      // the device copy of A's ffhip solver when it has one (refreshed if the
+      // script changed A since), else a temporary upload (half matrices
+      // expanded to the full symmetric operator)
+      HipSolverBase *sb = H->vsolver ? dynamic_cast<HipSolverBase *>(H->vsolver) : nullptr;
+      std::unique_ptr<Core> tmp;
+      Core *core = nullptr;
+      if (sb) {
+        core = &sb->core;
+      } else {
+        tmp.reset(new Core(H, hip_ctx(), FFHIP_CG_FP64));
+        core = tmp.get();
+      }
+      core->note_changes(H->GetReDoNumerics(), H->GetReDoSymbolic());
+      hip_check(core->refresh(), "LinearCG: matrix to the GPU");
       double eps_out = eps;
       ffhip_cg_stats st;
-      rc = ffhip_cg_solve_affine(cg, b ? (double *)*b : nullptr, (double *)x, 0, eps, (int)nbitermax,
-                                 jacobi ? FFHIP_CG2_JACOBI : FFHIP_CG2_IDENTITY, &eps_out, &st);
-      ffhip_cg_free(cg);
-      ffhip_mat_free(M);
+      const int rc = ffhip_cg_solve_affine(core->cg, b ? (double *)*b : nullptr, (double *)x, 0, eps,
+                                           (int)nbitermax, jacobi ? FFHIP_CG2_JACOBI : FFHIP_CG2_IDENTITY,
+                                           &eps_out, &st);
       hip_check(rc, "CG2: Matrix is not defined (/0) or ro is NaN");
       if (veps) *veps = -eps_out;
       if (verb > 1 || (verb && st.converged != 1))

@@ -3,7 +3,10 @@
 # (HashMatrix, CG, QuadratureFormular, and the Mesh3 / FESpace3 / P1-P2
 # element sources), libMesh's libmeshb7.c (used by Mesh3dn.cpp) and the
 # no-graphics host globals of src/Graphics/gggg.cpp, compiled where they lie
-# under the reference tree, plus the test driver ref_driver.cpp.
+# under the reference tree, plus the test drivers ref_driver.cpp and
+# seam_driver.cpp (the plugin's glue, freefem-sources_amd/plugin/
+# ffhip_glue.hpp, run against these objects; it links libffhip.so, which must
+# be built first).
 # -DCMAKE is the reference's own switch that skips the autoconf config.h
 # (src/Graphics/rgraph.hpp:55-57).  Flags follow the reference build:
 # g++ -O3 -mavx2 -std=gnu++14 (no FMA contraction).
@@ -25,7 +28,12 @@ gcc -O2 -fPIC -w -I"$REF/src/libMesh" -c "$REF/src/libMesh/libmeshb7.c" -o "$OUT
 wait
 g++ $CXXFLAGS $INC -c "$REF/src/Graphics/gggg.cpp" -o "$OUT/gggg.o"
 g++ $CXXFLAGS $INC -c "$HERE/ref_driver.cpp" -o "$OUT/ref_driver.o"
+PKG="$HERE/../freefem-sources_amd"
+[ -f "$PKG/libffhip.so" ] || { echo "build libffhip.so first (make -C freefem-sources_amd)" >&2; exit 4; }
+g++ $CXXFLAGS $INC -I"$HERE/../include" -I"$PKG/plugin" -c "$HERE/seam_driver.cpp" -o "$OUT/seam_driver.o"
 OBJS=""
 for f in $FEMLIB; do OBJS="$OBJS $OUT/$f.o"; done
-g++ -shared -o "$OUT/libffref_ref.so" "$OUT/ref_driver.o" $OBJS "$OUT/gggg.o" "$OUT/libmeshb7.o"
+g++ -shared -o "$OUT/libffref_ref.so" "$OUT/ref_driver.o" "$OUT/seam_driver.o" $OBJS "$OUT/gggg.o" \
+  "$OUT/libmeshb7.o" -L"$PKG" -lffhip -Wl,-rpath,'$ORIGIN/../../freefem-sources_amd'
+
 echo "built $OUT/libffref_ref.so"

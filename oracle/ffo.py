@@ -26,10 +26,11 @@ COEF_CONST, COEF_SIN3, COEF_P0, COEF_QP = 0, 1, 2, 3
 
 
 def lib():
-    """Load libffo.so (build it with `make -C oracle`)."""
+    """Load libffo.so (build it with `make -C oracle`); FFO_LIB names another
+    build of the same sources (the sanitized oracle/libffo_asan.so)."""
     global _LIB
     if _LIB is None:
-        path = os.path.join(HERE, "libffo.so")
+        path = os.environ.get("FFO_LIB") or os.path.join(HERE, "libffo.so")
         L = ctypes.CDLL(path)
         sig = {
             "ffo_cube_counts": (None, [_i, _i, _i, _i64p]),
@@ -79,6 +80,7 @@ def lib():
             "ffo_fgmres": (_i, [_i, _i64p, _i32p, _f64p, _f64p, _f64p, _d, _i, _i, _d, _f64p]),
             "ffo_qf_gl3": (_i, [_f64p, _f64p]),
             "ffo_rhs_p1_coef": (_i, [_i, _i, _l, _f64p, _i32p, _i, _d, _vp, _f64p]),
+            "ffo_p1_at_qp": (_i, [_i, _l, _i32p, _f64p, _f64p]),
             "ffo_p1_boundary": (_i, [_i, _i, _l, _f64p, _i32p, _l, _i32p, _i32p, _i, _i32p, _i, _d,
                                      _vp, _vp, _vp, _vp, _vp]),
         }
@@ -95,9 +97,17 @@ def ref_available():
 
 
 def ref():
-    """Load the compiled reference subset (built by oracle/build_ref.sh)."""
+    """Load the compiled reference subset (built by oracle/build_ref.sh).
+
+    The library also holds the plugin-glue driver, which links libffhip: as
+    for the ffhip package, torch (when importable) is imported first so the
+    process has one HIP runtime, the one torch's wheel ships."""
     global _REF
     if _REF is None:
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(os.path.join(HERE, "_ref", "libffref_ref.so"))
         sig = {
             "ref_qf_tet5": (_i, [_f64p, _f64p]),
@@ -121,6 +131,14 @@ def ref():
             "ref_hm_read": (_l, [ctypes.c_char_p, _l, _i64p, _vp, _vp, _vp]),
             "ref_hm_coo": (_l, [_vp, _i64p, _i64p, _f64p]),
             "ref_hm_from_csr_half": (_vp, [_i, _i64p, _i32p, _f64p]),
+            # the plugin's glue run against the compiled objects (seam_driver.cpp)
+            "seam_mesh3_roundtrip": (_i, [_i, _f64p, _l, _i32p, _l, _i32p, _i32p, _f64p, _i32p,
+                                          _i32p, _i32p]),
+            "seam_hm_from_csr": (_vp, [_i, _i, _i64p, _i32p, _f64p]),
+            "seam_hm_state": (_i, [_vp, _i32p]),
+            "seam_core_solve": (_i, [_vp, _i, _f64p, _f64p, _d, _i, _d, _i, _f64p]),
+            "seam_resident_flow": (_vp, [_i, _i, _d, _d, _f64p, _f64p, _f64p, _f64p, _f64p, _f64p,
+                                         _f64p, _f64p, ctypes.POINTER(_i)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -218,6 +236,18 @@ def rhs_p1_3d(m, f=1.0):
     b = np.zeros(m["nv"])
     lib().ffo_rhs_p1_3d(m["nv"], m["nt"], m["xyz"], m["tets"], f, b)
     return b
+
+
+def p1_at_qp(m, u):
+    """a P1 function (vertex values u) at the Tet_5 / T_5 points of every
+    element, FreeFEM's evaluation: (nt * nq) values, element-major"""
+    dim = 3 if "tets" in m else 2
+    nq = 14 if dim == 3 else 7
+    out = np.zeros(m["nt"] * nq)
+    rc = lib().ffo_p1_at_qp(dim, m["nt"], m["tets"] if dim == 3 else m["tris"],
+                            np.ascontiguousarray(u, np.float64), out)
+    assert rc == nq, rc
+    return out
 
 
 def rhs_p1_coef(m, kind, f0=0.0, fcoef=None):

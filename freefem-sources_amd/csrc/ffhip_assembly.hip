@@ -27,6 +27,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "ffhip_fem.hpp"
@@ -1032,6 +1033,97 @@ int read_flag(ffhip_ctx ctx, int *d) {
   return h;
 }
 
+// ---------------------------------------------------------------- misfit
+// J = 1/2 int (u - ud)^2 (partials of kMisfitBlocks blocks, summed on the host
+// in block order) and r = M (u - ud) with the Dirichlet rows zeroed, on the
+// given buffers: part (kMisfitBlocks doubles, when J), onbc (nv bytes) and
+// labs (nlab int32, when r and nlab > 0)
+constexpr int kMisfitBlocks = 1024;
+struct MisfitBuffers {
+  double *part = nullptr;
+  uint8_t *onbc = nullptr;
+  int32_t *labs = nullptr;
+};
+
+__global__ void k_fill_u8(int64_t n, uint8_t *__restrict__ a, uint8_t v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+void misfit_run(ffhip_ctx ctx, ffhip_mesh m, const double *u, const double *ud,
+                const int32_t *labels, int nlab, double *J, double *r, const MisfitBuffers &b) {
+  hipStream_t s = ctx->stream;
+  ctx->tic();
+  if (J) {
+    k_p1_misfit_elem<<<kMisfitBlocks, 256, 0, s>>>(m->nt, m->xyz, m->elems, u, ud, b.part);
+    FFHIP_CHECK(hipGetLastError());
+    double h[kMisfitBlocks];
+    FFHIP_CHECK(hipMemcpyAsync(h, b.part, sizeof(h), hipMemcpyDeviceToHost, s));
+    FFHIP_CHECK(hipStreamSynchronize(s));
+    double tot = 0.;
+    for (int i = 0; i < kMisfitBlocks; ++i) tot += h[i];
+    *J = tot;
+  }
+  if (r) {
+    const uint8_t *onbc = nullptr;
+    if (nlab > 0) {
+      FFHIP_CHECK(hipMemcpyAsync(b.labs, labels, 4 * nlab, hipMemcpyHostToDevice, s));
+      k_fill_u8<<<nblocks(m->nv, 256), 256, 0, s>>>(m->nv, b.onbc, 0);
+      if (m->nbe)
+        k_mark_bc<<<nblocks(m->nbe, 256), 256, 0, s>>>(m->nbe, m->nbloc, m->belems, m->blab, b.labs,
+                                                       nlab, b.onbc);
+      onbc = b.onbc;
+    }
+    k_p1_mass_apply<<<nblocks(m->nv, 256), 256, 0, s>>>(m->nv, m->xyz, m->elems, m->v2e_ptr, m->v2e,
+                                                        u, ud, onbc, r);
+    FFHIP_CHECK(hipGetLastError());
+  }
+  ctx->toc();
+}
+
+// ---------------------------------------------------------------- no-allocation scratch
+// The scratch of ffhip_p1_misfit_noalloc: a file-scope device array, one copy
+// per device, reached through the address hipGetSymbolAddress gives on that
+// device (the array's name is a host-side stand-in in host code).  Layout:
+// misfit partials | labels | Dirichlet marks.  Every call on a device shares
+// it: overlapping calls on one device race.
+constexpr int64_t kNoallocMaxVertices = (int64_t)1 << 24;
+constexpr int kNoallocMaxLabels = 64;
+constexpr size_t kScratchPartOff = 0;
+constexpr size_t kScratchLabelsOff = 8 * kMisfitBlocks;
+constexpr size_t kScratchOnbcOff = kScratchLabelsOff + 4 * kNoallocMaxLabels;
+constexpr size_t kScratchBytes = kScratchOnbcOff + (size_t)kNoallocMaxVertices;
+constexpr int kMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+std::mutex &scratch_mutex() {
+  static std::mutex mu;
+  return mu;
+}
+unsigned char *&scratch_ptr(int device) {
+  static unsigned char *ptr[kMaxDevices] = {};
+  return ptr[device];
+}
+
+// resolves the device address of the scratch array on `device`, which must be
+// the current device (called by ffhip_init, once per device)
+void noalloc_scratch_setup(int device) {
+  FFHIP_REQUIRE(device >= 0 && device < kMaxDevices, FFHIP_ERR_UNSUPPORTED, "device %d", device);
+  std::lock_guard<std::mutex> g(scratch_mutex());
+  if (scratch_ptr(device)) return;
+  void *p = nullptr;
+  FFHIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  scratch_ptr(device) = static_cast<unsigned char *>(p);
+}
+
+unsigned char *noalloc_scratch(int device) {
+  std::lock_guard<std::mutex> g(scratch_mutex());
+  unsigned char *p = (device >= 0 && device < kMaxDevices) ? scratch_ptr(device) : nullptr;
+  FFHIP_REQUIRE(p, FFHIP_ERR_INTERNAL, "no-allocation scratch not set up on device %d", device);
+  return p;
+}
+
 }  // namespace ffhip
 
 using namespace ffhip;
@@ -1318,42 +1410,45 @@ int ffhip_p1_misfit(ffhip_ctx ctx, ffhip_mesh m, const double *u, const double *
                     const int32_t *labels, int nlab, double *J, double *r) {
   return guard([&] {
     FFHIP_REQUIRE(ctx && m && u && ud && m->dim == 3, FFHIP_ERR_ARG, "ffhip_p1_misfit: bad arguments");
-    hipStream_t s = ctx->stream;
+    FFHIP_REQUIRE(nlab >= 0 && (nlab == 0 || labels), FFHIP_ERR_ARG, "ffhip_p1_misfit: labels");
     build_v2e(m);
-    ctx->tic();
-    if (J) {
-      const int GB = 1024;
-      double *part = dalloc<double>(GB);
-      k_p1_misfit_elem<<<GB, 256, 0, s>>>(m->nt, m->xyz, m->elems, u, ud, part);
-      FFHIP_CHECK(hipGetLastError());
-      std::vector<double> h(GB);
-      FFHIP_CHECK(hipMemcpyAsync(h.data(), part, 8 * GB, hipMemcpyDeviceToHost, s));
-      FFHIP_CHECK(hipStreamSynchronize(s));
-      double tot = 0.;
-      for (int i = 0; i < GB; ++i) tot += h[i];
-      *J = tot;
-      dfree(part);
+    MisfitBuffers b;
+    b.part = J ? dalloc<double>(kMisfitBlocks) : nullptr;
+    if (r && nlab > 0) {
+      b.onbc = dalloc<uint8_t>(m->nv);
+      b.labs = dalloc<int32_t>(nlab);
     }
-    if (r) {
-      uint8_t *onbc = nullptr;
-      int32_t *labs = nullptr;
-      if (nlab > 0) {
-        onbc = dalloc<uint8_t>(m->nv);
-        labs = dalloc<int32_t>(nlab);
-        FFHIP_CHECK(hipMemcpyAsync(labs, labels, 4 * nlab, hipMemcpyHostToDevice, s));
-        FFHIP_CHECK(hipMemsetAsync(onbc, 0, m->nv, s));
-        if (m->nbe)
-          k_mark_bc<<<nblocks(m->nbe, 256), 256, 0, s>>>(m->nbe, m->nbloc, m->belems, m->blab, labs,
-                                                         nlab, onbc);
-      }
-      k_p1_mass_apply<<<nblocks(m->nv, 256), 256, 0, s>>>(m->nv, m->xyz, m->elems, m->v2e_ptr, m->v2e,
-                                                          u, ud, onbc, r);
-      FFHIP_CHECK(hipGetLastError());
-      FFHIP_CHECK(hipStreamSynchronize(s));
-      dfree(onbc);
-      dfree(labs);
+    try {
+      misfit_run(ctx, m, u, ud, labels, nlab, J, r, b);
+    } catch (...) {
+      (void)hipStreamSynchronize(ctx->stream);
+      dfree(b.part); dfree(b.onbc); dfree(b.labs);
+      throw;
     }
-    ctx->toc();
+    dfree(b.part); dfree(b.onbc); dfree(b.labs);
+  });
+}
+
+int ffhip_p1_misfit_noalloc(ffhip_ctx ctx, ffhip_mesh m, const double *u, const double *ud,
+                            const int32_t *labels, int nlab, double *J, double *r) {
+  return guard([&] {
+    FFHIP_REQUIRE(ctx && m && u && ud && m->dim == 3, FFHIP_ERR_ARG,
+                  "ffhip_p1_misfit_noalloc: bad arguments");
+    FFHIP_REQUIRE(nlab >= 0 && (nlab == 0 || labels), FFHIP_ERR_ARG, "ffhip_p1_misfit_noalloc: labels");
+    FFHIP_REQUIRE(m->nv <= kNoallocMaxVertices, FFHIP_ERR_ARG,
+                  "ffhip_p1_misfit_noalloc: %lld vertices, more than its maximum %lld",
+                  (long long)m->nv, (long long)kNoallocMaxVertices);
+    FFHIP_REQUIRE(nlab <= kNoallocMaxLabels, FFHIP_ERR_ARG,
+                  "ffhip_p1_misfit_noalloc: %d labels, more than its maximum %d", nlab, kNoallocMaxLabels);
+    FFHIP_REQUIRE(!r || m->v2e_ptr, FFHIP_ERR_ARG,
+                  "ffhip_p1_misfit_noalloc: the mesh's vertex incidence is not built yet (an assembly "
+                  "or ffhip_p1_misfit on this mesh builds it; this entry point allocates nothing)");
+    unsigned char *base = noalloc_scratch(ctx->device);
+    MisfitBuffers b;
+    b.part = reinterpret_cast<double *>(base + kScratchPartOff);
+    b.labs = reinterpret_cast<int32_t *>(base + kScratchLabelsOff);
+    b.onbc = base + kScratchOnbcOff;
+    misfit_run(ctx, m, u, ud, labels, nlab, J, r, b);
   });
 }
 

@@ -329,6 +329,7 @@ int ffhip_init(int device, ffhip_ctx *out) {
     FFHIP_REQUIRE(device >= 0 && device < ndev, FFHIP_ERR_ARG,
                   "ffhip_init: device %d of %d", device, ndev);
     FFHIP_CHECK(hipSetDevice(device));
+    noalloc_scratch_setup(device);
     ffhip_ctx c = new ffhip_ctx_s();
     c->device = device;
     // a blocking stream: ordered with the legacy default stream, which is

@@ -222,4 +222,8 @@ struct Arena {
 };
 int64_t read_i64(ffhip_ctx ctx, const int64_t *p);
 
+// the per-device address of the no-allocation entry points' scratch array
+// (ffhip_assembly.hip); set up by ffhip_init with the device current
+void noalloc_scratch_setup(int device);
+
 }  // namespace ffhip

@@ -111,6 +111,7 @@ _SIGS = {
     "ffhip_last_ms": (_d, [_vp]),
     "ffhip_bench_copy": (_i, [_vp, _l, _i, ctypes.POINTER(_d)]),
     "ffhip_p1_misfit": (_i, [_vp, _vp, _vp, _vp, _vp, _i, ctypes.POINTER(_d), _vp]),
+    "ffhip_p1_misfit_noalloc": (_i, [_vp, _vp, _vp, _vp, _vp, _i, ctypes.POINTER(_d), _vp]),
     "ffhip_p1_grad_kappa": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "ffhip_p2v_create": (_i, [_vp, _vp, _pvp]),
     "ffhip_p2v_info": (_i, [_vp, _pl, _pl]),

@@ -237,6 +237,18 @@ int ffhip_mat_read_text(ffhip_ctx ctx, const char *path, ffhip_mat *mat);
 int ffhip_p1_misfit(ffhip_ctx ctx, ffhip_mesh mesh, const double *u,
                     const double *ud, const int32_t *labels, int nlab,
                     double *J, double *r);
+/* ffhip_p1_misfit without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Its
+ * temporaries (the J partials, the label list, the Dirichlet marks) live in a
+ * fixed device array of the library, one per device, of 16.8 MB: meshes of
+ * at most 2^24 vertices, at most 64 labels (FFHIP_ERR_ARG beyond).  The
+ * mesh's vertex incidence must already exist when r is wanted (any assembly
+ * or ffhip_p1_misfit call on the mesh builds it).  Calls that overlap on one
+ * device share that array and race; the results equal ffhip_p1_misfit's. */
+int ffhip_p1_misfit_noalloc(ffhip_ctx ctx, ffhip_mesh mesh, const double *u,
+                            const double *ud, const int32_t *labels, int nlab,
+                            double *J, double *r);
 /* g[e] = dJ/dkappa_e = - int_K grad u . grad lambda  (= -|K| grad u_K . grad
  * lambda_K for P1, exact), the P0 linear form int3d(Th)(-(grad u.grad l)*w)
  * of the reference's composed varfs.  Gradients are recomputed from the

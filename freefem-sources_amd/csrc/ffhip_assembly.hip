@@ -246,16 +246,39 @@ constexpr int kRowBuf = 64;
 
 constexpr int kHash = 128;  // LDS hash slots per row (a row has at most 1 + 3 * 32 columns)
 
-template <int NLOC>
+// Where the pass writes a row it builds.  Buffered: the 64-entry row
+// buffers (rowbuf, rowval), the length and the diagonal offset, moved to CSR
+// by k_pattern_compact after the scan of the lengths.  DIRECT: the row
+// pointers are known beforehand (the vertex-link counts, k_link_len) and the
+// row goes straight to colidx / vals / diag; a row whose length differs from
+// its pointers sets *mismatch and is not written (the caller then takes the
+// buffered pass).
+struct PatternOut {
+  int64_t *rowlen = nullptr;
+  int32_t *rowbuf = nullptr;
+  int8_t *dposr = nullptr;
+  double *rowval = nullptr;
+  const int64_t *rowptr = nullptr;
+  int32_t *colidx = nullptr;
+  int64_t *diag = nullptr;
+  double *vals = nullptr;
+  int *mismatch = nullptr;
+};
+
+// VALS (3D P1 stiffness): the same pass also forms the row's values from the
+// element scales Ce (the rows of k_p1_values_rows).
+template <int NLOC, bool VALS, bool DIRECT>
 __global__ __launch_bounds__(256) void k_pattern_half(
     int64_t n, const int32_t *__restrict__ elems, const int64_t *__restrict__ v2e_ptr,
-    const int32_t *__restrict__ v2e, int64_t *__restrict__ rowlen, int32_t *__restrict__ rowbuf,
-    int8_t *__restrict__ dposr, uint32_t *__restrict__ slotmap, uint8_t *__restrict__ big,
-    int *__restrict__ any_big) {
+    const int32_t *__restrict__ v2e, uint32_t *__restrict__ slotmap, uint8_t *__restrict__ big,
+    int *__restrict__ any_big, PatternOut out, const double *__restrict__ xyz = nullptr,
+    const double *__restrict__ Ce = nullptr) {
   constexpr int NH = 256 / kHalf;
   __shared__ int32_t hk[NH][kHash];      // open-addressing set of the row's columns
   __shared__ int32_t ucol[NH][kRowBuf];  // unique columns, then sorted
   __shared__ int32_t ucnt[NH];
+  __shared__ double sv[VALS ? NH : 1][VALS ? 4 * kHalf : 1];   // incidence rows (VALS)
+  __shared__ uint32_t so[VALS ? NH : 1][VALS ? kHalf : 1];      // their slot bytes (VALS)
   const int hl = threadIdx.x % kHalf, hid = threadIdx.x / kHalf;
   const int64_t v = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kHalf;
   int64_t k0 = 0;
@@ -266,8 +289,12 @@ __global__ __launch_bounds__(256) void k_pattern_half(
   for (int i = hl; i < kHash; i += kHalf) hk[hid][i] = INT_MAX;
   if (hl == 0) ucnt[hid] = 0;
   int32_t c[4] = {INT_MAX, INT_MAX, INT_MAX, INT_MAX};
+  int64_t e = 0;
+  int lv = 0;
   if (hl < deg) {
-    const int64_t e = v2e[k0 + hl] / NLOC;
+    const int32_t idx = v2e[k0 + hl];
+    e = idx / NLOC;
+    lv = idx - (int)e * NLOC;
     if (NLOC == 4) {
       const int4 t = reinterpret_cast<const int4 *>(elems)[e];
       c[0] = t.x; c[1] = t.y; c[2] = t.z; c[3] = t.w;
@@ -298,6 +325,12 @@ __global__ __launch_bounds__(256) void k_pattern_half(
   __syncthreads();
   const int len = ucnt[hid];
   if (len > kRowBuf) mine = false;
+  int64_t r0 = 0;
+  bool wrong = false;  // DIRECT: the row's length is not its pointers'
+  if (DIRECT && mine) {
+    r0 = out.rowptr[v];
+    wrong = out.rowptr[v + 1] - r0 != len;
+  }
   // counting sort: the rank of a column is the number of smaller columns
   int32_t u[2] = {INT_MAX, INT_MAX};
   int rk[2] = {0, 0};
@@ -320,7 +353,8 @@ __global__ __launch_bounds__(256) void k_pattern_half(
     for (int h2 = 0; h2 < 2; ++h2)
       if (hl + h2 * kHalf < len) {
         ucol[hid][rk[h2]] = u[h2];
-        rowbuf[kRowBuf * v + rk[h2]] = u[h2];
+        if (!DIRECT) out.rowbuf[kRowBuf * v + rk[h2]] = u[h2];
+        else if (!wrong) out.colidx[r0 + rk[h2]] = u[h2];
         if ((int64_t)u[h2] == v) dpos = rk[h2];
       }
   }
@@ -333,7 +367,9 @@ __global__ __launch_bounds__(256) void k_pattern_half(
   if (v < n && hl == 0) {
     big[v] = mine ? 0 : 1;
     if (!mine) atomicOr(any_big, 1);
-    if (mine) { rowlen[v] = len; dposr[v] = (int8_t)dpos; }
+    if (DIRECT && wrong) atomicOr(out.mismatch, 1);
+    if (mine && !DIRECT) { out.rowlen[v] = len; out.dposr[v] = (int8_t)dpos; }
+    if (mine && DIRECT && !wrong) out.diag[v] = dpos < 0 ? -1 : r0 + dpos;
   }
   if (mine && hl < deg) {
     uint32_t pk = 0;
@@ -347,20 +383,95 @@ __global__ __launch_bounds__(256) void k_pattern_half(
       pk |= (uint32_t)lo << (8 * m);
     }
     slotmap[k0 + hl] = pk;
+    if (VALS) {
+      double vv[4][3], r[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) vv[i][k] = xyz[3 * (int64_t)c[i] + k];
+      p1_row_entries(vv, lv, Ce[e], r);
+#pragma unroll
+      for (int m = 0; m < 4; ++m) sv[hid][4 * hl + m] = r[m];
+      so[hid][hl] = pk;
+    }
   }
+  if (VALS) {  // the row's values, each slot summed by its lane in element order
+    __syncthreads();
+    const int nq = mine ? deg : 0;
+    const uint32_t me0 = (uint32_t)hl * 0x01010101u, me1 = (uint32_t)(hl + kHalf) * 0x01010101u;
+    double acc0 = 0., acc1 = 0.;
+    for (int q = 0; q < nq; ++q) {
+      const uint32_t pk = so[hid][q];
+      const uint32_t t0 = pk ^ me0, z0 = (t0 - 0x01010101u) & ~t0 & 0x80808080u;
+      if (z0) acc0 += sv[hid][4 * q + (__builtin_ctz(z0) >> 3)];
+      if (len > kHalf) {
+        const uint32_t t1 = pk ^ me1, z1 = (t1 - 0x01010101u) & ~t1 & 0x80808080u;
+        if (z1) acc1 += sv[hid][4 * q + (__builtin_ctz(z1) >> 3)];
+      }
+    }
+    if (mine && !DIRECT) {
+      if (hl < len) out.rowval[kRowBuf * v + hl] = acc0;
+      if (hl + kHalf < len) out.rowval[kRowBuf * v + hl + kHalf] = acc1;
+    }
+    if (mine && DIRECT && !wrong) {
+      if (hl < len) out.vals[r0 + hl] = acc0;
+      if (hl + kHalf < len) out.vals[r0 + hl + kHalf] = acc1;
+    }
+  }
+}
+
+// The P1 row lengths a manifold simplicial mesh implies, from the counts of
+// incident elements (deg) and boundary elements (b) of each vertex: the link
+// of a vertex is a sphere (3D) / circle (2D) inside and a disk / path on the
+// boundary, so Euler's formula gives its vertex count — 3D: 2 + deg/2 inside,
+// 1 + (deg + b)/2 on the boundary; 2D: deg inside, deg + 1 on the boundary —
+// and the row holds those vertices and its own.  Meshes where this does not
+// hold (boundary elements inside the domain, non-manifold vertices) are found
+// by the pattern pass, which compares every row with these lengths.
+__global__ void k_bcount(int64_t m, const int32_t *__restrict__ belems, uint32_t *__restrict__ cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) atomicAdd(&cnt[belems[i]], 1u);
+}
+
+__global__ void k_link_len(int64_t n, int dim, const int64_t *__restrict__ v2e_ptr,
+                           const uint32_t *__restrict__ bcnt, int64_t *__restrict__ len) {
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= n) return;
+  const int64_t deg = v2e_ptr[v + 1] - v2e_ptr[v], b = bcnt[v];
+  int64_t l;
+  if (dim == 3) l = b == 0 ? 3 + deg / 2 : 2 + (deg + b) / 2;
+  else l = b == 0 ? 1 + deg : 2 + deg;
+  len[v] = deg == 0 ? 0 : l;
 }
 
 // buffered columns to their CSR place; diagonal slots.  One lane per row:
 // the row's buffer is read as 16-byte groups, all issued before the stores,
 // so a wave keeps 64 rows' loads in flight.
+// With rowval, the row values too (2 doubles per 16-byte load).
 __global__ __launch_bounds__(256) void k_pattern_compact(
     int64_t n, const int64_t *__restrict__ rowptr, const int32_t *__restrict__ rowbuf,
     const int8_t *__restrict__ dposr, const uint8_t *__restrict__ big,
-    int32_t *__restrict__ colidx, int64_t *__restrict__ diag) {
+    int32_t *__restrict__ colidx, int64_t *__restrict__ diag,
+    const double *__restrict__ rowval = nullptr, double *__restrict__ vals = nullptr) {
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= n || big[v]) return;
   const int64_t r0 = rowptr[v];
   const int len = (int)(rowptr[v + 1] - r0);
+  if (rowval) {
+    const double2 *vs = reinterpret_cast<const double2 *>(rowval + kRowBuf * v);
+    double *vd = vals + r0;
+    for (int g0 = 0; g0 < len; g0 += 8) {  // 8 values (4 loads) per round
+      double2 q[4];
+#pragma unroll
+      for (int g = 0; g < 4; ++g)
+        if (g0 + 2 * g < len) q[g] = vs[(g0 >> 1) + g];
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        if (g0 + 2 * g < len) vd[g0 + 2 * g] = q[g].x;
+        if (g0 + 2 * g + 1 < len) vd[g0 + 2 * g + 1] = q[g].y;
+      }
+    }
+  }
   const int4 *src = reinterpret_cast<const int4 *>(rowbuf + kRowBuf * v);
   int32_t *dst = colidx + r0;
   for (int g0 = 0; g0 < len; g0 += 16) {  // 16 columns (4 groups) per round
@@ -688,6 +799,88 @@ __global__ __launch_bounds__(256) void k_p1_values_half(
   if (hl + kHalf < len) vals[r0 + hl + kHalf] = acc1;
 }
 
+// ---------------------------------------------------------------- 3D stiffness in two passes
+// K_e(l, m) = S_e grad l_l . grad l_m (problem.cpp:6387-6392) with
+// grad l_l = N_l / (6 |K|), N_l the cross products of the edge vectors from
+// vertex 0 (Tet::Gradlambda, Mesh3dn.hpp:126-137, before its division) and
+// N_0 = -(N_1 + N_2 + N_3): the element pass stores the one scale
+// C_e = S_e / (6 |K|)^2 per element, and the row pass forms the row's entries
+// C_e N_l . N_m from the vertices — no element rows through HBM.
+__global__ __launch_bounds__(256) void k_p1_scale_3d(int64_t nt, const double *__restrict__ xyz,
+                                                     const int32_t *__restrict__ tets, int kind,
+                                                     double c0, const double *__restrict__ kappa_e,
+                                                     double *__restrict__ Ce) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nt) return;
+  double v[4][3];
+  int32_t id[4];
+  load_tet(xyz, tets, e, id, v);
+  const double mes = tet_mesure(v);
+  const double S = p1_kappa_integral(e, id, v, mes, kind, c0, kappa_e);
+  const double d1 = 1. / (6. * mes);
+  Ce[e] = (S * d1) * d1;
+}
+
+// values, one row per 32-lane half-wave (rows of up to 32 elements and 64
+// columns; big[] rows are left to the lane-per-row gather): lane q takes
+// incidence q (element order), forms its element's row l from the vertices
+// and C_e, and stages the 4 entries with their slot bytes (the element-to-row
+// map of the pattern) in LDS; lane j then adds, in element order, the entries
+// whose slot is j (lanes own slots j and j + 32) — the summation order of
+// HashMatrix::operator+= over the element loop.  Blocks take contiguous row
+// ranges per XCD (the gathered vertices and element scales of neighbouring
+// rows stay in that XCD's L2).
+__global__ __launch_bounds__(256) void k_p1_values_rows(
+    int64_t n, const int64_t *__restrict__ rowptr, const int64_t *__restrict__ v2e_ptr,
+    const int32_t *__restrict__ v2e, const uint32_t *__restrict__ slotmap,
+    const double *__restrict__ xyz, const int32_t *__restrict__ tets, const double *__restrict__ Ce,
+    const uint8_t *__restrict__ big, double *__restrict__ vals) {
+  __shared__ double sv[256 / kHalf][4 * kHalf];
+  __shared__ uint32_t so[256 / kHalf][kHalf];
+  const int hl = threadIdx.x % kHalf, hid = threadIdx.x / kHalf;
+  const int64_t v = ((int64_t)xcd_block_id() * blockDim.x + threadIdx.x) / kHalf;
+  int64_t k0 = 0, r0 = 0;
+  int deg = 0, len = 0;
+  if (v < n && !big[v]) {
+    k0 = v2e_ptr[v];
+    deg = (int)(v2e_ptr[v + 1] - k0);
+    r0 = rowptr[v];
+    len = (int)(rowptr[v + 1] - r0);
+  }
+  if (hl < deg) {
+    const int32_t idx = v2e[k0 + hl];
+    const int64_t e = idx >> 2;
+    const int l = idx & 3;
+    const uint32_t pk = slotmap[k0 + hl];
+    const int4 tv = reinterpret_cast<const int4 *>(tets)[e];
+    const int32_t id[4] = {tv.x, tv.y, tv.z, tv.w};
+    const double C = Ce[e];
+    double vv[4][3], r[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) vv[i][k] = xyz[3 * (int64_t)id[i] + k];
+    p1_row_entries(vv, l, C, r);
+#pragma unroll
+    for (int m = 0; m < 4; ++m) sv[hid][4 * hl + m] = r[m];
+    so[hid][hl] = pk;
+  }
+  __syncthreads();
+  const uint32_t me0 = (uint32_t)hl * 0x01010101u, me1 = (uint32_t)(hl + kHalf) * 0x01010101u;
+  double acc0 = 0., acc1 = 0.;
+  for (int q = 0; q < deg; ++q) {
+    const uint32_t pk = so[hid][q];
+    const uint32_t t0 = pk ^ me0, z0 = (t0 - 0x01010101u) & ~t0 & 0x80808080u;
+    if (z0) acc0 += sv[hid][4 * q + (__builtin_ctz(z0) >> 3)];
+    if (len > kHalf) {
+      const uint32_t t1 = pk ^ me1, z1 = (t1 - 0x01010101u) & ~t1 & 0x80808080u;
+      if (z1) acc1 += sv[hid][4 * q + (__builtin_ctz(z1) >> 3)];
+    }
+  }
+  if (hl < len) vals[r0 + hl] = acc0;
+  if (hl + kHalf < len) vals[r0 + hl + kHalf] = acc1;
+}
+
 // ---------------------------------------------------------------- Dirichlet
 __global__ void k_mark_bc(int64_t nbe, int nbloc, const int32_t *__restrict__ belems,
                           const int32_t *__restrict__ blab, const int32_t *__restrict__ labs,
@@ -1003,7 +1196,18 @@ void build_incidence(ffhip_ctx ctx, const int32_t *elems, int64_t nt, int nloc, 
 void build_v2e(ffhip_mesh m) {
   if (m->v2e_ptr) return;
   build_incidence(m->ctx, m->elems, m->nt, m->nloc, m->nv, &m->v2e_ptr, &m->v2e, &m->v2e_big,
-                  &m->v2e_inv);
+                  nullptr);
+}
+
+// the position of every (element, local vertex) item in the incidence lists
+// (built on first use: only the element-row assembly path scatters by it)
+void build_v2e_inv(ffhip_mesh m) {
+  build_v2e(m);
+  if (m->v2e_inv) return;
+  const int64_t M = m->nt * m->nloc;
+  m->v2e_inv = dalloc<int32_t>(M);
+  k_v2e_inverse<<<nblocks(M, 256), 256, 0, m->ctx->stream>>>(M, m->v2e, m->v2e_inv);
+  FFHIP_CHECK(hipGetLastError());
 }
 
 // a_ii = tgv on the marked rows (HashMatrix::SetBC); *missing set when a
@@ -1124,6 +1328,148 @@ unsigned char *noalloc_scratch(int device) {
   return p;
 }
 
+// The P1 pattern (ffhip_pattern_p1).  With Ce (3D: the element scales of
+// k_p1_scale_3d, ready when the event ce_ready has passed) the half-wave
+// pattern pass also forms the values of the rows it builds; *values_done
+// tells whether that covered every row (no lane-per-row rows).
+//
+// First try: the row pointers from the vertex-link counts (k_link_len), the
+// pattern pass writing straight to CSR.  If some row is not what they
+// predict, or needs the lane-per-row kernels, the buffered passes run.
+ffhip_mat pattern_p1(ffhip_ctx ctx, ffhip_mesh m, const double *Ce, hipEvent_t ce_ready,
+                     bool *values_done) {
+  hipStream_t s = ctx->stream;
+  build_v2e(m);
+  const int64_t n = m->nv;
+  const bool vals_pass = Ce && m->nloc == 4;
+  ffhip_mat A = new ffhip_mat_s();
+  A->ctx = ctx; A->n = n; A->mesh = m;
+  try {
+    A->rowptr = dalloc<int64_t>(n + 1);
+    A->diag = dalloc<int64_t>(n);
+    A->slotmap = dalloc<uint32_t>(m->nt * m->nloc);
+    A->big = dalloc<uint8_t>(n);
+    if (ce_ready) FFHIP_CHECK(hipStreamWaitEvent(s, ce_ready, 0));
+    bool direct_ok = false;
+    {  // direct pass
+      Arena ar(ctx, Arena::bytes(n, 4) + Arena::bytes(n + 1, 8) + Arena::bytes(2, 4));
+      uint32_t *bcnt = ar.take<uint32_t>(n);
+      int64_t *len = ar.take<int64_t>(n + 1);
+      int *flags = ar.take<int>(2);  // any big row, any row unlike its link count
+      FFHIP_CHECK(hipMemsetAsync(bcnt, 0, 4 * n, s));
+      FFHIP_CHECK(hipMemsetAsync(len + n, 0, 8, s));
+      FFHIP_CHECK(hipMemsetAsync(flags, 0, 2 * sizeof(int), s));
+      if (m->nbe)
+        k_bcount<<<nblocks(m->nbe * m->nbloc, 256), 256, 0, s>>>(m->nbe * m->nbloc, m->belems, bcnt);
+      k_link_len<<<nblocks(n, 256), 256, 0, s>>>(n, m->dim, m->v2e_ptr, bcnt, len);
+      FFHIP_CHECK(hipGetLastError());
+      exclusive_scan_i64(ctx, len, A->rowptr, n + 1);
+      A->nnz = read_i64(ctx, A->rowptr + n);
+      A->colidx = dalloc<int32_t>(A->nnz);
+      A->vals = dalloc<double>(A->nnz);
+      PatternOut out;
+      out.rowptr = A->rowptr; out.colidx = A->colidx; out.diag = A->diag; out.vals = A->vals;
+      out.mismatch = flags + 1;
+      if (vals_pass)
+        k_pattern_half<4, true, true><<<nblocks(n * kHalf, 256), 256, 0, s>>>(
+            n, m->elems, m->v2e_ptr, m->v2e, A->slotmap, A->big, flags, out, m->xyz, Ce);
+      else if (m->nloc == 4)
+        k_pattern_half<4, false, true><<<nblocks(n * kHalf, 256), 256, 0, s>>>(
+            n, m->elems, m->v2e_ptr, m->v2e, A->slotmap, A->big, flags, out);
+      else
+        k_pattern_half<3, false, true><<<nblocks(n * kHalf, 256), 256, 0, s>>>(
+            n, m->elems, m->v2e_ptr, m->v2e, A->slotmap, A->big, flags, out);
+      FFHIP_CHECK(hipGetLastError());
+      int h[2] = {0, 0};
+      FFHIP_CHECK(hipMemcpyAsync(h, flags, sizeof(h), hipMemcpyDeviceToHost, s));
+      FFHIP_CHECK(hipStreamSynchronize(s));
+      direct_ok = !h[0] && !h[1];
+      if (!direct_ok) { dfree(A->colidx); dfree(A->vals); }
+    }
+    if (direct_ok) {
+      if (!vals_pass) FFHIP_CHECK(hipMemsetAsync(A->vals, 0, 8 * A->nnz, s));
+      A->any_big = 0;
+      A->max_row = kRowBuf;
+      if (values_done) *values_done = vals_pass;
+      return A;
+    }
+    // buffered passes
+    Arena ar(ctx, Arena::bytes(n + 1, 8) + Arena::bytes(1, 4) + Arena::bytes(kRowBuf * n, 4) +
+                      Arena::bytes(n, 1) + Arena::bytes(1, 4) +
+                      (vals_pass ? Arena::bytes(kRowBuf * n, 8) : 0));
+    int64_t *len = ar.take<int64_t>(n + 1);
+    int *ovf = ar.take<int>(1);
+    FFHIP_CHECK(hipMemsetAsync(len, 0, 8 * (n + 1), s));
+    FFHIP_CHECK(hipMemsetAsync(ovf, 0, sizeof(int), s));
+    // rows with up to 32 incident elements and 64 columns: one half-wave
+    // pass into row buffers (big = 0); the rest: the lane-per-row merge with
+    // 64 LDS columns per lane (big = 1), then with 256 (big = 2: the slot
+    // offsets of the element-to-row map are bytes)
+    constexpr int B = 128, CAP = 64, BW = 32, CAPW = 256;
+    PatternOut out;
+    out.rowlen = len;
+    out.rowbuf = ar.take<int32_t>(kRowBuf * n);
+    out.dposr = ar.take<int8_t>(n);
+    int *anyb = ar.take<int>(1);
+    out.rowval = vals_pass ? ar.take<double>(kRowBuf * n) : nullptr;
+    FFHIP_CHECK(hipMemsetAsync(anyb, 0, sizeof(int), s));
+    if (vals_pass)
+      k_pattern_half<4, true, false><<<nblocks(n * kHalf, 256), 256, 0, s>>>(
+          n, m->elems, m->v2e_ptr, m->v2e, A->slotmap, A->big, anyb, out, m->xyz, Ce);
+    else if (m->nloc == 4)
+      k_pattern_half<4, false, false><<<nblocks(n * kHalf, 256), 256, 0, s>>>(
+          n, m->elems, m->v2e_ptr, m->v2e, A->slotmap, A->big, anyb, out);
+    else
+      k_pattern_half<3, false, false><<<nblocks(n * kHalf, 256), 256, 0, s>>>(
+          n, m->elems, m->v2e_ptr, m->v2e, A->slotmap, A->big, anyb, out);
+    FFHIP_CHECK(hipGetLastError());
+    A->any_big = read_flag(ctx, anyb);
+    bool wide = false;
+    if (A->any_big) {
+      k_pattern_rows<B, CAP, false><<<nblocks(n, B), B, 0, s>>>(
+          n, m->nloc, m->elems, m->v2e_ptr, m->v2e, len, nullptr, nullptr, nullptr, nullptr,
+          nullptr, ovf, A->big, 1);
+      FFHIP_CHECK(hipGetLastError());
+      wide = read_flag(ctx, ovf) != 0;
+    }
+    if (wide) {
+      FFHIP_CHECK(hipMemsetAsync(ovf, 0, sizeof(int), s));
+      k_pattern_rows<BW, CAPW, false><<<nblocks(n, BW), BW, 0, s>>>(
+          n, m->nloc, m->elems, m->v2e_ptr, m->v2e, len, nullptr, nullptr, nullptr, nullptr,
+          nullptr, ovf, A->big, 2);
+      FFHIP_CHECK(hipGetLastError());
+      FFHIP_REQUIRE(read_flag(ctx, ovf) == 0, FFHIP_ERR_UNSUPPORTED,
+                    "ffhip_pattern_p1: a row has more than %d columns", CAPW);
+    }
+    exclusive_scan_i64(ctx, len, A->rowptr, n + 1);
+    A->nnz = read_i64(ctx, A->rowptr + n);
+    A->colidx = dalloc<int32_t>(A->nnz);
+    A->vals = dalloc<double>(A->nnz);
+    k_pattern_compact<<<nblocks(n, 256), 256, 0, s>>>(n, A->rowptr, out.rowbuf, out.dposr, A->big,
+                                                      A->colidx, A->diag, out.rowval,
+                                                      vals_pass ? A->vals : nullptr);
+    if (A->any_big)
+      k_pattern_rows<B, CAP, true><<<nblocks(n, B), B, 0, s>>>(
+          n, m->nloc, m->elems, m->v2e_ptr, m->v2e, nullptr, A->rowptr, A->colidx, A->diag,
+          A->slotmap, nullptr, ovf, A->big, 1);
+    if (wide)
+      k_pattern_rows<BW, CAPW, true><<<nblocks(n, BW), BW, 0, s>>>(
+          n, m->nloc, m->elems, m->v2e_ptr, m->v2e, nullptr, A->rowptr, A->colidx, A->diag,
+          A->slotmap, nullptr, ovf, A->big, 2);
+    FFHIP_CHECK(hipGetLastError());
+    if (!vals_pass) FFHIP_CHECK(hipMemsetAsync(A->vals, 0, 8 * A->nnz, s));
+    if (values_done) *values_done = vals_pass && !A->any_big;
+    A->max_row = wide ? CAPW : CAP;
+  } catch (...) {
+    (void)hipStreamSynchronize(s);
+    dfree(A->rowptr); dfree(A->colidx); dfree(A->vals); dfree(A->diag);
+    dfree(A->slotmap); dfree(A->big);
+    delete A;
+    throw;
+  }
+  return A;
+}
+
 }  // namespace ffhip
 
 using namespace ffhip;
@@ -1134,82 +1480,55 @@ int ffhip_pattern_p1(ffhip_ctx ctx, ffhip_mesh m, ffhip_mat *out) {
   return guard([&] {
     FFHIP_REQUIRE(ctx && m && out, FFHIP_ERR_ARG, "ffhip_pattern_p1: null argument");
     FFHIP_REQUIRE(m->nloc <= 4, FFHIP_ERR_UNSUPPORTED, "ffhip_pattern_p1: P1 on simplices only");
-    hipStream_t s = ctx->stream;
     ctx->tic();
-    build_v2e(m);
-    const int64_t n = m->nv;
-    ffhip_mat A = new ffhip_mat_s();
-    A->ctx = ctx; A->n = n; A->mesh = m;
+    ffhip_mat A = pattern_p1(ctx, m, nullptr, nullptr, nullptr);
+    ctx->toc();
+    *out = A;
+  });
+}
+
+int ffhip_pattern_assemble_p1(ffhip_ctx ctx, ffhip_mesh m, int kind, double c0, const double *kappa_e,
+                              int kappa_on_device, ffhip_mat *out) {
+  return guard([&] {
+    FFHIP_REQUIRE(ctx && m && out, FFHIP_ERR_ARG, "ffhip_pattern_assemble_p1: null argument");
+    FFHIP_REQUIRE(m->dim == 3 && m->nloc == 4, FFHIP_ERR_UNSUPPORTED,
+                  "ffhip_pattern_assemble_p1: 3D P1 meshes (2D: ffhip_pattern_p1 + ffhip_assemble_p1)");
+    FFHIP_REQUIRE(kind == FFHIP_COEF_CONST || kind == FFHIP_COEF_SIN3 || kind == FFHIP_COEF_P0 ||
+                      kind == FFHIP_COEF_QP || kind == FFHIP_COEF_P1,
+                  FFHIP_ERR_ARG, "ffhip_pattern_assemble_p1: coefficient kind %d", kind);
+    const bool arr = kind == FFHIP_COEF_P0 || kind == FFHIP_COEF_QP || kind == FFHIP_COEF_P1;
+    FFHIP_REQUIRE(!arr || kappa_e, FFHIP_ERR_ARG, "ffhip_pattern_assemble_p1: kappa_e");
+    hipStream_t s = ctx->stream;
+    const double *kd = kappa_e;
+    double *kbuf = nullptr, *Ce = nullptr;
+    ffhip_mat A = nullptr;
     try {
-      Arena ar(ctx, Arena::bytes(n + 1, 8) + Arena::bytes(1, 4) + Arena::bytes(kRowBuf * n, 4) +
-                        Arena::bytes(n, 1) + Arena::bytes(1, 4));
-      int64_t *len = ar.take<int64_t>(n + 1);
-      int *ovf = ar.take<int>(1);
-      A->rowptr = dalloc<int64_t>(n + 1);
-      A->diag = dalloc<int64_t>(n);
-      A->slotmap = dalloc<uint32_t>(m->nt * m->nloc);
-      FFHIP_CHECK(hipMemsetAsync(len, 0, 8 * (n + 1), s));
-      FFHIP_CHECK(hipMemsetAsync(ovf, 0, sizeof(int), s));
-      // rows with up to 32 incident elements and 64 columns: one half-wave
-      // pass into row buffers (big = 0); the rest: the lane-per-row merge with
-      // 64 LDS columns per lane (big = 1), then with 256 (big = 2: the slot
-      // offsets of the element-to-row map are bytes)
-      constexpr int B = 128, CAP = 64, BW = 32, CAPW = 256;
-      int32_t *rowbuf = ar.take<int32_t>(kRowBuf * n);
-      int8_t *dposr = ar.take<int8_t>(n);
-      int *anyb = ar.take<int>(1);
-      A->big = dalloc<uint8_t>(n);
-      FFHIP_CHECK(hipMemsetAsync(anyb, 0, sizeof(int), s));
-      if (m->nloc == 4)
-        k_pattern_half<4><<<nblocks(n * kHalf, 256), 256, 0, s>>>(
-            n, m->elems, m->v2e_ptr, m->v2e, len, rowbuf, dposr, A->slotmap, A->big, anyb);
-      else
-        k_pattern_half<3><<<nblocks(n * kHalf, 256), 256, 0, s>>>(
-            n, m->elems, m->v2e_ptr, m->v2e, len, rowbuf, dposr, A->slotmap, A->big, anyb);
-      FFHIP_CHECK(hipGetLastError());
-      A->any_big = read_flag(ctx, anyb);
-      bool wide = false;
-      if (A->any_big) {
-        k_pattern_rows<B, CAP, false><<<nblocks(n, B), B, 0, s>>>(
-            n, m->nloc, m->elems, m->v2e_ptr, m->v2e, len, nullptr, nullptr, nullptr, nullptr,
-            nullptr, ovf, A->big, 1);
-        FFHIP_CHECK(hipGetLastError());
-        wide = read_flag(ctx, ovf) != 0;
+      if (arr && !kappa_on_device) {
+        const int64_t cnt = kind == FFHIP_COEF_P1 ? m->nv : kind == FFHIP_COEF_P0 ? m->nt : 14 * m->nt;
+        kbuf = dalloc<double>(cnt);
+        FFHIP_CHECK(hipMemcpyAsync(kbuf, kappa_e, 8 * cnt, hipMemcpyHostToDevice, s));
+        kd = kbuf;
       }
-      if (wide) {
-        FFHIP_CHECK(hipMemsetAsync(ovf, 0, sizeof(int), s));
-        k_pattern_rows<BW, CAPW, false><<<nblocks(n, BW), BW, 0, s>>>(
-            n, m->nloc, m->elems, m->v2e_ptr, m->v2e, len, nullptr, nullptr, nullptr, nullptr,
-            nullptr, ovf, A->big, 2);
-        FFHIP_CHECK(hipGetLastError());
-        FFHIP_REQUIRE(read_flag(ctx, ovf) == 0, FFHIP_ERR_UNSUPPORTED,
-                      "ffhip_pattern_p1: a row has more than %d columns", CAPW);
-      }
-      exclusive_scan_i64(ctx, len, A->rowptr, n + 1);
-      A->nnz = read_i64(ctx, A->rowptr + n);
-      A->colidx = dalloc<int32_t>(A->nnz);
-      A->vals = dalloc<double>(A->nnz);
-      k_pattern_compact<<<nblocks(n, 256), 256, 0, s>>>(n, A->rowptr, rowbuf, dposr, A->big,
-                                                        A->colidx, A->diag);
-      if (A->any_big)
-        k_pattern_rows<B, CAP, true><<<nblocks(n, B), B, 0, s>>>(
-            n, m->nloc, m->elems, m->v2e_ptr, m->v2e, nullptr, A->rowptr, A->colidx, A->diag,
-            A->slotmap, nullptr, ovf, A->big, 1);
-      if (wide)
-        k_pattern_rows<BW, CAPW, true><<<nblocks(n, BW), BW, 0, s>>>(
-            n, m->nloc, m->elems, m->v2e_ptr, m->v2e, nullptr, A->rowptr, A->colidx, A->diag,
-            A->slotmap, nullptr, ovf, A->big, 2);
+      Ce = dalloc<double>(m->nt);
+      ctx->tic();
+      // the element scales, then the incidence lists and the pattern pass,
+      // which forms the row values from them
+      k_p1_scale_3d<<<nblocks(m->nt, 256), 256, 0, s>>>(m->nt, m->xyz, m->elems, kind, c0, kd, Ce);
       FFHIP_CHECK(hipGetLastError());
-      FFHIP_CHECK(hipMemsetAsync(A->vals, 0, 8 * A->nnz, s));
+      bool done = false;
+      A = pattern_p1(ctx, m, Ce, nullptr, &done);
       ctx->toc();
-      A->max_row = wide ? CAPW : CAP;
+      if (!done) {  // rows for the lane-per-row kernels: all values by the element-row path
+        const int rc = ffhip_assemble_p1(ctx, m, A, kind, c0, kd, 1);
+        FFHIP_REQUIRE(rc == FFHIP_OK, rc, "ffhip_pattern_assemble_p1: %s", ffhip_last_error());
+      }
     } catch (...) {
       (void)hipStreamSynchronize(s);
-      dfree(A->rowptr); dfree(A->colidx); dfree(A->vals); dfree(A->diag);
-      dfree(A->slotmap); dfree(A->big);
-      delete A;
+      dfree(Ce); dfree(kbuf);
+      if (A) ffhip_mat_free(A);
       throw;
     }
+    dfree(Ce); dfree(kbuf);
     *out = A;
   });
 }
@@ -1250,10 +1569,14 @@ int ffhip_assemble_p1_form(ffhip_ctx ctx, ffhip_mesh m, ffhip_mat A, int kind, d
     const int kappa_on_device = on_device;
     hipStream_t s = ctx->stream;
     const int nu = m->nloc == 4 ? 10 : 6;
-    // 3D: the element rows in incidence order, and Ke only for the lane-per-row
-    // gather of long rows
+    // 3D stiffness without a reaction term and with every row in the
+    // half-wave kernels: element scales, then rows formed from the vertices.
+    // Otherwise the element rows in incidence order (3D), and Ke for the
+    // lane-per-row gather of long rows and 2D.
+    const bool rows_path = m->dim == 3 && mkind == FFHIP_COEF_NONE && !A->any_big;
     const bool needKe = m->nloc != 4 || A->any_big;
-    Arena ar(ctx, (m->dim == 3 ? Arena::bytes(16 * m->nt, 8) : 0) +
+    Arena ar(ctx, (rows_path ? Arena::bytes(m->nt, 8) : 0) +
+                      (!rows_path && m->dim == 3 ? Arena::bytes(16 * m->nt, 8) : 0) +
                       (needKe ? Arena::bytes(nu * m->nt, 8) : 0));
     double *Ke = needKe ? ar.take<double>(nu * m->nt) : nullptr;
     const double *kd = kappa_e, *md = mass_e;
@@ -1274,9 +1597,16 @@ int ffhip_assemble_p1_form(ffhip_ctx ctx, ffhip_mesh m, ffhip_mat A, int kind, d
       md = mbuf;
     }
     if (m->dim == 2) upload_t5_once(ctx->device);
+    if (!rows_path && m->dim == 3) build_v2e_inv(m);
     ctx->tic();
     double *Kinc = nullptr;
-    if (m->dim == 3) {
+    if (rows_path) {
+      double *Ce = ar.take<double>(m->nt);
+      k_p1_scale_3d<<<nblocks(m->nt, 256), 256, 0, s>>>(m->nt, m->xyz, m->elems, kind, c0, kd, Ce);
+      const int64_t g = (nblocks(A->n * kHalf, 256) + 7) / 8 * 8;  // a multiple of the 8 XCDs
+      k_p1_values_rows<<<(unsigned)g, 256, 0, s>>>(A->n, A->rowptr, m->v2e_ptr, m->v2e, A->slotmap,
+                                                   m->xyz, m->elems, Ce, A->big, A->vals);
+    } else if (m->dim == 3) {
       Kinc = ar.take<double>(16 * m->nt);
       if (mkind == FFHIP_COEF_NONE)
         k_p1_element_3d<false><<<nblocks(m->nt, 256), 256, 0, s>>>(
@@ -1290,16 +1620,16 @@ int ffhip_assemble_p1_form(ffhip_ctx ctx, ffhip_mesh m, ffhip_mat A, int kind, d
       k_p1_element_2d<<<nblocks(m->nt, 256), 256, 0, s>>>(m->nt, m->xyz, m->elems, kind, c0, kd,
                                                            mkind, m0, md, Ke);
     constexpr int B = 256, STAGE = 6144;
-    if (m->nloc == 4) {
+    if (!rows_path && m->nloc == 4) {
       k_p1_values_half<<<nblocks(A->n * kHalf, 256), 256, 0, s>>>(
           A->n, A->rowptr, m->v2e_ptr, A->slotmap, Kinc, A->big, A->vals);
       if (A->any_big)
         k_p1_gather<B, STAGE, 4><<<nblocks(A->n, B), B, 0, s>>>(
             A->n, A->rowptr, m->v2e_ptr, m->v2e, A->slotmap, Ke, A->vals, A->big);
-    }
-    else
+    } else if (m->nloc == 3) {
       k_p1_gather<B, STAGE, 3><<<nblocks(A->n, B), B, 0, s>>>(A->n, A->rowptr, m->v2e_ptr, m->v2e,
                                                                A->slotmap, Ke, A->vals);
+    }
     FFHIP_CHECK(hipGetLastError());
     ctx->toc();
     dfree(kbuf);

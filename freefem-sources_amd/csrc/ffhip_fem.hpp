@@ -72,6 +72,41 @@ __device__ inline double tet_geom(const double v[4][3], double G[4][3]) {
   return mes;
 }
 
+// the 4 cross products N_l of a tet's edge vectors from vertex 0, grad l_l
+// times 6 |K| (Tet::Gradlambda, Mesh3dn.hpp:126-137, before its division),
+// with N_0 = -(N_1 + N_2 + N_3)
+__device__ inline void tet_normals(const double v[4][3], double N[4][3]) {
+  double V1[3], V2[3], V3[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    V1[k] = v[1][k] - v[0][k]; V2[k] = v[2][k] - v[0][k]; V3[k] = v[3][k] - v[0][k];
+  }
+  N[1][0] = V2[1] * V3[2] - V2[2] * V3[1];
+  N[1][1] = V3[0] * V2[2] - V2[0] * V3[2];
+  N[1][2] = V2[0] * V3[1] - V2[1] * V3[0];
+  N[2][0] = V3[1] * V1[2] - V3[2] * V1[1];
+  N[2][1] = V1[0] * V3[2] - V3[0] * V1[2];
+  N[2][2] = V3[0] * V1[1] - V3[1] * V1[0];
+  N[3][0] = V1[1] * V2[2] - V1[2] * V2[1];
+  N[3][1] = V2[0] * V1[2] - V1[0] * V2[2];
+  N[3][2] = V1[0] * V2[1] - V1[1] * V2[0];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) N[0][k] = -N[1][k] - N[2][k] - N[3][k];
+}
+
+// the 4 entries K_e(l, m) = C N_l . N_m of row l of a P1 stiffness element
+// matrix whose scale is C = S_e / (6 |K|)^2 (K_e = S_e grad l . grad l)
+__device__ inline void p1_row_entries(const double v[4][3], int l, double C, double r[4]) {
+  double N[4][3];
+  tet_normals(v, N);
+  double Nl[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    Nl[k] = l == 0 ? N[0][k] : (l == 1 ? N[1][k] : (l == 2 ? N[2][k] : N[3][k]));
+#pragma unroll
+  for (int m = 0; m < 4; ++m) r[m] = C * (Nl[0] * N[m][0] + Nl[1] * N[m][1] + Nl[2] * N[m][2]);
+}
+
 __device__ inline double tet_mesure(const double v[4][3]) {
   double AB[3], AC[3], AD[3];
   for (int k = 0; k < 3; ++k) {

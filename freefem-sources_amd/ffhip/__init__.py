@@ -111,6 +111,7 @@ _SIGS = {
     "ffhip_last_ms": (_d, [_vp]),
     "ffhip_bench_copy": (_i, [_vp, _l, _i, ctypes.POINTER(_d)]),
     "ffhip_p1_misfit": (_i, [_vp, _vp, _vp, _vp, _vp, _i, ctypes.POINTER(_d), _vp]),
+    "ffhip_pattern_assemble_p1": (_i, [_vp, _vp, _i, _d, _vp, _i, _pvp]),
     "ffhip_p1_misfit_noalloc": (_i, [_vp, _vp, _vp, _vp, _vp, _i, ctypes.POINTER(_d), _vp]),
     "ffhip_p1_grad_kappa": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "ffhip_p2v_create": (_i, [_vp, _vp, _pvp]),
@@ -305,6 +306,23 @@ class Matrix:
     def p1(mesh):
         h = ctypes.c_void_p()
         check(lib().ffhip_pattern_p1(mesh.ctx.h, mesh.h, ctypes.byref(h)))
+        return Matrix(h, mesh.ctx, mesh)
+
+    @staticmethod
+    def p1_assembled(mesh, kind=COEF_SIN3, c0=1.0, kappa_e=None):
+        """`matrix A = va(Vh,Vh)` of int3d(kappa grad u . grad v) on a 3D P1
+        space in one call (pattern and values; the same matrix as p1() then
+        assemble_p1()).  kappa_e: numpy (host) or torch cuda tensor."""
+        on_dev, ptr = 0, None
+        if kappa_e is not None:
+            if isinstance(kappa_e, np.ndarray):
+                kappa_e = np.ascontiguousarray(kappa_e, np.float64)
+                ptr = _hp(kappa_e)
+            else:
+                ptr, on_dev = _dp(kappa_e), 1
+        h = ctypes.c_void_p()
+        check(lib().ffhip_pattern_assemble_p1(mesh.ctx.h, mesh.h, kind, c0, ptr, on_dev,
+                                              ctypes.byref(h)))
         return Matrix(h, mesh.ctx, mesh)
 
     @staticmethod

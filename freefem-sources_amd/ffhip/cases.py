@@ -5,13 +5,15 @@ quick timing:
     python3 freefem-sources_amd/ffhip/cases.py assembly_p2 32 3
     python3 freefem-sources_amd/ffhip/cases.py solve_p2 32
 
-assembly_p1: cube(n) P1 Poisson pattern + values + tgv rows + mixed solver
-setup, `reps` times (a fresh device mesh each time).
+assembly_p1: cube(n) P1 Poisson pattern and values (one call) + tgv rows +
+mixed solver setup, `reps` times (a fresh device mesh each time).
 assembly_p2: cube(n) [P2,P2,P2] numbering + pattern + Lame values, `reps` times.
 solve_p2: cube(n) Lame system, FP64 and MIXED solves in every solver layout,
 with per-kernel HIP-event timing.
 pcie_p1: the host-buffer path of the C-ABI on cube(n) P1: mesh upload from
 host arrays, device pattern and values, CSR download to host (wall times).
+copy: the copy kernel (1 GiB read + 1 GiB written per launch) whose PMC byte
+counts scale the other kernels'.
 """
 import os
 import sys
@@ -25,13 +27,11 @@ def assembly_p1(n=160, reps=3):
     import ffhip
     for _ in range(reps):
         M = ffhip.cube(n)
-        A = ffhip.Matrix.p1(M)
-        tp = M.ctx.last_ms()
-        tv = A.assemble_p1(ffhip.COEF_SIN3)
+        A = ffhip.Matrix.p1_assembled(M, ffhip.COEF_SIN3)
+        tpv = M.ctx.last_ms()
         tb = A.set_bc(LABELS)
         cg = ffhip.CG(A, ffhip.CG_MIXED)
-        print(f"P1 cube({n}) nnz {A.nnz}: pattern {tp:.3f} ms, values {tv:.3f} ms, bc {tb:.3f} ms",
-              flush=True)
+        print(f"P1 cube({n}) nnz {A.nnz}: pattern and values {tpv:.3f} ms, bc {tb:.3f} ms", flush=True)
         cg.free(); A.free(); M.free()
 
 
@@ -97,8 +97,17 @@ def pcie_p1(n=160, reps=2):
         A.free(); M.free()
 
 
+def copy(mib=1024, reps=5):
+    """the copy kernel the PMC byte counters are scaled by (reads and writes
+    mib MiB per launch)"""
+    import ffhip
+    ctx = ffhip.context()
+    ms = ctx.bench_copy(mib << 20, reps)
+    print(f"copy {mib} MiB: {ms:.3f} ms, {2 * (mib << 20) / ms / 1e6:.0f} GB/s", flush=True)
+
+
 if __name__ == "__main__":
     case = sys.argv[1]
     args = [int(a) for a in sys.argv[2:]]
     {"assembly_p1": assembly_p1, "assembly_p2": assembly_p2, "solve_p2": solve_p2,
-     "pcie_p1": pcie_p1}[case](*args)
+     "pcie_p1": pcie_p1, "copy": copy}[case](*args)

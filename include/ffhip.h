@@ -90,6 +90,15 @@ int ffhip_mesh_free(ffhip_mesh mesh);
  * (HashMatrix.cpp:858-867, Sortij 670-682, Buildp 992-1030).  Also builds the
  * element-to-row map used by the atomic-free assembly. */
 int ffhip_pattern_p1(ffhip_ctx ctx, ffhip_mesh mesh, ffhip_mat *mat);
+/* `matrix A = va(Vh,Vh)` of int3d(Th)(kappa*(dx(u)*dx(v)+dy(u)*dy(v)+
+ * dz(u)*dz(v))) on a 3D P1 space in one call: the pattern of
+ * ffhip_pattern_p1 and the values of ffhip_assemble_p1 (kappa described as
+ * there), the same matrix; what creationBlockOfMatrixToBilinearForm does the
+ * first time a matrix is assembled on a space (problem.hpp:1663-1714).  The
+ * element pass runs beside the incidence and pattern passes, which also form
+ * the row values. */
+int ffhip_pattern_assemble_p1(ffhip_ctx ctx, ffhip_mesh mesh, int kind, double c0,
+                              const double *kappa_e, int kappa_on_device, ffhip_mat *mat);
 int ffhip_mat_info(ffhip_mat mat, int64_t *n, int64_t *nnz);
 
 enum { FFHIP_COEF_NONE = -1, FFHIP_COEF_CONST = 0, FFHIP_COEF_SIN3 = 1, FFHIP_COEF_P0 = 2,

@@ -44,9 +44,9 @@ def oracle_pattern():
 
 @pytest.fixture(scope="module")
 def device_system():
+    # the one-call assembly the benchmark times (pattern + values)
     M = ffhip.cube(N)
-    A = ffhip.Matrix.p1(M)
-    A.assemble_p1(ffhip.COEF_SIN3)
+    A = ffhip.Matrix.p1_assembled(M, ffhip.COEF_SIN3)
     rp, ci, va = A.download()
     A.set_bc(LABELS)
     b = ffhip.rhs_p1(M, 1.0, LABELS)

@@ -447,3 +447,44 @@ def test_graph_replay_bitwise_equals_eager(mode, layout):
     xe, se = cg_e.solve(bo, eps=1e-10)
     assert np.array_equal(xp, xe)
     assert abs(sp["timed"] - sp["iters"]) <= 1 and sp["ms_matvec_avg"] > 0 and sp["ms_update_avg"] > 0
+
+
+@pytest.mark.parametrize("kind", ["sin3", "const", "p0", "qp", "p1"])
+def test_pattern_assemble_one_call_equals_two_steps(kind):
+    """ffhip_pattern_assemble_p1 (pattern and values in one call, the values
+    formed in the pattern pass) gives the matrix of ffhip_pattern_p1 +
+    ffhip_assemble_p1 bit for bit, and the oracle's within 1e-12"""
+    n = 9
+    M = ffhip.cube(n)
+    rng = np.random.default_rng(3)
+    arrs = {"sin3": (ffhip.COEF_SIN3, 1.0, None), "const": (ffhip.COEF_CONST, 2.5, None),
+            "p0": (ffhip.COEF_P0, 0.0, np.exp(rng.standard_normal(M.nt))),
+            "qp": (ffhip.COEF_QP, 0.0, np.exp(rng.standard_normal(14 * M.nt))),
+            "p1": (ffhip.COEF_P1, 0.0, np.exp(rng.standard_normal(M.nv)))}
+    k, c0, arr = arrs[kind]
+    A1 = ffhip.Matrix.p1_assembled(M, k, c0, arr)
+    A2 = ffhip.Matrix.p1(M)
+    A2.assemble_p1(k, c0, arr)
+    r1, c1, v1 = A1.download()
+    r2, c2, v2 = A2.download()
+    assert np.array_equal(r1, r2) and np.array_equal(c1, c2)
+    assert np.array_equal(v1.view(np.int64), v2.view(np.int64))
+    if kind in ("sin3", "const", "p0"):
+        mo = ffo.cube(n)
+        rpo, cio = ffo.pattern(mo["nv"], mo["tets"], 4)
+        vo = ffo.assemble_p1_3d(mo, rpo, cio, {"sin3": ffo.COEF_SIN3, "const": ffo.COEF_CONST,
+                                               "p0": ffo.COEF_P0}[kind], c0, arr)
+        assert np.array_equal(r1, rpo) and np.array_equal(c1, cio)
+        assert np.max(np.abs(v1 - vo)) <= 1e-12 * np.max(np.abs(vo))
+
+
+def test_pattern_assemble_one_call_with_lane_per_row_rows():
+    """a vertex of 103 columns: the one-call path falls back to the element-row
+    values for every row; the matrix equals the two-step one"""
+    m = _fan_mesh(100)
+    M = ffhip.mesh3_upload(m["xyz"], m["tets"], m["btri"], m["blab"])
+    A1 = ffhip.Matrix.p1_assembled(M, ffhip.COEF_SIN3)
+    A2 = ffhip.Matrix.p1(M)
+    A2.assemble_p1(ffhip.COEF_SIN3)
+    d1, d2 = A1.download(), A2.download()
+    assert all(np.array_equal(a, b) for a, b in zip(d1, d2))

@@ -281,9 +281,13 @@ __global__ __launch_bounds__(256) void k_pattern_half(
   __shared__ uint32_t so[VALS ? NH : 1][VALS ? kHalf : 1];      // their slot bytes (VALS)
   const int hl = threadIdx.x % kHalf, hid = threadIdx.x / kHalf;
   const int64_t v = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kHalf;
-  int64_t k0 = 0;
+  int64_t k0 = 0, p0 = 0, p1 = 0;
   int deg = 0;
-  if (v < n) { k0 = v2e_ptr[v]; deg = (int)(v2e_ptr[v + 1] - k0); }
+  if (v < n) {
+    k0 = v2e_ptr[v];
+    deg = (int)(v2e_ptr[v + 1] - k0);
+    if (DIRECT) { p0 = out.rowptr[v]; p1 = out.rowptr[v + 1]; }
+  }
   bool mine = v < n && deg <= kHalf;
   if (!mine) deg = 0;
   for (int i = hl; i < kHash; i += kHalf) hk[hid][i] = INT_MAX;
@@ -291,6 +295,7 @@ __global__ __launch_bounds__(256) void k_pattern_half(
   int32_t c[4] = {INT_MAX, INT_MAX, INT_MAX, INT_MAX};
   int64_t e = 0;
   int lv = 0;
+  double vv[VALS ? 4 : 1][3], Cv = 0.;
   if (hl < deg) {
     const int32_t idx = v2e[k0 + hl];
     e = idx / NLOC;
@@ -301,6 +306,13 @@ __global__ __launch_bounds__(256) void k_pattern_half(
     } else {
 #pragma unroll
       for (int m = 0; m < NLOC; ++m) c[m] = elems[e * NLOC + m];
+    }
+    if constexpr (VALS) {  // issued now, used after the set and sort phases (their latency hidden)
+#pragma unroll
+      for (int i = 0; i < (VALS ? 4 : 1); ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) vv[i][k] = xyz[3 * (int64_t)c[i] + k];
+      Cv = Ce[e];
     }
   }
   __syncthreads();
@@ -325,12 +337,8 @@ __global__ __launch_bounds__(256) void k_pattern_half(
   __syncthreads();
   const int len = ucnt[hid];
   if (len > kRowBuf) mine = false;
-  int64_t r0 = 0;
-  bool wrong = false;  // DIRECT: the row's length is not its pointers'
-  if (DIRECT && mine) {
-    r0 = out.rowptr[v];
-    wrong = out.rowptr[v + 1] - r0 != len;
-  }
+  const int64_t r0 = p0;
+  const bool wrong = DIRECT && mine && p1 - p0 != len;  // the row's length is not its pointers' 
   // counting sort: the rank of a column is the number of smaller columns
   int32_t u[2] = {INT_MAX, INT_MAX};
   int rk[2] = {0, 0};
@@ -383,19 +391,15 @@ __global__ __launch_bounds__(256) void k_pattern_half(
       pk |= (uint32_t)lo << (8 * m);
     }
     slotmap[k0 + hl] = pk;
-    if (VALS) {
-      double vv[4][3], r[4];
-#pragma unroll
-      for (int i = 0; i < 4; ++i)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) vv[i][k] = xyz[3 * (int64_t)c[i] + k];
-      p1_row_entries(vv, lv, Ce[e], r);
+    if constexpr (VALS) {
+      double r[4];
+      p1_row_entries(vv, lv, Cv, r);
 #pragma unroll
       for (int m = 0; m < 4; ++m) sv[hid][4 * hl + m] = r[m];
       so[hid][hl] = pk;
     }
   }
-  if (VALS) {  // the row's values, each slot summed by its lane in element order
+  if constexpr (VALS) {  // the row's values, each slot summed by its lane in element order
     __syncthreads();
     const int nq = mine ? deg : 0;
     const uint32_t me0 = (uint32_t)hl * 0x01010101u, me1 = (uint32_t)(hl + kHalf) * 0x01010101u;

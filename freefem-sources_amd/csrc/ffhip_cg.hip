@@ -47,7 +47,7 @@ constexpr int kWavesPerBlock = kBlock / kSlice;
 #define FFHIP_SELL_U 16
 #endif
 #ifndef FFHIP_WPE
-#define FFHIP_WPE 4
+#define FFHIP_WPE 3
 #endif
 #ifndef FFHIP_GRID_PER_CU
 #define FFHIP_GRID_PER_CU (FFHIP_WPE)
@@ -137,6 +137,18 @@ __device__ inline ACC sell_row(const IT *__restrict__ colp, const V *__restrict_
 // a lane's row are one 8-byte load (layout [g][lane][4] inside a slice), so a
 // round of U entries issues U/4 index loads ahead of the U value loads.
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// FP64 values of a kSell16 slice of width L are stored in entry pairs: pair m
+// (entries 2m, 2m+1) of lane l at 2 (64 m + l), so one 16-byte load per lane
+// reads two entries of its row; for odd L the last entry follows the L/2 pair
+// planes as one plane of single values.  Position of entry k of lane l,
+// relative to the slice start:
+__host__ __device__ inline int64_t pair_pos(int L, int k, int lane) {
+  const int h = L >> 1;
+  return (k >> 1) < h ? ((int64_t)(k >> 1) * kSlice + lane) * 2 + (k & 1) : (int64_t)h * 2 * kSlice + lane;
+}
+
 template <int U, class V, class ACC, class F>
 __device__ inline ACC sell_row16(const int16_t *__restrict__ colp, const V *__restrict__ valp,
                                  int L, int32_t row0, F xval) {
@@ -232,7 +244,7 @@ __global__ __launch_bounds__(256) void k_sell_fill(
     const int64_t *__restrict__ sptr, const int32_t *__restrict__ len,
     const double *__restrict__ dsc, int32_t *__restrict__ scol,
     const int64_t *__restrict__ sptr16, int16_t *__restrict__ scol16, double *__restrict__ sval,
-    float *__restrict__ sval32) {
+    float *__restrict__ sval32, bool pairs) {
   __shared__ int32_t lc[STAGE];
   __shared__ double lv[STAGE];
   const int t = threadIdx.x;
@@ -257,16 +269,17 @@ __global__ __launch_bounds__(256) void k_sell_fill(
   const int32_t *cp = staged ? lc + (rp - seg0) : colidx + rp;
   for (int k = 0; k < L; ++k) {
     const int64_t idx = base + (int64_t)k * kSlice;
+    const int64_t vidx = pairs ? sptr[s] + pair_pos(L, k, lane) : idx;
     if (k < rl) {
       const int32_t c = cp[k];
       const double a = staged ? lv[rp - seg0 + k] : (tmap ? vals[tmap[rp + k]] : vals[rp + k]);
       if (scol) scol[idx] = c;
-      if (sval) sval[idx] = a;
+      if (sval) sval[vidx] = a;
       if (sval32) sval32[idx] = (float)(a * dsc[r] * dsc[c]);
     } else {
       const int32_t c = r < n ? (int32_t)r : 0;
       if (scol) scol[idx] = c;
-      if (sval) sval[idx] = 0.;
+      if (sval) sval[vidx] = 0.;
       if (sval32) sval32[idx] = 0.f;
     }
   }
@@ -468,11 +481,100 @@ struct MatFmt {
   const int32_t *perm;   // kBsr3: node row of each sorted position
 };
 
+// Buffer resource over [p, p + bytes): a buffer load at a byte offset past
+// `bytes` returns 0 and touches no memory (raw buffer, stride 0).
+__device__ inline __amdgpu_buffer_rsrc_t slice_rsrc(const void *p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, (int)bytes, 0x00020000);
+}
+constexpr int kBufNT = 2;  // cache policy of the streamed matrix loads: non-temporal
+
+// Entries k >= U of a kSell16 FP64 row (rows longer than U), summed into acc
+// in entry order, four per round (one packed delta group).
+template <class ACC, class F>
+__device__ inline ACC sell16_tail(const int16_t *__restrict__ col16, const double *__restrict__ val,
+                                  int L, int lane, int32_t row0, F xval, ACC acc) {
+  for (int k = kU; k < L; k += 4) {
+    const u32x2 pk = ntload(reinterpret_cast<const u32x2 *>(col16 + (int64_t)(k >> 2) * (4 * kSlice) + 4 * lane));
+    const int32_t j[4] = {row0 + (int32_t)(int16_t)(pk.x & 0xffffu), row0 + (int32_t)(int16_t)(pk.x >> 16),
+                          row0 + (int32_t)(int16_t)(pk.y & 0xffffu), row0 + (int32_t)(int16_t)(pk.y >> 16)};
+    double a[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) a[t] = k + t < L ? ntload(val + pair_pos(L, k + t, lane)) : 0.;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) acc += (ACC)a[t] * xval(j[t]);
+  }
+  return acc;
+}
+
+// The FP64 kSell16 walk over this wave's slices.  A slice's values (entry
+// pairs, 16-byte loads) and packed deltas are read through buffer resources
+// that end at the slice's last entry, so the loads of a round are issued
+// unconditionally: entries past the slice width read as value 0, delta 0
+// (the row itself).  pre(r) (the row's own epilogue operands) is issued with
+// the gathers.  epi(r, acc, pre(r), r < n) is called for every lane (rows
+// past n only in the last slice, whose pre was given a clamped row).  The
+// products are summed in entry order.
+template <class ACC, class F, class P, class E>
+__device__ inline void sell16_walk(const MatFmt<double> &M, F xval, P pre, E epi) {
+  constexpr int U = kU, G4 = kU / 4, NP = kU / 2;
+  static_assert(kU % 4 == 0, "unroll by whole index groups");
+  int64_t s, s1, ds;
+  wave_slices<kCyclicSell>(M.S, s, s1, ds);
+  const int lane = threadIdx.x % kSlice;
+  for (; s < s1; s += ds) {
+    const int64_t vb = uniform(M.ptr[s]), cb = uniform(M.ptr16[s]);
+    const int L = __builtin_amdgcn_readfirstlane(M.len[s]);
+    const int Lh = L >> 1;
+    const int64_t r = s * kSlice + lane;
+    // rows past n (last slice) have only padding: their column is clamped
+    const int32_t r32 = (int32_t)(r < M.n ? r : M.n - 1);
+    const auto rc = slice_rsrc(M.col16 + cb, (uint32_t)((L + 3) >> 2) * (4 * kSlice) * 2);
+    const auto rp = slice_rsrc(M.val + vb, (uint32_t)Lh * (2 * kSlice) * 8);
+    const auto r1 = slice_rsrc(M.val + vb + (int64_t)Lh * (2 * kSlice), (uint32_t)(L & 1) * kSlice * 8);
+    u32x2 pk[G4];
+#pragma unroll
+    for (int g = 0; g < G4; ++g)
+      pk[g] = __builtin_amdgcn_raw_buffer_load_b64(rc, (g * kSlice + lane) * 8, 0, kBufNT);
+    u32x4 q[NP];
+#pragma unroll
+    for (int m = 0; m < NP; ++m)
+      q[m] = __builtin_amdgcn_raw_buffer_load_b128(rp, (m * kSlice + lane) * 16, 0, kBufNT);
+    const u32x2 odd = __builtin_amdgcn_raw_buffer_load_b64(r1, lane * 8, 0, kBufNT);
+    int32_t j[U];
+#pragma unroll
+    for (int g = 0; g < G4; ++g) {
+      j[4 * g + 0] = r32 + (int32_t)(int16_t)(pk[g].x & 0xffffu);
+      j[4 * g + 1] = r32 + (int32_t)(int16_t)(pk[g].x >> 16);
+      j[4 * g + 2] = r32 + (int32_t)(int16_t)(pk[g].y & 0xffffu);
+      j[4 * g + 3] = r32 + (int32_t)(int16_t)(pk[g].y >> 16);
+    }
+    double a[U];
+#pragma unroll
+    for (int m = 0; m < NP; ++m) {
+      const u32x2 lo = m == Lh ? odd : u32x2{q[m].x, q[m].y};
+      a[2 * m] = __builtin_bit_cast(double, lo);
+      a[2 * m + 1] = __builtin_bit_cast(double, u32x2{q[m].z, q[m].w});
+    }
+    ACC xv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) xv[u] = xval(j[u]);
+    const auto pv = pre(r32);
+    ACC acc = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) acc += (ACC)a[u] * xv[u];
+    if (L > U) acc = sell16_tail(M.col16 + cb, M.val + vb, L, lane, r32, xval, acc);
+    epi(r, acc, pv, r < M.n);
+  }
+}
+
 // Calls epi(r, sum_k a_rk X(j_k)) once for every row r of this block's range;
 // for kCsrV only the first lane of each 16-lane group calls it.
 template <int KIND, class V, class ACC, class F, class E>
 __device__ inline void matvec_rows(const MatFmt<V> &M, F xval, E epi) {
-  if (KIND == kSell || KIND == kSell16) {
+  if constexpr (KIND == kSell16 && sizeof(V) == 8) {
+    sell16_walk<ACC>(M, xval, [](int32_t) { return 0; },
+                     [&](int64_t r, ACC acc, int, bool row) { if (row) epi(r, acc); });
+  } else if constexpr (KIND == kSell || KIND == kSell16) {
     int64_t s0, s1, ds;
     wave_slices<kCyclicSell>(M.S, s0, s1, ds);
     const int lane = threadIdx.x % kSlice;
@@ -483,14 +585,14 @@ __device__ inline void matvec_rows(const MatFmt<V> &M, F xval, E epi) {
       // rows past n (last slice) have only padding: their column is clamped
       const int32_t r32 = (int32_t)(r < M.n ? r : M.n - 1);
       ACC acc;
-      if (KIND == kSell16)
+      if constexpr (KIND == kSell16)
         acc = sell_row16<kU, V, ACC>(M.col16 + uniform(M.ptr16[s]) + 4 * lane, M.val + base, L, r32,
                                      xval);
       else
         acc = sell_row<kU, V, ACC, int32_t>(M.col + base, M.val + base, L, r32, xval);
       if (r < M.n) epi(r, acc);
     }
-  } else if (KIND == kBsr3) {
+  } else if constexpr (KIND == kBsr3) {
     constexpr int U = 2;
     const int64_t S3 = (M.nb + kSlice - 1) / kSlice;
     int64_t s0, s1, ds;
@@ -566,6 +668,20 @@ __device__ inline void matvec_rows(const MatFmt<V> &M, F xval, E epi) {
       for (int off = kLanesPerRow / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kWave);
       if (r < r1 && lane == 0) epi(r, acc);
     }
+  }
+}
+
+// Calls epi(r, sum_k a_rk X(j_k), pre(r), row) for the rows r of this block's
+// range, row = false for the padding lanes of the last kSell16 slice (their
+// r is past n: epi stores nothing there, and pre was given a clamped row);
+// pre(r) (operands of the epilogue) is loaded ahead where the layout allows.
+// For kCsrV only the first lane of each 16-lane group calls epi.
+template <int KIND, class V, class ACC, class F, class P, class E>
+__device__ inline void matvec_rows_pre(const MatFmt<V> &M, F xval, P pre, E epi) {
+  if constexpr (KIND == kSell16 && sizeof(V) == 8) {
+    sell16_walk<ACC>(M, xval, pre, epi);
+  } else {
+    matvec_rows<KIND, V, ACC>(M, xval, [&](int64_t r, ACC acc) { epi(r, acc, pre(r), true); });
   }
 }
 
@@ -653,9 +769,11 @@ __device__ inline void block_rows4(int64_t r0, int64_t r1, int64_t &a4, int64_t 
 __global__ __launch_bounds__(kBlock) void k_pcg_dirvec(int c, int64_t n,
                                                        const double *__restrict__ Hold,
                                                        const double *__restrict__ CG,
+                                                       const double *__restrict__ G,
                                                        double *__restrict__ Hnew,
                                                        double *__restrict__ x,
-                                                       const double *__restrict__ pin, CgState *st) {
+                                                       const double *__restrict__ pin,
+                                                       double *__restrict__ pout, CgState *st) {
   __shared__ double lds[kBlock / kWave];
   if (st->done) return;  // set by an earlier launch only: every block of this one agrees
   double gamma;
@@ -664,14 +782,18 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dirvec(int c, int64_t n,
   const double rho = st->rho;
   int64_t r0, r1;
   block_rows(n, r0, r1);
-  if (go && step) {
+  if (go) {
+    // the new direction and its partial (g, h), the numerator of rho
+    double part = 0.;
     for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) {
       const double h = Hold[r];
-      x[r] += rho * h;
-      Hnew[r] = gamma * h - CG[r];
+      if (step) x[r] += rho * h;
+      const double hn = gamma * h - CG[r];
+      Hnew[r] = hn;
+      part += G[r] * hn;
     }
-  } else if (go) {
-    for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) Hnew[r] = gamma * Hold[r] - CG[r];
+    const double gh = block_sum<kBlock>(part, lds);
+    if (threadIdx.x == 0) pout[2 * xcd_block_id() + 1] = gh;
   } else if (step) {
     for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) x[r] += rho * Hold[r];
   }
@@ -680,7 +802,6 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dirvec(int c, int64_t n,
 // K2: AH = A H, partials (H, AH), (g, H)
 template <int KIND>
 __global__ FFHIP_MATVEC_ATTR void k_pcg_matvec(MatFmt<double> M, const double *__restrict__ H,
-                                               const double *__restrict__ G,
                                                double *__restrict__ AH, double *__restrict__ pout,
                                                CgState *st) {
   __shared__ double lds[kBlock / kWave];
@@ -689,22 +810,23 @@ __global__ FFHIP_MATVEC_ATTR void k_pcg_matvec(MatFmt<double> M, const double *_
     return;
   }
   if (st->done) return;
-  double p_hah = 0., p_gh = 0.;
-  matvec_rows<KIND, double, double>(
-      M, [&](int32_t j) { return H[j]; },
-      [&](int64_t r, double acc) {
-        const double h = H[r];
-        AH[r] = acc;
-        p_hah += h * acc;
-        p_gh += G[r] * h;
-      });
-  const double a = block_sum<kBlock>(p_hah, lds);
-  const double gh = block_sum<kBlock>(p_gh, lds);
-  if (threadIdx.x == 0) {
-    const int lb = xcd_block_id();
-    pout[2 * lb] = a;
-    pout[2 * lb + 1] = gh;
+  double p_hah = 0.;
+  auto epi = [&](int64_t r, double acc, double h, bool row) {
+    if (row) AH[r] = acc;
+    p_hah += (row ? h : 0.) * acc;
+  };
+  if constexpr (KIND == kSell16) {
+    // gathers through a buffer resource over H (n < 2^28 in this layout)
+    const auto hr = slice_rsrc(H, (uint32_t)(M.n * 8));
+    matvec_rows_pre<KIND, double, double>(
+        M, [&](int32_t j) { return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(hr, j * 8, 0, 0)); },
+        [&](int64_t r) { return H[r]; }, epi);
+  } else {
+    matvec_rows_pre<KIND, double, double>(M, [&](int32_t j) { return H[j]; }, [&](int64_t r) { return H[r]; },
+                                          epi);
   }
+  const double a = block_sum<kBlock>(p_hah, lds);
+  if (threadIdx.x == 0) pout[2 * xcd_block_id()] = a;  // (g, h) comes from the direction kernel
 }
 
 __global__ __launch_bounds__(kBlock) void k_pcg_update(
@@ -1021,7 +1143,14 @@ __global__ __launch_bounds__(kBlock) void k_gm_update(int64_t n, int k, const do
 // plain mat-vec for measurement: y = A x
 template <int KIND, class V, class X>
 __global__ FFHIP_MATVEC_ATTR void k_matvec(MatFmt<V> M, const X *__restrict__ x, X *__restrict__ y) {
-  matvec_rows<KIND, V, X>(M, [&](int32_t j) { return x[j]; }, [&](int64_t r, X acc) { y[r] = acc; });
+  if constexpr (KIND == kSell16 && sizeof(X) == 8) {
+    const auto xr = slice_rsrc(x, (uint32_t)(M.n * 8));  // n < 2^28 in this layout
+    matvec_rows<KIND, V, X>(
+        M, [&](int32_t j) { return __builtin_bit_cast(X, __builtin_amdgcn_raw_buffer_load_b64(xr, j * 8, 0, 0)); },
+        [&](int64_t r, X acc) { y[r] = acc; });
+  } else {
+    matvec_rows<KIND, V, X>(M, [&](int32_t j) { return x[j]; }, [&](int64_t r, X acc) { y[r] = acc; });
+  }
 }
 
 // ---------------------------------------------------------------- MIXED (FP32 inner)
@@ -1422,7 +1551,8 @@ static void cg_plan_layout(ffhip_cg cg, int layout, bool index32) {
         FFHIP_CHECK(hipGetLastError());
         far = read_flag(ctx, d);
       }
-      if (far) {
+      // 16-bit deltas, and gathers through buffer resources with 32-bit byte offsets
+      if (far || A->n >= ((int64_t)1 << 28)) {
         cg->fmt = kSell;
         cg->scol = dalloc<int32_t>(cg->ents);
       } else {
@@ -1488,7 +1618,7 @@ static void cg_fill_values(ffhip_cg cg, bool transposed) {
     k_sell_fill<4096><<<nblk(cg->S * kSlice, 256), 256, 0, s>>>(
         A->n, cg->S, A->rowptr, A->colidx, A->vals, tmap, cg->sptr, cg->slen, cg->dsc,
         transposed ? nullptr : cg->scol, cg->sptr16, transposed ? nullptr : cg->scol16, dv,
-        mixed ? fv : nullptr);
+        mixed ? fv : nullptr, cg->fmt == kSell16);
   } else {
     double *dst = nullptr;
     if (transposed) {
@@ -1571,10 +1701,10 @@ static void launch_pcg_chunk(ffhip_cg cg, const double *sval, bool marks) {
   const MatFmt<double> M = cg->fmt_of(sval);
   for (int c = 0; c < kChunk; ++c) {
     if (marks) cg->mark(4 * c);
-    k_pcg_dirvec<<<G, kBlock, 0, s>>>(c, cg->n, cg->H[c & 1], cg->CG, cg->H[(c + 1) & 1], cg->xw,
-                                      cg->pA, cg->st);
+    k_pcg_dirvec<<<G, kBlock, 0, s>>>(c, cg->n, cg->H[c & 1], cg->CG, cg->G, cg->H[(c + 1) & 1],
+                                      cg->xw, cg->pA, cg->pB, cg->st);
     if (marks) cg->mark(4 * c + 1);
-    FFHIP_LAUNCH_FMT(cg, k_pcg_matvec, G, s, M, cg->H[(c + 1) & 1], cg->G, cg->AH, cg->pB, cg->st);
+    FFHIP_LAUNCH_FMT(cg, k_pcg_matvec, G, s, M, cg->H[(c + 1) & 1], cg->AH, cg->pB, cg->st);
     if (marks) cg->mark(4 * c + 2);
     k_pcg_update<<<G, kBlock, 0, s>>>(cg->n, cg->AH, cg->dinv, cg->G, cg->CG, cg->pB, cg->pA,
                                       cg->st);

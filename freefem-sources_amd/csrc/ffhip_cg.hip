@@ -332,45 +332,74 @@ __global__ void k_bsr_len(int64_t nb, int64_t S3, const int64_t *__restrict__ bp
 
 // 3x3 blocks of a vector-P2 matrix from its scalar CSR: entry k of sorted node
 // row i (node a = perm[i]) holds rows 3a..3a+2 at offsets 3k..3k+2 of those
-// rows (values optionally through tmap; FP32 copy of D^-1/2 A D^-1/2)
-__global__ void k_bsr_fill(int64_t nb, int64_t ents, const int64_t *__restrict__ bptr,
-                           const int32_t *__restrict__ perm, const int64_t *__restrict__ sptr,
-                           const int32_t *__restrict__ slen, const int64_t *__restrict__ rowptr,
-                           const int32_t *__restrict__ colidx, const double *__restrict__ vals,
-                           const int64_t *__restrict__ tmap, const double *__restrict__ dsc,
-                           int32_t *__restrict__ bcol, double *__restrict__ bval,
-                           float *__restrict__ bval32) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t S3 = (nb + kSlice - 1) / kSlice;
-  if (i >= S3 * kSlice) return;
-  const int64_t s = i / kSlice;
-  const int lane = (int)(i % kSlice);
-  const int32_t a = i < nb ? perm[i] : 0;
-  const int64_t L = i < nb ? bptr[a + 1] - bptr[a] : 0;
+// rows (values optionally through tmap; FP32 copy of D^-1/2 A D^-1/2).
+// One block per 64-row slice: the slice's rows are read KC blocks at a time,
+// each of its 3 x 64 scalar rows as one contiguous run of 3 KC values, into
+// LDS, and written out plane by plane in slice order (every store
+// coalesced); padding entries get value 0 and the row's own block column.
+constexpr int kBsrKC = 8;
+__global__ __launch_bounds__(256) void k_bsr_fill(
+    int64_t nb, int64_t ents, const int64_t *__restrict__ bptr, const int32_t *__restrict__ perm,
+    const int64_t *__restrict__ sptr, const int32_t *__restrict__ slen,
+    const int64_t *__restrict__ rowptr, const int32_t *__restrict__ colidx,
+    const double *__restrict__ vals, const int64_t *__restrict__ tmap,
+    const double *__restrict__ dsc, int32_t *__restrict__ bcol, double *__restrict__ bval,
+    float *__restrict__ bval32) {
+  constexpr int KC = kBsrKC, P = kSlice + 1;  // P: padded lane stride (LDS banks)
+  __shared__ double T[9 * KC * P];
+  __shared__ int32_t Cb[KC * kSlice];
+  __shared__ int32_t An[kSlice], Ln[kSlice];
+  __shared__ int64_t R0[3 * kSlice];
+  const int64_t s = blockIdx.x;
+  const int t = threadIdx.x;
+  if (t < kSlice) {
+    const int64_t i = s * kSlice + t;
+    const int32_t a = i < nb ? perm[i] : -1;
+    An[t] = a;
+    Ln[t] = a >= 0 ? (int32_t)(bptr[a + 1] - bptr[a]) : 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) R0[3 * t + c] = a >= 0 ? rowptr[3 * (int64_t)a + c] : 0;
+  }
+  __syncthreads();
   const int W = slen[s];
-  for (int k = 0; k < W; ++k) {
-    const int64_t e = sptr[s] + (int64_t)k * kSlice + lane;
-    if (k < L) {
-      const int32_t c0 = colidx[rowptr[3 * (int64_t)a] + 3 * k];
-      if (bcol) bcol[e] = c0 / 3;
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const int64_t r = 3 * (int64_t)a + c, src = rowptr[r] + 3 * k;
-#pragma unroll
-        for (int cp = 0; cp < 3; ++cp) {
-          const double v = tmap ? vals[tmap[src + cp]] : vals[src + cp];
-          if (bval) bval[(3 * c + cp) * ents + e] = v;
-          if (bval32) bval32[(3 * c + cp) * ents + e] = (float)(v * dsc[r] * dsc[c0 + cp]);
-        }
+  const int64_t base = sptr[s];
+  for (int k0 = 0; k0 < W; k0 += KC) {
+    const int kc = W - k0 < KC ? W - k0 : KC;
+    // read: item = ((li * 3 + c) * 3 KC + x), x = 3 kk + cp
+    for (int it = t; it < kSlice * 9 * KC; it += 256) {
+      const int x = it % (3 * KC), rc = it / (3 * KC), c = rc % 3, li = rc / 3;
+      const int kk = x / 3, cp = x % 3, k = k0 + kk;
+      double v = 0.;
+      if (kk < kc && k < Ln[li]) {
+        const int64_t src = R0[3 * li + c] + 3 * (int64_t)k + cp;
+        v = tmap ? vals[tmap[src]] : vals[src];
       }
-    } else {
-      if (bcol) bcol[e] = a;
-#pragma unroll
-      for (int q = 0; q < 9; ++q) {
-        if (bval) bval[q * ents + e] = 0.;
-        if (bval32) bval32[q * ents + e] = 0.f;
+      T[((3 * c + cp) * KC + kk) * P + li] = v;
+    }
+    for (int it = t; it < KC * kSlice; it += 256) {
+      const int li = it % kSlice, kk = it / kSlice, k = k0 + kk;
+      Cb[it] = kk < kc && k < Ln[li] ? colidx[R0[3 * li] + 3 * (int64_t)k] / 3 : -1;
+    }
+    __syncthreads();
+    // write: plane q, entry kk, lane li
+    for (int it = t; it < 9 * kc * kSlice; it += 256) {
+      const int li = it % kSlice, kk = (it / kSlice) % kc, q = it / (kSlice * kc);
+      const int64_t e = base + (int64_t)(k0 + kk) * kSlice + li;
+      const double v = T[(q * KC + kk) * P + li];
+      if (bval) bval[q * ents + e] = v;
+      if (bval32) {
+        const int cb = Cb[kk * kSlice + li];
+        bval32[q * ents + e] =
+            cb < 0 ? 0.f : (float)(v * dsc[3 * (int64_t)An[li] + q / 3] * dsc[3 * (int64_t)cb + q % 3]);
       }
     }
+    if (bcol)
+      for (int it = t; it < kc * kSlice; it += 256) {
+        const int li = it % kSlice, kk = it / kSlice;
+        const int cb = Cb[kk * kSlice + li];
+        bcol[base + (int64_t)(k0 + kk) * kSlice + li] = cb >= 0 ? cb : (An[li] >= 0 ? An[li] : 0);
+      }
+    __syncthreads();
   }
 }
 
@@ -1610,7 +1639,7 @@ static void cg_fill_values(ffhip_cg cg, bool transposed) {
   if (cg->fmt == kBsr3) {
     if (!dv) dv = dalloc<double>(nval);
     const int64_t nb = A->n / 3, S3 = (nb + kSlice - 1) / kSlice;
-    k_bsr_fill<<<nblk(S3 * kSlice, 256), 256, 0, s>>>(
+    k_bsr_fill<<<S3, 256, 0, s>>>(
         nb, cg->ents, A->nodeptr, cg->bperm, cg->sptr, cg->slen, A->rowptr, A->colidx, A->vals, tmap,
         cg->dsc, transposed ? nullptr : cg->bcol, dv, mixed ? fv : nullptr);
   } else if (cg->fmt != kCsrV) {

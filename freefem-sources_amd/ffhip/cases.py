@@ -12,6 +12,11 @@ solve_p2: cube(n) Lame system, FP64 and MIXED solves in every solver layout,
 with per-kernel HIP-event timing.
 pcie_p1: the host-buffer path of the C-ABI on cube(n) P1: mesh upload from
 host arrays, device pattern and values, CSR download to host (wall times).
+plugin_p1: the sequence the FreeFEM plugin runs for `matrix A = va(Vh,Vh)`
+with kappa a P1 finite element function (wall times): host mesh upload, the
+assembly call with kappa's host coefficient array (FFHIP_COEF_P1), the tgv
+rows, the CSR download into host arrays (FreeFEM's matrix; the solver keeps
+the device copy).
 copy: the copy kernel (1 GiB read + 1 GiB written per launch) whose PMC byte
 counts scale the other kernels'.
 """
@@ -97,6 +102,41 @@ def pcie_p1(n=160, reps=2):
         A.free(); M.free()
 
 
+def plugin_p1(n=160, reps=2, quiet=False):
+    """returns the stage times (ms) of the last repetition"""
+    import time
+    import numpy as np
+    import ffhip
+    G = ffhip.cube(n)
+    m = G.download()
+    G.free()
+    X = m["xyz"].reshape(-1, 3)
+    kappa = 1 + 0.5 * np.sin(2 * np.pi * X[:, 0]) * np.sin(2 * np.pi * X[:, 1]) * np.sin(2 * np.pi * X[:, 2])
+    out = {}
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        M = ffhip.mesh3_upload(m["xyz"], m["elems"], m["belems"], m["blab"])
+        M.ctx.sync()
+        t1 = time.perf_counter()
+        A = ffhip.Matrix.p1_assembled(M, ffhip.COEF_P1, kappa_e=kappa)
+        M.ctx.sync()
+        t2 = time.perf_counter()
+        A.set_bc(LABELS)
+        M.ctx.sync()
+        t3 = time.perf_counter()
+        rp, ci, va = A.download()
+        t4 = time.perf_counter()
+        nnz = A.nnz
+        out = {"mesh_upload_ms": 1e3 * (t1 - t0), "assembly_with_coefficients_ms": 1e3 * (t2 - t1),
+               "tgv_rows_ms": 1e3 * (t3 - t2), "csr_download_ms": 1e3 * (t4 - t3),
+               "total_ms": 1e3 * (t4 - t0), "nnz": nnz, "Mnnz_per_s": nnz / (t4 - t0) / 1e6}
+        if not quiet:
+            print(f"plugin path cube({n}): " + ", ".join(f"{k} {v:.1f}" for k, v in out.items()), flush=True)
+        del rp, ci, va
+        A.free(); M.free()
+    return out
+
+
 def copy(mib=1024, reps=5):
     """the copy kernel the PMC byte counters are scaled by (reads and writes
     mib MiB per launch)"""
@@ -110,4 +150,4 @@ if __name__ == "__main__":
     case = sys.argv[1]
     args = [int(a) for a in sys.argv[2:]]
     {"assembly_p1": assembly_p1, "assembly_p2": assembly_p2, "solve_p2": solve_p2,
-     "pcie_p1": pcie_p1, "copy": copy}[case](*args)
+     "pcie_p1": pcie_p1, "plugin_p1": plugin_p1, "copy": copy}[case](*args)

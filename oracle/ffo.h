@@ -246,6 +246,21 @@ int ffo_p2_mark_bc(int nv, int64_t nt, const int32_t *tets, const int32_t *elem_
                    int64_t nbe, const int32_t *btri, const int32_t *blab, int nlab,
                    const int32_t *labs, int ncomp, int64_t ndof, uint8_t *onbc);
 
+/* ---- all-core variant of the 3D P1 path (ffo_omp.c, libffo_omp.so) ----- */
+struct ffo_omp_sys;
+int ffo_omp_max_threads(void);
+/* pattern + element-order values, bit-identical to ffo_pattern_* and
+ * ffo_assemble_p1_3d; NULL on failure */
+struct ffo_omp_sys *ffo_omp_assemble_p1_3d(int nv, int64_t nt, const double *xyz,
+                                           const int32_t *tets, int kind, double c0,
+                                           const double *kappa_e);
+int64_t ffo_omp_nnz(const struct ffo_omp_sys *S);
+void ffo_omp_arrays(const struct ffo_omp_sys *S, int64_t *rowptr, int32_t *colidx, double *vals);
+void ffo_omp_setbc(struct ffo_omp_sys *S, const uint8_t *onbc, double tgv);
+int ffo_omp_pcg(const struct ffo_omp_sys *S, const double *b, double *x, double eps, int itmax,
+                double tgv, double *stats);
+void ffo_omp_free(struct ffo_omp_sys *S);
+
 #ifdef __cplusplus
 }
 #endif

@@ -92,6 +92,70 @@ def lib():
     return _LIB
 
 
+_OMP = None
+
+
+def omp_lib():
+    """Load libffo_omp.so, the all-core (OpenMP) variant of the 3D P1 path."""
+    global _OMP
+    if _OMP is None:
+        L = ctypes.CDLL(os.path.join(HERE, "libffo_omp.so"))
+        sig = {
+            "ffo_omp_max_threads": (_i, []),
+            "ffo_omp_assemble_p1_3d": (_vp, [_i, _l, _f64p, _i32p, _i, _d, _vp]),
+            "ffo_omp_nnz": (_l, [_vp]),
+            "ffo_omp_arrays": (None, [_vp, _i64p, _i32p, _f64p]),
+            "ffo_omp_setbc": (None, [_vp, _u8p, _d]),
+            "ffo_omp_pcg": (_i, [_vp, _f64p, _f64p, _d, _i, _d, _f64p]),
+            "ffo_omp_free": (None, [_vp]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _OMP = L
+    return _OMP
+
+
+class OmpP1:
+    """3D P1 system assembled on all host cores (pattern, element-order values
+    bit-identical to pattern() + assemble_p1_3d())."""
+
+    def __init__(self, m, kind=COEF_SIN3, c0=1.0, kappa_e=None):
+        L = omp_lib()
+        ke = None if kappa_e is None else np.ascontiguousarray(kappa_e, np.float64)
+        self.h = L.ffo_omp_assemble_p1_3d(m["nv"], m["nt"], m["xyz"], m["tets"], kind, c0,
+                                          None if ke is None else ke.ctypes.data)
+        if not self.h:
+            raise RuntimeError("ffo_omp_assemble_p1_3d failed")
+        self.n = m["nv"]
+        self.nnz = L.ffo_omp_nnz(self.h)
+
+    def arrays(self):
+        rp = np.zeros(self.n + 1, np.int64)
+        ci = np.zeros(self.nnz, np.int32)
+        va = np.zeros(self.nnz)
+        omp_lib().ffo_omp_arrays(self.h, rp, ci, va)
+        return rp, ci, va
+
+    def set_bc(self, onbc, tgv=1e30):
+        omp_lib().ffo_omp_setbc(self.h, np.ascontiguousarray(onbc, np.uint8), tgv)
+
+    def pcg(self, b, eps=1e-10, itmax=0, tgv=1e30):
+        x = np.zeros(self.n)
+        st = np.zeros(4)
+        omp_lib().ffo_omp_pcg(self.h, np.ascontiguousarray(b, np.float64), x, eps, itmax, tgv, st)
+        return x, {"iters": int(st[0]), "gcg0": st[1], "gcg": st[2], "nret": int(st[3])}
+
+    def free(self):
+        if self.h:
+            omp_lib().ffo_omp_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.free()
+
+
 def ref_available():
     return os.path.exists(os.path.join(HERE, "_ref", "libffref_ref.so"))
 

@@ -42,6 +42,23 @@ def test_p1_kind_3d_matrix(with_mass):
     assert _close(A1.download()[2], A2.download()[2])
 
 
+@pytest.mark.parametrize("n", [6, 13])
+def test_p1_kind_one_call_assembly(n):
+    """the one-call pattern+values path the plugin takes, with kappa's host
+    coefficient array, against the QP kind fed the oracle's interpolation"""
+    mo = ffo.cube(n)
+    k = _u(mo["nv"], 3)
+    kq = ffo.p1_at_qp(mo, k)
+    M = ffhip.cube(n)
+    A1 = ffhip.Matrix.p1_assembled(M, ffhip.COEF_P1, 0.0, k)
+    A2 = ffhip.Matrix.p1(M)
+    A2.assemble_p1(ffhip.COEF_QP, 0.0, kq)
+    r1, c1, v1 = A1.download()
+    r2, c2, v2 = A2.download()
+    assert np.array_equal(r1, r2) and np.array_equal(c1, c2)
+    assert _close(v1, v2)
+
+
 def test_p1_kind_device_pointer_equals_host_pointer():
     import torch
     n = 5

@@ -41,13 +41,20 @@ constexpr int kSlice = 64;
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / kSlice;
 
-// Tuning of the streaming mat-vec kernels: unroll depth of a row, the waves
-// per SIMD the register allocation targets, and resident blocks per CU.
+// Tuning of the streaming mat-vec kernels: unroll depth of a row (sliced ELL;
+// 3x3-block rows with FP64 and with FP32 values), the waves per SIMD the
+// register allocation targets, and resident blocks per CU.
 #ifndef FFHIP_SELL_U
 #define FFHIP_SELL_U 16
 #endif
 #ifndef FFHIP_WPE
 #define FFHIP_WPE 3
+#endif
+#ifndef FFHIP_BSR_U
+#define FFHIP_BSR_U 4
+#endif
+#ifndef FFHIP_BSR_U32
+#define FFHIP_BSR_U32 8
 #endif
 #ifndef FFHIP_GRID_PER_CU
 #define FFHIP_GRID_PER_CU (FFHIP_WPE)
@@ -622,7 +629,7 @@ __device__ inline void matvec_rows(const MatFmt<V> &M, F xval, E epi) {
       if (r < M.n) epi(r, acc);
     }
   } else if constexpr (KIND == kBsr3) {
-    constexpr int U = 2;
+    constexpr int U = sizeof(V) == 8 ? FFHIP_BSR_U : FFHIP_BSR_U32;
     const int64_t S3 = (M.nb + kSlice - 1) / kSlice;
     int64_t s0, s1, ds;
     wave_slices<kCyclicBsr>(S3, s0, s1, ds);

@@ -300,6 +300,16 @@ __device__ inline double pick4(const double v[4], int i) {
   return i == 0 ? v[0] : (i == 1 ? v[1] : (i == 2 ? v[2] : v[3]));
 }
 
+// orders the LDS accesses of one wave (cross-lane reuse of the row staging)
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// element rows a node-row gather keeps in flight
+constexpr int kGatherD = 4;
+
 constexpr int kElemBlock = 256;
 constexpr int kElemWaves = kElemBlock / kWave;
 constexpr int kMs = 33;  // LDS row stride of M (doubles)
@@ -411,7 +421,7 @@ __global__ __launch_bounds__(kElemBlock) void k_p2_element(
         M[(16 + i) * kMs + 16 + j] = acc11[vv];
       }
     }
-    __syncthreads();
+    wave_lds_sync();  // M is this wave's own: the waves of the block run independently
     if (LAP && e < nt) {
       for (int o = lane; o < 100; o += kWave) {
         const int i = o / 10, j = o % 10;
@@ -430,17 +440,11 @@ __global__ __launch_bounds__(kElemBlock) void k_p2_element(
         Ke[(int64_t)inv[10 * e + i] * 90 + rem] = k;
       }
     }
-    __syncthreads();
+    wave_lds_sync();
   }
 }
 
 // ---------------------------------------------------------------- row gather
-// orders the LDS accesses of one wave (cross-lane reuse of the row staging)
-__device__ inline void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 __global__ __launch_bounds__(256) void k_p2v_gather(
     int64_t nn, const int64_t *__restrict__ nodeptr, const int64_t *__restrict__ n2e_ptr,
@@ -465,27 +469,37 @@ __global__ __launch_bounds__(256) void k_p2v_gather(
   const bool two = lane < 90 - kWave;
   wave_lds_sync();
   const int64_t k0 = n2e_ptr[a], k1 = n2e_ptr[a + 1];
-  // the node's blocks are contiguous (incidence order = element order):
-  // the next block's loads are issued before the current one is added
-  double v0 = 0., v1 = 0.;
-  int o0 = 0, o1 = 0;
-  if (k0 < k1) {
-    v0 = Ke[k0 * 90 + lane];
-    o0 = slot8[k0 * 10 + tj[0]];
-    if (two) { v1 = Ke[k0 * 90 + lane + kWave]; o1 = slot8[k0 * 10 + tj[1]]; }
-  }
-  for (int64_t k = k0; k < k1; ++k) {
-    double n0 = 0., n1 = 0.;
-    int m0 = 0, m1 = 0;
-    if (k + 1 < k1) {
-      n0 = Ke[(k + 1) * 90 + lane];
-      m0 = slot8[(k + 1) * 10 + tj[0]];
-      if (two) { n1 = Ke[(k + 1) * 90 + lane + kWave]; m1 = slot8[(k + 1) * 10 + tj[1]]; }
+  // the node's blocks are contiguous (incidence order = element order) and
+  // are added one after the other; they are loaded kGatherD at a time, the
+  // next group's loads issued before the current group is added
+  constexpr int D = kGatherD;
+  double va[D], vb[D], na[D], nb[D];
+  int oa[D], ob[D], ma[D], mb[D];
+  auto load = [&](int64_t kg, double (&A)[D], double (&B)[D], int (&OA)[D], int (&OB)[D]) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const int64_t k = kg + d;
+      A[d] = 0.; B[d] = 0.; OA[d] = 0; OB[d] = 0;
+      if (k < k1) {
+        A[d] = Ke[k * 90 + lane];
+        OA[d] = slot8[k * 10 + tj[0]];
+        if (two) { B[d] = Ke[k * 90 + lane + kWave]; OB[d] = slot8[k * 10 + tj[1]]; }
+      }
     }
-    acc[tdst[0] + 3 * o0] += v0;
-    if (two) acc[tdst[1] + 3 * o1] += v1;
-    wave_lds_sync();
-    v0 = n0; v1 = n1; o0 = m0; o1 = m1;
+  };
+  load(k0, va, vb, oa, ob);
+  for (int64_t kg = k0; kg < k1; kg += D) {
+    load(kg + D, na, nb, ma, mb);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      if (kg + d < k1) {  // wave-uniform
+        acc[tdst[0] + 3 * oa[d]] += va[d];
+        if (two) acc[tdst[1] + 3 * ob[d]] += vb[d];
+        wave_lds_sync();
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) { va[d] = na[d]; vb[d] = nb[d]; oa[d] = ma[d]; ob[d] = mb[d]; }
   }
   double *out = vals + 9 * p0;
   for (int t = lane; t < Wd; t += kWave) out[t] = acc[t];

@@ -241,10 +241,13 @@ __global__ void k_sell_len(int64_t n, int64_t S, const int64_t *__restrict__ row
 }
 
 // vals_src may be permuted through tmap (transposed values)
-// The block's 256 rows (4 slices) read their CSR segment once, coalesced, into
-// LDS (when it fits in STAGE entries), then every lane writes its row
-// transposed into the slice layout.
-template <int STAGE>
+// One block per 64-row slice: the slice's CSR segment is read once, coalesced,
+// into LDS (when it fits in kFillStage entries; otherwise the rows are read
+// from memory where they lie), then the block writes the slice layout plane
+// by plane, consecutive threads on consecutive rows, so every store is
+// coalesced (FP64 entry pairs as one 16-byte store).  Entries past a row's
+// end are zeros at the row's own column.
+constexpr int kFillStage = 2048;
 __global__ __launch_bounds__(256) void k_sell_fill(
     int64_t n, int64_t S, const int64_t *__restrict__ rowptr, const int32_t *__restrict__ colidx,
     const double *__restrict__ vals, const int64_t *__restrict__ tmap,
@@ -252,52 +255,75 @@ __global__ __launch_bounds__(256) void k_sell_fill(
     const double *__restrict__ dsc, int32_t *__restrict__ scol,
     const int64_t *__restrict__ sptr16, int16_t *__restrict__ scol16, double *__restrict__ sval,
     float *__restrict__ sval32, bool pairs) {
-  __shared__ int32_t lc[STAGE];
-  __shared__ double lv[STAGE];
+  __shared__ int32_t lc[kFillStage];
+  __shared__ double lv[kFillStage];
+  __shared__ int64_t rs[kSlice];  // row start (in the stage, or in the CSR)
+  __shared__ int32_t rl[kSlice];  // row length
   const int t = threadIdx.x;
-  const int64_t v0 = (int64_t)blockIdx.x * blockDim.x;
-  const int64_t vend = v0 + blockDim.x < n ? v0 + blockDim.x : n;
-  const int64_t seg0 = v0 < n ? rowptr[v0] : 0, seg1 = v0 < n ? rowptr[vend] : 0;
-  const bool staged = seg1 - seg0 <= STAGE;
-  if (staged) {
+  const int64_t sl = blockIdx.x;
+  const int64_t r0 = sl * kSlice;
+  const int64_t rend = r0 + kSlice < n ? r0 + kSlice : n;
+  const int64_t seg0 = r0 < n ? rowptr[r0] : 0, seg1 = r0 < n ? rowptr[rend] : 0;
+  const bool staged = seg1 - seg0 <= kFillStage;
+  if (t < kSlice) {
+    const int64_t r = r0 + t;
+    const int64_t p = r < n ? rowptr[r] : 0;
+    rs[t] = staged ? p - seg0 : p;
+    rl[t] = r < n ? (int32_t)(rowptr[r + 1] - p) : 0;
+  }
+  if (staged)
     for (int64_t i = t; i < seg1 - seg0; i += blockDim.x) {
       lc[i] = colidx[seg0 + i];
       lv[i] = tmap ? vals[tmap[seg0 + i]] : vals[seg0 + i];
     }
-  }
   __syncthreads();
-  const int64_t r = v0 + t;
-  if (r >= S * kSlice) return;
-  const int64_t s = r / kSlice;
-  const int lane = (int)(r % kSlice);
-  const int L = len[s];
-  const int64_t base = sptr[s] + lane;
-  const int64_t rp = r < n ? rowptr[r] : 0, rl = r < n ? rowptr[r + 1] - rp : 0;
-  const int32_t *cp = staged ? lc + (rp - seg0) : colidx + rp;
-  for (int k = 0; k < L; ++k) {
-    const int64_t idx = base + (int64_t)k * kSlice;
-    const int64_t vidx = pairs ? sptr[s] + pair_pos(L, k, lane) : idx;
-    if (k < rl) {
-      const int32_t c = cp[k];
-      const double a = staged ? lv[rp - seg0 + k] : (tmap ? vals[tmap[rp + k]] : vals[rp + k]);
+  const int L = len[sl];
+  const int64_t base = sptr[sl];
+  // entry k of lane li's row; past its end a zero at the row's own column
+  auto col = [&](int li, int k) -> int32_t {
+    if (k < rl[li]) return staged ? lc[rs[li] + k] : colidx[rs[li] + k];
+    const int64_t r = r0 + li;
+    return r < n ? (int32_t)r : 0;
+  };
+  auto val = [&](int li, int k) -> double {
+    if (k >= rl[li]) return 0.;
+    const int64_t q = rs[li] + k;
+    return staged ? lv[q] : (tmap ? vals[tmap[q]] : vals[q]);
+  };
+  if (sval && pairs) {
+    const int Lh = L >> 1;
+    for (int it = t; it < (Lh + (L & 1)) * kSlice; it += blockDim.x) {
+      const int li = it % kSlice, kp = it / kSlice;
+      if (kp < Lh)
+        *reinterpret_cast<double2 *>(sval + base + ((int64_t)kp * kSlice + li) * 2) =
+            double2{val(li, 2 * kp), val(li, 2 * kp + 1)};
+      else  // the odd last entry: its own plane after the pairs
+        sval[base + pair_pos(L, L - 1, li)] = val(li, L - 1);
+    }
+  }
+  for (int it = t; it < L * kSlice; it += blockDim.x) {
+    const int li = it % kSlice, k = it / kSlice;
+    const int64_t idx = base + it;  // entry k of lane li: k kSlice + li
+    if (sval && !pairs) sval[idx] = val(li, k);
+    if (scol || sval32) {
+      const int32_t c = col(li, k);
       if (scol) scol[idx] = c;
-      if (sval) sval[vidx] = a;
-      if (sval32) sval32[idx] = (float)(a * dsc[r] * dsc[c]);
-    } else {
-      const int32_t c = r < n ? (int32_t)r : 0;
-      if (scol) scol[idx] = c;
-      if (sval) sval[vidx] = 0.;
-      if (sval32) sval32[idx] = 0.f;
+      if (sval32)
+        sval32[idx] = k < rl[li] ? (float)(val(li, k) * dsc[r0 + li] * dsc[c]) : 0.f;
     }
   }
   if (scol16) {  // [g][lane][4] packed deltas, the group tail padded with 0: one 8-byte store per group
-    int16_t *dst = scol16 + sptr16[s] + 4 * lane;
-    for (int k = 0; k < L; k += 4) {
+    for (int it = t; it < ((L + 3) >> 2) * kSlice; it += blockDim.x) {
+      const int li = it % kSlice, g = it / kSlice;
+      const int64_t r = r0 + li;
       uint32_t h[4];
 #pragma unroll
-      for (int t = 0; t < 4; ++t) h[t] = k + t < rl ? (uint32_t)(uint16_t)(int16_t)(cp[k + t] - r) : 0u;
-      u32x2 pk = {h[0] | (h[1] << 16), h[2] | (h[3] << 16)};
-      *reinterpret_cast<u32x2 *>(dst + (k >> 2) * (4 * kSlice)) = pk;
+      for (int q = 0; q < 4; ++q) {
+        const int k = 4 * g + q;
+        h[q] = k < rl[li] ? (uint32_t)(uint16_t)(int16_t)(col(li, k) - r) : 0u;
+      }
+      const u32x2 pk = {h[0] | (h[1] << 16), h[2] | (h[3] << 16)};
+      *reinterpret_cast<u32x2 *>(scol16 + sptr16[sl] + (int64_t)it * 4) = pk;
     }
   }
 }
@@ -1651,7 +1677,7 @@ static void cg_fill_values(ffhip_cg cg, bool transposed) {
         cg->dsc, transposed ? nullptr : cg->bcol, dv, mixed ? fv : nullptr);
   } else if (cg->fmt != kCsrV) {
     if (!dv) dv = dalloc<double>(cg->ents);
-    k_sell_fill<4096><<<nblk(cg->S * kSlice, 256), 256, 0, s>>>(
+    k_sell_fill<<<cg->S, 256, 0, s>>>(
         A->n, cg->S, A->rowptr, A->colidx, A->vals, tmap, cg->sptr, cg->slen, cg->dsc,
         transposed ? nullptr : cg->scol, cg->sptr16, transposed ? nullptr : cg->scol16, dv,
         mixed ? fv : nullptr, cg->fmt == kSell16);

@@ -38,7 +38,7 @@ int ffo_omp_max_threads(void) { return omp_get_max_threads(); }
 static int build_v2e(int n, int64_t nt, const int32_t *tets, int64_t **pptr, int64_t **plist) {
   int64_t *ptr = (int64_t *)calloc((size_t)n + 1, sizeof(int64_t));
   int64_t *list = (int64_t *)malloc(sizeof(int64_t) * (size_t)(4 * nt + 1));
-  int64_t *cur = (int64_t *)malloc(sizeof(int64_t) * (size_t)n + 1);
+  int64_t *cur = (int64_t *)malloc(sizeof(int64_t) * ((size_t)n + 1));
   if (!ptr || !list || !cur) { free(ptr); free(list); free(cur); return -1; }
 #pragma omp parallel for schedule(static)
   for (int64_t k = 0; k < 4 * nt; ++k) {
@@ -101,8 +101,9 @@ struct ffo_omp_sys *ffo_omp_assemble_p1_3d(int nv, int64_t nt, const double *xyz
   for (int i = 0; i < nv; ++i)
     if (ptr[i + 1] - ptr[i] > maxinc) maxinc = (int)(ptr[i + 1] - ptr[i]);
   struct ffo_omp_sys *S = (struct ffo_omp_sys *)calloc(1, sizeof(*S));
+  if (S) S->rowptr = (int64_t *)malloc(sizeof(int64_t) * ((size_t)nv + 1));
+  if (!S || !S->rowptr) { free(S); free(ptr); free(list); return NULL; }
   S->n = nv;
-  S->rowptr = (int64_t *)malloc(sizeof(int64_t) * ((size_t)nv + 1));
   int err = 0;
   /* pattern: row lengths, prefix sum, columns */
 #pragma omp parallel

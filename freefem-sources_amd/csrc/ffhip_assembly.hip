@@ -274,6 +274,8 @@ __global__ __launch_bounds__(256) void k_pattern_half(
     int *__restrict__ any_big, PatternOut out, const double *__restrict__ xyz = nullptr,
     const double *__restrict__ Ce = nullptr) {
   constexpr int NH = 256 / kHalf;
+  // every LDS array below is per row (half-wave), so a wave orders its own
+  // accesses (wave_lds_sync) and the waves of a block run independently
   __shared__ int32_t hk[NH][kHash];      // open-addressing set of the row's columns
   __shared__ int32_t ucol[NH][kRowBuf];  // unique columns, then sorted
   __shared__ int32_t ucnt[NH];
@@ -315,7 +317,7 @@ __global__ __launch_bounds__(256) void k_pattern_half(
       Cv = Ce[e];
     }
   }
-  __syncthreads();
+  wave_lds_sync();
   // insert the candidates; the lane that claims a free slot appends the column.
   // The row's own vertex is in every incident element: only lane 0 inserts it
   // (a quarter fewer atomics; c[] keeps it for the element-to-row map below)
@@ -334,7 +336,7 @@ __global__ __launch_bounds__(256) void k_pattern_half(
       h = (h + 1) & (kHash - 1);
     }
   }
-  __syncthreads();
+  wave_lds_sync();
   const int len = ucnt[hid];
   if (len > kRowBuf) mine = false;
   const int64_t r0 = p0;
@@ -354,7 +356,7 @@ __global__ __launch_bounds__(256) void k_pattern_half(
       }
     }
   }
-  __syncthreads();
+  wave_lds_sync();
   int dpos = -1;
   if (mine) {
 #pragma unroll
@@ -371,7 +373,7 @@ __global__ __launch_bounds__(256) void k_pattern_half(
     const int t = __shfl_xor(dpos, off, kWave);
     dpos = t > dpos ? t : dpos;
   }
-  __syncthreads();
+  wave_lds_sync();
   if (v < n && hl == 0) {
     big[v] = mine ? 0 : 1;
     if (!mine) atomicOr(any_big, 1);
@@ -400,7 +402,7 @@ __global__ __launch_bounds__(256) void k_pattern_half(
     }
   }
   if constexpr (VALS) {  // the row's values, each slot summed by its lane in element order
-    __syncthreads();
+    wave_lds_sync();
     const int nq = mine ? deg : 0;
     const uint32_t me0 = (uint32_t)hl * 0x01010101u, me1 = (uint32_t)(hl + kHalf) * 0x01010101u;
     double acc0 = 0., acc1 = 0.;
@@ -869,7 +871,7 @@ __global__ __launch_bounds__(256) void k_p1_values_rows(
     for (int m = 0; m < 4; ++m) sv[hid][4 * hl + m] = r[m];
     so[hid][hl] = pk;
   }
-  __syncthreads();
+  wave_lds_sync();  // sv / so are this row's (half-wave's) own
   const uint32_t me0 = (uint32_t)hl * 0x01010101u, me1 = (uint32_t)(hl + kHalf) * 0x01010101u;
   double acc0 = 0., acc1 = 0.;
   for (int q = 0; q < deg; ++q) {

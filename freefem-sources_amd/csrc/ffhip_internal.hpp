@@ -201,6 +201,13 @@ __device__ inline double block_sum(double v, double *lds /* BLOCK/64 */) {
 // XCD-aware logical block id: the hardware dispatches block b to XCD b % 8;
 // logical ids are renumbered so each XCD owns a contiguous block range (and
 // so a contiguous row range whose x-gathers stay in that XCD's L2).
+// orders the LDS accesses of one wave (cross-lane reuse of the row staging)
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 __device__ inline int xcd_block_id() {
   const int G = gridDim.x, b = blockIdx.x;
   if (G % 8) return b;

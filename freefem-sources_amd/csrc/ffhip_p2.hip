@@ -300,12 +300,6 @@ __device__ inline double pick4(const double v[4], int i) {
   return i == 0 ? v[0] : (i == 1 ? v[1] : (i == 2 ? v[2] : v[3]));
 }
 
-// orders the LDS accesses of one wave (cross-lane reuse of the row staging)
-__device__ inline void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // element rows a node-row gather keeps in flight
 constexpr int kGatherD = 4;
@@ -437,7 +431,8 @@ __global__ __launch_bounds__(kElemBlock) void k_p2_element(
           k += mu * (M[(3 * i) * kMs + 3 * j] + M[(3 * i + 1) * kMs + 3 * j + 1] +
                      M[(3 * i + 2) * kMs + 3 * j + 2]);
         // node block i goes where its incidence entry sits in node i's list
-        Ke[(int64_t)inv[10 * e + i] * 90 + rem] = k;
+        // (streamed: non-temporal, the gather reads it once)
+        __builtin_nontemporal_store(k, &Ke[(int64_t)inv[10 * e + i] * 90 + rem]);
       }
     }
     wave_lds_sync();
@@ -481,9 +476,9 @@ __global__ __launch_bounds__(256) void k_p2v_gather(
       const int64_t k = kg + d;
       A[d] = 0.; B[d] = 0.; OA[d] = 0; OB[d] = 0;
       if (k < k1) {
-        A[d] = Ke[k * 90 + lane];
+        A[d] = __builtin_nontemporal_load(&Ke[k * 90 + lane]);
         OA[d] = slot8[k * 10 + tj[0]];
-        if (two) { B[d] = Ke[k * 90 + lane + kWave]; OB[d] = slot8[k * 10 + tj[1]]; }
+        if (two) { B[d] = __builtin_nontemporal_load(&Ke[k * 90 + lane + kWave]); OB[d] = slot8[k * 10 + tj[1]]; }
       }
     }
   };

@@ -68,7 +68,7 @@ def cpu_baseline(n=160, iters_1core=64):
     and the oracle's OpenMP variant on all host cores."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import ffo  # the checker / CPU baseline only
+    import ffo  # the oracle: the CPU baseline only
 
     m = ffo.cube(n)
     onbc = ffo.mark_bc(m["nv"], m["btri"], m["blab"], 3, LABELS)

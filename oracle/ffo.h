@@ -1,7 +1,7 @@
 /*
  * ffo.h — CPU restatement ("oracle") of FreeFEM's assembly + Jacobi-CG path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker: only tests/,
+ * TEST INFRASTRUCTURE ONLY.  This library is what the tests compare against: only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and only
  * to compare against or to time as the CPU baseline.  The product (libffhip)
  * never links, loads or calls it.

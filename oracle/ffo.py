@@ -2,7 +2,7 @@
 (oracle/_ref/libffref_ref.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py as the checker.  The product never imports it.
+cpu_baseline leg of bench.py, as what results are compared against.  The product never imports it.
 """
 import ctypes
 import os

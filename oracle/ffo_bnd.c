@@ -2,7 +2,7 @@
  * ffo_bnd.c — oracle restatement of FreeFEM's boundary integrals on P1:
  * int2d(Th3, labels)(alpha*u*v) in a varf and int2d(Th3, labels)(g*v) in a
  * linear form, and their int1d forms on 2D meshes (Robin / Neumann terms).
- * TEST INFRASTRUCTURE ONLY (see ffo.h): the checker, never the product.
+ * TEST INFRASTRUCTURE ONLY (see ffo.h): what the tests compare against, never the product.
  *
  * Follows the reference literally: each listed boundary element is mapped to
  * its element and local face (Mesh::BoundaryElement), the element's four

@@ -1,7 +1,7 @@
 /*
  * ffo_cg.c — oracle restatement of FreeFEM's sparse mat-vec and Jacobi-
  * preconditioned conjugate gradient (SolverCG + ConjugueGradient).
- * TEST INFRASTRUCTURE ONLY (see ffo.h): the checker, never the product.
+ * TEST INFRASTRUCTURE ONLY (see ffo.h): what the tests compare against, never the product.
  */
 #include <math.h>
 #include <stdlib.h>

@@ -1,7 +1,7 @@
 /*
  * ffo_fem.c — oracle restatement of FreeFEM's quadrature, P1 element matrices,
  * HashMatrix pattern/accumulation, tgv Dirichlet and right-hand side.
- * TEST INFRASTRUCTURE ONLY (see ffo.h): the checker, never the product.
+ * TEST INFRASTRUCTURE ONLY (see ffo.h): what the tests compare against, never the product.
  */
 #include <math.h>
 #include <stdlib.h>

@@ -1,6 +1,6 @@
 /*
  * ffo_mesh.c — oracle restatement of FreeFEM's structured mesh builders.
- * TEST INFRASTRUCTURE ONLY (see ffo.h): the checker, never the product.
+ * TEST INFRASTRUCTURE ONLY (see ffo.h): what the tests compare against, never the product.
  */
 #include "ffo.h"
 

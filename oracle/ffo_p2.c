@@ -2,7 +2,7 @@
  * ffo_p2.c — oracle restatement of FreeFEM's P2 node numbering, P2 basis and
  * the vector-P2 Lame (linear elasticity) element matrix, right-hand side and
  * clamped boundary.
- * TEST INFRASTRUCTURE ONLY (see ffo.h): the checker, never the product.
+ * TEST INFRASTRUCTURE ONLY (see ffo.h): what the tests compare against, never the product.
  */
 #include <math.h>
 #include <stdlib.h>

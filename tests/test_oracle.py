@@ -1,4 +1,4 @@
-"""Pins the oracle (CPU restatement) before it is trusted as the checker.
+"""Pins the oracle (CPU restatement) before the GPU results are compared with it.
 
 Each check is chosen so that a plausible slip in the restatement (a dropped
 quadrature point, a wrong face-label map, a transposed element matrix, a wrong

@@ -144,50 +144,44 @@ __global__ __launch_bounds__(256) void k_v2e_sort_half(int64_t n, const int64_t 
   if (hl < deg) v2e[k0 + hl] = x;
 }
 
-// one value per lane, sequence index = lane (64 entries)
-template <class T>
-__device__ inline T bitonic64(T x, int lane) {
-#pragma unroll
-  for (int k = 2; k <= 64; k <<= 1)
-#pragma unroll
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      const T p = __shfl_xor(x, j, kWave);
-      const bool up = (lane & k) == 0, lower = (lane & j) == 0;
-      x = (lower == up) ? tmin(x, p) : tmax(x, p);
+// Incidence buckets (up to 32 entries per dof, in no particular order; the
+// sort below puts them in element order) with the atomics aggregated per
+// block: the block's 256 items are counted per dof in an LDS hash set (an
+// item's rank among the block's items of its dof from the LDS atomicAdd),
+// then each distinct dof of the block reserves its positions with one
+// atomicAdd on its global count.
+constexpr int kBucketHash = 512;  // >= 2x the items of a block: probing always ends
+__global__ __launch_bounds__(256) void k_v2e_bucket_block(const int32_t *__restrict__ elems, int64_t m,
+                                                          uint32_t *__restrict__ deg,
+                                                          int32_t *__restrict__ bucket,
+                                                          int *__restrict__ ovf) {
+  __shared__ int32_t hkey[kBucketHash];
+  __shared__ uint32_t hcnt[kBucketHash];
+  __shared__ uint32_t hbase[kBucketHash];
+  const int t = threadIdx.x;
+  for (int i = t; i < kBucketHash; i += blockDim.x) { hkey[i] = -1; hcnt[i] = 0; }
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + t;
+  const int32_t v = i < m ? elems[i] : -1;
+  uint32_t h = 0, rank = 0;
+  if (v >= 0) {
+    h = ((uint32_t)v * 2654435761u) >> (32 - 9);
+    for (;;) {  // ends: at most 256 keys in 512 slots
+      const int32_t old = atomicCAS(&hkey[h], -1, v);
+      if (old == -1 || old == v) break;
+      h = (h + 1) & (kBucketHash - 1);
     }
-  return x;
-}
-
-// k_v2e_bucket with the atomics aggregated per wave: the wave's 64 items are
-// sorted by (dof, lane), each run of equal dofs takes its bucket positions
-// with one atomicAdd, and the items of a run keep their order
-__global__ __launch_bounds__(256) void k_v2e_bucket_wave(const int32_t *__restrict__ elems, int64_t m,
-                                                         uint32_t *__restrict__ deg,
-                                                         int32_t *__restrict__ bucket,
-                                                         int *__restrict__ ovf) {
-  const int lane = threadIdx.x % kWave;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t i0 = i - lane;
-  const uint32_t v = i < m ? (uint32_t)elems[i] : 0xffffffffu;
-  uint64_t key = ((uint64_t)v << 6) | (uint64_t)lane;
-  key = bitonic64(key, lane);
-  const uint32_t sv = (uint32_t)(key >> 6);
-  const uint32_t pv = (uint32_t)(__shfl_up(key, 1, kWave) >> 6);
-  const bool head = lane == 0 || sv != pv;
-  const uint64_t heads = __ballot(head);
-  // start of this lane's run: the highest head at or below the lane
-  const uint64_t upto = lane == 63 ? heads : (heads & ((2ull << lane) - 1));
-  const int start = 63 - __builtin_clzll(upto);
-  // end of the run: the lowest head above the start (or 64)
-  const uint64_t above = start == 63 ? 0ull : (heads & ~((2ull << start) - 1));
-  const int end = above ? __builtin_ctzll(above) : 64;
-  uint32_t base = 0;
-  if (head && sv != 0xffffffffu) base = atomicAdd(&deg[sv], (uint32_t)(end - start));
-  base = __shfl(base, start, kWave);
-  if (sv != 0xffffffffu) {
-    const uint32_t pos = base + (uint32_t)(lane - start);
-    const int32_t item = (int32_t)(i0 + (int64_t)(key & 63));
-    if (pos < 32) bucket[32 * (int64_t)sv + pos] = item;
+    rank = atomicAdd(&hcnt[h], 1u);
+  }
+  __syncthreads();
+  for (int q = t; q < kBucketHash; q += blockDim.x) {
+    const int32_t key = hkey[q];
+    if (key >= 0) hbase[q] = atomicAdd(&deg[key], hcnt[q]);
+  }
+  __syncthreads();
+  if (v >= 0) {
+    const uint32_t pos = hbase[h] + rank;
+    if (pos < 32) bucket[32 * (int64_t)v + pos] = (int32_t)i;
     else atomicOr(ovf, 1);
   }
 }
@@ -1173,7 +1167,7 @@ void build_incidence(ffhip_ctx ctx, const int32_t *elems, int64_t nt, int nloc, 
   FFHIP_CHECK(hipMemsetAsync(deg, 0, 4 * n, s));
   FFHIP_CHECK(hipMemsetAsync(deg64 + n, 0, 8, s));
   FFHIP_CHECK(hipMemsetAsync(flag, 0, sizeof(int), s));
-  k_v2e_bucket_wave<<<nblocks(M, 256), 256, 0, s>>>(elems, M, deg, bucket, flag);
+  k_v2e_bucket_block<<<nblocks(M, 256), 256, 0, s>>>(elems, M, deg, bucket, flag);
   k_u32_to_i64<<<nblocks(n, 256), 256, 0, s>>>(n, deg, deg64);
   exclusive_scan_i64(ctx, deg64, ptr, n + 1);
   FFHIP_CHECK(hipGetLastError());

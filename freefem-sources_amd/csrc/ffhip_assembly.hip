@@ -934,19 +934,45 @@ __global__ void k_tet_mes(int64_t nt, const double *__restrict__ xyz, const int3
   mes[e] = tet_mesure(v);
 }
 
-__global__ void k_rhs_p1_3d(int64_t n, const double *__restrict__ xyz,
+// Sum of a row's incidence terms in element order, one row per 32-lane
+// half-wave: lane q computes the term of incidence c0 + q of each chunk of 32
+// into its LDS slot, and every lane then adds the 32 slots in order (the order
+// of the element loop, so the sum rounds as FreeFEM's accumulation into the
+// row).  Slots past the row's end hold +0.0, and adding +0.0 leaves the sum
+// unchanged: a sum that starts at +0.0 is never -0.0 under round-to-nearest.
+template <class F>
+__device__ inline double row_sum_in_order(double *__restrict__ slot, int hl, int64_t deg, F term) {
+  double s = 0.;
+  for (int64_t c0 = 0; c0 < deg; c0 += kHalf) {
+    slot[hl] = c0 + hl < deg ? term(c0 + hl) : 0.;
+    wave_lds_sync();
+#pragma unroll
+    for (int q = 0; q < kHalf; ++q) s += slot[q];
+    wave_lds_sync();
+  }
+  return s;
+}
+
+// b_v = sum over the incident tets of int_K f lambda_v (Tet_5), one row per half-wave
+__global__ __launch_bounds__(256) void k_rhs_p1_3d(int64_t n, const double *__restrict__ xyz,
                             const int32_t *__restrict__ tets, const double *__restrict__ tmes,
                             const int64_t *__restrict__ v2e_ptr,
                             const int32_t *__restrict__ v2e, int kind, double f0,
                             const double *__restrict__ fcoef, const uint8_t *__restrict__ onbc,
                             double tgv, double g0, const double *__restrict__ gvals,
                             double *__restrict__ b) {
-  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= n) return;
-  if (onbc && onbc[v]) { b[v] = tgv * (gvals ? gvals[v] : g0); return; }
-  double s = 0.;
-  for (int64_t k = v2e_ptr[v]; k < v2e_ptr[v + 1]; ++k) {
-    const int32_t idx = v2e[k];
+  const int hl = threadIdx.x % kHalf;
+  const int64_t v = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kHalf;
+  if (v >= n) return;  // uniform in the half-wave
+  if (onbc && onbc[v]) {
+    if (hl == 0) b[v] = tgv * (gvals ? gvals[v] : g0);
+    return;
+  }
+  __shared__ double slots[256];
+  const int64_t k0 = v2e_ptr[v];
+  const double s = row_sum_in_order(slots + (threadIdx.x & ~(kHalf - 1)), hl, v2e_ptr[v + 1] - k0,
+                                    [&](int64_t k) {
+    const int32_t idx = v2e[k0 + k];
     const int64_t e = idx >> 2;
     const int l = idx & 3;
     double vv[4][3], fu[4] = {0., 0., 0., 0.};
@@ -977,9 +1003,9 @@ __global__ void k_rhs_p1_3d(int64_t n, const double *__restrict__ xyz,
       }
       r += (mes * qf5_weight(q)) * f * lam;
     }
-    s += r;
-  }
-  b[v] = s;
+    return r;
+  });
+  if (hl == 0) b[v] = s;
 }
 
 __global__ void k_rhs_p1_2d(int64_t n, const double *__restrict__ xy,
@@ -1048,18 +1074,24 @@ __global__ __launch_bounds__(256) void k_p1_misfit_elem(
 }
 
 // (M e)_v = sum over incident elements of |K|/20 (e_v + sum_K e_j); 0 on Dirichlet rows
-__global__ void k_p1_mass_apply(int64_t n, const double *__restrict__ xyz,
-                                const int32_t *__restrict__ tets, const int64_t *__restrict__ v2e_ptr,
-                                const int32_t *__restrict__ v2e, const double *__restrict__ u,
-                                const double *__restrict__ ud, const uint8_t *__restrict__ onbc,
-                                double *__restrict__ r) {
-  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= n) return;
-  if (onbc && onbc[v]) { r[v] = 0.; return; }
+__global__ __launch_bounds__(256) void k_p1_mass_apply(
+    int64_t n, const double *__restrict__ xyz, const int32_t *__restrict__ tets,
+    const int64_t *__restrict__ v2e_ptr, const int32_t *__restrict__ v2e,
+    const double *__restrict__ u, const double *__restrict__ ud, const uint8_t *__restrict__ onbc,
+    double *__restrict__ r) {
+  const int hl = threadIdx.x % kHalf;
+  const int64_t v = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kHalf;
+  if (v >= n) return;  // uniform in the half-wave
+  if (onbc && onbc[v]) {
+    if (hl == 0) r[v] = 0.;
+    return;
+  }
   const double ev = u[v] - ud[v];
-  double s = 0.;
-  for (int64_t k = v2e_ptr[v]; k < v2e_ptr[v + 1]; ++k) {
-    const int64_t e = v2e[k] >> 2;
+  __shared__ double slots[256];
+  const int64_t k0 = v2e_ptr[v];
+  const double s = row_sum_in_order(slots + (threadIdx.x & ~(kHalf - 1)), hl, v2e_ptr[v + 1] - k0,
+                                    [&](int64_t k) {
+    const int64_t e = v2e[k0 + k] >> 2;
     const int4 tv = reinterpret_cast<const int4 *>(tets)[e];
     const int32_t id[4] = {tv.x, tv.y, tv.z, tv.w};
     double vv[4][3], se = 0.;
@@ -1067,9 +1099,9 @@ __global__ void k_p1_mass_apply(int64_t n, const double *__restrict__ xyz,
       for (int kk = 0; kk < 3; ++kk) vv[i][kk] = xyz[3 * (int64_t)id[i] + kk];
       se += u[id[i]] - ud[id[i]];
     }
-    s += tet_mesure(vv) / 20. * (ev + se);
-  }
-  r[v] = s;
+    return tet_mesure(vv) / 20. * (ev + se);
+  });
+  if (hl == 0) r[v] = s;
 }
 
 // g_e = -|K| grad u_K . grad lambda_K
@@ -1278,7 +1310,7 @@ void misfit_run(ffhip_ctx ctx, ffhip_mesh m, const double *u, const double *ud,
                                                        nlab, b.onbc);
       onbc = b.onbc;
     }
-    k_p1_mass_apply<<<nblocks(m->nv, 256), 256, 0, s>>>(m->nv, m->xyz, m->elems, m->v2e_ptr, m->v2e,
+    k_p1_mass_apply<<<nblocks(m->nv * kHalf, 256), 256, 0, s>>>(m->nv, m->xyz, m->elems, m->v2e_ptr, m->v2e,
                                                         u, ud, onbc, r);
     FFHIP_CHECK(hipGetLastError());
   }
@@ -1709,7 +1741,7 @@ int ffhip_rhs_p1_form(ffhip_ctx ctx, ffhip_mesh m, int kind, double f0, const do
     ctx->tic();
     if (m->dim == 3) {
       k_tet_mes<<<nblocks(m->nt, 256), 256, 0, s>>>(m->nt, m->xyz, m->elems, tmes);
-      k_rhs_p1_3d<<<nblocks(n, 256), 256, 0, s>>>(n, m->xyz, m->elems, tmes, m->v2e_ptr, m->v2e, kind,
+      k_rhs_p1_3d<<<nblocks(n * kHalf, 256), 256, 0, s>>>(n, m->xyz, m->elems, tmes, m->v2e_ptr, m->v2e, kind,
                                                    f0, fd, onbc, tgv, g0, gd, bd);
     }
     else

@@ -145,6 +145,7 @@ __device__ inline ACC sell_row(const IT *__restrict__ colp, const V *__restrict_
 // round of U entries issues U/4 index loads ahead of the U value loads.
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // FP64 values of a kSell16 slice of width L are stored in entry pairs: pair m
 // (entries 2m, 2m+1) of lane l at 2 (64 m + l), so one 16-byte load per lane
@@ -368,9 +369,19 @@ __global__ void k_bsr_len(int64_t nb, int64_t S3, const int64_t *__restrict__ bp
 // rows (values optionally through tmap; FP32 copy of D^-1/2 A D^-1/2).
 // One block per 64-row slice: the slice's rows are read KC blocks at a time,
 // each of its 3 x 64 scalar rows as one contiguous run of 3 KC values, into
-// LDS, and written out plane by plane in slice order (every store
-// coalesced); padding entries get value 0 and the row's own block column.
+// LDS, and written out entry-major (the 9 x 64 values of block entry k of the
+// slice at 9 ptr[s] + 9 64 k, placed by bsr_pos); padding entries get value 0
+// and the row's own block column.
 constexpr int kBsrKC = 8;
+// Position of value q (3c + c') of lane li inside one slice entry of the
+// 3x3-block values (9 x 64 values per entry): values 0..7 in vectors of W = 16
+// bytes / sizeof(value) per lane (FP64 pairs, FP32 quads), [vector][lane][W],
+// then value 8 of the 64 lanes.
+template <class V>
+__host__ __device__ constexpr int bsr_pos(int q, int li) {
+  constexpr int W = 16 / (int)sizeof(V);
+  return q < 8 ? ((q / W) * kSlice + li) * W + q % W : 8 * kSlice + li;
+}
 __global__ __launch_bounds__(256) void k_bsr_fill(
     int64_t nb, int64_t ents, const int64_t *__restrict__ bptr, const int32_t *__restrict__ perm,
     const int64_t *__restrict__ sptr, const int32_t *__restrict__ slen,
@@ -414,15 +425,15 @@ __global__ __launch_bounds__(256) void k_bsr_fill(
       Cb[it] = kk < kc && k < Ln[li] ? colidx[R0[3 * li] + 3 * (int64_t)k] / 3 : -1;
     }
     __syncthreads();
-    // write: plane q, entry kk, lane li
+    // write: entry kk, value q, lane li (one contiguous run of 9 x 64 kc values)
     for (int it = t; it < 9 * kc * kSlice; it += 256) {
-      const int li = it % kSlice, kk = (it / kSlice) % kc, q = it / (kSlice * kc);
-      const int64_t e = base + (int64_t)(k0 + kk) * kSlice + li;
+      const int li = it % kSlice, q = (it / kSlice) % 9, kk = it / (9 * kSlice);
+      const int64_t eb = 9 * (base + (int64_t)(k0 + kk) * kSlice);
       const double v = T[(q * KC + kk) * P + li];
-      if (bval) bval[q * ents + e] = v;
+      if (bval) bval[eb + bsr_pos<double>(q, li)] = v;
       if (bval32) {
         const int cb = Cb[kk * kSlice + li];
-        bval32[q * ents + e] =
+        bval32[eb + bsr_pos<float>(q, li)] =
             cb < 0 ? 0.f : (float)(v * dsc[3 * (int64_t)An[li] + q / 3] * dsc[3 * (int64_t)cb + q % 3]);
       }
     }
@@ -434,6 +445,23 @@ __global__ __launch_bounds__(256) void k_bsr_fill(
       }
     __syncthreads();
   }
+}
+
+// First slice of each of nw waves when the S3 slices are split over them by
+// entries: wave w takes the slices that start in [w E / nw, (w+1) E / nw),
+// E = sptr[S3].
+__global__ void k_bsr_waves(int64_t S3, const int64_t *__restrict__ sptr, int nw,
+                            int32_t *__restrict__ wstart) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w > nw) return;
+  const int64_t e = w == nw ? sptr[S3] : (int64_t)w * sptr[S3] / nw;
+  int64_t lo = 0, hi = S3;  // first slice s with sptr[s] >= e
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) / 2;
+    if (sptr[mid] < e) lo = mid + 1;
+    else hi = mid;
+  }
+  wstart[w] = (int32_t)lo;
 }
 
 // flags a row with a column farther than 16 bits from the row index
@@ -528,8 +556,9 @@ __global__ void k_init_bc(int64_t n, const uint8_t *__restrict__ wcl, const doub
 //   kBsr3:   3x3 blocks of a vector-P2 matrix, sliced ELL over node rows:
 //            the node rows are sorted by length inside windows of 4096 rows
 //            (so a 64-row slice pads little), one lane per node row (3 scalar
-//            rows), one int32 block column and 9 value planes per slice entry
-//            — 76 B per 9 FP64 entries instead of 108, every load coalesced.
+//            rows), one int32 block column and 9 values per slice entry, the
+//            values of a slice entry-major (9 x 64 per entry) — 76 B per 9
+//            FP64 entries instead of 108, every load coalesced.
 template <class V>
 struct MatFmt {
   int64_t n, S;
@@ -538,9 +567,11 @@ struct MatFmt {
   const int32_t *col;  // kSell, kCsrV
   const int16_t *col16;  // kSell16: col - row, packed [group of 4][lane][4] per slice
   const int64_t *ptr16;  // kSell16: slice offsets of col16
-  const V *val;          // kBsr3: 9 planes of ents values (entry 3c+c' of every block)
+  const V *val;          // kBsr3: 9 ents values, entry k of slice s at 9 ptr[s] + 9 64 k (bsr_pos)
   int64_t nb, ents;      // kBsr3: node rows, slice entries (ptr/len/col: slices, perm: rows)
   const int32_t *perm;   // kBsr3: node row of each sorted position
+  const int32_t *wstart; // kBsr3: first slice of each of the nw waves (nw + 1), split by entries
+  int nw;
 };
 
 // Buffer resource over [p, p + bytes): a buffer load at a byte offset past
@@ -655,43 +686,92 @@ __device__ inline void matvec_rows(const MatFmt<V> &M, F xval, E epi) {
       if (r < M.n) epi(r, acc);
     }
   } else if constexpr (KIND == kBsr3) {
+    // Groups of U block entries; the next group's block columns are loaded
+    // with the current group's values, so the gathers of x need no second
+    // round trip.  The slice's columns and values are read through buffer
+    // resources bounded to the slice: past its width they read 0 without a
+    // memory access, and those blocks' sums are dropped by a uniform select.
     constexpr int U = sizeof(V) == 8 ? FFHIP_BSR_U : FFHIP_BSR_U32;
     const int64_t S3 = (M.nb + kSlice - 1) / kSlice;
     int64_t s0, s1, ds;
-    wave_slices<kCyclicBsr>(S3, s0, s1, ds);
+    if (M.wstart && (int)gridDim.x * kWavesPerBlock == M.nw) {
+      // slices split over the waves by entries (rows of vertex and of edge
+      // nodes differ in length by about 2.5x)
+      const int wg = xcd_block_id() * kWavesPerBlock + wave_id();
+      s0 = __builtin_amdgcn_readfirstlane(M.wstart[wg]);
+      s1 = __builtin_amdgcn_readfirstlane(M.wstart[wg + 1]);
+      ds = 1;
+    } else {
+      wave_slices<kCyclicBsr>(S3, s0, s1, ds);
+    }
     const int lane = threadIdx.x % kSlice;
     for (int64_t s = s0; s < s1; s += ds) {
       const int64_t i = s * kSlice + lane;
       const int32_t a = i < M.nb ? M.perm[i] : -1;
-      const int64_t base = uniform(M.ptr[s]) + lane;
+      const int64_t sb = uniform(M.ptr[s]);
       const int L = __builtin_amdgcn_readfirstlane(M.len[s]);
+      const auto rc = slice_rsrc(M.col + sb, (uint32_t)L * kSlice * 4);
+      const auto rv = slice_rsrc(M.val + 9 * sb, (uint32_t)L * (9 * kSlice) * (uint32_t)sizeof(V));
       ACC s0v = 0, s1v = 0, s2v = 0;
+      int32_t bn[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        bn[u] = (int32_t)__builtin_amdgcn_raw_buffer_load_b32(rc, (u * kSlice + lane) * 4, 0, kBufNT);
       for (int k = 0; k < L; k += U) {
         int32_t b[U];
+        ACC xv[U][3];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          b[u] = bn[u];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) xv[u][c] = xval(3 * b[u] + c);
+        }
         V wv[U][9];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          b[u] = a < 0 ? 0 : a;  // past the row end: the row's own block, zero values
+          constexpr int W = 16 / (int)sizeof(V), Z = (int)sizeof(V);
+          const int eb = 9 * kSlice * (k + u);
 #pragma unroll
-          for (int q = 0; q < 9; ++q) wv[u][q] = 0;
-          if (k + u < L) {
-            const int64_t e = base + (int64_t)(k + u) * kSlice;
-            b[u] = ntload(M.col + e);
-#pragma unroll
-            for (int q = 0; q < 9; ++q) wv[u][q] = ntload(M.val + q * M.ents + e);
+          for (int g = 0; g < 8 / W; ++g) {
+            const u32x4 r = __builtin_amdgcn_raw_buffer_load_b128(rv, (eb + bsr_pos<V>(g * W, lane)) * Z, 0, kBufNT);
+            if constexpr (sizeof(V) == 8) {
+              wv[u][2 * g] = __builtin_bit_cast(V, u32x2{r.x, r.y});
+              wv[u][2 * g + 1] = __builtin_bit_cast(V, u32x2{r.z, r.w});
+            } else {
+              // the whole vector is reinterpreted (a bit cast of one component
+              // lvalue, r.y, reads the vector's first component)
+              const f32x4 f = __builtin_bit_cast(f32x4, r);
+              wv[u][4 * g] = f.x;
+              wv[u][4 * g + 1] = f.y;
+              wv[u][4 * g + 2] = f.z;
+              wv[u][4 * g + 3] = f.w;
+            }
           }
+          const int o8 = (eb + bsr_pos<V>(8, lane)) * Z;
+          if constexpr (sizeof(V) == 8)
+            wv[u][8] = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(rv, o8, 0, kBufNT));
+          else
+            wv[u][8] = __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b32(rv, o8, 0, kBufNT));
         }
-        ACC xv[U][3];
 #pragma unroll
         for (int u = 0; u < U; ++u)
-#pragma unroll
-          for (int c = 0; c < 3; ++c) xv[u][c] = xval(3 * b[u] + c);
+          bn[u] = (int32_t)__builtin_amdgcn_raw_buffer_load_b32(rc, ((k + U + u) * kSlice + lane) * 4, 0,
+                                                                kBufNT);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          s0v += (ACC)wv[u][0] * xv[u][0] + (ACC)wv[u][1] * xv[u][1] + (ACC)wv[u][2] * xv[u][2];
-          s1v += (ACC)wv[u][3] * xv[u][0] + (ACC)wv[u][4] * xv[u][1] + (ACC)wv[u][5] * xv[u][2];
-          s2v += (ACC)wv[u][6] * xv[u][0] + (ACC)wv[u][7] * xv[u][1] + (ACC)wv[u][8] * xv[u][2];
+          const ACC t0 = (ACC)wv[u][0] * xv[u][0] + (ACC)wv[u][1] * xv[u][1] + (ACC)wv[u][2] * xv[u][2];
+          const ACC t1 = (ACC)wv[u][3] * xv[u][0] + (ACC)wv[u][4] * xv[u][1] + (ACC)wv[u][5] * xv[u][2];
+          const ACC t2 = (ACC)wv[u][6] * xv[u][0] + (ACC)wv[u][7] * xv[u][1] + (ACC)wv[u][8] * xv[u][2];
+          const bool in = k + u < L;
+          s0v += in ? t0 : (ACC)0;
+          s1v += in ? t1 : (ACC)0;
+          s2v += in ? t2 : (ACC)0;
         }
+        // keeps the next group's column loads in this iteration (the compiler
+        // would otherwise load them again at the top of the next one and wait
+        // for them there before the gathers)
+#pragma unroll
+        for (int u = 0; u < U; ++u) asm volatile("" : "+v"(bn[u]));
       }
       if (a >= 0) {
         epi(3 * (int64_t)a, s0v);
@@ -1430,6 +1510,7 @@ struct ffhip_cg_s {
   int64_t *sptr16 = nullptr;  // kSell16 slice offsets of scol16
   int32_t *bcol = nullptr;    // kBsr3 block columns (slices over sorted node rows)
   int32_t *bperm = nullptr;   // kBsr3 node row of each sorted position
+  int32_t *bwave = nullptr;   // kBsr3 first slice of each mat-vec wave (grid x 4 + 1)
   // values in the solver's layout; for kCsrV, sval is A's value array (borrowed)
   double *sval = nullptr;    // A
   double *svalT = nullptr;   // A^T (built on first transposed solve)
@@ -1534,6 +1615,8 @@ struct ffhip_cg_s {
     if (fmt == kSell || fmt == kSell16) { M.ptr = sptr; M.len = slen; M.col = scol; }
     else if (fmt == kBsr3) { M.ptr = sptr; M.len = slen; M.col = bcol; M.nb = n / 3; }
     else { M.ptr = A->rowptr; M.len = nullptr; M.col = A->colidx; }
+    M.wstart = fmt == kBsr3 ? bwave : nullptr;
+    M.nw = fmt == kBsr3 && bwave ? grid * kWavesPerBlock : 0;
     M.val = val;
     return M;
   }
@@ -1649,6 +1732,13 @@ static void cg_plan_layout(ffhip_cg cg, int layout, bool index32) {
   if (g >= 64) g -= g % 8;  // whole XCD groups
   if (g < 1) g = 1;
   cg->grid = (int)g;
+  if (cg->fmt == kBsr3) {
+    const int nw = cg->grid * kWavesPerBlock;
+    cg->bwave = dalloc<int32_t>(nw + 1);
+    k_bsr_waves<<<nblk(nw + 1, 256), 256, 0, s>>>((A->n / 3 + kSlice - 1) / kSlice, cg->sptr, nw,
+                                                  cg->bwave);
+    FFHIP_CHECK(hipGetLastError());
+  }
 }
 
 // values in the solver's layout (A or A^T; FP32 scaled copy for MIXED)
@@ -1882,7 +1972,7 @@ static void cg_release(ffhip_cg cg) {
   for (hipEvent_t e : cg->ev) (void)hipEventDestroy(e);
   for (auto &g : cg->graphs) (void)hipGraphExecDestroy(g.exec);
   if (cg->fmt == kCsrV) cg->sval = nullptr;  // A's array
-  dfree(cg->sptr); dfree(cg->slen); dfree(cg->scol); dfree(cg->scol16); dfree(cg->sptr16); dfree(cg->bcol); dfree(cg->bperm); dfree(cg->sval);
+  dfree(cg->sptr); dfree(cg->slen); dfree(cg->scol); dfree(cg->scol16); dfree(cg->sptr16); dfree(cg->bcol); dfree(cg->bperm); dfree(cg->bwave); dfree(cg->sval);
   dfree(cg->svalT);
   dfree(cg->sval32); dfree(cg->sval32T); dfree(cg->dinv); dfree(cg->dsc); dfree(cg->wcl);
   dfree(cg->xw); dfree(cg->bw); dfree(cg->G); dfree(cg->CG); dfree(cg->H[0]); dfree(cg->H[1]);

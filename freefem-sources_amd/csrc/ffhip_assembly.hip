@@ -1381,7 +1381,6 @@ ffhip_mat pattern_p1(ffhip_ctx ctx, ffhip_mesh m, const double *Ce, hipEvent_t c
     A->diag = dalloc<int64_t>(n);
     A->slotmap = dalloc<uint32_t>(m->nt * m->nloc);
     A->big = dalloc<uint8_t>(n);
-    if (ce_ready) FFHIP_CHECK(hipStreamWaitEvent(s, ce_ready, 0));
     bool direct_ok = false;
     {  // direct pass
       Arena ar(ctx, Arena::bytes(n, 4) + Arena::bytes(n + 1, 8) + Arena::bytes(2, 4));
@@ -1402,6 +1401,9 @@ ffhip_mat pattern_p1(ffhip_ctx ctx, ffhip_mesh m, const double *Ce, hipEvent_t c
       PatternOut out;
       out.rowptr = A->rowptr; out.colidx = A->colidx; out.diag = A->diag; out.vals = A->vals;
       out.mismatch = flags + 1;
+      // the element scales are needed from here on (the buffered passes come
+      // after this in stream order)
+      if (ce_ready) FFHIP_CHECK(hipStreamWaitEvent(s, ce_ready, 0));
       if (vals_pass)
         k_pattern_half<4, true, true><<<nblocks(n * kHalf, 256), 256, 0, s>>>(
             n, m->elems, m->v2e_ptr, m->v2e, A->slotmap, A->big, flags, out, m->xyz, Ce);
@@ -1534,6 +1536,7 @@ int ffhip_pattern_assemble_p1(ffhip_ctx ctx, ffhip_mesh m, int kind, double c0, 
     const double *kd = kappa_e;
     double *kbuf = nullptr, *Ce = nullptr;
     ffhip_mat A = nullptr;
+    bool side_running = false;
     try {
       if (arr && !kappa_on_device) {
         const int64_t cnt = kind == FFHIP_COEF_P1 ? m->nv : kind == FFHIP_COEF_P0 ? m->nt : 14 * m->nt;
@@ -1543,18 +1546,26 @@ int ffhip_pattern_assemble_p1(ffhip_ctx ctx, ffhip_mesh m, int kind, double c0, 
       }
       Ce = dalloc<double>(m->nt);
       ctx->tic();
-      // the element scales, then the incidence lists and the pattern pass,
-      // which forms the row values from them
-      k_p1_scale_3d<<<nblocks(m->nt, 256), 256, 0, s>>>(m->nt, m->xyz, m->elems, kind, c0, kd, Ce);
+      // the element scales (arithmetic: the coefficient at 14 points per
+      // tet) on the side stream, beside the incidence lists and row pointers
+      // (memory and atomics) on the main one; the pattern pass, which forms
+      // the row values from the scales, waits for them
+      hipStream_t ss = ctx->side_stream();
+      FFHIP_CHECK(hipEventRecord(ctx->ev_fork, s));
+      FFHIP_CHECK(hipStreamWaitEvent(ss, ctx->ev_fork, 0));
+      side_running = true;
+      k_p1_scale_3d<<<nblocks(m->nt, 256), 256, 0, ss>>>(m->nt, m->xyz, m->elems, kind, c0, kd, Ce);
       FFHIP_CHECK(hipGetLastError());
+      FFHIP_CHECK(hipEventRecord(ctx->ev_join, ss));
       bool done = false;
-      A = pattern_p1(ctx, m, Ce, nullptr, &done);
+      A = pattern_p1(ctx, m, Ce, ctx->ev_join, &done);
       ctx->toc();
       if (!done) {  // rows for the lane-per-row kernels: all values by the element-row path
         const int rc = ffhip_assemble_p1(ctx, m, A, kind, c0, kd, 1);
         FFHIP_REQUIRE(rc == FFHIP_OK, rc, "ffhip_pattern_assemble_p1: %s", ffhip_last_error());
       }
     } catch (...) {
+      if (side_running) (void)hipStreamSynchronize(ctx->side);
       (void)hipStreamSynchronize(s);
       dfree(Ce); dfree(kbuf);
       if (A) ffhip_mat_free(A);

@@ -261,12 +261,16 @@ struct PatternOut {
 
 // VALS (3D P1 stiffness): the same pass also forms the row's values from the
 // element scales Ce (the rows of k_p1_values_rows).
+//
+// With bucket (the incidence lists unsorted, 32 entries per row, every list
+// within its bucket): the half-wave sorts its row's list into element order
+// and writes it to v2e, in place of k_v2e_sort_bucket.
 template <int NLOC, bool VALS, bool DIRECT>
 __global__ __launch_bounds__(256) void k_pattern_half(
     int64_t n, const int32_t *__restrict__ elems, const int64_t *__restrict__ v2e_ptr,
-    const int32_t *__restrict__ v2e, uint32_t *__restrict__ slotmap, uint8_t *__restrict__ big,
+    int32_t *__restrict__ v2e, uint32_t *__restrict__ slotmap, uint8_t *__restrict__ big,
     int *__restrict__ any_big, PatternOut out, const double *__restrict__ xyz = nullptr,
-    const double *__restrict__ Ce = nullptr) {
+    const double *__restrict__ Ce = nullptr, const int32_t *__restrict__ bucket = nullptr) {
   constexpr int NH = 256 / kHalf;
   // every LDS array below is per row (half-wave), so a wave orders its own
   // accesses (wave_lds_sync) and the waves of a block run independently
@@ -292,8 +296,15 @@ __global__ __launch_bounds__(256) void k_pattern_half(
   int64_t e = 0;
   int lv = 0;
   double vv[VALS ? 4 : 1][3], Cv = 0.;
+  int32_t idx = INT_MAX;
+  if (bucket) {  // every lane of the half-wave takes part in the sort
+    idx = hl < deg ? bucket[32 * v + hl] : INT_MAX;
+    idx = bitonic32(idx, hl);
+    if (hl < deg) v2e[k0 + hl] = idx;
+  } else if (hl < deg) {
+    idx = v2e[k0 + hl];
+  }
   if (hl < deg) {
-    const int32_t idx = v2e[k0 + hl];
     e = idx / NLOC;
     lv = idx - (int)e * NLOC;
     if (NLOC == 4) {
@@ -1184,14 +1195,16 @@ static void upload_t5_once(int dev) {
 // dof -> (element * nloc + local) incidence lists in element order, for any
 // element-to-dof table (P1 vertices, P2 nodes)
 void build_incidence(ffhip_ctx ctx, const int32_t *elems, int64_t nt, int nloc, int64_t n,
-                     int64_t **ptr_out, int32_t **list_out, int *big_out, int32_t **inv_out) {
+                     int64_t **ptr_out, int32_t **list_out, int *big_out, int32_t **inv_out,
+                     int32_t **bucket_out) {
   hipStream_t s = ctx->stream;
   const int64_t M = nt * nloc;
-  Arena ar(ctx, Arena::bytes(n, 4) + Arena::bytes(n + 1, 8) + Arena::bytes(32 * n, 4) +
+  Arena ar(ctx, Arena::bytes(n, 4) + Arena::bytes(n + 1, 8) + (bucket_out ? 0 : Arena::bytes(32 * n, 4)) +
                     Arena::bytes(1, 4) + Arena::bytes(n, 4));
   uint32_t *deg = ar.take<uint32_t>(n);
   int64_t *deg64 = ar.take<int64_t>(n + 1);
-  int32_t *bucket = ar.take<int32_t>(32 * n);
+  int32_t *bucket = bucket_out ? dalloc<int32_t>(32 * n) : ar.take<int32_t>(32 * n);
+  if (bucket_out) *bucket_out = nullptr;
   int *flag = ar.take<int>(1);
   int64_t *ptr = dalloc<int64_t>(n + 1);
   int32_t *list = dalloc<int32_t>(M);
@@ -1204,9 +1217,12 @@ void build_incidence(ffhip_ctx ctx, const int32_t *elems, int64_t nt, int nloc, 
   exclusive_scan_i64(ctx, deg64, ptr, n + 1);
   FFHIP_CHECK(hipGetLastError());
   const int has_big = read_flag(ctx, flag);
-  if (!has_big) {
+  if (!has_big && bucket_out) {
+    *bucket_out = bucket;  // the caller sorts the buckets into the lists
+  } else if (!has_big) {
     k_v2e_sort_bucket<<<nblocks(n * kHalf, 256), 256, 0, s>>>(n, ptr, bucket, list, inv);
   } else {  // some list exceeds its bucket: fill by atomics, then sort
+    if (bucket_out) dfree(bucket);
     int32_t *cur = ar.take<int32_t>(n);
     FFHIP_CHECK(hipMemsetAsync(cur, 0, 4 * n, s));
     FFHIP_CHECK(hipMemsetAsync(flag, 0, sizeof(int), s));
@@ -1228,7 +1244,19 @@ void build_incidence(ffhip_ctx ctx, const int32_t *elems, int64_t nt, int nloc, 
 void build_v2e(ffhip_mesh m) {
   if (m->v2e_ptr) return;
   build_incidence(m->ctx, m->elems, m->nt, m->nloc, m->nv, &m->v2e_ptr, &m->v2e, &m->v2e_big,
-                  nullptr);
+                  nullptr, nullptr);
+}
+
+// As build_v2e, but where every list fits its 32-entry bucket the buckets are
+// returned unsorted and m->v2e is left unfilled: the caller must sort them
+// into m->v2e (k_pattern_half with the buckets does, k_v2e_sort_bucket
+// otherwise) and free the buckets before anything else reads m->v2e.
+int32_t *build_v2e_unsorted(ffhip_mesh m) {
+  if (m->v2e_ptr) return nullptr;
+  int32_t *bucket = nullptr;
+  build_incidence(m->ctx, m->elems, m->nt, m->nloc, m->nv, &m->v2e_ptr, &m->v2e, &m->v2e_big,
+                  nullptr, &bucket);
+  return bucket;
 }
 
 // the position of every (element, local vertex) item in the incidence lists
@@ -1371,11 +1399,22 @@ unsigned char *noalloc_scratch(int device) {
 ffhip_mat pattern_p1(ffhip_ctx ctx, ffhip_mesh m, const double *Ce, hipEvent_t ce_ready,
                      bool *values_done) {
   hipStream_t s = ctx->stream;
-  build_v2e(m);
-  const int64_t n = m->nv;
   const bool vals_pass = Ce && m->nloc == 4;
+  // the fused pass sorts the incidence buckets into the lists itself
+  int32_t *bucket = nullptr;
+  if (vals_pass) bucket = build_v2e_unsorted(m);
+  else build_v2e(m);
+  const int64_t n = m->nv;
   ffhip_mat A = new ffhip_mat_s();
   A->ctx = ctx; A->n = n; A->mesh = m;
+  // the lists sorted from the buckets by k_v2e_sort_bucket (when the fused
+  // pass did not run, or not for every row)
+  auto sort_buckets = [&] {
+    if (!bucket) return;
+    k_v2e_sort_bucket<<<nblocks(n * kHalf, 256), 256, 0, s>>>(n, m->v2e_ptr, bucket, m->v2e, nullptr);
+    FFHIP_CHECK(hipGetLastError());
+    dfree(bucket);
+  };
   try {
     A->rowptr = dalloc<int64_t>(n + 1);
     A->diag = dalloc<int64_t>(n);
@@ -1406,7 +1445,7 @@ ffhip_mat pattern_p1(ffhip_ctx ctx, ffhip_mesh m, const double *Ce, hipEvent_t c
       if (ce_ready) FFHIP_CHECK(hipStreamWaitEvent(s, ce_ready, 0));
       if (vals_pass)
         k_pattern_half<4, true, true><<<nblocks(n * kHalf, 256), 256, 0, s>>>(
-            n, m->elems, m->v2e_ptr, m->v2e, A->slotmap, A->big, flags, out, m->xyz, Ce);
+            n, m->elems, m->v2e_ptr, m->v2e, A->slotmap, A->big, flags, out, m->xyz, Ce, bucket);
       else if (m->nloc == 4)
         k_pattern_half<4, false, true><<<nblocks(n * kHalf, 256), 256, 0, s>>>(
             n, m->elems, m->v2e_ptr, m->v2e, A->slotmap, A->big, flags, out);
@@ -1419,6 +1458,8 @@ ffhip_mat pattern_p1(ffhip_ctx ctx, ffhip_mesh m, const double *Ce, hipEvent_t c
       FFHIP_CHECK(hipStreamSynchronize(s));
       direct_ok = !h[0] && !h[1];
       if (!direct_ok) { dfree(A->colidx); dfree(A->vals); }
+      if (direct_ok) dfree(bucket);  // every list sorted by the pass
+      else sort_buckets();
     }
     if (direct_ok) {
       if (!vals_pass) FFHIP_CHECK(hipMemsetAsync(A->vals, 0, 8 * A->nnz, s));
@@ -1496,6 +1537,11 @@ ffhip_mat pattern_p1(ffhip_ctx ctx, ffhip_mesh m, const double *Ce, hipEvent_t c
     A->max_row = wide ? CAPW : CAP;
   } catch (...) {
     (void)hipStreamSynchronize(s);
+    if (bucket) {  // the lists were never sorted: the mesh builds them again when next needed
+      dfree(bucket);
+      dfree(m->v2e_ptr);
+      dfree(m->v2e);
+    }
     dfree(A->rowptr); dfree(A->colidx); dfree(A->vals); dfree(A->diag);
     dfree(A->slotmap); dfree(A->big);
     delete A;

@@ -29,7 +29,8 @@
 namespace ffhip {
 
 void build_incidence(ffhip_ctx ctx, const int32_t *elems, int64_t nt, int nloc, int64_t n,
-                     int64_t **ptr_out, int32_t **list_out, int *big_out, int32_t **inv_out);
+                     int64_t **ptr_out, int32_t **list_out, int *big_out, int32_t **inv_out,
+                     int32_t **bucket_out);
 int read_flag(ffhip_ctx ctx, int *d);
 
 namespace {
@@ -186,7 +187,7 @@ __global__ void k_mark_labels(int64_t nbe, int nbloc, const int32_t *__restrict_
 void build_b2v(ffhip_mesh m) {
   if (m->b2e_ptr) return;
   build_incidence(m->ctx, m->belems, m->nbe, m->nbloc, m->nv, &m->b2e_ptr, &m->b2e, nullptr,
-                  nullptr);
+                  nullptr, nullptr);
 }
 
 // device copy of a coefficient array (nullptr when none is needed)

@@ -40,7 +40,8 @@
 
 namespace ffhip {
 void build_incidence(ffhip_ctx ctx, const int32_t *elems, int64_t nt, int nloc, int64_t n,
-                     int64_t **ptr_out, int32_t **list_out, int *big_out, int32_t **inv_out);
+                     int64_t **ptr_out, int32_t **list_out, int *big_out, int32_t **inv_out,
+                     int32_t **bucket_out);
 void set_diag_rows(ffhip_ctx ctx, int64_t n, const uint8_t *onbc, const int64_t *diag, double tgv,
                    double *vals, int *missing);
 int read_flag(ffhip_ctx ctx, int *d);
@@ -695,7 +696,7 @@ int ffhip_p2v_create(ffhip_ctx ctx, ffhip_mesh m, ffhip_p2v *out) {
       FFHIP_CHECK(hipStreamSynchronize(s));
       // the arena's temporaries are dead from here on (build_incidence reuses the work buffer)
       build_incidence(ctx, sp->elem_nodes, nt, 10, sp->nnodes, &sp->n2e_ptr, &sp->n2e, nullptr,
-                      &sp->n2e_inv);
+                      &sp->n2e_inv, nullptr);
       ctx->toc();
     } catch (...) {
       (void)hipStreamSynchronize(s);

@@ -478,6 +478,52 @@ def test_pattern_assemble_one_call_equals_two_steps(kind):
         assert np.max(np.abs(v1 - vo)) <= 1e-12 * np.max(np.abs(vo))
 
 
+@pytest.mark.parametrize("case", ["star", "no_boundary_triangles"])
+def test_pattern_assemble_one_call_fallback_sorts_incidences(case):
+    """one-call assembly on meshes whose incidence lists all fit the 32-entry
+    buckets (the pattern pass is to sort them into element order) but whose
+    direct pass is refused: a vertex of 30 tets that share only it (a row of
+    91 columns), or a cube without its boundary triangles (row lengths unlike
+    the vertex-link counts).  The matrix equals the two-step one, and a
+    right-hand side formed afterwards from the incidence lists equals that of
+    a fresh mesh, bit for bit (jittered coordinates and a P1 coefficient make
+    the element order of every sum visible)."""
+    rng = np.random.default_rng(5)
+    if case == "star":
+        pts, tets = [np.zeros(3)], []
+        for _ in range(30):
+            d = rng.standard_normal(3)
+            q = 2.0 * d / np.linalg.norm(d) + 0.3 * rng.standard_normal((3, 3))
+            i0 = len(pts)
+            pts += list(q)
+            t = [0, i0, i0 + 1, i0 + 2]
+            X = np.array(pts)
+            if np.linalg.det(np.c_[X[t[1]] - X[t[0]], X[t[2]] - X[t[0]], X[t[3]] - X[t[0]]]) < 0:
+                t[1], t[2] = t[2], t[1]
+            tets.append(t)
+        xyz = np.array(pts).ravel()
+        tets = np.array(tets, np.int32).ravel()
+    else:
+        mo = ffo.cube(6)
+        X = mo["xyz"].reshape(-1, 3).copy()
+        inner = np.all((X > 1e-12) & (X < 1 - 1e-12), axis=1)
+        X[inner] += 0.02 * rng.standard_normal((int(inner.sum()), 3))
+        xyz, tets = X.ravel(), mo["tets"]
+    none3, none1 = np.zeros(0, np.int32), np.zeros(0, np.int32)
+    M1 = ffhip.mesh3_upload(xyz, tets, none3, none1)
+    M2 = ffhip.mesh3_upload(xyz, tets, none3, none1)
+    A1 = ffhip.Matrix.p1_assembled(M1, ffhip.COEF_SIN3)
+    A2 = ffhip.Matrix.p1(M2)
+    A2.assemble_p1(ffhip.COEF_SIN3)
+    d1, d2 = A1.download(), A2.download()
+    assert np.array_equal(d1[0], d2[0]) and np.array_equal(d1[1], d2[1])
+    assert np.array_equal(d1[2].view(np.int64), d2[2].view(np.int64))
+    f = np.exp(rng.standard_normal(M1.nv))
+    b1 = ffhip.rhs_p1_form(M1, ffhip.COEF_P1, 0.0, f)
+    b2 = ffhip.rhs_p1_form(M2, ffhip.COEF_P1, 0.0, f)
+    assert np.array_equal(b1.view(np.int64), b2.view(np.int64))
+
+
 def test_pattern_assemble_one_call_with_lane_per_row_rows():
     """a vertex of 103 columns: the one-call path falls back to the element-row
     values for every row; the matrix equals the two-step one"""

@@ -504,11 +504,20 @@ __global__ void k_diag_vals(int64_t n, const int64_t *__restrict__ diag,
   has[i] = s >= 0;
 }
 
-// pass 1: max diagonal; pass 2: count of the max, max of the rest
+// What the solver set-up reads back to the host, in one copy: the tgv
+// figures (k_tgv_reduce) and the layout sizes (k_setup_sizes).
+struct SetupRb {
+  double ttgv, ntgv;      // the tgv value (0: none) and the count of rows holding it
+  int64_t ents, ents16;   // entries of the sliced layout, of its packed 16-bit deltas
+  int far;                // some column is more than 2^15 from its row
+};
+
+// pass 1: max diagonal; pass 2: count of the max (*top), max of the rest
 // (HashMatrix::gettgv, HashMatrix.cpp:1340-1371)
 __global__ void k_tgv_pass(int64_t n, const double *__restrict__ d, const uint8_t *__restrict__ has,
-                           double top, int pass, double *__restrict__ out) {
+                           const double *__restrict__ topp, int pass, double *__restrict__ out) {
   __shared__ double lds[kBlock / kWave];
+  const double top = pass == 2 ? *topp : 0.;
   double m = 0., c = 0.;  // the reference starts from ttgv = max1 = 0
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
     if (!has[i]) continue;
@@ -528,11 +537,52 @@ __global__ void k_tgv_pass(int64_t n, const double *__restrict__ d, const uint8_
   if (threadIdx.x == 0) { out[2 * blockIdx.x] = bm; out[2 * blockIdx.x + 1] = bc; }
 }
 
+// the partials of a k_tgv_pass, one block: pass 1 puts the max in rb->ttgv;
+// pass 2 applies gettgv's rule (a max not 1e6 x above the next distinct
+// value is no tgv) to that max, the count and the max of the rest.  Maxima
+// and integral counts are exact in any order.
+__global__ __launch_bounds__(kBlock) void k_tgv_reduce(int GB, const double *__restrict__ part,
+                                                       int pass, SetupRb *__restrict__ rb) {
+  __shared__ double lds[kBlock / kWave];
+  double m = 0., c = 0.;
+  for (int i = threadIdx.x; i < GB; i += kBlock) {
+    m = fmax(m, part[2 * i]);
+    c += part[2 * i + 1];
+  }
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, kWave));
+  const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  if (lane == 0) lds[w] = m;
+  __syncthreads();
+  double bm = 0.;
+  for (int i = 0; i < kBlock / kWave; ++i) bm = fmax(bm, lds[i]);
+  const double count = block_sum<kBlock>(c, lds);
+  if (threadIdx.x != 0) return;
+  if (pass == 1) {
+    rb->ttgv = bm;
+    return;
+  }
+  double ttgv = rb->ttgv, ntgv = count;
+  if (bm * 1e6 > ttgv) { ttgv = 0.; ntgv = 0.; }
+  rb->ttgv = ttgv;
+  rb->ntgv = ntgv;
+}
+
+// the layout sizes for the host (entries of the sliced layout and of the
+// packed deltas: the last slice offsets) and the far-column flag
+__global__ void k_setup_sizes(SetupRb *__restrict__ rb, const int64_t *__restrict__ ents,
+                              const int64_t *__restrict__ ents16, const int *__restrict__ far) {
+  rb->ents = ents ? *ents : 0;
+  rb->ents16 = ents16 ? *ents16 : 0;
+  rb->far = far ? *far : 1;
+}
+
 __global__ void k_jacobi(int64_t n, const double *__restrict__ d, const uint8_t *__restrict__ has,
-                         double tgve, int with_wcl, double *__restrict__ dinv,
+                         const SetupRb *__restrict__ rb, double *__restrict__ dinv,
                          double *__restrict__ dsc, uint8_t *__restrict__ wcl) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const double tgve = rb->ttgv <= 0 ? 1e200 : rb->ttgv;
+  const bool with_wcl = rb->ntgv > 0;
   const double a = has[i] ? d[i] : 0.;  // a missing diagonal reads as 0 (then 1)
   dinv[i] = (a * a < 1e-60) ? 1. : 1. / a;
   dsc[i] = a > 0. ? 1. / sqrt(a) : 1.;
@@ -1521,6 +1571,7 @@ struct ffhip_cg_s {
   uint8_t *wcl = nullptr;
   double tgvm = 0.;
   int ntgv = 0;
+  SetupRb *rb = nullptr;  // set-up figures for the host (one copy per set-up)
   // work
   double *xw = nullptr, *bw = nullptr, *G = nullptr, *CG = nullptr, *H[2] = {nullptr, nullptr},
          *AH = nullptr;
@@ -1643,6 +1694,17 @@ struct ffhip_cg_s {
       k_matvec<kCsrV, T, T><<<(cg)->grid, kBlock, 0, (stream)>>>(M, (const T *)(x), (T *)(y)); \
   } while (0)
 
+// the set-up figures to the host (one copy, after the stream's work so far);
+// the tgv figures into the solver
+static SetupRb setup_readback(ffhip_cg cg) {
+  SetupRb h;
+  FFHIP_CHECK(hipMemcpyAsync(&h, cg->rb, sizeof(h), hipMemcpyDeviceToHost, cg->ctx->stream));
+  FFHIP_CHECK(hipStreamSynchronize(cg->ctx->stream));
+  cg->tgvm = h.ttgv;
+  cg->ntgv = (int)h.ntgv;
+  return h;
+}
+
 // Layout choice: sliced ELL unless its slices pad the matrix by more than 25 %
 // (rows of very different lengths), then CSR with 16 lanes per row.
 static void cg_plan_layout(ffhip_cg cg, int layout, bool index32) {
@@ -1672,51 +1734,53 @@ static void cg_plan_layout(ffhip_cg cg, int layout, bool index32) {
     FFHIP_CHECK(hipMemsetAsync(cnt + S3, 0, 8, s));
     k_bsr_len<<<nblk(S3, 256), 256, 0, s>>>(nb, S3, A->nodeptr, cg->bperm, cg->slen, cnt);
     exclusive_scan_i64(ctx, cnt, cg->sptr, S3 + 1);
-    cg->ents = read_i64(ctx, cg->sptr + S3);
+    k_setup_sizes<<<1, 1, 0, s>>>(cg->rb, cg->sptr + S3, nullptr, nullptr);
+    cg->ents = setup_readback(cg).ents;
     dfree(cnt); dfree(tmp); dfree(keys); dfree(skeys); dfree(vals);
     cg->bcol = dalloc<int32_t>(cg->ents);
   } else if (layout == FFHIP_CG_LAYOUT_BSR3) {
     FFHIP_REQUIRE(false, FFHIP_ERR_UNSUPPORTED, "ffhip_cg_create: 3x3 blocks need a [P2,P2,P2] matrix");
   } else if (layout != FFHIP_CG_LAYOUT_CSR) {
-    Arena ar(ctx, 2 * Arena::bytes(cg->S + 1, 8) + Arena::bytes(1, 4));
+    // slice widths and offsets, the far-column check and the offsets of the
+    // packed 16-bit deltas, then one read-back decides among sliced ELL with
+    // 16-bit deltas, with int32 columns, and CSR
+    Arena ar(ctx, 3 * Arena::bytes(cg->S + 1, 8) + Arena::bytes(1, 4));
     cg->sptr = dalloc<int64_t>(cg->S + 1);
     cg->slen = dalloc<int32_t>(cg->S);
+    cg->sptr16 = dalloc<int64_t>(cg->S + 1);
     int64_t *cnt = ar.take<int64_t>(cg->S + 1);
+    int64_t *cnt4 = ar.take<int64_t>(cg->S + 1);
+    int *fard = ar.take<int>(1);
     FFHIP_CHECK(hipMemsetAsync(cnt + cg->S, 0, 8, s));
+    FFHIP_CHECK(hipMemsetAsync(cnt4 + cg->S, 0, 8, s));
+    FFHIP_CHECK(hipMemsetAsync(fard, 0, sizeof(int), s));
     k_sell_len<<<nblk(cg->S, 256), 256, 0, s>>>(A->n, cg->S, A->rowptr, cg->slen, cnt);
     exclusive_scan_i64(ctx, cnt, cg->sptr, cg->S + 1);
-    cg->ents = read_i64(ctx, cg->sptr + cg->S);
+    // 16-bit column deltas when every column is within 2^15 of its row
+    if (!index32) k_far_columns<<<nblk(A->n, 256), 256, 0, s>>>(A->n, A->rowptr, A->colidx, fard);
+    k_sell_len4<<<nblk(cg->S, 256), 256, 0, s>>>(cg->S, cg->slen, cnt4);
+    exclusive_scan_i64(ctx, cnt4, cg->sptr16, cg->S + 1);
+    k_setup_sizes<<<1, 1, 0, s>>>(cg->rb, cg->sptr + cg->S, cg->sptr16 + cg->S, index32 ? nullptr : fard);
+    FFHIP_CHECK(hipGetLastError());
+    const SetupRb h = setup_readback(cg);
+    cg->ents = h.ents;
     if (layout == FFHIP_CG_LAYOUT_SELL || 4 * cg->ents <= 5 * A->nnz) {
-      // 16-bit column deltas when every column is within 2^15 of its row
-      int far = 1;
-      if (!index32) {
-        int *d = ar.take<int>(1);
-        FFHIP_CHECK(hipMemsetAsync(d, 0, sizeof(int), s));
-        k_far_columns<<<nblk(A->n, 256), 256, 0, s>>>(A->n, A->rowptr, A->colidx, d);
-        FFHIP_CHECK(hipGetLastError());
-        far = read_flag(ctx, d);
-      }
       // 16-bit deltas, and gathers through buffer resources with 32-bit byte offsets
-      if (far || A->n >= ((int64_t)1 << 28)) {
+      if (h.far || A->n >= ((int64_t)1 << 28)) {
         cg->fmt = kSell;
         cg->scol = dalloc<int32_t>(cg->ents);
+        dfree(cg->sptr16);
       } else {
         cg->fmt = kSell16;
-        cg->sptr16 = dalloc<int64_t>(cg->S + 1);
-        int64_t *cnt4 = ar.take<int64_t>(cg->S + 1);
-        FFHIP_CHECK(hipMemsetAsync(cnt4 + cg->S, 0, 8, s));
-        k_sell_len4<<<nblk(cg->S, 256), 256, 0, s>>>(cg->S, cg->slen, cnt4);
-        exclusive_scan_i64(ctx, cnt4, cg->sptr16, cg->S + 1);
-        const int64_t ents16 = read_i64(ctx, cg->sptr16 + cg->S);
-        cg->scol16 = dalloc<int16_t>(ents16);
+        cg->scol16 = dalloc<int16_t>(h.ents16);
       }
     } else {
-      dfree(cg->sptr); dfree(cg->slen);
-      cg->sptr = nullptr; cg->slen = nullptr;
+      dfree(cg->sptr); dfree(cg->slen); dfree(cg->sptr16);
       cg->fmt = kCsrV;
     }
   } else {
     cg->fmt = kCsrV;
+    setup_readback(cg);  // the tgv figures
   }
   int64_t g;
   if (cg->fmt == kBsr3) {
@@ -1788,6 +1852,8 @@ static void cg_fill_values(ffhip_cg cg, bool transposed) {
   dfree(tmap);
 }
 
+// Jacobi scaling and the tgv rows, on the device (the tgv figures reach the
+// host with the next setup_readback)
 static void cg_setup_jacobi(ffhip_cg cg) {
   ffhip_ctx ctx = cg->ctx;
   ffhip_mat A = cg->A;
@@ -1797,30 +1863,14 @@ static void cg_setup_jacobi(ffhip_cg cg) {
   Arena ar(ctx, Arena::bytes(n, 8) + Arena::bytes(n, 1) + Arena::bytes(2 * GB, 8));
   double *d = ar.take<double>(n);
   uint8_t *has = ar.take<uint8_t>(n);
-  k_diag_vals<<<nblk(n, 256), 256, 0, s>>>(n, A->diag, A->vals, d, has);
   double *part = ar.take<double>(2 * GB);
-  std::vector<double> hp(2 * GB);
-  k_tgv_pass<<<GB, kBlock, 0, s>>>(n, d, has, 0., 1, part);
-  FFHIP_CHECK(hipMemcpyAsync(hp.data(), part, 16 * GB, hipMemcpyDeviceToHost, s));
-  FFHIP_CHECK(hipStreamSynchronize(s));
-  double top = 0.;
-  for (int i = 0; i < GB; ++i) top = std::max(top, hp[2 * i]);
-  k_tgv_pass<<<GB, kBlock, 0, s>>>(n, d, has, top, 2, part);
-  FFHIP_CHECK(hipMemcpyAsync(hp.data(), part, 16 * GB, hipMemcpyDeviceToHost, s));
-  FFHIP_CHECK(hipStreamSynchronize(s));
-  double second = 0., count = 0.;
-  for (int i = 0; i < GB; ++i) { second = std::max(second, hp[2 * i]); count += hp[2 * i + 1]; }
-  // gettgv: a top value not 1e6 x above the next distinct value is no tgv
-  double ttgv = top;
-  int ntgv = (int)count;
-  if (second * 1e6 > ttgv) { ttgv = 0; ntgv = 0; }
-  cg->tgvm = ttgv;
-  cg->ntgv = ntgv;
-  const double tgve = ttgv <= 0 ? 1e200 : ttgv;
-  k_jacobi<<<nblk(n, 256), 256, 0, s>>>(n, d, has, tgve, ntgv > 0, cg->dinv, cg->dsc, cg->wcl);
+  k_diag_vals<<<nblk(n, 256), 256, 0, s>>>(n, A->diag, A->vals, d, has);
+  k_tgv_pass<<<GB, kBlock, 0, s>>>(n, d, has, nullptr, 1, part);
+  k_tgv_reduce<<<1, kBlock, 0, s>>>(GB, part, 1, cg->rb);
+  k_tgv_pass<<<GB, kBlock, 0, s>>>(n, d, has, &cg->rb->ttgv, 2, part);
+  k_tgv_reduce<<<1, kBlock, 0, s>>>(GB, part, 2, cg->rb);
+  k_jacobi<<<nblk(n, 256), 256, 0, s>>>(n, d, has, cg->rb, cg->dinv, cg->dsc, cg->wcl);
   FFHIP_CHECK(hipGetLastError());
-  FFHIP_CHECK(hipStreamSynchronize(s));
-
 }
 
 // r = b - A x (rout may be null); the three partial sums of k_residual added on
@@ -1972,7 +2022,7 @@ static void cg_release(ffhip_cg cg) {
   for (hipEvent_t e : cg->ev) (void)hipEventDestroy(e);
   for (auto &g : cg->graphs) (void)hipGraphExecDestroy(g.exec);
   if (cg->fmt == kCsrV) cg->sval = nullptr;  // A's array
-  dfree(cg->sptr); dfree(cg->slen); dfree(cg->scol); dfree(cg->scol16); dfree(cg->sptr16); dfree(cg->bcol); dfree(cg->bperm); dfree(cg->bwave); dfree(cg->sval);
+  dfree(cg->sptr); dfree(cg->slen); dfree(cg->scol); dfree(cg->scol16); dfree(cg->sptr16); dfree(cg->bcol); dfree(cg->bperm); dfree(cg->bwave); dfree(cg->sval); dfree(cg->rb);
   dfree(cg->svalT);
   dfree(cg->sval32); dfree(cg->sval32T); dfree(cg->dinv); dfree(cg->dsc); dfree(cg->wcl);
   dfree(cg->xw); dfree(cg->bw); dfree(cg->G); dfree(cg->CG); dfree(cg->H[0]); dfree(cg->H[1]);
@@ -2007,8 +2057,9 @@ int ffhip_cg_create(ffhip_ctx ctx, ffhip_mat A, int mode, ffhip_cg *out) {
       cg->dinv = dalloc<double>(n);
       cg->dsc = dalloc<double>(n);
       cg->wcl = dalloc<uint8_t>(n);
+      cg->rb = dalloc<SetupRb>(1);
       cg_setup_jacobi(cg);
-      cg_plan_layout(cg, layout, index32);
+      cg_plan_layout(cg, layout, index32);  // reads the tgv figures back with its sizes
       cg_fill_values(cg, false);
       cg->xw = dalloc<double>(n); cg->bw = dalloc<double>(n);
       cg->G = dalloc<double>(n); cg->CG = dalloc<double>(n);
@@ -2048,6 +2099,7 @@ int ffhip_cg_update_values(ffhip_cg cg) {
   return guard([&] {
     FFHIP_REQUIRE(cg, FFHIP_ERR_ARG, "ffhip_cg_update_values: null solver");
     cg_setup_jacobi(cg);
+    setup_readback(cg);
     cg_fill_values(cg, false);
     if (cg->svalT) cg_fill_values(cg, true);
   });

@@ -275,10 +275,12 @@ __global__ __launch_bounds__(256) void k_pattern_half(
   // every LDS array below is per row (half-wave), so a wave orders its own
   // accesses (wave_lds_sync) and the waves of a block run independently
   __shared__ int32_t hk[NH][kHash];      // open-addressing set of the row's columns
+  __shared__ uint8_t hslot[NH][kRowBuf];  // the set slot of each unique column
+  __shared__ uint8_t hrank[NH][kHash];    // the offset in the row of the column in each set slot
   __shared__ int32_t ucol[NH][kRowBuf];  // unique columns, then sorted
   __shared__ int32_t ucnt[NH];
   __shared__ double sv[VALS ? NH : 1][VALS ? 4 * kHalf : 1];   // incidence rows (VALS)
-  __shared__ uint32_t so[VALS ? NH : 1][VALS ? kHalf : 1];      // their slot bytes (VALS)
+  __shared__ __align__(16) uint32_t so[VALS ? NH : 1][VALS ? kHalf : 1];  // their slot bytes (VALS)
   const int hl = threadIdx.x % kHalf, hid = threadIdx.x / kHalf;
   const int64_t v = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kHalf;
   int64_t k0 = 0, p0 = 0, p1 = 0;
@@ -334,7 +336,7 @@ __global__ __launch_bounds__(256) void k_pattern_half(
       const int32_t old = atomicCAS(&hk[hid][h], INT_MAX, c[r]);
       if (old == INT_MAX) {
         const int idx = atomicAdd(&ucnt[hid], 1);
-        if (idx < kRowBuf) ucol[hid][idx] = c[r];
+        if (idx < kRowBuf) { ucol[hid][idx] = c[r]; hslot[hid][idx] = (uint8_t)h; }
         break;
       }
       if (old == c[r]) break;
@@ -367,6 +369,7 @@ __global__ __launch_bounds__(256) void k_pattern_half(
 #pragma unroll
     for (int h2 = 0; h2 < 2; ++h2)
       if (hl + h2 * kHalf < len) {
+        hrank[hid][hslot[hid][hl + h2 * kHalf]] = (uint8_t)rk[h2];
         ucol[hid][rk[h2]] = u[h2];
         if (!DIRECT) out.rowbuf[kRowBuf * v + rk[h2]] = u[h2];
         else if (!wrong) out.colidx[r0 + rk[h2]] = u[h2];
@@ -390,12 +393,10 @@ __global__ __launch_bounds__(256) void k_pattern_half(
     uint32_t pk = 0;
 #pragma unroll
     for (int m = 0; m < NLOC; ++m) {
-      int lo = 0, hi = len - 1;  // the column is in the row: find its offset
-      while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (ucol[hid][mid] < c[m]) lo = mid + 1; else hi = mid;
-      }
-      pk |= (uint32_t)lo << (8 * m);
+      // the column is in the set: its slot holds its offset in the row
+      uint32_t h = ((uint32_t)c[m] * 2654435761u) >> (32 - 7);
+      while (hk[hid][h] != c[m]) h = (h + 1) & (kHash - 1);
+      pk |= (uint32_t)hrank[hid][h] << (8 * m);
     }
     slotmap[k0 + hl] = pk;
     if constexpr (VALS) {
@@ -411,12 +412,24 @@ __global__ __launch_bounds__(256) void k_pattern_half(
     const int nq = mine ? deg : 0;
     const uint32_t me0 = (uint32_t)hl * 0x01010101u, me1 = (uint32_t)(hl + kHalf) * 0x01010101u;
     double acc0 = 0., acc1 = 0.;
-    for (int q = 0; q < nq; ++q) {
-      const uint32_t pk = so[hid][q];
-      const uint32_t t0 = pk ^ me0, z0 = (t0 - 0x01010101u) & ~t0 & 0x80808080u;
-      if (z0) acc0 += sv[hid][4 * q + (__builtin_ctz(z0) >> 3)];
+    // the incidences' slot words, four per LDS read, all read before the sum;
+    // every incidence's value read unconditionally (its own slot's, or its
+    // first) and added when the incidence holds the slot, else +0.0, which
+    // leaves the sum unchanged: it starts at +0.0 and is never -0.0
+    uint32_t pkq[kHalf];
+#pragma unroll
+    for (int g = 0; g < kHalf / 4; ++g) {
+      const uint4 w = *reinterpret_cast<const uint4 *>(&so[hid][4 * g]);
+      pkq[4 * g] = w.x; pkq[4 * g + 1] = w.y; pkq[4 * g + 2] = w.z; pkq[4 * g + 3] = w.w;
+    }
+#pragma unroll
+    for (int q = 0; q < kHalf; ++q) {
+      if (q >= nq) break;
+      const uint32_t t0 = pkq[q] ^ me0, z0 = (t0 - 0x01010101u) & ~t0 & 0x80808080u;
+      const double a0 = sv[hid][4 * q + (z0 ? __builtin_ctz(z0) >> 3 : 0)];
+      acc0 += z0 ? a0 : 0.;
       if (len > kHalf) {
-        const uint32_t t1 = pk ^ me1, z1 = (t1 - 0x01010101u) & ~t1 & 0x80808080u;
+        const uint32_t t1 = pkq[q] ^ me1, z1 = (t1 - 0x01010101u) & ~t1 & 0x80808080u;
         if (z1) acc1 += sv[hid][4 * q + (__builtin_ctz(z1) >> 3)];
       }
     }

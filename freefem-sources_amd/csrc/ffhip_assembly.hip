@@ -412,27 +412,26 @@ __global__ __launch_bounds__(256) void k_pattern_half(
     const int nq = mine ? deg : 0;
     const uint32_t me0 = (uint32_t)hl * 0x01010101u, me1 = (uint32_t)(hl + kHalf) * 0x01010101u;
     double acc0 = 0., acc1 = 0.;
-    // the incidences' slot words, four per LDS read, all read before the sum;
-    // every incidence's value read unconditionally (its own slot's, or its
-    // first) and added when the incidence holds the slot, else +0.0, which
-    // leaves the sum unchanged: it starts at +0.0 and is never -0.0
-    uint32_t pkq[kHalf];
+    // incidences in chunks of 8 (q0 + 8 <= 32: every index inside the row's
+    // arrays), the chunk's slot words and values read before its adds; every
+    // value read unconditionally (the incidence's value in the slot, or its
+    // first) and added when the incidence holds the slot and is the row's,
+    // else +0.0, which leaves the sum unchanged: it starts at +0.0 and is
+    // never -0.0
+    for (int q0 = 0; q0 < nq; q0 += 8) {
 #pragma unroll
-    for (int g = 0; g < kHalf / 4; ++g) {
-      const uint4 w = *reinterpret_cast<const uint4 *>(&so[hid][4 * g]);
-      pkq[4 * g] = w.x; pkq[4 * g + 1] = w.y; pkq[4 * g + 2] = w.z; pkq[4 * g + 3] = w.w;
-    }
-#pragma unroll
-    for (int q = 0; q < kHalf; ++q) {
-      if (q >= nq) break;
-      const uint32_t t0 = pkq[q] ^ me0, z0 = (t0 - 0x01010101u) & ~t0 & 0x80808080u;
-      const double a0 = sv[hid][4 * q + (z0 ? __builtin_ctz(z0) >> 3 : 0)];
-      acc0 += z0 ? a0 : 0.;
-      if (len > kHalf) {
-        const uint32_t t1 = pkq[q] ^ me1, z1 = (t1 - 0x01010101u) & ~t1 & 0x80808080u;
-        if (z1) acc1 += sv[hid][4 * q + (__builtin_ctz(z1) >> 3)];
+      for (int qq = 0; qq < 8; ++qq) {
+        const int q = q0 + qq;
+        const uint32_t t0 = so[hid][q] ^ me0, z0 = (t0 - 0x01010101u) & ~t0 & 0x80808080u;
+        const double a0 = sv[hid][4 * q + (z0 ? __builtin_ctz(z0) >> 3 : 0)];
+        acc0 += (z0 && q < nq) ? a0 : 0.;
       }
     }
+    if (len > kHalf)  // rows of more than 32 columns: the second slot of each lane
+      for (int q = 0; q < nq; ++q) {
+        const uint32_t t1 = so[hid][q] ^ me1, z1 = (t1 - 0x01010101u) & ~t1 & 0x80808080u;
+        if (z1) acc1 += sv[hid][4 * q + (__builtin_ctz(z1) >> 3)];
+      }
     if (mine && !DIRECT) {
       if (hl < len) out.rowval[kRowBuf * v + hl] = acc0;
       if (hl + kHalf < len) out.rowval[kRowBuf * v + hl + kHalf] = acc1;

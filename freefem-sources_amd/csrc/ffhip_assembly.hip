@@ -146,11 +146,19 @@ __global__ __launch_bounds__(256) void k_v2e_sort_half(int64_t n, const int64_t 
 
 // Incidence buckets (up to 32 entries per dof, in no particular order; the
 // sort below puts them in element order) with the atomics aggregated per
-// block: the block's 256 items are counted per dof in an LDS hash set (an
+// block: the block's 256 x kBucketIpt items are counted per dof in an LDS hash set (an
 // item's rank among the block's items of its dof from the LDS atomicAdd),
 // then each distinct dof of the block reserves its positions with one
 // atomicAdd on its global count.
-constexpr int kBucketHash = 512;  // >= 2x the items of a block: probing always ends
+#ifndef FFHIP_BUCKET_IPT
+#define FFHIP_BUCKET_IPT 2
+#endif
+// items per thread (cube(160): 1 -> 2 saves about 0.15 ms of the one-call
+// assembly, 4 loses it again)
+constexpr int kBucketIpt = FFHIP_BUCKET_IPT;
+constexpr int kBucketLog = 9 + (kBucketIpt >= 4 ? 2 : kBucketIpt >= 2 ? 1 : 0);
+constexpr int kBucketHash = 1 << kBucketLog;              // >= 2x the items of a block: probing always ends
+static_assert(kBucketHash >= 2 * 256 * kBucketIpt, "hash set of twice the block's items");
 __global__ __launch_bounds__(256) void k_v2e_bucket_block(const int32_t *__restrict__ elems, int64_t m,
                                                           uint32_t *__restrict__ deg,
                                                           int32_t *__restrict__ bucket,
@@ -161,17 +169,26 @@ __global__ __launch_bounds__(256) void k_v2e_bucket_block(const int32_t *__restr
   const int t = threadIdx.x;
   for (int i = t; i < kBucketHash; i += blockDim.x) { hkey[i] = -1; hcnt[i] = 0; }
   __syncthreads();
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + t;
-  const int32_t v = i < m ? elems[i] : -1;
-  uint32_t h = 0, rank = 0;
-  if (v >= 0) {
-    h = ((uint32_t)v * 2654435761u) >> (32 - 9);
-    for (;;) {  // ends: at most 256 keys in 512 slots
-      const int32_t old = atomicCAS(&hkey[h], -1, v);
-      if (old == -1 || old == v) break;
-      h = (h + 1) & (kBucketHash - 1);
+  int32_t v[kBucketIpt];
+  uint32_t h[kBucketIpt], rank[kBucketIpt];
+#pragma unroll
+  for (int j = 0; j < kBucketIpt; ++j) {
+    const int64_t i = (int64_t)blockIdx.x * (256 * kBucketIpt) + j * 256 + t;
+    v[j] = i < m ? elems[i] : -1;
+  }
+#pragma unroll
+  for (int j = 0; j < kBucketIpt; ++j) {
+    h[j] = 0;
+    rank[j] = 0;
+    if (v[j] >= 0) {
+      h[j] = ((uint32_t)v[j] * 2654435761u) >> (32 - kBucketLog);
+      for (;;) {  // ends: at most half the slots hold keys
+        const int32_t old = atomicCAS(&hkey[h[j]], -1, v[j]);
+        if (old == -1 || old == v[j]) break;
+        h[j] = (h[j] + 1) & (kBucketHash - 1);
+      }
+      rank[j] = atomicAdd(&hcnt[h[j]], 1u);
     }
-    rank = atomicAdd(&hcnt[h], 1u);
   }
   __syncthreads();
   for (int q = t; q < kBucketHash; q += blockDim.x) {
@@ -179,9 +196,12 @@ __global__ __launch_bounds__(256) void k_v2e_bucket_block(const int32_t *__restr
     if (key >= 0) hbase[q] = atomicAdd(&deg[key], hcnt[q]);
   }
   __syncthreads();
-  if (v >= 0) {
-    const uint32_t pos = hbase[h] + rank;
-    if (pos < 32) bucket[32 * (int64_t)v + pos] = (int32_t)i;
+#pragma unroll
+  for (int j = 0; j < kBucketIpt; ++j) {
+    if (v[j] < 0) continue;
+    const int64_t i = (int64_t)blockIdx.x * (256 * kBucketIpt) + j * 256 + t;
+    const uint32_t pos = hbase[h[j]] + rank[j];
+    if (pos < 32) bucket[32 * (int64_t)v[j] + pos] = (int32_t)i;
     else atomicOr(ovf, 1);
   }
 }
@@ -1224,7 +1244,7 @@ void build_incidence(ffhip_ctx ctx, const int32_t *elems, int64_t nt, int nloc, 
   FFHIP_CHECK(hipMemsetAsync(deg, 0, 4 * n, s));
   FFHIP_CHECK(hipMemsetAsync(deg64 + n, 0, 8, s));
   FFHIP_CHECK(hipMemsetAsync(flag, 0, sizeof(int), s));
-  k_v2e_bucket_block<<<nblocks(M, 256), 256, 0, s>>>(elems, M, deg, bucket, flag);
+  k_v2e_bucket_block<<<nblocks(M, 256 * kBucketIpt), 256, 0, s>>>(elems, M, deg, bucket, flag);
   k_u32_to_i64<<<nblocks(n, 256), 256, 0, s>>>(n, deg, deg64);
   exclusive_scan_i64(ctx, deg64, ptr, n + 1);
   FFHIP_CHECK(hipGetLastError());

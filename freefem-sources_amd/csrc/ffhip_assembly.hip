@@ -301,6 +301,8 @@ __global__ __launch_bounds__(256) void k_pattern_half(
   __shared__ int32_t ucnt[NH];
   __shared__ double sv[VALS ? NH : 1][VALS ? 4 * kHalf : 1];   // incidence rows (VALS)
   __shared__ __align__(16) uint32_t so[VALS ? NH : 1][VALS ? kHalf : 1];  // their slot bytes (VALS)
+  __shared__ __align__(16) double dq[VALS ? NH : 1][VALS ? kHalf : 1];    // their diagonal entries (VALS)
+  __shared__ uint32_t smask[VALS ? NH : 1][VALS ? kHalf : 1];  // per off-diagonal slot: its incidences (VALS)
   const int hl = threadIdx.x % kHalf, hid = threadIdx.x / kHalf;
   const int64_t v = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kHalf;
   int64_t k0 = 0, p0 = 0, p1 = 0;
@@ -314,6 +316,7 @@ __global__ __launch_bounds__(256) void k_pattern_half(
   if (!mine) deg = 0;
   for (int i = hl; i < kHash; i += kHalf) hk[hid][i] = INT_MAX;
   if (hl == 0) ucnt[hid] = 0;
+  if constexpr (VALS) smask[hid][hl] = 0;
   int32_t c[4] = {INT_MAX, INT_MAX, INT_MAX, INT_MAX};
   int64_t e = 0;
   int lv = 0;
@@ -423,7 +426,12 @@ __global__ __launch_bounds__(256) void k_pattern_half(
       double r[4];
       p1_row_entries(vv, lv, Cv, r);
 #pragma unroll
-      for (int m = 0; m < 4; ++m) sv[hid][4 * hl + m] = r[m];
+      for (int m = 0; m < 4; ++m) {
+        sv[hid][4 * hl + m] = r[m];
+        const uint32_t slot = (pk >> (8 * m)) & 0xffu;
+        if (m != lv && slot < kHalf) atomicOr(&smask[hid][slot], 1u << hl);
+      }
+      dq[hid][hl] = r[lv];
       so[hid][hl] = pk;
     }
   }
@@ -432,19 +440,39 @@ __global__ __launch_bounds__(256) void k_pattern_half(
     const int nq = mine ? deg : 0;
     const uint32_t me0 = (uint32_t)hl * 0x01010101u, me1 = (uint32_t)(hl + kHalf) * 0x01010101u;
     double acc0 = 0., acc1 = 0.;
-    // incidences in chunks of 8 (q0 + 8 <= 32: every index inside the row's
-    // arrays), the chunk's slot words and values read before its adds; every
-    // value read unconditionally (the incidence's value in the slot, or its
-    // first) and added when the incidence holds the slot and is the row's,
-    // else +0.0, which leaves the sum unchanged: it starts at +0.0 and is
-    // never -0.0
-    for (int q0 = 0; q0 < nq; q0 += 8) {
+    // Every incidence holds the diagonal slot: its lane adds the incidences'
+    // diagonal entries in order, 8 per round (reads before adds; past the
+    // row's count +0.0, which leaves the sum unchanged: it starts at +0.0
+    // and is never -0.0).  Another slot's lane adds the entries of the
+    // incidences in its mask (bit q: incidence q holds the slot), lowest
+    // first, 4 per round.  Both orders are the element order.
+    if (hl == dpos) {
+      for (int q0 = 0; q0 < nq; q0 += 8) {
+        double a[8];
 #pragma unroll
-      for (int qq = 0; qq < 8; ++qq) {
-        const int q = q0 + qq;
-        const uint32_t t0 = so[hid][q] ^ me0, z0 = (t0 - 0x01010101u) & ~t0 & 0x80808080u;
-        const double a0 = sv[hid][4 * q + (z0 ? __builtin_ctz(z0) >> 3 : 0)];
-        acc0 += (z0 && q < nq) ? a0 : 0.;
+        for (int i = 0; i < 8; ++i) a[i] = dq[hid][q0 + i];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) acc0 += q0 + i < nq ? a[i] : 0.;
+      }
+    } else if (hl < len) {
+      uint32_t mask = mine ? smask[hid][hl] : 0u;
+      while (mask) {
+        int q[4];
+        bool ok[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          ok[i] = mask != 0;
+          q[i] = ok[i] ? __builtin_ctz(mask) : 0;
+          mask &= mask - 1;
+        }
+        double a[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const uint32_t t0 = so[hid][q[i]] ^ me0, z0 = (t0 - 0x01010101u) & ~t0 & 0x80808080u;
+          a[i] = sv[hid][4 * q[i] + (z0 ? __builtin_ctz(z0) >> 3 : 0)];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc0 += ok[i] ? a[i] : 0.;
       }
     }
     if (len > kHalf)  // rows of more than 32 columns: the second slot of each lane

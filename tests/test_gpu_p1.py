@@ -478,30 +478,39 @@ def test_pattern_assemble_one_call_equals_two_steps(kind):
         assert np.max(np.abs(v1 - vo)) <= 1e-12 * np.max(np.abs(vo))
 
 
-@pytest.mark.parametrize("case", ["star", "no_boundary_triangles"])
+@pytest.mark.parametrize("case", ["star", "star12_centre_last", "no_boundary_triangles"])
 def test_pattern_assemble_one_call_fallback_sorts_incidences(case):
     """one-call assembly on meshes whose incidence lists all fit the 32-entry
     buckets (the pattern pass is to sort them into element order) but whose
     direct pass is refused: a vertex of 30 tets that share only it (a row of
-    91 columns), or a cube without its boundary triangles (row lengths unlike
-    the vertex-link counts).  The matrix equals the two-step one, and a
-    right-hand side formed afterwards from the incidence lists equals that of
-    a fresh mesh, bit for bit (jittered coordinates and a P1 coefficient make
-    the element order of every sum visible)."""
+    91 columns), a vertex of 12 such tets numbered last (37 columns, the
+    diagonal at offset 36: the row's values past the first 32 slots), or a
+    cube without its boundary triangles (row lengths unlike the vertex-link
+    counts).  The matrix equals the two-step one, and a right-hand side
+    formed afterwards from the incidence lists equals that of a fresh mesh,
+    bit for bit (jittered coordinates and a P1 coefficient make the element
+    order of every sum visible)."""
     rng = np.random.default_rng(5)
-    if case == "star":
+    if case.startswith("star"):
+        k = 30 if case == "star" else 12
         pts, tets = [np.zeros(3)], []
-        for _ in range(30):
+        for _ in range(k):
             d = rng.standard_normal(3)
             q = 2.0 * d / np.linalg.norm(d) + 0.3 * rng.standard_normal((3, 3))
             i0 = len(pts)
             pts += list(q)
-            t = [0, i0, i0 + 1, i0 + 2]
-            X = np.array(pts)
+            tets.append([0, i0, i0 + 1, i0 + 2])
+        X = np.array(pts)
+        if case != "star":  # the centre numbered last
+            perm = np.r_[1:len(X), 0]          # new vertex i is old vertex perm[i]
+            inv = np.empty_like(perm)
+            inv[perm] = np.arange(len(X))
+            X = X[perm]
+            tets = [[int(inv[a]) for a in t] for t in tets]
+        for t in tets:
             if np.linalg.det(np.c_[X[t[1]] - X[t[0]], X[t[2]] - X[t[0]], X[t[3]] - X[t[0]]]) < 0:
                 t[1], t[2] = t[2], t[1]
-            tets.append(t)
-        xyz = np.array(pts).ravel()
+        xyz = X.ravel()
         tets = np.array(tets, np.int32).ravel()
     else:
         mo = ffo.cube(6)

@@ -283,8 +283,8 @@ struct PatternOut {
 // element scales Ce (the rows of k_p1_values_rows).
 //
 // With bucket (the incidence lists unsorted, 32 entries per row, every list
-// within its bucket): the half-wave sorts its row's list into element order
-// and writes it to v2e, in place of k_v2e_sort_bucket.
+// within its bucket; VALS only): each lane ranks its incidence among the
+// row's and writes it to v2e at that rank, in place of k_v2e_sort_bucket.
 template <int NLOC, bool VALS, bool DIRECT>
 __global__ __launch_bounds__(256) void k_pattern_half(
     int64_t n, const int32_t *__restrict__ elems, const int64_t *__restrict__ v2e_ptr,
@@ -322,13 +322,28 @@ __global__ __launch_bounds__(256) void k_pattern_half(
   int lv = 0;
   double vv[VALS ? 4 : 1][3], Cv = 0.;
   int32_t idx = INT_MAX;
-  if (bucket) {  // every lane of the half-wave takes part in the sort
-    idx = hl < deg ? bucket[32 * v + hl] : INT_MAX;
-    idx = bitonic32(idx, hl);
-    if (hl < deg) v2e[k0 + hl] = idx;
-  } else if (hl < deg) {
-    idx = v2e[k0 + hl];
+  int qp = hl;  // the incidence's position in the row's list (element order)
+  bool ranked = false;
+  if constexpr (VALS) {
+    if (bucket) {
+      // unsorted bucket: each lane's rank among the row's entries (element
+      // ids are distinct) places its incidence, and everything it stores
+      // per incidence below, in element order
+      idx = hl < deg ? bucket[32 * v + hl] : INT_MAX;
+      so[hid][hl] = (uint32_t)idx;
+      wave_lds_sync();
+      int rho = 0;
+#pragma unroll
+      for (int g = 0; g < kHalf / 4; ++g) {
+        const uint4 w = *reinterpret_cast<const uint4 *>(&so[hid][4 * g]);
+        rho += ((int32_t)w.x < idx) + ((int32_t)w.y < idx) + ((int32_t)w.z < idx) + ((int32_t)w.w < idx);
+      }
+      qp = rho;
+      if (hl < deg) v2e[k0 + qp] = idx;
+      ranked = true;
+    }
   }
+  if (!ranked && hl < deg) idx = v2e[k0 + hl];
   if (hl < deg) {
     e = idx / NLOC;
     lv = idx - (int)e * NLOC;
@@ -421,18 +436,18 @@ __global__ __launch_bounds__(256) void k_pattern_half(
       while (hk[hid][h] != c[m]) h = (h + 1) & (kHash - 1);
       pk |= (uint32_t)hrank[hid][h] << (8 * m);
     }
-    slotmap[k0 + hl] = pk;
+    slotmap[k0 + qp] = pk;
     if constexpr (VALS) {
       double r[4];
       p1_row_entries(vv, lv, Cv, r);
 #pragma unroll
       for (int m = 0; m < 4; ++m) {
-        sv[hid][4 * hl + m] = r[m];
+        sv[hid][4 * qp + m] = r[m];
         const uint32_t slot = (pk >> (8 * m)) & 0xffu;
-        if (m != lv && slot < kHalf) atomicOr(&smask[hid][slot], 1u << hl);
+        if (m != lv && slot < kHalf) atomicOr(&smask[hid][slot], 1u << qp);
       }
-      dq[hid][hl] = r[lv];
-      so[hid][hl] = pk;
+      dq[hid][qp] = r[lv];
+      so[hid][qp] = pk;
     }
   }
   if constexpr (VALS) {  // the row's values, each slot summed by its lane in element order

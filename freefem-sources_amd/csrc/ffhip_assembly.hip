@@ -206,22 +206,32 @@ __global__ __launch_bounds__(256) void k_v2e_bucket_block(const int32_t *__restr
   }
 }
 
-// bucketed lists (up to 32 entries) sorted into element order and packed
-// at their CSR offsets, two dofs per wave
+// bucketed lists (up to 32 entries) put into element order and packed at
+// their CSR offsets, two dofs per wave: each entry goes to its rank among
+// the dof's entries (distinct items), counted over the bucket staged in LDS
 __global__ __launch_bounds__(256) void k_v2e_sort_bucket(int64_t n, const int64_t *__restrict__ ptr,
                                                          const int32_t *__restrict__ bucket,
                                                          int32_t *__restrict__ v2e,
                                                          int32_t *__restrict__ inv) {
+  __shared__ __align__(16) int32_t ids[256];
   const int hl = threadIdx.x % kHalf;
   const int64_t v = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kHalf;
   int64_t k0 = 0;
   int deg = 0;
   if (v < n) { k0 = ptr[v]; deg = (int)(ptr[v + 1] - k0); }
-  int32_t x = hl < deg ? bucket[32 * v + hl] : INT_MAX;
-  x = bitonic32(x, hl);
+  const int32_t x = hl < deg ? bucket[32 * v + hl] : INT_MAX;
+  ids[threadIdx.x] = x;
+  wave_lds_sync();
+  const int4 *row = reinterpret_cast<const int4 *>(ids + (threadIdx.x & ~(kHalf - 1)));
+  int rho = 0;
+#pragma unroll
+  for (int g = 0; g < kHalf / 4; ++g) {
+    const int4 w = row[g];
+    rho += (w.x < x) + (w.y < x) + (w.z < x) + (w.w < x);
+  }
   if (hl < deg) {
-    v2e[k0 + hl] = x;
-    if (inv) inv[x] = (int32_t)(k0 + hl);
+    v2e[k0 + rho] = x;
+    if (inv) inv[x] = (int32_t)(k0 + rho);
   }
 }
 

@@ -72,14 +72,17 @@ __global__ void k_p2_vfirst(int64_t nt, const int32_t *__restrict__ tets, int32_
 }
 
 // edge keys and their items, in item order (the radix sort is stable)
-__global__ void k_p2_edge_keys(int64_t nt, const int32_t *__restrict__ tets,
+// sort keys of the edge items: min << vb | max (vb the bits of a vertex id,
+// so the radix sort runs over 2 vb bits, not 64)
+__global__ void k_p2_edge_keys(int64_t nt, const int32_t *__restrict__ tets, int vb,
                                uint64_t *__restrict__ keys, int32_t *__restrict__ items) {
   const int64_t m = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= 6 * nt) return;
   const int64_t k = m / 6;
   const int j = (int)(m % 6);
   const int32_t *t = tets + 4 * k;
-  keys[m] = edge_key(t[c_edge[j][0]], t[c_edge[j][1]]);
+  const int32_t a = t[c_edge[j][0]], b = t[c_edge[j][1]];
+  keys[m] = ((uint64_t)(uint32_t)(a < b ? a : b) << vb) | (uint32_t)(a < b ? b : a);
   items[m] = (int32_t)(10 * k + 4 + j);
 }
 
@@ -95,7 +98,7 @@ __global__ void k_p2_edge_heads(int64_t E, const uint64_t *__restrict__ skeys,
 }
 
 // edge segment of every edge item; unique keys and their first items
-__global__ void k_p2_edge_seg(int64_t E, const uint64_t *__restrict__ skeys,
+__global__ void k_p2_edge_seg(int64_t E, const uint64_t *__restrict__ skeys, int vb,
                               const int32_t *__restrict__ sitems, const int64_t *__restrict__ hscan,
                               const int64_t *__restrict__ head, int32_t *__restrict__ item_seg,
                               uint64_t *__restrict__ ukeys, int32_t *__restrict__ segfirst) {
@@ -104,7 +107,11 @@ __global__ void k_p2_edge_seg(int64_t E, const uint64_t *__restrict__ skeys,
   const int64_t seg = hscan[p] + head[p] - 1;  // inclusive scan - 1
   const int32_t item = sitems[p];
   item_seg[(item / 10) * 6 + (item % 10 - 4)] = (int32_t)seg;
-  if (head[p]) { ukeys[seg] = skeys[p]; segfirst[seg] = item; }
+  if (head[p]) {  // the unique keys in edge_key form (min << 32 | max)
+    const uint64_t k = skeys[p];
+    ukeys[seg] = ((k >> vb) << 32) | (k & ((uint64_t(1) << vb) - 1));
+    segfirst[seg] = item;
+  }
 }
 
 __global__ void k_p2_vmark(int64_t nt, const int32_t *__restrict__ tets,
@@ -651,13 +658,15 @@ int ffhip_p2v_create(ffhip_ctx ctx, ffhip_mesh m, ffhip_p2v *out) {
                   "ffhip_p2v_create: %lld elements (need 10 nt < 2^31)", (long long)m->nt);
     hipStream_t s = ctx->stream;
     const int64_t nt = m->nt, nv = m->nv, E = 6 * nt, NI = 10 * nt;
+    int vb = 1;  // bits of a vertex id
+    while (vb < 31 && (int64_t(1) << vb) < nv) ++vb;
     ffhip_p2v sp = new ffhip_p2v_s();
     sp->ctx = ctx; sp->mesh = m;
     try {
       size_t tb = 0;
       FFHIP_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, (uint64_t *)nullptr, (uint64_t *)nullptr,
                                                      (int32_t *)nullptr, (int32_t *)nullptr, (int)E, 0,
-                                                     64, s));
+                                                     2 * vb, s));
       Arena ar(ctx, Arena::bytes(nv, 4) + 2 * Arena::bytes(E, 8) + 4 * Arena::bytes(E, 4) +
                         Arena::bytes(tb, 1) + 2 * Arena::bytes(E + 1, 8) + 2 * Arena::bytes(NI + 1, 8));
       ctx->tic();
@@ -666,10 +675,10 @@ int ffhip_p2v_create(ffhip_ctx ctx, ffhip_mesh m, ffhip_p2v *out) {
       k_p2_vfirst<<<nblk(4 * nt, 256), 256, 0, s>>>(nt, m->elems, firstv);
       uint64_t *keys = ar.take<uint64_t>(E), *skeys = ar.take<uint64_t>(E);
       int32_t *items = ar.take<int32_t>(E), *sitems = ar.take<int32_t>(E);
-      k_p2_edge_keys<<<nblk(E, 256), 256, 0, s>>>(nt, m->elems, keys, items);
+      k_p2_edge_keys<<<nblk(E, 256), 256, 0, s>>>(nt, m->elems, vb, keys, items);
       void *tmp = ar.take<char>(tb);
       FFHIP_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, keys, skeys, items, sitems, (int)E, 0,
-                                                     64, s));
+                                                     2 * vb, s));
       int64_t *head = ar.take<int64_t>(E + 1), *hscan = ar.take<int64_t>(E + 1);
       int64_t *mark = ar.take<int64_t>(NI + 1), *rank = ar.take<int64_t>(NI + 1);
       FFHIP_CHECK(hipMemsetAsync(head + E, 0, 8, s));
@@ -681,7 +690,7 @@ int ffhip_p2v_create(ffhip_ctx ctx, ffhip_mesh m, ffhip_p2v *out) {
       int32_t *item_seg = ar.take<int32_t>(E);
       int32_t *segfirst = ar.take<int32_t>(E);  // ne <= E
       sp->ekeys = dalloc<uint64_t>(sp->ne);
-      k_p2_edge_seg<<<nblk(E, 256), 256, 0, s>>>(E, skeys, sitems, hscan, head, item_seg, sp->ekeys,
+      k_p2_edge_seg<<<nblk(E, 256), 256, 0, s>>>(E, skeys, vb, sitems, hscan, head, item_seg, sp->ekeys,
                                                  segfirst);
       exclusive_scan_i64(ctx, mark, rank, NI + 1);
       sp->nnodes = read_i64(ctx, rank + NI);

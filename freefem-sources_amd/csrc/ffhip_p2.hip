@@ -257,29 +257,36 @@ __global__ __launch_bounds__(256) void k_p2_pattern_wave(
 }
 
 // scalar rows 3a + c of node a: for every node column b, columns 3b, 3b+1, 3b+2
-// (node columns from ncol at nodeptr, or from the row buffers when rowbuf != null)
-__global__ void k_p2v_expand(int64_t nn, const int64_t *__restrict__ nodeptr,
-                             const int32_t *__restrict__ ncol, const int64_t *__restrict__ ndiag,
-                             const int32_t *__restrict__ rowbuf, const int8_t *__restrict__ dposr,
-                             int64_t *__restrict__ rowptr, int32_t *__restrict__ colidx,
-                             int64_t *__restrict__ diag) {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= 3 * nn) return;
-  const int64_t a = r / 3;
-  const int c = (int)(r % 3);
+// (node columns from ncol at nodeptr, or from the row buffers when rowbuf != null).
+// One wave per node: the node's 3 rows are one run of 9 L columns at 9 p0,
+// written with consecutive lanes on consecutive entries.
+__global__ __launch_bounds__(256) void k_p2v_expand(int64_t nn, const int64_t *__restrict__ nodeptr,
+                                                    const int32_t *__restrict__ ncol,
+                                                    const int64_t *__restrict__ ndiag,
+                                                    const int32_t *__restrict__ rowbuf,
+                                                    const int8_t *__restrict__ dposr,
+                                                    int64_t *__restrict__ rowptr,
+                                                    int32_t *__restrict__ colidx,
+                                                    int64_t *__restrict__ diag) {
+  const int lane = threadIdx.x % kWave;
+  const int64_t a = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  if (a >= nn) return;  // uniform in the wave
   const int64_t p0 = nodeptr[a], L = nodeptr[a + 1] - p0;
-  const int64_t base = 9 * p0 + c * 3 * L;
-  rowptr[r] = base;
-  if (r == 3 * nn - 1) rowptr[3 * nn] = 9 * nodeptr[nn];
   const int32_t *cols = rowbuf ? rowbuf + kNodeCap * a : ncol + p0;
-  for (int64_t q = 0; q < L; ++q) {
-    const int32_t b = cols[q];
-    colidx[base + 3 * q] = 3 * b;
-    colidx[base + 3 * q + 1] = 3 * b + 1;
-    colidx[base + 3 * q + 2] = 3 * b + 2;
+  const int64_t base0 = 9 * p0;
+  for (int c = 0; c < 3; ++c)
+    for (int64_t j = lane; j < 3 * L; j += kWave) {
+      const int64_t q = j / 3;
+      colidx[base0 + c * 3 * L + j] = 3 * cols[q] + (int32_t)(j - 3 * q);
+    }
+  if (lane < 3) {
+    const int c = lane;
+    const int64_t r = 3 * a + c, base = base0 + c * 3 * L;
+    rowptr[r] = base;
+    if (r == 3 * nn - 1) rowptr[3 * nn] = 9 * nodeptr[nn];
+    const int64_t d = rowbuf ? (dposr[a] < 0 ? -1 : dposr[a]) : (ndiag[a] < 0 ? -1 : ndiag[a] - p0);
+    diag[r] = d < 0 ? -1 : base + 3 * d + c;
   }
-  const int64_t d = rowbuf ? (dposr[a] < 0 ? -1 : dposr[a]) : (ndiag[a] < 0 ? -1 : ndiag[a] - p0);
-  diag[r] = d < 0 ? -1 : base + 3 * d + c;
 }
 
 // scalar P2: the node rows are the matrix rows (columns from ncol at nodeptr,
@@ -800,7 +807,7 @@ static void p2_pattern(ffhip_ctx ctx, ffhip_p2v sp, int ncomp, ffhip_mat *out) {
       A->diag = dalloc<int64_t>(n);
       A->vals = dalloc<double>(A->nnz);
       if (ncomp == 3)
-        k_p2v_expand<<<nblk(n, 256), 256, 0, s>>>(nn, A->nodeptr, ncol, ndiag,
+        k_p2v_expand<<<nblk(nn * kWave, 256), 256, 0, s>>>(nn, A->nodeptr, ncol, ndiag,
                                                   wave_ok ? rowbuf : nullptr, dposr, A->rowptr,
                                                   A->colidx, A->diag);
       else {

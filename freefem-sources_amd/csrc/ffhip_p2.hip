@@ -159,7 +159,7 @@ constexpr int kNodeHash = 256;  // LDS hash slots per node row
 // probing); the unique node columns are ordered by a counting sort and
 // written to the row's buffer (kNodeCap entries), with the row length, the
 // relative diagonal position and the 10 slot bytes of every incidence
-// (binary search in the sorted columns).  A row with more than 64 incident
+// (from a rank table indexed by the column's hash slot).  A row with more than 64 incident
 // elements, more than kNodeCap columns or a full set raises *ovf (the
 // lane-per-row merge then builds the whole pattern).
 __global__ __launch_bounds__(256) void k_p2_pattern_wave(
@@ -169,6 +169,8 @@ __global__ __launch_bounds__(256) void k_p2_pattern_wave(
   constexpr int NW = 256 / kWave;
   __shared__ int32_t hk[NW][kNodeHash];
   __shared__ int32_t ucol[NW][kNodeCap];
+  __shared__ uint8_t hslot[NW][kNodeCap];   // the set slot of each unique column
+  __shared__ uint8_t hrank[NW][kNodeHash];  // the offset in the row of the column in each set slot
   __shared__ int32_t ucnt[NW];
   const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
   const int64_t a = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kWave;
@@ -196,7 +198,7 @@ __global__ __launch_bounds__(256) void k_p2_pattern_wave(
         const int32_t old = atomicCAS(&hk[w][h], INT_MAX, c[i]);
         if (old == INT_MAX) {
           const int idx = atomicAdd(&ucnt[w], 1);
-          if (idx < kNodeCap) ucol[w][idx] = c[i];
+          if (idx < kNodeCap) { ucol[w][idx] = c[i]; hslot[w][idx] = (uint8_t)h; }
           placed = true;
         } else if (old == c[i]) {
           placed = true;
@@ -230,6 +232,7 @@ __global__ __launch_bounds__(256) void k_p2_pattern_wave(
 #pragma unroll
     for (int h2 = 0; h2 < 2; ++h2)
       if (lane + h2 * kWave < len) {
+        hrank[w][hslot[w][lane + h2 * kWave]] = (uint8_t)rk[h2];
         ucol[w][rk[h2]] = u[h2];
         rowbuf[kNodeCap * a + rk[h2]] = u[h2];
         if ((int64_t)u[h2] == a) dpos = rk[h2];
@@ -246,12 +249,10 @@ __global__ __launch_bounds__(256) void k_p2_pattern_wave(
     uint8_t *dst = slot8 + (k0 + lane) * 10;
 #pragma unroll
     for (int i = 0; i < 10; ++i) {
-      int lo = 0, hi = len - 1;
-      while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (ucol[w][mid] < c[i]) lo = mid + 1; else hi = mid;
-      }
-      dst[i] = (uint8_t)lo;
+      // the column is in the set: its slot holds its offset in the row
+      uint32_t h = ((uint32_t)c[i] * 2654435761u) >> (32 - 8);
+      while (hk[w][h] != c[i]) h = (h + 1) & (kNodeHash - 1);
+      dst[i] = hrank[w][h];
     }
   }
 }

@@ -917,6 +917,77 @@ __global__ __launch_bounds__(256) void k_p1_scale_3d(int64_t nt, const double *_
   Ce[e] = (S * d1) * d1;
 }
 
+// The incidence buckets of k_v2e_bucket_block fused with the element scales
+// of k_p1_scale_3d (tets): a block takes 256 tets (1024 items, 4 per thread,
+// a 2048-slot LDS hash set); the thread's tet is loaded with its items, the
+// block's global count atomics are issued before the tet's scale is
+// computed (their round trip hidden by the arithmetic), and the items are
+// placed after it.  The scale is the same computation as k_p1_scale_3d's.
+// (5 waves per SIMD: 86 VGPRs, no spills; uncapped it takes 108 and runs 4)
+constexpr int kFuseLog = 11;
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 5))) void k_v2e_bucket_scale(const int32_t *__restrict__ tets, int64_t nt,
+                                                          ScaleArgs sc, uint32_t *__restrict__ deg,
+                                                          int32_t *__restrict__ bucket,
+                                                          int *__restrict__ ovf) {
+  constexpr int IPT = 4, HS = 1 << kFuseLog;
+  __shared__ int32_t hkey[HS];
+  __shared__ uint32_t hcnt[HS];
+  __shared__ uint32_t hbase[HS];
+  const int t = threadIdx.x;
+  const int64_t m = 4 * nt, b0 = (int64_t)blockIdx.x * (256 * IPT);
+  for (int i = t; i < HS; i += blockDim.x) { hkey[i] = -1; hcnt[i] = 0; }
+  int32_t v[IPT];
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    const int64_t i = b0 + j * 256 + t;
+    v[j] = i < m ? tets[i] : -1;
+  }
+  const int64_t e = (int64_t)blockIdx.x * 256 + t;
+  double tv[4][3];
+  int32_t id[4];
+  if (e < nt) load_tet(sc.xyz, tets, e, id, tv);
+  __syncthreads();
+  uint32_t h[IPT], rank[IPT];
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    h[j] = 0;
+    rank[j] = 0;
+    if (v[j] >= 0) {
+      h[j] = ((uint32_t)v[j] * 2654435761u) >> (32 - kFuseLog);
+      for (;;) {  // ends: at most half the slots hold keys
+        const int32_t old = atomicCAS(&hkey[h[j]], -1, v[j]);
+        if (old == -1 || old == v[j]) break;
+        h[j] = (h[j] + 1) & (HS - 1);
+      }
+      rank[j] = atomicAdd(&hcnt[h[j]], 1u);
+    }
+  }
+  __syncthreads();
+  uint32_t hb[HS / 256];
+#pragma unroll
+  for (int q = 0; q < HS / 256; ++q) {
+    const int32_t key = hkey[t + q * 256];
+    hb[q] = key >= 0 ? atomicAdd(&deg[key], hcnt[t + q * 256]) : 0u;
+  }
+  if (e < nt) {
+    const double mes = tet_mesure(tv);
+    const double S = p1_kappa_integral(e, id, tv, mes, sc.kind, sc.c0, sc.kappa_e);
+    const double d1 = 1. / (6. * mes);
+    sc.Ce[e] = (S * d1) * d1;
+  }
+#pragma unroll
+  for (int q = 0; q < HS / 256; ++q) hbase[t + q * 256] = hb[q];
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < IPT; ++j) {
+    if (v[j] < 0) continue;
+    const int64_t i = b0 + j * 256 + t;
+    const uint32_t pos = hbase[h[j]] + rank[j];
+    if (pos < 32) bucket[32 * (int64_t)v[j] + pos] = (int32_t)i;
+    else atomicOr(ovf, 1);
+  }
+}
+
 // values, one row per 32-lane half-wave (rows of up to 32 elements and 64
 // columns; big[] rows are left to the lane-per-row gather): lane q takes
 // incidence q (element order), forms its element's row l from the vertices
@@ -1281,7 +1352,7 @@ static void upload_t5_once(int dev) {
 // element-to-dof table (P1 vertices, P2 nodes)
 void build_incidence(ffhip_ctx ctx, const int32_t *elems, int64_t nt, int nloc, int64_t n,
                      int64_t **ptr_out, int32_t **list_out, int *big_out, int32_t **inv_out,
-                     int32_t **bucket_out) {
+                     int32_t **bucket_out, const ScaleArgs *sc) {
   hipStream_t s = ctx->stream;
   const int64_t M = nt * nloc;
   Arena ar(ctx, Arena::bytes(n, 4) + Arena::bytes(n + 1, 8) + (bucket_out ? 0 : Arena::bytes(32 * n, 4)) +
@@ -1297,7 +1368,10 @@ void build_incidence(ffhip_ctx ctx, const int32_t *elems, int64_t nt, int nloc, 
   FFHIP_CHECK(hipMemsetAsync(deg, 0, 4 * n, s));
   FFHIP_CHECK(hipMemsetAsync(deg64 + n, 0, 8, s));
   FFHIP_CHECK(hipMemsetAsync(flag, 0, sizeof(int), s));
-  k_v2e_bucket_block<<<nblocks(M, 256 * kBucketIpt), 256, 0, s>>>(elems, M, deg, bucket, flag);
+  if (sc && nloc == 4)  // the element scales computed in the same pass
+    k_v2e_bucket_scale<<<nblocks(nt, 256), 256, 0, s>>>(elems, nt, *sc, deg, bucket, flag);
+  else
+    k_v2e_bucket_block<<<nblocks(M, 256 * kBucketIpt), 256, 0, s>>>(elems, M, deg, bucket, flag);
   k_u32_to_i64<<<nblocks(n, 256), 256, 0, s>>>(n, deg, deg64);
   exclusive_scan_i64(ctx, deg64, ptr, n + 1);
   FFHIP_CHECK(hipGetLastError());
@@ -1329,18 +1403,22 @@ void build_incidence(ffhip_ctx ctx, const int32_t *elems, int64_t nt, int nloc, 
 void build_v2e(ffhip_mesh m) {
   if (m->v2e_ptr) return;
   build_incidence(m->ctx, m->elems, m->nt, m->nloc, m->nv, &m->v2e_ptr, &m->v2e, &m->v2e_big,
-                  nullptr, nullptr);
+                  nullptr, nullptr, nullptr);
 }
 
 // As build_v2e, but where every list fits its 32-entry bucket the buckets are
 // returned unsorted and m->v2e is left unfilled: the caller must sort them
 // into m->v2e (k_pattern_half with the buckets does, k_v2e_sort_bucket
 // otherwise) and free the buckets before anything else reads m->v2e.
-int32_t *build_v2e_unsorted(ffhip_mesh m) {
+// With sc (a tetrahedral mesh), the incidence build also computes the
+// element scales (*scaled = true); when the lists exist already it does not.
+int32_t *build_v2e_unsorted(ffhip_mesh m, const ScaleArgs *sc, bool *scaled) {
+  *scaled = false;
   if (m->v2e_ptr) return nullptr;
   int32_t *bucket = nullptr;
   build_incidence(m->ctx, m->elems, m->nt, m->nloc, m->nv, &m->v2e_ptr, &m->v2e, &m->v2e_big,
-                  nullptr, &bucket);
+                  nullptr, &bucket, sc);
+  *scaled = sc && m->nloc == 4;
   return bucket;
 }
 
@@ -1482,13 +1560,24 @@ unsigned char *noalloc_scratch(int device) {
 // pattern pass writing straight to CSR.  If some row is not what they
 // predict, or needs the lane-per-row kernels, the buffered passes run.
 ffhip_mat pattern_p1(ffhip_ctx ctx, ffhip_mesh m, const double *Ce, hipEvent_t ce_ready,
-                     bool *values_done) {
+                     bool *values_done, const ScaleArgs *sc) {
   hipStream_t s = ctx->stream;
   const bool vals_pass = Ce && m->nloc == 4;
-  // the fused pass sorts the incidence buckets into the lists itself
+  // the fused pass sorts the incidence buckets into the lists itself; with
+  // sc the incidence build computes the element scales Ce as well (or, on a
+  // mesh whose lists exist, k_p1_scale_3d does)
   int32_t *bucket = nullptr;
-  if (vals_pass) bucket = build_v2e_unsorted(m);
-  else build_v2e(m);
+  if (vals_pass) {
+    bool scaled = false;
+    bucket = build_v2e_unsorted(m, sc, &scaled);
+    if (sc && !scaled) {
+      k_p1_scale_3d<<<nblocks(m->nt, 256), 256, 0, s>>>(m->nt, sc->xyz, m->elems, sc->kind, sc->c0,
+                                                        sc->kappa_e, sc->Ce);
+      FFHIP_CHECK(hipGetLastError());
+    }
+  } else {
+    build_v2e(m);
+  }
   const int64_t n = m->nv;
   ffhip_mat A = new ffhip_mat_s();
   A->ctx = ctx; A->n = n; A->mesh = m;
@@ -1646,7 +1735,7 @@ int ffhip_pattern_p1(ffhip_ctx ctx, ffhip_mesh m, ffhip_mat *out) {
     FFHIP_REQUIRE(ctx && m && out, FFHIP_ERR_ARG, "ffhip_pattern_p1: null argument");
     FFHIP_REQUIRE(m->nloc <= 4, FFHIP_ERR_UNSUPPORTED, "ffhip_pattern_p1: P1 on simplices only");
     ctx->tic();
-    ffhip_mat A = pattern_p1(ctx, m, nullptr, nullptr, nullptr);
+    ffhip_mat A = pattern_p1(ctx, m, nullptr, nullptr, nullptr, nullptr);
     ctx->toc();
     *out = A;
   });
@@ -1667,7 +1756,6 @@ int ffhip_pattern_assemble_p1(ffhip_ctx ctx, ffhip_mesh m, int kind, double c0, 
     const double *kd = kappa_e;
     double *kbuf = nullptr, *Ce = nullptr;
     ffhip_mat A = nullptr;
-    bool side_running = false;
     try {
       if (arr && !kappa_on_device) {
         const int64_t cnt = kind == FFHIP_COEF_P1 ? m->nv : kind == FFHIP_COEF_P0 ? m->nt : 14 * m->nt;
@@ -1677,26 +1765,18 @@ int ffhip_pattern_assemble_p1(ffhip_ctx ctx, ffhip_mesh m, int kind, double c0, 
       }
       Ce = dalloc<double>(m->nt);
       ctx->tic();
-      // the element scales (arithmetic: the coefficient at 14 points per
-      // tet) on the side stream, beside the incidence lists and row pointers
-      // (memory and atomics) on the main one; the pattern pass, which forms
-      // the row values from the scales, waits for them
-      hipStream_t ss = ctx->side_stream();
-      FFHIP_CHECK(hipEventRecord(ctx->ev_fork, s));
-      FFHIP_CHECK(hipStreamWaitEvent(ss, ctx->ev_fork, 0));
-      side_running = true;
-      k_p1_scale_3d<<<nblocks(m->nt, 256), 256, 0, ss>>>(m->nt, m->xyz, m->elems, kind, c0, kd, Ce);
-      FFHIP_CHECK(hipGetLastError());
-      FFHIP_CHECK(hipEventRecord(ctx->ev_join, ss));
+      // the element scales are computed by the incidence build (one pass:
+      // the coefficient at 14 points per tet hides the atomics' round trips),
+      // then the pattern pass forms the row values from them
+      const ScaleArgs sc{m->xyz, kind, c0, kd, Ce};
       bool done = false;
-      A = pattern_p1(ctx, m, Ce, ctx->ev_join, &done);
+      A = pattern_p1(ctx, m, Ce, nullptr, &done, &sc);
       ctx->toc();
       if (!done) {  // rows for the lane-per-row kernels: all values by the element-row path
         const int rc = ffhip_assemble_p1(ctx, m, A, kind, c0, kd, 1);
         FFHIP_REQUIRE(rc == FFHIP_OK, rc, "ffhip_pattern_assemble_p1: %s", ffhip_last_error());
       }
     } catch (...) {
-      if (side_running) (void)hipStreamSynchronize(ctx->side);
       (void)hipStreamSynchronize(s);
       dfree(Ce); dfree(kbuf);
       if (A) ffhip_mat_free(A);

@@ -28,9 +28,6 @@
 
 namespace ffhip {
 
-void build_incidence(ffhip_ctx ctx, const int32_t *elems, int64_t nt, int nloc, int64_t n,
-                     int64_t **ptr_out, int32_t **list_out, int *big_out, int32_t **inv_out,
-                     int32_t **bucket_out);
 int read_flag(ffhip_ctx ctx, int *d);
 
 namespace {

@@ -214,6 +214,22 @@ __device__ inline int xcd_block_id() {
   return (b % 8) * (G / 8) + (b / 8);
 }
 
+// What the one-call P1 assembly needs for the element scales (C_e of the
+// stiffness); the incidence build of a tetrahedral mesh computes them when
+// given these.
+struct ScaleArgs {
+  const double *xyz;
+  int kind;
+  double c0;
+  const double *kappa_e;
+  double *Ce;
+};
+
+// dof -> (element, local dof) incidence lists in element order (ffhip_assembly.hip)
+void build_incidence(ffhip_ctx ctx, const int32_t *elems, int64_t nt, int nloc, int64_t n,
+                     int64_t **ptr_out, int32_t **list_out, int *big_out, int32_t **inv_out,
+                     int32_t **bucket_out, const ScaleArgs *sc = nullptr);
+
 void exclusive_scan_i64(ffhip_ctx ctx, const int64_t *in, int64_t *out,
                         int64_t n);
 

@@ -39,9 +39,6 @@
 #include "ffhip_fem.hpp"
 
 namespace ffhip {
-void build_incidence(ffhip_ctx ctx, const int32_t *elems, int64_t nt, int nloc, int64_t n,
-                     int64_t **ptr_out, int32_t **list_out, int *big_out, int32_t **inv_out,
-                     int32_t **bucket_out);
 void set_diag_rows(ffhip_ctx ctx, int64_t n, const uint8_t *onbc, const int64_t *diag, double tgv,
                    double *vals, int *missing);
 int read_flag(ffhip_ctx ctx, int *d);

@@ -960,7 +960,7 @@ __device__ inline void block_rows4(int64_t r0, int64_t r1, int64_t &a4, int64_t 
 // nothing else of the iteration; the stopping launch still applies it.
 __global__ __launch_bounds__(kBlock) void k_pcg_dirvec(int c, int64_t n,
                                                        const double *__restrict__ Hold,
-                                                       const double *__restrict__ CG,
+                                                       const double *__restrict__ dinv,
                                                        const double *__restrict__ G,
                                                        double *__restrict__ Hnew,
                                                        double *__restrict__ x,
@@ -975,14 +975,17 @@ __global__ __launch_bounds__(kBlock) void k_pcg_dirvec(int c, int64_t n,
   int64_t r0, r1;
   block_rows(n, r0, r1);
   if (go) {
-    // the new direction and its partial (g, h), the numerator of rho
+    // the new direction and its partial (g, h), the numerator of rho; C g is
+    // formed here from D^-1 and g (the update keeps no copy of it), the
+    // product rounded on its own as the stored copy was
     double part = 0.;
     for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) {
       const double h = Hold[r];
       if (step) x[r] += rho * h;
-      const double hn = gamma * h - CG[r];
+      const double g = G[r];
+      const double hn = gamma * h - __dmul_rn(dinv[r], g);
       Hnew[r] = hn;
-      part += G[r] * hn;
+      part += g * hn;
     }
     const double gh = block_sum<kBlock>(part, lds);
     if (threadIdx.x == 0) pout[2 * xcd_block_id() + 1] = gh;
@@ -1023,8 +1026,7 @@ __global__ FFHIP_MATVEC_ATTR void k_pcg_matvec(MatFmt<double> M, const double *_
 
 __global__ __launch_bounds__(kBlock) void k_pcg_update(
     int64_t n, const double *__restrict__ AH, const double *__restrict__ dinv,
-    double *__restrict__ G, double *__restrict__ CG, const double *__restrict__ pin,
-    double *__restrict__ pout, CgState *st) {
+    double *__restrict__ G, const double *__restrict__ pin, double *__restrict__ pout, CgState *st) {
   __shared__ double lds[kBlock / kWave];
   if (st->done) return;
   const int P = gridDim.x;
@@ -1038,8 +1040,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(
   for (int64_t r = r0 + threadIdx.x; r < r1; r += kBlock) {
     const double g = G[r] + rho * AH[r];
     G[r] = g;
-    const double c = dinv[r] * g;
-    CG[r] = c;
+    const double c = __dmul_rn(dinv[r], g);  // C g, formed again by the next direction kernel
     part += g * c;
   }
   const double tot = block_sum<kBlock>(part, lds);
@@ -1903,13 +1904,12 @@ static void launch_pcg_chunk(ffhip_cg cg, const double *sval, bool marks) {
   const MatFmt<double> M = cg->fmt_of(sval);
   for (int c = 0; c < kChunk; ++c) {
     if (marks) cg->mark(4 * c);
-    k_pcg_dirvec<<<G, kBlock, 0, s>>>(c, cg->n, cg->H[c & 1], cg->CG, cg->G, cg->H[(c + 1) & 1],
+    k_pcg_dirvec<<<G, kBlock, 0, s>>>(c, cg->n, cg->H[c & 1], cg->dinv, cg->G, cg->H[(c + 1) & 1],
                                       cg->xw, cg->pA, cg->pB, cg->st);
     if (marks) cg->mark(4 * c + 1);
     FFHIP_LAUNCH_FMT(cg, k_pcg_matvec, G, s, M, cg->H[(c + 1) & 1], cg->AH, cg->pB, cg->st);
     if (marks) cg->mark(4 * c + 2);
-    k_pcg_update<<<G, kBlock, 0, s>>>(cg->n, cg->AH, cg->dinv, cg->G, cg->CG, cg->pB, cg->pA,
-                                      cg->st);
+    k_pcg_update<<<G, kBlock, 0, s>>>(cg->n, cg->AH, cg->dinv, cg->G, cg->pB, cg->pA, cg->st);
     if (marks) cg->mark(4 * c + 3);
   }
   k_cg_advance<<<1, 64, 0, s>>>(cg->st, kChunk);

@@ -248,7 +248,9 @@ __global__ void k_sell_len(int64_t n, int64_t S, const int64_t *__restrict__ row
 // by plane, consecutive threads on consecutive rows, so every store is
 // coalesced (FP64 entry pairs as one 16-byte store).  Entries past a row's
 // end are zeros at the row's own column.
-constexpr int kFillStage = 2048;
+// 1280 entries (a P1 slice of 64 rows of 15 holds 960): 15 KB of LDS, 10
+// blocks per CU (2048 entries, 6 blocks per CU: cube(160) 401 -> 330 us)
+constexpr int kFillStage = 1280;
 __global__ __launch_bounds__(256) void k_sell_fill(
     int64_t n, int64_t S, const int64_t *__restrict__ rowptr, const int32_t *__restrict__ colidx,
     const double *__restrict__ vals, const int64_t *__restrict__ tmap,

@@ -437,15 +437,23 @@ __global__ __launch_bounds__(kElemBlock) void k_p2_element(
         Ke[(int64_t)inv[10 * e + i] * 10 + j] = k;
       }
     } else if (e < nt) {
-      for (int o = lane; o < 900; o += kWave) {
-        const int i = o / 90, rem = o % 90, c = rem / 30, r2 = rem % 30, j = r2 / 3, cp = r2 % 3;
+      // entry pairs (rem, rem + 1): a pair stays in its 90-value node-block
+      // row, and rows start at multiples of 720 bytes, so one 16-byte store
+      auto entry = [&](int i, int rem) {
+        const int c = rem / 30, r2 = rem % 30, j = r2 / 3, cp = r2 % 3;
         double k = lambda * M[(3 * i + c) * kMs + 3 * j + cp] + mu * M[(3 * i + cp) * kMs + 3 * j + c];
         if (c == cp)
           k += mu * (M[(3 * i) * kMs + 3 * j] + M[(3 * i + 1) * kMs + 3 * j + 1] +
                      M[(3 * i + 2) * kMs + 3 * j + 2]);
+        return k;
+      };
+      for (int o = lane; o < 450; o += kWave) {
+        const int i = o / 45, rem = 2 * (o % 45);
+        typedef double f64x2 __attribute__((ext_vector_type(2)));
+        const f64x2 kk = {entry(i, rem), entry(i, rem + 1)};
         // node block i goes where its incidence entry sits in node i's list
         // (streamed: non-temporal, the gather reads it once)
-        __builtin_nontemporal_store(k, &Ke[(int64_t)inv[10 * e + i] * 90 + rem]);
+        __builtin_nontemporal_store(kk, reinterpret_cast<f64x2 *>(&Ke[(int64_t)inv[10 * e + i] * 90 + rem]));
       }
     }
     wave_lds_sync();

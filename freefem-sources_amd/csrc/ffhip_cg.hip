@@ -51,7 +51,7 @@ constexpr int kWavesPerBlock = kBlock / kSlice;
 #define FFHIP_WPE 3
 #endif
 #ifndef FFHIP_BSR_U
-#define FFHIP_BSR_U 4
+#define FFHIP_BSR_U 2
 #endif
 #ifndef FFHIP_BSR_U32
 #define FFHIP_BSR_U32 8

@@ -533,6 +533,27 @@ def test_pattern_assemble_one_call_fallback_sorts_incidences(case):
     assert np.array_equal(b1.view(np.int64), b2.view(np.int64))
 
 
+@pytest.mark.parametrize("kind", ["sin3", "p0"])
+def test_pattern_assemble_one_call_twice_on_one_mesh(kind):
+    """the first one-call assembly on a mesh builds its incidence lists and
+    the element scales in one pass; a second one on the same mesh (lists
+    kept) computes the scales on their own: both matrices bit for bit equal,
+    and equal to the two-step matrix"""
+    n = 7
+    M = ffhip.cube(n)
+    rng = np.random.default_rng(11)
+    k, c0, arr = ((ffhip.COEF_SIN3, 1.0, None) if kind == "sin3"
+                  else (ffhip.COEF_P0, 0.0, np.exp(rng.standard_normal(M.nt))))
+    A1 = ffhip.Matrix.p1_assembled(M, k, c0, arr)
+    A2 = ffhip.Matrix.p1_assembled(M, k, c0, arr)
+    A3 = ffhip.Matrix.p1(M)
+    A3.assemble_p1(k, c0, arr)
+    d1, d2, d3 = A1.download(), A2.download(), A3.download()
+    for a, b in ((d1, d2), (d1, d3)):
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+        assert np.array_equal(a[2].view(np.int64), b[2].view(np.int64))
+
+
 def test_pattern_assemble_one_call_with_lane_per_row_rows():
     """a vertex of 103 columns: the one-call path falls back to the element-row
     values for every row; the matrix equals the two-step one"""

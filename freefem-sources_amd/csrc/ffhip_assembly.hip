@@ -3,9 +3,12 @@
 //
 // Design (MI355X-first, deterministic):
 //   1. dof -> (element, local vertex) incidence lists, sorted by element id:
-//      counts by integer atomics, a scan, an atomic fill, then every list of
-//      up to 32 entries is sorted by a bitonic network in a 32-lane half-wave
-//      (longer lists: per-dof insertion sort).
+//      each block counts its items per dof in an LDS hash set and reserves
+//      their places in the dofs' 32-entry buckets with one global atomic per
+//      dof, a scan gives the list offsets, and every entry goes to its rank
+//      among its dof's entries (k_v2e_sort_bucket, or the fused pattern pass
+//      of the one-call assembly).  Lists longer than 32: an atomic fill, a
+//      bitonic sort of the first 32 in a half-wave, insertion sort beyond.
 //   2. pattern: one half-wave per row loads the vertices of the row's
 //      incident elements (4 per lane), deduplicates them in an LDS hash set
 //      and orders the unique columns by a counting sort.  One pass writes
@@ -95,35 +98,6 @@ __device__ inline T bitonic32(T x, int hl) {
       x = (lower == up) ? tmin(x, p) : tmax(x, p);
     }
   return x;
-}
-
-// four values per lane, sequence index = 4 * lane-in-half + r (128 entries)
-template <class T>
-__device__ inline void bitonic128(T c[4], int hl) {
-#pragma unroll
-  for (int k = 2; k <= 128; k <<= 1)
-#pragma unroll
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      if (j < 4) {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          if (r & j) continue;
-          const bool up = ((hl * 4 + r) & k) == 0;
-          const T a = c[r], b = c[r | j];
-          const bool sw = up ? (b < a) : (a < b);
-          c[r] = sw ? b : a;
-          c[r | j] = sw ? a : b;
-        }
-      } else {
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const T p = __shfl_xor(c[r], j >> 2, kWave);
-          const int idx = hl * 4 + r;
-          const bool up = (idx & k) == 0, lower = (idx & j) == 0;
-          c[r] = (lower == up) ? tmin(c[r], p) : tmax(c[r], p);
-        }
-      }
-    }
 }
 
 // incidence lists of up to 32 entries sorted into element order, two rows
@@ -1551,16 +1525,17 @@ unsigned char *noalloc_scratch(int device) {
   return p;
 }
 
-// The P1 pattern (ffhip_pattern_p1).  With Ce (3D: the element scales of
-// k_p1_scale_3d, ready when the event ce_ready has passed) the half-wave
-// pattern pass also forms the values of the rows it builds; *values_done
-// tells whether that covered every row (no lane-per-row rows).
+// The P1 pattern (ffhip_pattern_p1).  With Ce (3D: the element scales,
+// computed from sc by the incidence build, or by k_p1_scale_3d when the
+// mesh's lists exist) the half-wave pattern pass also forms the values of
+// the rows it builds; *values_done tells whether that covered every row (no
+// lane-per-row rows).
 //
 // First try: the row pointers from the vertex-link counts (k_link_len), the
 // pattern pass writing straight to CSR.  If some row is not what they
 // predict, or needs the lane-per-row kernels, the buffered passes run.
-ffhip_mat pattern_p1(ffhip_ctx ctx, ffhip_mesh m, const double *Ce, hipEvent_t ce_ready,
-                     bool *values_done, const ScaleArgs *sc) {
+ffhip_mat pattern_p1(ffhip_ctx ctx, ffhip_mesh m, const double *Ce, bool *values_done,
+                     const ScaleArgs *sc) {
   hipStream_t s = ctx->stream;
   const bool vals_pass = Ce && m->nloc == 4;
   // the fused pass sorts the incidence buckets into the lists itself; with
@@ -1614,9 +1589,6 @@ ffhip_mat pattern_p1(ffhip_ctx ctx, ffhip_mesh m, const double *Ce, hipEvent_t c
       PatternOut out;
       out.rowptr = A->rowptr; out.colidx = A->colidx; out.diag = A->diag; out.vals = A->vals;
       out.mismatch = flags + 1;
-      // the element scales are needed from here on (the buffered passes come
-      // after this in stream order)
-      if (ce_ready) FFHIP_CHECK(hipStreamWaitEvent(s, ce_ready, 0));
       if (vals_pass)
         k_pattern_half<4, true, true><<<nblocks(n * kHalf, 256), 256, 0, s>>>(
             n, m->elems, m->v2e_ptr, m->v2e, A->slotmap, A->big, flags, out, m->xyz, Ce, bucket);
@@ -1735,7 +1707,7 @@ int ffhip_pattern_p1(ffhip_ctx ctx, ffhip_mesh m, ffhip_mat *out) {
     FFHIP_REQUIRE(ctx && m && out, FFHIP_ERR_ARG, "ffhip_pattern_p1: null argument");
     FFHIP_REQUIRE(m->nloc <= 4, FFHIP_ERR_UNSUPPORTED, "ffhip_pattern_p1: P1 on simplices only");
     ctx->tic();
-    ffhip_mat A = pattern_p1(ctx, m, nullptr, nullptr, nullptr, nullptr);
+    ffhip_mat A = pattern_p1(ctx, m, nullptr, nullptr, nullptr);
     ctx->toc();
     *out = A;
   });
@@ -1770,7 +1742,7 @@ int ffhip_pattern_assemble_p1(ffhip_ctx ctx, ffhip_mesh m, int kind, double c0, 
       // then the pattern pass forms the row values from them
       const ScaleArgs sc{m->xyz, kind, c0, kd, Ce};
       bool done = false;
-      A = pattern_p1(ctx, m, Ce, nullptr, &done, &sc);
+      A = pattern_p1(ctx, m, Ce, &done, &sc);
       ctx->toc();
       if (!done) {  // rows for the lane-per-row kernels: all values by the element-row path
         const int rc = ffhip_assemble_p1(ctx, m, A, kind, c0, kd, 1);

@@ -345,25 +345,10 @@ int ffhip_init(int device, ffhip_ctx *out) {
   });
 }
 
-hipStream_t ffhip_ctx_s::side_stream() {
-  if (!side) {
-    FFHIP_CHECK(hipStreamCreate(&side));
-    FFHIP_CHECK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
-    FFHIP_CHECK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
-  }
-  return side;
-}
-
 int ffhip_ctx_free(ffhip_ctx c) {
   return guard([&] {
     if (!c) return;
     (void)hipStreamSynchronize(c->stream);
-    if (c->side) {
-      (void)hipStreamSynchronize(c->side);
-      (void)hipEventDestroy(c->ev_fork);
-      (void)hipEventDestroy(c->ev_join);
-      (void)hipStreamDestroy(c->side);
-    }
     if (c->scratch) (void)hipFree(c->scratch);
     if (c->work) (void)hipFree(c->work);
     (void)hipEventDestroy(c->ev0);

@@ -82,12 +82,6 @@ struct ffhip_ctx_s {
   int device = 0;
   hipStream_t stream = nullptr;  // default priority (never a priority stream)
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  // a second stream (default priority as well) for work that runs beside the
-  // main stream's inside one call, joined back before the call returns;
-  // fork/join events without timing
-  hipStream_t side = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-  hipStream_t side_stream();  // created on first use
   int num_cu = 256;
   // reusable scratch for library-internal temporaries (scans, partials)
   void *scratch = nullptr;

@@ -1324,9 +1324,29 @@ static void upload_t5_once(int dev) {
 
 // dof -> (element * nloc + local) incidence lists in element order, for any
 // element-to-dof table (P1 vertices, P2 nodes)
+// The lists when some exceeds its 32-entry bucket: an atomic fill at the
+// list offsets, then the lists of up to 32 entries sorted in a half-wave and
+// the longer ones per dof (temporaries from the block cache: callable while
+// an arena is held).
+void v2e_fill_sort(ffhip_ctx ctx, const int32_t *elems, int64_t M, int64_t n, const int64_t *ptr,
+                   int32_t *list, int32_t *inv) {
+  hipStream_t s = ctx->stream;
+  int32_t *cur = dalloc<int32_t>(n);
+  int *flag = dalloc<int>(1);
+  FFHIP_CHECK(hipMemsetAsync(cur, 0, 4 * n, s));
+  FFHIP_CHECK(hipMemsetAsync(flag, 0, sizeof(int), s));
+  k_v2e_fill<<<nblocks(M, 256), 256, 0, s>>>(elems, M, ptr, cur, list);
+  k_v2e_sort_half<<<nblocks(n * kHalf, 256), 256, 0, s>>>(n, ptr, list, flag);
+  k_v2e_sort<<<nblocks(n, 256), 256, 0, s>>>(n, ptr, list);
+  if (inv) k_v2e_inverse<<<nblocks(M, 256), 256, 0, s>>>(M, list, inv);
+  FFHIP_CHECK(hipGetLastError());
+  dfree(cur);
+  dfree(flag);
+}
+
 void build_incidence(ffhip_ctx ctx, const int32_t *elems, int64_t nt, int nloc, int64_t n,
                      int64_t **ptr_out, int32_t **list_out, int *big_out, int32_t **inv_out,
-                     int32_t **bucket_out, const ScaleArgs *sc) {
+                     int32_t **bucket_out, const ScaleArgs *sc, int *ovf_dev) {
   hipStream_t s = ctx->stream;
   const int64_t M = nt * nloc;
   Arena ar(ctx, Arena::bytes(n, 4) + Arena::bytes(n + 1, 8) + (bucket_out ? 0 : Arena::bytes(32 * n, 4)) +
@@ -1335,7 +1355,9 @@ void build_incidence(ffhip_ctx ctx, const int32_t *elems, int64_t nt, int nloc, 
   int64_t *deg64 = ar.take<int64_t>(n + 1);
   int32_t *bucket = bucket_out ? dalloc<int32_t>(32 * n) : ar.take<int32_t>(32 * n);
   if (bucket_out) *bucket_out = nullptr;
-  int *flag = ar.take<int>(1);
+  // with ovf_dev (and bucket_out) the overflow flag is the caller's, read by
+  // the caller later: nothing here waits for the device
+  int *flag = ovf_dev && bucket_out ? ovf_dev : ar.take<int>(1);
   int64_t *ptr = dalloc<int64_t>(n + 1);
   int32_t *list = dalloc<int32_t>(M);
   int32_t *inv = inv_out ? dalloc<int32_t>(M) : nullptr;
@@ -1349,6 +1371,14 @@ void build_incidence(ffhip_ctx ctx, const int32_t *elems, int64_t nt, int nloc, 
   k_u32_to_i64<<<nblocks(n, 256), 256, 0, s>>>(n, deg, deg64);
   exclusive_scan_i64(ctx, deg64, ptr, n + 1);
   FFHIP_CHECK(hipGetLastError());
+  if (ovf_dev && bucket_out) {  // the caller checks the flag and sorts the buckets
+    *bucket_out = bucket;
+    if (big_out) *big_out = 0;
+    if (inv_out) *inv_out = inv;
+    *ptr_out = ptr;
+    *list_out = list;
+    return;
+  }
   const int has_big = read_flag(ctx, flag);
   if (!has_big && bucket_out) {
     *bucket_out = bucket;  // the caller sorts the buckets into the lists
@@ -1356,13 +1386,7 @@ void build_incidence(ffhip_ctx ctx, const int32_t *elems, int64_t nt, int nloc, 
     k_v2e_sort_bucket<<<nblocks(n * kHalf, 256), 256, 0, s>>>(n, ptr, bucket, list, inv);
   } else {  // some list exceeds its bucket: fill by atomics, then sort
     if (bucket_out) dfree(bucket);
-    int32_t *cur = ar.take<int32_t>(n);
-    FFHIP_CHECK(hipMemsetAsync(cur, 0, 4 * n, s));
-    FFHIP_CHECK(hipMemsetAsync(flag, 0, sizeof(int), s));
-    k_v2e_fill<<<nblocks(M, 256), 256, 0, s>>>(elems, M, ptr, cur, list);
-    k_v2e_sort_half<<<nblocks(n * kHalf, 256), 256, 0, s>>>(n, ptr, list, flag);
-    k_v2e_sort<<<nblocks(n, 256), 256, 0, s>>>(n, ptr, list);
-    if (inv) k_v2e_inverse<<<nblocks(M, 256), 256, 0, s>>>(M, list, inv);
+    v2e_fill_sort(ctx, elems, M, n, ptr, list, inv);
   }
   FFHIP_CHECK(hipGetLastError());
   FFHIP_CHECK(hipStreamSynchronize(s));
@@ -1386,12 +1410,14 @@ void build_v2e(ffhip_mesh m) {
 // otherwise) and free the buckets before anything else reads m->v2e.
 // With sc (a tetrahedral mesh), the incidence build also computes the
 // element scales (*scaled = true); when the lists exist already it does not.
-int32_t *build_v2e_unsorted(ffhip_mesh m, const ScaleArgs *sc, bool *scaled) {
+// ovf (device int): raised when some list exceeds its bucket; the caller
+// reads it and then builds the lists with v2e_fill_sort instead.
+int32_t *build_v2e_unsorted(ffhip_mesh m, const ScaleArgs *sc, bool *scaled, int *ovf) {
   *scaled = false;
   if (m->v2e_ptr) return nullptr;
   int32_t *bucket = nullptr;
   build_incidence(m->ctx, m->elems, m->nt, m->nloc, m->nv, &m->v2e_ptr, &m->v2e, &m->v2e_big,
-                  nullptr, &bucket, sc);
+                  nullptr, &bucket, sc, ovf);
   *scaled = sc && m->nloc == 4;
   return bucket;
 }
@@ -1542,9 +1568,12 @@ ffhip_mat pattern_p1(ffhip_ctx ctx, ffhip_mesh m, const double *Ce, bool *values
   // sc the incidence build computes the element scales Ce as well (or, on a
   // mesh whose lists exist, k_p1_scale_3d does)
   int32_t *bucket = nullptr;
+  int *ovf = nullptr;  // bucket overflow, read back with the row pointers
   if (vals_pass) {
     bool scaled = false;
-    bucket = build_v2e_unsorted(m, sc, &scaled);
+    ovf = dalloc<int>(1);
+    FFHIP_CHECK(hipMemsetAsync(ovf, 0, sizeof(int), s));
+    bucket = build_v2e_unsorted(m, sc, &scaled, ovf);
     if (sc && !scaled) {
       k_p1_scale_3d<<<nblocks(m->nt, 256), 256, 0, s>>>(m->nt, sc->xyz, m->elems, sc->kind, sc->c0,
                                                         sc->kappa_e, sc->Ce);
@@ -1583,7 +1612,21 @@ ffhip_mat pattern_p1(ffhip_ctx ctx, ffhip_mesh m, const double *Ce, bool *values
       k_link_len<<<nblocks(n, 256), 256, 0, s>>>(n, m->dim, m->v2e_ptr, bcnt, len);
       FFHIP_CHECK(hipGetLastError());
       exclusive_scan_i64(ctx, len, A->rowptr, n + 1);
-      A->nnz = read_i64(ctx, A->rowptr + n);
+      if (bucket) {  // the entry count and the bucket overflow flag in one wait
+        int64_t nnz = 0;
+        int of = 0;
+        FFHIP_CHECK(hipMemcpyAsync(&nnz, A->rowptr + n, sizeof(nnz), hipMemcpyDeviceToHost, s));
+        FFHIP_CHECK(hipMemcpyAsync(&of, ovf, sizeof(of), hipMemcpyDeviceToHost, s));
+        FFHIP_CHECK(hipStreamSynchronize(s));
+        A->nnz = nnz;
+        if (of) {  // some list exceeds its bucket: the lists by the atomic fill
+          dfree(bucket);
+          v2e_fill_sort(ctx, m->elems, m->nt * m->nloc, n, m->v2e_ptr, m->v2e, nullptr);
+          m->v2e_big = 1;
+        }
+      } else {
+        A->nnz = read_i64(ctx, A->rowptr + n);
+      }
       A->colidx = dalloc<int32_t>(A->nnz);
       A->vals = dalloc<double>(A->nnz);
       PatternOut out;
@@ -1606,6 +1649,7 @@ ffhip_mat pattern_p1(ffhip_ctx ctx, ffhip_mesh m, const double *Ce, bool *values
       if (!direct_ok) { dfree(A->colidx); dfree(A->vals); }
       if (direct_ok) dfree(bucket);  // every list sorted by the pass
       else sort_buckets();
+      dfree(ovf);
     }
     if (direct_ok) {
       if (!vals_pass) FFHIP_CHECK(hipMemsetAsync(A->vals, 0, 8 * A->nnz, s));
@@ -1683,6 +1727,7 @@ ffhip_mat pattern_p1(ffhip_ctx ctx, ffhip_mesh m, const double *Ce, bool *values
     A->max_row = wide ? CAPW : CAP;
   } catch (...) {
     (void)hipStreamSynchronize(s);
+    dfree(ovf);
     if (bucket) {  // the lists were never sorted: the mesh builds them again when next needed
       dfree(bucket);
       dfree(m->v2e_ptr);

@@ -192,9 +192,6 @@ __device__ inline double block_sum(double v, double *lds /* BLOCK/64 */) {
   return s;
 }
 
-// XCD-aware logical block id: the hardware dispatches block b to XCD b % 8;
-// logical ids are renumbered so each XCD owns a contiguous block range (and
-// so a contiguous row range whose x-gathers stay in that XCD's L2).
 // orders the LDS accesses of one wave (cross-lane reuse of the row staging)
 __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -202,6 +199,9 @@ __device__ inline void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// XCD-aware logical block id: the hardware dispatches block b to XCD b % 8;
+// logical ids are renumbered so each XCD owns a contiguous block range (and
+// so a contiguous row range whose x-gathers stay in that XCD's L2).
 __device__ inline int xcd_block_id() {
   const int G = gridDim.x, b = blockIdx.x;
   if (G % 8) return b;
@@ -222,7 +222,7 @@ struct ScaleArgs {
 // dof -> (element, local dof) incidence lists in element order (ffhip_assembly.hip)
 void build_incidence(ffhip_ctx ctx, const int32_t *elems, int64_t nt, int nloc, int64_t n,
                      int64_t **ptr_out, int32_t **list_out, int *big_out, int32_t **inv_out,
-                     int32_t **bucket_out, const ScaleArgs *sc = nullptr);
+                     int32_t **bucket_out, const ScaleArgs *sc = nullptr, int *ovf_dev = nullptr);
 
 void exclusive_scan_i64(ffhip_ctx ctx, const int64_t *in, int64_t *out,
                         int64_t n);

@@ -257,7 +257,8 @@ __global__ __launch_bounds__(256) void k_sell_fill(
     const int64_t *__restrict__ sptr, const int32_t *__restrict__ len,
     const double *__restrict__ dsc, int32_t *__restrict__ scol,
     const int64_t *__restrict__ sptr16, int16_t *__restrict__ scol16, double *__restrict__ sval,
-    float *__restrict__ sval32, bool pairs) {
+    float *__restrict__ sval32, bool pairs, const int64_t *__restrict__ diag,
+    double *__restrict__ dout, uint8_t *__restrict__ hout) {
   __shared__ int32_t lc[kFillStage];
   __shared__ double lv[kFillStage];
   __shared__ int64_t rs[kSlice];  // row start (in the stage, or in the CSR)
@@ -280,6 +281,11 @@ __global__ __launch_bounds__(256) void k_sell_fill(
       lv[i] = tmap ? vals[tmap[seg0 + i]] : vals[seg0 + i];
     }
   __syncthreads();
+  if (dout && t < kSlice && r0 + t < n) {  // the diagonal of the Jacobi set-up (as k_diag_vals)
+    const int64_t r = r0 + t, q = diag[r];
+    dout[r] = q < 0 ? 0. : (staged && !tmap && q >= seg0 && q < seg1 ? lv[q - seg0] : vals[q]);
+    hout[r] = q >= 0;
+  }
   const int L = len[sl];
   const int64_t base = sptr[sl];
   // entry k of lane li's row; past its end a zero at the row's own column
@@ -1808,8 +1814,23 @@ static void cg_plan_layout(ffhip_cg cg, int layout, bool index32) {
   }
 }
 
-// values in the solver's layout (A or A^T; FP32 scaled copy for MIXED)
-static void cg_fill_values(ffhip_cg cg, bool transposed) {
+// The diagonal values and presence flags the Jacobi set-up starts from, and
+// the partials of its tgv passes.
+struct DiagBufs {
+  static constexpr int GB = 512;  // blocks of a tgv pass
+  double *d;
+  uint8_t *has;
+  double *part;
+  static size_t bytes(int64_t n) { return Arena::bytes(n, 8) + Arena::bytes(n, 1) + Arena::bytes(2 * GB, 8); }
+  DiagBufs(Arena &ar, int64_t n) : d(ar.take<double>(n)), has(ar.take<uint8_t>(n)), part(ar.take<double>(2 * GB)) {}
+};
+static void cg_jacobi_tail(ffhip_cg cg, const DiagBufs &b);
+
+// values in the solver's layout (A or A^T; FP32 scaled copy for MIXED).  With
+// db the sliced-ELL fill also writes the diagonal and the Jacobi set-up runs
+// behind it; with read_rb the tgv figures come back with the fill's wait.
+static void cg_fill_values(ffhip_cg cg, bool transposed, const DiagBufs *db = nullptr,
+                           bool read_rb = false) {
   ffhip_ctx ctx = cg->ctx;
   ffhip_mat A = cg->A;
   hipStream_t s = ctx->stream;
@@ -1837,7 +1858,12 @@ static void cg_fill_values(ffhip_cg cg, bool transposed) {
     k_sell_fill<<<cg->S, 256, 0, s>>>(
         A->n, cg->S, A->rowptr, A->colidx, A->vals, tmap, cg->sptr, cg->slen, cg->dsc,
         transposed ? nullptr : cg->scol, cg->sptr16, transposed ? nullptr : cg->scol16, dv,
-        mixed ? fv : nullptr, cg->fmt == kSell16);
+        mixed ? fv : nullptr, cg->fmt == kSell16, A->diag, db ? db->d : nullptr,
+        db ? db->has : nullptr);
+    if (db) {
+      FFHIP_CHECK(hipGetLastError());
+      cg_jacobi_tail(cg, *db);
+    }
   } else {
     double *dst = nullptr;
     if (transposed) {
@@ -1851,23 +1877,34 @@ static void cg_fill_values(ffhip_cg cg, bool transposed) {
           A->n, A->rowptr, A->colidx, A->vals, tmap, cg->dsc, dst, mixed ? fv : nullptr);
   }
   FFHIP_CHECK(hipGetLastError());
+  SetupRb h{};
+  if (read_rb) FFHIP_CHECK(hipMemcpyAsync(&h, cg->rb, sizeof(h), hipMemcpyDeviceToHost, s));
   FFHIP_CHECK(hipStreamSynchronize(s));
+  if (read_rb) {
+    cg->tgvm = h.ttgv;
+    cg->ntgv = (int)h.ntgv;
+  }
   dfree(tmap);
 }
 
 // Jacobi scaling and the tgv rows, on the device (the tgv figures reach the
 // host with the next setup_readback)
 static void cg_setup_jacobi(ffhip_cg cg) {
-  ffhip_ctx ctx = cg->ctx;
-  ffhip_mat A = cg->A;
-  hipStream_t s = ctx->stream;
-  const int64_t n = A->n;
-  const int GB = 512;
-  Arena ar(ctx, Arena::bytes(n, 8) + Arena::bytes(n, 1) + Arena::bytes(2 * GB, 8));
-  double *d = ar.take<double>(n);
-  uint8_t *has = ar.take<uint8_t>(n);
-  double *part = ar.take<double>(2 * GB);
-  k_diag_vals<<<nblk(n, 256), 256, 0, s>>>(n, A->diag, A->vals, d, has);
+  const int64_t n = cg->A->n;
+  Arena ar(cg->ctx, DiagBufs::bytes(n));
+  const DiagBufs b(ar, n);
+  k_diag_vals<<<nblk(n, 256), 256, 0, cg->ctx->stream>>>(n, cg->A->diag, cg->A->vals, b.d, b.has);
+  cg_jacobi_tail(cg, b);
+}
+
+// the tgv passes and the Jacobi scaling from the diagonal in b
+static void cg_jacobi_tail(ffhip_cg cg, const DiagBufs &b) {
+  hipStream_t s = cg->ctx->stream;
+  const int64_t n = cg->A->n;
+  const int GB = DiagBufs::GB;
+  const double *d = b.d;
+  const uint8_t *has = b.has;
+  double *part = b.part;
   k_tgv_pass<<<GB, kBlock, 0, s>>>(n, d, has, nullptr, 1, part);
   k_tgv_reduce<<<1, kBlock, 0, s>>>(GB, part, 1, cg->rb);
   k_tgv_pass<<<GB, kBlock, 0, s>>>(n, d, has, &cg->rb->ttgv, 2, part);
@@ -2060,9 +2097,22 @@ int ffhip_cg_create(ffhip_ctx ctx, ffhip_mat A, int mode, ffhip_cg *out) {
       cg->dsc = dalloc<double>(n);
       cg->wcl = dalloc<uint8_t>(n);
       cg->rb = dalloc<SetupRb>(1);
-      cg_setup_jacobi(cg);
-      cg_plan_layout(cg, layout, index32);  // reads the tgv figures back with its sizes
-      cg_fill_values(cg, false);
+      if (prec == FFHIP_CG_MIXED) {  // the FP32 fill scales by the Jacobi diagonal
+        cg_setup_jacobi(cg);
+        cg_plan_layout(cg, layout, index32);  // reads the tgv figures back with its sizes
+        cg_fill_values(cg, false);
+      } else {  // FP64: the Jacobi set-up after the layout, from the fill's diagonal on sliced ELL
+        FFHIP_CHECK(hipMemsetAsync(cg->rb, 0, sizeof(SetupRb), ctx->stream));
+        cg_plan_layout(cg, layout, index32);
+        Arena ar(ctx, DiagBufs::bytes(n));
+        const DiagBufs b(ar, n);
+        const bool in_fill = cg->fmt == kSell || cg->fmt == kSell16;
+        if (!in_fill) {
+          k_diag_vals<<<nblk(n, 256), 256, 0, ctx->stream>>>(n, A->diag, A->vals, b.d, b.has);
+          cg_jacobi_tail(cg, b);
+        }
+        cg_fill_values(cg, false, in_fill ? &b : nullptr, true);
+      }
       cg->xw = dalloc<double>(n); cg->bw = dalloc<double>(n);
       cg->G = dalloc<double>(n); cg->CG = dalloc<double>(n);
       cg->H[0] = dalloc<double>(n); cg->H[1] = dalloc<double>(n);

@@ -28,14 +28,15 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 LABELS = (1, 2, 3, 4, 5, 6)
 
 
-def assembly_p1(n=160, reps=3):
+def assembly_p1(n=160, reps=3, fp64=0):
+    # fp64=1: the FP64 solver's set-up (the benchmark's) instead of the MIXED one
     import ffhip
     for _ in range(reps):
         M = ffhip.cube(n)
         A = ffhip.Matrix.p1_assembled(M, ffhip.COEF_SIN3)
         tpv = M.ctx.last_ms()
         tb = A.set_bc(LABELS)
-        cg = ffhip.CG(A, ffhip.CG_MIXED)
+        cg = ffhip.CG(A, ffhip.CG_FP64 if fp64 else ffhip.CG_MIXED)
         print(f"P1 cube({n}) nnz {A.nnz}: pattern and values {tpv:.3f} ms, bc {tb:.3f} ms", flush=True)
         cg.free(); A.free(); M.free()
 

@@ -258,7 +258,7 @@ __global__ __launch_bounds__(256) void k_sell_fill(
     const double *__restrict__ dsc, int32_t *__restrict__ scol,
     const int64_t *__restrict__ sptr16, int16_t *__restrict__ scol16, double *__restrict__ sval,
     float *__restrict__ sval32, bool pairs, const int64_t *__restrict__ diag,
-    double *__restrict__ dout, uint8_t *__restrict__ hout) {
+    double *__restrict__ dout, uint8_t *__restrict__ hout, int *__restrict__ far) {
   __shared__ int32_t lc[kFillStage];
   __shared__ double lv[kFillStage];
   __shared__ int64_t rs[kSlice];  // row start (in the stage, or in the CSR)
@@ -329,7 +329,9 @@ __global__ __launch_bounds__(256) void k_sell_fill(
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int k = 4 * g + q;
-        h[q] = k < rl[li] ? (uint32_t)(uint16_t)(int16_t)(col(li, k) - r) : 0u;
+        const int64_t dl = k < rl[li] ? (int64_t)col(li, k) - r : 0;
+        if (far && (dl < -32768 || dl > 32767)) atomicOr(far, 1);  // then refilled with int32 columns
+        h[q] = (uint32_t)(uint16_t)(int16_t)dl;
       }
       const u32x2 pk = {h[0] | (h[1] << 16), h[2] | (h[3] << 16)};
       *reinterpret_cast<u32x2 *>(scol16 + sptr16[sl] + (int64_t)it * 4) = pk;
@@ -474,16 +476,6 @@ __global__ void k_bsr_waves(int64_t S3, const int64_t *__restrict__ sptr, int nw
 
 // flags a row with a column farther than 16 bits from the row index
 // the columns of a row are sorted: its first and last bound the deltas
-__global__ void k_far_columns(int64_t n, const int64_t *__restrict__ rowptr,
-                              const int32_t *__restrict__ colidx, int *__restrict__ far) {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= n) return;
-  const int64_t k0 = rowptr[r], k1 = rowptr[r + 1];
-  if (k0 == k1) return;
-  const int64_t d0 = (int64_t)colidx[k0] - r, d1 = (int64_t)colidx[k1 - 1] - r;
-  if (d0 < -32768 || d1 > 32767) atomicOr(far, 1);
-}
-
 // CSR-layout values: A (optionally transposed through tmap) and/or the FP32
 // copy of D^-1/2 A D^-1/2, one 16-lane group per row
 __global__ void k_csr_fill(int64_t n, const int64_t *__restrict__ rowptr,
@@ -1560,6 +1552,7 @@ struct ffhip_cg_s {
   ffhip_mat A = nullptr;
   int mode = 0;
   int fmt = kSell;
+  int far_pending = 0;  // kSell16 chosen before the fill has checked every column delta
   int64_t n = 0, S = 0, ents = 0;
   int grid = 0;
   // kSell layout (kCsrV streams A's own rowptr/colidx)
@@ -1750,9 +1743,9 @@ static void cg_plan_layout(ffhip_cg cg, int layout, bool index32) {
   } else if (layout == FFHIP_CG_LAYOUT_BSR3) {
     FFHIP_REQUIRE(false, FFHIP_ERR_UNSUPPORTED, "ffhip_cg_create: 3x3 blocks need a [P2,P2,P2] matrix");
   } else if (layout != FFHIP_CG_LAYOUT_CSR) {
-    // slice widths and offsets, the far-column check and the offsets of the
-    // packed 16-bit deltas, then one read-back decides among sliced ELL with
-    // 16-bit deltas, with int32 columns, and CSR
+    // slice widths and offsets and the offsets of the packed 16-bit deltas,
+    // then one read-back decides among sliced ELL with 16-bit deltas (the fill
+    // then checks them), with int32 columns, and CSR
     Arena ar(ctx, 3 * Arena::bytes(cg->S + 1, 8) + Arena::bytes(1, 4));
     cg->sptr = dalloc<int64_t>(cg->S + 1);
     cg->slen = dalloc<int32_t>(cg->S);
@@ -1765,8 +1758,8 @@ static void cg_plan_layout(ffhip_cg cg, int layout, bool index32) {
     FFHIP_CHECK(hipMemsetAsync(fard, 0, sizeof(int), s));
     k_sell_len<<<nblk(cg->S, 256), 256, 0, s>>>(A->n, cg->S, A->rowptr, cg->slen, cnt);
     exclusive_scan_i64(ctx, cnt, cg->sptr, cg->S + 1);
-    // 16-bit column deltas when every column is within 2^15 of its row
-    if (!index32) k_far_columns<<<nblk(A->n, 256), 256, 0, s>>>(A->n, A->rowptr, A->colidx, fard);
+    // 16-bit column deltas unless index32: the fill checks every delta
+    // against 2^15 and falls back to int32 columns (far_pending)
     k_sell_len4<<<nblk(cg->S, 256), 256, 0, s>>>(cg->S, cg->slen, cnt4);
     exclusive_scan_i64(ctx, cnt4, cg->sptr16, cg->S + 1);
     k_setup_sizes<<<1, 1, 0, s>>>(cg->rb, cg->sptr + cg->S, cg->sptr16 + cg->S, index32 ? nullptr : fard);
@@ -1782,6 +1775,7 @@ static void cg_plan_layout(ffhip_cg cg, int layout, bool index32) {
       } else {
         cg->fmt = kSell16;
         cg->scol16 = dalloc<int16_t>(h.ents16);
+        cg->far_pending = 1;
       }
     } else {
       dfree(cg->sptr); dfree(cg->slen); dfree(cg->sptr16);
@@ -1843,6 +1837,7 @@ static void cg_fill_values(ffhip_cg cg, bool transposed, const DiagBufs *db = nu
     FFHIP_REQUIRE(ok, FFHIP_ERR_UNSUPPORTED, "transposed solve needs a symmetric pattern");
   }
   const bool mixed = cg->mode == FFHIP_CG_MIXED;
+  const bool check_far = !transposed && cg->far_pending && cg->fmt == kSell16;
   double *&dv = transposed ? cg->svalT : cg->sval;
   float *&fv = transposed ? cg->sval32T : cg->sval32;
   const int64_t nval = cg->fmt == kBsr3 ? 9 * cg->ents : cg->ents;
@@ -1859,7 +1854,7 @@ static void cg_fill_values(ffhip_cg cg, bool transposed, const DiagBufs *db = nu
         A->n, cg->S, A->rowptr, A->colidx, A->vals, tmap, cg->sptr, cg->slen, cg->dsc,
         transposed ? nullptr : cg->scol, cg->sptr16, transposed ? nullptr : cg->scol16, dv,
         mixed ? fv : nullptr, cg->fmt == kSell16, A->diag, db ? db->d : nullptr,
-        db ? db->has : nullptr);
+        db ? db->has : nullptr, check_far ? &cg->rb->far : nullptr);
     if (db) {
       FFHIP_CHECK(hipGetLastError());
       cg_jacobi_tail(cg, *db);
@@ -1878,13 +1873,23 @@ static void cg_fill_values(ffhip_cg cg, bool transposed, const DiagBufs *db = nu
   }
   FFHIP_CHECK(hipGetLastError());
   SetupRb h{};
-  if (read_rb) FFHIP_CHECK(hipMemcpyAsync(&h, cg->rb, sizeof(h), hipMemcpyDeviceToHost, s));
+  if (read_rb || check_far) FFHIP_CHECK(hipMemcpyAsync(&h, cg->rb, sizeof(h), hipMemcpyDeviceToHost, s));
   FFHIP_CHECK(hipStreamSynchronize(s));
   if (read_rb) {
     cg->tgvm = h.ttgv;
     cg->ntgv = (int)h.ntgv;
   }
   dfree(tmap);
+  if (check_far) {
+    cg->far_pending = 0;
+    if (h.far) {  // some column more than 2^15 from its row: int32 columns, filled again
+      dfree(cg->scol16);
+      dfree(cg->sptr16);
+      cg->fmt = kSell;
+      cg->scol = dalloc<int32_t>(cg->ents);
+      cg_fill_values(cg, transposed, db, read_rb);
+    }
+  }
 }
 
 // Jacobi scaling and the tgv rows, on the device (the tgv figures reach the

@@ -357,12 +357,19 @@ def test_far_columns_use_32bit_layout():
         A.assemble_p1(ffhip.COEF_SIN3)
         A.set_bc(LABELS)
         b = ffhip.rhs_p1(M, 1.0, LABELS)
-        cg = ffhip.CG(A, ffhip.CG_MIXED)
-        outs.append((cg.layout()[0], cg.solve(b, eps=1e-10)))
-    assert outs[0][0] == "sell16" and outs[1][0] != "sell16"
-    x0, x1 = outs[0][1][0], outs[1][1][0]
-    assert outs[1][1][1]["converged"] == 1
-    assert np.linalg.norm(x1[perm] - x0) <= 1e-8 * np.linalg.norm(x0)
+        for mode in (ffhip.CG_MIXED, ffhip.CG_FP64):
+            cg = ffhip.CG(A, mode)
+            outs.append((cg.layout()[0], cg.solve(b, eps=1e-10)))
+    # relabelled: int32 columns or CSR (a 16-bit layout tried first is filled
+    # again with int32 columns once the fill finds a far column)
+    assert outs[0][0] == "sell16" and outs[1][0] == "sell16"
+    assert outs[2][0] != "sell16" and outs[3][0] != "sell16"
+    for k in (0, 1):
+        x0, x1 = outs[k][1][0], outs[2 + k][1][0]
+        assert outs[2 + k][1][1]["converged"] == 1
+        assert np.linalg.norm(x1[perm] - x0) <= 1e-8 * np.linalg.norm(x0)
+    # FP64: the same iteration count on the relabelled mesh (the same system)
+    assert abs(outs[3][1][1]["iters"] - outs[1][1][1]["iters"]) <= 1
 
 
 def _fan_mesh(m=20):
